@@ -1,0 +1,355 @@
+"""Pins of the FP64 oracle against independent mathematics (CPU only).
+
+Each test states which oracle function it pins and by what: brute force on tiny
+inputs, closed forms, invariants, or known-answer cases chosen so that a
+plausible mistake (dropped term, sign or index error, transposed operand) fails.
+The reference ships no golden vectors for this path (its only registrator test,
+``test/test_zorro.py:38-75``, is disabled), so these checks are the pins.
+"""
+import math
+
+import numpy as np
+import pytest
+import scipy.ndimage
+import scipy.optimize
+
+from oracle import zorro_oracle as O
+
+
+RNG = np.random.default_rng(1234)
+
+
+# ----------------------------------------------------------------- transforms
+def test_raw_ifft2_closed_forms():
+    # delta -> constant 1 (raw sum); constant c -> c*N at DC; Parseval
+    X = np.zeros((8, 6), complex)
+    X[0, 0] = 1.0
+    assert np.allclose(O.raw_ifft2(X), 1.0)
+    X = np.full((8, 6), 2.0 + 0j)
+    y = O.raw_ifft2(X)
+    assert np.isclose(y[0, 0], 96.0) and np.allclose(y.ravel()[1:], 0.0)
+    x = RNG.standard_normal((8, 6))
+    F = np.fft.fft2(x)
+    assert np.isclose(np.sum(np.abs(F) ** 2), 48 * np.sum(x ** 2))
+    # raw inverse of the forward transform returns N * x
+    assert np.allclose(O.raw_ifft2(F), 48 * x)
+
+
+def test_fourier_crop_is_decimation_of_bandlimited_image():
+    n = 64
+    F = np.zeros((n, n), complex)
+    k = RNG.standard_normal((15, 15)) + 1j * RNG.standard_normal((15, 15))
+    F[:8, :8] = k[:8, :8]
+    F[-7:, :8] = k[8:, :8]
+    F[:8, -7:] = k[:8, 8:]
+    F[-7:, -7:] = k[8:, 8:]
+    img = np.real(np.fft.ifft2(F))
+    Fc = O.fourier_crop(np.fft.fft2(img), (32, 32))
+    dec = np.real(np.fft.ifft2(Fc)) * (32 * 32) / (64 * 64)
+    # the band limit is below the crop's Nyquist except for the real-part fold
+    assert np.allclose(dec, img[::2, ::2], atol=1e-12)
+
+
+def test_shape_cropped_rules():
+    assert O.shape_cropped((0, 0), (1024, 1024)) == (1024, 1024)
+    assert O.shape_cropped((3073, 3071), (4096, 4096)) == (3072, 3070)
+    with pytest.raises(ValueError):
+        O.shape_cropped((4098, 10), (4096, 4096))
+
+
+# ------------------------------------------------------------------ mnxc2
+def _brute_masked_ncc(fI, fJ, m, mshift):
+    """Padfield masked NCC by direct real-space sums over circular lags."""
+    n = fI.size
+    H, W = fI.shape
+    out = np.zeros((H, W))
+    origin = None
+    for dy in range(H):
+        for dx in range(W):
+            sJ = np.roll(np.roll(fJ, -dy, 0), -dx, 1)     # fJ(x+d)
+            sm = np.roll(np.roll(m, -dy, 0), -dx, 1)      # m(x+d)
+            num = np.sum(fI * sJ) / n
+            ct = np.sum(fI * sm) / n
+            cb = np.sum(m * sJ) / n
+            mp = np.sum(m * sm) / n
+            t1 = np.sum(fI * fI * sm) / n
+            t2 = np.sum(m * sJ * sJ) / n
+            den = max(math.sqrt((t2 - cb * cb / mp) * (t1 - ct * ct / mp)), 1.0)
+            out[dy, dx] = (num - ct * cb / mp) / den
+    return out
+
+
+def test_mnxc2_matches_bruteforce_masked_ncc():
+    H = W = 24
+    mask = np.zeros((H, W))
+    mask[:19, :17] = 1.0
+    fI = mask * (5 + 3 * RNG.standard_normal((H, W)))
+    fJ = mask * (5 + 3 * RNG.standard_normal((H, W)))
+    brute = _brute_masked_ncc(fI, fJ, mask, mask)
+    M, mp = O.mask_spectrum(mask > 0, (H, W))
+    FI, QI = np.fft.fft2(fI), np.fft.fft2(fI * fI)
+    FJ, QJ = np.fft.fft2(fJ), np.fft.fft2(fJ * fJ)
+    m = 6
+    W_or, origin = O.mnxc2_window(FI, QI, FJ, QJ, M, mp, (H, W), m, suppressOrigin=False)
+    expect = np.fft.ifftshift(brute)[H // 2 - m:H // 2 + m, W // 2 - m:W // 2 + m]
+    assert np.allclose(W_or, expect, rtol=1e-9, atol=1e-9)
+    assert np.isclose(origin, brute[0, 0])
+
+
+def test_mnxc2_roll_known_answer():
+    # J = I rolled by d: the peak sits at lag d with C = 1 (variance > 1, no clip)
+    H = W = 64
+    f = 10 + 4 * RNG.standard_normal((H, W))
+    d = (5, -3)
+    g = np.roll(np.roll(f, d[0], 0), d[1], 1)
+    F, G = np.fft.fft2(f), np.fft.fft2(g)
+    m = 16
+    Wn, _ = O.mnxc2_window_unmasked(F, G, f.sum(), (f * f).sum(), g.sum(), (g * g).sum(), (H, W), (H, W), m)
+    iy, ix = np.unravel_index(np.argmax(Wn), Wn.shape)
+    assert (iy - m, ix - m) == d
+    assert np.isclose(Wn[iy, ix], 1.0, atol=1e-12)
+
+
+def test_unmasked_closed_form_equals_general_formula_with_crop():
+    H = W = 64
+    shapeC = (48, 40)
+    f = 3 + RNG.standard_normal((H, W))
+    g = np.roll(f, 4, 0) + 0.5 * RNG.standard_normal((H, W))
+    mask = np.ones((H, W), bool)
+    M, mp = O.mask_spectrum(mask, shapeC)
+    F = O.fourier_crop(np.fft.fft2(f), shapeC)
+    Q = O.fourier_crop(np.fft.fft2(f * f), shapeC)
+    G = O.fourier_crop(np.fft.fft2(g), shapeC)
+    R = O.fourier_crop(np.fft.fft2(g * g), shapeC)
+    a, oa = O.mnxc2_window(F, Q, G, R, M, mp, shapeC, 12)
+    b, ob = O.mnxc2_window_unmasked(F, G, f.sum(), (f * f).sum(), g.sum(), (g * g).sum(), (H, W), shapeC, 12)
+    assert np.allclose(a, b, rtol=1e-8, atol=1e-10)
+    assert np.isclose(oa, ob, rtol=1e-8)
+
+
+def test_origin_suppression_stencil_quirk():
+    C = np.arange(64.0).reshape(8, 8)
+    W, origin = O.window_from_full(C, 4, True)
+    # lag (0,0) is window index (4,4); the stencil repeats C[-1,0] and C[-1,1]
+    expect = 0.125 * (C[1, 0] + C[0, 1] + 2 * C[-1, 0] + C[1, 1] + 2 * C[-1, 1] + C[-1, -1])
+    assert W[4, 4] == expect and origin == 0.0
+    assert W[5, 4] == C[1, 0] and W[3, 3] == C[-1, -1]
+
+
+# --------------------------------------------------------------- locatePeak
+def test_b_filter_conv_equals_explicit_reflect_gaussian():
+    C = RNG.standard_normal((80, 80))
+    sig = 1024 / (math.sqrt(2) * math.pi * 150)
+    got = O.b_filter(C, "conv", 150, (1024, 1024))
+    r = int(4.0 * sig + 0.5)
+    k = np.exp(-0.5 * (np.arange(-r, r + 1) / sig) ** 2)
+    k /= k.sum()
+    P = np.pad(C, r, mode="symmetric")          # scipy 'reflect' == numpy 'symmetric'
+    t = np.array([[np.dot(k, P[i:i + 2 * r + 1, j + r]) for j in range(80)] for i in range(80)])
+    P2 = np.pad(t, ((0, 0), (r, r)), mode="symmetric")
+    expect = np.array([[np.dot(k, P2[i, j:j + 2 * r + 1]) for j in range(80)] for i in range(80)])
+    assert np.allclose(got, expect, atol=1e-12)
+
+
+def _brute_subpixel(patch, P=16):
+    n = 16 * P
+    f = np.arange(-8, 8)
+    tau = np.arange(n)[:, None] - 16 * np.arange(16)[None, :]
+    K = np.exp(2j * np.pi * f[None, None, :] * tau[:, :, None] / n).sum(axis=2)   # [n,16]
+    Y = K @ patch @ K.T
+    return np.abs(Y)
+
+
+def test_subpixel_matches_bruteforce_interpolation():
+    m = 40
+    yy, xx = np.mgrid[0:2 * m, 0:2 * m]
+    y0, x0 = m + 7.3, m - 11.71
+    C = np.exp(-((yy - y0) ** 2 + (xx - x0) ** 2) / (2 * 4.0 ** 2)) + 0.01 * RNG.standard_normal((2 * m, 2 * m))
+    p = O.Params(maxShift=m, Bmode="none", subPixReg=16)
+    shift, corr, ps, cmax = O.locate_peak(C, p, (1024, 1024))
+    patch = C[cmax[0] - 8:cmax[0] + 8, cmax[1] - 8:cmax[1] + 8]
+    A = _brute_subpixel(patch)
+    s = np.unravel_index(np.argmax(A), A.shape)
+    rp = np.array(cmax, float) - m
+    expect = rp + (np.array(s) - 128 + np.sign(rp)) / 16
+    assert np.allclose(shift, expect)
+    assert np.isclose(corr, A[s] / 256)
+
+
+def test_subpixel_known_answer_on_bandlimited_patch():
+    # A patch that is exactly a 16-periodic trig polynomial is interpolated
+    # exactly: the 1/16 grid point of the true peak wins, plus the Q4 bias.
+    m = 40
+    yy, xx = np.mgrid[0:2 * m, 0:2 * m].astype(float)
+    y0, x0 = m + 7 + 5 / 16, m - 11 - 3 / 16
+    C = 3 + np.cos(2 * np.pi * (yy - y0) / 16) + np.cos(2 * np.pi * (xx - x0) / 16)
+    C[(np.abs(yy - y0) > 9) | (np.abs(xx - x0) > 9)] = 0.0
+    p = O.Params(maxShift=m, Bmode="none", subPixReg=16)
+    shift, corr, ps, cmax = O.locate_peak(C, p, (1024, 1024))
+    assert np.allclose(shift, [y0 - m + 1 / 16, x0 - m - 1 / 16])
+    assert np.isclose(corr, 5.0)
+
+
+def test_subpixel_edge_fallback_uses_unfiltered_integer_peak():
+    m = 40
+    C = np.zeros((80, 80))
+    C[40, 3] = 5.0                     # inside the disc, 3 px from the window edge
+    p = O.Params(maxShift=m, Bmode="none", subPixReg=16)
+    shift, corr, ps, cmax = O.locate_peak(C, p, (1024, 1024))
+    assert cmax == (40, 3) and np.allclose(shift, [0, -37]) and corr == 5.0
+
+
+def test_peak_argmax_restricted_to_disc_and_first_occurrence():
+    m = 40
+    C = np.full((80, 80), -1.0)
+    C[0, 0] = 100.0                    # outside the disc: masked to zero
+    C[50, 50] = 2.0
+    C[60, 30] = 2.0                    # tie later in C order
+    p = O.Params(maxShift=m, Bmode="none", subPixReg=1)
+    shift, corr, ps, cmax = O.locate_peak(C, p, (1024, 1024))
+    assert cmax == (50, 50) and np.allclose(shift, [10, 10])
+
+
+# ------------------------------------------------------------------- solver
+def test_build_equations_tri_and_diag_intervals():
+    for tri in (O.tri_indices(6, "tri")[:-1, 1:], O.tri_indices(6, "diag", diagWidth=2)[:-1, 1:]):
+        A = O.build_equations(tri)
+        rows = [(I, J) for I in range(5) for J in range(5) if tri[I, J]]
+        assert A.shape == (len(rows), 5)
+        for r, (I, j) in enumerate(rows):
+            expect = np.zeros(5)
+            expect[I:j + 1] = 1.0             # pair (I, J=j+1) sums x_I..x_{J-1}
+            assert np.array_equal(A[r], expect)
+
+
+def test_l1_solver_exact_on_consistent_system_and_optimal_otherwise():
+    N = 12
+    tri = O.tri_indices(N + 1, "tri")[:-1, 1:]
+    A = O.build_equations(tri)
+    x_true = np.round(RNG.normal(0.8, 0.5, N) * 16) / 16
+    b = A @ x_true
+    w = RNG.uniform(0.1, 1, A.shape[0])
+    x = O.l1_solve(A, b, w, 1e3)
+    xe = O.exact_vertex(A, b, x)
+    assert np.array_equal(xe, x_true)
+    # noisy: the LP optimum is no worse than random restarts of a local method
+    b2 = b + RNG.standard_normal(b.size) * 0.2
+    xo = O.l1_solve(A, b2, w, 1e3)
+    fo = np.sum(w * np.abs(A @ xo - b2))
+    for s in range(5):
+        r = scipy.optimize.minimize(lambda z: np.sum(w * np.abs(A @ z - b2)), RNG.standard_normal(N),
+                                    method="Powell")
+        assert fo <= r.fun + 1e-9
+
+
+def test_logistic_fit_no_worse_than_reference_basinhopping():
+    ps = np.concatenate([RNG.normal(4, 1, 300), RNG.normal(12, 2, 480)])
+    ps = ps[ps > 0]
+    pdf, edges = np.histogram(ps, bins=int(ps.size / 7.0))
+    h = edges[:-1]
+    cdf = np.cumsum(pdf.astype(float))
+    cdf /= cdf[-1]
+    thr, K, nu, f = O.logistic_fit(h, cdf, ps.min(), ps.max())
+    assert ps.min() <= thr <= ps.max() and 0.1 <= K <= 20 and 0.05 <= nu <= 5
+    assert np.isclose(f, O.logistic_objective([thr, K, nu], h, cdf))
+    # the reference's optimiser (zorro/zorro.py:1998-2008), seeded
+    bh = scipy.optimize.basinhopping(O.logistic_objective, [np.mean(ps), 5.0, 1.0], niter=50, seed=7,
+                                     minimizer_kwargs=dict(bounds=((ps.min(), ps.max()), (0.1, 20), (0.05, 5)),
+                                                           tol=1e-6, method="L-BFGS-B", args=(h, cdf)))
+    assert f <= bh.fun * (1 + 1e-6) + 1e-9
+
+
+# -------------------------------------------------------------- applyShifts
+def test_lanczos_integer_shift_is_roll_and_probe_values():
+    img = RNG.standard_normal((3, 20, 24))
+    t = np.array([[2.0, -3.0], [0.0, 0.0], [-5.0, 7.0]])
+    p = O.Params(shiftMethod="lanczos", originMode=None, subPixReg=16)
+    out, s, _ = O.apply_shifts(img, t, p)
+    for j in range(3):
+        assert np.array_equal(out[j], np.roll(np.roll(img[j], int(t[j, 0]), 0), int(t[j, 1]), 1))
+    # kernel centre of mass (applied sub-pixel shift) at rem=0.7 is +0.0095 px
+    k = O.lanczos_kernel([0.7, 0.7])
+    kx = k.sum(axis=0)
+    assert abs(np.dot(kx, [-1, 0, 1]) - 0.0095) < 5e-5
+    # t=0.25 applies -0.0417 px; rem -> 0 gives a half-pixel shift (the Q9 discontinuity)
+    assert abs(np.dot(O.lanczos_kernel([0.25, 0.25]).sum(axis=0), [-1, 0, 1]) + 1 / 24) < 1e-12
+    assert abs(np.dot(O.lanczos_kernel([1e-7, 1e-7]).sum(axis=0), [-1, 0, 1]) + 0.5) < 1e-6
+    assert np.isclose(k.sum(), 1.0)
+
+
+def test_lanczos_convolution_reflect_boundary():
+    img = RNG.standard_normal((1, 9, 11))
+    t = np.array([[0.3, 1.6]])
+    p = O.Params(shiftMethod="lanczos", originMode=None, subPixReg=16)
+    out, _, _ = O.apply_shifts(img, t, p)
+    g = np.roll(img[0], 2, axis=1)      # round(1.6) = 2, round(0.3) = 0
+    K = O.lanczos_kernel(np.remainder(t[0], 1))
+    P = np.pad(g, 1, mode="symmetric")
+    expect = np.zeros_like(g)
+    for i in range(9):
+        for j in range(11):
+            expect[i, j] = sum(K[a, b] * P[i + 2 - a, j + 2 - b] for a in range(3) for b in range(3))
+    assert np.allclose(out[0], expect)
+
+
+def test_bankers_rounding_of_half_shifts():
+    img = RNG.standard_normal((2, 8, 8))
+    t = np.array([[0.5, 2.5], [-0.5, 1.5]])
+    p = O.Params(shiftMethod=None, originMode=None, subPixReg=16)
+    out, _, _ = O.apply_shifts(img, t, p)
+    assert np.array_equal(out[0], np.roll(img[0], 2, axis=1))              # round(0.5)=0, round(2.5)=2
+    assert np.array_equal(out[1], np.roll(np.roll(img[1], 0, 0), 2, 1))    # round(-0.5)=0, round(1.5)=2
+
+
+# ---------------------------------------------------------------------- FRC
+def test_rotmean_constant_and_linear_profile():
+    r, ax = O.rotmean(np.full((64, 64), 3.0))
+    assert np.allclose(r[:45], 3.0)
+    yy, xx = np.mgrid[-32:32, -32:32]
+    r, _ = O.rotmean(np.sqrt(yy ** 2 + xx ** 2))
+    assert np.allclose(r[5:30], np.arange(5, 30), atol=0.05)
+
+
+def test_frc_identical_halves_is_one_and_noise_is_small():
+    a = RNG.standard_normal((700, 700))
+    trans = np.zeros((4, 2))
+    r, frc2, nt = O.tiled_frc(a, a.copy(), trans, box=128)
+    assert nt > 0 and np.allclose(r[:64], 1.0)
+    # positive means (like counts) make the DC term of every tile agree, so the
+    # [0,0] normalisation is the tile count
+    b = RNG.standard_normal((700, 700))
+    # (a small mean: the Hann window's own spectrum correlates the low rings)
+    r, _, nt = O.tiled_frc(a + 3, b + 3, trans, box=128)
+    assert np.all(np.abs(r[10:64]) < 6.0 / math.sqrt(nt * 2 * math.pi * 10))
+
+
+def test_frc_crop_limit_quirk():
+    t = np.array([[-3.2, 7.5], [2.0, -0.4]])
+    c = O.frc_crop_limits(t)
+    assert list(c) == [5, 8, 4, 1]        # end crops follow min(floor(.),5), not max
+
+
+# --------------------------------------------------------------- dose filter
+def test_dose_filter_constant_image_keeps_dc_raw_scaled():
+    n, h, w = 4, 32, 32
+    imgs = np.full((n, h, w), 2.5)
+    p = O.Params(pixelsize=0.1, filterMode="dose")
+    fs, info = O.dose_filter(imgs, imgs.sum(0), p, (h, w))
+    assert np.allclose(fs, n * 2.5 * h * w)
+    assert np.isclose(info["dosePerFrame"], 2.5)
+
+
+def test_dose_filter_single_cosine_follows_published_exposure_curve():
+    # one frame, cosine at (ky,kx) = (0,5): filtSum = |cos| * H*W * exp(-Ne/(2 Nc(q))) * cutoff
+    h = w = 64
+    yy, xx = np.mgrid[0:h, 0:w]
+    img = 10.0 + np.cos(2 * np.pi * 5 * xx / w)
+    p = O.Params(pixelsize=0.1, filterMode="dose")
+    fs, info = O.dose_filter(img[None], img, p, (h, w))
+    q = 5 / (w * 1.0)                                   # 1/Angstrom at 1 A/px
+    crit = 0.24499 * q ** -1.6649 + 2.8141              # Grant & Grigorieff 2015
+    dose = info["accumDose"][1]
+    tq = info["thresQ"][0]
+    g = math.exp(-0.5 * dose / crit) * math.exp(-(q / tq) ** 32)
+    expect = np.abs(10.0 + g * np.cos(2 * np.pi * 5 * xx / w)) * h * w
+    assert np.allclose(fs, expect, rtol=1e-10)
