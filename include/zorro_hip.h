@@ -1,0 +1,143 @@
+/* zorro_hip.h -- C ABI of the MI355X drift-registration library (libzorro_hip.so).
+ *
+ * The reference (LBEM-CH/zorro) has no FFI: its seam is the Python object
+ * zorro.ImageRegistrator (zorro/zorro.py:78), configured by attributes
+ * (zorro/zorro.py:82-243) and run by alignImageStack() (zorro/zorro.py:1569).
+ * Each entry point below replaces one stage of that method; the comment on
+ * each names the reference function it stands in for.  A ctypes binding that a
+ * maintainer would add beside the pyFFTW/numexprz path is shown in
+ * INTEGRATION.md; zorro_amd/_lib.py is that binding.
+ *
+ * Conventions
+ *   - every int-returning call returns ZR_OK (0) or a negative ZR_ERR_* code;
+ *     zr_last_error() gives the message of the last failure on this thread;
+ *   - shapes are row-major [N][H][W] float32, shifts are (y, x);
+ *   - host buffers are owned by the caller; device buffers by the movie;
+ *   - a zr_movie is bound to one device and one HIP stream and is not re-entrant;
+ *     stage calls are asynchronous on that stream, zr_get_* calls synchronise.
+ */
+#ifndef ZORRO_HIP_H
+#define ZORRO_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  ZR_OK = 0,
+  ZR_ERR_ARG = -1,          /* invalid argument / shape (reference: ValueError, IndexError) */
+  ZR_ERR_HIP = -2,          /* HIP runtime failure */
+  ZR_ERR_UNSUPPORTED = -3,  /* option outside the built path (reference: NotImplemented/broken mode) */
+  ZR_ERR_STATE = -4,        /* stage called before its inputs exist */
+  ZR_ERR_NOMEM = -5         /* device allocation failed */
+};
+
+/* triMode (zorro/zorro.py:207, :559-581) */
+enum { ZR_TRI_TRI = 0, ZR_TRI_DIAG = 1, ZR_TRI_AUTO = 2 };
+/* Bmode (zorro/zorro.py:117, :1104-1141) */
+enum { ZR_B_NONE = 0, ZR_B_CONV = 1, ZR_B_OPTI = 2 };
+/* shiftMethod (zorro/zorro.py:190, :1734-1808) */
+enum { ZR_SHIFT_INTEGER = 0, ZR_SHIFT_LANCZOS = 1 };
+/* weightMode (zorro/zorro.py:199, :1335-1360) */
+enum { ZR_W_AUTOLOGISTIC = 0, ZR_W_LOGISTIC = 1, ZR_W_CORR = 2, ZR_W_UNWEIGHTED = 3, ZR_W_NORM = 4 };
+
+/* The hot-path subset of ImageRegistrator's attributes (zorro/zorro.py:82-243). */
+typedef struct zr_params {
+  int tri_mode, diag_width, diag_start;   /* triMode, diagWidth, diagStart */
+  int max_shift;                          /* maxShift (cropped pixels) */
+  int b_mode;                             /* Bmode */
+  double b_rad;                           /* Brad */
+  int sub_pix_reg;                        /* subPixReg */
+  int shift_method;                       /* shiftMethod */
+  int weight_mode;                        /* weightMode */
+  double peaksig_thres, logistic_k, logistic_nu; /* fixed 'logistic' weights */
+  int origin_centroid;                    /* originMode == 'centroid' */
+  int suppress_origin;                    /* suppressOrigin */
+  int fou_crop[2];                        /* fouCrop; {0,0} = none */
+  int shape_padded[2];                    /* shapePadded */
+  int do_dose;                            /* 'dose' in filterMode */
+  double dose_a, dose_b, dose_c, dose_order, dose_missing; /* doseFiltParam[1..5] */
+  double pixelsize_nm;                    /* pixelsize (nm) */
+  double voltage_kv;                      /* voltage */
+  int do_frc;                             /* doLazyFRC */
+  int frc_box;                            /* tiledFRC box */
+  double frc_overlap;                     /* tiledFRC overlap */
+} zr_params;
+
+typedef struct zr_movie zr_movie;
+
+/* Message of the last failing call on this thread ("" if none). */
+const char* zr_last_error(void);
+
+/* Number of HIP devices visible. */
+int zr_device_count(int* n);
+
+/* Plan one movie shape: tables, pair list and device workspaces.
+ * Stands in for ImageRegistrator.__init_xcorrnm2 (zorro/zorro.py:540-637). */
+int zr_movie_create(int device, const zr_params* p, int n_frames, int h, int w, zr_movie** out);
+void zr_movie_destroy(zr_movie* mv);
+
+/* Run the movie's work on a caller-provided HIP stream (NULL: the movie's own). */
+int zr_movie_set_stream(zr_movie* mv, void* hip_stream);
+
+/* Provide the frames f32 [N][H][W].  on_device=1: a device pointer that the
+ * caller keeps alive (no copy); on_device=0: host memory, copied.
+ * Replaces the assignment of ImageRegistrator.images (zorro/zorro.py:116). */
+int zr_movie_set_frames(zr_movie* mv, const float* frames, int on_device);
+
+/* All-pairs masked normalised cross-correlation, peak location, significance
+ * and sub-pixel shifts into device trimats.
+ * Replaces xcorrnm2_tri + mnxc2 + locatePeak (zorro/zorro.py:773-1231). */
+int zr_xcorr(zr_movie* mv);
+
+/* Weights and the weighted-L1 shift solution (FP64), translations.
+ * Replaces shiftsSolver + cdfLogisticCurve (zorro/zorro.py:1260-1567, :1981-2022). */
+int zr_solve(zr_movie* mv);
+
+/* Lanczos/integer shift of every frame, imageSum and even/odd half sums;
+ * aligned_out (device, [N][H][W], nullable) receives the shifted frames.
+ * Replaces applyShifts (zorro/zorro.py:1734-1808) and the half sums of
+ * lazyFouRingCorr (zorro/zorro.py:2226-2235). */
+int zr_apply_shifts(zr_movie* mv, float* aligned_out);
+
+/* Tiled even/odd Fourier ring correlation.
+ * Replaces tiledFRC + rotmean (zorro/zorro.py:2246-2310, zorro/zorro_util.py:147-224). */
+int zr_frc(zr_movie* mv);
+
+/* Unblur-style dose-filtered sum (normalize=False).
+ * Replaces doseFilter (zorro/zorro.py:2349-2494). */
+int zr_dose_filter(zr_movie* mv);
+
+/* Every stage enabled by the parameters, in alignImageStack order
+ * (zorro/zorro.py:1614-1712). */
+int zr_align(zr_movie* mv);
+
+int zr_synchronize(zr_movie* mv);
+
+/* Result read-back (host buffers, synchronising).
+ * shifts f32 [N][N][2], corr/peaksig/origin f32 [N][N] (errorDict trimats). */
+int zr_get_trimats(zr_movie* mv, float* shifts, float* corr, float* peaksig, float* origin);
+/* windows f32 [P][2m][2m] in canonical pair order (row-major over I<J). */
+int zr_get_windows(zr_movie* mv, float* windows);
+int zr_pair_count(zr_movie* mv, int* npairs);
+/* translations f64 [N][2] (after the centroid when originMode='centroid'),
+ * relative_est f64 [N-1][2], weights f64 [M], logistic f64 [3] (thr, K, nu),
+ * error f64 [M][2] (errorXY), error_stats f64 [2] (mean/std errorNorm),
+ * error_xy_frames f64 [N][2] (errorX, errorY).  Any pointer may be NULL. */
+int zr_get_solution(zr_movie* mv, double* translations, double* relative_est, double* weights, double* logistic,
+                    double* error, double* error_stats, double* error_xy_frames);
+/* imageSum, filtSum, even and odd half sums f32 [H][W]; any may be NULL. */
+int zr_get_sums(zr_movie* mv, float* image_sum, float* filt_sum, float* even_sum, float* odd_sum);
+/* FRC f64 [box/2]; n receives box/2. */
+int zr_get_frc(zr_movie* mv, double* frc, int* n);
+
+/* Device pointers of results for zero-copy consumers (valid until destroy). */
+int zr_device_results(zr_movie* mv, const float** image_sum, const float** filt_sum, const double** translations);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -331,10 +331,11 @@ def xcorrnm2_tri(images: np.ndarray, mask: np.ndarray, p: Params, return_windows
         M, mp = mask_spectrum(mask[0], shapeC)
     sums = images.reshape(n, -1).sum(axis=1)
     sumsq = (images.reshape(n, -1) ** 2).sum(axis=1)
-    shifts = np.zeros([n, n, 2])
-    corr = np.zeros([n, n])
-    peaksig = np.zeros([n, n])
-    origin = np.zeros([n, n])
+    # trimats are float32 as in the reference (float_dtype, zorro/zorro.py:592-595)
+    shifts = np.zeros([n, n, 2], dtype=np.float32)
+    corr = np.zeros([n, n], dtype=np.float32)
+    peaksig = np.zeros([n, n], dtype=np.float32)
+    origin = np.zeros([n, n], dtype=np.float32)
     windows = {}
     for I in range(n):
         for J in np.nonzero(tri[I])[0]:
@@ -352,9 +353,9 @@ def xcorrnm2_tri(images: np.ndarray, mask: np.ndarray, p: Params, return_windows
                 windows[(I, J)] = W
             if p.triMode == "auto" and (ps <= p.peaksigThres):
                 break
-    if np.any(p.fouCrop):                                                  # :1004-1006
-        shifts[:, :, 0] *= shapeP[0] / shapeC[0]
-        shifts[:, :, 1] *= shapeP[1] / shapeC[1]
+    if np.any(p.fouCrop):                                                  # :1004-1006 (float32 multiply)
+        shifts[:, :, 0] *= np.float32(shapeP[0] / shapeC[0])
+        shifts[:, :, 1] *= np.float32(shapeP[1] / shapeC[1])
     out = dict(shiftsTriMat=shifts, corrTriMat=corr, peaksigTriMat=peaksig, originTriMat=origin,
                shapeCropped=shapeC)
     if return_windows:

@@ -1,0 +1,115 @@
+"""ctypes binding of libzorro_hip.so (declared in include/zorro_hip.h).
+
+This is the binding a maintainer would place beside the reference's
+pyFFTW/numexprz path.  Loading fails loudly when the library is missing: the
+product path has no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libzorro_hip.so")
+
+ZR_OK, ZR_ERR_ARG, ZR_ERR_HIP, ZR_ERR_UNSUPPORTED, ZR_ERR_STATE, ZR_ERR_NOMEM = 0, -1, -2, -3, -4, -5
+ZR_TRI_TRI, ZR_TRI_DIAG, ZR_TRI_AUTO = 0, 1, 2
+ZR_B_NONE, ZR_B_CONV, ZR_B_OPTI = 0, 1, 2
+ZR_SHIFT_INTEGER, ZR_SHIFT_LANCZOS = 0, 1
+ZR_W_AUTOLOGISTIC, ZR_W_LOGISTIC, ZR_W_CORR, ZR_W_UNWEIGHTED, ZR_W_NORM = 0, 1, 2, 3, 4
+
+
+class ZrParams(C.Structure):
+    _fields_ = [
+        ("tri_mode", C.c_int), ("diag_width", C.c_int), ("diag_start", C.c_int),
+        ("max_shift", C.c_int),
+        ("b_mode", C.c_int), ("b_rad", C.c_double),
+        ("sub_pix_reg", C.c_int),
+        ("shift_method", C.c_int),
+        ("weight_mode", C.c_int), ("peaksig_thres", C.c_double), ("logistic_k", C.c_double),
+        ("logistic_nu", C.c_double),
+        ("origin_centroid", C.c_int), ("suppress_origin", C.c_int),
+        ("fou_crop", C.c_int * 2), ("shape_padded", C.c_int * 2),
+        ("do_dose", C.c_int), ("dose_a", C.c_double), ("dose_b", C.c_double), ("dose_c", C.c_double),
+        ("dose_order", C.c_double), ("dose_missing", C.c_double),
+        ("pixelsize_nm", C.c_double), ("voltage_kv", C.c_double),
+        ("do_frc", C.c_int), ("frc_box", C.c_int), ("frc_overlap", C.c_double),
+    ]
+
+
+class ZrError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"libzorro_hip error {code}: {msg}")
+        self.code = code
+
+
+_P = C.c_void_p
+_F = C.POINTER(C.c_float)
+_D = C.POINTER(C.c_double)
+_I = C.POINTER(C.c_int)
+
+# name -> (restype, argtypes)
+SIGNATURES = {
+    "zr_last_error": (C.c_char_p, []),
+    "zr_device_count": (C.c_int, [_I]),
+    "zr_movie_create": (C.c_int, [C.c_int, C.POINTER(ZrParams), C.c_int, C.c_int, C.c_int, C.POINTER(_P)]),
+    "zr_movie_destroy": (None, [_P]),
+    "zr_movie_set_stream": (C.c_int, [_P, _P]),
+    "zr_movie_set_frames": (C.c_int, [_P, _P, C.c_int]),
+    "zr_xcorr": (C.c_int, [_P]),
+    "zr_solve": (C.c_int, [_P]),
+    "zr_apply_shifts": (C.c_int, [_P, _P]),
+    "zr_frc": (C.c_int, [_P]),
+    "zr_dose_filter": (C.c_int, [_P]),
+    "zr_align": (C.c_int, [_P]),
+    "zr_synchronize": (C.c_int, [_P]),
+    "zr_get_trimats": (C.c_int, [_P, _F, _F, _F, _F]),
+    "zr_get_windows": (C.c_int, [_P, _F]),
+    "zr_pair_count": (C.c_int, [_P, _I]),
+    "zr_get_solution": (C.c_int, [_P, _D, _D, _D, _D, _D, _D, _D]),
+    "zr_get_sums": (C.c_int, [_P, _F, _F, _F, _F]),
+    "zr_get_frc": (C.c_int, [_P, _D, _I]),
+    "zr_device_results": (C.c_int, [_P, C.POINTER(_F), C.POINTER(_F), C.POINTER(_D)]),
+}
+
+_lib = None
+
+
+def header_symbols():
+    """Function names declared in include/zorro_hip.h."""
+    import re
+    hdr = os.path.join(HERE, "..", "include", "zorro_hip.h")
+    txt = open(hdr).read()
+    return sorted(set(re.findall(r"\b(zr_[a-z_]+)\s*\(", txt)))
+
+
+def load(path: str = LIB_PATH):
+    """Load and type the library once; raises if it is not built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise ImportError(f"{path} is missing: build it with `python -m zorro_amd.build` "
+                          "(there is no CPU fallback for the HIP path)")
+    lib = C.CDLL(path)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def check(rc: int):
+    if rc != ZR_OK:
+        raise ZrError(rc, _lib.zr_last_error().decode())
+    return rc
+
+
+def fptr(a):
+    """float* of a C-contiguous float32 numpy array."""
+    return a.ctypes.data_as(_F)
+
+
+def dptr(a):
+    return a.ctypes.data_as(_D)

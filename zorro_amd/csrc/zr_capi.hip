@@ -1,0 +1,441 @@
+// C ABI of libzorro_hip.so: movie plans, device workspaces, stage sequencing.
+// See include/zorro_hip.h for the contract of every entry point.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/zorro_hip.h"
+#include "zr_launch.h"
+#include "zr_stages.h"
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+#define ZR_HIP(call)                                                                                \
+  do {                                                                                              \
+    hipError_t e_ = (call);                                                                         \
+    if (e_ != hipSuccess) return fail(ZR_ERR_HIP, "%s failed: %s (%s:%d)", #call, hipGetErrorString(e_), \
+                                      __FILE__, __LINE__);                                          \
+  } while (0)
+
+bool smooth235(int n) {
+  if (n <= 0) return false;
+  while (n % 2 == 0) n /= 2;
+  while (n % 3 == 0) n /= 3;
+  while (n % 5 == 0) n /= 5;
+  return n == 1;
+}
+
+bool supported_len(int n) {
+  static const int ok[] = {256, 512, 1024, 2048, 3072, 3840, 4096};
+  for (int v : ok)
+    if (v == n) return true;
+  return false;
+}
+
+// exp(-2 pi i t / n), t in [0, n)
+std::vector<float2> fwd_table(int n) {
+  std::vector<float2> t(n);
+  for (int i = 0; i < n; ++i) {
+    double a = -2.0 * M_PI * (double)i / (double)n;
+    t[i] = make_float2((float)cos(a), (float)sin(a));
+  }
+  return t;
+}
+
+template <typename T>
+int dev_alloc(T** p, size_t count) {
+  *p = nullptr;
+  if (count == 0) return ZR_OK;
+  hipError_t e = hipMalloc((void**)p, count * sizeof(T));
+  if (e != hipSuccess) return fail(ZR_ERR_NOMEM, "hipMalloc of %zu bytes failed: %s", count * sizeof(T), hipGetErrorString(e));
+  return ZR_OK;
+}
+
+template <typename T>
+int dev_upload(T** p, const std::vector<T>& v) {
+  int rc = dev_alloc(p, v.size());
+  if (rc) return rc;
+  if (!v.empty()) {
+    hipError_t e = hipMemcpy(*p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice);
+    if (e != hipSuccess) return fail(ZR_ERR_HIP, "hipMemcpy upload failed: %s", hipGetErrorString(e));
+  }
+  return ZR_OK;
+}
+
+}  // namespace
+
+struct zr_movie {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipStream_t own_stream = nullptr;
+  zr_params p{};
+  int N = 0, H = 0, W = 0, Hp = 0, Wp = 0, Hc = 0, Wc = 0, Wh = 0, m = 0, Lper = 0;
+  bool crop = false, unmasked = true;
+  int batch = 1, rb = 4;
+  // frames
+  const float* frames = nullptr;
+  float* frames_owned = nullptr;
+  // tables
+  float2 *twWp2 = nullptr, *twWp = nullptr, *twHp = nullptr, *twHc = nullptr, *twWc = nullptr, *ktab = nullptr;
+  float *gy = nullptr, *gx = nullptr;
+  int ry = 0, rx = 0;
+  // pairs
+  int P = 0;
+  int4 *pairs = nullptr, *work = nullptr;
+  std::vector<int4> pairs_host;
+  // workspaces
+  float2 *T = nullptr, *S = nullptr, *Sside = nullptr, *Z = nullptr;
+  double *part = nullptr, *stats = nullptr, *den = nullptr;
+  float* windows = nullptr;
+  float *shifts = nullptr, *corr = nullptr, *peaksig = nullptr, *origin = nullptr;
+  bool have_xcorr = false;
+  zr_stage_state* st = nullptr;   // solver / shift / FRC / dose state (zr_stages.h)
+};
+
+extern "C" const char* zr_last_error(void) { return g_err.c_str(); }
+
+extern "C" int zr_device_count(int* n) {
+  if (!n) return fail(ZR_ERR_ARG, "n is NULL");
+  ZR_HIP(hipGetDeviceCount(n));
+  return ZR_OK;
+}
+
+static void free_movie(zr_movie* mv) {
+  if (!mv) return;
+  hipSetDevice(mv->device);
+  void* ptrs[] = {mv->frames_owned, mv->twWp2, mv->twWp, mv->twHp, mv->twHc, mv->twWc, mv->ktab, mv->gy, mv->gx,
+                  mv->pairs, mv->work, mv->T, mv->S, mv->Sside, mv->Z, mv->part, mv->stats, mv->den,
+                  mv->windows, mv->shifts, mv->corr, mv->peaksig, mv->origin};
+  for (void* q : ptrs)
+    if (q) hipFree(q);
+  zr_stage_free(mv->st);
+  if (mv->own_stream) hipStreamDestroy(mv->own_stream);
+  delete mv;
+}
+
+extern "C" void zr_movie_destroy(zr_movie* mv) { free_movie(mv); }
+
+static int build_pairs(zr_movie* mv) {
+  const zr_params& p = mv->p;
+  const int N = mv->N;
+  std::vector<int4> pr;
+  for (int I = 0; I < N; ++I)
+    for (int J = I + 1; J < N; ++J) {
+      int d = J - I;
+      bool take = false;
+      if (p.tri_mode == ZR_TRI_TRI) take = true;
+      else if (p.tri_mode == ZR_TRI_DIAG) {
+        int wdt = p.diag_width < 0 ? N : p.diag_width;
+        take = (d <= wdt + p.diag_start) && (d > p.diag_start);
+      } else return fail(ZR_ERR_UNSUPPORTED, "triMode %d is not built (tri, diag only)", p.tri_mode);
+      if (take) pr.push_back(make_int4(I, J, (int)pr.size(), 0));
+    }
+  if (pr.empty()) return fail(ZR_ERR_ARG, "no frame pairs to correlate");
+  mv->P = (int)pr.size();
+  mv->pairs_host = pr;
+  // work order: 8x8 frame tiles so concurrently running blocks share spectra in L2
+  std::vector<int4> wk = pr;
+  std::stable_sort(wk.begin(), wk.end(), [](const int4& a, const int4& b) {
+    int ta = (a.x / 8) * 1024 + a.y / 8, tb = (b.x / 8) * 1024 + b.y / 8;
+    return ta < tb;
+  });
+  int rc = dev_upload(&mv->pairs, pr);
+  if (rc) return rc;
+  return dev_upload(&mv->work, wk);
+}
+
+extern "C" int zr_movie_create(int device, const zr_params* p, int n_frames, int h, int w, zr_movie** out) {
+  if (!p || !out) return fail(ZR_ERR_ARG, "NULL argument");
+  *out = nullptr;
+  if (n_frames < 2 || h < 16 || w < 16) return fail(ZR_ERR_ARG, "need >= 2 frames of >= 16x16 (got %d x %d x %d)", n_frames, h, w);
+  zr_movie* mv = new zr_movie();
+  mv->device = device;
+  mv->p = *p;
+  mv->N = n_frames;
+  mv->H = h;
+  mv->W = w;
+  hipError_t e = hipSetDevice(device);
+  if (e != hipSuccess) { free_movie(mv); return fail(ZR_ERR_HIP, "hipSetDevice(%d): %s", device, hipGetErrorString(e)); }
+  mv->Hp = p->shape_padded[0] > 0 ? p->shape_padded[0] : h;
+  mv->Wp = p->shape_padded[1] > 0 ? p->shape_padded[1] : w;
+  if (mv->Hp < h || mv->Wp < w) { free_movie(mv); return fail(ZR_ERR_ARG, "shapePadded (%d,%d) smaller than frames (%d,%d)", mv->Hp, mv->Wp, h, w); }
+  mv->unmasked = (mv->Hp == h && mv->Wp == w);
+  if (!mv->unmasked) { free_movie(mv); return fail(ZR_ERR_UNSUPPORTED, "padded frames (masked correlation) are not built yet: frames %dx%d, shapePadded %dx%d", h, w, mv->Hp, mv->Wp); }
+  mv->crop = (p->fou_crop[0] != 0 || p->fou_crop[1] != 0);
+  if (mv->crop) {
+    mv->Hc = 2 * (p->fou_crop[0] / 2);
+    mv->Wc = 2 * (p->fou_crop[1] / 2);
+    if (mv->Hc > mv->Hp || mv->Wc > mv->Wp || mv->Hc <= 0 || mv->Wc <= 0) { free_movie(mv); return fail(ZR_ERR_ARG, "fouCrop must be in (0, shapePadded]"); }
+  } else {
+    mv->Hc = mv->Hp;
+    mv->Wc = mv->Wp;
+  }
+  if (!supported_len(mv->Wp) || !supported_len(mv->Hp)) { free_movie(mv); return fail(ZR_ERR_UNSUPPORTED, "shapePadded %dx%d: built FFT lengths are 256..4096 (2^a, 3072, 3840)", mv->Hp, mv->Wp); }
+  {
+    const int l = mv->Hc / 64;
+    if (mv->Hc % 64 != 0 || !(l == 4 || l == 8 || l == 16 || l == 32 || l == 48 || l == 64)) { free_movie(mv); return fail(ZR_ERR_UNSUPPORTED, "cropped height %d: built sizes are 256, 512, 1024, 2048, 3072, 4096", mv->Hc); }
+  }
+  if (!smooth235(mv->Wc)) { free_movie(mv); return fail(ZR_ERR_UNSUPPORTED, "cropped width %d is not 2^a3^b5^c", mv->Wc); }
+  mv->Lper = mv->Hc / 64;
+  mv->Wh = mv->Wc / 2 + 1;
+  mv->m = p->max_shift;
+  if (mv->m <= 0 || 2 * mv->m > 128 || 2 * mv->m > mv->Hc || 2 * mv->m > mv->Wc) { free_movie(mv); return fail(ZR_ERR_UNSUPPORTED, "maxShift %d: built range is 1..64 (and 2*maxShift <= cropped size)", mv->m); }
+  if (p->b_mode == ZR_B_OPTI) { free_movie(mv); return fail(ZR_ERR_UNSUPPORTED, "Bmode='opti' is not built yet; use 'conv' or none"); }
+  if (p->sub_pix_reg > 1 && (p->sub_pix_reg & (p->sub_pix_reg - 1)) != 0) { free_movie(mv); return fail(ZR_ERR_UNSUPPORTED, "subPixReg must be a power of two"); }
+  if (p->sub_pix_reg > 32) { free_movie(mv); return fail(ZR_ERR_UNSUPPORTED, "subPixReg > 32 is not built"); }
+  int rc;
+  e = hipStreamCreateWithFlags(&mv->own_stream, hipStreamNonBlocking);
+  if (e != hipSuccess) { free_movie(mv); return fail(ZR_ERR_HIP, "hipStreamCreate: %s", hipGetErrorString(e)); }
+  mv->stream = mv->own_stream;
+  // tables
+  if ((rc = dev_upload(&mv->twWp2, fwd_table(mv->Wp / 2))) || (rc = dev_upload(&mv->twWp, fwd_table(mv->Wp))) ||
+      (rc = dev_upload(&mv->twHp, fwd_table(mv->Hp))) || (rc = dev_upload(&mv->twHc, fwd_table(mv->Hc))) ||
+      (rc = dev_upload(&mv->twWc, fwd_table(mv->Wc)))) { free_movie(mv); return rc; }
+  {
+    int P = p->sub_pix_reg > 1 ? p->sub_pix_reg : 1;
+    int G = 16 * P;
+    std::vector<float2> kt(G);
+    for (int t = 0; t < G; ++t) {
+      double re = 0, im = 0;
+      for (int f = -8; f < 8; ++f) {
+        double a = 2.0 * M_PI * f * t / G;
+        re += cos(a);
+        im += sin(a);
+      }
+      kt[t] = make_float2((float)re, (float)im);
+    }
+    if ((rc = dev_upload(&mv->ktab, kt))) { free_movie(mv); return rc; }
+  }
+  {
+    // scipy.ndimage.gaussian_filter weights: radius int(4 sigma + 0.5), exp(-x^2/(2 sigma^2)) / sum
+    std::vector<float> wy(1, 1.f), wx(1, 1.f);
+    if (p->b_mode == ZR_B_CONV && p->b_rad > 0) {
+      for (int ax = 0; ax < 2; ++ax) {
+        double sig = (ax == 0 ? mv->Hp : mv->Wp) / (std::sqrt(2.0) * M_PI * p->b_rad);
+        int r = (int)(4.0 * sig + 0.5);
+        std::vector<double> k(2 * r + 1);
+        double s = 0;
+        for (int i = -r; i <= r; ++i) { k[i + r] = std::exp(-0.5 / (sig * sig) * i * i); s += k[i + r]; }
+        std::vector<float>& o = ax == 0 ? wy : wx;
+        o.resize(2 * r + 1);
+        for (int i = 0; i <= 2 * r; ++i) o[i] = (float)(k[i] / s);
+        (ax == 0 ? mv->ry : mv->rx) = r;
+      }
+    }
+    if ((rc = dev_upload(&mv->gy, wy)) || (rc = dev_upload(&mv->gx, wx))) { free_movie(mv); return rc; }
+  }
+  if ((rc = build_pairs(mv))) { free_movie(mv); return rc; }
+  // workspaces
+  const size_t N = mv->N;
+  mv->batch = (int)std::min<size_t>(N, 8);
+  mv->rb = (mv->Wp >= 4096) ? 4 : 8;
+  const int nrb = mv->Hp / mv->rb;
+  if ((rc = dev_alloc(&mv->T, (size_t)mv->batch * mv->Wh * mv->Hp)) ||
+      (rc = dev_alloc(&mv->S, N * mv->Wh * mv->Hc)) || (rc = dev_alloc(&mv->Sside, N * mv->Wh)) ||
+      (rc = dev_alloc(&mv->Z, (size_t)mv->P * mv->Wh * 2 * mv->m)) || (rc = dev_alloc(&mv->part, N * nrb * 2)) ||
+      (rc = dev_alloc(&mv->stats, N * 2)) || (rc = dev_alloc(&mv->den, (size_t)mv->P)) ||
+      (rc = dev_alloc(&mv->windows, (size_t)mv->P * 4 * mv->m * mv->m)) || (rc = dev_alloc(&mv->shifts, N * N * 2)) ||
+      (rc = dev_alloc(&mv->corr, N * N)) || (rc = dev_alloc(&mv->peaksig, N * N)) || (rc = dev_alloc(&mv->origin, N * N))) {
+    free_movie(mv);
+    return rc;
+  }
+  if ((rc = zr_stage_create(&mv->st, mv->p, mv->N, mv->H, mv->W, mv->Hp, mv->Wp, mv->pairs_host))) {
+    free_movie(mv);
+    return fail(rc, "%s", zr_stage_error());
+  }
+  *out = mv;
+  return ZR_OK;
+}
+
+extern "C" int zr_movie_set_stream(zr_movie* mv, void* s) {
+  if (!mv) return fail(ZR_ERR_ARG, "NULL movie");
+  mv->stream = s ? (hipStream_t)s : mv->own_stream;
+  return ZR_OK;
+}
+
+extern "C" int zr_movie_set_frames(zr_movie* mv, const float* frames, int on_device) {
+  if (!mv || !frames) return fail(ZR_ERR_ARG, "NULL argument");
+  ZR_HIP(hipSetDevice(mv->device));
+  if (on_device) {
+    mv->frames = frames;
+  } else {
+    size_t bytes = (size_t)mv->N * mv->H * mv->W * sizeof(float);
+    if (!mv->frames_owned) ZR_HIP(hipMalloc((void**)&mv->frames_owned, bytes));
+    ZR_HIP(hipMemcpyAsync(mv->frames_owned, frames, bytes, hipMemcpyHostToDevice, mv->stream));
+    mv->frames = mv->frames_owned;
+  }
+  mv->have_xcorr = false;
+  return ZR_OK;
+}
+
+extern "C" int zr_xcorr(zr_movie* mv) {
+  if (!mv) return fail(ZR_ERR_ARG, "NULL movie");
+  if (!mv->frames) return fail(ZR_ERR_STATE, "frames not set");
+  ZR_HIP(hipSetDevice(mv->device));
+  hipStream_t s = mv->stream;
+  int rb = 0;
+  // K1: spectra in batches of frames
+  for (int f0 = 0; f0 < mv->N; f0 += mv->batch) {
+    int nb = std::min(mv->batch, mv->N - f0);
+    ZR_HIP(zr_launch_k1_rows(mv->Wp, mv->frames, mv->H, mv->W, mv->Hp, mv->Wh, f0, nb, mv->twWp2, mv->twWp, mv->T,
+                             mv->part, &rb, s));
+    ZR_HIP(zr_launch_k1_cols(mv->Hp, mv->T, mv->Wh, mv->Hc, f0, nb, mv->twHp, mv->S, mv->Sside, s));
+  }
+  ZR_HIP(zr_launch_k1_stats(mv->part, mv->Hp / rb, mv->N, mv->stats, s));
+  const double nc2 = 1.0 / std::pow((double)mv->Hc * (double)mv->Wc, 2.0);
+  ZR_HIP(zr_launch_k2_den(mv->pairs, mv->P, mv->stats, (double)mv->Hp * (double)mv->Wp, nc2, mv->den, s));
+  // K2
+  ZR_HIP(zr_launch_k2_cols(mv->Lper, mv->S, mv->Sside, mv->work, mv->P, mv->Wh, mv->Hc, mv->Wc, mv->m, 8,
+                           mv->unmasked ? 1 : 0, mv->twHc, mv->Z, s));
+  ZR_HIP(zr_launch_k2_rows(mv->Z, mv->pairs, mv->P, mv->Wh, mv->Wc, mv->m, mv->twWc, mv->den, (float)nc2,
+                           mv->windows, s));
+  // peaks
+  ZR_HIP(hipMemsetAsync(mv->shifts, 0, sizeof(float) * 2 * mv->N * mv->N, s));
+  ZR_HIP(hipMemsetAsync(mv->corr, 0, sizeof(float) * mv->N * mv->N, s));
+  ZR_HIP(hipMemsetAsync(mv->peaksig, 0, sizeof(float) * mv->N * mv->N, s));
+  ZR_HIP(hipMemsetAsync(mv->origin, 0, sizeof(float) * mv->N * mv->N, s));
+  PeakParamsHost hp;
+  hp.m = mv->m;
+  hp.suppress = mv->p.suppress_origin ? 1 : 0;
+  hp.bconv = (mv->p.b_mode == ZR_B_CONV && mv->p.b_rad > 0) ? 1 : 0;
+  hp.ry = mv->ry;
+  hp.rx = mv->rx;
+  hp.subpix = mv->p.sub_pix_reg;
+  hp.scale_y = mv->crop ? (float)((double)mv->Hp / (double)mv->Hc) : 1.f;
+  hp.scale_x = mv->crop ? (float)((double)mv->Wp / (double)mv->Wc) : 1.f;
+  hp.n = mv->N;
+  ZR_HIP(zr_launch_peaks(mv->windows, mv->pairs, mv->P, &hp, mv->gy, mv->gx, mv->ktab, mv->shifts, mv->corr,
+                         mv->peaksig, mv->origin, s));
+  mv->have_xcorr = true;
+  return ZR_OK;
+}
+
+extern "C" int zr_solve(zr_movie* mv) {
+  if (!mv) return fail(ZR_ERR_ARG, "NULL movie");
+  if (!mv->have_xcorr) return fail(ZR_ERR_STATE, "zr_solve before zr_xcorr");
+  ZR_HIP(hipSetDevice(mv->device));
+  int rc = zr_stage_solve(mv->st, mv->shifts, mv->corr, mv->peaksig, mv->stream);
+  return rc ? fail(rc, "%s", zr_stage_error()) : ZR_OK;
+}
+
+extern "C" int zr_apply_shifts(zr_movie* mv, float* aligned_out) {
+  if (!mv) return fail(ZR_ERR_ARG, "NULL movie");
+  if (!mv->frames) return fail(ZR_ERR_STATE, "frames not set");
+  ZR_HIP(hipSetDevice(mv->device));
+  int rc = zr_stage_shifts(mv->st, mv->frames, aligned_out, mv->stream);
+  return rc ? fail(rc, "%s", zr_stage_error()) : ZR_OK;
+}
+
+extern "C" int zr_frc(zr_movie* mv) {
+  if (!mv) return fail(ZR_ERR_ARG, "NULL movie");
+  ZR_HIP(hipSetDevice(mv->device));
+  int rc = zr_stage_frc(mv->st, mv->stream);
+  return rc ? fail(rc, "%s", zr_stage_error()) : ZR_OK;
+}
+
+extern "C" int zr_dose_filter(zr_movie* mv) {
+  if (!mv) return fail(ZR_ERR_ARG, "NULL movie");
+  ZR_HIP(hipSetDevice(mv->device));
+  int rc = zr_stage_dose(mv->st, mv->frames, mv->stream);
+  return rc ? fail(rc, "%s", zr_stage_error()) : ZR_OK;
+}
+
+extern "C" int zr_align(zr_movie* mv) {
+  int rc;
+  if ((rc = zr_xcorr(mv))) return rc;
+  if ((rc = zr_solve(mv))) return rc;
+  if (mv->p.do_dose) {
+    // the dose stage computes the aligned frames, sums and half sums itself
+    if ((rc = zr_dose_filter(mv))) return rc;
+  } else if ((rc = zr_apply_shifts(mv, nullptr))) {
+    return rc;
+  }
+  if (mv->p.do_frc && (rc = zr_frc(mv))) return rc;
+  return ZR_OK;
+}
+
+extern "C" int zr_synchronize(zr_movie* mv) {
+  if (!mv) return fail(ZR_ERR_ARG, "NULL movie");
+  ZR_HIP(hipSetDevice(mv->device));
+  ZR_HIP(hipStreamSynchronize(mv->stream));
+  return ZR_OK;
+}
+
+extern "C" int zr_pair_count(zr_movie* mv, int* npairs) {
+  if (!mv || !npairs) return fail(ZR_ERR_ARG, "NULL argument");
+  *npairs = mv->P;
+  return ZR_OK;
+}
+
+extern "C" int zr_get_trimats(zr_movie* mv, float* shifts, float* corr, float* peaksig, float* origin) {
+  if (!mv) return fail(ZR_ERR_ARG, "NULL movie");
+  if (!mv->have_xcorr) return fail(ZR_ERR_STATE, "no correlation results");
+  ZR_HIP(hipSetDevice(mv->device));
+  ZR_HIP(hipStreamSynchronize(mv->stream));
+  size_t nn = (size_t)mv->N * mv->N;
+  if (shifts) ZR_HIP(hipMemcpy(shifts, mv->shifts, nn * 2 * sizeof(float), hipMemcpyDeviceToHost));
+  if (corr) ZR_HIP(hipMemcpy(corr, mv->corr, nn * sizeof(float), hipMemcpyDeviceToHost));
+  if (peaksig) ZR_HIP(hipMemcpy(peaksig, mv->peaksig, nn * sizeof(float), hipMemcpyDeviceToHost));
+  if (origin) ZR_HIP(hipMemcpy(origin, mv->origin, nn * sizeof(float), hipMemcpyDeviceToHost));
+  return ZR_OK;
+}
+
+extern "C" int zr_get_windows(zr_movie* mv, float* windows) {
+  if (!mv || !windows) return fail(ZR_ERR_ARG, "NULL argument");
+  if (!mv->have_xcorr) return fail(ZR_ERR_STATE, "no correlation results");
+  ZR_HIP(hipSetDevice(mv->device));
+  ZR_HIP(hipStreamSynchronize(mv->stream));
+  ZR_HIP(hipMemcpy(windows, mv->windows, (size_t)mv->P * 4 * mv->m * mv->m * sizeof(float), hipMemcpyDeviceToHost));
+  return ZR_OK;
+}
+
+extern "C" int zr_get_solution(zr_movie* mv, double* translations, double* relative_est, double* weights,
+                               double* logistic, double* error, double* error_stats, double* error_xy_frames) {
+  if (!mv) return fail(ZR_ERR_ARG, "NULL movie");
+  ZR_HIP(hipSetDevice(mv->device));
+  ZR_HIP(hipStreamSynchronize(mv->stream));
+  int rc = zr_stage_get_solution(mv->st, translations, relative_est, weights, logistic, error, error_stats,
+                                 error_xy_frames);
+  return rc ? fail(rc, "%s", zr_stage_error()) : ZR_OK;
+}
+
+extern "C" int zr_get_sums(zr_movie* mv, float* image_sum, float* filt_sum, float* even_sum, float* odd_sum) {
+  if (!mv) return fail(ZR_ERR_ARG, "NULL movie");
+  ZR_HIP(hipSetDevice(mv->device));
+  ZR_HIP(hipStreamSynchronize(mv->stream));
+  int rc = zr_stage_get_sums(mv->st, image_sum, filt_sum, even_sum, odd_sum);
+  return rc ? fail(rc, "%s", zr_stage_error()) : ZR_OK;
+}
+
+extern "C" int zr_get_frc(zr_movie* mv, double* frc, int* n) {
+  if (!mv) return fail(ZR_ERR_ARG, "NULL movie");
+  ZR_HIP(hipSetDevice(mv->device));
+  ZR_HIP(hipStreamSynchronize(mv->stream));
+  int rc = zr_stage_get_frc(mv->st, frc, n);
+  return rc ? fail(rc, "%s", zr_stage_error()) : ZR_OK;
+}
+
+extern "C" int zr_device_results(zr_movie* mv, const float** image_sum, const float** filt_sum,
+                                 const double** translations) {
+  if (!mv) return fail(ZR_ERR_ARG, "NULL movie");
+  int rc = zr_stage_device_results(mv->st, image_sum, filt_sum, translations);
+  return rc ? fail(rc, "%s", zr_stage_error()) : ZR_OK;
+}

@@ -1,0 +1,27 @@
+// Internal launcher declarations shared by the kernel files and the C ABI.
+#pragma once
+#include <hip/hip_runtime.h>
+
+struct PeakParamsHost {
+  int m, suppress, bconv, ry, rx, subpix;
+  float scale_y, scale_x;
+  int n;
+};
+
+// zr_xcorr.hip
+hipError_t zr_launch_k1_rows(int Wp, const float* frames, int H, int W, int Hp, int Wh, int frame0, int nb,
+                             const float2* twL, const float2* tw2L, float2* T, double* part, int* rb_out,
+                             hipStream_t st);
+hipError_t zr_launch_k1_cols(int Hp, const float2* T, int Wh, int Hc, int frame0, int nb, const float2* twH,
+                             float2* S, float2* Sside, hipStream_t st);
+hipError_t zr_launch_k1_stats(const double* part, int nrb, int n, double* stats, hipStream_t st);
+hipError_t zr_launch_k2_cols(int Lper, const float2* S, const float2* Sside, const int4* work, int npairs, int Wh,
+                             int Hc, int Wc, int m, int cols_per_wave, int dc_zero, const float2* twHc, float2* Z,
+                             hipStream_t st);
+hipError_t zr_launch_k2_rows(const float2* Z, const int4* pairs, int npairs, int Wh, int Wc, int m,
+                             const float2* twWc, const double* den, float nc2, float* windows, hipStream_t st);
+hipError_t zr_launch_k2_den(const int4* pairs, int npairs, const double* stats, double npix, double nc2, double* den,
+                            hipStream_t st);
+hipError_t zr_launch_peaks(const float* windows, const int4* pairs, int npairs, const PeakParamsHost* hp,
+                           const float* gy, const float* gx, const float2* ktab, float* shifts, float* corr,
+                           float* peaksig, float* origin, hipStream_t st);

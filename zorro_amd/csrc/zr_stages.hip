@@ -1,0 +1,1307 @@
+// Stages after the pair correlation, all on the GPU:
+//   S1  peak-significance histogram/CDF and the logistic weight fit (FP64)
+//   S2  weighted-L1 shift solve per axis (FP64): IRLS warm start, then exact
+//       network-simplex descent over spanning trees of the frame-position graph
+//   S3  per-frame shift parameters, translations, error statistics, FRC crops
+//   K3  Lanczos/integer shift + imageSum + even/odd half sums
+//   K4  tiled Hann FRC (row FFTs, column FFTs with FP64 accumulation, ring means)
+//   KD  dose filter (aligned-row FFTs, per-column filtered accumulation, inverse)
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "zr_common.h"
+#include "zr_stages.h"
+
+// ---------------------------------------------------------------- constants
+// Deterministic logistic-fit schedule (same constants as the oracle).
+constexpr int LOGI_G0 = 32, LOGI_G1 = 32, LOGI_G2 = 32;
+constexpr int LOGI_ROUNDS = 30, LOGI_MAX_ITERS = 400;
+constexpr double LOGI_K0 = 0.1, LOGI_K1 = 20.0, LOGI_N0 = 0.05, LOGI_N1 = 5.0;
+constexpr int MAXN = 64;          // frames handled by the one-wave solver
+constexpr int IRLS_ITERS = 24;
+constexpr int MAX_PIVOTS = 400;
+
+struct FrameShift {
+  int iy, ix, conv, pad;
+  float k[9];
+};
+
+struct SolveInfo {           // device-resident scalars of the solve
+  int nps, nbins, status_x, status_y;
+  int pivots_x, pivots_y, tiles_y, tiles_x;
+  int crop[4];
+  double logi[3];
+  double dpf;
+};
+
+// ------------------------------------------------------------------ S1
+ZR_DEV float f32_sub(float a, float b) { return __fsub_rn(a, b); }
+ZR_DEV float f32_mul(float a, float b) { return __fmul_rn(a, b); }
+ZR_DEV float f32_add(float a, float b) { return __fadd_rn(a, b); }
+ZR_DEV float f32_div(float a, float b) { return __fdiv_rn(a, b); }
+
+// Histogram of positive peak significances (numpy.histogram with an integer
+// bin count on float32 data: float32 edges from linspace, index by scaling,
+// then the +-1 edge corrections), CDF in float32.
+__global__ __launch_bounds__(256) void k_ps_hist(const float* __restrict__ peaksig, int nn, float* __restrict__ hs,
+                                                 float* __restrict__ cdf, SolveInfo* info, int maxbins) {
+  __shared__ float smin[4], smax[4];
+  __shared__ int scnt[4];
+  __shared__ unsigned counts[2048];
+  const int tid = threadIdx.x;
+  float mn = INFINITY, mx = -INFINITY;
+  int c = 0;
+  for (int i = tid; i < nn; i += 256) {
+    float v = peaksig[i];
+    if (v > 0.f) { mn = fminf(mn, v); mx = fmaxf(mx, v); ++c; }
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    mn = fminf(mn, __shfl_xor(mn, o, 64));
+    mx = fmaxf(mx, __shfl_xor(mx, o, 64));
+    c += __shfl_xor(c, o, 64);
+  }
+  if ((tid & 63) == 0) { smin[tid >> 6] = mn; smax[tid >> 6] = mx; scnt[tid >> 6] = c; }
+  __syncthreads();
+  mn = fminf(fminf(smin[0], smin[1]), fminf(smin[2], smin[3]));
+  mx = fmaxf(fmaxf(smax[0], smax[1]), fmaxf(smax[2], smax[3]));
+  c = scnt[0] + scnt[1] + scnt[2] + scnt[3];
+  int nb = (int)((double)c / 7.0);
+  if (nb < 1 || nb > maxbins || nb > 2048) {
+    if (tid == 0) { info->nps = c; info->nbins = 0; }
+    return;
+  }
+  float first = mn, last = mx;
+  if (first == last) { first = f32_sub(first, 0.5f); last = f32_add(last, 0.5f); }
+  const float delta = f32_sub(last, first);
+  const float step = f32_div(delta, (float)nb);
+  for (int i = tid; i < nb; i += 256) counts[i] = 0;
+  __syncthreads();
+  auto edge = [&](int i) -> float { return i == nb ? last : f32_add(f32_mul((float)i, step), first); };
+  for (int i = tid; i < nn; i += 256) {
+    float v = peaksig[i];
+    if (!(v > 0.f)) continue;
+    float fi = f32_mul(f32_div(f32_sub(v, first), delta), (float)nb);
+    int k = (int)fi;
+    if (k == nb) k -= 1;
+    if (v < edge(k)) k -= 1;
+    if (v >= edge(k + 1) && k != nb - 1) k += 1;
+    atomicAdd(&counts[k], 1u);
+  }
+  __syncthreads();
+  if (tid == 0) {
+    float run = 0.f;
+    for (int i = 0; i < nb; ++i) { run = f32_add(run, (float)counts[i]); cdf[i] = run; }
+    for (int i = 0; i < nb; ++i) { cdf[i] = f32_div(cdf[i], run); hs[i] = edge(i); }
+    info->nps = c;
+    info->nbins = nb;
+    info->logi[0] = mn;      // thr bounds of the fit: data min/max
+    info->logi[1] = mx;
+  }
+}
+
+ZR_DEV double logistic_d(double h, double thr, double K, double nu) {
+  return 1.0 - 1.0 / pow(1.0 + exp(-K * (-h + thr)), nu);
+}
+
+ZR_DEV double logi_obj(const float* hs, const float* cdf, int nb, double thr, double K, double nu) {
+  double s = 0.0;
+  for (int i = 0; i < nb; ++i) s += fabs((double)cdf[i] - logistic_d((double)hs[i], thr, K, nu));
+  return s;
+}
+
+__global__ __launch_bounds__(256) void k_logi_grid(const float* __restrict__ hs, const float* __restrict__ cdf,
+                                                  const SolveInfo* info, double* __restrict__ obj) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int total = LOGI_G0 * LOGI_G1 * LOGI_G2;
+  if (i >= total || info->nbins < 1) return;
+  const int a = i / (LOGI_G1 * LOGI_G2), b = (i / LOGI_G2) % LOGI_G1, c = i % LOGI_G2;
+  const double lo = info->logi[0], hi = info->logi[1];
+  const double thr = lo + (hi - lo) * (double)a / (LOGI_G0 - 1);
+  const double lk0 = log(LOGI_K0), lk1 = log(LOGI_K1), ln0 = log(LOGI_N0), ln1 = log(LOGI_N1);
+  const double lk = lk0 + (lk1 - lk0) * (double)b / (LOGI_G1 - 1);
+  const double ln = ln0 + (ln1 - ln0) * (double)c / (LOGI_G2 - 1);
+  obj[i] = logi_obj(hs, cdf, info->nbins, thr, exp(lk), exp(ln));
+}
+
+// argmin of the grid (first index) then the 3x3x3 pattern search.  One block of 64.
+__global__ __launch_bounds__(64) void k_logi_refine(const float* __restrict__ hs, const float* __restrict__ cdf,
+                                                   const double* __restrict__ obj, SolveInfo* info) {
+  const int lane = threadIdx.x;
+  if (info->nbins < 1) return;
+  const int total = LOGI_G0 * LOGI_G1 * LOGI_G2;
+  double bv = INFINITY;
+  int bi = 0x7fffffff;
+  for (int i = lane; i < total; i += 64) {
+    double v = obj[i];
+    if (v < bv || (v == bv && i < bi)) { bv = v; bi = i; }
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    double ov = __shfl_xor(bv, o, 64);
+    int oi = __shfl_xor(bi, o, 64);
+    if (ov < bv || (ov == bv && oi < bi)) { bv = ov; bi = oi; }
+  }
+  const double lo = info->logi[0], hi = info->logi[1];
+  const double lk0 = log(LOGI_K0), lk1 = log(LOGI_K1), ln0 = log(LOGI_N0), ln1 = log(LOGI_N1);
+  double cur[3];
+  {
+    int a = bi / (LOGI_G1 * LOGI_G2), b = (bi / LOGI_G2) % LOGI_G1, c = bi % LOGI_G2;
+    cur[0] = lo + (hi - lo) * (double)a / (LOGI_G0 - 1);
+    cur[1] = lk0 + (lk1 - lk0) * (double)b / (LOGI_G1 - 1);
+    cur[2] = ln0 + (ln1 - ln0) * (double)c / (LOGI_G2 - 1);
+  }
+  double fcur = bv;
+  double step[3] = {(hi - lo) / (LOGI_G0 - 1), (lk1 - lk0) / (LOGI_G1 - 1), (ln1 - ln0) / (LOGI_G2 - 1)};
+  const double lob[3] = {lo, lk0, ln0}, hib[3] = {hi, lk1, ln1};
+  const int nb = info->nbins;
+  int halvings = 0;
+  for (int it = 0; it < LOGI_MAX_ITERS; ++it) {
+    double v = INFINITY;
+    double cand[3] = {0, 0, 0};
+    if (lane < 27) {
+      int o0 = lane / 9 - 1, o1 = (lane / 3) % 3 - 1, o2 = lane % 3 - 1;
+      cand[0] = fmin(fmax(cur[0] + o0 * step[0], lob[0]), hib[0]);
+      cand[1] = fmin(fmax(cur[1] + o1 * step[1], lob[1]), hib[1]);
+      cand[2] = fmin(fmax(cur[2] + o2 * step[2], lob[2]), hib[2]);
+      v = logi_obj(hs, cdf, nb, cand[0], exp(cand[1]), exp(cand[2]));
+    }
+    double mv = v;
+    int mi = lane;
+    for (int o = 32; o > 0; o >>= 1) {
+      double ov = __shfl_xor(mv, o, 64);
+      int oi = __shfl_xor(mi, o, 64);
+      if (ov < mv || (ov == mv && oi < mi)) { mv = ov; mi = oi; }
+    }
+    if (mv < fcur) {
+      double c0 = __shfl(cand[0], mi, 64), c1 = __shfl(cand[1], mi, 64), c2 = __shfl(cand[2], mi, 64);
+      cur[0] = c0; cur[1] = c1; cur[2] = c2;
+      fcur = mv;
+    } else {
+      step[0] *= 0.5; step[1] *= 0.5; step[2] *= 0.5;
+      if (++halvings >= LOGI_ROUNDS) break;
+    }
+  }
+  if (lane == 0) {
+    info->logi[0] = cur[0];
+    info->logi[1] = exp(cur[1]);
+    info->logi[2] = exp(cur[2]);
+  }
+}
+
+// ------------------------------------------------------------------ S2
+struct SolveArgs {
+  const float* shifts;    // [N][N][2]
+  const float* corr;      // [N][N]
+  const float* peaksig;   // [N][N]
+  const int2* edges;      // [M] (I, J), canonical order
+  int N, M;
+  int weight_mode;
+  double logi_fixed[3];
+  double bound;
+  double* weights;        // [M]   (unclipped)
+  double* resid;          // [M][2]
+  double* pos;            // [2][N]  positions p (p_0 = 0)
+  SolveInfo* info;
+};
+
+// One wave per axis.  Solves min sum_e w_e |p_J - p_I - b_e| over positions.
+__global__ __launch_bounds__(64) void k_l1_solve(SolveArgs a) {
+  const int lane = threadIdx.x;
+  const int axis = blockIdx.x;            // 0: y (index 0), 1: x (index 1)
+  const int N = a.N, M = a.M, n1 = N - 1;
+  __shared__ double Lm[MAXN * (MAXN + 1)];
+  __shared__ double g[MAXN], pv[MAXN];
+  __shared__ int tree_e[MAXN], parent[MAXN], pedge[MAXN], order[MAXN];
+  __shared__ unsigned long long sub[MAXN];
+  __shared__ double slope_p[MAXN], slope_m[MAXN];
+  __shared__ int in_tree_flag[2048];
+  extern __shared__ double dyn[];         // b[M], w[M], r[M]
+  double* b = dyn;
+  double* w = b + M;
+  double* r = w + M;
+  const int LD = MAXN + 1;
+  // ---- data
+  for (int e = lane; e < M; e += 64) {
+    int2 ed = a.edges[e];
+    b[e] = (double)a.shifts[((size_t)ed.x * N + ed.y) * 2 + axis];
+    double ps = (double)a.peaksig[ed.x * N + ed.y];
+    double we;
+    switch (a.weight_mode) {
+      case 0: we = logistic_d(ps, a.info->logi[0], a.info->logi[1], a.info->logi[2]); break;
+      case 1: we = logistic_d(ps, a.logi_fixed[0], a.logi_fixed[1], a.logi_fixed[2]); break;
+      case 3: we = 1.0; break;
+      default: we = ps; break;
+    }
+    w[e] = we;
+  }
+  __syncthreads();
+  if (a.weight_mode == 4) {   // 'norm': (ps - min)/(max - min)
+    double mn = INFINITY, mx = -INFINITY;
+    for (int e = lane; e < M; e += 64) { mn = fmin(mn, w[e]); mx = fmax(mx, w[e]); }
+    for (int o = 32; o > 0; o >>= 1) { mn = fmin(mn, __shfl_xor(mn, o, 64)); mx = fmax(mx, __shfl_xor(mx, o, 64)); }
+    __syncthreads();
+    for (int e = lane; e < M; e += 64) w[e] = (w[e] - mn) / (mx - mn);
+    __syncthreads();
+  }
+  if (axis == 0)
+    for (int e = lane; e < M; e += 64) a.weights[e] = w[e];
+  for (int k = lane; k < N; k += 64) pv[k] = 0.0;
+  __syncthreads();
+  // ---- IRLS warm start on the position Laplacian (node 0 fixed)
+  double eps = 1.0;
+  for (int it = 0; it < IRLS_ITERS; ++it) {
+    for (int i = lane; i < N * LD; i += 64) Lm[i] = 0.0;
+    for (int k = lane; k < N; k += 64) g[k] = 0.0;
+    __syncthreads();
+    for (int e = lane; e < M; e += 64) {
+      int2 ed = a.edges[e];
+      double re = pv[ed.y] - pv[ed.x] - b[e];
+      double d = fmax(w[e], 1e-12) / fmax(fabs(re), eps);
+      Lm[ed.x * LD + ed.y] = -d;   // one edge per ordered pair
+      Lm[ed.y * LD + ed.x] = -d;
+    }
+    __syncthreads();
+    for (int k = lane; k < N; k += 64) {
+      double s = 0.0, gg = 0.0;
+      for (int j = 0; j < N; ++j) s -= (j == k) ? 0.0 : Lm[k * LD + j];
+      Lm[k * LD + k] = s;
+      // rhs: sum over edges incident to k of d_e * b_e with sign
+      for (int e = 0; e < M; ++e) {
+        int2 ed = a.edges[e];
+        if (ed.y == k) gg += -Lm[ed.x * LD + ed.y] * b[e];
+        else if (ed.x == k) gg -= -Lm[ed.x * LD + ed.y] * b[e];
+      }
+      g[k] = gg;
+    }
+    __syncthreads();
+    // Cholesky of rows/cols 1..N-1 in place (lower), forward/back substitution
+    for (int k = 1; k < N; ++k) {
+      if (lane == 0) {
+        double dkk = sqrt(fmax(Lm[k * LD + k], 1e-300));
+        Lm[k * LD + k] = dkk;
+      }
+      __syncthreads();
+      double dkk = Lm[k * LD + k];
+      for (int i = k + 1 + lane; i < N; i += 64) Lm[i * LD + k] /= dkk;
+      __syncthreads();
+      for (int i = k + 1 + lane; i < N; i += 64) {
+        double lik = Lm[i * LD + k];
+        for (int j = k + 1; j <= i; ++j) Lm[i * LD + j] -= lik * Lm[j * LD + k];
+      }
+      __syncthreads();
+    }
+    if (lane == 0) {
+      for (int i = 1; i < N; ++i) {       // L y = g
+        double s = g[i];
+        for (int j = 1; j < i; ++j) s -= Lm[i * LD + j] * g[j];
+        g[i] = s / Lm[i * LD + i];
+      }
+      for (int i = N - 1; i >= 1; --i) {  // L^T p = y
+        double s = g[i];
+        for (int j = i + 1; j < N; ++j) s -= Lm[j * LD + i] * pv[j];
+        pv[i] = s / Lm[i * LD + i];
+      }
+      pv[0] = 0.0;
+    }
+    __syncthreads();
+    eps = fmax(eps * 0.5, 1e-9);
+  }
+  // ---- initial tree: Kruskal over |residual| ascending (ties: edge index)
+  {
+    int P2 = 1;
+    while (P2 < M) P2 <<= 1;
+    double* key = Lm;                                   // reuse the matrix storage
+    int* val = reinterpret_cast<int*>(Lm + 2048);
+    for (int i = lane; i < P2; i += 64) {
+      if (i < M) {
+        int2 ed = a.edges[i];
+        key[i] = fabs(pv[ed.y] - pv[ed.x] - b[i]);
+        val[i] = i;
+      } else {
+        key[i] = INFINITY;
+        val[i] = 0x7fffffff;
+      }
+    }
+    __syncthreads();
+    for (int kk = 2; kk <= P2; kk <<= 1)
+      for (int jj = kk >> 1; jj > 0; jj >>= 1) {
+        for (int i = lane; i < P2; i += 64) {
+          int l = i ^ jj;
+          if (l > i) {
+            bool up = (i & kk) == 0;
+            double ki = key[i], kl = key[l];
+            int vi = val[i], vl = val[l];
+            bool gt = (ki > kl) || (ki == kl && vi > vl);
+            if (gt == up) { key[i] = kl; key[l] = ki; val[i] = vl; val[l] = vi; }
+          }
+        }
+        __syncthreads();
+      }
+    if (lane == 0) {
+      int comp[MAXN];
+      for (int k = 0; k < N; ++k) comp[k] = k;
+      for (int e = 0; e < M; ++e) in_tree_flag[e] = 0;
+      int used = 0;
+      for (int t = 0; t < M && used < n1; ++t) {
+        int e = val[t];
+        int2 ed = a.edges[e];
+        int rx = ed.x, ry = ed.y;
+        while (comp[rx] != rx) rx = comp[rx];
+        while (comp[ry] != ry) ry = comp[ry];
+        if (rx == ry) continue;
+        comp[ry] = rx;
+        in_tree_flag[e] = 1;
+        tree_e[used++] = e;
+      }
+      if (used < n1) {   // disconnected equation graph
+        if (axis == 0) a.info->status_y = -1; else a.info->status_x = -1;
+      }
+    }
+    __syncthreads();
+  }
+  // ---- network-simplex descent on spanning trees
+  int piv = 0;
+  bool optimal = false;
+  for (; piv <= MAX_PIVOTS; ++piv) {
+    if (lane == 0) {
+      // parents by BFS from node 0; exact positions by summing measured shifts
+      for (int k = 0; k < N; ++k) { parent[k] = -2; sub[k] = 0ull; }
+      parent[0] = -1;
+      pv[0] = 0.0;
+      int head = 0, tail = 0;
+      order[tail++] = 0;
+      while (head < tail) {
+        int u = order[head++];
+        for (int t = 0; t < n1; ++t) {
+          int e = tree_e[t];
+          int2 ed = a.edges[e];
+          int v = -1;
+          double pvv = 0.0;
+          if (ed.x == u && parent[ed.y] == -2) { v = ed.y; pvv = pv[u] + b[e]; }
+          else if (ed.y == u && parent[ed.x] == -2) { v = ed.x; pvv = pv[u] - b[e]; }
+          if (v >= 0) { parent[v] = u; pedge[v] = e; pv[v] = pvv; order[tail++] = v; }
+        }
+      }
+      // subtree masks in reverse BFS order
+      for (int i = tail - 1; i >= 0; --i) {
+        int v = order[i];
+        sub[v] |= (1ull << v);
+        if (parent[v] >= 0) sub[parent[v]] |= sub[v];
+      }
+    }
+    __syncthreads();
+    for (int e = lane; e < M; e += 64) {
+      int2 ed = a.edges[e];
+      r[e] = pv[ed.y] - pv[ed.x] - b[e];
+      if (in_tree_flag[e]) r[e] = 0.0;
+    }
+    __syncthreads();
+    // directional slopes for moving each subtree by +-delta
+    for (int v = 1; v < N; ++v) {
+      double sp = 0.0, sm = 0.0;
+      const unsigned long long S = sub[v];
+      for (int e = lane; e < M; e += 64) {
+        if (in_tree_flag[e]) continue;
+        int2 ed = a.edges[e];
+        bool iin = (S >> ed.x) & 1ull, jin = (S >> ed.y) & 1ull;
+        if (iin == jin) continue;
+        double dir = jin ? 1.0 : -1.0;
+        double re = r[e];
+        if (re == 0.0) { sp += w[e]; sm += w[e]; }
+        else {
+          double sg = re > 0 ? 1.0 : -1.0;
+          sp += w[e] * dir * sg;
+          sm -= w[e] * dir * sg;
+        }
+      }
+      for (int o = 32; o > 0; o >>= 1) { sp += __shfl_xor(sp, o, 64); sm += __shfl_xor(sm, o, 64); }
+      if (lane == 0) {
+        double wt = w[pedge[v]];
+        slope_p[v] = sp + wt;
+        slope_m[v] = sm + wt;
+      }
+    }
+    __syncthreads();
+    // most negative slope
+    double wsum = 0.0;
+    for (int e = lane; e < M; e += 64) wsum += w[e];
+    for (int o = 32; o > 0; o >>= 1) wsum += __shfl_xor(wsum, o, 64);
+    double best = -1e-12 * (1.0 + wsum);
+    int bv = -1, bs = 0;
+    for (int v = 1; v < N; ++v) {
+      if (slope_p[v] < best) { best = slope_p[v]; bv = v; bs = 1; }
+      if (slope_m[v] < best) { best = slope_m[v]; bv = v; bs = -1; }
+    }
+    if (bv < 0) { optimal = true; break; }
+    if (piv == MAX_PIVOTS) break;
+    // line search: breakpoints of crossing non-tree edges
+    const unsigned long long S = sub[bv];
+    double slope = best;
+    double last = 0.0;
+    int last_e = -1;
+    int enter = -1;
+    while (slope < 0.0) {
+      double md = INFINITY;
+      int me = 0x7fffffff;
+      for (int e = lane; e < M; e += 64) {
+        if (in_tree_flag[e]) continue;
+        int2 ed = a.edges[e];
+        bool iin = (S >> ed.x) & 1ull, jin = (S >> ed.y) & 1ull;
+        if (iin == jin) continue;
+        double dir = (jin ? 1.0 : -1.0) * bs;
+        double re = r[e];
+        if (re == 0.0 || (re > 0) == (dir > 0)) continue;   // residual grows: no breakpoint
+        double d = -re / dir;
+        // strictly after the last breakpoint (ties ordered by edge index)
+        if (d < last || (d == last && e <= last_e)) continue;
+        if (d < md || (d == md && e < me)) { md = d; me = e; }
+      }
+      for (int o = 32; o > 0; o >>= 1) {
+        double od = __shfl_xor(md, o, 64);
+        int oe = __shfl_xor(me, o, 64);
+        if (od < md || (od == md && oe < me)) { md = od; me = oe; }
+      }
+      if (me == 0x7fffffff) break;
+      slope += 2.0 * w[me];
+      last = md;
+      last_e = me;
+      enter = me;
+    }
+    if (enter < 0) break;   // cannot improve (numerically flat)
+    __syncthreads();
+    if (lane == 0) {
+      int leave = pedge[bv];
+      for (int t = 0; t < n1; ++t)
+        if (tree_e[t] == leave) { tree_e[t] = enter; break; }
+      in_tree_flag[leave] = 0;
+      in_tree_flag[enter] = 1;
+    }
+    __syncthreads();
+  }
+  __syncthreads();
+  if (lane == 0) {
+    if (axis == 0) { a.info->pivots_y = piv; if (a.info->status_y == 0) a.info->status_y = optimal ? 1 : 2; }
+    else { a.info->pivots_x = piv; if (a.info->status_x == 0) a.info->status_x = optimal ? 1 : 2; }
+  }
+  for (int k = lane; k < N; k += 64) a.pos[axis * N + k] = pv[k];
+  for (int e = lane; e < M; e += 64) {
+    int2 ed = a.edges[e];
+    a.resid[e * 2 + axis] = pv[ed.y] - pv[ed.x] - b[e];
+  }
+}
+
+// ------------------------------------------------------------------ S3
+ZR_DEV double sinc_d(double x) {
+  double y = M_PI * (x == 0.0 ? 1.0e-20 : x);
+  return sin(y) / y;
+}
+
+struct PostArgs {
+  const double* pos;      // [2][N]
+  const double* weights;  // [M]
+  const double* resid;    // [M][2]
+  const int2* edges;
+  int N, M;
+  int centroid, lanczos, subpix;
+  int H, W, box;
+  double overlap;
+  double* trans_raw;      // [N][2]
+  double* trans;          // [N][2] (centred)
+  double* rel;            // [N-1][2]
+  double* err_stats;      // [2]
+  double* err_frames;     // [N][2]  (errorY, errorX)
+  FrameShift* fs;
+  SolveInfo* info;
+};
+
+__global__ __launch_bounds__(64) void k_solution_post(PostArgs a) {
+  const int lane = threadIdx.x;
+  const int N = a.N, M = a.M;
+  __shared__ double cen[2];
+  if (lane == 0) {
+    double s0 = 0.0, s1 = 0.0;
+    for (int k = 0; k < N; ++k) {
+      double ty = -a.pos[k], tx = -a.pos[N + k];
+      a.trans_raw[2 * k] = ty;
+      a.trans_raw[2 * k + 1] = tx;
+      s0 += ty;
+      s1 += tx;
+      if (k + 1 < N) {
+        a.rel[2 * k] = a.pos[k + 1] - a.pos[k];
+        a.rel[2 * k + 1] = a.pos[N + k + 1] - a.pos[N + k];
+      }
+    }
+    cen[0] = a.centroid ? s0 / N : 0.0;
+    cen[1] = a.centroid ? s1 / N : 0.0;
+    // error statistics (weights clipped to >= 1e-6)
+    double sw = 0, swe = 0, swe2 = 0;
+    for (int e = 0; e < M; ++e) {
+      double w = fmax(a.weights[e], 1e-6);
+      double ey = a.resid[2 * e], ex = a.resid[2 * e + 1];
+      double en = sqrt(ey * ey + ex * ex);
+      sw += w; swe += w * en; swe2 += w * en * en;
+    }
+    double mean = swe / sw;
+    a.err_stats[0] = mean;
+    a.err_stats[1] = sqrt(swe2 / sw - mean * mean);
+  }
+  __syncthreads();
+  for (int k = lane; k < N; k += 64) {
+    double ty = a.trans_raw[2 * k] - cen[0], tx = a.trans_raw[2 * k + 1] - cen[1];
+    a.trans[2 * k] = ty;
+    a.trans[2 * k + 1] = tx;
+    FrameShift f;
+    f.iy = (int)rint(ty);
+    f.ix = (int)rint(tx);
+    f.pad = 0;
+    double ry = ty - floor(ty), rx = tx - floor(tx);
+    f.conv = (a.subpix > 1 && a.lanczos && !(fabs(ry) <= 1e-8 && fabs(rx) <= 1e-8)) ? 1 : 0;
+    if (f.conv) {
+      double xr[3], yr[3], kk[9], s = 0.0;
+      for (int i = 0; i < 3; ++i) {
+        xr[i] = (2.0 / 3.0) * ((-1.5 + i) + 1.0 - rx);
+        yr[i] = (2.0 / 3.0) * ((-1.5 + i) + 1.0 - ry);
+      }
+      for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+          kk[i * 3 + j] = sinc_d(xr[j] * 2.0) * sinc_d(xr[j]) * sinc_d(yr[i] * 2.0) * sinc_d(yr[i]);
+          s += kk[i * 3 + j];
+        }
+      for (int i = 0; i < 9; ++i) f.k[i] = (float)(kk[i] / s);
+    } else {
+      for (int i = 0; i < 9; ++i) f.k[i] = 0.f;
+    }
+    a.fs[k] = f;
+    // per-frame error estimates with the reference's (J, J-1) indexing
+    // padded[i][j] holds the equation (i, j) for frame pair (i, j), i < j
+    double nx = 0, ny = 0, dn = 0;
+    for (int e = 0; e < M; ++e) {
+      int2 ed = a.edges[e];
+      double w = fmax(a.weights[e], 1e-6);
+      bool row = (ed.x == k);                           // padded[k][:]
+      bool col = (ed.y == ((k - 1 + N) % N));           // padded[:][k-1] (k=0 -> last column)
+      if (row || col) {
+        double c = (row ? 1.0 : 0.0) + (col ? 1.0 : 0.0);
+        nx += c * w * fabs(a.resid[2 * e + 1]);
+        ny += c * w * fabs(a.resid[2 * e]);
+        dn += c * w;
+      }
+    }
+    a.err_frames[2 * k] = ny / dn;
+    a.err_frames[2 * k + 1] = nx / dn;
+  }
+  __syncthreads();
+  if (lane == 0) {
+    // FRC crop limits (zorro/zorro.py:2264-2274) and tile counts
+    double mny = INFINITY, mxy = -INFINITY, mnx = INFINITY, mxx = -INFINITY;
+    bool any = false;
+    for (int k = 0; k < N; ++k) {
+      double ty = a.trans[2 * k], tx = a.trans[2 * k + 1];
+      mny = fmin(mny, ty); mxy = fmax(mxy, ty); mnx = fmin(mnx, tx); mxx = fmax(mxx, tx);
+      if (ty != 0.0 || tx != 0.0) any = true;
+    }
+    int c0, c1, c2, c3;
+    if (!any) { c0 = c1 = c2 = c3 = 5; }
+    else {
+      c2 = (int)(-fmin(floor(mny), 5.0));
+      c3 = (int)(-fmin(floor(mnx), 5.0));
+      c0 = (int)fmax(ceil(mxy), 5.0);
+      c1 = (int)fmax(ceil(mxx), 5.0);
+    }
+    a.info->crop[0] = c0; a.info->crop[1] = c1; a.info->crop[2] = c2; a.info->crop[3] = c3;
+    double ov = a.overlap > 0.8 ? 0.8 : a.overlap;
+    a.info->tiles_x = (int)floor((double)(a.W - c1 - c3 - a.box) / a.box / (1.0 - ov));
+    a.info->tiles_y = (int)floor((double)(a.H - c0 - c2 - a.box) / a.box / (1.0 - ov));
+    if (a.info->tiles_x < 0) a.info->tiles_x = 0;
+    if (a.info->tiles_y < 0) a.info->tiles_y = 0;
+  }
+}
+
+// ------------------------------------------------------------------ K3
+ZR_DEV int wrapi(int i, int n) {
+  i %= n;
+  return i < 0 ? i + n : i;
+}
+ZR_DEV int refl1(int i, int n) { return i < 0 ? -1 - i : (i >= n ? 2 * n - 1 - i : i); }
+// numpy 'symmetric' pad index for i in [0, 2n) (and beyond, periodically)
+ZR_DEV int sym_idx(int i, int n) {
+  int p = 2 * n;
+  i %= p;
+  if (i < 0) i += p;
+  return i < n ? i : p - 1 - i;
+}
+
+// aligned frame value at (y, x): roll by (iy, ix) then 3x3 reflect convolution
+// out[y,x] = sum_ab K[a][b] g[refl(y+1-a), refl(x+1-b)],  g[i,j] = f[(i-iy) mod H, (j-ix) mod W]
+ZR_DEV float aligned_px(const float* __restrict__ f, const FrameShift& s, int y, int x, int H, int W) {
+  if (!s.conv) return f[(size_t)wrapi(y - s.iy, H) * W + wrapi(x - s.ix, W)];
+  float acc = 0.f;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const int yy = wrapi(refl1(y + 1 - a, H) - s.iy, H);
+    const float* row = f + (size_t)yy * W;
+#pragma unroll
+    for (int b = 0; b < 3; ++b) {
+      const int xx = wrapi(refl1(x + 1 - b, W) - s.ix, W);
+      acc += s.k[a * 3 + b] * row[xx];
+    }
+  }
+  return acc;
+}
+
+__global__ __launch_bounds__(256) void k_shift_sum(const float* __restrict__ frames, const FrameShift* __restrict__ fs,
+                                                  int N, int H, int W, float* __restrict__ img_sum,
+                                                  float* __restrict__ even_sum, float* __restrict__ odd_sum,
+                                                  float* __restrict__ aligned, double* __restrict__ rowsum) {
+  const int x = blockIdx.x * 64 + (threadIdx.x & 63);
+  const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
+  __shared__ double red[4];
+  double part = 0.0;
+  if (y < H && x < W) {
+    float se = 0.f, so = 0.f, st = 0.f;
+    for (int J = 0; J < N; ++J) {
+      const FrameShift s = fs[J];
+      float v = aligned_px(frames + (size_t)J * H * W, s, y, x, H, W);
+      st += v;
+      if (J & 1) so += v; else se += v;
+      if (aligned) aligned[((size_t)J * H + y) * W + x] = v;
+    }
+    img_sum[(size_t)y * W + x] = st;
+    even_sum[(size_t)y * W + x] = se;
+    odd_sum[(size_t)y * W + x] = so;
+    part = st;
+  }
+  part = wave_sum_d(part);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = part;
+  __syncthreads();
+  if (threadIdx.x == 0) rowsum[blockIdx.y * gridDim.x + blockIdx.x] = red[0] + red[1] + red[2] + red[3];
+}
+
+// ------------------------------------------------------------------ K4
+// Row pass over FRC tiles: Hann-windowed tile rows -> real FFT (box) -> kx-major.
+template <int BOX>
+__global__ __launch_bounds__(256) void k_frc_rows(const float* __restrict__ even, const float* __restrict__ odd,
+                                                 const SolveInfo* __restrict__ info, int W, int tmax_x, double step,
+                                                 const float2* __restrict__ twL, const float2* __restrict__ tw2L,
+                                                 float2* __restrict__ T) {
+  constexpr int L = BOX / 2, RB = 4, NT = 256;
+  __shared__ float2 buf[RB][L];
+  const int tid = threadIdx.x;
+  const int nrb = BOX / RB;
+  int blk = blockIdx.x;
+  const int rb = blk % nrb;
+  blk /= nrb;
+  const int par = blk % 2;
+  const int tile = blk / 2;
+  const int ty = tile / tmax_x, tx = tile % tmax_x;
+  if (ty >= info->tiles_y || tx >= info->tiles_x) return;
+  const int oy = (int)(ty * BOX * step + info->crop[0]);
+  const int ox = (int)(tx * BOX * step + info->crop[1]);
+  const float* img = par ? odd : even;
+  for (int r = 0; r < RB; ++r) {
+    const int yl = rb * RB + r;
+    const float* row = img + (size_t)(oy + yl) * W + ox;
+    const double yv = (double)(yl - BOX / 2) / (BOX / 2);
+    for (int n = tid; n < L; n += NT) {
+      float v2[2];
+      for (int q = 0; q < 2; ++q) {
+        int xl = 2 * n + q;
+        double xv = (double)(xl - BOX / 2) / (BOX / 2);
+        double rr = xv * xv + yv * yv;
+        float hw = rr <= 1.0 ? (float)(0.5 * (1.0 + cos(M_PI * sqrt(rr)))) : 0.f;
+        v2[q] = hw * row[xl];
+      }
+      buf[r][n] = make_float2(v2[0], v2[1]);
+    }
+  }
+  __syncthreads();
+  for (int r = 0; r < RB; ++r) wg_fft<L, -1, NT>(buf[r], twL, tid);
+  const int Wh = L + 1;
+  for (int k = tid; k < Wh; k += NT) {
+    float2 w = tw2L[k];
+    for (int r = 0; r < RB; ++r) {
+      float2 zk = buf[r][k % L];
+      float2 zn = c_conj(buf[r][(L - k) % L]);
+      float2 e = c_scale(c_add(zk, zn), 0.5f);
+      float2 o = c_scale(c_sub(zk, zn), 0.5f);
+      float2 Ok = make_float2(o.y, -o.x);
+      T[(((size_t)tile * 2 + par) * Wh + k) * BOX + rb * RB + r] = c_add(e, c_mul(w, Ok));
+    }
+  }
+}
+
+// Column pass: per kx, FFT the even and odd columns of every tile and accumulate
+// Re(E conj(O)) / sqrt(|O|^2 |E|^2) in FP64.
+template <int BOX>
+__global__ __launch_bounds__(256) void k_frc_cols(const float2* __restrict__ T, const SolveInfo* __restrict__ info,
+                                                 int tmax_x, int ntiles_max, const float2* __restrict__ twB,
+                                                 double* __restrict__ frc2d) {
+  constexpr int NT = 256, PER = BOX / NT;
+  __shared__ float2 be[BOX], bo[BOX];
+  const int tid = threadIdx.x;
+  const int k = blockIdx.x;
+  const int Wh = BOX / 2 + 1;
+  double acc[PER];
+  for (int i = 0; i < PER; ++i) acc[i] = 0.0;
+  for (int tile = 0; tile < ntiles_max; ++tile) {
+    const int ty = tile / tmax_x, tx = tile % tmax_x;
+    if (ty >= info->tiles_y || tx >= info->tiles_x) continue;
+    const float2* ce = T + (((size_t)tile * 2 + 0) * Wh + k) * BOX;
+    const float2* co = T + (((size_t)tile * 2 + 1) * Wh + k) * BOX;
+    for (int n = tid; n < BOX; n += NT) { be[n] = ce[n]; bo[n] = co[n]; }
+    __syncthreads();
+    wg_fft<BOX, -1, NT>(be, twB, tid);
+    wg_fft<BOX, -1, NT>(bo, twB, tid);
+    for (int i = 0; i < PER; ++i) {
+      int n = tid + i * NT;
+      float2 E = be[n], O = bo[n];
+      float num = E.x * O.x + E.y * O.y;
+      float den = sqrtf((O.x * O.x + O.y * O.y) * (E.x * E.x + E.y * E.y));
+      acc[i] += (double)(num / den);
+    }
+    __syncthreads();
+  }
+  for (int i = 0; i < PER; ++i) frc2d[(size_t)(tid + i * NT) * Wh + k] = acc[i];
+}
+
+// Ring means of the normalised, fftshifted 2D FRC (bilinear radial weights).
+// One block per output ring k < box/2; reads the half-plane via Hermitian symmetry.
+__global__ __launch_bounds__(256) void k_frc_rings(const double* __restrict__ frc2d, int box, double* __restrict__ frc) {
+  const int k = blockIdx.x;
+  const int half = box / 2, Wh = half + 1;
+  const double norm = frc2d[0];
+  double num = 0.0, den = 0.0;
+  const int lo = -(k + 1), hi = k + 1;
+  const int span = hi - lo + 1;
+  for (int t = threadIdx.x; t < span * span; t += 256) {
+    int yy = lo + t / span, xx = lo + t % span;
+    if (yy < -half || yy >= half || xx < -half || xx >= half) continue;
+    double r = sqrt((double)yy * yy + (double)xx * xx);
+    double rf = floor(r);
+    double rem = r - rf;
+    int ri = (int)rf;
+    double wgt = 0.0;
+    if (ri == k) wgt += 1.0 - rem;
+    if (ri + 1 == k) wgt += rem;
+    if (wgt == 0.0) continue;
+    // fftshifted index (yy+half, xx+half) <-> FFT frequency (yy, xx)
+    int fy = (yy + box) % box, fx = (xx + box) % box;
+    double v;
+    if (fx <= half) v = frc2d[(size_t)fy * Wh + fx];
+    else v = frc2d[(size_t)((box - fy) % box) * Wh + (box - fx)];
+    num += wgt * (v / norm);
+    den += wgt;
+  }
+  __shared__ double sn[4], sd[4];
+  num = wave_sum_d(num);
+  den = wave_sum_d(den);
+  if ((threadIdx.x & 63) == 0) { sn[threadIdx.x >> 6] = num; sd[threadIdx.x >> 6] = den; }
+  __syncthreads();
+  if (threadIdx.x == 0) frc[k] = (sn[0] + sn[1] + sn[2] + sn[3]) / (sd[0] + sd[1] + sd[2] + sd[3]);
+}
+
+// ------------------------------------------------------------------ KD
+// Row pass: aligned (and symmetric-padded) rows of every frame -> real FFT (Wp)
+// -> T[J][kx][Y]; the image, even and odd sums of the unpadded rows are formed
+// on the way (one block = 4 rows, 1024 threads = 4 groups of 256).
+template <int WP>
+__global__ __launch_bounds__(1024) void k_dose_rows(const float* __restrict__ frames, const FrameShift* __restrict__ fs,
+                                                   int N, int H, int W, int Hp, const float2* __restrict__ twL,
+                                                   const float2* __restrict__ tw2L, float2* __restrict__ T,
+                                                   float* __restrict__ img_sum, float* __restrict__ even_sum,
+                                                   float* __restrict__ odd_sum, float* __restrict__ aligned,
+                                                   double* __restrict__ rowsum) {
+  constexpr int L = WP / 2, RB = 4, NTG = 256, PX = (WP + NTG - 1) / NTG;
+  extern __shared__ float2 kd_lds[];
+  float2 (*buf)[L] = reinterpret_cast<float2 (*)[L]>(kd_lds);
+  __shared__ double red[16];
+  const int grp = threadIdx.x >> 8, gtid = threadIdx.x & 255;
+  const int blk = xcd_remap(blockIdx.x, gridDim.x);
+  const int Y = blk * RB + grp;                 // padded row
+  const int ysrc = sym_idx(Y, H);               // aligned row it copies
+  const bool own = (Y < H);
+  const int Wh = L + 1;
+  float st[PX], se[PX], so[PX];
+  for (int i = 0; i < PX; ++i) { st[i] = 0.f; se[i] = 0.f; so[i] = 0.f; }
+  for (int J = 0; J < N; ++J) {
+    const FrameShift s = fs[J];
+    const float* f = frames + (size_t)J * H * W;
+    float* br = reinterpret_cast<float*>(buf[grp]);
+    for (int i = 0; i < PX; ++i) {
+      int X = gtid + i * NTG;
+      if (X < WP) {
+        const float v = aligned_px(f, s, ysrc, sym_idx(X, W), H, W);
+        br[X] = v;
+        if (own && X < W) {
+          st[i] += v;
+          if (J & 1) so[i] += v; else se[i] += v;
+          if (aligned) aligned[((size_t)J * H + Y) * W + X] = v;
+        }
+      }
+    }
+    __syncthreads();
+    wg_fft<L, -1, NTG>(buf[grp], twL, gtid);
+    for (int k = gtid; k < Wh; k += NTG) {
+      float2 w = tw2L[k];
+      float2 zk = buf[grp][k % L];
+      float2 zn = c_conj(buf[grp][(L - k) % L]);
+      float2 e = c_scale(c_add(zk, zn), 0.5f);
+      float2 o = c_scale(c_sub(zk, zn), 0.5f);
+      float2 Ok = make_float2(o.y, -o.x);
+      T[((size_t)J * Wh + k) * Hp + Y] = c_add(e, c_mul(w, Ok));
+    }
+    __syncthreads();
+  }
+  double part = 0.0;
+  if (own) {
+    for (int i = 0; i < PX; ++i) {
+      int X = gtid + i * NTG;
+      if (X < W) {
+        img_sum[(size_t)Y * W + X] = st[i];
+        even_sum[(size_t)Y * W + X] = se[i];
+        odd_sum[(size_t)Y * W + X] = so[i];
+        part += st[i];
+      }
+    }
+  }
+  part = wave_sum_d(part);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = part;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (int i = 0; i < 16; ++i) t += red[i];
+    rowsum[blk] = t;
+  }
+}
+
+struct DoseArgs {
+  int N, H, W, Hp, Wp;
+  double A, B, C, order, missing, vs, ipx, ipy;
+  const double* rowsum;
+  int nrows;
+  double* accum;    // [N+1]
+  double* thresq;   // [N]
+  SolveInfo* info;
+};
+
+// dose per frame from mean(imageSum)/N, accumulated dose, per-frame cutoff radius
+__global__ __launch_bounds__(64) void k_dose_params(DoseArgs a) {
+  if (threadIdx.x != 0) return;
+  double s = 0.0;
+  for (int i = 0; i < a.nrows; ++i) s += a.rowsum[i];
+  const double dpf = s / ((double)a.H * a.W) / a.N;
+  const double missing = dpf * a.missing;
+  a.info->dpf = dpf;
+  a.accum[0] = missing;
+  double run = 0.0;
+  for (int j = 1; j <= a.N; ++j) { run += dpf; a.accum[j] = run + missing; }
+  const int nq = a.Hp / 2;
+  const double qs = sqrt(a.ipx * a.ipy);
+  for (int j = 0; j < a.N; ++j) {
+    const double d1 = a.accum[j + 1], d0 = a.accum[j];
+    int last = -1;
+    const double o1 = 2.51284 * a.vs * (a.A * pow(1.0 * qs, a.B) + a.C);
+    for (int k = 0; k < nq; ++k) {
+      double opt = (k == 0) ? o1 * o1 : 2.51284 * a.vs * (a.A * pow(k * qs, a.B) + a.C);
+      if (fabs(d1 - opt) < fabs(d0 - opt)) last = k;
+    }
+    a.thresq[j] = last >= 0 ? last * qs : NAN;
+  }
+}
+
+// Column pass: per kx, accumulate FFT_y(T_J[kx]) * exp(-D_J/(2 Nc(q))) * exp(-(q/qc_J)^order)
+// over all frames, then the inverse transform along y.
+template <int HP>
+__global__ __launch_bounds__(256) void k_dose_cols(const float2* __restrict__ T, DoseArgs a,
+                                                  const float2* __restrict__ twH, float2* __restrict__ G) {
+  constexpr int NT = 256, PER = HP / NT;
+  __shared__ float2 buf[HP];
+  const int tid = threadIdx.x;
+  const int kx = xcd_remap(blockIdx.x, gridDim.x);
+  const int Wh = a.Wp / 2 + 1;
+  float invc[PER], q[PER];
+  float2 acc[PER];
+  for (int i = 0; i < PER; ++i) {
+    const int ky = tid + i * NT;
+    const double fy = ky < HP / 2 ? ky : ky - HP;
+    const double fx = kx;     // kx = Wp/2 is the Nyquist column (|f| identical)
+    auto crit = [&](double yy, double xx) {
+      double qq = sqrt(xx * xx * a.ipx * a.ipx + yy * yy * a.ipy * a.ipy);
+      return a.vs * (a.A * pow(qq, a.B) + a.C);
+    };
+    double qq = sqrt(fx * fx * a.ipx * a.ipx + fy * fy * a.ipy * a.ipy);
+    double c = crit(fy, fx);
+    if (ky == HP / 2 && 2 * kx == a.Wp) {   // Nyquist corner takes its left neighbour squared
+      double cl = crit(-HP / 2, a.Wp / 2 - 1);
+      c = cl * cl;
+    }
+    invc[i] = (float)(1.0 / c);   // q = 0 -> c = inf -> 0
+    q[i] = (float)qq;
+    acc[i] = make_float2(0.f, 0.f);
+  }
+  for (int J = 0; J < a.N; ++J) {
+    const float2* src = T + ((size_t)J * Wh + kx) * HP;
+    for (int i = 0; i < PER; ++i) buf[tid + i * NT] = src[tid + i * NT];
+    __syncthreads();
+    wg_fft<HP, -1, NT>(buf, twH, tid);
+    const float mhd = (float)(-0.5 * a.accum[J + 1]);
+    const float tq = (float)a.thresq[J];
+    const float ord = (float)a.order;
+    for (int i = 0; i < PER; ++i) {
+      float filt = expf(mhd * invc[i]);
+      float thr = expf(-powf(q[i] / tq, ord));
+      float2 v = buf[tid + i * NT];
+      float g = thr * filt;
+      acc[i].x += v.x * g;
+      acc[i].y += v.y * g;
+    }
+    __syncthreads();
+  }
+  for (int i = 0; i < PER; ++i) buf[tid + i * NT] = acc[i];
+  __syncthreads();
+  wg_fft<HP, 1, NT>(buf, twH, tid);
+  for (int y = tid; y < a.H; y += NT) G[(size_t)kx * a.H + y] = buf[y];
+}
+
+// Final rows: c2r of the accumulated half spectrum (raw), modulus, crop to W.
+template <int WP>
+__global__ __launch_bounds__(256) void k_dose_out(const float2* __restrict__ G, int H, int W,
+                                                 const float2* __restrict__ twL, const float2* __restrict__ tw2L,
+                                                 float* __restrict__ filt_sum) {
+  constexpr int L = WP / 2, NT = 256;
+  __shared__ float2 buf[L];
+  const int y = xcd_remap(blockIdx.x, gridDim.x);
+  const int tid = threadIdx.x;
+  for (int k = tid; k < L; k += NT) {
+    float2 xk = G[(size_t)k * H + y];
+    float2 xn = c_conj(G[(size_t)(L - k) * H + y]);
+    float2 e = c_scale(c_add(xk, xn), 0.5f);
+    float2 d = c_scale(c_sub(xk, xn), 0.5f);
+    float2 w = tw2L[k];
+    float2 o = c_mul(d, make_float2(w.x, -w.y));    // * exp(+2 pi i k / Wp)
+    buf[k] = make_float2(e.x - o.y, e.y + o.x);      // E + i O
+  }
+  __syncthreads();
+  wg_fft<L, 1, NT>(buf, twL, tid);
+  for (int n = tid; n < L; n += NT) {
+    float2 z = buf[n];
+    int x0 = 2 * n, x1 = 2 * n + 1;
+    if (x0 < W) filt_sum[(size_t)y * W + x0] = fabsf(2.f * z.x);
+    if (x1 < W) filt_sum[(size_t)y * W + x1] = fabsf(2.f * z.y);
+  }
+}
+
+// ==================================================================== host
+namespace {
+thread_local std::string s_err;
+int serr(int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  s_err = buf;
+  return code;
+}
+#define ZS_HIP(call)                                                                                     \
+  do {                                                                                                   \
+    hipError_t e_ = (call);                                                                              \
+    if (e_ != hipSuccess) return serr(ZR_ERR_HIP, "%s: %s (%s:%d)", #call, hipGetErrorString(e_), __FILE__, __LINE__); \
+  } while (0)
+
+std::vector<float2> fwd_table(int n) {
+  std::vector<float2> t(n);
+  for (int i = 0; i < n; ++i) {
+    double a = -2.0 * M_PI * (double)i / (double)n;
+    t[i] = make_float2((float)cos(a), (float)sin(a));
+  }
+  return t;
+}
+template <typename T>
+hipError_t upload(T** p, const std::vector<T>& v) {
+  hipError_t e = hipMalloc((void**)p, v.size() * sizeof(T));
+  if (e != hipSuccess) return e;
+  return hipMemcpy(*p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice);
+}
+}  // namespace
+
+struct zr_stage_state {
+  zr_params p{};
+  int N = 0, H = 0, W = 0, Hp = 0, Wp = 0, M = 0;
+  int2* edges = nullptr;
+  float *hs = nullptr, *cdf = nullptr;
+  double* obj = nullptr;
+  SolveInfo* info = nullptr;
+  double *weights = nullptr, *resid = nullptr, *pos = nullptr;
+  double *trans_raw = nullptr, *trans = nullptr, *rel = nullptr, *err_stats = nullptr, *err_frames = nullptr;
+  FrameShift* fs = nullptr;
+  float *img_sum = nullptr, *even = nullptr, *odd = nullptr, *filt = nullptr;
+  double* rowsum = nullptr;
+  bool have_solution = false, have_sums = false, have_frc = false, have_dose = false;
+  // FRC
+  int box = 512, tmax_y = 0, tmax_x = 0;
+  float2 *twB = nullptr, *twB2 = nullptr, *Tfrc = nullptr;
+  double *frc2d = nullptr, *frc = nullptr;
+  // dose
+  float2 *twHp = nullptr, *twWp = nullptr, *twWp2 = nullptr, *Tdose = nullptr, *G = nullptr;
+  double *accum = nullptr, *thresq = nullptr, *drow = nullptr;
+};
+
+const char* zr_stage_error() { return s_err.c_str(); }
+
+void zr_stage_free(zr_stage_state* s) {
+  if (!s) return;
+  void* ptrs[] = {s->edges, s->hs, s->cdf, s->obj, s->info, s->weights, s->resid, s->pos, s->trans_raw, s->trans,
+                  s->rel, s->err_stats, s->err_frames, s->fs, s->img_sum, s->even, s->odd, s->filt, s->rowsum,
+                  s->twB, s->twB2, s->Tfrc, s->frc2d, s->frc, s->twHp, s->twWp, s->twWp2, s->Tdose, s->G,
+                  s->accum, s->thresq, s->drow};
+  for (void* q : ptrs)
+    if (q) hipFree(q);
+  delete s;
+}
+
+int zr_stage_create(zr_stage_state** out, const zr_params& p, int N, int H, int W, int Hp, int Wp,
+                    const std::vector<int4>& pairs) {
+  *out = nullptr;
+  if (N > MAXN) return serr(ZR_ERR_UNSUPPORTED, "the solver handles up to %d frames (got %d)", MAXN, N);
+  zr_stage_state* s = new zr_stage_state();
+  s->p = p;
+  s->N = N; s->H = H; s->W = W; s->Hp = Hp; s->Wp = Wp;
+  s->M = (int)pairs.size();
+  if (s->M > 2016) { zr_stage_free(s); return serr(ZR_ERR_UNSUPPORTED, "too many equations"); }
+  std::vector<int2> ed(s->M);
+  for (int i = 0; i < s->M; ++i) ed[i] = make_int2(pairs[i].x, pairs[i].y);
+  // the equations are the pair intervals [I, J) (rot90 construction of the
+  // reference for tri/diag patterns): edge I-J in position space
+#define ZC(x) do { hipError_t e_ = (x); if (e_ != hipSuccess) { zr_stage_free(s); return serr(ZR_ERR_NOMEM, "%s: %s", #x, hipGetErrorString(e_)); } } while (0)
+  ZC(upload(&s->edges, ed));
+  const int maxbins = std::max(1, s->M / 7 + 1);
+  ZC(hipMalloc((void**)&s->hs, sizeof(float) * maxbins));
+  ZC(hipMalloc((void**)&s->cdf, sizeof(float) * maxbins));
+  ZC(hipMalloc((void**)&s->obj, sizeof(double) * LOGI_G0 * LOGI_G1 * LOGI_G2));
+  ZC(hipMalloc((void**)&s->info, sizeof(SolveInfo)));
+  ZC(hipMalloc((void**)&s->weights, sizeof(double) * s->M));
+  ZC(hipMalloc((void**)&s->resid, sizeof(double) * s->M * 2));
+  ZC(hipMalloc((void**)&s->pos, sizeof(double) * N * 2));
+  ZC(hipMalloc((void**)&s->trans_raw, sizeof(double) * N * 2));
+  ZC(hipMalloc((void**)&s->trans, sizeof(double) * N * 2));
+  ZC(hipMalloc((void**)&s->rel, sizeof(double) * N * 2));
+  ZC(hipMalloc((void**)&s->err_stats, sizeof(double) * 2));
+  ZC(hipMalloc((void**)&s->err_frames, sizeof(double) * N * 2));
+  ZC(hipMalloc((void**)&s->fs, sizeof(FrameShift) * N));
+  ZC(hipMalloc((void**)&s->img_sum, sizeof(float) * H * W));
+  ZC(hipMalloc((void**)&s->even, sizeof(float) * H * W));
+  ZC(hipMalloc((void**)&s->odd, sizeof(float) * H * W));
+  ZC(hipMalloc((void**)&s->filt, sizeof(float) * H * W));
+  const int nrow_blocks = ((W + 63) / 64) * ((H + 3) / 4);
+  ZC(hipMalloc((void**)&s->rowsum, sizeof(double) * std::max(nrow_blocks, Hp / 4 + 1)));
+  if (p.do_frc) {
+    s->box = p.frc_box;
+    if (s->box != 512 && s->box != 256 && s->box != 1024) { zr_stage_free(s); return serr(ZR_ERR_UNSUPPORTED, "FRC box %d: built sizes 256, 512, 1024", s->box); }
+    double ov = p.frc_overlap > 0.8 ? 0.8 : p.frc_overlap;
+    s->tmax_y = std::max(0, (int)std::floor((double)(H - s->box) / s->box / (1.0 - ov)));
+    s->tmax_x = std::max(0, (int)std::floor((double)(W - s->box) / s->box / (1.0 - ov)));
+    size_t nt = (size_t)std::max(1, s->tmax_y * s->tmax_x);
+    ZC(upload(&s->twB, fwd_table(s->box)));
+    ZC(upload(&s->twB2, fwd_table(s->box / 2)));
+    ZC(hipMalloc((void**)&s->Tfrc, sizeof(float2) * nt * 2 * (s->box / 2 + 1) * s->box));
+    ZC(hipMalloc((void**)&s->frc2d, sizeof(double) * s->box * (s->box / 2 + 1)));
+    ZC(hipMalloc((void**)&s->frc, sizeof(double) * s->box));
+  }
+  if (p.do_dose) {
+    if (!(p.pixelsize_nm > 0)) { zr_stage_free(s); return serr(ZR_ERR_ARG, "dose filter needs pixelsize"); }
+    if (!(p.voltage_kv > 0)) { zr_stage_free(s); return serr(ZR_ERR_ARG, "Accelerating voltage not set in advance for dose filtering"); }
+    ZC(upload(&s->twHp, fwd_table(Hp)));
+    ZC(upload(&s->twWp, fwd_table(Wp)));
+    ZC(upload(&s->twWp2, fwd_table(Wp / 2)));
+    ZC(hipMalloc((void**)&s->Tdose, sizeof(float2) * (size_t)N * (Wp / 2 + 1) * Hp));
+    ZC(hipMalloc((void**)&s->G, sizeof(float2) * (size_t)(Wp / 2 + 1) * H));
+    ZC(hipMalloc((void**)&s->accum, sizeof(double) * (N + 1)));
+    ZC(hipMalloc((void**)&s->thresq, sizeof(double) * N));
+    ZC(hipMalloc((void**)&s->drow, sizeof(double) * (Hp / 4 + 1)));
+  }
+#undef ZC
+  *out = s;
+  return ZR_OK;
+}
+
+int zr_stage_solve(zr_stage_state* s, const float* shifts, const float* corr, const float* peaksig, hipStream_t st) {
+  if (!s) return serr(ZR_ERR_STATE, "no stage state");
+  const int N = s->N, M = s->M;
+  ZS_HIP(hipMemsetAsync(s->info, 0, sizeof(SolveInfo), st));
+  const int wm = s->p.weight_mode;
+  if (wm == ZR_W_AUTOLOGISTIC) {
+    const int maxbins = std::max(1, M / 7 + 1);
+    hipLaunchKernelGGL(k_ps_hist, dim3(1), dim3(256), 0, st, peaksig, N * N, s->hs, s->cdf, s->info, maxbins);
+    ZS_HIP(hipGetLastError());
+    const int total = LOGI_G0 * LOGI_G1 * LOGI_G2;
+    hipLaunchKernelGGL(k_logi_grid, dim3((total + 255) / 256), dim3(256), 0, st, s->hs, s->cdf, s->info, s->obj);
+    ZS_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_logi_refine, dim3(1), dim3(64), 0, st, s->hs, s->cdf, s->obj, s->info);
+    ZS_HIP(hipGetLastError());
+  }
+  SolveArgs a;
+  a.shifts = shifts; a.corr = corr; a.peaksig = peaksig; a.edges = s->edges;
+  a.N = N; a.M = M; a.weight_mode = wm;
+  a.logi_fixed[0] = s->p.peaksig_thres; a.logi_fixed[1] = s->p.logistic_k; a.logi_fixed[2] = s->p.logistic_nu;
+  a.bound = 0;
+  a.weights = s->weights; a.resid = s->resid; a.pos = s->pos; a.info = s->info;
+  size_t dyn = sizeof(double) * 3 * M;
+  static bool attr = false;
+  if (!attr) {
+    hipFuncSetAttribute((const void*)k_l1_solve, hipFuncAttributeMaxDynamicSharedMemorySize, (int)(sizeof(double) * 3 * 2016));
+    attr = true;
+  }
+  hipLaunchKernelGGL(k_l1_solve, dim3(2), dim3(64), dyn, st, a);
+  ZS_HIP(hipGetLastError());
+  PostArgs q;
+  q.pos = s->pos; q.weights = s->weights; q.resid = s->resid; q.edges = s->edges;
+  q.N = N; q.M = M; q.centroid = s->p.origin_centroid; q.lanczos = s->p.shift_method == ZR_SHIFT_LANCZOS;
+  q.subpix = s->p.sub_pix_reg; q.H = s->H; q.W = s->W; q.box = s->box; q.overlap = s->p.frc_overlap;
+  q.trans_raw = s->trans_raw; q.trans = s->trans; q.rel = s->rel; q.err_stats = s->err_stats;
+  q.err_frames = s->err_frames; q.fs = s->fs; q.info = s->info;
+  hipLaunchKernelGGL(k_solution_post, dim3(1), dim3(64), 0, st, q);
+  ZS_HIP(hipGetLastError());
+  s->have_solution = true;
+  return ZR_OK;
+}
+
+int zr_stage_shifts(zr_stage_state* s, const float* frames, float* aligned_out, hipStream_t st) {
+  if (!s || !s->have_solution) return serr(ZR_ERR_STATE, "shifts before solve");
+  dim3 grid((s->W + 63) / 64, (s->H + 3) / 4);
+  hipLaunchKernelGGL(k_shift_sum, grid, dim3(256), 0, st, frames, s->fs, s->N, s->H, s->W, s->img_sum, s->even,
+                     s->odd, aligned_out, s->rowsum);
+  ZS_HIP(hipGetLastError());
+  s->have_sums = true;
+  return ZR_OK;
+}
+
+int zr_stage_frc(zr_stage_state* s, hipStream_t st) {
+  if (!s || !s->have_sums) return serr(ZR_ERR_STATE, "FRC before the half sums");
+  if (!s->p.do_frc) return serr(ZR_ERR_STATE, "FRC disabled in the parameters");
+  const int nt = s->tmax_y * s->tmax_x;
+  if (nt < 1) return serr(ZR_ERR_ARG, "frame smaller than one FRC tile");
+  const double step = 1.0 - (s->p.frc_overlap > 0.8 ? 0.8 : s->p.frc_overlap);
+  const int B = s->box;
+  switch (B) {
+#define ZR_FRC(BX)                                                                                            \
+  case BX:                                                                                                    \
+    hipLaunchKernelGGL(k_frc_rows<BX>, dim3(nt * 2 * (BX / 4)), dim3(256), 0, st, s->even, s->odd, s->info, s->W, \
+                       s->tmax_x, step, s->twB2, s->twB, s->Tfrc);                                            \
+    ZS_HIP(hipGetLastError());                                                                                \
+    hipLaunchKernelGGL(k_frc_cols<BX>, dim3(BX / 2 + 1), dim3(256), 0, st, s->Tfrc, s->info, s->tmax_x, nt, s->twB, \
+                       s->frc2d);                                                                             \
+    ZS_HIP(hipGetLastError());                                                                                \
+    break;
+    ZR_FRC(256) ZR_FRC(512) ZR_FRC(1024)
+#undef ZR_FRC
+    default: return serr(ZR_ERR_UNSUPPORTED, "FRC box");
+  }
+  hipLaunchKernelGGL(k_frc_rings, dim3(B / 2), dim3(256), 0, st, s->frc2d, B, s->frc);
+  ZS_HIP(hipGetLastError());
+  s->have_frc = true;
+  return ZR_OK;
+}
+
+int zr_stage_dose(zr_stage_state* s, const float* frames, hipStream_t st) {
+  if (!s || !s->have_solution) return serr(ZR_ERR_STATE, "dose filter before solve");
+  if (!s->p.do_dose) return serr(ZR_ERR_STATE, "dose filter disabled in the parameters");
+  const int Hp = s->Hp, Wp = s->Wp;
+  const int nblk = Hp / 4;
+  size_t sh = (size_t)4 * (Wp / 2) * sizeof(float2);
+  switch (Wp) {
+#define ZR_DR(X)                                                                                              \
+  case X: {                                                                                                   \
+    static bool attr = false;                                                                                 \
+    if (!attr) { hipFuncSetAttribute((const void*)k_dose_rows<X>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh); attr = true; } \
+    hipLaunchKernelGGL(k_dose_rows<X>, dim3(nblk), dim3(1024), sh, st, frames, s->fs, s->N, s->H, s->W, Hp, s->twWp2, \
+                       s->twWp, s->Tdose, s->img_sum, s->even, s->odd, (float*)nullptr, s->drow);            \
+  } break;
+    ZR_DR(256) ZR_DR(512) ZR_DR(1024) ZR_DR(2048) ZR_DR(3072) ZR_DR(3840) ZR_DR(4096)
+#undef ZR_DR
+    default: return serr(ZR_ERR_UNSUPPORTED, "dose width %d", Wp);
+  }
+  ZS_HIP(hipGetLastError());
+  s->have_sums = true;
+  DoseArgs a;
+  a.N = s->N; a.H = s->H; a.W = s->W; a.Hp = Hp; a.Wp = Wp;
+  a.A = s->p.dose_a; a.B = s->p.dose_b; a.C = s->p.dose_c; a.order = s->p.dose_order; a.missing = s->p.dose_missing;
+  a.vs = std::sqrt(s->p.voltage_kv / 300.0);
+  a.ipx = 1.0 / (Wp * (s->p.pixelsize_nm * 10.0));
+  a.ipy = 1.0 / (Hp * (s->p.pixelsize_nm * 10.0));
+  a.rowsum = s->drow; a.nrows = nblk; a.accum = s->accum; a.thresq = s->thresq; a.info = s->info;
+  hipLaunchKernelGGL(k_dose_params, dim3(1), dim3(64), 0, st, a);
+  ZS_HIP(hipGetLastError());
+  switch (Hp) {
+#define ZR_DC(X) case X: hipLaunchKernelGGL(k_dose_cols<X>, dim3(Wp / 2 + 1), dim3(256), 0, st, s->Tdose, a, s->twHp, s->G); break;
+    ZR_DC(256) ZR_DC(512) ZR_DC(1024) ZR_DC(2048) ZR_DC(3072) ZR_DC(3840) ZR_DC(4096)
+#undef ZR_DC
+    default: return serr(ZR_ERR_UNSUPPORTED, "dose height %d", Hp);
+  }
+  ZS_HIP(hipGetLastError());
+  switch (Wp) {
+#define ZR_DO(X) case X: hipLaunchKernelGGL(k_dose_out<X>, dim3(s->H), dim3(256), 0, st, s->G, s->H, s->W, s->twWp2, s->twWp, s->filt); break;
+    ZR_DO(256) ZR_DO(512) ZR_DO(1024) ZR_DO(2048) ZR_DO(3072) ZR_DO(3840) ZR_DO(4096)
+#undef ZR_DO
+    default: return serr(ZR_ERR_UNSUPPORTED, "dose width %d", Wp);
+  }
+  ZS_HIP(hipGetLastError());
+  s->have_dose = true;
+  return ZR_OK;
+}
+
+int zr_stage_get_solution(zr_stage_state* s, double* translations, double* relative_est, double* weights,
+                          double* logistic, double* error, double* error_stats, double* error_xy_frames) {
+  if (!s || !s->have_solution) return serr(ZR_ERR_STATE, "no solution");
+  SolveInfo inf;
+  ZS_HIP(hipMemcpy(&inf, s->info, sizeof(inf), hipMemcpyDeviceToHost));
+  if (inf.status_x < 0 || inf.status_y < 0) return serr(ZR_ERR_ARG, "equation graph is disconnected");
+  if (s->p.weight_mode == ZR_W_AUTOLOGISTIC && inf.nbins < 1)
+    return serr(ZR_ERR_ARG, "autologistic weights need >= 7 positive peak significances (got %d)", inf.nps);
+  const int N = s->N, M = s->M;
+  if (translations) ZS_HIP(hipMemcpy(translations, s->trans, sizeof(double) * 2 * N, hipMemcpyDeviceToHost));
+  if (relative_est) ZS_HIP(hipMemcpy(relative_est, s->rel, sizeof(double) * 2 * (N - 1), hipMemcpyDeviceToHost));
+  if (weights) ZS_HIP(hipMemcpy(weights, s->weights, sizeof(double) * M, hipMemcpyDeviceToHost));
+  if (logistic) {
+    if (s->p.weight_mode == ZR_W_AUTOLOGISTIC) for (int i = 0; i < 3; ++i) logistic[i] = inf.logi[i];
+    else { logistic[0] = s->p.peaksig_thres; logistic[1] = s->p.logistic_k; logistic[2] = s->p.logistic_nu; }
+  }
+  if (error) ZS_HIP(hipMemcpy(error, s->resid, sizeof(double) * 2 * M, hipMemcpyDeviceToHost));
+  if (error_stats) ZS_HIP(hipMemcpy(error_stats, s->err_stats, sizeof(double) * 2, hipMemcpyDeviceToHost));
+  if (error_xy_frames) ZS_HIP(hipMemcpy(error_xy_frames, s->err_frames, sizeof(double) * 2 * N, hipMemcpyDeviceToHost));
+  return ZR_OK;
+}
+
+int zr_stage_get_sums(zr_stage_state* s, float* image_sum, float* filt_sum, float* even_sum, float* odd_sum) {
+  if (!s || !s->have_sums) return serr(ZR_ERR_STATE, "no sums");
+  size_t b = sizeof(float) * s->H * s->W;
+  if (image_sum) ZS_HIP(hipMemcpy(image_sum, s->img_sum, b, hipMemcpyDeviceToHost));
+  if (even_sum) ZS_HIP(hipMemcpy(even_sum, s->even, b, hipMemcpyDeviceToHost));
+  if (odd_sum) ZS_HIP(hipMemcpy(odd_sum, s->odd, b, hipMemcpyDeviceToHost));
+  if (filt_sum) {
+    if (!s->have_dose) return serr(ZR_ERR_STATE, "no dose-filtered sum");
+    ZS_HIP(hipMemcpy(filt_sum, s->filt, b, hipMemcpyDeviceToHost));
+  }
+  return ZR_OK;
+}
+
+int zr_stage_get_frc(zr_stage_state* s, double* frc, int* n) {
+  if (!s || !s->have_frc) return serr(ZR_ERR_STATE, "no FRC");
+  if (n) *n = s->box / 2;
+  if (frc) ZS_HIP(hipMemcpy(frc, s->frc, sizeof(double) * (s->box / 2), hipMemcpyDeviceToHost));
+  return ZR_OK;
+}
+
+int zr_stage_device_results(zr_stage_state* s, const float** image_sum, const float** filt_sum,
+                            const double** translations) {
+  if (!s) return serr(ZR_ERR_STATE, "no stage state");
+  if (image_sum) *image_sum = s->img_sum;
+  if (filt_sum) *filt_sum = s->filt;
+  if (translations) *translations = s->trans;
+  return ZR_OK;
+}

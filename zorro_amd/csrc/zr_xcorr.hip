@@ -1,0 +1,595 @@
+// Pair cross-correlation path for gfx950:
+//   K1  frame -> cropped half-spectrum (row r2c pass + column c2c pass)
+//   K2  pair  -> 2m x 2m lag window of the raw inverse DFT of conj(F_I) F_J
+//       (pass 1: pruned column transform along ky, pass 2: row transform along kx)
+//   KP  window -> origin suppression, B filter, disc argmax, peak significance,
+//       Fourier-interpolated sub-pixel shift
+//
+// Layouts (all row-major, complex64 as float2):
+//   frames  f32 [N][H][W]
+//   T       c64 [B][Wh][Hp]     row-pass output, kx-major (Wh = Wc/2+1)
+//   S       c64 [N][Wh][Hc]     cropped half-spectrum, kx-major; row ky' in FFT
+//                               order of the cropped grid (ky' >= Hc/2 -> ky'-Hc)
+//   Sside   c64 [N][Wh]         original row +Hc/2 (absent from the cropped grid)
+//   Z       c64 [P][Wh][2m]     pass-1 output
+//   windows f32 [P][2m][2m]     index i <-> lag i-m
+#include "zr_common.h"
+
+// --------------------------------------------------------------------- K1
+// Row pass: real FFT of length WP (as a complex FFT of WP/2), kept columns
+// kx in [0, Wh).  One block = RB consecutive rows of one frame.
+template <int WP, int RB, int NT>
+__global__ __launch_bounds__(NT) void k1_rows(const float* __restrict__ frames, int H, int W, int Hp, int Wh,
+                                              int frame0, const float2* __restrict__ twL,
+                                              const float2* __restrict__ tw2L, float2* __restrict__ T,
+                                              double* __restrict__ part) {
+  constexpr int L = WP / 2;
+  extern __shared__ float2 k1_lds[];
+  float2 (*buf)[L] = reinterpret_cast<float2 (*)[L]>(k1_lds);
+  __shared__ double red[2][NT / 64];
+  const int tid = threadIdx.x;
+  const int nrb = Hp / RB;
+  const int blk = xcd_remap(blockIdx.x, gridDim.x);
+  const int fb = blk / nrb;                 // frame within batch
+  const int y0 = (blk % nrb) * RB;
+  const int f = frame0 + fb;
+  double s1 = 0.0, s2 = 0.0;
+#pragma unroll
+  for (int r = 0; r < RB; ++r) {
+    const int y = y0 + r;
+    const float* row = frames + ((size_t)f * H + y) * W;
+    for (int n = tid; n < L; n += NT) {
+      float a = 0.f, b = 0.f;
+      if (y < H) {
+        int x = 2 * n;
+        if (x < W) a = row[x];
+        if (x + 1 < W) b = row[x + 1];
+      }
+      s1 += (double)a + (double)b;
+      s2 += (double)a * a + (double)b * b;
+      buf[r][n] = make_float2(a, b);
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < RB; ++r) wg_fft<L, -1, NT>(buf[r], twL, tid);
+  // post-process and write transposed: T[fb][k][y0..y0+RB)
+  for (int k = tid; k < Wh; k += NT) {
+    float2 w = tw2L[k];   // exp(-2*pi*i*k/WP)
+    float2 out[RB];
+#pragma unroll
+    for (int r = 0; r < RB; ++r) {
+      float2 zk = buf[r][k % L];
+      float2 zn = c_conj(buf[r][(L - k) % L]);
+      float2 e = c_scale(c_add(zk, zn), 0.5f);
+      float2 o = c_scale(c_sub(zk, zn), 0.5f);           // = i * O
+      // X = E + w * O,  O = -i * o
+      float2 Ok = make_float2(o.y, -o.x);
+      out[r] = c_add(e, c_mul(w, Ok));
+    }
+    float2* dst = T + ((size_t)fb * Wh + k) * Hp + y0;
+#pragma unroll
+    for (int r = 0; r < RB; ++r) dst[r] = out[r];
+  }
+  // frame sums (for the intensity normalisation)
+  s1 = wave_sum_d(s1);
+  s2 = wave_sum_d(s2);
+  if ((tid & 63) == 0) { red[0][tid >> 6] = s1; red[1][tid >> 6] = s2; }
+  __syncthreads();
+  if (tid == 0) {
+    double a = 0, b = 0;
+    for (int i = 0; i < NT / 64; ++i) { a += red[0][i]; b += red[1][i]; }
+    part[((size_t)f * nrb + (blk % nrb)) * 2 + 0] = a;
+    part[((size_t)f * nrb + (blk % nrb)) * 2 + 1] = b;
+  }
+}
+
+// Column pass: complex FFT of length HP along y for each kept kx, then the
+// four-quadrant crop to Hc rows and the side row +Hc/2.
+template <int HP, int NT>
+__global__ __launch_bounds__(NT) void k1_cols(const float2* __restrict__ T, int Wh, int Hc, int frame0,
+                                              const float2* __restrict__ twH, float2* __restrict__ S,
+                                              float2* __restrict__ Sside) {
+  __shared__ float2 buf[HP];
+  const int tid = threadIdx.x;
+  const int blk = xcd_remap(blockIdx.x, gridDim.x);
+  const int fb = blk / Wh, k = blk % Wh;
+  const float2* src = T + ((size_t)fb * Wh + k) * HP;
+  for (int n = tid; n < HP; n += NT) buf[n] = src[n];
+  __syncthreads();
+  wg_fft<HP, -1, NT>(buf, twH, tid);
+  const int f = frame0 + fb;
+  float2* dst = S + ((size_t)f * Wh + k) * Hc;
+  const int h2 = Hc / 2;
+  for (int n = tid; n < Hc; n += NT) dst[n] = buf[n < h2 ? n : n + HP - Hc];
+  if (tid == 0) Sside[(size_t)f * Wh + k] = buf[h2];
+}
+
+__global__ void k1_frame_stats(const double* __restrict__ part, int nrb, int n, double* __restrict__ stats) {
+  // stats[f] = (sum, sumsq); one thread per frame, fixed summation order
+  int f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= n) return;
+  double a = 0, b = 0;
+  for (int i = 0; i < nrb; ++i) { a += part[((size_t)f * nrb + i) * 2]; b += part[((size_t)f * nrb + i) * 2 + 1]; }
+  stats[2 * f] = a;
+  stats[2 * f + 1] = b;
+}
+
+// --------------------------------------------------------------------- K2
+// Pass 1.  One wave = one (pair, column kx).  ky = a + 64*b with lane a:
+//   Y_a[r] = sum_b x[a+64b] e^{+2 pi i b r / L}                  (per lane, registers)
+//   Z[dy]  = sum_a e^{+2 pi i a dy / Hc} Y_a[dy mod L]             (LDS exchange)
+// with x = c(kx) * conj(F_I) F_J (DC removed on the unmasked path) and the
+// Nyquist row combining the original rows -Hc/2 and +Hc/2.
+template <int L, int WPB>
+__global__ __launch_bounds__(64 * WPB) void k2_cols(const float2* __restrict__ S, const float2* __restrict__ Sside,
+                                                     const int4* __restrict__ work, int nwork_pairs, int Wh,
+                                                     int Hc, int Wc, int m, int cols_per_wave, int dc_zero,
+                                                     const float2* __restrict__ twHc, float2* __restrict__ Z) {
+  constexpr int LP = 65;  // padded LDS row (conflict-free 64-bit reads per half wave)
+  extern __shared__ float2 k2_lds[];
+  const int lane = threadIdx.x & 63;
+  const int w = threadIdx.x >> 6;
+  const int blk = xcd_remap(blockIdx.x, gridDim.x);
+  const int kgroup = blk / nwork_pairs;
+  const int4 wk = work[blk % nwork_pairs];          // (I, J, pair index, unused)
+  const int I = wk.x, J = wk.y, p = wk.z;
+  const int twoM = 2 * m;
+  // twiddles for this lane's outputs, computed once: e^{+2 pi i a dy/Hc},
+  // a = 8*a1 + a0, for dy = lane - m and lane + 64 - m
+  const int nrounds = (twoM + 63) / 64;
+  float2 T0[2][8], T1[2][8];
+  int rr[2];
+#pragma unroll
+  for (int q = 0; q < 2; ++q) {
+    int dy = lane + 64 * q - m;
+    int dym = ((dy % Hc) + Hc) % Hc;
+    rr[q] = ((dy % L) + L) % L;
+#pragma unroll
+    for (int a0 = 0; a0 < 8; ++a0) {
+      float2 t = twHc[(long)a0 * dym % Hc];
+      T0[q][a0] = make_float2(t.x, -t.y);
+      float2 u = twHc[(long)(8 * a0) * dym % Hc];
+      T1[q][a0] = make_float2(u.x, -u.y);
+    }
+  }
+  const int kbase = kgroup * WPB * cols_per_wave;
+  for (int c = 0; c < cols_per_wave; ++c) {
+    const int kx = kbase + c * WPB + w;
+    if (kx >= Wh) break;
+    const float2* sI = S + ((size_t)I * Wh + kx) * Hc;
+    const float2* sJ = S + ((size_t)J * Wh + kx) * Hc;
+    const float cw = (kx == 0 || 2 * kx == Wc) ? 1.f : 2.f;
+    float2 v[L];
+#pragma unroll
+    for (int b = 0; b < L; ++b) {
+      float2 xi = sI[lane + 64 * b];
+      float2 xj = sJ[lane + 64 * b];
+      v[b] = c_scale(c_cjmul(xi, xj), cw);
+    }
+    if (lane == 0) {
+      if (dc_zero && kx == 0) v[0] = make_float2(0.f, 0.f);
+      // Nyquist row (ky' = Hc/2 <-> lane 0, b = L/2): rows -Hc/2 (main) and +Hc/2 (side)
+      float2 nm = c_cjmul(sI[Hc / 2], sJ[Hc / 2]);
+      float2 ns = c_cjmul(Sside[(size_t)I * Wh + kx], Sside[(size_t)J * Wh + kx]);
+      float2 nv;
+      if (kx == 0) nv = nm;
+      else if (2 * kx == Wc) nv = ns;
+      else nv = c_add(nm, ns);
+      v[L / 2] = nv;
+    }
+    Dft<L, 1>::run(v);
+    float2* my = k2_lds + (size_t)w * L * LP;
+#pragma unroll
+    for (int r = 0; r < L; ++r) my[r * LP + lane] = v[r];
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    float2* zdst = Z + ((size_t)p * Wh + kx) * twoM;
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+      if (q < nrounds) {
+        int t = lane + 64 * q;
+        const float2* yr = my + rr[q] * LP;
+        float2 acc = make_float2(0.f, 0.f);
+#pragma unroll
+        for (int a1 = 0; a1 < 8; ++a1) {
+          float2 part = make_float2(0.f, 0.f);
+#pragma unroll
+          for (int a0 = 0; a0 < 8; ++a0) {
+            float2 yv = yr[8 * a1 + a0];
+            part.x += T0[q][a0].x * yv.x - T0[q][a0].y * yv.y;
+            part.y += T0[q][a0].x * yv.y + T0[q][a0].y * yv.x;
+          }
+          acc.x += T1[q][a1].x * part.x - T1[q][a1].y * part.y;
+          acc.y += T1[q][a1].x * part.y + T1[q][a1].y * part.x;
+        }
+        if (t < twoM) zdst[t] = acc;
+      }
+    }
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  }
+}
+
+// Pass 2.  One block per pair: W(dy,dx) = sum_kx Re(Z[kx][dy] e^{+2 pi i kx dx/Wc}),
+// then C = nc2 * W / max(1, Denom).  Thread owns 4 dx x 5 dy outputs.
+constexpr int K2R_DX = 4, K2R_DY = 5, K2R_KC = 32;
+__global__ __launch_bounds__(1024) void k2_rows(const float2* __restrict__ Z, const int4* __restrict__ pairs, int Wh,
+                                                int Wc, int m, const float2* __restrict__ twWc,
+                                                const double* __restrict__ den, float nc2,
+                                                float* __restrict__ windows) {
+  extern __shared__ float2 zs[];  // [K2R_KC][2m]
+  const int twoM = 2 * m;
+  const int ndxg = (twoM + K2R_DX - 1) / K2R_DX;
+  const int p = blockIdx.x;
+  const int tid = threadIdx.x;
+  const int gx = tid % ndxg, gy = tid / ndxg;
+  const int dx0 = gx * K2R_DX, dy0 = gy * K2R_DY;
+  float acc[K2R_DY][K2R_DX];
+#pragma unroll
+  for (int i = 0; i < K2R_DY; ++i)
+#pragma unroll
+    for (int j = 0; j < K2R_DX; ++j) acc[i][j] = 0.f;
+  int idx[K2R_DX], dd[K2R_DX];
+#pragma unroll
+  for (int j = 0; j < K2R_DX; ++j) {
+    int d = dx0 + j - m;
+    dd[j] = ((d % Wc) + Wc) % Wc;
+    idx[j] = 0;
+  }
+  const float2* zp = Z + (size_t)p * Wh * twoM;
+  for (int k0 = 0; k0 < Wh; k0 += K2R_KC) {
+    const int kc = min(K2R_KC, Wh - k0);
+    __syncthreads();
+    for (int e = tid; e < kc * twoM; e += blockDim.x) zs[e] = zp[(size_t)k0 * twoM + e];
+    __syncthreads();
+    if (dy0 < twoM) {
+      for (int k = 0; k < kc; ++k) {
+        float2 tw[K2R_DX];
+#pragma unroll
+        for (int j = 0; j < K2R_DX; ++j) {
+          float2 t = twWc[idx[j]];
+          tw[j] = make_float2(t.x, -t.y);
+          idx[j] += dd[j];
+          if (idx[j] >= Wc) idx[j] -= Wc;
+        }
+        const float2* zr = zs + k * twoM;
+#pragma unroll
+        for (int i = 0; i < K2R_DY; ++i) {
+          int dy = dy0 + i;
+          float2 z = (dy < twoM) ? zr[dy] : make_float2(0.f, 0.f);
+#pragma unroll
+          for (int j = 0; j < K2R_DX; ++j) acc[i][j] += z.x * tw[j].x - z.y * tw[j].y;
+        }
+      }
+    }
+  }
+  if (dy0 < twoM && dx0 < twoM) {
+    const float s = (float)((double)nc2 / fmax(1.0, den[p]));
+    float* wdst = windows + (size_t)p * twoM * twoM;
+#pragma unroll
+    for (int i = 0; i < K2R_DY; ++i)
+#pragma unroll
+      for (int j = 0; j < K2R_DX; ++j) {
+        int dy = dy0 + i, dx = dx0 + j;
+        if (dy < twoM && dx < twoM) wdst[dy * twoM + dx] = acc[i][j] * s;
+      }
+  }
+}
+
+// Per-pair normalisation denominator on the unmasked path (FP64):
+// D = nc2 * sqrt((Np*SQ_J - S_J^2)(Np*SQ_I - S_I^2))
+__global__ void k2_denominators(const int4* __restrict__ pairs, int npairs, const double* __restrict__ stats,
+                                double npix, double nc2, double* __restrict__ den) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= npairs) return;
+  int4 pr = pairs[i];
+  double sI = stats[2 * pr.x], qI = stats[2 * pr.x + 1];
+  double sJ = stats[2 * pr.y], qJ = stats[2 * pr.y + 1];
+  double a = fmax(npix * qJ - sJ * sJ, 0.0), b = fmax(npix * qI - sI * sI, 0.0);
+  den[pr.z] = nc2 * sqrt(a * b);
+}
+
+// --------------------------------------------------------------------- KP
+struct PeakParams {
+  int m;             // maxShift
+  int suppress;      // suppressOrigin
+  int bconv;         // 1: Gaussian on the window
+  int ry, rx;        // Gaussian radii
+  int subpix;        // subPixReg (<=1: integer)
+  float scale_y, scale_x;   // Hp/Hc, Wp/Wc (1 when no crop)
+  int n;             // frames (trimat side)
+};
+
+ZR_DEV int reflect_idx(int i, int n) {
+  int p = 2 * n;
+  i %= p;
+  if (i < 0) i += p;
+  return i < n ? i : p - 1 - i;
+}
+
+// One block (256 threads) per pair.
+__global__ __launch_bounds__(256) void k_peaks(const float* __restrict__ windows, const int4* __restrict__ pairs,
+                                              PeakParams pp, const float* __restrict__ gy,
+                                              const float* __restrict__ gx, const float2* __restrict__ ktab,
+                                              float* __restrict__ shifts, float* __restrict__ corr,
+                                              float* __restrict__ peaksig, float* __restrict__ origin) {
+  extern __shared__ float smem[];
+  const int m = pp.m, n2 = 2 * pp.m, nn = n2 * n2;
+  float* C = smem;            // raw window (after suppression)
+  float* Cf = C + nn;         // filtered
+  float* tmp = Cf + nn;       // filter temp
+  __shared__ double rs[2][4];
+  __shared__ float rv[4];
+  __shared__ int ri[4];
+  __shared__ int s_cmax;
+  const int tid = threadIdx.x;
+  const int4 pr = pairs[blockIdx.x];
+  const int p = pr.z;
+  const float* win = windows + (size_t)p * nn;
+  for (int e = tid; e < nn; e += 256) C[e] = win[e];
+  __syncthreads();
+  if (tid == 0) {
+    const int c = m * n2 + m;
+    origin[pr.x * pp.n + pr.y] = C[c];
+    if (pp.suppress) {
+      // lag (ly,lx) -> index (m+ly)*2m + (m+lx); stencil with duplicated terms
+      float v = C[(m + 1) * n2 + m] + C[m * n2 + m + 1] + C[(m - 1) * n2 + m] + C[(m - 1) * n2 + m] +
+                C[(m + 1) * n2 + m + 1] + C[(m - 1) * n2 + m + 1] + C[(m - 1) * n2 + m + 1] +
+                C[(m - 1) * n2 + m - 1];
+      C[c] = 0.125f * v;
+    }
+  }
+  __syncthreads();
+  if (pp.bconv) {
+    for (int e = tid; e < nn; e += 256) {
+      int i = e / n2, j = e % n2;
+      float s = 0.f;
+      for (int k = -pp.ry; k <= pp.ry; ++k) s += gy[k + pp.ry] * C[reflect_idx(i + k, n2) * n2 + j];
+      tmp[e] = s;
+    }
+    __syncthreads();
+    for (int e = tid; e < nn; e += 256) {
+      int i = e / n2, j = e % n2;
+      float s = 0.f;
+      for (int k = -pp.rx; k <= pp.rx; ++k) s += gx[k + pp.rx] * tmp[i * n2 + reflect_idx(j + k, n2)];
+      Cf[e] = s;
+    }
+  } else {
+    for (int e = tid; e < nn; e += 256) Cf[e] = C[e];
+  }
+  __syncthreads();
+  // disc statistics and argmax of Cf*disc (zeros outside take part; first index wins ties)
+  double s1 = 0.0, s2 = 0.0, cnt = 0.0;
+  float bv = -INFINITY;
+  int bi = 0x7fffffff;
+  const long m2 = (long)m * m;
+  for (int e = tid; e < nn; e += 256) {
+    int i = e / n2, j = e % n2;
+    long yy = i - m, xx = j - m;
+    bool in = (yy * yy + xx * xx) < m2;
+    float v = in ? Cf[e] : 0.f;
+    if (in) { s1 += v; cnt += 1.0; }
+    if (v > bv || (v == bv && e < bi)) { bv = v; bi = e; }
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    float ov = __shfl_xor(bv, o, 64);
+    int oi = __shfl_xor(bi, o, 64);
+    if (ov > bv || (ov == bv && oi < bi)) { bv = ov; bi = oi; }
+    s1 += __shfl_xor(s1, o, 64);
+    cnt += __shfl_xor(cnt, o, 64);
+  }
+  if ((tid & 63) == 0) { rv[tid >> 6] = bv; ri[tid >> 6] = bi; rs[0][tid >> 6] = s1; rs[1][tid >> 6] = cnt; }
+  __syncthreads();
+  double mean, total_cnt;
+  {
+    float v = rv[0]; int ix = ri[0];
+    double a = rs[0][0], c = rs[1][0];
+    for (int k = 1; k < 4; ++k) {
+      if (rv[k] > v || (rv[k] == v && ri[k] < ix)) { v = rv[k]; ix = ri[k]; }
+      a += rs[0][k]; c += rs[1][k];
+    }
+    bv = v; bi = ix; mean = a / c; total_cnt = c;
+  }
+  __syncthreads();
+  for (int e = tid; e < nn; e += 256) {
+    int i = e / n2, j = e % n2;
+    long yy = i - m, xx = j - m;
+    if ((yy * yy + xx * xx) < m2) { double d = (double)Cf[e] - mean; s2 += d * d; }
+  }
+  s2 = wave_sum_d(s2);
+  if ((tid & 63) == 0) rs[0][tid >> 6] = s2;
+  __syncthreads();
+  const double var = (rs[0][0] + rs[0][1] + rs[0][2] + rs[0][3]) / total_cnt;
+  const int cy = bi / n2, cx = bi % n2;
+  const float psig = (float)(((double)bv - mean) / sqrt(var));
+  const int I = pr.x, J = pr.y, n = pp.n;
+  float sh_y = (float)(cy - m), sh_x = (float)(cx - m), cval = C[bi];
+  // sub-pixel: python-slice semantics of C[c-8:c+8]
+  bool valid = false;
+  int sy = cy - 8, sx = cx - 8;
+  if (pp.subpix > 1) {
+    auto slice_ok = [&](int c0, int& s) {
+      int st = c0 - 8, en = c0 + 8;
+      if (st < 0) { st += n2; if (st < 0) st = 0; }
+      en = en < n2 ? en : n2;
+      s = st;
+      return (en - st) == 16;
+    };
+    valid = slice_ok(cy, sy) & slice_ok(cx, sx);
+  }
+  if (valid) {
+    const int P = pp.subpix, G = 16 * P;
+    float2* Tm = reinterpret_cast<float2*>(tmp);     // [16][G]
+    float2* Ks = reinterpret_cast<float2*>(tmp + (size_t)32 * G);   // K table copy [G]
+    __syncthreads();
+    for (int e = tid; e < G; e += 256) Ks[e] = ktab[e];
+    __syncthreads();
+    // T[p][t] = sum_q c[p][q] K(t - P q)   (K is G-periodic)
+    for (int e = tid; e < 16 * G; e += 256) {
+      int pr_ = e / G, t = e % G;
+      float2 acc = make_float2(0.f, 0.f);
+      for (int q = 0; q < 16; ++q) {
+        float cv = Cf[(sy + pr_) * n2 + sx + q];
+        int ki = t - P * q;
+        ki += (ki < 0) ? G : 0;
+        float2 kv = Ks[ki];
+        acc.x += cv * kv.x;
+        acc.y += cv * kv.y;
+      }
+      Tm[e] = acc;
+    }
+    __syncthreads();
+    // Y[s][t] = sum_p K(s - P p) T[p][t];  argmax |Y|^2, first in C order
+    float best = -1.f;
+    int bidx = 0x7fffffff;
+    for (int t = tid; t < G; t += 256) {
+      float2 tc[16];
+      for (int q = 0; q < 16; ++q) tc[q] = Tm[q * G + t];
+      for (int s = 0; s < G; ++s) {
+        float2 acc = make_float2(0.f, 0.f);
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+          int ki = s - P * q;
+          ki += (ki < 0) ? G : 0;
+          float2 kv = Ks[ki];
+          acc.x += kv.x * tc[q].x - kv.y * tc[q].y;
+          acc.y += kv.x * tc[q].y + kv.y * tc[q].x;
+        }
+        float a2 = acc.x * acc.x + acc.y * acc.y;
+        int id = s * G + t;
+        if (a2 > best || (a2 == best && id < bidx)) { best = a2; bidx = id; }
+      }
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+      float ov = __shfl_xor(best, o, 64);
+      int oi = __shfl_xor(bidx, o, 64);
+      if (ov > best || (ov == best && oi < bidx)) { best = ov; bidx = oi; }
+    }
+    __syncthreads();
+    if ((tid & 63) == 0) { rv[tid >> 6] = best; ri[tid >> 6] = bidx; }
+    __syncthreads();
+    if (tid == 0) {
+      float v = rv[0]; int ix = ri[0];
+      for (int k = 1; k < 4; ++k)
+        if (rv[k] > v || (rv[k] == v && ri[k] < ix)) { v = rv[k]; ix = ri[k]; }
+      int smy = ix / G, smx = ix % G;
+      float rpy = (float)(cy - m), rpx = (float)(cx - m);
+      float sgy = (rpy > 0.f) - (rpy < 0.f), sgx = (rpx > 0.f) - (rpx < 0.f);
+      sh_y = rpy + ((float)smy - (float)(G / 2) + sgy) / (float)P;
+      sh_x = rpx + ((float)smx - (float)(G / 2) + sgx) / (float)P;
+      cval = sqrtf(v) / 256.f;
+    }
+  }
+  if (tid == 0) {
+    shifts[((size_t)I * n + J) * 2 + 0] = sh_y * pp.scale_y;
+    shifts[((size_t)I * n + J) * 2 + 1] = sh_x * pp.scale_x;
+    corr[I * n + J] = cval;
+    peaksig[I * n + J] = psig;
+  }
+}
+
+// ----------------------------------------------------------- launchers
+// Host-side launch helpers (instantiated for the supported sizes).
+#include "zr_launch.h"
+
+template <int WP>
+static hipError_t launch_k1_rows_t(const float* frames, int H, int W, int Hp, int Wh, int frame0, int nb,
+                                   const float2* twL, const float2* tw2L, float2* T, double* part, hipStream_t st) {
+  constexpr int RB = (WP >= 4096) ? 4 : 8;
+  constexpr int NT = 256;
+  int grid = nb * (Hp / RB);
+  size_t sh = (size_t)RB * (WP / 2) * sizeof(float2);
+  static bool attr = false;
+  if (!attr) {
+    hipFuncSetAttribute((const void*)k1_rows<WP, RB, NT>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh);
+    attr = true;
+  }
+  hipLaunchKernelGGL((k1_rows<WP, RB, NT>), dim3(grid), dim3(NT), sh, st, frames, H, W, Hp, Wh, frame0, twL, tw2L,
+                     T, part);
+  return hipGetLastError();
+}
+
+hipError_t zr_launch_k1_rows(int Wp, const float* frames, int H, int W, int Hp, int Wh, int frame0, int nb,
+                             const float2* twL, const float2* tw2L, float2* T, double* part, int* rb_out,
+                             hipStream_t st) {
+  switch (Wp) {
+#define ZR_CASE(X) case X: *rb_out = (X >= 4096) ? 4 : 8; return launch_k1_rows_t<X>(frames, H, W, Hp, Wh, frame0, nb, twL, tw2L, T, part, st);
+    ZR_CASE(256) ZR_CASE(512) ZR_CASE(1024) ZR_CASE(2048) ZR_CASE(3072) ZR_CASE(3840) ZR_CASE(4096)
+#undef ZR_CASE
+    default: return hipErrorInvalidValue;
+  }
+}
+
+hipError_t zr_launch_k1_cols(int Hp, const float2* T, int Wh, int Hc, int frame0, int nb, const float2* twH,
+                             float2* S, float2* Sside, hipStream_t st) {
+  constexpr int NT = 256;
+  int grid = nb * Wh;
+  switch (Hp) {
+#define ZR_CASE(X) case X: hipLaunchKernelGGL((k1_cols<X, NT>), dim3(grid), dim3(NT), 0, st, T, Wh, Hc, frame0, twH, S, Sside); return hipGetLastError();
+    ZR_CASE(256) ZR_CASE(512) ZR_CASE(1024) ZR_CASE(2048) ZR_CASE(3072) ZR_CASE(3840) ZR_CASE(4096)
+#undef ZR_CASE
+    default: return hipErrorInvalidValue;
+  }
+}
+
+hipError_t zr_launch_k1_stats(const double* part, int nrb, int n, double* stats, hipStream_t st) {
+  hipLaunchKernelGGL(k1_frame_stats, dim3((n + 63) / 64), dim3(64), 0, st, part, nrb, n, stats);
+  return hipGetLastError();
+}
+
+static constexpr int K2C_WPB = 2;
+hipError_t zr_launch_k2_cols(int Lper, const float2* S, const float2* Sside, const int4* work, int npairs, int Wh,
+                             int Hc, int Wc, int m, int cols_per_wave, int dc_zero, const float2* twHc, float2* Z,
+                             hipStream_t st) {
+  int cols_per_block = K2C_WPB * cols_per_wave;
+  int ngroups = (Wh + cols_per_block - 1) / cols_per_block;
+  int grid = ngroups * npairs;
+  switch (Lper) {
+#define ZR_CASE(X) case X: { size_t sh = (size_t)K2C_WPB * X * 65 * sizeof(float2); static bool attr = false; \
+      if (!attr) { hipFuncSetAttribute((const void*)k2_cols<X, K2C_WPB>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh); attr = true; } \
+      hipLaunchKernelGGL((k2_cols<X, K2C_WPB>), dim3(grid), dim3(64 * K2C_WPB), sh, st, S, Sside, work, npairs, Wh, Hc, Wc, m, cols_per_wave, dc_zero, twHc, Z); return hipGetLastError(); }
+    ZR_CASE(4) ZR_CASE(8) ZR_CASE(16) ZR_CASE(32) ZR_CASE(48) ZR_CASE(64)
+#undef ZR_CASE
+    default: return hipErrorInvalidValue;
+  }
+}
+
+hipError_t zr_launch_k2_rows(const float2* Z, const int4* pairs, int npairs, int Wh, int Wc, int m,
+                             const float2* twWc, const double* den, float nc2, float* windows, hipStream_t st) {
+  int twoM = 2 * m;
+  int ndxg = (twoM + K2R_DX - 1) / K2R_DX, ndyg = (twoM + K2R_DY - 1) / K2R_DY;
+  int nt = ndxg * ndyg;
+  if (nt > 1024) return hipErrorInvalidValue;
+  nt = ((nt + 63) / 64) * 64;
+  size_t sh = (size_t)K2R_KC * twoM * sizeof(float2);
+  hipLaunchKernelGGL(k2_rows, dim3(npairs), dim3(nt), sh, st, Z, pairs, Wh, Wc, m, twWc, den, nc2, windows);
+  return hipGetLastError();
+}
+
+hipError_t zr_launch_k2_den(const int4* pairs, int npairs, const double* stats, double npix, double nc2, double* den,
+                            hipStream_t st) {
+  hipLaunchKernelGGL(k2_denominators, dim3((npairs + 127) / 128), dim3(128), 0, st, pairs, npairs, stats, npix, nc2,
+                     den);
+  return hipGetLastError();
+}
+
+hipError_t zr_launch_peaks(const float* windows, const int4* pairs, int npairs, const PeakParamsHost* hp,
+                           const float* gy, const float* gx, const float2* ktab, float* shifts, float* corr,
+                           float* peaksig, float* origin, hipStream_t st) {
+  PeakParams pp;
+  pp.m = hp->m; pp.suppress = hp->suppress; pp.bconv = hp->bconv; pp.ry = hp->ry; pp.rx = hp->rx;
+  pp.subpix = hp->subpix; pp.scale_y = hp->scale_y; pp.scale_x = hp->scale_x; pp.n = hp->n;
+  int nn = 4 * pp.m * pp.m;
+  int G = 16 * (pp.subpix > 1 ? pp.subpix : 1);
+  size_t tmp_floats = (size_t)nn > (size_t)34 * G ? (size_t)nn : (size_t)34 * G;
+  size_t sh = (2 * (size_t)nn + tmp_floats) * sizeof(float);
+  static int attr = 0;
+  if ((int)sh > attr) {
+    hipFuncSetAttribute((const void*)k_peaks, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh);
+    attr = (int)sh;
+  }
+  hipLaunchKernelGGL(k_peaks, dim3(npairs), dim3(256), sh, st, windows, pairs, pp, gy, gx, ktab, shifts, corr,
+                     peaksig, origin);
+  return hipGetLastError();
+}

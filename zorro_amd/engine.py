@@ -1,0 +1,177 @@
+"""Thin object layer over the C ABI: one ``MoviePlan`` per movie shape.
+
+``MoviePlan`` owns a ``zr_movie`` (device workspaces for one frame count and
+shape) and exposes the stages of ``ImageRegistrator.alignImageStack``
+(``zorro/zorro.py:1569-1716``) one by one, plus result read-back.  Frames may be
+a NumPy array (copied to the device) or a torch CUDA tensor (borrowed, no
+copy).  Nothing here computes on the CPU.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib as L
+
+TRI = {"tri": L.ZR_TRI_TRI, "diag": L.ZR_TRI_DIAG, "auto": L.ZR_TRI_AUTO}
+BMODE = {None: L.ZR_B_NONE, "none": L.ZR_B_NONE, "conv": L.ZR_B_CONV, "convolution": L.ZR_B_CONV,
+         "opti": L.ZR_B_OPTI}
+WMODE = {"autologistic": L.ZR_W_AUTOLOGISTIC, "logistic": L.ZR_W_LOGISTIC, "corr": L.ZR_W_CORR, None: L.ZR_W_CORR,
+         "unweighted": L.ZR_W_UNWEIGHTED, "norm": L.ZR_W_NORM, "normalized": L.ZR_W_NORM}
+
+
+def make_params(triMode="diag", diagWidth=5, diagStart=0, maxShift=100, Bmode="opti", Brad=512.0, subPixReg=16,
+                shiftMethod="lanczos", weightMode="autologistic", peaksigThres=6.0, logisticK=5.0,
+                logisticNu=0.15, originMode="centroid", suppressOrigin=True, fouCrop=(3072, 3072),
+                shapePadded=(4096, 4096), filterMode=None, doseFiltParam=(None, 0.24499, -1.6649, 2.8141, 32, 0),
+                pixelsize=None, voltage=300.0, doLazyFRC=True, frc_box=512, frc_overlap=0.5) -> L.ZrParams:
+    """ZrParams from ImageRegistrator attribute values (defaults of zorro/zorro.py:82-243)."""
+    p = L.ZrParams()
+    if triMode not in TRI:
+        raise NotImplementedError(f"triMode {triMode!r} is not built")
+    p.tri_mode = TRI[triMode]
+    p.diag_width = -1 if diagWidth is None else int(diagWidth)
+    p.diag_start = int(diagStart)
+    if not maxShift:
+        raise NotImplementedError("maxShift=None is not built")
+    p.max_shift = int(maxShift)
+    if Bmode == "fourier":
+        raise NotImplementedError("Bmode='fourier' is broken in the reference (zorro/zorro.py:1125-1133)")
+    if not Brad and Bmode in ("conv",):
+        Bmode = None
+    p.b_mode = BMODE.get(Bmode, L.ZR_B_NONE)
+    p.b_rad = float(Brad or 0.0)
+    p.sub_pix_reg = int(subPixReg or 1)
+    if shiftMethod == "fourier":
+        raise NotImplementedError("shiftMethod='fourier' is broken in the reference (zorro/zorro.py:891-898)")
+    p.shift_method = L.ZR_SHIFT_LANCZOS if shiftMethod == "lanczos" else L.ZR_SHIFT_INTEGER
+    p.weight_mode = WMODE.get(weightMode, L.ZR_W_CORR)
+    p.peaksig_thres, p.logistic_k, p.logistic_nu = float(peaksigThres), float(logisticK), float(logisticNu)
+    p.origin_centroid = 1 if originMode == "centroid" else 0
+    p.suppress_origin = 1 if suppressOrigin else 0
+    fc = (0, 0) if fouCrop is None or not np.any(fouCrop) else tuple(int(v) for v in fouCrop)
+    p.fou_crop[0], p.fou_crop[1] = fc
+    p.shape_padded[0], p.shape_padded[1] = (int(shapePadded[0]), int(shapePadded[1]))
+    modes = [] if not filterMode else filterMode.lower().replace(" ", "").split(",")
+    p.do_dose = 1 if "dose" in modes else 0
+    if p.do_dose and doseFiltParam[0] is not None:
+        raise NotImplementedError("doseFiltParam[0] must be None (the reference raises NameError otherwise, Q11)")
+    p.dose_a, p.dose_b, p.dose_c, p.dose_order, p.dose_missing = (float(v) for v in doseFiltParam[1:6])
+    p.pixelsize_nm = float(pixelsize) if pixelsize else 0.0
+    p.voltage_kv = float(voltage or 0.0)
+    p.do_frc = 1 if doLazyFRC else 0
+    p.frc_box = int(frc_box)
+    p.frc_overlap = float(frc_overlap)
+    return p
+
+
+class MoviePlan:
+    """Device plan for one movie of ``n`` frames of ``h`` x ``w``."""
+
+    def __init__(self, params: L.ZrParams, n: int, h: int, w: int, device: int = 0):
+        self.lib = L.load()
+        self.params = params
+        self.n, self.h, self.w = int(n), int(h), int(w)
+        self._mv = C.c_void_p()
+        L.check(self.lib.zr_movie_create(int(device), C.byref(params), self.n, self.h, self.w, C.byref(self._mv)))
+        npairs = C.c_int()
+        L.check(self.lib.zr_pair_count(self._mv, C.byref(npairs)))
+        self.npairs = npairs.value
+        self.m = params.max_shift
+        self._frames_ref = None
+
+    def close(self):
+        if self._mv:
+            self.lib.zr_movie_destroy(self._mv)
+            self._mv = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_stream(self, stream_handle: int | None):
+        L.check(self.lib.zr_movie_set_stream(self._mv, C.c_void_p(stream_handle or 0)))
+
+    def set_frames(self, frames):
+        """NumPy float32 [n,h,w] (copied) or a contiguous torch float32 CUDA tensor (borrowed)."""
+        if hasattr(frames, "data_ptr") and getattr(frames, "is_cuda", False):
+            if frames.dtype.__str__() != "torch.float32" or not frames.is_contiguous():
+                raise ValueError("frames must be a contiguous float32 CUDA tensor")
+            if tuple(frames.shape) != (self.n, self.h, self.w):
+                raise ValueError(f"frames shape {tuple(frames.shape)} != plan {(self.n, self.h, self.w)}")
+            self._frames_ref = frames
+            L.check(self.lib.zr_movie_set_frames(self._mv, C.c_void_p(frames.data_ptr()), 1))
+        else:
+            a = np.ascontiguousarray(frames, dtype=np.float32)
+            if a.shape != (self.n, self.h, self.w):
+                raise ValueError(f"frames shape {a.shape} != plan {(self.n, self.h, self.w)}")
+            self._frames_ref = a
+            L.check(self.lib.zr_movie_set_frames(self._mv, C.c_void_p(a.ctypes.data), 0))
+
+    # ---- stages
+    def xcorr(self):
+        L.check(self.lib.zr_xcorr(self._mv))
+
+    def solve(self):
+        L.check(self.lib.zr_solve(self._mv))
+
+    def apply_shifts(self, aligned_out=None):
+        ptr = C.c_void_p(aligned_out.data_ptr()) if aligned_out is not None else C.c_void_p(0)
+        L.check(self.lib.zr_apply_shifts(self._mv, ptr))
+
+    def frc(self):
+        L.check(self.lib.zr_frc(self._mv))
+
+    def dose_filter(self):
+        L.check(self.lib.zr_dose_filter(self._mv))
+
+    def align(self):
+        L.check(self.lib.zr_align(self._mv))
+
+    def synchronize(self):
+        L.check(self.lib.zr_synchronize(self._mv))
+
+    # ---- results
+    def trimats(self):
+        n = self.n
+        sh = np.zeros((n, n, 2), np.float32)
+        co = np.zeros((n, n), np.float32)
+        ps = np.zeros((n, n), np.float32)
+        og = np.zeros((n, n), np.float32)
+        L.check(self.lib.zr_get_trimats(self._mv, L.fptr(sh), L.fptr(co), L.fptr(ps), L.fptr(og)))
+        return dict(shiftsTriMat=sh, corrTriMat=co, peaksigTriMat=ps, originTriMat=og)
+
+    def windows(self):
+        w = np.zeros((self.npairs, 2 * self.m, 2 * self.m), np.float32)
+        L.check(self.lib.zr_get_windows(self._mv, L.fptr(w)))
+        return w
+
+    def solution(self):
+        n = self.n
+        M = self.npairs
+        t = np.zeros((n, 2))
+        rel = np.zeros((n - 1, 2))
+        w = np.zeros(M)
+        lg = np.zeros(3)
+        err = np.zeros((M, 2))
+        es = np.zeros(2)
+        exy = np.zeros((n, 2))
+        L.check(self.lib.zr_get_solution(self._mv, L.dptr(t), L.dptr(rel), L.dptr(w), L.dptr(lg), L.dptr(err),
+                                         L.dptr(es), L.dptr(exy)))
+        return dict(translations=t, relativeEst=rel, weights=w, logistic=lg, errorXY=err,
+                    mean_errorNorm=es[0], std_errorNorm=es[1], errorX=exy[:, 1], errorY=exy[:, 0])
+
+    def sums(self):
+        h, w = self.h, self.w
+        out = [np.zeros((h, w), np.float32) for _ in range(4)]
+        L.check(self.lib.zr_get_sums(self._mv, *(L.fptr(a) for a in out)))
+        return dict(imageSum=out[0], filtSum=out[1], evenSum=out[2], oddSum=out[3])
+
+    def frc_curve(self):
+        n = C.c_int()
+        buf = np.zeros(4096)
+        L.check(self.lib.zr_get_frc(self._mv, L.dptr(buf), C.byref(n)))
+        return buf[:n.value].copy()
