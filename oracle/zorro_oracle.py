@@ -392,8 +392,9 @@ def logistic(h, thr, K, nu):
 
 
 def logistic_objective(x, hSigma, cdf):
-    """``minLogistic`` without the float32 cast.  ``zorro/zorro_util.py:862-863``."""
-    return float(np.sum(np.abs(cdf - logistic(hSigma, x[0], x[1], x[2]))))
+    """``minLogistic`` without the float32 cast, summed in index order.
+    ``zorro/zorro_util.py:862-863``."""
+    return float(np.cumsum(np.abs(np.asarray(cdf, np.float64) - logistic(np.asarray(hSigma, np.float64), x[0], x[1], x[2])))[-1])
 
 
 # Deterministic replacement for basinhopping in cdfLogisticCurve (documented in
@@ -414,8 +415,8 @@ def logistic_fit(hSigma, cdf, lo, hi):
     best candidate on strict improvement (first in C order on ties), otherwise
     halve the steps; stop after LOGI_ROUNDS halvings or LOGI_MAX_ITERS steps.
     """
-    hSigma = np.asarray(hSigma, dtype=np.float64)
-    cdf = np.asarray(cdf, dtype=np.float64)
+    hSigma = np.asarray(hSigma).astype(np.float64)
+    cdf = np.asarray(cdf).astype(np.float64)
     g0, g1, g2 = LOGI_GRID
     lk0, lk1 = math.log(LOGI_K_BOUNDS[0]), math.log(LOGI_K_BOUNDS[1])
     ln0, ln1 = math.log(LOGI_NU_BOUNDS[0]), math.log(LOGI_NU_BOUNDS[1])
@@ -423,8 +424,8 @@ def logistic_fit(hSigma, cdf, lo, hi):
     lk = lk0 + (lk1 - lk0) * np.arange(g1) / (g1 - 1)
     ln = ln0 + (ln1 - ln0) * np.arange(g2) / (g2 - 1)
     T, LK, LN = np.meshgrid(thr, lk, ln, indexing="ij")
-    obj = np.abs(cdf[None, :] - logistic(hSigma[None, :], T.reshape(-1, 1), np.exp(LK).reshape(-1, 1),
-                                         np.exp(LN).reshape(-1, 1))).sum(axis=1)
+    obj = np.cumsum(np.abs(cdf[None, :] - logistic(hSigma[None, :], T.reshape(-1, 1), np.exp(LK).reshape(-1, 1),
+                                                   np.exp(LN).reshape(-1, 1))), axis=1)[:, -1]
     best = int(np.argmin(obj))
     cur = np.array([T.reshape(-1)[best], LK.reshape(-1)[best], LN.reshape(-1)[best]])
     fcur = float(obj[best])
@@ -435,8 +436,8 @@ def logistic_fit(hSigma, cdf, lo, hi):
     halvings = 0
     for _ in range(LOGI_MAX_ITERS):
         cand = np.clip(cur[None, :] + offs * step[None, :], lob, hib)
-        vals = np.abs(cdf[None, :] - logistic(hSigma[None, :], cand[:, :1], np.exp(cand[:, 1:2]),
-                                              np.exp(cand[:, 2:3]))).sum(axis=1)
+        vals = np.cumsum(np.abs(cdf[None, :] - logistic(hSigma[None, :], cand[:, :1], np.exp(cand[:, 1:2]),
+                                                        np.exp(cand[:, 2:3]))), axis=1)[:, -1]
         k = int(np.argmin(vals))
         if vals[k] < fcur:
             cur, fcur = cand[k], float(vals[k])
@@ -451,11 +452,11 @@ def logistic_fit(hSigma, cdf, lo, hi):
 def cdf_logistic_curve(peaksigTriMat: np.ndarray):
     """Histogram CDF of positive peak significances and its logistic fit.
     ``zorro/zorro.py:1981-2022``."""
-    ps = peaksigTriMat[peaksigTriMat > 0.0]
+    ps = peaksigTriMat[peaksigTriMat > 0.0]          # float32, as the reference trimat
     bins = int(ps.size / 7.0)
-    pdf, edges = np.histogram(ps, bins=bins)
+    pdf, edges = np.histogram(ps, bins=bins)          # float32 edges (numpy dtype rules)
     hSigma = edges[:-1]
-    pdf = pdf.astype(np.float64)
+    pdf = pdf.astype(np.float32)                      # :1996-1998
     cdf = np.cumsum(pdf)
     cdf /= cdf[-1]
     thr, K, nu, f = logistic_fit(hSigma, cdf, float(ps.min()), float(ps.max()))

@@ -248,8 +248,8 @@ __global__ __launch_bounds__(64) void k_l1_solve(SolveArgs a) {
     for (int e = lane; e < M; e += 64) w[e] = (w[e] - mn) / (mx - mn);
     __syncthreads();
   }
-  if (axis == 0)
-    for (int e = lane; e < M; e += 64) a.weights[e] = w[e];
+  if (axis == 0)   // errorDict['weights'] holds the weights clipped to >= 1e-6
+    for (int e = lane; e < M; e += 64) a.weights[e] = fmax(w[e], 1e-6);
   for (int k = lane; k < N; k += 64) pv[k] = 0.0;
   __syncthreads();
   // ---- IRLS warm start on the position Laplacian (node 0 fixed)
