@@ -1,0 +1,145 @@
+"""Benchmark: registration + dose filter + sum of 40-frame 4096x4096 movies on one MI355X.
+
+One step = one movie through the whole alignImageStack path (forward spectra,
+780-pair tri cross-correlation at fouCrop 3072, peaks, FP64 shift solver,
+Lanczos shift + sums, even/odd FRC, dose filter), with the frames already
+resident in HBM.  Prints ONE JSON line (rank 0).
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+
+METRIC = "movies/min (40x4096² fp32, tri-xcorr+dosefilt+sum); p50/p99 latency"
+HBM_PEAK_GBS = 8000.0
+
+WORKLOAD = dict(n=40, h=4096, w=4096, seed=2, dose=1.0, jump=3.0)
+PARAMS = dict(triMode="tri", maxShift=40, Bmode="conv", Brad=150, subPixReg=16, shiftMethod="lanczos",
+              weightMode="autologistic", originMode="centroid", suppressOrigin=True, fouCrop=(3072, 3072),
+              shapePadded=(4096, 4096), filterMode="dose", pixelsize=0.1, voltage=300.0, doLazyFRC=True,
+              frc_box=512, frc_overlap=0.5)
+
+
+def k2_algorithmic_bytes(info):
+    """Per launch of the pair transform: each pair reads two cropped half spectra
+    and writes one 2m x 2m float32 window plus its peak record."""
+    hs = info["Hc"] * (info["Wc"] // 2 + 1) * 8
+    m = info["maxShift"]
+    return info["pairs"] * (2 * hs + 4 * (2 * m) ** 2 + 16)
+
+
+def load_traffic():
+    """HBM bytes per K2 launch from the committed rocprofv3 PMC summary, if any."""
+    p = os.path.join(HERE, "profiles", "k2_pairs_traffic.json")
+    if os.path.exists(p):
+        with open(p) as f:
+            return json.load(f).get("traffic_bytes")
+    return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path "
+              "(movies are independent; see DESIGN.md for how the work would shard).", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from zorro_amd.engine import MoviePlan, make_params
+    from zorro_amd.synth import make_movie_torch
+
+    if not torch.cuda.is_available():
+        print("no GPU visible", file=sys.stderr)
+        sys.exit(1)
+    dev = torch.device("cuda:0")
+    c = WORKLOAD
+    print(f"[bench] generating {c['n']}x{c['h']}x{c['w']} movie on {dev}", file=sys.stderr, flush=True)
+    frames, drift = make_movie_torch(c["n"], c["h"], c["w"], seed=c["seed"], dose=c["dose"], jump=c["jump"],
+                                     device=dev)
+    torch.cuda.synchronize()
+    plan = MoviePlan(make_params(**PARAMS), c["n"], c["h"], c["w"])
+    plan.set_frames(frames)
+    info = plan.info()
+
+    print(f"[bench] plan ready {info}; warmup {args.warmup}", file=sys.stderr, flush=True)
+    for _ in range(args.warmup):
+        plan.align()
+    plan.synchronize()
+    torch.cuda.synchronize()
+
+    plan.set_timing(True)
+    plan.stage_times(reset=True)
+    lat = []
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ts = time.perf_counter()
+        plan.align()
+        plan.synchronize()
+        lat.append((time.perf_counter() - ts) * 1e3)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    st = plan.stage_times()
+    plan.set_timing(False)
+    print(f"[bench] {args.steps} steps: {(t1 - t0) * 1e3 / args.steps:.2f} ms/step", file=sys.stderr, flush=True)
+
+    ms_step = (t1 - t0) * 1e3 / args.steps
+    movies_min = 60000.0 / ms_step
+    sol = plan.solution()
+    truth = -(drift - drift.mean(axis=0))
+    rms = float(np.sqrt(np.mean((sol["translations"] - truth) ** 2)))
+
+    k2_ms = st["pairs"][0] / max(st["pairs"][1], 1)
+    alg = k2_algorithmic_bytes(info)
+    achieved = alg / (k2_ms * 1e-3) / 1e9
+    traffic = load_traffic()
+    roofline = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
+                    frac=round(achieved / HBM_PEAK_GBS, 4), traffic=traffic,
+                    kernel="pair transform (k2_cols + k2_rows)", launch_ms=round(k2_ms, 4),
+                    algorithmic_bytes=alg)
+
+    out = dict(metric=METRIC, value=round(movies_min, 2), unit="movies/min", n_gpus=1, steps=args.steps,
+               warmup=args.warmup, ms_per_step=round(ms_step, 3), higher_is_better=True, scaling="weak",
+               vs_baseline=None, dtype="f32", data="synthetic",
+               config=dict(workload="40 frames x 4096x4096 float32 Poisson movie (1.0 e/px/frame, random-walk "
+                                    "drift + 3 px exponential jump): 780-pair tri xcorr at fouCrop 3072, "
+                                    "maxShift 40, Bmode conv Brad 150, subPixReg 16, autologistic weights, "
+                                    "L1 shifts solver, Lanczos shift + sum, even/odd FRC (512 tiles), "
+                                    "dose filter (1 A/px, 300 kV)", frames=c["n"], shape=[c["h"], c["w"]],
+                           pairs=info["pairs"], fouCrop=[info["Hc"], info["Wc"]], inputs="resident in HBM"),
+               frames_per_s=round(c["n"] * 1000.0 / ms_step, 1),
+               latency_ms=dict(p50=round(float(np.percentile(lat, 50)), 3),
+                               p99=round(float(np.percentile(lat, 99)), 3)),
+               stages_ms={k: round(v[0] / max(v[1], 1), 4) for k, v in st.items() if v[1]},
+               roofline=roofline,
+               check=dict(rms_vs_true_drift_px=round(rms, 4), solver=plan.solver_info()))
+
+    if not args.no_cpu_baseline:
+        print("[bench] timing the CPU baseline sample", file=sys.stderr, flush=True)
+        from oracle.cpu_reference_structure import time_movie
+        v, d = time_movie(n_frames=c["n"], shape=c["h"], crop=3072)
+        out["cpu_baseline"] = dict(value=round(v, 5), unit="movies/min", cores=d["cores"], kind="port",
+                                   sample=d["sample"], stage_s={k: round(x, 2) for k, x in d["stage_s"].items()})
+        out["speedup_vs_cpu"] = round(movies_min / v, 1)
+    plan.close()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

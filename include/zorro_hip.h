@@ -129,10 +129,32 @@ int zr_pair_count(zr_movie* mv, int* npairs);
  * error_xy_frames f64 [N][2] (errorX, errorY).  Any pointer may be NULL. */
 int zr_get_solution(zr_movie* mv, double* translations, double* relative_est, double* weights, double* logistic,
                     double* error, double* error_stats, double* error_xy_frames);
+/* Solver diagnostics: info6 = {positive peak significances, histogram bins,
+ * status y, status x (1 optimal vertex, 2 pivot limit, -1 disconnected),
+ * simplex pivots y, pivots x}. */
+int zr_get_solver_info(zr_movie* mv, int* info6);
 /* imageSum, filtSum, even and odd half sums f32 [H][W]; any may be NULL. */
 int zr_get_sums(zr_movie* mv, float* image_sum, float* filt_sum, float* even_sum, float* odd_sum);
 /* FRC f64 [box/2]; n receives box/2. */
 int zr_get_frc(zr_movie* mv, double* frc, int* n);
+
+/* Per-stage timing with HIP events on the movie's stream; stage ids follow the
+ * reference's bench dict (zorro/zorro.py:109: xcorr, solve, shifts, dose, frc). */
+enum {
+  ZR_STAGE_SPECTRA = 0,   /* forward spectra of every frame (xcorr0..2) */
+  ZR_STAGE_PAIRS = 1,     /* pair transforms to the lag windows (xcorr2..3) */
+  ZR_STAGE_PEAKS = 2,     /* B filter, peak, significance, sub-pixel (xcorr2..3) */
+  ZR_STAGE_SOLVE = 3,     /* weights + shift solver (solve0..1) */
+  ZR_STAGE_SHIFTS = 4,    /* shift + sums without the dose filter (shifts0..1) */
+  ZR_STAGE_DOSE = 5,      /* shift + sums + dose filter (shifts, dose0..1) */
+  ZR_STAGE_FRC = 6,       /* even/odd FRC (frc0..1) */
+  ZR_NSTAGES = 7
+};
+int zr_set_timing(zr_movie* mv, int enable);
+/* ms[ZR_NSTAGES] accumulated milliseconds, counts[ZR_NSTAGES] calls; reset!=0 clears. */
+int zr_get_stage_times(zr_movie* mv, double* ms, long* counts, int reset);
+/* dims[12] = N, H, W, Hp, Wp, Hc, Wc, Wh, maxShift, pairs, unmasked, K1 batch. */
+int zr_movie_info(zr_movie* mv, int* dims);
 
 /* Device pointers of results for zero-copy consumers (valid until destroy). */
 int zr_device_results(zr_movie* mv, const float** image_sum, const float** filt_sum, const double** translations);

@@ -20,13 +20,14 @@ pytestmark = pytest.mark.gpu
 CASES = {
     "512sq_12f": dict(n=12, h=512, w=512, seed=3, dose=1.5, box=256),
     "C1_1024sq_20f": dict(n=20, h=1024, w=1024, seed=1, dose=1.5, box=512),
+    "1024sq_10f_nodose": dict(n=10, h=1024, w=1024, seed=9, dose=1.5, box=512, filter=None),
 }
 
 
 def _kw(c):
     return dict(triMode="tri", maxShift=40, Bmode="conv", Brad=150, subPixReg=16, shiftMethod="lanczos",
                 weightMode="autologistic", originMode="centroid", suppressOrigin=True, fouCrop=(0, 0),
-                shapePadded=(c["h"], c["w"]), filterMode="dose", pixelsize=0.1, voltage=300.0, doLazyFRC=True,
+                shapePadded=(c["h"], c["w"]), filterMode=c.get("filter", "dose"), pixelsize=0.1, voltage=300.0, doLazyFRC=True,
                 frc_box=c["box"])
 
 
@@ -41,7 +42,8 @@ def run(request):
     plan.set_frames(frames)
     plan.align()
     out = dict(name=name, c=c, frames=frames, drift=drift, kw=kw, tm=plan.trimats(), sol=plan.solution(),
-               sums=plan.sums(), frc=plan.frc_curve())
+               sums=plan.sums(), frc=plan.frc_curve(), solver=plan.solver_info())
+    print(f"[gpu pipeline] case={name} solver {out['solver']}", flush=True)
     plan.close()
     print(f"[gpu pipeline] case={name} GPU done", flush=True)
     return out
@@ -55,6 +57,7 @@ def test_solver_matches_oracle_on_same_trimats(run):
     t_or = ed["translations"] - ed["translations"].mean(axis=0)
     t_gpu = run["sol"]["translations"]
     d = np.max(np.abs(t_gpu - t_or))
+    assert run["solver"]["status_x"] == 1 and run["solver"]["status_y"] == 1, run["solver"]
     lg = run["sol"]["logistic"]
     print(f"[gpu pipeline] {run['name']}: solver |dt|max={d:.3e} logistic gpu={lg} "
           f"oracle={(ed['peaksigThres'], ed['logisticK'], ed['logisticNu'])}", flush=True)
@@ -79,6 +82,8 @@ def test_sums_frc_dose_match_oracle_given_translations(run):
     dfrc = np.max(np.abs(run["frc"] - frc))
     print(f"[gpu pipeline] {run['name']}: FRC max abs diff {dfrc:.2e}", flush=True)
     assert run["frc"].shape == frc.shape and dfrc <= 1e-3
+    if run["kw"]["filterMode"] != "dose":
+        return
     fs, info = O.dose_filter(aligned, isum, p, (run["c"]["h"], run["c"]["w"]))
     rel = np.linalg.norm(g["filtSum"] - fs) / np.linalg.norm(fs)
     print(f"[gpu pipeline] {run['name']}: filtSum rel L2 {rel:.2e}", flush=True)

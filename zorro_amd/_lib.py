@@ -70,7 +70,13 @@ SIGNATURES = {
     "zr_get_sums": (C.c_int, [_P, _F, _F, _F, _F]),
     "zr_get_frc": (C.c_int, [_P, _D, _I]),
     "zr_device_results": (C.c_int, [_P, C.POINTER(_F), C.POINTER(_F), C.POINTER(_D)]),
+    "zr_get_solver_info": (C.c_int, [_P, _I]),
+    "zr_set_timing": (C.c_int, [_P, C.c_int]),
+    "zr_get_stage_times": (C.c_int, [_P, _D, C.POINTER(C.c_long), C.c_int]),
+    "zr_movie_info": (C.c_int, [_P, _I]),
 }
+
+STAGES = ["spectra", "pairs", "peaks", "solve", "shifts", "dose", "frc"]
 
 _lib = None
 

@@ -80,6 +80,50 @@ int dev_upload(T** p, const std::vector<T>& v) {
   return ZR_OK;
 }
 
+// Per-stage HIP-event timing on the movie's stream (accumulated on read-out).
+struct StageTimer {
+  struct Pending { int stage; hipEvent_t a, b; };
+  std::vector<Pending> pending;
+  std::vector<hipEvent_t> pool;
+  double ms[ZR_NSTAGES] = {0};
+  long count[ZR_NSTAGES] = {0};
+  bool on = false;
+  hipEvent_t get() {
+    if (!pool.empty()) { hipEvent_t e = pool.back(); pool.pop_back(); return e; }
+    hipEvent_t e = nullptr;
+    hipEventCreate(&e);
+    return e;
+  }
+  hipEvent_t begin(hipStream_t s) {
+    if (!on) return nullptr;
+    hipEvent_t e = get();
+    hipEventRecord(e, s);
+    return e;
+  }
+  void end(int stage, hipEvent_t a, hipStream_t s) {
+    if (!on || !a) return;
+    hipEvent_t b = get();
+    hipEventRecord(b, s);
+    pending.push_back({stage, a, b});
+  }
+  void harvest() {
+    for (auto& p : pending) {
+      hipEventSynchronize(p.b);
+      float t = 0.f;
+      hipEventElapsedTime(&t, p.a, p.b);
+      ms[p.stage] += t;
+      count[p.stage] += 1;
+      pool.push_back(p.a);
+      pool.push_back(p.b);
+    }
+    pending.clear();
+  }
+  ~StageTimer() {
+    for (auto& p : pending) { hipEventDestroy(p.a); hipEventDestroy(p.b); }
+    for (auto e : pool) hipEventDestroy(e);
+  }
+};
+
 }  // namespace
 
 struct zr_movie {
@@ -108,6 +152,7 @@ struct zr_movie {
   float *shifts = nullptr, *corr = nullptr, *peaksig = nullptr, *origin = nullptr;
   bool have_xcorr = false;
   zr_stage_state* st = nullptr;   // solver / shift / FRC / dose state (zr_stages.h)
+  StageTimer timer;
 };
 
 extern "C" const char* zr_last_error(void) { return g_err.c_str(); }
@@ -292,6 +337,7 @@ extern "C" int zr_xcorr(zr_movie* mv) {
   ZR_HIP(hipSetDevice(mv->device));
   hipStream_t s = mv->stream;
   int rb = 0;
+  hipEvent_t t0 = mv->timer.begin(s);
   // K1: spectra in batches of frames
   for (int f0 = 0; f0 < mv->N; f0 += mv->batch) {
     int nb = std::min(mv->batch, mv->N - f0);
@@ -302,12 +348,16 @@ extern "C" int zr_xcorr(zr_movie* mv) {
   ZR_HIP(zr_launch_k1_stats(mv->part, mv->Hp / rb, mv->N, mv->stats, s));
   const double nc2 = 1.0 / std::pow((double)mv->Hc * (double)mv->Wc, 2.0);
   ZR_HIP(zr_launch_k2_den(mv->pairs, mv->P, mv->stats, (double)mv->Hp * (double)mv->Wp, nc2, mv->den, s));
+  mv->timer.end(ZR_STAGE_SPECTRA, t0, s);
   // K2
+  hipEvent_t t1 = mv->timer.begin(s);
   ZR_HIP(zr_launch_k2_cols(mv->Lper, mv->S, mv->Sside, mv->work, mv->P, mv->Wh, mv->Hc, mv->Wc, mv->m, 8,
                            mv->unmasked ? 1 : 0, mv->twHc, mv->Z, s));
   ZR_HIP(zr_launch_k2_rows(mv->Z, mv->pairs, mv->P, mv->Wh, mv->Wc, mv->m, mv->twWc, mv->den, (float)nc2,
                            mv->windows, s));
+  mv->timer.end(ZR_STAGE_PAIRS, t1, s);
   // peaks
+  hipEvent_t t2 = mv->timer.begin(s);
   ZR_HIP(hipMemsetAsync(mv->shifts, 0, sizeof(float) * 2 * mv->N * mv->N, s));
   ZR_HIP(hipMemsetAsync(mv->corr, 0, sizeof(float) * mv->N * mv->N, s));
   ZR_HIP(hipMemsetAsync(mv->peaksig, 0, sizeof(float) * mv->N * mv->N, s));
@@ -324,6 +374,7 @@ extern "C" int zr_xcorr(zr_movie* mv) {
   hp.n = mv->N;
   ZR_HIP(zr_launch_peaks(mv->windows, mv->pairs, mv->P, &hp, mv->gy, mv->gx, mv->ktab, mv->shifts, mv->corr,
                          mv->peaksig, mv->origin, s));
+  mv->timer.end(ZR_STAGE_PEAKS, t2, s);
   mv->have_xcorr = true;
   return ZR_OK;
 }
@@ -332,7 +383,9 @@ extern "C" int zr_solve(zr_movie* mv) {
   if (!mv) return fail(ZR_ERR_ARG, "NULL movie");
   if (!mv->have_xcorr) return fail(ZR_ERR_STATE, "zr_solve before zr_xcorr");
   ZR_HIP(hipSetDevice(mv->device));
+  hipEvent_t t0 = mv->timer.begin(mv->stream);
   int rc = zr_stage_solve(mv->st, mv->shifts, mv->corr, mv->peaksig, mv->stream);
+  mv->timer.end(ZR_STAGE_SOLVE, t0, mv->stream);
   return rc ? fail(rc, "%s", zr_stage_error()) : ZR_OK;
 }
 
@@ -340,21 +393,27 @@ extern "C" int zr_apply_shifts(zr_movie* mv, float* aligned_out) {
   if (!mv) return fail(ZR_ERR_ARG, "NULL movie");
   if (!mv->frames) return fail(ZR_ERR_STATE, "frames not set");
   ZR_HIP(hipSetDevice(mv->device));
+  hipEvent_t t0 = mv->timer.begin(mv->stream);
   int rc = zr_stage_shifts(mv->st, mv->frames, aligned_out, mv->stream);
+  mv->timer.end(ZR_STAGE_SHIFTS, t0, mv->stream);
   return rc ? fail(rc, "%s", zr_stage_error()) : ZR_OK;
 }
 
 extern "C" int zr_frc(zr_movie* mv) {
   if (!mv) return fail(ZR_ERR_ARG, "NULL movie");
   ZR_HIP(hipSetDevice(mv->device));
+  hipEvent_t t0 = mv->timer.begin(mv->stream);
   int rc = zr_stage_frc(mv->st, mv->stream);
+  mv->timer.end(ZR_STAGE_FRC, t0, mv->stream);
   return rc ? fail(rc, "%s", zr_stage_error()) : ZR_OK;
 }
 
 extern "C" int zr_dose_filter(zr_movie* mv) {
   if (!mv) return fail(ZR_ERR_ARG, "NULL movie");
   ZR_HIP(hipSetDevice(mv->device));
+  hipEvent_t t0 = mv->timer.begin(mv->stream);
   int rc = zr_stage_dose(mv->st, mv->frames, mv->stream);
+  mv->timer.end(ZR_STAGE_DOSE, t0, mv->stream);
   return rc ? fail(rc, "%s", zr_stage_error()) : ZR_OK;
 }
 
@@ -417,6 +476,14 @@ extern "C" int zr_get_solution(zr_movie* mv, double* translations, double* relat
   return rc ? fail(rc, "%s", zr_stage_error()) : ZR_OK;
 }
 
+extern "C" int zr_get_solver_info(zr_movie* mv, int* info6) {
+  if (!mv || !info6) return fail(ZR_ERR_ARG, "NULL argument");
+  ZR_HIP(hipSetDevice(mv->device));
+  ZR_HIP(hipStreamSynchronize(mv->stream));
+  int rc = zr_stage_solver_info(mv->st, info6);
+  return rc ? fail(rc, "%s", zr_stage_error()) : ZR_OK;
+}
+
 extern "C" int zr_get_sums(zr_movie* mv, float* image_sum, float* filt_sum, float* even_sum, float* odd_sum) {
   if (!mv) return fail(ZR_ERR_ARG, "NULL movie");
   ZR_HIP(hipSetDevice(mv->device));
@@ -438,4 +505,29 @@ extern "C" int zr_device_results(zr_movie* mv, const float** image_sum, const fl
   if (!mv) return fail(ZR_ERR_ARG, "NULL movie");
   int rc = zr_stage_device_results(mv->st, image_sum, filt_sum, translations);
   return rc ? fail(rc, "%s", zr_stage_error()) : ZR_OK;
+}
+
+extern "C" int zr_set_timing(zr_movie* mv, int enable) {
+  if (!mv) return fail(ZR_ERR_ARG, "NULL movie");
+  mv->timer.on = enable != 0;
+  return ZR_OK;
+}
+
+extern "C" int zr_get_stage_times(zr_movie* mv, double* ms, long* counts, int reset) {
+  if (!mv) return fail(ZR_ERR_ARG, "NULL movie");
+  ZR_HIP(hipSetDevice(mv->device));
+  mv->timer.harvest();
+  for (int i = 0; i < ZR_NSTAGES; ++i) {
+    if (ms) ms[i] = mv->timer.ms[i];
+    if (counts) counts[i] = mv->timer.count[i];
+    if (reset) { mv->timer.ms[i] = 0.0; mv->timer.count[i] = 0; }
+  }
+  return ZR_OK;
+}
+
+extern "C" int zr_movie_info(zr_movie* mv, int* dims) {
+  if (!mv || !dims) return fail(ZR_ERR_ARG, "NULL argument");
+  const int v[12] = {mv->N, mv->H, mv->W, mv->Hp, mv->Wp, mv->Hc, mv->Wc, mv->Wh, mv->m, mv->P, mv->unmasked ? 1 : 0, mv->batch};
+  for (int i = 0; i < 12; ++i) dims[i] = v[i];
+  return ZR_OK;
 }

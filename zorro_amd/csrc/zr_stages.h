@@ -21,5 +21,6 @@ int zr_stage_get_solution(zr_stage_state* s, double* translations, double* relat
                           double* logistic, double* error, double* error_stats, double* error_xy_frames);
 int zr_stage_get_sums(zr_stage_state* s, float* image_sum, float* filt_sum, float* even_sum, float* odd_sum);
 int zr_stage_get_frc(zr_stage_state* s, double* frc, int* n);
+int zr_stage_solver_info(zr_stage_state* s, int* out6);
 int zr_stage_device_results(zr_stage_state* s, const float** image_sum, const float** filt_sum,
                             const double** translations);

@@ -129,15 +129,21 @@ __global__ __launch_bounds__(256) void k_logi_grid(const float* __restrict__ hs,
   obj[i] = logi_obj(hs, cdf, info->nbins, thr, exp(lk), exp(ln));
 }
 
-// argmin of the grid (first index) then the 3x3x3 pattern search.  One block of 64.
-__global__ __launch_bounds__(64) void k_logi_refine(const float* __restrict__ hs, const float* __restrict__ cdf,
-                                                   const double* __restrict__ obj, SolveInfo* info) {
-  const int lane = threadIdx.x;
+// argmin of the grid (first index) then the 3x3x3 pattern search.  One block of
+// 256: every (candidate, bin) term is evaluated in parallel, each candidate's
+// objective is then summed over bins in index order.
+__global__ __launch_bounds__(256) void k_logi_refine(const float* __restrict__ hs, const float* __restrict__ cdf,
+                                                    const double* __restrict__ obj, SolveInfo* info) {
+  __shared__ double terms[27 * 290];
+  __shared__ double cval[27], ccand[27][3];
+  __shared__ double rv[4];
+  __shared__ int ri[4];
+  const int tid = threadIdx.x;
   if (info->nbins < 1) return;
   const int total = LOGI_G0 * LOGI_G1 * LOGI_G2;
   double bv = INFINITY;
   int bi = 0x7fffffff;
-  for (int i = lane; i < total; i += 64) {
+  for (int i = tid; i < total; i += 256) {
     double v = obj[i];
     if (v < bv || (v == bv && i < bi)) { bv = v; bi = i; }
   }
@@ -146,6 +152,11 @@ __global__ __launch_bounds__(64) void k_logi_refine(const float* __restrict__ hs
     int oi = __shfl_xor(bi, o, 64);
     if (ov < bv || (ov == bv && oi < bi)) { bv = ov; bi = oi; }
   }
+  if ((tid & 63) == 0) { rv[tid >> 6] = bv; ri[tid >> 6] = bi; }
+  __syncthreads();
+  bv = rv[0]; bi = ri[0];
+  for (int k = 1; k < 4; ++k)
+    if (rv[k] < bv || (rv[k] == bv && ri[k] < bi)) { bv = rv[k]; bi = ri[k]; }
   const double lo = info->logi[0], hi = info->logi[1];
   const double lk0 = log(LOGI_K0), lk1 = log(LOGI_K1), ln0 = log(LOGI_N0), ln1 = log(LOGI_N1);
   double cur[3];
@@ -161,32 +172,38 @@ __global__ __launch_bounds__(64) void k_logi_refine(const float* __restrict__ hs
   const int nb = info->nbins;
   int halvings = 0;
   for (int it = 0; it < LOGI_MAX_ITERS; ++it) {
-    double v = INFINITY;
-    double cand[3] = {0, 0, 0};
-    if (lane < 27) {
-      int o0 = lane / 9 - 1, o1 = (lane / 3) % 3 - 1, o2 = lane % 3 - 1;
-      cand[0] = fmin(fmax(cur[0] + o0 * step[0], lob[0]), hib[0]);
-      cand[1] = fmin(fmax(cur[1] + o1 * step[1], lob[1]), hib[1]);
-      cand[2] = fmin(fmax(cur[2] + o2 * step[2], lob[2]), hib[2]);
-      v = logi_obj(hs, cdf, nb, cand[0], exp(cand[1]), exp(cand[2]));
+    if (tid < 27) {
+      int o0 = tid / 9 - 1, o1 = (tid / 3) % 3 - 1, o2 = tid % 3 - 1;
+      ccand[tid][0] = fmin(fmax(cur[0] + o0 * step[0], lob[0]), hib[0]);
+      ccand[tid][1] = fmin(fmax(cur[1] + o1 * step[1], lob[1]), hib[1]);
+      ccand[tid][2] = fmin(fmax(cur[2] + o2 * step[2], lob[2]), hib[2]);
     }
-    double mv = v;
-    int mi = lane;
-    for (int o = 32; o > 0; o >>= 1) {
-      double ov = __shfl_xor(mv, o, 64);
-      int oi = __shfl_xor(mi, o, 64);
-      if (ov < mv || (ov == mv && oi < mi)) { mv = ov; mi = oi; }
+    __syncthreads();
+    for (int e = tid; e < 27 * nb; e += 256) {
+      int c = e / nb, i = e % nb;
+      terms[e] = fabs((double)cdf[i] - logistic_d((double)hs[i], ccand[c][0], exp(ccand[c][1]), exp(ccand[c][2])));
     }
+    __syncthreads();
+    if (tid < 27) {
+      double sum = 0.0;
+      for (int i = 0; i < nb; ++i) sum += terms[tid * nb + i];
+      cval[tid] = sum;
+    }
+    __syncthreads();
+    double mv = cval[0];
+    int mi = 0;
+    for (int c = 1; c < 27; ++c)
+      if (cval[c] < mv) { mv = cval[c]; mi = c; }
     if (mv < fcur) {
-      double c0 = __shfl(cand[0], mi, 64), c1 = __shfl(cand[1], mi, 64), c2 = __shfl(cand[2], mi, 64);
-      cur[0] = c0; cur[1] = c1; cur[2] = c2;
+      cur[0] = ccand[mi][0]; cur[1] = ccand[mi][1]; cur[2] = ccand[mi][2];
       fcur = mv;
     } else {
       step[0] *= 0.5; step[1] *= 0.5; step[2] *= 0.5;
       if (++halvings >= LOGI_ROUNDS) break;
     }
+    __syncthreads();
   }
-  if (lane == 0) {
+  if (tid == 0) {
     info->logi[0] = cur[0];
     info->logi[1] = exp(cur[1]);
     info->logi[2] = exp(cur[2]);
@@ -214,20 +231,27 @@ __global__ __launch_bounds__(64) void k_l1_solve(SolveArgs a) {
   const int lane = threadIdx.x;
   const int axis = blockIdx.x;            // 0: y (index 0), 1: x (index 1)
   const int N = a.N, M = a.M, n1 = N - 1;
-  __shared__ double Lm[MAXN * (MAXN + 1)];
+  const int LD = N + 1;
   __shared__ double g[MAXN], pv[MAXN];
   __shared__ int tree_e[MAXN], parent[MAXN], pedge[MAXN], order[MAXN];
   __shared__ unsigned long long sub[MAXN];
   __shared__ double slope_p[MAXN], slope_m[MAXN];
-  __shared__ int in_tree_flag[2048];
-  extern __shared__ double dyn[];         // b[M], w[M], r[M]
-  double* b = dyn;
+  // dynamic: Lm[N*LD], Bm[N*LD] (signed measured shifts per ordered pair),
+  // b[M], w[M], r[M], edges[M], in-tree flags[M]
+  extern __shared__ double dyn[];
+  double* Lm = dyn;
+  double* Bm = Lm + N * LD;
+  double* b = Bm + N * LD;
   double* w = b + M;
   double* r = w + M;
-  const int LD = MAXN + 1;
+  int2* es = reinterpret_cast<int2*>(r + M);
+  int* in_tree_flag = reinterpret_cast<int*>(es + M);
+  for (int i = lane; i < N * LD; i += 64) Bm[i] = 0.0;
+  for (int e = lane; e < M; e += 64) es[e] = a.edges[e];
+  __syncthreads();
   // ---- data
   for (int e = lane; e < M; e += 64) {
-    int2 ed = a.edges[e];
+    int2 ed = es[e];
     b[e] = (double)a.shifts[((size_t)ed.x * N + ed.y) * 2 + axis];
     double ps = (double)a.peaksig[ed.x * N + ed.y];
     double we;
@@ -238,6 +262,8 @@ __global__ __launch_bounds__(64) void k_l1_solve(SolveArgs a) {
       default: we = ps; break;
     }
     w[e] = we;
+    Bm[ed.x * LD + ed.y] = b[e];
+    Bm[ed.y * LD + ed.x] = -b[e];
   }
   __syncthreads();
   if (a.weight_mode == 4) {   // 'norm': (ps - min)/(max - min)
@@ -259,7 +285,7 @@ __global__ __launch_bounds__(64) void k_l1_solve(SolveArgs a) {
     for (int k = lane; k < N; k += 64) g[k] = 0.0;
     __syncthreads();
     for (int e = lane; e < M; e += 64) {
-      int2 ed = a.edges[e];
+      int2 ed = es[e];
       double re = pv[ed.y] - pv[ed.x] - b[e];
       double d = fmax(w[e], 1e-12) / fmax(fabs(re), eps);
       Lm[ed.x * LD + ed.y] = -d;   // one edge per ordered pair
@@ -267,15 +293,11 @@ __global__ __launch_bounds__(64) void k_l1_solve(SolveArgs a) {
     }
     __syncthreads();
     for (int k = lane; k < N; k += 64) {
+      // diagonal = -(row sum of off-diagonals); rhs g_k = sum_j L[k][j] * Bm[k][j]
       double s = 0.0, gg = 0.0;
-      for (int j = 0; j < N; ++j) s -= (j == k) ? 0.0 : Lm[k * LD + j];
+      for (int j = 0; j < N; ++j)
+        if (j != k) { s -= Lm[k * LD + j]; gg += Lm[k * LD + j] * Bm[k * LD + j]; }
       Lm[k * LD + k] = s;
-      // rhs: sum over edges incident to k of d_e * b_e with sign
-      for (int e = 0; e < M; ++e) {
-        int2 ed = a.edges[e];
-        if (ed.y == k) gg += -Lm[ed.x * LD + ed.y] * b[e];
-        else if (ed.x == k) gg -= -Lm[ed.x * LD + ed.y] * b[e];
-      }
       g[k] = gg;
     }
     __syncthreads();
@@ -316,10 +338,10 @@ __global__ __launch_bounds__(64) void k_l1_solve(SolveArgs a) {
     int P2 = 1;
     while (P2 < M) P2 <<= 1;
     double* key = Lm;                                   // reuse the matrix storage
-    int* val = reinterpret_cast<int*>(Lm + 2048);
+    int* val = reinterpret_cast<int*>(Lm + P2);
     for (int i = lane; i < P2; i += 64) {
       if (i < M) {
-        int2 ed = a.edges[i];
+        int2 ed = es[i];
         key[i] = fabs(pv[ed.y] - pv[ed.x] - b[i]);
         val[i] = i;
       } else {
@@ -349,7 +371,7 @@ __global__ __launch_bounds__(64) void k_l1_solve(SolveArgs a) {
       int used = 0;
       for (int t = 0; t < M && used < n1; ++t) {
         int e = val[t];
-        int2 ed = a.edges[e];
+        int2 ed = es[e];
         int rx = ed.x, ry = ed.y;
         while (comp[rx] != rx) rx = comp[rx];
         while (comp[ry] != ry) ry = comp[ry];
@@ -379,7 +401,7 @@ __global__ __launch_bounds__(64) void k_l1_solve(SolveArgs a) {
         int u = order[head++];
         for (int t = 0; t < n1; ++t) {
           int e = tree_e[t];
-          int2 ed = a.edges[e];
+          int2 ed = es[e];
           int v = -1;
           double pvv = 0.0;
           if (ed.x == u && parent[ed.y] == -2) { v = ed.y; pvv = pv[u] + b[e]; }
@@ -396,7 +418,7 @@ __global__ __launch_bounds__(64) void k_l1_solve(SolveArgs a) {
     }
     __syncthreads();
     for (int e = lane; e < M; e += 64) {
-      int2 ed = a.edges[e];
+      int2 ed = es[e];
       r[e] = pv[ed.y] - pv[ed.x] - b[e];
       if (in_tree_flag[e]) r[e] = 0.0;
     }
@@ -407,7 +429,7 @@ __global__ __launch_bounds__(64) void k_l1_solve(SolveArgs a) {
       const unsigned long long S = sub[v];
       for (int e = lane; e < M; e += 64) {
         if (in_tree_flag[e]) continue;
-        int2 ed = a.edges[e];
+        int2 ed = es[e];
         bool iin = (S >> ed.x) & 1ull, jin = (S >> ed.y) & 1ull;
         if (iin == jin) continue;
         double dir = jin ? 1.0 : -1.0;
@@ -450,7 +472,7 @@ __global__ __launch_bounds__(64) void k_l1_solve(SolveArgs a) {
       int me = 0x7fffffff;
       for (int e = lane; e < M; e += 64) {
         if (in_tree_flag[e]) continue;
-        int2 ed = a.edges[e];
+        int2 ed = es[e];
         bool iin = (S >> ed.x) & 1ull, jin = (S >> ed.y) & 1ull;
         if (iin == jin) continue;
         double dir = (jin ? 1.0 : -1.0) * bs;
@@ -490,7 +512,7 @@ __global__ __launch_bounds__(64) void k_l1_solve(SolveArgs a) {
   }
   for (int k = lane; k < N; k += 64) a.pos[axis * N + k] = pv[k];
   for (int e = lane; e < M; e += 64) {
-    int2 ed = a.edges[e];
+    int2 ed = es[e];
     a.resid[e * 2 + axis] = pv[ed.y] - pv[ed.x] - b[e];
   }
 }
@@ -538,17 +560,22 @@ __global__ __launch_bounds__(64) void k_solution_post(PostArgs a) {
     }
     cen[0] = a.centroid ? s0 / N : 0.0;
     cen[1] = a.centroid ? s1 / N : 0.0;
+  }
+  {
     // error statistics (weights clipped to >= 1e-6)
     double sw = 0, swe = 0, swe2 = 0;
-    for (int e = 0; e < M; ++e) {
+    for (int e = lane; e < M; e += 64) {
       double w = fmax(a.weights[e], 1e-6);
       double ey = a.resid[2 * e], ex = a.resid[2 * e + 1];
       double en = sqrt(ey * ey + ex * ex);
       sw += w; swe += w * en; swe2 += w * en * en;
     }
-    double mean = swe / sw;
-    a.err_stats[0] = mean;
-    a.err_stats[1] = sqrt(swe2 / sw - mean * mean);
+    sw = wave_sum_d(sw); swe = wave_sum_d(swe); swe2 = wave_sum_d(swe2);
+    if (lane == 0) {
+      double mean = swe / sw;
+      a.err_stats[0] = mean;
+      a.err_stats[1] = sqrt(swe2 / sw - mean * mean);
+    }
   }
   __syncthreads();
   for (int k = lane; k < N; k += 64) {
@@ -636,22 +663,45 @@ ZR_DEV int sym_idx(int i, int n) {
   return i < n ? i : p - 1 - i;
 }
 
-// aligned frame value at (y, x): roll by (iy, ix) then 3x3 reflect convolution
-// out[y,x] = sum_ab K[a][b] g[refl(y+1-a), refl(x+1-b)],  g[i,j] = f[(i-iy) mod H, (j-ix) mod W]
-ZR_DEV float aligned_px(const float* __restrict__ f, const FrameShift& s, int y, int x, int H, int W) {
-  if (!s.conv) return f[(size_t)wrapi(y - s.iy, H) * W + wrapi(x - s.ix, W)];
+// Aligned frame = roll by (iy, ix) then 3x3 convolution with reflect edges:
+//   out[y,x] = sum_ab K[a][b] g[refl(y+1-a), refl(x+1-b)],  g[i,j] = f[(i-iy) mod H, (j-ix) mod W]
+// RowSrc holds the three source rows of one output row (computed once per row).
+struct RowSrc {
+  const float* r[3];
+};
+// wrap v into [0, n) for v in (-n, 2n)
+ZR_DEV int wrap1(int v, int n) {
+  v += (v < 0) ? n : 0;
+  return v - ((v >= n) ? n : 0);
+}
+ZR_DEV RowSrc row_sources(const float* __restrict__ f, const FrameShift& s, int y, int H, int W) {
+  RowSrc rs;
+  const int iy = s.iy;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const int yy = s.conv ? wrap1(refl1(y + 1 - a, H) - iy, H) : wrap1(y - iy, H);
+    rs.r[a] = f + (size_t)yy * W;
+  }
+  return rs;
+}
+ZR_DEV float aligned_px_row(const RowSrc& rs, const FrameShift& s, int x, int W) {
+  if (!s.conv) return rs.r[0][wrap1(x - s.ix, W)];
+  const int xm = wrap1(refl1(x + 1, W) - s.ix, W);
+  const int x0 = wrap1(x - s.ix, W);
+  const int xp = wrap1(refl1(x - 1, W) - s.ix, W);
   float acc = 0.f;
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
-    const int yy = wrapi(refl1(y + 1 - a, H) - s.iy, H);
-    const float* row = f + (size_t)yy * W;
-#pragma unroll
-    for (int b = 0; b < 3; ++b) {
-      const int xx = wrapi(refl1(x + 1 - b, W) - s.ix, W);
-      acc += s.k[a * 3 + b] * row[xx];
-    }
+    const float* row = rs.r[a];
+    acc += s.k[a * 3 + 0] * row[xm] + s.k[a * 3 + 1] * row[x0] + s.k[a * 3 + 2] * row[xp];
   }
   return acc;
+}
+// integer shifts reduced once per frame so |shift| < size (the wrap helpers need it)
+ZR_DEV FrameShift reduce_shift(FrameShift s, int H, int W) {
+  s.iy %= H;
+  s.ix %= W;
+  return s;
 }
 
 __global__ __launch_bounds__(256) void k_shift_sum(const float* __restrict__ frames, const FrameShift* __restrict__ fs,
@@ -665,8 +715,9 @@ __global__ __launch_bounds__(256) void k_shift_sum(const float* __restrict__ fra
   if (y < H && x < W) {
     float se = 0.f, so = 0.f, st = 0.f;
     for (int J = 0; J < N; ++J) {
-      const FrameShift s = fs[J];
-      float v = aligned_px(frames + (size_t)J * H * W, s, y, x, H, W);
+      const FrameShift s = reduce_shift(fs[J], H, W);
+      const RowSrc rs = row_sources(frames + (size_t)J * H * W, s, y, H, W);
+      float v = aligned_px_row(rs, s, x, W);
       st += v;
       if (J & 1) so += v; else se += v;
       if (aligned) aligned[((size_t)J * H + y) * W + x] = v;
@@ -829,13 +880,13 @@ __global__ __launch_bounds__(1024) void k_dose_rows(const float* __restrict__ fr
   float st[PX], se[PX], so[PX];
   for (int i = 0; i < PX; ++i) { st[i] = 0.f; se[i] = 0.f; so[i] = 0.f; }
   for (int J = 0; J < N; ++J) {
-    const FrameShift s = fs[J];
-    const float* f = frames + (size_t)J * H * W;
+    const FrameShift s = reduce_shift(fs[J], H, W);
+    const RowSrc rs = row_sources(frames + (size_t)J * H * W, s, ysrc, H, W);
     float* br = reinterpret_cast<float*>(buf[grp]);
     for (int i = 0; i < PX; ++i) {
       int X = gtid + i * NTG;
       if (X < WP) {
-        const float v = aligned_px(f, s, ysrc, sym_idx(X, W), H, W);
+        const float v = aligned_px_row(rs, s, X < W ? X : sym_idx(X, W), W);
         br[X] = v;
         if (own && X < W) {
           st[i] += v;
@@ -889,28 +940,50 @@ struct DoseArgs {
   SolveInfo* info;
 };
 
-// dose per frame from mean(imageSum)/N, accumulated dose, per-frame cutoff radius
-__global__ __launch_bounds__(64) void k_dose_params(DoseArgs a) {
-  if (threadIdx.x != 0) return;
+// dose per frame from mean(imageSum)/N, accumulated dose, per-frame cutoff
+// radius: the last radial index whose optimal exposure is closer to the dose at
+// the end of the frame than at its start.  One block of 256 threads.
+__global__ __launch_bounds__(256) void k_dose_params(DoseArgs a) {
+  __shared__ double opti[2048];
+  __shared__ double red[4];
+  __shared__ int redi[4];
+  __shared__ double s_dpf;
+  const int tid = threadIdx.x;
   double s = 0.0;
-  for (int i = 0; i < a.nrows; ++i) s += a.rowsum[i];
-  const double dpf = s / ((double)a.H * a.W) / a.N;
-  const double missing = dpf * a.missing;
-  a.info->dpf = dpf;
-  a.accum[0] = missing;
-  double run = 0.0;
-  for (int j = 1; j <= a.N; ++j) { run += dpf; a.accum[j] = run + missing; }
+  for (int i = tid; i < a.nrows; i += 256) s += a.rowsum[i];
+  s = wave_sum_d(s);
+  if ((tid & 63) == 0) red[tid >> 6] = s;
+  __syncthreads();
+  if (tid == 0) {
+    const double tot = red[0] + red[1] + red[2] + red[3];
+    const double dpf = tot / ((double)a.H * a.W) / a.N;
+    const double missing = dpf * a.missing;
+    a.info->dpf = dpf;
+    a.accum[0] = missing;
+    double run = 0.0;
+    for (int j = 1; j <= a.N; ++j) { run += dpf; a.accum[j] = run + missing; }
+    s_dpf = dpf;
+  }
   const int nq = a.Hp / 2;
   const double qs = sqrt(a.ipx * a.ipy);
+  for (int k = tid; k < nq; k += 256)
+    opti[k] = 2.51284 * a.vs * (a.A * pow(k * qs, a.B) + a.C);
+  __syncthreads();
+  if (tid == 0) opti[0] = opti[1] * opti[1];
+  __syncthreads();
   for (int j = 0; j < a.N; ++j) {
     const double d1 = a.accum[j + 1], d0 = a.accum[j];
     int last = -1;
-    const double o1 = 2.51284 * a.vs * (a.A * pow(1.0 * qs, a.B) + a.C);
-    for (int k = 0; k < nq; ++k) {
-      double opt = (k == 0) ? o1 * o1 : 2.51284 * a.vs * (a.A * pow(k * qs, a.B) + a.C);
-      if (fabs(d1 - opt) < fabs(d0 - opt)) last = k;
+    for (int k = tid; k < nq; k += 256)
+      if (fabs(d1 - opti[k]) < fabs(d0 - opti[k])) last = k;
+    for (int o = 32; o > 0; o >>= 1) last = max(last, __shfl_xor(last, o, 64));
+    if ((tid & 63) == 0) redi[tid >> 6] = last;
+    __syncthreads();
+    if (tid == 0) {
+      int l = max(max(redi[0], redi[1]), max(redi[2], redi[3]));
+      a.thresq[j] = l >= 0 ? l * qs : NAN;
     }
-    a.thresq[j] = last >= 0 ? last * qs : NAN;
+    __syncthreads();
   }
 }
 
@@ -1142,7 +1215,7 @@ int zr_stage_solve(zr_stage_state* s, const float* shifts, const float* corr, co
     const int total = LOGI_G0 * LOGI_G1 * LOGI_G2;
     hipLaunchKernelGGL(k_logi_grid, dim3((total + 255) / 256), dim3(256), 0, st, s->hs, s->cdf, s->info, s->obj);
     ZS_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_logi_refine, dim3(1), dim3(64), 0, st, s->hs, s->cdf, s->obj, s->info);
+    hipLaunchKernelGGL(k_logi_refine, dim3(1), dim3(256), 0, st, s->hs, s->cdf, s->obj, s->info);
     ZS_HIP(hipGetLastError());
   }
   SolveArgs a;
@@ -1151,11 +1224,11 @@ int zr_stage_solve(zr_stage_state* s, const float* shifts, const float* corr, co
   a.logi_fixed[0] = s->p.peaksig_thres; a.logi_fixed[1] = s->p.logistic_k; a.logi_fixed[2] = s->p.logistic_nu;
   a.bound = 0;
   a.weights = s->weights; a.resid = s->resid; a.pos = s->pos; a.info = s->info;
-  size_t dyn = sizeof(double) * 3 * M;
-  static bool attr = false;
-  if (!attr) {
-    hipFuncSetAttribute((const void*)k_l1_solve, hipFuncAttributeMaxDynamicSharedMemorySize, (int)(sizeof(double) * 3 * 2016));
-    attr = true;
+  size_t dyn = sizeof(double) * (2 * (size_t)N * (N + 1) + 3 * (size_t)M) + sizeof(int2) * M + sizeof(int) * M;
+  static int attr = 0;
+  if ((int)dyn > attr) {
+    hipFuncSetAttribute((const void*)k_l1_solve, hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn);
+    attr = (int)dyn;
   }
   hipLaunchKernelGGL(k_l1_solve, dim3(2), dim3(64), dyn, st, a);
   ZS_HIP(hipGetLastError());
@@ -1235,7 +1308,7 @@ int zr_stage_dose(zr_stage_state* s, const float* frames, hipStream_t st) {
   a.ipx = 1.0 / (Wp * (s->p.pixelsize_nm * 10.0));
   a.ipy = 1.0 / (Hp * (s->p.pixelsize_nm * 10.0));
   a.rowsum = s->drow; a.nrows = nblk; a.accum = s->accum; a.thresq = s->thresq; a.info = s->info;
-  hipLaunchKernelGGL(k_dose_params, dim3(1), dim3(64), 0, st, a);
+  hipLaunchKernelGGL(k_dose_params, dim3(1), dim3(256), 0, st, a);
   ZS_HIP(hipGetLastError());
   switch (Hp) {
 #define ZR_DC(X) case X: hipLaunchKernelGGL(k_dose_cols<X>, dim3(Wp / 2 + 1), dim3(256), 0, st, s->Tdose, a, s->twHp, s->G); break;
@@ -1274,6 +1347,15 @@ int zr_stage_get_solution(zr_stage_state* s, double* translations, double* relat
   if (error) ZS_HIP(hipMemcpy(error, s->resid, sizeof(double) * 2 * M, hipMemcpyDeviceToHost));
   if (error_stats) ZS_HIP(hipMemcpy(error_stats, s->err_stats, sizeof(double) * 2, hipMemcpyDeviceToHost));
   if (error_xy_frames) ZS_HIP(hipMemcpy(error_xy_frames, s->err_frames, sizeof(double) * 2 * N, hipMemcpyDeviceToHost));
+  return ZR_OK;
+}
+
+int zr_stage_solver_info(zr_stage_state* s, int* out6) {
+  if (!s || !s->have_solution) return serr(ZR_ERR_STATE, "no solution");
+  SolveInfo inf;
+  ZS_HIP(hipMemcpy(&inf, s->info, sizeof(inf), hipMemcpyDeviceToHost));
+  const int v[6] = {inf.nps, inf.nbins, inf.status_y, inf.status_x, inf.pivots_y, inf.pivots_x};
+  for (int i = 0; i < 6; ++i) out6[i] = v[i];
   return ZR_OK;
 }
 

@@ -134,6 +134,22 @@ class MoviePlan:
     def synchronize(self):
         L.check(self.lib.zr_synchronize(self._mv))
 
+    # ---- timing / info
+    def set_timing(self, on: bool = True):
+        L.check(self.lib.zr_set_timing(self._mv, 1 if on else 0))
+
+    def stage_times(self, reset: bool = False):
+        ms = np.zeros(len(L.STAGES))
+        cnt = (C.c_long * len(L.STAGES))()
+        L.check(self.lib.zr_get_stage_times(self._mv, L.dptr(ms), cnt, 1 if reset else 0))
+        return {k: (ms[i], cnt[i]) for i, k in enumerate(L.STAGES)}
+
+    def info(self):
+        d = (C.c_int * 12)()
+        L.check(self.lib.zr_movie_info(self._mv, d))
+        keys = ["N", "H", "W", "Hp", "Wp", "Hc", "Wc", "Wh", "maxShift", "pairs", "unmasked", "batch"]
+        return dict(zip(keys, list(d)))
+
     # ---- results
     def trimats(self):
         n = self.n
@@ -164,11 +180,23 @@ class MoviePlan:
         return dict(translations=t, relativeEst=rel, weights=w, logistic=lg, errorXY=err,
                     mean_errorNorm=es[0], std_errorNorm=es[1], errorX=exy[:, 1], errorY=exy[:, 0])
 
+    def solver_info(self):
+        d = (C.c_int * 6)()
+        L.check(self.lib.zr_get_solver_info(self._mv, d))
+        return dict(zip(["n_peaksig", "bins", "status_y", "status_x", "pivots_y", "pivots_x"], list(d)))
+
     def sums(self):
+        """imageSum, evenSum, oddSum and (when the dose filter ran) filtSum."""
         h, w = self.h, self.w
         out = [np.zeros((h, w), np.float32) for _ in range(4)]
-        L.check(self.lib.zr_get_sums(self._mv, *(L.fptr(a) for a in out)))
-        return dict(imageSum=out[0], filtSum=out[1], evenSum=out[2], oddSum=out[3])
+        ptrs = [L.fptr(a) for a in out]
+        if not self.params.do_dose:
+            ptrs[1] = None
+        L.check(self.lib.zr_get_sums(self._mv, *ptrs))
+        res = dict(imageSum=out[0], evenSum=out[2], oddSum=out[3])
+        if self.params.do_dose:
+            res["filtSum"] = out[1]
+        return res
 
     def frc_curve(self):
         n = C.c_int()
