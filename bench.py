@@ -54,6 +54,8 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="run a plain 1 GiB device copy first (reference for rocprofv3 FETCH/WRITE counters)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -73,6 +75,13 @@ def main():
     frames, drift = make_movie_torch(c["n"], c["h"], c["w"], seed=c["seed"], dose=c["dose"], jump=c["jump"],
                                      device=dev)
     torch.cuda.synchronize()
+    if args.copy_calibration:
+        src = torch.empty(1 << 28, dtype=torch.float32, device=dev).fill_(1.0)
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     plan = MoviePlan(make_params(**PARAMS), c["n"], c["h"], c["w"])
     plan.set_frames(frames)
     info = plan.info()

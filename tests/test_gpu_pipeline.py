@@ -21,6 +21,8 @@ CASES = {
     "512sq_12f": dict(n=12, h=512, w=512, seed=3, dose=1.5, box=256),
     "C1_1024sq_20f": dict(n=20, h=1024, w=1024, seed=1, dose=1.5, box=512),
     "1024sq_10f_nodose": dict(n=10, h=1024, w=1024, seed=9, dose=1.5, box=512, filter=None),
+    # non-square frames take the direct dose path (per-frame row FFTs of the aligned frames)
+    "512x1024_8f_direct_dose": dict(n=8, h=512, w=1024, seed=13, dose=1.5, box=256),
 }
 
 

@@ -133,6 +133,10 @@ struct zr_movie {
   zr_params p{};
   int N = 0, H = 0, W = 0, Hp = 0, Wp = 0, Hc = 0, Wc = 0, Wh = 0, m = 0, Lper = 0;
   bool crop = false, unmasked = true;
+  // spectra layout: Wk kept columns, Hs stored rows per column, row offset of the
+  // cropped rows >= Hc/2, index of the +Hc/2 row inside a stored column (-1: Sside)
+  bool spectral = false;
+  int Wk = 0, Hs = 0, hoff = 0, side_in_col = -1;
   int batch = 1, rb = 4;
   // frames
   const float* frames = nullptr;
@@ -293,8 +297,13 @@ extern "C" int zr_movie_create(int device, const zr_params* p, int n_frames, int
   mv->batch = (int)std::min<size_t>(N, 8);
   mv->rb = (mv->Wp >= 4096) ? 4 : 8;
   const int nrb = mv->Hp / mv->rb;
-  if ((rc = dev_alloc(&mv->T, (size_t)mv->batch * mv->Wh * mv->Hp)) ||
-      (rc = dev_alloc(&mv->S, N * mv->Wh * mv->Hc)) || (rc = dev_alloc(&mv->Sside, N * mv->Wh)) ||
+  mv->spectral = p->do_dose && mv->unmasked && mv->H == mv->W;
+  mv->Wk = mv->spectral ? mv->Wp / 2 + 1 : mv->Wh;
+  mv->Hs = mv->spectral ? mv->Hp : mv->Hc;
+  mv->hoff = mv->spectral ? mv->Hp - mv->Hc : 0;
+  mv->side_in_col = mv->spectral ? mv->Hc / 2 : -1;
+  if ((rc = dev_alloc(&mv->T, (size_t)mv->batch * mv->Wk * mv->Hp)) ||
+      (rc = dev_alloc(&mv->S, N * mv->Wk * mv->Hs)) || (rc = dev_alloc(&mv->Sside, mv->spectral ? 0 : N * mv->Wk)) ||
       (rc = dev_alloc(&mv->Z, (size_t)mv->P * mv->Wh * 2 * mv->m)) || (rc = dev_alloc(&mv->part, N * nrb * 2)) ||
       (rc = dev_alloc(&mv->stats, N * 2)) || (rc = dev_alloc(&mv->den, (size_t)mv->P)) ||
       (rc = dev_alloc(&mv->windows, (size_t)mv->P * 4 * mv->m * mv->m)) || (rc = dev_alloc(&mv->shifts, N * N * 2)) ||
@@ -302,7 +311,7 @@ extern "C" int zr_movie_create(int device, const zr_params* p, int n_frames, int
     free_movie(mv);
     return rc;
   }
-  if ((rc = zr_stage_create(&mv->st, mv->p, mv->N, mv->H, mv->W, mv->Hp, mv->Wp, mv->pairs_host))) {
+  if ((rc = zr_stage_create(&mv->st, mv->p, mv->N, mv->H, mv->W, mv->Hp, mv->Wp, mv->pairs_host, mv->spectral ? 1 : 0))) {
     free_movie(mv);
     return fail(rc, "%s", zr_stage_error());
   }
@@ -341,9 +350,9 @@ extern "C" int zr_xcorr(zr_movie* mv) {
   // K1: spectra in batches of frames
   for (int f0 = 0; f0 < mv->N; f0 += mv->batch) {
     int nb = std::min(mv->batch, mv->N - f0);
-    ZR_HIP(zr_launch_k1_rows(mv->Wp, mv->frames, mv->H, mv->W, mv->Hp, mv->Wh, f0, nb, mv->twWp2, mv->twWp, mv->T,
+    ZR_HIP(zr_launch_k1_rows(mv->Wp, mv->frames, mv->H, mv->W, mv->Hp, mv->Wk, f0, nb, mv->twWp2, mv->twWp, mv->T,
                              mv->part, &rb, s));
-    ZR_HIP(zr_launch_k1_cols(mv->Hp, mv->T, mv->Wh, mv->Hc, f0, nb, mv->twHp, mv->S, mv->Sside, s));
+    ZR_HIP(zr_launch_k1_cols(mv->Hp, mv->T, mv->Wk, mv->Hc, f0, nb, mv->twHp, mv->S, mv->Sside, mv->spectral ? 1 : 0, s));
   }
   ZR_HIP(zr_launch_k1_stats(mv->part, mv->Hp / rb, mv->N, mv->stats, s));
   const double nc2 = 1.0 / std::pow((double)mv->Hc * (double)mv->Wc, 2.0);
@@ -352,7 +361,7 @@ extern "C" int zr_xcorr(zr_movie* mv) {
   // K2
   hipEvent_t t1 = mv->timer.begin(s);
   ZR_HIP(zr_launch_k2_cols(mv->Lper, mv->S, mv->Sside, mv->work, mv->P, mv->Wh, mv->Hc, mv->Wc, mv->m, 8,
-                           mv->unmasked ? 1 : 0, mv->twHc, mv->Z, s));
+                           mv->unmasked ? 1 : 0, mv->twHc, mv->Z, mv->Wk, mv->Hs, mv->hoff, mv->side_in_col, s));
   ZR_HIP(zr_launch_k2_rows(mv->Z, mv->pairs, mv->P, mv->Wh, mv->Wc, mv->m, mv->twWc, mv->den, (float)nc2,
                            mv->windows, s));
   mv->timer.end(ZR_STAGE_PAIRS, t1, s);
@@ -412,7 +421,8 @@ extern "C" int zr_dose_filter(zr_movie* mv) {
   if (!mv) return fail(ZR_ERR_ARG, "NULL movie");
   ZR_HIP(hipSetDevice(mv->device));
   hipEvent_t t0 = mv->timer.begin(mv->stream);
-  int rc = zr_stage_dose(mv->st, mv->frames, mv->stream);
+  if (mv->spectral && !mv->have_xcorr) return fail(ZR_ERR_STATE, "dose filter needs the spectra of zr_xcorr");
+  int rc = zr_stage_dose(mv->st, mv->frames, mv->S, mv->Wk, mv->stream);
   mv->timer.end(ZR_STAGE_DOSE, t0, mv->stream);
   return rc ? fail(rc, "%s", zr_stage_error()) : ZR_OK;
 }

@@ -13,11 +13,11 @@ hipError_t zr_launch_k1_rows(int Wp, const float* frames, int H, int W, int Hp, 
                              const float2* twL, const float2* tw2L, float2* T, double* part, int* rb_out,
                              hipStream_t st);
 hipError_t zr_launch_k1_cols(int Hp, const float2* T, int Wh, int Hc, int frame0, int nb, const float2* twH,
-                             float2* S, float2* Sside, hipStream_t st);
+                             float2* S, float2* Sside, int full, hipStream_t st);
 hipError_t zr_launch_k1_stats(const double* part, int nrb, int n, double* stats, hipStream_t st);
 hipError_t zr_launch_k2_cols(int Lper, const float2* S, const float2* Sside, const int4* work, int npairs, int Wh,
                              int Hc, int Wc, int m, int cols_per_wave, int dc_zero, const float2* twHc, float2* Z,
-                             hipStream_t st);
+                             int Wk, int Hs, int hoff, int side_in_col, hipStream_t st);
 hipError_t zr_launch_k2_rows(const float2* Z, const int4* pairs, int npairs, int Wh, int Wc, int m,
                              const float2* twWc, const double* den, float nc2, float* windows, hipStream_t st);
 hipError_t zr_launch_k2_den(const int4* pairs, int npairs, const double* stats, double npix, double nc2, double* den,

@@ -11,12 +11,14 @@ struct zr_stage_state;
 
 const char* zr_stage_error();
 int zr_stage_create(zr_stage_state** out, const zr_params& p, int N, int H, int W, int Hp, int Wp,
-                    const std::vector<int4>& pairs);
+                    const std::vector<int4>& pairs, int spectral);
 void zr_stage_free(zr_stage_state* s);
 int zr_stage_solve(zr_stage_state* s, const float* shifts, const float* corr, const float* peaksig, hipStream_t st);
 int zr_stage_shifts(zr_stage_state* s, const float* frames, float* aligned_out, hipStream_t st);
 int zr_stage_frc(zr_stage_state* s, hipStream_t st);
-int zr_stage_dose(zr_stage_state* s, const float* frames, hipStream_t st);
+// S: kx-major full-resolution half spectra [N][Wk][Hp] (used when the state was
+// created with spectral != 0; may be NULL otherwise).
+int zr_stage_dose(zr_stage_state* s, const float* frames, const float2* S, int Wk, hipStream_t st);
 int zr_stage_get_solution(zr_stage_state* s, double* translations, double* relative_est, double* weights,
                           double* logistic, double* error, double* error_stats, double* error_xy_frames);
 int zr_stage_get_sums(zr_stage_state* s, float* image_sum, float* filt_sum, float* even_sum, float* odd_sum);

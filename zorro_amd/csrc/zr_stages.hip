@@ -29,7 +29,8 @@ constexpr int MAX_PIVOTS = 400;
 
 struct FrameShift {
   int iy, ix, conv, pad;
-  float k[9];
+  float k[9];        // 3x3 Lanczos kernel, k[a][b] = u[a] v[b]
+  float u[3], v[3];  // its separable factors (rows, columns)
 };
 
 struct SolveInfo {           // device-resident scalars of the solve
@@ -226,16 +227,27 @@ struct SolveArgs {
   SolveInfo* info;
 };
 
-// One wave per axis.  Solves min sum_e w_e |p_J - p_I - b_e| over positions.
-__global__ __launch_bounds__(64) void k_l1_solve(SolveArgs a) {
-  const int lane = threadIdx.x;
+// One block (256 threads) per axis.  Solves min sum_e w_e |p_J - p_I - b_e| over
+// frame positions p (p_0 = 0):
+//   1. IRLS on the weighted position Laplacian (Cholesky, node 0 removed);
+//   2. Kruskal spanning tree over |residual| ascending (ties: edge index);
+//   3. network-simplex descent: move a subtree while that lowers the objective,
+//      entering the non-tree edge at the breakpoint; stop at the optimal vertex.
+// Positions are always recomputed by summing measured shifts along the tree,
+// so the returned vertex is exact.
+constexpr int L1_NT = 256;
+__global__ __launch_bounds__(L1_NT) void k_l1_solve(SolveArgs a) {
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
   const int axis = blockIdx.x;            // 0: y (index 0), 1: x (index 1)
   const int N = a.N, M = a.M, n1 = N - 1;
   const int LD = N + 1;
   __shared__ double g[MAXN], pv[MAXN];
-  __shared__ int tree_e[MAXN], parent[MAXN], pedge[MAXN], order[MAXN];
+  __shared__ int tree_e[MAXN], parent[MAXN], pedge[MAXN], deg[MAXN], adj[MAXN][MAXN];
+  __shared__ int changed;
   __shared__ unsigned long long sub[MAXN];
   __shared__ double slope_p[MAXN], slope_m[MAXN];
+  __shared__ double red_d[4];
+  __shared__ int red_i[4];
   // dynamic: Lm[N*LD], Bm[N*LD] (signed measured shifts per ordered pair),
   // b[M], w[M], r[M], edges[M], in-tree flags[M]
   extern __shared__ double dyn[];
@@ -246,11 +258,11 @@ __global__ __launch_bounds__(64) void k_l1_solve(SolveArgs a) {
   double* r = w + M;
   int2* es = reinterpret_cast<int2*>(r + M);
   int* in_tree_flag = reinterpret_cast<int*>(es + M);
-  for (int i = lane; i < N * LD; i += 64) Bm[i] = 0.0;
-  for (int e = lane; e < M; e += 64) es[e] = a.edges[e];
+  for (int i = tid; i < N * LD; i += L1_NT) Bm[i] = 0.0;
+  for (int e = tid; e < M; e += L1_NT) es[e] = a.edges[e];
   __syncthreads();
   // ---- data
-  for (int e = lane; e < M; e += 64) {
+  for (int e = tid; e < M; e += L1_NT) {
     int2 ed = es[e];
     b[e] = (double)a.shifts[((size_t)ed.x * N + ed.y) * 2 + axis];
     double ps = (double)a.peaksig[ed.x * N + ed.y];
@@ -268,23 +280,26 @@ __global__ __launch_bounds__(64) void k_l1_solve(SolveArgs a) {
   __syncthreads();
   if (a.weight_mode == 4) {   // 'norm': (ps - min)/(max - min)
     double mn = INFINITY, mx = -INFINITY;
-    for (int e = lane; e < M; e += 64) { mn = fmin(mn, w[e]); mx = fmax(mx, w[e]); }
+    for (int e = tid; e < M; e += L1_NT) { mn = fmin(mn, w[e]); mx = fmax(mx, w[e]); }
     for (int o = 32; o > 0; o >>= 1) { mn = fmin(mn, __shfl_xor(mn, o, 64)); mx = fmax(mx, __shfl_xor(mx, o, 64)); }
+    if (lane == 0) { red_d[wid] = mn; slope_p[wid] = mx; }
     __syncthreads();
-    for (int e = lane; e < M; e += 64) w[e] = (w[e] - mn) / (mx - mn);
+    mn = fmin(fmin(red_d[0], red_d[1]), fmin(red_d[2], red_d[3]));
+    mx = fmax(fmax(slope_p[0], slope_p[1]), fmax(slope_p[2], slope_p[3]));
+    __syncthreads();
+    for (int e = tid; e < M; e += L1_NT) w[e] = (w[e] - mn) / (mx - mn);
     __syncthreads();
   }
   if (axis == 0)   // errorDict['weights'] holds the weights clipped to >= 1e-6
-    for (int e = lane; e < M; e += 64) a.weights[e] = fmax(w[e], 1e-6);
-  for (int k = lane; k < N; k += 64) pv[k] = 0.0;
+    for (int e = tid; e < M; e += L1_NT) a.weights[e] = fmax(w[e], 1e-6);
+  for (int k = tid; k < N; k += L1_NT) pv[k] = 0.0;
   __syncthreads();
   // ---- IRLS warm start on the position Laplacian (node 0 fixed)
   double eps = 1.0;
   for (int it = 0; it < IRLS_ITERS; ++it) {
-    for (int i = lane; i < N * LD; i += 64) Lm[i] = 0.0;
-    for (int k = lane; k < N; k += 64) g[k] = 0.0;
+    for (int i = tid; i < N * LD; i += L1_NT) Lm[i] = 0.0;
     __syncthreads();
-    for (int e = lane; e < M; e += 64) {
+    for (int e = tid; e < M; e += L1_NT) {
       int2 ed = es[e];
       double re = pv[ed.y] - pv[ed.x] - b[e];
       double d = fmax(w[e], 1e-12) / fmax(fabs(re), eps);
@@ -292,43 +307,47 @@ __global__ __launch_bounds__(64) void k_l1_solve(SolveArgs a) {
       Lm[ed.y * LD + ed.x] = -d;
     }
     __syncthreads();
-    for (int k = lane; k < N; k += 64) {
+    for (int k = tid; k < N; k += L1_NT) {
       // diagonal = -(row sum of off-diagonals); rhs g_k = sum_j L[k][j] * Bm[k][j]
-      double s = 0.0, gg = 0.0;
+      double sd = 0.0, gg = 0.0;
       for (int j = 0; j < N; ++j)
-        if (j != k) { s -= Lm[k * LD + j]; gg += Lm[k * LD + j] * Bm[k * LD + j]; }
-      Lm[k * LD + k] = s;
+        if (j != k) { sd -= Lm[k * LD + j]; gg += Lm[k * LD + j] * Bm[k * LD + j]; }
+      Lm[k * LD + k] = sd;
       g[k] = gg;
     }
     __syncthreads();
-    // Cholesky of rows/cols 1..N-1 in place (lower), forward/back substitution
+    // Cholesky (lower) of rows/cols 1..N-1, trailing updates spread over all threads
     for (int k = 1; k < N; ++k) {
-      if (lane == 0) {
-        double dkk = sqrt(fmax(Lm[k * LD + k], 1e-300));
-        Lm[k * LD + k] = dkk;
-      }
+      const double dkk = sqrt(fmax(Lm[k * LD + k], 1e-300));
       __syncthreads();
-      double dkk = Lm[k * LD + k];
-      for (int i = k + 1 + lane; i < N; i += 64) Lm[i * LD + k] /= dkk;
+      if (tid == 0) Lm[k * LD + k] = dkk;
+      for (int i = k + 1 + tid; i < N; i += L1_NT) Lm[i * LD + k] /= dkk;
       __syncthreads();
-      for (int i = k + 1 + lane; i < N; i += 64) {
-        double lik = Lm[i * LD + k];
-        for (int j = k + 1; j <= i; ++j) Lm[i * LD + j] -= lik * Lm[j * LD + k];
+      const int nt = N - k - 1;
+      for (int q = tid; q < nt * nt; q += L1_NT) {
+        int i = k + 1 + q / nt, j = k + 1 + q % nt;
+        if (j <= i) Lm[i * LD + j] -= Lm[i * LD + k] * Lm[j * LD + k];
       }
       __syncthreads();
     }
-    if (lane == 0) {
-      for (int i = 1; i < N; ++i) {       // L y = g
-        double s = g[i];
-        for (int j = 1; j < i; ++j) s -= Lm[i * LD + j] * g[j];
-        g[i] = s / Lm[i * LD + i];
+    // triangular solves in wave 0: lane j keeps y_j / p_j in a register
+    if (wid == 0) {
+      double yv = 0.0;
+      for (int i = 1; i < N; ++i) {            // L y = g
+        double t = (lane >= 1 && lane < i) ? Lm[i * LD + lane] * yv : 0.0;
+        t = wave_sum_d(t);
+        double yi = (g[i] - t) / Lm[i * LD + i];
+        if (lane == i) yv = yi;
       }
-      for (int i = N - 1; i >= 1; --i) {  // L^T p = y
-        double s = g[i];
-        for (int j = i + 1; j < N; ++j) s -= Lm[j * LD + i] * pv[j];
-        pv[i] = s / Lm[i * LD + i];
+      double pvv = 0.0;
+      for (int i = N - 1; i >= 1; --i) {       // L^T p = y
+        double t = (lane > i && lane < N) ? Lm[lane * LD + i] * pvv : 0.0;
+        t = wave_sum_d(t);
+        double yi = __shfl(yv, i, 64);
+        double pi = (yi - t) / Lm[i * LD + i];
+        if (lane == i) pvv = pi;
       }
-      pv[0] = 0.0;
+      if (lane < N) pv[lane] = (lane == 0) ? 0.0 : pvv;
     }
     __syncthreads();
     eps = fmax(eps * 0.5, 1e-9);
@@ -339,7 +358,7 @@ __global__ __launch_bounds__(64) void k_l1_solve(SolveArgs a) {
     while (P2 < M) P2 <<= 1;
     double* key = Lm;                                   // reuse the matrix storage
     int* val = reinterpret_cast<int*>(Lm + P2);
-    for (int i = lane; i < P2; i += 64) {
+    for (int i = tid; i < P2; i += L1_NT) {
       if (i < M) {
         int2 ed = es[i];
         key[i] = fabs(pv[ed.y] - pv[ed.x] - b[i]);
@@ -349,10 +368,11 @@ __global__ __launch_bounds__(64) void k_l1_solve(SolveArgs a) {
         val[i] = 0x7fffffff;
       }
     }
+    for (int e = tid; e < M; e += L1_NT) in_tree_flag[e] = 0;
     __syncthreads();
     for (int kk = 2; kk <= P2; kk <<= 1)
       for (int jj = kk >> 1; jj > 0; jj >>= 1) {
-        for (int i = lane; i < P2; i += 64) {
+        for (int i = tid; i < P2; i += L1_NT) {
           int l = i ^ jj;
           if (l > i) {
             bool up = (i & kk) == 0;
@@ -364,10 +384,9 @@ __global__ __launch_bounds__(64) void k_l1_solve(SolveArgs a) {
         }
         __syncthreads();
       }
-    if (lane == 0) {
+    if (tid == 0) {
       int comp[MAXN];
       for (int k = 0; k < N; ++k) comp[k] = k;
-      for (int e = 0; e < M; ++e) in_tree_flag[e] = 0;
       int used = 0;
       for (int t = 0; t < M && used < n1; ++t) {
         int e = val[t];
@@ -389,42 +408,60 @@ __global__ __launch_bounds__(64) void k_l1_solve(SolveArgs a) {
   // ---- network-simplex descent on spanning trees
   int piv = 0;
   bool optimal = false;
+  double wsum = 0.0;
+  for (int e = tid; e < M; e += L1_NT) wsum += w[e];
+  wsum = wave_sum_d(wsum);
+  if (lane == 0) red_d[wid] = wsum;
+  __syncthreads();
+  wsum = red_d[0] + red_d[1] + red_d[2] + red_d[3];
+  const double tol = -1e-12 * (1.0 + wsum);
   for (; piv <= MAX_PIVOTS; ++piv) {
-    if (lane == 0) {
-      // parents by BFS from node 0; exact positions by summing measured shifts
-      for (int k = 0; k < N; ++k) { parent[k] = -2; sub[k] = 0ull; }
-      parent[0] = -1;
-      pv[0] = 0.0;
-      int head = 0, tail = 0;
-      order[tail++] = 0;
-      while (head < tail) {
-        int u = order[head++];
-        for (int t = 0; t < n1; ++t) {
-          int e = tree_e[t];
+    // tree adjacency, then level-synchronous BFS from node 0 (exact positions)
+    for (int k = tid; k < N; k += L1_NT) { deg[k] = 0; parent[k] = -2; sub[k] = 0ull; }
+    __syncthreads();
+    for (int t = tid; t < n1; t += L1_NT) {
+      int e = tree_e[t];
+      int2 ed = es[e];
+      int i0 = atomicAdd(&deg[ed.x], 1);
+      adj[ed.x][i0] = t;
+      int i1 = atomicAdd(&deg[ed.y], 1);
+      adj[ed.y][i1] = t;
+    }
+    if (tid == 0) { parent[0] = -1; pv[0] = 0.0; }
+    __syncthreads();
+    for (int round = 0; round < N; ++round) {
+      if (tid == 0) changed = 0;
+      __syncthreads();
+      int np = -2, ne = -1;
+      double npv = 0.0;
+      if (tid < N && parent[tid] == -2) {
+        for (int q = 0; q < deg[tid]; ++q) {
+          int e = tree_e[adj[tid][q]];
           int2 ed = es[e];
-          int v = -1;
-          double pvv = 0.0;
-          if (ed.x == u && parent[ed.y] == -2) { v = ed.y; pvv = pv[u] + b[e]; }
-          else if (ed.y == u && parent[ed.x] == -2) { v = ed.x; pvv = pv[u] - b[e]; }
-          if (v >= 0) { parent[v] = u; pedge[v] = e; pv[v] = pvv; order[tail++] = v; }
+          int u = (ed.x == tid) ? ed.y : ed.x;
+          if (parent[u] != -2) { np = u; ne = e; npv = (ed.x == u) ? pv[u] + b[e] : pv[u] - b[e]; break; }
         }
       }
-      // subtree masks in reverse BFS order
-      for (int i = tail - 1; i >= 0; --i) {
-        int v = order[i];
-        sub[v] |= (1ull << v);
-        if (parent[v] >= 0) sub[parent[v]] |= sub[v];
-      }
+      __syncthreads();
+      if (np != -2) { parent[tid] = np; pedge[tid] = ne; pv[tid] = npv; changed = 1; }
+      __syncthreads();
+      if (!changed) break;
+    }
+    // subtree masks: every node marks itself in all its ancestors
+    if (tid < N) {
+      unsigned long long bit = 1ull << tid;
+      int v = tid;
+      int guard = 0;
+      while (v >= 0 && guard++ <= N) { atomicOr(&sub[v], bit); v = parent[v]; }
     }
     __syncthreads();
-    for (int e = lane; e < M; e += 64) {
+    for (int e = tid; e < M; e += L1_NT) {
       int2 ed = es[e];
-      r[e] = pv[ed.y] - pv[ed.x] - b[e];
-      if (in_tree_flag[e]) r[e] = 0.0;
+      r[e] = in_tree_flag[e] ? 0.0 : pv[ed.y] - pv[ed.x] - b[e];
     }
     __syncthreads();
-    // directional slopes for moving each subtree by +-delta
-    for (int v = 1; v < N; ++v) {
+    // directional slopes for moving each subtree by +-delta (waves split the nodes)
+    for (int v = 1 + wid; v < N; v += 4) {
       double sp = 0.0, sm = 0.0;
       const unsigned long long S = sub[v];
       for (int e = lane; e < M; e += 64) {
@@ -441,7 +478,8 @@ __global__ __launch_bounds__(64) void k_l1_solve(SolveArgs a) {
           sm -= w[e] * dir * sg;
         }
       }
-      for (int o = 32; o > 0; o >>= 1) { sp += __shfl_xor(sp, o, 64); sm += __shfl_xor(sm, o, 64); }
+      sp = wave_sum_d(sp);
+      sm = wave_sum_d(sm);
       if (lane == 0) {
         double wt = w[pedge[v]];
         slope_p[v] = sp + wt;
@@ -449,11 +487,8 @@ __global__ __launch_bounds__(64) void k_l1_solve(SolveArgs a) {
       }
     }
     __syncthreads();
-    // most negative slope
-    double wsum = 0.0;
-    for (int e = lane; e < M; e += 64) wsum += w[e];
-    for (int o = 32; o > 0; o >>= 1) wsum += __shfl_xor(wsum, o, 64);
-    double best = -1e-12 * (1.0 + wsum);
+    // most negative slope (scan in node order; ties keep the first)
+    double best = tol;
     int bv = -1, bs = 0;
     for (int v = 1; v < N; ++v) {
       if (slope_p[v] < best) { best = slope_p[v]; bv = v; bs = 1; }
@@ -461,42 +496,46 @@ __global__ __launch_bounds__(64) void k_l1_solve(SolveArgs a) {
     }
     if (bv < 0) { optimal = true; break; }
     if (piv == MAX_PIVOTS) break;
-    // line search: breakpoints of crossing non-tree edges
-    const unsigned long long S = sub[bv];
-    double slope = best;
-    double last = 0.0;
-    int last_e = -1;
-    int enter = -1;
-    while (slope < 0.0) {
-      double md = INFINITY;
-      int me = 0x7fffffff;
-      for (int e = lane; e < M; e += 64) {
-        if (in_tree_flag[e]) continue;
-        int2 ed = es[e];
-        bool iin = (S >> ed.x) & 1ull, jin = (S >> ed.y) & 1ull;
-        if (iin == jin) continue;
-        double dir = (jin ? 1.0 : -1.0) * bs;
-        double re = r[e];
-        if (re == 0.0 || (re > 0) == (dir > 0)) continue;   // residual grows: no breakpoint
-        double d = -re / dir;
-        // strictly after the last breakpoint (ties ordered by edge index)
-        if (d < last || (d == last && e <= last_e)) continue;
-        if (d < md || (d == md && e < me)) { md = d; me = e; }
+    // line search over breakpoints of crossing non-tree edges (wave 0)
+    __shared__ int s_enter;
+    if (wid == 0) {
+      const unsigned long long S = sub[bv];
+      double slope = best;
+      double last = 0.0;
+      int last_e = -1;
+      int enter = -1;
+      while (slope < 0.0) {
+        double md = INFINITY;
+        int me = 0x7fffffff;
+        for (int e = lane; e < M; e += 64) {
+          if (in_tree_flag[e]) continue;
+          int2 ed = es[e];
+          bool iin = (S >> ed.x) & 1ull, jin = (S >> ed.y) & 1ull;
+          if (iin == jin) continue;
+          double dir = (jin ? 1.0 : -1.0) * bs;
+          double re = r[e];
+          if (re == 0.0 || (re > 0) == (dir > 0)) continue;   // residual grows: no breakpoint
+          double d = -re / dir;
+          if (d < last || (d == last && e <= last_e)) continue;
+          if (d < md || (d == md && e < me)) { md = d; me = e; }
+        }
+        for (int o = 32; o > 0; o >>= 1) {
+          double od = __shfl_xor(md, o, 64);
+          int oe = __shfl_xor(me, o, 64);
+          if (od < md || (od == md && oe < me)) { md = od; me = oe; }
+        }
+        if (me == 0x7fffffff) break;
+        slope += 2.0 * w[me];
+        last = md;
+        last_e = me;
+        enter = me;
       }
-      for (int o = 32; o > 0; o >>= 1) {
-        double od = __shfl_xor(md, o, 64);
-        int oe = __shfl_xor(me, o, 64);
-        if (od < md || (od == md && oe < me)) { md = od; me = oe; }
-      }
-      if (me == 0x7fffffff) break;
-      slope += 2.0 * w[me];
-      last = md;
-      last_e = me;
-      enter = me;
+      if (lane == 0) s_enter = enter;
     }
-    if (enter < 0) break;   // cannot improve (numerically flat)
     __syncthreads();
-    if (lane == 0) {
+    const int enter = s_enter;
+    if (enter < 0) break;   // cannot improve (numerically flat)
+    if (tid == 0) {
       int leave = pedge[bv];
       for (int t = 0; t < n1; ++t)
         if (tree_e[t] == leave) { tree_e[t] = enter; break; }
@@ -506,12 +545,12 @@ __global__ __launch_bounds__(64) void k_l1_solve(SolveArgs a) {
     __syncthreads();
   }
   __syncthreads();
-  if (lane == 0) {
+  if (tid == 0) {
     if (axis == 0) { a.info->pivots_y = piv; if (a.info->status_y == 0) a.info->status_y = optimal ? 1 : 2; }
     else { a.info->pivots_x = piv; if (a.info->status_x == 0) a.info->status_x = optimal ? 1 : 2; }
   }
-  for (int k = lane; k < N; k += 64) a.pos[axis * N + k] = pv[k];
-  for (int e = lane; e < M; e += 64) {
+  for (int k = tid; k < N; k += L1_NT) a.pos[axis * N + k] = pv[k];
+  for (int e = tid; e < M; e += L1_NT) {
     int2 ed = es[e];
     a.resid[e * 2 + axis] = pv[ed.y] - pv[ed.x] - b[e];
   }
@@ -600,8 +639,17 @@ __global__ __launch_bounds__(64) void k_solution_post(PostArgs a) {
           s += kk[i * 3 + j];
         }
       for (int i = 0; i < 9; ++i) f.k[i] = (float)(kk[i] / s);
+      double su = 0.0, sv = 0.0, uu[3], vv[3];
+      for (int i = 0; i < 3; ++i) {
+        uu[i] = sinc_d(yr[i] * 2.0) * sinc_d(yr[i]);
+        vv[i] = sinc_d(xr[i] * 2.0) * sinc_d(xr[i]);
+        su += uu[i];
+        sv += vv[i];
+      }
+      for (int i = 0; i < 3; ++i) { f.u[i] = (float)(uu[i] / su); f.v[i] = (float)(vv[i] / sv); }
     } else {
       for (int i = 0; i < 9; ++i) f.k[i] = 0.f;
+      for (int i = 0; i < 3; ++i) { f.u[i] = 0.f; f.v[i] = 0.f; }
     }
     a.fs[k] = f;
     // per-frame error estimates with the reference's (J, J-1) indexing
@@ -1069,6 +1117,160 @@ __global__ __launch_bounds__(256) void k_dose_out(const float2* __restrict__ G, 
   }
 }
 
+// ---------------------------------------------------------------- KD (spectral)
+// For unpadded frames the dose filter is applied to spectra built from the
+// forward spectra F_J the correlation already computed:
+//   FFT(aligned_J)(k) = F_J(k) * Py_J(ky) * Px_J(kx) + Ebar_J(k)
+// Py/Px carry the circular roll (phase ramp) and the separable 3x3 Lanczos
+// kernel as a circular convolution; Ebar_J is the DFT of the difference between
+// the reference's reflect boundaries and circular ones, which is non-zero only
+// on the four border lines of the frame:
+//   Ebar(ky,kx) = R0(kx) + RH(kx) e^{-2 pi i ky (H-1)/H} + C0(ky) + CW(ky) e^{-2 pi i kx (W-1)/W}
+// (R0/RH: rows 0 and H-1, C0/CW: columns 0 and W-1 without the corners).
+
+// Border difference value at border pixel (y, x).
+ZR_DEV float border_diff(const float* __restrict__ f, const FrameShift& s, int y, int x, int H, int W) {
+  float acc = 0.f;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const int yr = y + 1 - a;
+    const int yref = wrap1(refl1(yr, H) - s.iy, H), ycir = wrap1(wrap1(yr, H) - s.iy, H);
+#pragma unroll
+    for (int b = 0; b < 3; ++b) {
+      const int xr = x + 1 - b;
+      const int xref = wrap1(refl1(xr, W) - s.ix, W), xcir = wrap1(wrap1(xr, W) - s.ix, W);
+      acc += s.k[a * 3 + b] * (f[(size_t)yref * W + xref] - f[(size_t)ycir * W + xcir]);
+    }
+  }
+  return acc;
+}
+
+// One block per (frame, border line): line 0 = row 0, 1 = row H-1 (half spectra
+// over kx), 2 = column 0, 3 = column W-1 (corners excluded; all ky).  Blocks of
+// lines 0 and 2 also write the frame's Px and Py factors.
+template <int LEN>
+__global__ __launch_bounds__(256) void k_dose_border(const float* __restrict__ frames, const FrameShift* __restrict__ fs,
+                                                    int H, int W, const float2* __restrict__ tw,
+                                                    const float2* __restrict__ twH, const float2* __restrict__ twW,
+                                                    float2* __restrict__ R0, float2* __restrict__ RH,
+                                                    float2* __restrict__ C0, float2* __restrict__ CW,
+                                                    float2* __restrict__ Py, float2* __restrict__ Px) {
+  constexpr int NT = 256;
+  __shared__ float2 buf[LEN];
+  const int J = blockIdx.x / 4, line = blockIdx.x % 4;
+  const FrameShift s = reduce_shift(fs[J], H, W);
+  const float* f = frames + (size_t)J * H * W;
+  const int tid = threadIdx.x;
+  const int Whalf = W / 2 + 1;
+  if (line < 2) {   // rows: Px on line 0
+    const int y = line == 0 ? 0 : H - 1;
+    for (int x = tid; x < W; x += NT) buf[x] = make_float2(s.conv ? border_diff(f, s, y, x, H, W) : 0.f, 0.f);
+    __syncthreads();
+    wg_fft<LEN, -1, NT>(buf, tw, tid);
+    float2* dst = (line == 0 ? R0 : RH) + (size_t)J * Whalf;
+    for (int k = tid; k < Whalf; k += NT) dst[k] = buf[k];
+    if (line == 0) {
+      for (int k = tid; k < Whalf; k += NT) {
+        // e^{-2 pi i k ix / W} * sum_b v_b e^{-2 pi i k (b-1)/W}
+        const int ph = (int)(((long)k * ((s.ix % W + W) % W)) % W);
+        float2 ramp = twW[ph];
+        float2 kh = make_float2(1.f, 0.f);
+        if (s.conv) {
+          float2 e1 = twW[k % W];                 // e^{-2 pi i k / W}
+          float2 em = make_float2(e1.x, -e1.y);   // e^{+2 pi i k / W}
+          kh = make_float2(s.v[0] * em.x + s.v[1] + s.v[2] * e1.x, s.v[0] * em.y + s.v[2] * e1.y);
+        }
+        Px[(size_t)J * Whalf + k] = c_mul(ramp, kh);
+      }
+    }
+  } else {          // columns: Py on line 2
+    const int x = line == 2 ? 0 : W - 1;
+    for (int y = tid; y < H; y += NT) {
+      float v = 0.f;
+      if (s.conv && y > 0 && y < H - 1) v = border_diff(f, s, y, x, H, W);
+      buf[y] = make_float2(v, 0.f);
+    }
+    __syncthreads();
+    wg_fft<LEN, -1, NT>(buf, tw, tid);
+    float2* dst = (line == 2 ? C0 : CW) + (size_t)J * H;
+    for (int k = tid; k < H; k += NT) dst[k] = buf[k];
+    if (line == 2) {
+      for (int k = tid; k < H; k += NT) {
+        const int ph = (int)(((long)k * ((s.iy % H + H) % H)) % H);
+        float2 ramp = twH[ph];
+        float2 kh = make_float2(1.f, 0.f);
+        if (s.conv) {
+          float2 e1 = twH[k % H];
+          float2 em = make_float2(e1.x, -e1.y);
+          kh = make_float2(s.u[0] * em.x + s.u[1] + s.u[2] * e1.x, s.u[0] * em.y + s.u[2] * e1.y);
+        }
+        Py[(size_t)J * H + k] = c_mul(ramp, kh);
+      }
+    }
+  }
+}
+
+// Per kx column: accumulate the filtered aligned spectra of all frames, then the
+// inverse transform along y.  S is the full-resolution kx-major half spectrum.
+template <int HP>
+__global__ __launch_bounds__(256) void k_dose_cols_spec(const float2* __restrict__ S, int Wk, DoseArgs a,
+                                                       const float2* __restrict__ R0, const float2* __restrict__ RH,
+                                                       const float2* __restrict__ C0, const float2* __restrict__ CW,
+                                                       const float2* __restrict__ Py, const float2* __restrict__ Px,
+                                                       const float2* __restrict__ twH, const float2* __restrict__ twW,
+                                                       float2* __restrict__ G) {
+  constexpr int NT = 256, PER = HP / NT;
+  __shared__ float2 buf[HP];
+  const int tid = threadIdx.x;
+  const int kx = xcd_remap(blockIdx.x, gridDim.x);
+  const int Whalf = a.Wp / 2 + 1;
+  float invc[PER], q[PER];
+  float2 acc[PER], ey[PER];
+  for (int i = 0; i < PER; ++i) {
+    const int ky = tid + i * NT;
+    const double fy = ky < HP / 2 ? ky : ky - HP;
+    const double fx = kx;
+    auto crit = [&](double yy, double xx) {
+      double qq = sqrt(xx * xx * a.ipx * a.ipx + yy * yy * a.ipy * a.ipy);
+      return a.vs * (a.A * pow(qq, a.B) + a.C);
+    };
+    double qq = sqrt(fx * fx * a.ipx * a.ipx + fy * fy * a.ipy * a.ipy);
+    double c = crit(fy, fx);
+    if (ky == HP / 2 && 2 * kx == a.Wp) {   // Nyquist corner takes its left neighbour squared
+      double cl = crit(-HP / 2, a.Wp / 2 - 1);
+      c = cl * cl;
+    }
+    invc[i] = (float)(1.0 / c);
+    q[i] = (float)qq;
+    acc[i] = make_float2(0.f, 0.f);
+    ey[i] = twH[(int)(((long)ky * (a.H - 1)) % a.H)];          // e^{-2 pi i ky (H-1)/H}
+  }
+  const float2 ex = twW[(int)(((long)kx * (a.W - 1)) % a.W)];  // e^{-2 pi i kx (W-1)/W}
+  const float ord = (float)a.order;
+  for (int J = 0; J < a.N; ++J) {
+    const float2* col = S + ((size_t)J * Wk + kx) * HP;
+    const float2 px = Px[(size_t)J * Whalf + kx];
+    const float2 r0 = R0[(size_t)J * Whalf + kx], rh = RH[(size_t)J * Whalf + kx];
+    const float mhd = (float)(-0.5 * a.accum[J + 1]);
+    const float tq = (float)a.thresq[J];
+    const float2 rhx = rh;
+    for (int i = 0; i < PER; ++i) {
+      const int ky = tid + i * NT;
+      float2 F = col[ky];
+      float2 sp = c_mul(c_mul(F, Py[(size_t)J * a.H + ky]), px);
+      sp = c_add(sp, c_add(r0, c_mul(rhx, ey[i])));
+      sp = c_add(sp, c_add(C0[(size_t)J * a.H + ky], c_mul(CW[(size_t)J * a.H + ky], ex)));
+      const float gval = expf(-powf(q[i] / tq, ord)) * expf(mhd * invc[i]);
+      acc[i].x += sp.x * gval;
+      acc[i].y += sp.y * gval;
+    }
+  }
+  for (int i = 0; i < PER; ++i) buf[tid + i * NT] = acc[i];
+  __syncthreads();
+  wg_fft<HP, 1, NT>(buf, twH, tid);
+  for (int y = tid; y < a.H; y += NT) G[(size_t)kx * a.H + y] = buf[y];
+}
+
 // ==================================================================== host
 namespace {
 thread_local std::string s_err;
@@ -1123,6 +1325,8 @@ struct zr_stage_state {
   // dose
   float2 *twHp = nullptr, *twWp = nullptr, *twWp2 = nullptr, *Tdose = nullptr, *G = nullptr;
   double *accum = nullptr, *thresq = nullptr, *drow = nullptr;
+  bool spectral = false;       // dose filter from the forward spectra (unpadded frames)
+  float2 *R0 = nullptr, *RH = nullptr, *C0 = nullptr, *CW = nullptr, *Py = nullptr, *Px = nullptr;
 };
 
 const char* zr_stage_error() { return s_err.c_str(); }
@@ -1132,14 +1336,14 @@ void zr_stage_free(zr_stage_state* s) {
   void* ptrs[] = {s->edges, s->hs, s->cdf, s->obj, s->info, s->weights, s->resid, s->pos, s->trans_raw, s->trans,
                   s->rel, s->err_stats, s->err_frames, s->fs, s->img_sum, s->even, s->odd, s->filt, s->rowsum,
                   s->twB, s->twB2, s->Tfrc, s->frc2d, s->frc, s->twHp, s->twWp, s->twWp2, s->Tdose, s->G,
-                  s->accum, s->thresq, s->drow};
+                  s->accum, s->thresq, s->drow, s->R0, s->RH, s->C0, s->CW, s->Py, s->Px};
   for (void* q : ptrs)
     if (q) hipFree(q);
   delete s;
 }
 
 int zr_stage_create(zr_stage_state** out, const zr_params& p, int N, int H, int W, int Hp, int Wp,
-                    const std::vector<int4>& pairs) {
+                    const std::vector<int4>& pairs, int spectral) {
   *out = nullptr;
   if (N > MAXN) return serr(ZR_ERR_UNSUPPORTED, "the solver handles up to %d frames (got %d)", MAXN, N);
   zr_stage_state* s = new zr_stage_state();
@@ -1192,7 +1396,18 @@ int zr_stage_create(zr_stage_state** out, const zr_params& p, int N, int H, int 
     ZC(upload(&s->twHp, fwd_table(Hp)));
     ZC(upload(&s->twWp, fwd_table(Wp)));
     ZC(upload(&s->twWp2, fwd_table(Wp / 2)));
-    ZC(hipMalloc((void**)&s->Tdose, sizeof(float2) * (size_t)N * (Wp / 2 + 1) * Hp));
+    s->spectral = spectral != 0;
+    if (s->spectral) {
+      const size_t wh = (size_t)N * (Wp / 2 + 1), hh = (size_t)N * Hp;
+      ZC(hipMalloc((void**)&s->R0, sizeof(float2) * wh));
+      ZC(hipMalloc((void**)&s->RH, sizeof(float2) * wh));
+      ZC(hipMalloc((void**)&s->Px, sizeof(float2) * wh));
+      ZC(hipMalloc((void**)&s->C0, sizeof(float2) * hh));
+      ZC(hipMalloc((void**)&s->CW, sizeof(float2) * hh));
+      ZC(hipMalloc((void**)&s->Py, sizeof(float2) * hh));
+    } else {
+      ZC(hipMalloc((void**)&s->Tdose, sizeof(float2) * (size_t)N * (Wp / 2 + 1) * Hp));
+    }
     ZC(hipMalloc((void**)&s->G, sizeof(float2) * (size_t)(Wp / 2 + 1) * H));
     ZC(hipMalloc((void**)&s->accum, sizeof(double) * (N + 1)));
     ZC(hipMalloc((void**)&s->thresq, sizeof(double) * N));
@@ -1230,7 +1445,7 @@ int zr_stage_solve(zr_stage_state* s, const float* shifts, const float* corr, co
     hipFuncSetAttribute((const void*)k_l1_solve, hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn);
     attr = (int)dyn;
   }
-  hipLaunchKernelGGL(k_l1_solve, dim3(2), dim3(64), dyn, st, a);
+  hipLaunchKernelGGL(k_l1_solve, dim3(2), dim3(L1_NT), dyn, st, a);
   ZS_HIP(hipGetLastError());
   PostArgs q;
   q.pos = s->pos; q.weights = s->weights; q.resid = s->resid; q.edges = s->edges;
@@ -1281,9 +1496,57 @@ int zr_stage_frc(zr_stage_state* s, hipStream_t st) {
   return ZR_OK;
 }
 
-int zr_stage_dose(zr_stage_state* s, const float* frames, hipStream_t st) {
+static void fill_dose_args(zr_stage_state* s, DoseArgs& a, const double* rowsum, int nrows) {
+  a.N = s->N; a.H = s->H; a.W = s->W; a.Hp = s->Hp; a.Wp = s->Wp;
+  a.A = s->p.dose_a; a.B = s->p.dose_b; a.C = s->p.dose_c; a.order = s->p.dose_order; a.missing = s->p.dose_missing;
+  a.vs = std::sqrt(s->p.voltage_kv / 300.0);
+  a.ipx = 1.0 / (s->Wp * (s->p.pixelsize_nm * 10.0));
+  a.ipy = 1.0 / (s->Hp * (s->p.pixelsize_nm * 10.0));
+  a.rowsum = rowsum; a.nrows = nrows; a.accum = s->accum; a.thresq = s->thresq; a.info = s->info;
+}
+
+static int dose_spectral(zr_stage_state* s, const float* frames, const float2* S, int Wk, hipStream_t st) {
+  const int H = s->H, W = s->W;
+  // aligned sums (imageSum, even, odd) and their total for the dose per frame
+  dim3 grid((W + 63) / 64, (H + 3) / 4);
+  hipLaunchKernelGGL(k_shift_sum, grid, dim3(256), 0, st, frames, s->fs, s->N, H, W, s->img_sum, s->even, s->odd,
+                     (float*)nullptr, s->rowsum);
+  ZS_HIP(hipGetLastError());
+  s->have_sums = true;
+  DoseArgs a;
+  fill_dose_args(s, a, s->rowsum, (int)(grid.x * grid.y));
+  hipLaunchKernelGGL(k_dose_params, dim3(1), dim3(256), 0, st, a);
+  ZS_HIP(hipGetLastError());
+  if (H != W) return serr(ZR_ERR_UNSUPPORTED, "spectral dose path needs square frames");
+  switch (H) {
+#define ZR_DB(X) case X: hipLaunchKernelGGL(k_dose_border<X>, dim3(s->N * 4), dim3(256), 0, st, frames, s->fs, H, W, s->twHp, s->twHp, s->twWp, s->R0, s->RH, s->C0, s->CW, s->Py, s->Px); break;
+    ZR_DB(256) ZR_DB(512) ZR_DB(1024) ZR_DB(2048) ZR_DB(3072) ZR_DB(3840) ZR_DB(4096)
+#undef ZR_DB
+    default: return serr(ZR_ERR_UNSUPPORTED, "dose size %d", H);
+  }
+  ZS_HIP(hipGetLastError());
+  switch (s->Hp) {
+#define ZR_DS(X) case X: hipLaunchKernelGGL(k_dose_cols_spec<X>, dim3(s->Wp / 2 + 1), dim3(256), 0, st, S, Wk, a, s->R0, s->RH, s->C0, s->CW, s->Py, s->Px, s->twHp, s->twWp, s->G); break;
+    ZR_DS(256) ZR_DS(512) ZR_DS(1024) ZR_DS(2048) ZR_DS(3072) ZR_DS(3840) ZR_DS(4096)
+#undef ZR_DS
+    default: return serr(ZR_ERR_UNSUPPORTED, "dose height %d", s->Hp);
+  }
+  ZS_HIP(hipGetLastError());
+  switch (s->Wp) {
+#define ZR_DO(X) case X: hipLaunchKernelGGL(k_dose_out<X>, dim3(s->H), dim3(256), 0, st, s->G, s->H, s->W, s->twWp2, s->twWp, s->filt); break;
+    ZR_DO(256) ZR_DO(512) ZR_DO(1024) ZR_DO(2048) ZR_DO(3072) ZR_DO(3840) ZR_DO(4096)
+#undef ZR_DO
+    default: return serr(ZR_ERR_UNSUPPORTED, "dose width %d", s->Wp);
+  }
+  ZS_HIP(hipGetLastError());
+  s->have_dose = true;
+  return ZR_OK;
+}
+
+int zr_stage_dose(zr_stage_state* s, const float* frames, const float2* S, int Wk, hipStream_t st) {
   if (!s || !s->have_solution) return serr(ZR_ERR_STATE, "dose filter before solve");
   if (!s->p.do_dose) return serr(ZR_ERR_STATE, "dose filter disabled in the parameters");
+  if (s->spectral) return dose_spectral(s, frames, S, Wk, st);
   const int Hp = s->Hp, Wp = s->Wp;
   const int nblk = Hp / 4;
   size_t sh = (size_t)4 * (Wp / 2) * sizeof(float2);
@@ -1302,12 +1565,7 @@ int zr_stage_dose(zr_stage_state* s, const float* frames, hipStream_t st) {
   ZS_HIP(hipGetLastError());
   s->have_sums = true;
   DoseArgs a;
-  a.N = s->N; a.H = s->H; a.W = s->W; a.Hp = Hp; a.Wp = Wp;
-  a.A = s->p.dose_a; a.B = s->p.dose_b; a.C = s->p.dose_c; a.order = s->p.dose_order; a.missing = s->p.dose_missing;
-  a.vs = std::sqrt(s->p.voltage_kv / 300.0);
-  a.ipx = 1.0 / (Wp * (s->p.pixelsize_nm * 10.0));
-  a.ipy = 1.0 / (Hp * (s->p.pixelsize_nm * 10.0));
-  a.rowsum = s->drow; a.nrows = nblk; a.accum = s->accum; a.thresq = s->thresq; a.info = s->info;
+  fill_dose_args(s, a, s->drow, nblk);
   hipLaunchKernelGGL(k_dose_params, dim3(1), dim3(256), 0, st, a);
   ZS_HIP(hipGetLastError());
   switch (Hp) {

@@ -7,10 +7,12 @@
 //
 // Layouts (all row-major, complex64 as float2):
 //   frames  f32 [N][H][W]
-//   T       c64 [B][Wh][Hp]     row-pass output, kx-major (Wh = Wc/2+1)
-//   S       c64 [N][Wh][Hc]     cropped half-spectrum, kx-major; row ky' in FFT
-//                               order of the cropped grid (ky' >= Hc/2 -> ky'-Hc)
-//   Sside   c64 [N][Wh]         original row +Hc/2 (absent from the cropped grid)
+//   T       c64 [B][Wk][Hp]     row-pass output, kx-major (Wk kept columns)
+//   S       c64 [N][Wk][Hs]     kx-major half-spectrum.  Either cropped (Hs = Hc,
+//                               rows in FFT order of the cropped grid, plus Sside
+//                               [N][Wk] holding the original row +Hc/2), or full
+//                               (Hs = Hp, all rows; the crop is a row map with
+//                               offset Hp-Hc for ky' >= Hc/2).
 //   Z       c64 [P][Wh][2m]     pass-1 output
 //   windows f32 [P][2m][2m]     index i <-> lag i-m
 #include "zr_common.h"
@@ -89,7 +91,7 @@ __global__ __launch_bounds__(NT) void k1_rows(const float* __restrict__ frames, 
 template <int HP, int NT>
 __global__ __launch_bounds__(NT) void k1_cols(const float2* __restrict__ T, int Wh, int Hc, int frame0,
                                               const float2* __restrict__ twH, float2* __restrict__ S,
-                                              float2* __restrict__ Sside) {
+                                              float2* __restrict__ Sside, int full) {
   __shared__ float2 buf[HP];
   const int tid = threadIdx.x;
   const int blk = xcd_remap(blockIdx.x, gridDim.x);
@@ -99,6 +101,11 @@ __global__ __launch_bounds__(NT) void k1_cols(const float2* __restrict__ T, int 
   __syncthreads();
   wg_fft<HP, -1, NT>(buf, twH, tid);
   const int f = frame0 + fb;
+  if (full) {
+    float2* dst = S + ((size_t)f * Wh + k) * HP;
+    for (int n = tid; n < HP; n += NT) dst[n] = buf[n];
+    return;
+  }
   float2* dst = S + ((size_t)f * Wh + k) * Hc;
   const int h2 = Hc / 2;
   for (int n = tid; n < Hc; n += NT) dst[n] = buf[n < h2 ? n : n + HP - Hc];
@@ -125,7 +132,8 @@ template <int L, int WPB>
 __global__ __launch_bounds__(64 * WPB) void k2_cols(const float2* __restrict__ S, const float2* __restrict__ Sside,
                                                      const int4* __restrict__ work, int nwork_pairs, int Wh,
                                                      int Hc, int Wc, int m, int cols_per_wave, int dc_zero,
-                                                     const float2* __restrict__ twHc, float2* __restrict__ Z) {
+                                                     const float2* __restrict__ twHc, float2* __restrict__ Z,
+                                                     int Wk, int Hs, int hoff, int side_in_col) {
   constexpr int LP = 65;  // padded LDS row (conflict-free 64-bit reads per half wave)
   extern __shared__ float2 k2_lds[];
   const int lane = threadIdx.x & 63;
@@ -157,21 +165,23 @@ __global__ __launch_bounds__(64 * WPB) void k2_cols(const float2* __restrict__ S
   for (int c = 0; c < cols_per_wave; ++c) {
     const int kx = kbase + c * WPB + w;
     if (kx >= Wh) break;
-    const float2* sI = S + ((size_t)I * Wh + kx) * Hc;
-    const float2* sJ = S + ((size_t)J * Wh + kx) * Hc;
+    const float2* sI = S + ((size_t)I * Wk + kx) * Hs;
+    const float2* sJ = S + ((size_t)J * Wk + kx) * Hs;
     const float cw = (kx == 0 || 2 * kx == Wc) ? 1.f : 2.f;
     float2 v[L];
 #pragma unroll
     for (int b = 0; b < L; ++b) {
-      float2 xi = sI[lane + 64 * b];
-      float2 xj = sJ[lane + 64 * b];
+      const int off = (b >= L / 2) ? hoff : 0;   // cropped row ky' -> stored row
+      float2 xi = sI[lane + 64 * b + off];
+      float2 xj = sJ[lane + 64 * b + off];
       v[b] = c_scale(c_cjmul(xi, xj), cw);
     }
     if (lane == 0) {
       if (dc_zero && kx == 0) v[0] = make_float2(0.f, 0.f);
       // Nyquist row (ky' = Hc/2 <-> lane 0, b = L/2): rows -Hc/2 (main) and +Hc/2 (side)
-      float2 nm = c_cjmul(sI[Hc / 2], sJ[Hc / 2]);
-      float2 ns = c_cjmul(Sside[(size_t)I * Wh + kx], Sside[(size_t)J * Wh + kx]);
+      float2 nm = c_cjmul(sI[Hc / 2 + hoff], sJ[Hc / 2 + hoff]);
+      float2 ns = side_in_col >= 0 ? c_cjmul(sI[side_in_col], sJ[side_in_col])
+                                   : c_cjmul(Sside[(size_t)I * Wk + kx], Sside[(size_t)J * Wk + kx]);
       float2 nv;
       if (kx == 0) nv = nm;
       else if (2 * kx == Wc) nv = ns;
@@ -522,11 +532,11 @@ hipError_t zr_launch_k1_rows(int Wp, const float* frames, int H, int W, int Hp, 
 }
 
 hipError_t zr_launch_k1_cols(int Hp, const float2* T, int Wh, int Hc, int frame0, int nb, const float2* twH,
-                             float2* S, float2* Sside, hipStream_t st) {
+                             float2* S, float2* Sside, int full, hipStream_t st) {
   constexpr int NT = 256;
   int grid = nb * Wh;
   switch (Hp) {
-#define ZR_CASE(X) case X: hipLaunchKernelGGL((k1_cols<X, NT>), dim3(grid), dim3(NT), 0, st, T, Wh, Hc, frame0, twH, S, Sside); return hipGetLastError();
+#define ZR_CASE(X) case X: hipLaunchKernelGGL((k1_cols<X, NT>), dim3(grid), dim3(NT), 0, st, T, Wh, Hc, frame0, twH, S, Sside, full); return hipGetLastError();
     ZR_CASE(256) ZR_CASE(512) ZR_CASE(1024) ZR_CASE(2048) ZR_CASE(3072) ZR_CASE(3840) ZR_CASE(4096)
 #undef ZR_CASE
     default: return hipErrorInvalidValue;
@@ -541,14 +551,14 @@ hipError_t zr_launch_k1_stats(const double* part, int nrb, int n, double* stats,
 static constexpr int K2C_WPB = 2;
 hipError_t zr_launch_k2_cols(int Lper, const float2* S, const float2* Sside, const int4* work, int npairs, int Wh,
                              int Hc, int Wc, int m, int cols_per_wave, int dc_zero, const float2* twHc, float2* Z,
-                             hipStream_t st) {
+                             int Wk, int Hs, int hoff, int side_in_col, hipStream_t st) {
   int cols_per_block = K2C_WPB * cols_per_wave;
   int ngroups = (Wh + cols_per_block - 1) / cols_per_block;
   int grid = ngroups * npairs;
   switch (Lper) {
 #define ZR_CASE(X) case X: { size_t sh = (size_t)K2C_WPB * X * 65 * sizeof(float2); static bool attr = false; \
       if (!attr) { hipFuncSetAttribute((const void*)k2_cols<X, K2C_WPB>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh); attr = true; } \
-      hipLaunchKernelGGL((k2_cols<X, K2C_WPB>), dim3(grid), dim3(64 * K2C_WPB), sh, st, S, Sside, work, npairs, Wh, Hc, Wc, m, cols_per_wave, dc_zero, twHc, Z); return hipGetLastError(); }
+      hipLaunchKernelGGL((k2_cols<X, K2C_WPB>), dim3(grid), dim3(64 * K2C_WPB), sh, st, S, Sside, work, npairs, Wh, Hc, Wc, m, cols_per_wave, dc_zero, twHc, Z, Wk, Hs, hoff, side_in_col); return hipGetLastError(); }
     ZR_CASE(4) ZR_CASE(8) ZR_CASE(16) ZR_CASE(32) ZR_CASE(48) ZR_CASE(64)
 #undef ZR_CASE
     default: return hipErrorInvalidValue;

@@ -1,0 +1,175 @@
+"""Drop-in ``ImageRegistrator`` whose ``alignImageStack`` runs on the MI355X.
+
+Mirrors the reference class ``zorro.ImageRegistrator`` (``zorro/zorro.py:78``)
+for the ``xcorrMode='zorro'`` path: the same attribute names and defaults
+(``zorro/zorro.py:82-243``), the same ``alignImageStack()`` entry point
+(``zorro/zorro.py:1569``) and the same results: ``translations``, ``imageSum``,
+``filtSum``, ``FRC`` and ``errorDictList[-1]`` with the keys the reference fills
+(``zorro/zorro.py:1550-1565``, ``:2016-2021``).  The compute runs in
+libzorro_hip.so; this class only moves configuration and results.
+
+Differences from the reference, all deliberate:
+* ``shiftsSolver`` returns the exact weighted-L1 optimum and
+  ``cdfLogisticCurve`` a deterministic fit of the same objective (the reference
+  uses random basinhopping, ``zorro/zorro.py:1407-1425``, ``:2008``);
+* ``Bmode='conv'`` accepts list-valued ``shapePadded`` (the reference divides
+  a list by a float, ``zorro/zorro.py:1136``);
+* modes that are broken in the reference raise ``NotImplementedError``
+  (``shiftMethod='fourier'``, ``Bmode='fourier'``, ``doseFiltParam[0]`` set).
+"""
+from __future__ import annotations
+
+import time
+
+import numpy as np
+
+from .engine import MoviePlan, make_params
+
+
+class ImageRegistrator:
+    """GPU twin of ``zorro.ImageRegistrator`` for the registration hot path."""
+
+    def __init__(self):
+        self.verbose = 0
+        self.xcorrMode = "zorro"
+        self.n_threads = None
+        self.pixelsize = None
+        self.voltage = 300.0
+        self.C3 = 2.7
+        self.bench = {}
+        self.saveC = False
+        self.Brad = 512
+        self.Bmode = "opti"
+        self.BfiltType = "gaussian"
+        self.fouCrop = [3072, 3072]
+        self.images = None
+        self.imageSum = None
+        self.filtSum = None
+        self.filterMode = None
+        self.doseFiltParam = [None, 0.24499, -1.6649, 2.8141, 32, 0]
+        self.masks = None
+        self.translations = None
+        self.errorDictList = []
+        self.doLazyFRC = True
+        self.doEvenOddFRC = False
+        self.FRC = None
+        self.shapePadded = [4096, 4096]
+        self.shapeOriginal = None
+        self.shapeBinned = None
+        self.subPixReg = 16
+        self.shiftMethod = "lanczos"
+        self.maxShift = 100
+        self.preShift = False
+        self.peakLocMode = "interpolated"
+        self.weightMode = "autologistic"
+        self.peaksigThres = 6.0
+        self.logisticK = 5.0
+        self.logisticNu = 0.15
+        self.originMode = "centroid"
+        self.suppressOrigin = True
+        self.triMode = "diag"
+        self.startFrame = 0
+        self.endFrame = 0
+        self.diagStart = 0
+        self.diagWidth = 5
+        self.autoMax = 10
+        self.corrThres = None
+        # device selection and the last plan (kept for re-use on same-shape movies)
+        self.device = 0
+        self._plan = None
+        self._plan_key = None
+
+    # ------------------------------------------------------------------
+    def _params(self):
+        if self.xcorrMode.lower() != "zorro":
+            raise NotImplementedError(f"xcorrMode {self.xcorrMode!r}: only 'zorro' runs on the GPU")
+        if self.preShift:
+            raise NotImplementedError("preShift is not built")
+        if self.masks is not None:
+            raise NotImplementedError("user masks are not built (the single automatic mask is)")
+        if self.doEvenOddFRC:
+            raise NotImplementedError("evenOddFouRingCorr is broken in the reference (zorro/zorro.py:2099-2224)")
+        if np.any(self.shapeBinned):
+            raise NotImplementedError("binStack is not built")
+        if self.filterMode and "hot" in self.filterMode.lower():
+            raise NotImplementedError("hotpixFilter is not built")
+        return make_params(triMode=self.triMode, diagWidth=self.diagWidth, diagStart=self.diagStart,
+                           maxShift=self.maxShift, Bmode=self.Bmode, Brad=self.Brad, subPixReg=self.subPixReg,
+                           shiftMethod=self.shiftMethod, weightMode=self.weightMode,
+                           peaksigThres=self.peaksigThres, logisticK=self.logisticK, logisticNu=self.logisticNu,
+                           originMode=self.originMode, suppressOrigin=self.suppressOrigin, fouCrop=self.fouCrop,
+                           shapePadded=self.shapePadded, filterMode=self.filterMode,
+                           doseFiltParam=self.doseFiltParam, pixelsize=self.pixelsize, voltage=self.voltage,
+                           doLazyFRC=self.doLazyFRC)
+
+    def _plan_for(self, n, h, w):
+        p = self._params()
+        key = (n, h, w, bytes(memoryview(p)))
+        if self._plan is None or self._plan_key != key:
+            if self._plan is not None:
+                self._plan.close()
+            self._plan = MoviePlan(p, n, h, w, device=self.device)
+            self._plan_key = key
+        return self._plan
+
+    def alignImageStack(self):
+        """Register, sum, FRC and dose-filter ``self.images`` (``zorro/zorro.py:1569-1716``)."""
+        if self.images is None or np.ndim(self.images) != 3:
+            raise ValueError("images must be a [N, H, W] stack")
+        self.bench["total0"] = time.time()
+        n, h, w = (int(v) for v in self.images.shape)
+        plan = self._plan_for(n, h, w)
+        plan.set_frames(self.images)
+        self.bench["xcorr0"] = time.time()
+        plan.xcorr()
+        plan.synchronize()
+        self.bench["xcorr3"] = self.bench["solve0"] = time.time()
+        plan.solve()
+        plan.synchronize()
+        self.bench["solve1"] = time.time()
+        tm = plan.trimats()
+        sol = plan.solution()
+        info = plan.solver_info()
+        pairs = [(I, J) for I in range(n) for J in range(I + 1, n) if tm["corrTriMat"][I, J] != 0]
+        ed = dict(tm)
+        ed.update(translations=sol["translations_raw"] if "translations_raw" in sol else None)
+        # errorDict keys of shiftsSolver (zorro/zorro.py:1550-1565)
+        unr = (np.array([p[0] for p in pairs], int), np.array([p[1] - 1 for p in pairs], int))
+        en = np.sqrt(np.sum(sol["errorXY"] ** 2, axis=1))
+        eun = np.zeros((n - 1, n - 1))
+        eun[unr] = en[:len(pairs)]
+        ed.update(relativeEst=sol["relativeEst"], acceptedEqns=np.ones(len(pairs), bool), weights=sol["weights"],
+                  errorXY=sol["errorXY"], errorX=sol["errorX"], errorY=sol["errorY"], errorUnraveled=eun,
+                  mean_errorNorm=sol["mean_errorNorm"], std_errorNorm=sol["std_errorNorm"], M=len(pairs),
+                  Maccepted=len(pairs), solverInfo=info)
+        if self.weightMode == "autologistic":
+            thr, K, nu = sol["logistic"]
+            self.peaksigThres, self.logisticK, self.logisticNu = thr, K, nu
+            ed.update(peaksigThres=thr, logisticK=K, logisticNu=nu)
+        trans_c = sol["translations"]
+        ed["translations"] = -np.vstack((np.zeros([1, 2]), np.cumsum(sol["relativeEst"], axis=0)))
+        self.errorDictList.append(ed)
+        self.bench["shifts0"] = time.time()
+        if self.filterMode and "dose" in self.filterMode.lower():
+            self.bench["dose0"] = time.time()
+            plan.dose_filter()
+        else:
+            plan.apply_shifts()
+        if self.doLazyFRC:
+            self.bench["frc0"] = time.time()
+            plan.frc()
+        plan.synchronize()
+        self.bench["shifts1"] = self.bench["frc1"] = self.bench["dose1"] = time.time()
+        sums = plan.sums()
+        self.translations = trans_c.copy()
+        self.imageSum = sums["imageSum"]
+        self.filtSum = sums.get("filtSum")
+        if self.doLazyFRC:
+            self.FRC = plan.frc_curve()
+        self.bench["total1"] = time.time()
+        return self
+
+    def close(self):
+        if self._plan is not None:
+            self._plan.close()
+            self._plan = None
