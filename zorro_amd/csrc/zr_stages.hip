@@ -978,6 +978,14 @@ __global__ __launch_bounds__(1024) void k_dose_rows(const float* __restrict__ fr
   }
 }
 
+// (q/qc)^order: repeated squaring for power-of-two orders (the default 32),
+// powf otherwise
+ZR_DEV float cutoff_pow(float x, float ord, int sq) {
+  if (sq < 0) return powf(x, ord);
+  for (int i = 0; i < sq; ++i) x *= x;
+  return x;
+}
+
 struct DoseArgs {
   int N, H, W, Hp, Wp;
   double A, B, C, order, missing, vs, ipx, ipy;
@@ -986,6 +994,7 @@ struct DoseArgs {
   double* accum;    // [N+1]
   double* thresq;   // [N]
   SolveInfo* info;
+  int order_sq;     // order = 2^order_sq, or -1
 };
 
 // dose per frame from mean(imageSum)/N, accumulated dose, per-frame cutoff
@@ -1075,7 +1084,7 @@ __global__ __launch_bounds__(256) void k_dose_cols(const float2* __restrict__ T,
     const float ord = (float)a.order;
     for (int i = 0; i < PER; ++i) {
       float filt = expf(mhd * invc[i]);
-      float thr = expf(-powf(q[i] / tq, ord));
+      float thr = expf(-cutoff_pow(q[i] / tq, ord, a.order_sq));
       float2 v = buf[tid + i * NT];
       float g = thr * filt;
       acc[i].x += v.x * g;
@@ -1260,7 +1269,7 @@ __global__ __launch_bounds__(256) void k_dose_cols_spec(const float2* __restrict
       float2 sp = c_mul(c_mul(F, Py[(size_t)J * a.H + ky]), px);
       sp = c_add(sp, c_add(r0, c_mul(rhx, ey[i])));
       sp = c_add(sp, c_add(C0[(size_t)J * a.H + ky], c_mul(CW[(size_t)J * a.H + ky], ex)));
-      const float gval = expf(-powf(q[i] / tq, ord)) * expf(mhd * invc[i]);
+      const float gval = expf(-cutoff_pow(q[i] / tq, ord, a.order_sq)) * expf(mhd * invc[i]);
       acc[i].x += sp.x * gval;
       acc[i].y += sp.y * gval;
     }
@@ -1503,6 +1512,9 @@ static void fill_dose_args(zr_stage_state* s, DoseArgs& a, const double* rowsum,
   a.ipx = 1.0 / (s->Wp * (s->p.pixelsize_nm * 10.0));
   a.ipy = 1.0 / (s->Hp * (s->p.pixelsize_nm * 10.0));
   a.rowsum = rowsum; a.nrows = nrows; a.accum = s->accum; a.thresq = s->thresq; a.info = s->info;
+  a.order_sq = -1;
+  for (int k = 0; k < 8; ++k)
+    if (a.order == (double)(1 << k)) a.order_sq = k;
 }
 
 static int dose_spectral(zr_stage_state* s, const float* frames, const float2* S, int Wk, hipStream_t st) {

@@ -127,97 +127,142 @@ __global__ void k1_frame_stats(const double* __restrict__ part, int nrb, int n, 
 //   Y_a[r] = sum_b x[a+64b] e^{+2 pi i b r / L}                  (per lane, registers)
 //   Z[dy]  = sum_a e^{+2 pi i a dy / Hc} Y_a[dy mod L]             (LDS exchange)
 // with x = c(kx) * conj(F_I) F_J (DC removed on the unmasked path) and the
-// Nyquist row combining the original rows -Hc/2 and +Hc/2.
+// Nyquist row combining the original rows -Hc/2 and +Hc/2.  The twiddles
+// e^{+2 pi i a dy/Hc} = T1[dy][a/8] T0[dy][a%8] live in LDS, shared by the block.
+// Loads are issued in groups so that at most K2_LOADG products are in flight,
+// which keeps the kernel at two or more waves per SIMD.
+constexpr int K2_LOADG = 8;
 template <int L, int WPB>
-__global__ __launch_bounds__(64 * WPB) void k2_cols(const float2* __restrict__ S, const float2* __restrict__ Sside,
-                                                     const int4* __restrict__ work, int nwork_pairs, int Wh,
-                                                     int Hc, int Wc, int m, int cols_per_wave, int dc_zero,
-                                                     const float2* __restrict__ twHc, float2* __restrict__ Z,
-                                                     int Wk, int Hs, int hoff, int side_in_col) {
+__global__ __launch_bounds__(64 * WPB, 2) void k2_cols(const float2* __restrict__ S, const float2* __restrict__ Sside,
+                                                        const int4* __restrict__ work, int nwork_pairs, int Wh,
+                                                        int Hc, int Wc, int m, int cols_per_wave, int dc_zero,
+                                                        const float2* __restrict__ twHc, float2* __restrict__ Z,
+                                                        int Wk, int Hs, int hoff, int side_in_col) {
   constexpr int LP = 65;  // padded LDS row (conflict-free 64-bit reads per half wave)
   extern __shared__ float2 k2_lds[];
   const int lane = threadIdx.x & 63;
   const int w = threadIdx.x >> 6;
+  const int twoM = 2 * m;
+  constexpr int LH = L / 2;                         // Y rows exchanged per half
+  float2* T0 = k2_lds + (size_t)WPB * LH * LP;     // [2m][8]
+  float2* T1 = T0 + (size_t)twoM * 8;              // [2m][8]
   const int blk = xcd_remap(blockIdx.x, gridDim.x);
   const int kgroup = blk / nwork_pairs;
   const int4 wk = work[blk % nwork_pairs];          // (I, J, pair index, unused)
   const int I = wk.x, J = wk.y, p = wk.z;
-  const int twoM = 2 * m;
-  // twiddles for this lane's outputs, computed once: e^{+2 pi i a dy/Hc},
-  // a = 8*a1 + a0, for dy = lane - m and lane + 64 - m
-  const int nrounds = (twoM + 63) / 64;
-  float2 T0[2][8], T1[2][8];
-  int rr[2];
-#pragma unroll
-  for (int q = 0; q < 2; ++q) {
-    int dy = lane + 64 * q - m;
-    int dym = ((dy % Hc) + Hc) % Hc;
-    rr[q] = ((dy % L) + L) % L;
-#pragma unroll
-    for (int a0 = 0; a0 < 8; ++a0) {
-      float2 t = twHc[(long)a0 * dym % Hc];
-      T0[q][a0] = make_float2(t.x, -t.y);
-      float2 u = twHc[(long)(8 * a0) * dym % Hc];
-      T1[q][a0] = make_float2(u.x, -u.y);
-    }
+  for (int e = threadIdx.x; e < twoM * 8; e += 64 * WPB) {
+    const int t = e >> 3, a0 = e & 7;
+    const int dy = t - m;
+    const long dym = ((dy % Hc) + Hc) % Hc;
+    const float2 u = twHc[(a0 * dym) % Hc], v = twHc[(8 * a0 * dym) % Hc];
+    T0[e] = make_float2(u.x, -u.y);
+    T1[e] = make_float2(v.x, -v.y);
   }
+  __syncthreads();
+  // round 2 (outputs 64..2m-1) is spread over `sf` lanes per output
+  const int c2 = twoM > 64 ? twoM - 64 : 0;
+  const int sf = c2 == 0 ? 1 : (c2 <= 16 ? 4 : (c2 <= 32 ? 2 : 1));
+  const int span = 64 / sf;                          // lanes per part
+  const int o2 = lane % span, part = lane / span;
+  const int t1 = lane, t2 = 64 + o2;
+  const int r1 = (((t1 - m) % L) + L) % L, r2 = (((t2 - m) % L) + L) % L;
+  const int a1lo = part * (8 / sf), a1hi = a1lo + 8 / sf;
   const int kbase = kgroup * WPB * cols_per_wave;
+  float2* my = k2_lds + (size_t)w * LH * LP;
   for (int c = 0; c < cols_per_wave; ++c) {
     const int kx = kbase + c * WPB + w;
     if (kx >= Wh) break;
     const float2* sI = S + ((size_t)I * Wk + kx) * Hs;
     const float2* sJ = S + ((size_t)J * Wk + kx) * Hs;
     const float cw = (kx == 0 || 2 * kx == Wc) ? 1.f : 2.f;
+    // Nyquist row (ky' = Hc/2 <-> lane 0, b = L/2): rows -Hc/2 (main) and +Hc/2
+    // (side), combined per column; wave-uniform loads, applied by a lane select
+    float2 nv;
+    {
+      const float2 nm = c_cjmul(sI[Hc / 2 + hoff], sJ[Hc / 2 + hoff]);
+      const float2 ns = side_in_col >= 0 ? c_cjmul(sI[side_in_col], sJ[side_in_col])
+                                         : c_cjmul(Sside[(size_t)I * Wk + kx], Sside[(size_t)J * Wk + kx]);
+      nv = (kx == 0) ? nm : ((2 * kx == Wc) ? ns : c_add(nm, ns));
+    }
     float2 v[L];
 #pragma unroll
-    for (int b = 0; b < L; ++b) {
-      const int off = (b >= L / 2) ? hoff : 0;   // cropped row ky' -> stored row
-      float2 xi = sI[lane + 64 * b + off];
-      float2 xj = sJ[lane + 64 * b + off];
-      v[b] = c_scale(c_cjmul(xi, xj), cw);
+    for (int g0 = 0; g0 < L; g0 += K2_LOADG) {
+      float2 xi[K2_LOADG], xj[K2_LOADG];
+#pragma unroll
+      for (int j = 0; j < K2_LOADG; ++j) {
+        const int b = g0 + j;
+        const int off = (b >= L / 2) ? hoff : 0;   // cropped row ky' -> stored row
+        xi[j] = sI[lane + 64 * b + off];
+        xj[j] = sJ[lane + 64 * b + off];
+      }
+#pragma unroll
+      for (int j = 0; j < K2_LOADG; ++j) v[g0 + j] = c_scale(c_cjmul(xi[j], xj[j]), cw);
+      __builtin_amdgcn_sched_barrier(0);
     }
-    if (lane == 0) {
-      if (dc_zero && kx == 0) v[0] = make_float2(0.f, 0.f);
-      // Nyquist row (ky' = Hc/2 <-> lane 0, b = L/2): rows -Hc/2 (main) and +Hc/2 (side)
-      float2 nm = c_cjmul(sI[Hc / 2 + hoff], sJ[Hc / 2 + hoff]);
-      float2 ns = side_in_col >= 0 ? c_cjmul(sI[side_in_col], sJ[side_in_col])
-                                   : c_cjmul(Sside[(size_t)I * Wk + kx], Sside[(size_t)J * Wk + kx]);
-      float2 nv;
-      if (kx == 0) nv = nm;
-      else if (2 * kx == Wc) nv = ns;
-      else nv = c_add(nm, ns);
-      v[L / 2] = nv;
+    {
+      const bool l0 = (lane == 0);
+      const bool dc = l0 && dc_zero && kx == 0;
+      v[0] = dc ? make_float2(0.f, 0.f) : v[0];
+      v[L / 2] = l0 ? nv : v[L / 2];
     }
     Dft<L, 1>::run(v);
-    float2* my = k2_lds + (size_t)w * L * LP;
-#pragma unroll
-    for (int r = 0; r < L; ++r) my[r * LP + lane] = v[r];
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     float2* zdst = Z + ((size_t)p * Wh + kx) * twoM;
+    // two halves of r through LDS; each output reads the half holding its r
 #pragma unroll
-    for (int q = 0; q < 2; ++q) {
-      if (q < nrounds) {
-        int t = lane + 64 * q;
-        const float2* yr = my + rr[q] * LP;
+    for (int h = 0; h < 2; ++h) {
+#pragma unroll
+      for (int r = 0; r < LH; ++r) my[r * LP + lane] = v[h * LH + r];
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      if (t1 < twoM && (r1 / LH) == h) {   // round 1: outputs 0..63, one per lane
+        const float2* yr = my + (r1 - h * LH) * LP;
+        const float2* t0 = T0 + t1 * 8;
+        const float2* tt1 = T1 + t1 * 8;
         float2 acc = make_float2(0.f, 0.f);
 #pragma unroll
         for (int a1 = 0; a1 < 8; ++a1) {
-          float2 part = make_float2(0.f, 0.f);
+          float2 pa = make_float2(0.f, 0.f);
 #pragma unroll
           for (int a0 = 0; a0 < 8; ++a0) {
-            float2 yv = yr[8 * a1 + a0];
-            part.x += T0[q][a0].x * yv.x - T0[q][a0].y * yv.y;
-            part.y += T0[q][a0].x * yv.y + T0[q][a0].y * yv.x;
+            const float2 yv = yr[8 * a1 + a0], tw = t0[a0];
+            pa.x += tw.x * yv.x - tw.y * yv.y;
+            pa.y += tw.x * yv.y + tw.y * yv.x;
           }
-          acc.x += T1[q][a1].x * part.x - T1[q][a1].y * part.y;
-          acc.y += T1[q][a1].x * part.y + T1[q][a1].y * part.x;
+          const float2 tw = tt1[a1];
+          acc.x += tw.x * pa.x - tw.y * pa.y;
+          acc.y += tw.x * pa.y + tw.y * pa.x;
         }
-        if (t < twoM) zdst[t] = acc;
+        zdst[t1] = acc;
       }
+      if (c2 > 0) {      // round 2: outputs 64..2m-1, `sf` partial sums each
+        float2 acc = make_float2(0.f, 0.f);
+        const bool mine = (o2 < c2) && ((r2 / LH) == h);
+        if (mine) {
+          const float2* yr = my + (r2 - h * LH) * LP;
+          const float2* t0 = T0 + t2 * 8;
+          const float2* tt1 = T1 + t2 * 8;
+          for (int a1 = a1lo; a1 < a1hi; ++a1) {
+            float2 pa = make_float2(0.f, 0.f);
+#pragma unroll
+            for (int a0 = 0; a0 < 8; ++a0) {
+              const float2 yv = yr[8 * a1 + a0], tw = t0[a0];
+              pa.x += tw.x * yv.x - tw.y * yv.y;
+              pa.y += tw.x * yv.y + tw.y * yv.x;
+            }
+            const float2 tw = tt1[a1];
+            acc.x += tw.x * pa.x - tw.y * pa.y;
+            acc.y += tw.x * pa.y + tw.y * pa.x;
+          }
+        }
+        for (int o = span; o < 64; o <<= 1) {
+          acc.x += __shfl_xor(acc.x, o, 64);
+          acc.y += __shfl_xor(acc.y, o, 64);
+        }
+        if (part == 0 && mine) zdst[t2] = acc;
+      }
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     }
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
   }
 }
 
@@ -556,7 +601,7 @@ hipError_t zr_launch_k2_cols(int Lper, const float2* S, const float2* Sside, con
   int ngroups = (Wh + cols_per_block - 1) / cols_per_block;
   int grid = ngroups * npairs;
   switch (Lper) {
-#define ZR_CASE(X) case X: { size_t sh = (size_t)K2C_WPB * X * 65 * sizeof(float2); static bool attr = false; \
+#define ZR_CASE(X) case X: { size_t sh = ((size_t)K2C_WPB * (X / 2) * 65 + (size_t)2 * 2 * m * 8) * sizeof(float2); static bool attr = false; \
       if (!attr) { hipFuncSetAttribute((const void*)k2_cols<X, K2C_WPB>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh); attr = true; } \
       hipLaunchKernelGGL((k2_cols<X, K2C_WPB>), dim3(grid), dim3(64 * K2C_WPB), sh, st, S, Sside, work, npairs, Wh, Hc, Wc, m, cols_per_wave, dc_zero, twHc, Z, Wk, Hs, hoff, side_in_col); return hipGetLastError(); }
     ZR_CASE(4) ZR_CASE(8) ZR_CASE(16) ZR_CASE(32) ZR_CASE(48) ZR_CASE(64)
