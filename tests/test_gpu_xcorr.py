@@ -25,6 +25,8 @@ CASES = [
     ("1024sq_C1", 20, 1024, 1024, (0, 0), 40, 150, 1),
     ("2048sq_crop1024", 6, 2048, 2048, (1024, 1024), 40, 150, 5),
     ("4096sq_crop3072", 4, 4096, 4096, (3072, 3072), 40, 150, 2),
+    # largest built window: 128x128 lags with a 51-tap Gaussian (LDS-bound peak stage)
+    ("4096sq_crop3072_m64", 3, 4096, 4096, (3072, 3072), 64, 150, 3),
 ]
 
 
@@ -82,3 +84,11 @@ def test_xcorr_parity(case):
         rel = np.max(np.abs(a - b) / np.maximum(np.abs(b), 1e-12 + 1e-3 * np.max(np.abs(b))))
         print(f"[gpu xcorr] {name}: {key} max rel {rel:.2e}", flush=True)
         assert rel <= tol, key
+
+
+def test_peak_lds_limit_rejected():
+    """A Gaussian too wide for the peak stage's LDS at maxShift 64 is refused at plan time."""
+    kw = dict(triMode="tri", maxShift=64, Bmode="conv", Brad=20, subPixReg=16, fouCrop=(3072, 3072),
+              shapePadded=(4096, 4096))
+    with pytest.raises(RuntimeError, match="LDS"):
+        MoviePlan(make_params(**kw), 3, 4096, 4096)

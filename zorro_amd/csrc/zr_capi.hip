@@ -211,6 +211,20 @@ static int build_pairs(zr_movie* mv) {
   return dev_upload(&mv->work, wk);
 }
 
+static PeakParamsHost peak_params(const zr_movie* mv) {
+  PeakParamsHost hp;
+  hp.m = mv->m;
+  hp.suppress = mv->p.suppress_origin ? 1 : 0;
+  hp.bconv = (mv->p.b_mode == ZR_B_CONV && mv->p.b_rad > 0) ? 1 : 0;
+  hp.ry = mv->ry;
+  hp.rx = mv->rx;
+  hp.subpix = mv->p.sub_pix_reg;
+  hp.scale_y = mv->crop ? (float)((double)mv->Hp / (double)mv->Hc) : 1.f;
+  hp.scale_x = mv->crop ? (float)((double)mv->Wp / (double)mv->Wc) : 1.f;
+  hp.n = mv->N;
+  return hp;
+}
+
 extern "C" int zr_movie_create(int device, const zr_params* p, int n_frames, int h, int w, zr_movie** out) {
   if (!p || !out) return fail(ZR_ERR_ARG, "NULL argument");
   *out = nullptr;
@@ -290,6 +304,16 @@ extern "C" int zr_movie_create(int device, const zr_params* p, int n_frames, int
       }
     }
     if ((rc = dev_upload(&mv->gy, wy)) || (rc = dev_upload(&mv->gx, wx))) { free_movie(mv); return rc; }
+  }
+  {
+    const PeakParamsHost hp = peak_params(mv);
+    const size_t lds = zr_peaks_lds_bytes(&hp, nullptr);
+    if (lds > (size_t)ZR_LDS_BYTES) {
+      free_movie(mv);
+      return fail(ZR_ERR_UNSUPPORTED, "peak search needs %zu bytes of LDS (> %d): maxShift %d with a %d-tap Gaussian "
+                  "(Brad %g) does not fit; raise Brad or lower maxShift", lds, ZR_LDS_BYTES, mv->m,
+                  2 * std::max(mv->ry, mv->rx) + 1, p->b_rad);
+    }
   }
   if ((rc = build_pairs(mv))) { free_movie(mv); return rc; }
   // workspaces
@@ -371,16 +395,7 @@ extern "C" int zr_xcorr(zr_movie* mv) {
   ZR_HIP(hipMemsetAsync(mv->corr, 0, sizeof(float) * mv->N * mv->N, s));
   ZR_HIP(hipMemsetAsync(mv->peaksig, 0, sizeof(float) * mv->N * mv->N, s));
   ZR_HIP(hipMemsetAsync(mv->origin, 0, sizeof(float) * mv->N * mv->N, s));
-  PeakParamsHost hp;
-  hp.m = mv->m;
-  hp.suppress = mv->p.suppress_origin ? 1 : 0;
-  hp.bconv = (mv->p.b_mode == ZR_B_CONV && mv->p.b_rad > 0) ? 1 : 0;
-  hp.ry = mv->ry;
-  hp.rx = mv->rx;
-  hp.subpix = mv->p.sub_pix_reg;
-  hp.scale_y = mv->crop ? (float)((double)mv->Hp / (double)mv->Hc) : 1.f;
-  hp.scale_x = mv->crop ? (float)((double)mv->Wp / (double)mv->Wc) : 1.f;
-  hp.n = mv->N;
+  const PeakParamsHost hp = peak_params(mv);
   ZR_HIP(zr_launch_peaks(mv->windows, mv->pairs, mv->P, &hp, mv->gy, mv->gx, mv->ktab, mv->shifts, mv->corr,
                          mv->peaksig, mv->origin, s));
   mv->timer.end(ZR_STAGE_PEAKS, t2, s);

@@ -8,6 +8,26 @@ struct PeakParamsHost {
   int n;
 };
 
+// LDS available to one workgroup on gfx950
+constexpr int ZR_LDS_BYTES = 160 * 1024;
+
+// Dynamic LDS of the peak kernel: a scratch region that holds the
+// reflect-padded window of the Gaussian passes and later the sub-pixel
+// transform, followed by the filtered 2m x 2m window.
+inline size_t zr_peaks_lds_bytes(const PeakParamsHost* hp, int* tmp_floats) {
+  const size_t n2 = 2 * (size_t)hp->m, nn = n2 * n2;
+  const size_t G = 16 * (size_t)(hp->subpix > 1 ? hp->subpix : 1);
+  size_t t = nn > 34 * G ? nn : 34 * G;
+  if (hp->bconv) {
+    const size_t r = (size_t)(hp->ry > hp->rx ? hp->ry : hp->rx);
+    const size_t padded = (n2 + 2 * r) * n2;
+    t = t > padded ? t : padded;
+  }
+  t = (t + 63) & ~(size_t)63;
+  if (tmp_floats) *tmp_floats = (int)t;
+  return (t + nn) * sizeof(float);
+}
+
 // zr_xcorr.hip
 hipError_t zr_launch_k1_rows(int Wp, const float* frames, int H, int W, int Hp, int Wh, int frame0, int nb,
                              const float2* twL, const float2* tw2L, float2* T, double* part, int* rb_out,

@@ -752,28 +752,63 @@ ZR_DEV FrameShift reduce_shift(FrameShift s, int H, int W) {
   return s;
 }
 
+// Shift + sums: each thread forms 4 consecutive aligned pixels of one row for
+// every frame (18 loads per frame instead of 36) and accumulates imageSum and
+// the even/odd half sums in frame order.
+constexpr int SS_PX = 4;
 __global__ __launch_bounds__(256) void k_shift_sum(const float* __restrict__ frames, const FrameShift* __restrict__ fs,
                                                   int N, int H, int W, float* __restrict__ img_sum,
                                                   float* __restrict__ even_sum, float* __restrict__ odd_sum,
                                                   float* __restrict__ aligned, double* __restrict__ rowsum) {
-  const int x = blockIdx.x * 64 + (threadIdx.x & 63);
+  const int x0 = (blockIdx.x * 64 + (threadIdx.x & 63)) * SS_PX;
   const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
   __shared__ double red[4];
   double part = 0.0;
-  if (y < H && x < W) {
-    float se = 0.f, so = 0.f, st = 0.f;
+  if (y < H && x0 < W) {
+    float se[SS_PX], so[SS_PX], st[SS_PX];
+#pragma unroll
+    for (int i = 0; i < SS_PX; ++i) { se[i] = 0.f; so[i] = 0.f; st[i] = 0.f; }
     for (int J = 0; J < N; ++J) {
       const FrameShift s = reduce_shift(fs[J], H, W);
       const RowSrc rs = row_sources(frames + (size_t)J * H * W, s, y, H, W);
-      float v = aligned_px_row(rs, s, x, W);
-      st += v;
-      if (J & 1) so += v; else se += v;
-      if (aligned) aligned[((size_t)J * H + y) * W + x] = v;
+      float v[SS_PX];
+      if (!s.conv) {
+#pragma unroll
+        for (int i = 0; i < SS_PX; ++i) v[i] = (x0 + i < W) ? rs.r[0][wrap1(x0 + i - s.ix, W)] : 0.f;
+      } else {
+        // source columns x0-1 .. x0+SS_PX (reflected at the frame edge, then rolled)
+        int cx[SS_PX + 2];
+#pragma unroll
+        for (int c = 0; c < SS_PX + 2; ++c) cx[c] = wrap1(refl1(min(x0 - 1 + c, W), W) - s.ix, W);
+#pragma unroll
+        for (int i = 0; i < SS_PX; ++i) v[i] = 0.f;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+          const float* row = rs.r[a];
+          float g[SS_PX + 2];
+#pragma unroll
+          for (int c = 0; c < SS_PX + 2; ++c) g[c] = row[cx[c]];
+#pragma unroll
+          for (int i = 0; i < SS_PX; ++i)      // out[x] = sum_b k[a][b] g[x+1-b]
+            v[i] += s.k[a * 3 + 0] * g[i + 2] + s.k[a * 3 + 1] * g[i + 1] + s.k[a * 3 + 2] * g[i];
+        }
+      }
+#pragma unroll
+      for (int i = 0; i < SS_PX; ++i) {
+        st[i] += v[i];
+        if (J & 1) so[i] += v[i]; else se[i] += v[i];
+        if (aligned && x0 + i < W) aligned[((size_t)J * H + y) * W + x0 + i] = v[i];
+      }
     }
-    img_sum[(size_t)y * W + x] = st;
-    even_sum[(size_t)y * W + x] = se;
-    odd_sum[(size_t)y * W + x] = so;
-    part = st;
+#pragma unroll
+    for (int i = 0; i < SS_PX; ++i) {
+      if (x0 + i < W) {
+        img_sum[(size_t)y * W + x0 + i] = st[i];
+        even_sum[(size_t)y * W + x0 + i] = se[i];
+        odd_sum[(size_t)y * W + x0 + i] = so[i];
+        part += st[i];
+      }
+    }
   }
   part = wave_sum_d(part);
   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = part;
@@ -1470,7 +1505,7 @@ int zr_stage_solve(zr_stage_state* s, const float* shifts, const float* corr, co
 
 int zr_stage_shifts(zr_stage_state* s, const float* frames, float* aligned_out, hipStream_t st) {
   if (!s || !s->have_solution) return serr(ZR_ERR_STATE, "shifts before solve");
-  dim3 grid((s->W + 63) / 64, (s->H + 3) / 4);
+  dim3 grid((s->W + 64 * SS_PX - 1) / (64 * SS_PX), (s->H + 3) / 4);
   hipLaunchKernelGGL(k_shift_sum, grid, dim3(256), 0, st, frames, s->fs, s->N, s->H, s->W, s->img_sum, s->even,
                      s->odd, aligned_out, s->rowsum);
   ZS_HIP(hipGetLastError());
@@ -1520,7 +1555,7 @@ static void fill_dose_args(zr_stage_state* s, DoseArgs& a, const double* rowsum,
 static int dose_spectral(zr_stage_state* s, const float* frames, const float2* S, int Wk, hipStream_t st) {
   const int H = s->H, W = s->W;
   // aligned sums (imageSum, even, odd) and their total for the dose per frame
-  dim3 grid((W + 63) / 64, (H + 3) / 4);
+  dim3 grid((W + 64 * SS_PX - 1) / (64 * SS_PX), (H + 3) / 4);
   hipLaunchKernelGGL(k_shift_sum, grid, dim3(256), 0, st, frames, s->fs, s->N, H, W, s->img_sum, s->even, s->odd,
                      (float*)nullptr, s->rowsum);
   ZS_HIP(hipGetLastError());

@@ -112,14 +112,21 @@ __global__ __launch_bounds__(NT) void k1_cols(const float2* __restrict__ T, int 
   if (tid == 0) Sside[(size_t)f * Wh + k] = buf[h2];
 }
 
-__global__ void k1_frame_stats(const double* __restrict__ part, int nrb, int n, double* __restrict__ stats) {
-  // stats[f] = (sum, sumsq); one thread per frame, fixed summation order
-  int f = blockIdx.x * blockDim.x + threadIdx.x;
-  if (f >= n) return;
+__global__ __launch_bounds__(256) void k1_frame_stats(const double* __restrict__ part, int nrb, int n,
+                                                      double* __restrict__ stats) {
+  // stats[f] = (sum, sumsq); one block per frame, fixed reduction order
+  __shared__ double ra[4], rb[4];
+  const int f = blockIdx.x, tid = threadIdx.x;
   double a = 0, b = 0;
-  for (int i = 0; i < nrb; ++i) { a += part[((size_t)f * nrb + i) * 2]; b += part[((size_t)f * nrb + i) * 2 + 1]; }
-  stats[2 * f] = a;
-  stats[2 * f + 1] = b;
+  for (int i = tid; i < nrb; i += 256) { a += part[((size_t)f * nrb + i) * 2]; b += part[((size_t)f * nrb + i) * 2 + 1]; }
+  a = wave_sum_d(a);
+  b = wave_sum_d(b);
+  if ((tid & 63) == 0) { ra[tid >> 6] = a; rb[tid >> 6] = b; }
+  __syncthreads();
+  if (tid == 0) {
+    stats[2 * f] = (ra[0] + ra[1]) + (ra[2] + ra[3]);
+    stats[2 * f + 1] = (rb[0] + rb[1]) + (rb[2] + rb[3]);
+  }
 }
 
 // --------------------------------------------------------------------- K2
@@ -354,6 +361,7 @@ struct PeakParams {
   int subpix;        // subPixReg (<=1: integer)
   float scale_y, scale_x;   // Hp/Hc, Wp/Wc (1 when no crop)
   int n;             // frames (trimat side)
+  int tmp_floats;    // LDS floats ahead of the filtered window
 };
 
 ZR_DEV int reflect_idx(int i, int n) {
@@ -369,49 +377,67 @@ __global__ __launch_bounds__(256) void k_peaks(const float* __restrict__ windows
                                               const float* __restrict__ gx, const float2* __restrict__ ktab,
                                               float* __restrict__ shifts, float* __restrict__ corr,
                                               float* __restrict__ peaksig, float* __restrict__ origin) {
+  // LDS: [tmp: max(2m*2m, reflect-padded window, 34*16*subpix)] [Cf: 2m*2m]
   extern __shared__ float smem[];
   const int m = pp.m, n2 = 2 * pp.m, nn = n2 * n2;
-  float* C = smem;            // raw window (after suppression)
-  float* Cf = C + nn;         // filtered
-  float* tmp = Cf + nn;       // filter temp
+  float* tmp = smem;          // padded filter input, then sub-pixel scratch
+  float* Cf = smem + pp.tmp_floats;   // filtered window
   __shared__ double rs[2][4];
   __shared__ float rv[4];
   __shared__ int ri[4];
-  __shared__ int s_cmax;
+  __shared__ float s_sup;
   const int tid = threadIdx.x;
   const int4 pr = pairs[blockIdx.x];
-  const int p = pr.z;
-  const float* win = windows + (size_t)p * nn;
-  for (int e = tid; e < nn; e += 256) C[e] = win[e];
-  __syncthreads();
+  const float* win = windows + (size_t)pr.z * nn;
+  const int cidx = m * n2 + m;   // zero lag
   if (tid == 0) {
-    const int c = m * n2 + m;
-    origin[pr.x * pp.n + pr.y] = C[c];
+    float v = win[cidx];
+    origin[pr.x * pp.n + pr.y] = v;
     if (pp.suppress) {
       // lag (ly,lx) -> index (m+ly)*2m + (m+lx); stencil with duplicated terms
-      float v = C[(m + 1) * n2 + m] + C[m * n2 + m + 1] + C[(m - 1) * n2 + m] + C[(m - 1) * n2 + m] +
-                C[(m + 1) * n2 + m + 1] + C[(m - 1) * n2 + m + 1] + C[(m - 1) * n2 + m + 1] +
-                C[(m - 1) * n2 + m - 1];
-      C[c] = 0.125f * v;
+      v = 0.125f * (win[(m + 1) * n2 + m] + win[m * n2 + m + 1] + win[(m - 1) * n2 + m] + win[(m - 1) * n2 + m] +
+                    win[(m + 1) * n2 + m + 1] + win[(m - 1) * n2 + m + 1] + win[(m - 1) * n2 + m + 1] +
+                    win[(m - 1) * n2 + m - 1]);
     }
+    s_sup = v;
   }
   __syncthreads();
+  const float sup = s_sup;   // window value at zero lag after suppression
   if (pp.bconv) {
-    for (int e = tid; e < nn; e += 256) {
-      int i = e / n2, j = e % n2;
-      float s = 0.f;
-      for (int k = -pp.ry; k <= pp.ry; ++k) s += gy[k + pp.ry] * C[reflect_idx(i + k, n2) * n2 + j];
-      tmp[e] = s;
+    // separable Gaussian, axis 0 then axis 1, through reflect-padded copies:
+    // P[(i+ry)][j] = C[refl(i)][j] for i in [-ry, 2m+ry)
+    float* P = tmp;
+    const int prows = n2 + 2 * pp.ry;
+    for (int e = tid; e < prows * n2; e += 256) {
+      const int i = e / n2 - pp.ry, j = e - (e / n2) * n2;
+      const int src = reflect_idx(i, n2) * n2 + j;
+      P[e] = src == cidx ? sup : win[src];
     }
     __syncthreads();
     for (int e = tid; e < nn; e += 256) {
-      int i = e / n2, j = e % n2;
+      const int i = e / n2, j = e - i * n2;
+      const float* col = P + i * n2 + j;
       float s = 0.f;
-      for (int k = -pp.rx; k <= pp.rx; ++k) s += gx[k + pp.rx] * tmp[i * n2 + reflect_idx(j + k, n2)];
+      for (int k = 0; k <= 2 * pp.ry; ++k) s += gy[k] * col[k * n2];
+      Cf[e] = s;
+    }
+    __syncthreads();
+    // rows: Q[i][j+rx] = Cf[i][refl(j)]
+    const int pc = n2 + 2 * pp.rx;
+    for (int e = tid; e < n2 * pc; e += 256) {
+      const int i = e / pc, j = e - i * pc - pp.rx;
+      P[e] = Cf[i * n2 + reflect_idx(j, n2)];
+    }
+    __syncthreads();
+    for (int e = tid; e < nn; e += 256) {
+      const int i = e / n2, j = e - i * n2;
+      const float* row = P + i * pc + j;
+      float s = 0.f;
+      for (int k = 0; k <= 2 * pp.rx; ++k) s += gx[k] * row[k];
       Cf[e] = s;
     }
   } else {
-    for (int e = tid; e < nn; e += 256) Cf[e] = C[e];
+    for (int e = tid; e < nn; e += 256) Cf[e] = e == cidx ? sup : win[e];
   }
   __syncthreads();
   // disc statistics and argmax of Cf*disc (zeros outside take part; first index wins ties)
@@ -459,7 +485,7 @@ __global__ __launch_bounds__(256) void k_peaks(const float* __restrict__ windows
   const int cy = bi / n2, cx = bi % n2;
   const float psig = (float)(((double)bv - mean) / sqrt(var));
   const int I = pr.x, J = pr.y, n = pp.n;
-  float sh_y = (float)(cy - m), sh_x = (float)(cx - m), cval = C[bi];
+  float sh_y = (float)(cy - m), sh_x = (float)(cx - m), cval = bi == cidx ? sup : win[bi];
   // sub-pixel: python-slice semantics of C[c-8:c+8]
   bool valid = false;
   int sy = cy - 8, sx = cx - 8;
@@ -548,6 +574,8 @@ __global__ __launch_bounds__(256) void k_peaks(const float* __restrict__ windows
 // Host-side launch helpers (instantiated for the supported sizes).
 #include "zr_launch.h"
 
+#include <algorithm>
+
 template <int WP>
 static hipError_t launch_k1_rows_t(const float* frames, int H, int W, int Hp, int Wh, int frame0, int nb,
                                    const float2* twL, const float2* tw2L, float2* T, double* part, hipStream_t st) {
@@ -589,7 +617,7 @@ hipError_t zr_launch_k1_cols(int Hp, const float2* T, int Wh, int Hc, int frame0
 }
 
 hipError_t zr_launch_k1_stats(const double* part, int nrb, int n, double* stats, hipStream_t st) {
-  hipLaunchKernelGGL(k1_frame_stats, dim3((n + 63) / 64), dim3(64), 0, st, part, nrb, n, stats);
+  hipLaunchKernelGGL(k1_frame_stats, dim3(n), dim3(256), 0, st, part, nrb, n, stats);
   return hipGetLastError();
 }
 
@@ -635,10 +663,8 @@ hipError_t zr_launch_peaks(const float* windows, const int4* pairs, int npairs, 
   PeakParams pp;
   pp.m = hp->m; pp.suppress = hp->suppress; pp.bconv = hp->bconv; pp.ry = hp->ry; pp.rx = hp->rx;
   pp.subpix = hp->subpix; pp.scale_y = hp->scale_y; pp.scale_x = hp->scale_x; pp.n = hp->n;
-  int nn = 4 * pp.m * pp.m;
-  int G = 16 * (pp.subpix > 1 ? pp.subpix : 1);
-  size_t tmp_floats = (size_t)nn > (size_t)34 * G ? (size_t)nn : (size_t)34 * G;
-  size_t sh = (2 * (size_t)nn + tmp_floats) * sizeof(float);
+  const size_t sh = zr_peaks_lds_bytes(hp, &pp.tmp_floats);
+  if (sh > (size_t)ZR_LDS_BYTES) return hipErrorInvalidConfiguration;
   static int attr = 0;
   if ((int)sh > attr) {
     hipFuncSetAttribute((const void*)k_peaks, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh);
