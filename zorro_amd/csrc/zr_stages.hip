@@ -1256,32 +1256,34 @@ __global__ __launch_bounds__(256) void k_dose_border(const float* __restrict__ f
 
 // Per kx column: accumulate the filtered aligned spectra of all frames, then the
 // inverse transform along y.  S is the full-resolution kx-major half spectrum.
-template <int HP>
-__global__ __launch_bounds__(256) void k_dose_cols_spec(const float2* __restrict__ S, int Wk, DoseArgs a,
-                                                       const float2* __restrict__ R0, const float2* __restrict__ RH,
-                                                       const float2* __restrict__ C0, const float2* __restrict__ CW,
-                                                       const float2* __restrict__ Py, const float2* __restrict__ Px,
-                                                       const float2* __restrict__ twH, const float2* __restrict__ twW,
-                                                       float2* __restrict__ G) {
-  constexpr int NT = 256, PER = HP / NT;
+template <int HP, int NT>
+__global__ __launch_bounds__(NT) void k_dose_cols_spec(const float2* __restrict__ S, int Wk, DoseArgs a,
+                                                      const float2* __restrict__ R0, const float2* __restrict__ RH,
+                                                      const float2* __restrict__ C0, const float2* __restrict__ CW,
+                                                      const float2* __restrict__ Py, const float2* __restrict__ Px,
+                                                      const float2* __restrict__ twH, const float2* __restrict__ twW,
+                                                      float2* __restrict__ G) {
+  constexpr int PER = HP / NT;
+  static_assert(HP % NT == 0, "column must split evenly over the block");
   __shared__ float2 buf[HP];
   const int tid = threadIdx.x;
   const int kx = xcd_remap(blockIdx.x, gridDim.x);
   const int Whalf = a.Wp / 2 + 1;
   float invc[PER], q[PER];
   float2 acc[PER], ey[PER];
+  auto crit = [&](double yy, double xx) {
+    const double qq = sqrt(xx * xx * a.ipx * a.ipx + yy * yy * a.ipy * a.ipy);
+    return a.vs * (a.A * pow(qq, a.B) + a.C);
+  };
+#pragma unroll
   for (int i = 0; i < PER; ++i) {
     const int ky = tid + i * NT;
     const double fy = ky < HP / 2 ? ky : ky - HP;
     const double fx = kx;
-    auto crit = [&](double yy, double xx) {
-      double qq = sqrt(xx * xx * a.ipx * a.ipx + yy * yy * a.ipy * a.ipy);
-      return a.vs * (a.A * pow(qq, a.B) + a.C);
-    };
-    double qq = sqrt(fx * fx * a.ipx * a.ipx + fy * fy * a.ipy * a.ipy);
+    const double qq = sqrt(fx * fx * a.ipx * a.ipx + fy * fy * a.ipy * a.ipy);
     double c = crit(fy, fx);
     if (ky == HP / 2 && 2 * kx == a.Wp) {   // Nyquist corner takes its left neighbour squared
-      double cl = crit(-HP / 2, a.Wp / 2 - 1);
+      const double cl = crit(-HP / 2, a.Wp / 2 - 1);
       c = cl * cl;
     }
     invc[i] = (float)(1.0 / c);
@@ -1296,23 +1298,33 @@ __global__ __launch_bounds__(256) void k_dose_cols_spec(const float2* __restrict
     const float2 px = Px[(size_t)J * Whalf + kx];
     const float2 r0 = R0[(size_t)J * Whalf + kx], rh = RH[(size_t)J * Whalf + kx];
     const float mhd = (float)(-0.5 * a.accum[J + 1]);
-    const float tq = (float)a.thresq[J];
-    const float2 rhx = rh;
+    const float itq = (float)(1.0 / a.thresq[J]);
+    float2 F[PER], py[PER], c0[PER], cw[PER];
+#pragma unroll
     for (int i = 0; i < PER; ++i) {
       const int ky = tid + i * NT;
-      float2 F = col[ky];
-      float2 sp = c_mul(c_mul(F, Py[(size_t)J * a.H + ky]), px);
-      sp = c_add(sp, c_add(r0, c_mul(rhx, ey[i])));
-      sp = c_add(sp, c_add(C0[(size_t)J * a.H + ky], c_mul(CW[(size_t)J * a.H + ky], ex)));
-      const float gval = expf(-cutoff_pow(q[i] / tq, ord, a.order_sq)) * expf(mhd * invc[i]);
+      F[i] = col[ky];
+      py[i] = Py[(size_t)J * a.H + ky];
+      c0[i] = C0[(size_t)J * a.H + ky];
+      cw[i] = CW[(size_t)J * a.H + ky];
+    }
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+      float2 sp = c_mul(c_mul(F[i], py[i]), px);
+      sp = c_add(sp, c_add(r0, c_mul(rh, ey[i])));
+      sp = c_add(sp, c_add(c0[i], c_mul(cw[i], ex)));
+      const float gval = expf(-cutoff_pow(q[i] * itq, ord, a.order_sq)) * expf(mhd * invc[i]);
       acc[i].x += sp.x * gval;
       acc[i].y += sp.y * gval;
     }
   }
+#pragma unroll
   for (int i = 0; i < PER; ++i) buf[tid + i * NT] = acc[i];
   __syncthreads();
   wg_fft<HP, 1, NT>(buf, twH, tid);
-  for (int y = tid; y < a.H; y += NT) G[(size_t)kx * a.H + y] = buf[y];
+#pragma unroll
+  for (int i = 0; i < PER; ++i)
+    if (tid + i * NT < a.H) G[(size_t)kx * a.H + tid + i * NT] = buf[tid + i * NT];
 }
 
 // ==================================================================== host
@@ -1573,7 +1585,7 @@ static int dose_spectral(zr_stage_state* s, const float* frames, const float2* S
   }
   ZS_HIP(hipGetLastError());
   switch (s->Hp) {
-#define ZR_DS(X) case X: hipLaunchKernelGGL(k_dose_cols_spec<X>, dim3(s->Wp / 2 + 1), dim3(256), 0, st, S, Wk, a, s->R0, s->RH, s->C0, s->CW, s->Py, s->Px, s->twHp, s->twWp, s->G); break;
+#define ZR_DS(X) case X: hipLaunchKernelGGL((k_dose_cols_spec<X, (X >= 1024 ? X / 4 : 256)>), dim3(s->Wp / 2 + 1), dim3(X >= 1024 ? X / 4 : 256), 0, st, S, Wk, a, s->R0, s->RH, s->C0, s->CW, s->Py, s->Px, s->twHp, s->twWp, s->G); break;
     ZR_DS(256) ZR_DS(512) ZR_DS(1024) ZR_DS(2048) ZR_DS(3072) ZR_DS(3840) ZR_DS(4096)
 #undef ZR_DS
     default: return serr(ZR_ERR_UNSUPPORTED, "dose height %d", s->Hp);

@@ -19,59 +19,69 @@
 
 // --------------------------------------------------------------------- K1
 // Row pass: real FFT of length WP (as a complex FFT of WP/2), kept columns
-// kx in [0, Wh).  One block = RB consecutive rows of one frame.
-template <int WP, int RB, int NT>
-__global__ __launch_bounds__(NT) void k1_rows(const float* __restrict__ frames, int H, int W, int Hp, int Wh,
-                                              int frame0, const float2* __restrict__ twL,
-                                              const float2* __restrict__ tw2L, float2* __restrict__ T,
-                                              double* __restrict__ part) {
-  constexpr int L = WP / 2;
+// kx in [0, Wh).  One block = RB consecutive rows of one frame; each row is
+// transformed by its own group of TPR threads (the groups share the block's
+// barriers, so the RB transforms proceed together).  Every thread issues all of
+// its row loads before the first use.
+template <int WP, int RB, int TPR>
+__global__ __launch_bounds__(RB * TPR) void k1_rows(const float* __restrict__ frames, int H, int W, int Hp, int Wh,
+                                                    int frame0, const float2* __restrict__ twL,
+                                                    const float2* __restrict__ tw2L, float2* __restrict__ T,
+                                                    double* __restrict__ part) {
+  constexpr int L = WP / 2, NT = RB * TPR, PER = L / TPR;
+  static_assert(L % TPR == 0, "row length must split evenly over the row group");
   extern __shared__ float2 k1_lds[];
   float2 (*buf)[L] = reinterpret_cast<float2 (*)[L]>(k1_lds);
   __shared__ double red[2][NT / 64];
   const int tid = threadIdx.x;
+  const int g = tid / TPR, gt = tid - g * TPR;     // row group, thread in group
   const int nrb = Hp / RB;
   const int blk = xcd_remap(blockIdx.x, gridDim.x);
   const int fb = blk / nrb;                 // frame within batch
   const int y0 = (blk % nrb) * RB;
   const int f = frame0 + fb;
-  double s1 = 0.0, s2 = 0.0;
+  const int y = y0 + g;
+  const float* row = frames + ((size_t)f * H + y) * W;
+  float2 v[PER];
+  if (y < H && W == WP) {
+    const float2* row2 = reinterpret_cast<const float2*>(row);
 #pragma unroll
-  for (int r = 0; r < RB; ++r) {
-    const int y = y0 + r;
-    const float* row = frames + ((size_t)f * H + y) * W;
-    for (int n = tid; n < L; n += NT) {
-      float a = 0.f, b = 0.f;
-      if (y < H) {
-        int x = 2 * n;
-        if (x < W) a = row[x];
-        if (x + 1 < W) b = row[x + 1];
-      }
-      s1 += (double)a + (double)b;
-      s2 += (double)a * a + (double)b * b;
-      buf[r][n] = make_float2(a, b);
+    for (int i = 0; i < PER; ++i) v[i] = row2[gt + i * TPR];
+  } else {
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+      const int x = 2 * (gt + i * TPR);
+      v[i].x = (y < H && x < W) ? row[x] : 0.f;
+      v[i].y = (y < H && x + 1 < W) ? row[x + 1] : 0.f;
     }
   }
-  __syncthreads();
+  double s1 = 0.0, s2 = 0.0;
 #pragma unroll
-  for (int r = 0; r < RB; ++r) wg_fft<L, -1, NT>(buf[r], twL, tid);
+  for (int i = 0; i < PER; ++i) {
+    const double a = v[i].x, b = v[i].y;
+    s1 += a + b;
+    s2 += a * a + b * b;
+    buf[g][gt + i * TPR] = v[i];
+  }
+  __syncthreads();
+  wg_fft<L, -1, TPR>(buf[g], twL, gt);
   // post-process and write transposed: T[fb][k][y0..y0+RB)
   for (int k = tid; k < Wh; k += NT) {
-    float2 w = tw2L[k];   // exp(-2*pi*i*k/WP)
+    const float2 w = tw2L[k];   // exp(-2*pi*i*k/WP)
     float2 out[RB];
 #pragma unroll
     for (int r = 0; r < RB; ++r) {
-      float2 zk = buf[r][k % L];
-      float2 zn = c_conj(buf[r][(L - k) % L]);
-      float2 e = c_scale(c_add(zk, zn), 0.5f);
-      float2 o = c_scale(c_sub(zk, zn), 0.5f);           // = i * O
+      const float2 zk = buf[r][k % L];
+      const float2 zn = c_conj(buf[r][(L - k) % L]);
+      const float2 e = c_scale(c_add(zk, zn), 0.5f);
+      const float2 o = c_scale(c_sub(zk, zn), 0.5f);           // = i * O
       // X = E + w * O,  O = -i * o
-      float2 Ok = make_float2(o.y, -o.x);
+      const float2 Ok = make_float2(o.y, -o.x);
       out[r] = c_add(e, c_mul(w, Ok));
     }
-    float2* dst = T + ((size_t)fb * Wh + k) * Hp + y0;
+    float4* dst = reinterpret_cast<float4*>(T + ((size_t)fb * Wh + k) * Hp + y0);
 #pragma unroll
-    for (int r = 0; r < RB; ++r) dst[r] = out[r];
+    for (int r = 0; r < RB; r += 2) dst[r / 2] = make_float4(out[r].x, out[r].y, out[r + 1].x, out[r + 1].y);
   }
   // frame sums (for the intensity normalisation)
   s1 = wave_sum_d(s1);
@@ -87,23 +97,33 @@ __global__ __launch_bounds__(NT) void k1_rows(const float* __restrict__ frames, 
 }
 
 // Column pass: complex FFT of length HP along y for each kept kx, then the
-// four-quadrant crop to Hc rows and the side row +Hc/2.
+// four-quadrant crop to Hc rows and the side row +Hc/2 (or all HP rows when
+// full != 0).  Loads are staged in registers before the LDS fill.
 template <int HP, int NT>
 __global__ __launch_bounds__(NT) void k1_cols(const float2* __restrict__ T, int Wh, int Hc, int frame0,
                                               const float2* __restrict__ twH, float2* __restrict__ S,
                                               float2* __restrict__ Sside, int full) {
+  constexpr int PER = (HP + NT - 1) / NT;
   __shared__ float2 buf[HP];
   const int tid = threadIdx.x;
   const int blk = xcd_remap(blockIdx.x, gridDim.x);
   const int fb = blk / Wh, k = blk % Wh;
   const float2* src = T + ((size_t)fb * Wh + k) * HP;
-  for (int n = tid; n < HP; n += NT) buf[n] = src[n];
+  float2 v[PER];
+#pragma unroll
+  for (int i = 0; i < PER; ++i)
+    if (HP % NT == 0 || tid + i * NT < HP) v[i] = src[tid + i * NT];
+#pragma unroll
+  for (int i = 0; i < PER; ++i)
+    if (HP % NT == 0 || tid + i * NT < HP) buf[tid + i * NT] = v[i];
   __syncthreads();
   wg_fft<HP, -1, NT>(buf, twH, tid);
   const int f = frame0 + fb;
   if (full) {
     float2* dst = S + ((size_t)f * Wh + k) * HP;
-    for (int n = tid; n < HP; n += NT) dst[n] = buf[n];
+#pragma unroll
+    for (int i = 0; i < PER; ++i)
+      if (HP % NT == 0 || tid + i * NT < HP) dst[tid + i * NT] = buf[tid + i * NT];
     return;
   }
   float2* dst = S + ((size_t)f * Wh + k) * Hc;
@@ -576,20 +596,24 @@ __global__ __launch_bounds__(256) void k_peaks(const float* __restrict__ windows
 
 #include <algorithm>
 
+// rows per k1_rows block for a padded width (also the granularity of the
+// frame-sum partials)
+constexpr int k1_rows_per_block(int wp) { return wp >= 3072 ? 4 : 8; }
+
 template <int WP>
 static hipError_t launch_k1_rows_t(const float* frames, int H, int W, int Hp, int Wh, int frame0, int nb,
                                    const float2* twL, const float2* tw2L, float2* T, double* part, hipStream_t st) {
-  constexpr int RB = (WP >= 4096) ? 4 : 8;
-  constexpr int NT = 256;
+  constexpr int RB = k1_rows_per_block(WP);
+  constexpr int TPR = (WP >= 2048) ? 128 : 64;
   int grid = nb * (Hp / RB);
   size_t sh = (size_t)RB * (WP / 2) * sizeof(float2);
   static bool attr = false;
   if (!attr) {
-    hipFuncSetAttribute((const void*)k1_rows<WP, RB, NT>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh);
+    hipFuncSetAttribute((const void*)k1_rows<WP, RB, TPR>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh);
     attr = true;
   }
-  hipLaunchKernelGGL((k1_rows<WP, RB, NT>), dim3(grid), dim3(NT), sh, st, frames, H, W, Hp, Wh, frame0, twL, tw2L,
-                     T, part);
+  hipLaunchKernelGGL((k1_rows<WP, RB, TPR>), dim3(grid), dim3(RB * TPR), sh, st, frames, H, W, Hp, Wh, frame0, twL,
+                     tw2L, T, part);
   return hipGetLastError();
 }
 
@@ -597,7 +621,7 @@ hipError_t zr_launch_k1_rows(int Wp, const float* frames, int H, int W, int Hp, 
                              const float2* twL, const float2* tw2L, float2* T, double* part, int* rb_out,
                              hipStream_t st) {
   switch (Wp) {
-#define ZR_CASE(X) case X: *rb_out = (X >= 4096) ? 4 : 8; return launch_k1_rows_t<X>(frames, H, W, Hp, Wh, frame0, nb, twL, tw2L, T, part, st);
+#define ZR_CASE(X) case X: *rb_out = k1_rows_per_block(X); return launch_k1_rows_t<X>(frames, H, W, Hp, Wh, frame0, nb, twL, tw2L, T, part, st);
     ZR_CASE(256) ZR_CASE(512) ZR_CASE(1024) ZR_CASE(2048) ZR_CASE(3072) ZR_CASE(3840) ZR_CASE(4096)
 #undef ZR_CASE
     default: return hipErrorInvalidValue;
@@ -606,10 +630,9 @@ hipError_t zr_launch_k1_rows(int Wp, const float* frames, int H, int W, int Hp, 
 
 hipError_t zr_launch_k1_cols(int Hp, const float2* T, int Wh, int Hc, int frame0, int nb, const float2* twH,
                              float2* S, float2* Sside, int full, hipStream_t st) {
-  constexpr int NT = 256;
   int grid = nb * Wh;
   switch (Hp) {
-#define ZR_CASE(X) case X: hipLaunchKernelGGL((k1_cols<X, NT>), dim3(grid), dim3(NT), 0, st, T, Wh, Hc, frame0, twH, S, Sside, full); return hipGetLastError();
+#define ZR_CASE(X) case X: hipLaunchKernelGGL((k1_cols<X, (X >= 2048 ? 512 : 256)>), dim3(grid), dim3(X >= 2048 ? 512 : 256), 0, st, T, Wh, Hc, frame0, twH, S, Sside, full); return hipGetLastError();
     ZR_CASE(256) ZR_CASE(512) ZR_CASE(1024) ZR_CASE(2048) ZR_CASE(3072) ZR_CASE(3840) ZR_CASE(4096)
 #undef ZR_CASE
     default: return hipErrorInvalidValue;
