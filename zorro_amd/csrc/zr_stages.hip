@@ -227,45 +227,99 @@ struct SolveArgs {
   SolveInfo* info;
 };
 
-// One block (256 threads) per axis.  Solves min sum_e w_e |p_J - p_I - b_e| over
-// frame positions p (p_0 = 0):
-//   1. IRLS on the weighted position Laplacian (Cholesky, node 0 removed);
+// One wave per axis (a 64-thread block each; blockIdx.x = axis); lane v stands
+// for frame v (N <= 64).  Solves min sum_e w_e |p_J - p_I - b_e| over frame
+// positions p (p_0 = 0):
+//   1. IRLS warm start on the weighted position Laplacian (Cholesky, node 0
+//      removed);
 //   2. Kruskal spanning tree over |residual| ascending (ties: edge index);
-//   3. network-simplex descent: move a subtree while that lowers the objective,
-//      entering the non-tree edge at the breakpoint; stop at the optimal vertex.
-// Positions are always recomputed by summing measured shifts along the tree,
-// so the returned vertex is exact.
-constexpr int L1_NT = 256;
+//   3. network-simplex descent: move the subtree below one tree edge while that
+//      lowers the objective, entering the non-tree edge at the breakpoint where
+//      the directional slope turns non-negative; stop at the optimal vertex.
+// The directional slope of every subtree comes from per-frame signed weight
+// sums accumulated over the subtree masks, so one pivot costs O(N^2 + M).
+// The returned positions are recomputed by summing measured shifts along the
+// final tree, so the vertex is exact.
+constexpr int L1_NT = 64;
+constexpr int L1_ES = MAXN + 1;          // row stride of the pair -> edge table
+constexpr int L1_DYN_MAX = 160 * 1024 - 20 * 1024;   // LDS left beside the static arrays
+
+ZR_DEV double lane_bcast_d(double v, int src) {   // src uniform
+  const unsigned long long u = __double_as_longlong(v);
+  const unsigned lo = __builtin_amdgcn_readlane((unsigned)u, src);
+  const unsigned hi = __builtin_amdgcn_readlane((unsigned)(u >> 32), src);
+  return __longlong_as_double(((unsigned long long)hi << 32) | lo);
+}
+ZR_DEV unsigned long long lane_bcast_u64(unsigned long long v, int src) {
+  const unsigned lo = __builtin_amdgcn_readlane((unsigned)v, src);
+  const unsigned hi = __builtin_amdgcn_readlane((unsigned)(v >> 32), src);
+  return ((unsigned long long)hi << 32) | lo;
+}
+ZR_DEV unsigned long long shfl_u64(unsigned long long v, int src) {
+  const unsigned lo = __shfl((unsigned)v, src, 64);
+  const unsigned hi = __shfl((unsigned)(v >> 32), src, 64);
+  return ((unsigned long long)hi << 32) | lo;
+}
+
+// doubles of the shared matrix / sort region
+inline __host__ __device__ int l1_region_doubles(int N, int M) {
+  int p2 = 1;
+  while (p2 < M) p2 <<= 1;
+  const int sortd = (3 * p2 + 1) / 2;     // double key + int value per slot
+  const int mat = N * (N + 1);
+  return mat > sortd ? mat : sortd;
+}
+
+// Subtree masks of the tree given by par (lane = node, -1 at the root and for
+// lanes >= N): sub[v] = {v} + descendants of v.  Ancestor sets by pointer
+// doubling, then one ballot per node.
+ZR_DEV unsigned long long l1_subtree_mask(int par, int N, int lane) {
+  unsigned long long anc = (par >= 0) ? (1ull << par) : 0ull;
+  int jmp = par;
+#pragma unroll
+  for (int s = 0; s < 6; ++s) {
+    const int src = jmp >= 0 ? jmp : lane;
+    const unsigned long long anc_j = shfl_u64(anc, src);
+    const int jmp_j = __shfl(jmp, src, 64);
+    if (jmp >= 0) { anc |= anc_j; jmp = jmp_j; }
+  }
+  unsigned long long mine = 0ull;
+  for (int v = 0; v < N; ++v) {
+    const unsigned long long m = __ballot(lane < N && ((anc >> v) & 1ull));
+    if (lane == v) mine = m | (1ull << v);
+  }
+  return mine;
+}
+
 __global__ __launch_bounds__(L1_NT) void k_l1_solve(SolveArgs a) {
-  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  const int lane = threadIdx.x;
   const int axis = blockIdx.x;            // 0: y (index 0), 1: x (index 1)
   const int N = a.N, M = a.M, n1 = N - 1;
   const int LD = N + 1;
-  __shared__ double g[MAXN], pv[MAXN];
-  __shared__ int tree_e[MAXN], parent[MAXN], pedge[MAXN], deg[MAXN], adj[MAXN][MAXN];
-  __shared__ int changed;
-  __shared__ unsigned long long sub[MAXN];
-  __shared__ double slope_p[MAXN], slope_m[MAXN];
-  __shared__ double red_d[4];
-  __shared__ int red_i[4];
-  // dynamic: Lm[N*LD], Bm[N*LD] (signed measured shifts per ordered pair),
-  // b[M], w[M], r[M], edges[M], in-tree flags[M]
+  __shared__ int eid[MAXN * L1_ES];       // edge of the frame pair {u, v}, -1: none
+  __shared__ double pvs[MAXN];
+  __shared__ int par_s[MAXN], ped_s[MAXN];
+  __shared__ int adj_o[MAXN + 1], adj[2 * MAXN];   // tree adjacency (offsets, edges)
+  // dynamic: region (Cholesky matrix / sort keys+values), b[M], w[M], r[M],
+  // cd[M] (breakpoints), es[M], flag[M], ci[M] (candidate / zero-residual lists)
   extern __shared__ double dyn[];
   double* Lm = dyn;
-  double* Bm = Lm + N * LD;
-  double* b = Bm + N * LD;
+  double* b = Lm + l1_region_doubles(N, M);
   double* w = b + M;
   double* r = w + M;
-  int2* es = reinterpret_cast<int2*>(r + M);
-  int* in_tree_flag = reinterpret_cast<int*>(es + M);
-  for (int i = tid; i < N * LD; i += L1_NT) Bm[i] = 0.0;
-  for (int e = tid; e < M; e += L1_NT) es[e] = a.edges[e];
+  double* cd = r + M;
+  int2* es = reinterpret_cast<int2*>(cd + M);
+  int* flag = reinterpret_cast<int*>(es + M);
+  int* ci = flag + M;
+
+  for (int k = lane; k < N * L1_ES; k += L1_NT) eid[k] = -1;
   __syncthreads();
   // ---- data
-  for (int e = tid; e < M; e += L1_NT) {
-    int2 ed = es[e];
+  for (int e = lane; e < M; e += L1_NT) {
+    const int2 ed = a.edges[e];
+    es[e] = ed;
     b[e] = (double)a.shifts[((size_t)ed.x * N + ed.y) * 2 + axis];
-    double ps = (double)a.peaksig[ed.x * N + ed.y];
+    const double ps = (double)a.peaksig[ed.x * N + ed.y];
     double we;
     switch (a.weight_mode) {
       case 0: we = logistic_d(ps, a.info->logi[0], a.info->logi[1], a.info->logi[2]); break;
@@ -274,285 +328,338 @@ __global__ __launch_bounds__(L1_NT) void k_l1_solve(SolveArgs a) {
       default: we = ps; break;
     }
     w[e] = we;
-    Bm[ed.x * LD + ed.y] = b[e];
-    Bm[ed.y * LD + ed.x] = -b[e];
+    flag[e] = 0;
+    eid[ed.x * L1_ES + ed.y] = e;
+    eid[ed.y * L1_ES + ed.x] = e;
   }
   __syncthreads();
   if (a.weight_mode == 4) {   // 'norm': (ps - min)/(max - min)
     double mn = INFINITY, mx = -INFINITY;
-    for (int e = tid; e < M; e += L1_NT) { mn = fmin(mn, w[e]); mx = fmax(mx, w[e]); }
+    for (int e = lane; e < M; e += L1_NT) { mn = fmin(mn, w[e]); mx = fmax(mx, w[e]); }
     for (int o = 32; o > 0; o >>= 1) { mn = fmin(mn, __shfl_xor(mn, o, 64)); mx = fmax(mx, __shfl_xor(mx, o, 64)); }
-    if (lane == 0) { red_d[wid] = mn; slope_p[wid] = mx; }
-    __syncthreads();
-    mn = fmin(fmin(red_d[0], red_d[1]), fmin(red_d[2], red_d[3]));
-    mx = fmax(fmax(slope_p[0], slope_p[1]), fmax(slope_p[2], slope_p[3]));
-    __syncthreads();
-    for (int e = tid; e < M; e += L1_NT) w[e] = (w[e] - mn) / (mx - mn);
+    for (int e = lane; e < M; e += L1_NT) w[e] = (w[e] - mn) / (mx - mn);
     __syncthreads();
   }
   if (axis == 0)   // errorDict['weights'] holds the weights clipped to >= 1e-6
-    for (int e = tid; e < M; e += L1_NT) a.weights[e] = fmax(w[e], 1e-6);
-  for (int k = tid; k < N; k += L1_NT) pv[k] = 0.0;
+    for (int e = lane; e < M; e += L1_NT) a.weights[e] = fmax(w[e], 1e-6);
+  if (lane < MAXN) pvs[lane] = 0.0;
   __syncthreads();
   // ---- IRLS warm start on the position Laplacian (node 0 fixed)
   double eps = 1.0;
   for (int it = 0; it < IRLS_ITERS; ++it) {
-    for (int i = tid; i < N * LD; i += L1_NT) Lm[i] = 0.0;
+    for (int k = lane; k < N * LD; k += L1_NT) Lm[k] = 0.0;
     __syncthreads();
-    for (int e = tid; e < M; e += L1_NT) {
-      int2 ed = es[e];
-      double re = pv[ed.y] - pv[ed.x] - b[e];
-      double d = fmax(w[e], 1e-12) / fmax(fabs(re), eps);
-      Lm[ed.x * LD + ed.y] = -d;   // one edge per ordered pair
+    for (int e = lane; e < M; e += L1_NT) {
+      const int2 ed = es[e];
+      const double re = pvs[ed.y] - pvs[ed.x] - b[e];
+      const double d = fmax(w[e], 1e-12) / fmax(fabs(re), eps);
+      Lm[ed.x * LD + ed.y] = -d;   // one edge per frame pair
       Lm[ed.y * LD + ed.x] = -d;
     }
     __syncthreads();
-    for (int k = tid; k < N; k += L1_NT) {
-      // diagonal = -(row sum of off-diagonals); rhs g_k = sum_j L[k][j] * Bm[k][j]
-      double sd = 0.0, gg = 0.0;
-      for (int j = 0; j < N; ++j)
-        if (j != k) { sd -= Lm[k * LD + j]; gg += Lm[k * LD + j] * Bm[k * LD + j]; }
-      Lm[k * LD + k] = sd;
-      g[k] = gg;
+    // lane k: diagonal = -(row sum of off-diagonals); rhs g_k = sum_j L[k][j] * (signed shift k -> j)
+    double gv = 0.0;
+    if (lane < N) {
+      double sd = 0.0;
+      for (int j = 0; j < N; ++j) {
+        if (j == lane) continue;
+        const double l = Lm[lane * LD + j];
+        sd -= l;
+        const int e = eid[lane * L1_ES + j];
+        if (e >= 0) gv += l * (es[e].x == lane ? b[e] : -b[e]);
+      }
+      Lm[lane * LD + lane] = sd;
     }
     __syncthreads();
-    // Cholesky (lower) of rows/cols 1..N-1, trailing updates spread over all threads
+    // Cholesky (lower) of rows/cols 1..N-1; trailing updates spread over the lanes
     for (int k = 1; k < N; ++k) {
       const double dkk = sqrt(fmax(Lm[k * LD + k], 1e-300));
+      if (lane > k && lane < N) Lm[lane * LD + k] /= dkk;
+      if (lane == k) Lm[k * LD + k] = dkk;
       __syncthreads();
-      if (tid == 0) Lm[k * LD + k] = dkk;
-      for (int i = k + 1 + tid; i < N; i += L1_NT) Lm[i * LD + k] /= dkk;
-      __syncthreads();
-      const int nt = N - k - 1;
-      for (int q = tid; q < nt * nt; q += L1_NT) {
-        int i = k + 1 + q / nt, j = k + 1 + q % nt;
-        if (j <= i) Lm[i * LD + j] -= Lm[i * LD + k] * Lm[j * LD + k];
+      const int nt = N - 1 - k;
+      const int T = nt * (nt + 1) / 2;
+      for (int q = lane; q < T; q += L1_NT) {
+        int rr = (int)((sqrtf(8.f * q + 1.f) - 1.f) * 0.5f);
+        while ((rr + 1) * (rr + 2) / 2 <= q) ++rr;
+        while (rr * (rr + 1) / 2 > q) --rr;
+        const int cc = q - rr * (rr + 1) / 2;
+        const int i = k + 1 + rr, j = k + 1 + cc;
+        Lm[i * LD + j] -= Lm[i * LD + k] * Lm[j * LD + k];
       }
       __syncthreads();
     }
-    // triangular solves in wave 0: lane j keeps y_j / p_j in a register
-    if (wid == 0) {
-      double yv = 0.0;
-      for (int i = 1; i < N; ++i) {            // L y = g
-        double t = (lane >= 1 && lane < i) ? Lm[i * LD + lane] * yv : 0.0;
-        t = wave_sum_d(t);
-        double yi = (g[i] - t) / Lm[i * LD + i];
-        if (lane == i) yv = yi;
-      }
-      double pvv = 0.0;
-      for (int i = N - 1; i >= 1; --i) {       // L^T p = y
-        double t = (lane > i && lane < N) ? Lm[lane * LD + i] * pvv : 0.0;
-        t = wave_sum_d(t);
-        double yi = __shfl(yv, i, 64);
-        double pi = (yi - t) / Lm[i * LD + i];
-        if (lane == i) pvv = pi;
-      }
-      if (lane < N) pv[lane] = (lane == 0) ? 0.0 : pvv;
+    // L y = g, then L^T p = y (lane i holds y_i, then p_i)
+    for (int j = 1; j < N; ++j) {
+      const double yj = lane_bcast_d(gv, j) / Lm[j * LD + j];
+      if (lane == j) gv = yj;
+      else if (lane > j && lane < N) gv -= Lm[lane * LD + j] * yj;
     }
+    for (int i = N - 1; i >= 1; --i) {
+      const double pi = lane_bcast_d(gv, i) / Lm[i * LD + i];
+      if (lane == i) gv = pi;
+      else if (lane >= 1 && lane < i) gv -= Lm[i * LD + lane] * pi;
+    }
+    __syncthreads();
+    if (lane < N) pvs[lane] = (lane == 0) ? 0.0 : gv;
     __syncthreads();
     eps = fmax(eps * 0.5, 1e-9);
   }
   // ---- initial tree: Kruskal over |residual| ascending (ties: edge index)
+  int used = 0;
   {
     int P2 = 1;
     while (P2 < M) P2 <<= 1;
-    double* key = Lm;                                   // reuse the matrix storage
+    double* key = Lm;
     int* val = reinterpret_cast<int*>(Lm + P2);
-    for (int i = tid; i < P2; i += L1_NT) {
+    for (int i = lane; i < P2; i += L1_NT) {
       if (i < M) {
-        int2 ed = es[i];
-        key[i] = fabs(pv[ed.y] - pv[ed.x] - b[i]);
+        const int2 ed = es[i];
+        key[i] = fabs(pvs[ed.y] - pvs[ed.x] - b[i]);
         val[i] = i;
       } else {
         key[i] = INFINITY;
         val[i] = 0x7fffffff;
       }
     }
-    for (int e = tid; e < M; e += L1_NT) in_tree_flag[e] = 0;
     __syncthreads();
     for (int kk = 2; kk <= P2; kk <<= 1)
       for (int jj = kk >> 1; jj > 0; jj >>= 1) {
-        for (int i = tid; i < P2; i += L1_NT) {
-          int l = i ^ jj;
-          if (l > i) {
-            bool up = (i & kk) == 0;
-            double ki = key[i], kl = key[l];
-            int vi = val[i], vl = val[l];
-            bool gt = (ki > kl) || (ki == kl && vi > vl);
-            if (gt == up) { key[i] = kl; key[l] = ki; val[i] = vl; val[l] = vi; }
-          }
+        for (int q = lane; q < P2 / 2; q += L1_NT) {
+          const int i = ((q & ~(jj - 1)) << 1) | (q & (jj - 1));
+          const int l = i | jj;
+          const bool up = (i & kk) == 0;
+          const double ki = key[i], kl = key[l];
+          const int vi = val[i], vl = val[l];
+          const bool gt = (ki > kl) || (ki == kl && vi > vl);
+          if (gt == up) { key[i] = kl; key[l] = ki; val[i] = vl; val[l] = vi; }
         }
         __syncthreads();
       }
-    if (tid == 0) {
-      int comp[MAXN];
-      for (int k = 0; k < N; ++k) comp[k] = k;
-      int used = 0;
-      for (int t = 0; t < M && used < n1; ++t) {
-        int e = val[t];
-        int2 ed = es[e];
-        int rx = ed.x, ry = ed.y;
-        while (comp[rx] != rx) rx = comp[rx];
-        while (comp[ry] != ry) ry = comp[ry];
-        if (rx == ry) continue;
-        comp[ry] = rx;
-        in_tree_flag[e] = 1;
-        tree_e[used++] = e;
-      }
-      if (used < n1) {   // disconnected equation graph
-        if (axis == 0) a.info->status_y = -1; else a.info->status_x = -1;
+    // union-find as per-lane component labels
+    int label = lane;
+    for (int t0 = 0; t0 < M && used < n1; t0 += L1_NT) {
+      const int myv = (t0 + lane < M) ? val[t0 + lane] : 0;
+      const int2 myed = es[myv];
+      const int cnt = min(L1_NT, M - t0);
+      for (int q = 0; q < cnt && used < n1; ++q) {
+        const int e = __builtin_amdgcn_readlane(myv, q);
+        const int x = __builtin_amdgcn_readlane(myed.x, q);
+        const int y = __builtin_amdgcn_readlane(myed.y, q);
+        const int lx = __builtin_amdgcn_readlane(label, x);
+        const int ly = __builtin_amdgcn_readlane(label, y);
+        if (lx != ly) {
+          label = (label == ly) ? lx : label;
+          if (lane == 0) flag[e] = 1;
+          ++used;
+        }
       }
     }
     __syncthreads();
   }
+  // tree from node 0: adjacency, BFS parents and exact positions (tree sums)
+  auto build_tree = [&]() {
+    int deg = 0;
+    if (lane < N)
+      for (int u = 0; u < N; ++u) {
+        const int e = eid[lane * L1_ES + u];
+        deg += (e >= 0 && flag[e]) ? 1 : 0;
+      }
+    int off = deg;                                   // inclusive scan over lanes
+    for (int o = 1; o < 64; o <<= 1) {
+      const int t = __shfl_up(off, o, 64);
+      if (lane >= o) off += t;
+    }
+    off -= deg;
+    if (lane < N) {
+      int d = off;
+      for (int u = 0; u < N; ++u) {
+        const int e = eid[lane * L1_ES + u];
+        if (e >= 0 && flag[e] && d < 2 * MAXN) adj[d++] = e;
+      }
+      adj_o[lane] = off;
+      if (lane == N - 1) adj_o[N] = off + deg;
+      par_s[lane] = (lane == 0) ? -1 : -2;
+      ped_s[lane] = -1;
+      if (lane == 0) pvs[0] = 0.0;
+    }
+    __syncthreads();
+    unsigned long long seen = 1ull;
+    for (int round = 0; round < N; ++round) {
+      bool got = false;
+      int np = -2, ne = -1;
+      double npv = 0.0;
+      if (lane < N && !((seen >> lane) & 1ull)) {
+        const int qe = min(adj_o[lane + 1], 2 * MAXN);
+        for (int q = adj_o[lane]; q < qe; ++q) {
+          const int e = adj[q];
+          const int2 ed = es[e];
+          const int u = (ed.x == lane) ? ed.y : ed.x;
+          if ((seen >> u) & 1ull) { np = u; ne = e; npv = (ed.x == u) ? pvs[u] + b[e] : pvs[u] - b[e]; got = true; break; }
+        }
+      }
+      const unsigned long long add = __ballot(got);
+      if (got) { par_s[lane] = np; ped_s[lane] = ne; pvs[lane] = npv; }
+      __syncthreads();
+      if (add == 0ull) break;
+      seen |= add;
+    }
+  };
+  build_tree();
+  if (used < n1) {   // disconnected equation graph
+    if (lane == 0) { if (axis == 0) a.info->status_y = -1; else a.info->status_x = -1; }
+  }
+  int par = (lane < N) ? par_s[lane] : -1;
+  if (par < 0) par = -1;
+  unsigned long long sub = l1_subtree_mask(par, N, lane);
   // ---- network-simplex descent on spanning trees
   int piv = 0;
   bool optimal = false;
   double wsum = 0.0;
-  for (int e = tid; e < M; e += L1_NT) wsum += w[e];
+  for (int e = lane; e < M; e += L1_NT) wsum += w[e];
   wsum = wave_sum_d(wsum);
-  if (lane == 0) red_d[wid] = wsum;
-  __syncthreads();
-  wsum = red_d[0] + red_d[1] + red_d[2] + red_d[3];
   const double tol = -1e-12 * (1.0 + wsum);
-  for (; piv <= MAX_PIVOTS; ++piv) {
-    // tree adjacency, then level-synchronous BFS from node 0 (exact positions)
-    for (int k = tid; k < N; k += L1_NT) { deg[k] = 0; parent[k] = -2; sub[k] = 0ull; }
-    __syncthreads();
-    for (int t = tid; t < n1; t += L1_NT) {
-      int e = tree_e[t];
-      int2 ed = es[e];
-      int i0 = atomicAdd(&deg[ed.x], 1);
-      adj[ed.x][i0] = t;
-      int i1 = atomicAdd(&deg[ed.y], 1);
-      adj[ed.y][i1] = t;
-    }
-    if (tid == 0) { parent[0] = -1; pv[0] = 0.0; }
-    __syncthreads();
-    for (int round = 0; round < N; ++round) {
-      if (tid == 0) changed = 0;
-      __syncthreads();
-      int np = -2, ne = -1;
-      double npv = 0.0;
-      if (tid < N && parent[tid] == -2) {
-        for (int q = 0; q < deg[tid]; ++q) {
-          int e = tree_e[adj[tid][q]];
-          int2 ed = es[e];
-          int u = (ed.x == tid) ? ed.y : ed.x;
-          if (parent[u] != -2) { np = u; ne = e; npv = (ed.x == u) ? pv[u] + b[e] : pv[u] - b[e]; break; }
-        }
+  for (; used == n1 && piv <= MAX_PIVOTS; ++piv) {
+    // residuals of the non-tree edges; zero-residual ones are listed in ci
+    int nz = 0;
+    for (int e0 = 0; e0 < M; e0 += L1_NT) {
+      const int e = e0 + lane;
+      bool z = false;
+      if (e < M) {
+        const int2 ed = es[e];
+        const double re = flag[e] ? 0.0 : pvs[ed.y] - pvs[ed.x] - b[e];
+        r[e] = re;
+        z = !flag[e] && re == 0.0;
       }
-      __syncthreads();
-      if (np != -2) { parent[tid] = np; pedge[tid] = ne; pv[tid] = npv; changed = 1; }
-      __syncthreads();
-      if (!changed) break;
-    }
-    // subtree masks: every node marks itself in all its ancestors
-    if (tid < N) {
-      unsigned long long bit = 1ull << tid;
-      int v = tid;
-      int guard = 0;
-      while (v >= 0 && guard++ <= N) { atomicOr(&sub[v], bit); v = parent[v]; }
+      const unsigned long long bz = __ballot(z);
+      if (z) ci[nz + __popcll(bz & ((1ull << lane) - 1ull))] = e;
+      nz += __popcll(bz);
     }
     __syncthreads();
-    for (int e = tid; e < M; e += L1_NT) {
-      int2 ed = es[e];
-      r[e] = in_tree_flag[e] ? 0.0 : pv[ed.y] - pv[ed.x] - b[e];
-    }
-    __syncthreads();
-    // directional slopes for moving each subtree by +-delta (waves split the nodes)
-    for (int v = 1 + wid; v < N; v += 4) {
-      double sp = 0.0, sm = 0.0;
-      const unsigned long long S = sub[v];
-      for (int e = lane; e < M; e += 64) {
-        if (in_tree_flag[e]) continue;
-        int2 ed = es[e];
-        bool iin = (S >> ed.x) & 1ull, jin = (S >> ed.y) & 1ull;
-        if (iin == jin) continue;
-        double dir = jin ? 1.0 : -1.0;
-        double re = r[e];
-        if (re == 0.0) { sp += w[e]; sm += w[e]; }
-        else {
-          double sg = re > 0 ? 1.0 : -1.0;
-          sp += w[e] * dir * sg;
-          sm -= w[e] * dir * sg;
-        }
+    // per frame: sum over incident non-tree edges of +-w sign(r) (+ at the J end)
+    double av = 0.0;
+    if (lane < N)
+      for (int u = 0; u < N; ++u) {
+        const int e = eid[lane * L1_ES + u];
+        if (e < 0 || flag[e]) continue;
+        const double re = r[e];
+        if (re == 0.0) continue;
+        const double c = re > 0.0 ? w[e] : -w[e];
+        av += (es[e].y == lane) ? c : -c;
       }
-      sp = wave_sum_d(sp);
-      sm = wave_sum_d(sm);
-      if (lane == 0) {
-        double wt = w[pedge[v]];
-        slope_p[v] = sp + wt;
-        slope_m[v] = sm + wt;
-      }
+    // subtree sums -> directional slopes of moving subtree v by +-delta
+    double A = 0.0;
+    for (int u = 0; u < N; ++u) {
+      const double au = lane_bcast_d(av, u);
+      if ((sub >> u) & 1ull) A += au;
     }
-    __syncthreads();
-    // most negative slope (scan in node order; ties keep the first)
-    double best = tol;
-    int bv = -1, bs = 0;
-    for (int v = 1; v < N; ++v) {
-      if (slope_p[v] < best) { best = slope_p[v]; bv = v; bs = 1; }
-      if (slope_m[v] < best) { best = slope_m[v]; bv = v; bs = -1; }
+    double Z = 0.0;
+    for (int t = 0; t < nz; ++t) {
+      const int e = ci[t];
+      const int2 ed = es[e];
+      if (((sub >> ed.x) & 1ull) != ((sub >> ed.y) & 1ull)) Z += w[e];
     }
-    if (bv < 0) { optimal = true; break; }
+    double bval = INFINITY;
+    int bkey = 0x7fffffff;     // 2*v + (0: +delta, 1: -delta)
+    if (lane >= 1 && lane < N) {
+      const double wt = w[ped_s[lane]];
+      const double sp = wt + A + Z, sm = wt - A + Z;
+      if (sm < sp) { bval = sm; bkey = 2 * lane + 1; } else { bval = sp; bkey = 2 * lane; }
+      if (!(bval < tol)) { bval = INFINITY; bkey = 0x7fffffff; }
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+      const double ov = __shfl_xor(bval, o, 64);
+      const int ok = __shfl_xor(bkey, o, 64);
+      if (ov < bval || (ov == bval && ok < bkey)) { bval = ov; bkey = ok; }
+    }
+    if (bkey == 0x7fffffff) { optimal = true; break; }
     if (piv == MAX_PIVOTS) break;
-    // line search over breakpoints of crossing non-tree edges (wave 0)
-    __shared__ int s_enter;
-    if (wid == 0) {
-      const unsigned long long S = sub[bv];
-      double slope = best;
-      double last = 0.0;
-      int last_e = -1;
-      int enter = -1;
-      while (slope < 0.0) {
-        double md = INFINITY;
-        int me = 0x7fffffff;
-        for (int e = lane; e < M; e += 64) {
-          if (in_tree_flag[e]) continue;
-          int2 ed = es[e];
-          bool iin = (S >> ed.x) & 1ull, jin = (S >> ed.y) & 1ull;
-          if (iin == jin) continue;
-          double dir = (jin ? 1.0 : -1.0) * bs;
-          double re = r[e];
-          if (re == 0.0 || (re > 0) == (dir > 0)) continue;   // residual grows: no breakpoint
-          double d = -re / dir;
-          if (d < last || (d == last && e <= last_e)) continue;
-          if (d < md || (d == md && e < me)) { md = d; me = e; }
+    const int bv = bkey >> 1;
+    const double bs = (bkey & 1) ? -1.0 : 1.0;
+    const unsigned long long S = lane_bcast_u64(sub, bv);
+    // line search over the breakpoints of crossing non-tree edges whose
+    // residual moves towards zero
+    int nc = 0;
+    for (int e0 = 0; e0 < M; e0 += L1_NT) {
+      const int e = e0 + lane;
+      bool cand = false;
+      double d = 0.0;
+      if (e < M && !flag[e]) {
+        const int2 ed = es[e];
+        const bool iin = (S >> ed.x) & 1ull, jin = (S >> ed.y) & 1ull;
+        if (iin != jin) {
+          const double dir = (jin ? 1.0 : -1.0) * bs;
+          const double re = r[e];
+          if (re != 0.0 && (re > 0) != (dir > 0)) { d = -re / dir; cand = true; }
         }
-        for (int o = 32; o > 0; o >>= 1) {
-          double od = __shfl_xor(md, o, 64);
-          int oe = __shfl_xor(me, o, 64);
-          if (od < md || (od == md && oe < me)) { md = od; me = oe; }
-        }
-        if (me == 0x7fffffff) break;
-        slope += 2.0 * w[me];
-        last = md;
-        last_e = me;
-        enter = me;
       }
-      if (lane == 0) s_enter = enter;
+      const unsigned long long bc = __ballot(cand);
+      if (cand) {
+        const int pos = nc + __popcll(bc & ((1ull << lane) - 1ull));
+        cd[pos] = d;
+        ci[pos] = e;
+      }
+      nc += __popcll(bc);
     }
     __syncthreads();
-    const int enter = s_enter;
+    double slope = bval, last = 0.0;
+    int last_e = -1, enter = -1;
+    while (slope < 0.0) {
+      double md = INFINITY;
+      int me = 0x7fffffff;
+      for (int t = lane; t < nc; t += L1_NT) {
+        const double d = cd[t];
+        const int e = ci[t];
+        if (d < last || (d == last && e <= last_e)) continue;
+        if (d < md || (d == md && e < me)) { md = d; me = e; }
+      }
+      for (int o = 32; o > 0; o >>= 1) {
+        const double od = __shfl_xor(md, o, 64);
+        const int oe = __shfl_xor(me, o, 64);
+        if (od < md || (od == md && oe < me)) { md = od; me = oe; }
+      }
+      if (me == 0x7fffffff) break;
+      slope += 2.0 * w[me];
+      last = md;
+      last_e = me;
+      enter = me;
+    }
     if (enter < 0) break;   // cannot improve (numerically flat)
-    if (tid == 0) {
-      int leave = pedge[bv];
-      for (int t = 0; t < n1; ++t)
-        if (tree_e[t] == leave) { tree_e[t] = enter; break; }
-      in_tree_flag[leave] = 0;
-      in_tree_flag[enter] = 1;
+    // pivot: the subtree moves by bs * last, the entering edge replaces the
+    // parent edge of bv and the path from its inner end up to bv is re-rooted
+    if ((S >> lane) & 1ull) pvs[lane] += bs * last;
+    const int2 ee = es[enter];
+    const int s_in = ((S >> ee.x) & 1ull) ? ee.x : ee.y;
+    const int s_out = (s_in == ee.x) ? ee.y : ee.x;
+    if (lane == 0) {
+      flag[ped_s[bv]] = 0;
+      flag[enter] = 1;
+      int v = s_in, pu = s_out, pe = enter;
+      for (int guard = 0; guard < N; ++guard) {
+        const int nv = par_s[v], ne = ped_s[v];
+        par_s[v] = pu;
+        ped_s[v] = pe;
+        if (v == bv) break;
+        pu = v;
+        pe = ne;
+        v = nv;
+      }
     }
     __syncthreads();
+    par = (lane < N) ? par_s[lane] : -1;
+    if (par < 0) par = -1;
+    sub = l1_subtree_mask(par, N, lane);
   }
-  __syncthreads();
-  if (tid == 0) {
+  // exact positions by tree sums along the final tree
+  build_tree();
+  if (lane == 0) {
     if (axis == 0) { a.info->pivots_y = piv; if (a.info->status_y == 0) a.info->status_y = optimal ? 1 : 2; }
     else { a.info->pivots_x = piv; if (a.info->status_x == 0) a.info->status_x = optimal ? 1 : 2; }
   }
-  for (int k = tid; k < N; k += L1_NT) a.pos[axis * N + k] = pv[k];
-  for (int e = tid; e < M; e += L1_NT) {
-    int2 ed = es[e];
-    a.resid[e * 2 + axis] = pv[ed.y] - pv[ed.x] - b[e];
+  for (int k = lane; k < N; k += L1_NT) a.pos[axis * N + k] = pvs[k];
+  for (int e = lane; e < M; e += L1_NT) {
+    const int2 ed = es[e];
+    a.resid[e * 2 + axis] = pvs[ed.y] - pvs[ed.x] - b[e];
   }
 }
 
@@ -753,51 +860,70 @@ ZR_DEV FrameShift reduce_shift(FrameShift s, int H, int W) {
 }
 
 // Shift + sums: each thread forms 4 consecutive aligned pixels of one row for
-// every frame (18 loads per frame instead of 36) and accumulates imageSum and
-// the even/odd half sums in frame order.
-constexpr int SS_PX = 4;
+// every frame and accumulates imageSum and the even/odd half sums in frame
+// order.  The per-frame shifts are reduced once per block into LDS; a frame
+// with an integer shift takes the identity 3x3 kernel (k = delta), which gives
+// exactly the rolled pixel.  Frames are processed two at a time with every
+// load of the pair issued before the first multiply.
+constexpr int SS_PX = 4, SS_UF = 2;
 __global__ __launch_bounds__(256) void k_shift_sum(const float* __restrict__ frames, const FrameShift* __restrict__ fs,
                                                   int N, int H, int W, float* __restrict__ img_sum,
                                                   float* __restrict__ even_sum, float* __restrict__ odd_sum,
                                                   float* __restrict__ aligned, double* __restrict__ rowsum) {
+  __shared__ FrameShift sfs[MAXN];
+  __shared__ double red[4];
+  for (int j = threadIdx.x; j < N; j += 256) {
+    FrameShift f = reduce_shift(fs[j], H, W);
+    if (!f.conv) {
+#pragma unroll
+      for (int i = 0; i < 9; ++i) f.k[i] = (i == 4) ? 1.f : 0.f;
+      f.conv = 1;
+    }
+    sfs[j] = f;
+  }
+  __syncthreads();
   const int x0 = (blockIdx.x * 64 + (threadIdx.x & 63)) * SS_PX;
   const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
-  __shared__ double red[4];
   double part = 0.0;
   if (y < H && x0 < W) {
     float se[SS_PX], so[SS_PX], st[SS_PX];
 #pragma unroll
     for (int i = 0; i < SS_PX; ++i) { se[i] = 0.f; so[i] = 0.f; st[i] = 0.f; }
-    for (int J = 0; J < N; ++J) {
-      const FrameShift s = reduce_shift(fs[J], H, W);
-      const RowSrc rs = row_sources(frames + (size_t)J * H * W, s, y, H, W);
-      float v[SS_PX];
-      if (!s.conv) {
+    for (int J0 = 0; J0 < N; J0 += SS_UF) {
+      float g[SS_UF][3][SS_PX + 2];
 #pragma unroll
-        for (int i = 0; i < SS_PX; ++i) v[i] = (x0 + i < W) ? rs.r[0][wrap1(x0 + i - s.ix, W)] : 0.f;
-      } else {
+      for (int u = 0; u < SS_UF; ++u) {
+        const int J = min(J0 + u, N - 1);
+        const FrameShift& s = sfs[J];
+        const RowSrc rs = row_sources(frames + (size_t)J * H * W, s, y, H, W);
         // source columns x0-1 .. x0+SS_PX (reflected at the frame edge, then rolled)
-        int cx[SS_PX + 2];
 #pragma unroll
-        for (int c = 0; c < SS_PX + 2; ++c) cx[c] = wrap1(refl1(min(x0 - 1 + c, W), W) - s.ix, W);
+        for (int c = 0; c < SS_PX + 2; ++c) {
+          const int cx = wrap1(refl1(min(x0 - 1 + c, W), W) - s.ix, W);
 #pragma unroll
-        for (int i = 0; i < SS_PX; ++i) v[i] = 0.f;
-#pragma unroll
-        for (int a = 0; a < 3; ++a) {
-          const float* row = rs.r[a];
-          float g[SS_PX + 2];
-#pragma unroll
-          for (int c = 0; c < SS_PX + 2; ++c) g[c] = row[cx[c]];
-#pragma unroll
-          for (int i = 0; i < SS_PX; ++i)      // out[x] = sum_b k[a][b] g[x+1-b]
-            v[i] += s.k[a * 3 + 0] * g[i + 2] + s.k[a * 3 + 1] * g[i + 1] + s.k[a * 3 + 2] * g[i];
+          for (int a = 0; a < 3; ++a) g[u][a][c] = rs.r[a][cx];
         }
       }
 #pragma unroll
-      for (int i = 0; i < SS_PX; ++i) {
-        st[i] += v[i];
-        if (J & 1) so[i] += v[i]; else se[i] += v[i];
-        if (aligned && x0 + i < W) aligned[((size_t)J * H + y) * W + x0 + i] = v[i];
+      for (int u = 0; u < SS_UF; ++u) {
+        const int J = J0 + u;
+        if (J < N) {
+          const FrameShift& s = sfs[J];
+          float v[SS_PX];
+#pragma unroll
+          for (int i = 0; i < SS_PX; ++i) v[i] = 0.f;
+#pragma unroll
+          for (int a = 0; a < 3; ++a)
+#pragma unroll
+            for (int i = 0; i < SS_PX; ++i)      // out[x] = sum_b k[a][b] g[x+1-b]
+              v[i] += s.k[a * 3 + 0] * g[u][a][i + 2] + s.k[a * 3 + 1] * g[u][a][i + 1] + s.k[a * 3 + 2] * g[u][a][i];
+#pragma unroll
+          for (int i = 0; i < SS_PX; ++i) {
+            st[i] += v[i];
+            if (J & 1) so[i] += v[i]; else se[i] += v[i];
+            if (aligned && x0 + i < W) aligned[((size_t)J * H + y) * W + x0 + i] = v[i];
+          }
+        }
       }
     }
 #pragma unroll
@@ -1495,7 +1621,9 @@ int zr_stage_solve(zr_stage_state* s, const float* shifts, const float* corr, co
   a.logi_fixed[0] = s->p.peaksig_thres; a.logi_fixed[1] = s->p.logistic_k; a.logi_fixed[2] = s->p.logistic_nu;
   a.bound = 0;
   a.weights = s->weights; a.resid = s->resid; a.pos = s->pos; a.info = s->info;
-  size_t dyn = sizeof(double) * (2 * (size_t)N * (N + 1) + 3 * (size_t)M) + sizeof(int2) * M + sizeof(int) * M;
+  const size_t dyn = sizeof(double) * ((size_t)l1_region_doubles(N, M) + 4 * (size_t)M) + sizeof(int2) * M +
+                     2 * sizeof(int) * M;
+  if (dyn > (size_t)L1_DYN_MAX) return serr(ZR_ERR_UNSUPPORTED, "solver workspace %zu B exceeds LDS", dyn);
   static int attr = 0;
   if ((int)dyn > attr) {
     hipFuncSetAttribute((const void*)k_l1_solve, hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn);
