@@ -97,6 +97,13 @@ int zr_xcorr(zr_movie* mv);
  * Replaces shiftsSolver + cdfLogisticCurve (zorro/zorro.py:1260-1567, :1981-2022). */
 int zr_solve(zr_movie* mv);
 
+/* Replace the correlation results with caller-supplied triangular matrices
+ * (host, row-major: shifts [N][N][2] (y, x), corr [N][N], peaksig [N][N]) so
+ * that zr_solve runs on them.  Replaces the inputs of shiftsSolver
+ * (zorro/zorro.py:1260-1261: shiftsTriMat_in, corrTriMat_in, peaksigTriMat_in);
+ * equations are the movie's pairs (triMode). */
+int zr_set_trimats(zr_movie* mv, const float* shifts, const float* corr, const float* peaksig);
+
 /* Lanczos/integer shift of every frame, imageSum and even/odd half sums;
  * aligned_out (device, [N][H][W], nullable) receives the shifted frames.
  * Replaces applyShifts (zorro/zorro.py:1734-1808) and the half sums of

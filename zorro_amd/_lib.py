@@ -58,6 +58,7 @@ SIGNATURES = {
     "zr_movie_set_frames": (C.c_int, [_P, _P, C.c_int]),
     "zr_xcorr": (C.c_int, [_P]),
     "zr_solve": (C.c_int, [_P]),
+    "zr_set_trimats": (C.c_int, [_P, _F, _F, _F]),
     "zr_apply_shifts": (C.c_int, [_P, _P]),
     "zr_frc": (C.c_int, [_P]),
     "zr_dose_filter": (C.c_int, [_P]),

@@ -413,6 +413,19 @@ extern "C" int zr_solve(zr_movie* mv) {
   return rc ? fail(rc, "%s", zr_stage_error()) : ZR_OK;
 }
 
+extern "C" int zr_set_trimats(zr_movie* mv, const float* shifts, const float* corr, const float* peaksig) {
+  if (!mv || !shifts || !corr || !peaksig) return fail(ZR_ERR_ARG, "NULL argument");
+  ZR_HIP(hipSetDevice(mv->device));
+  const size_t nn = (size_t)mv->N * mv->N;
+  ZR_HIP(hipMemcpyAsync(mv->shifts, shifts, nn * 2 * sizeof(float), hipMemcpyHostToDevice, mv->stream));
+  ZR_HIP(hipMemcpyAsync(mv->corr, corr, nn * sizeof(float), hipMemcpyHostToDevice, mv->stream));
+  ZR_HIP(hipMemcpyAsync(mv->peaksig, peaksig, nn * sizeof(float), hipMemcpyHostToDevice, mv->stream));
+  ZR_HIP(hipMemsetAsync(mv->origin, 0, nn * sizeof(float), mv->stream));
+  ZR_HIP(hipStreamSynchronize(mv->stream));   // the host arrays may go away on return
+  mv->have_xcorr = true;
+  return ZR_OK;
+}
+
 extern "C" int zr_apply_shifts(zr_movie* mv, float* aligned_out) {
   if (!mv) return fail(ZR_ERR_ARG, "NULL movie");
   if (!mv->frames) return fail(ZR_ERR_STATE, "frames not set");

@@ -24,7 +24,7 @@ constexpr int LOGI_G0 = 32, LOGI_G1 = 32, LOGI_G2 = 32;
 constexpr int LOGI_ROUNDS = 30, LOGI_MAX_ITERS = 400;
 constexpr double LOGI_K0 = 0.1, LOGI_K1 = 20.0, LOGI_N0 = 0.05, LOGI_N1 = 5.0;
 constexpr int MAXN = 64;          // frames handled by the one-wave solver
-constexpr int IRLS_ITERS = 24;
+constexpr int IRLS_ITERS = 2;
 constexpr int MAX_PIVOTS = 400;
 
 struct FrameShift {
@@ -227,22 +227,28 @@ struct SolveArgs {
   SolveInfo* info;
 };
 
-// One wave per axis (a 64-thread block each; blockIdx.x = axis); lane v stands
-// for frame v (N <= 64).  Solves min sum_e w_e |p_J - p_I - b_e| over frame
-// positions p (p_0 = 0):
+// One 256-thread block per axis (blockIdx.x).  Solves min sum_e w_e |p_J - p_I - b_e|
+// over frame positions p (p_0 = 0) exactly:
 //   1. IRLS warm start on the weighted position Laplacian (Cholesky, node 0
 //      removed);
 //   2. Kruskal spanning tree over |residual| ascending (ties: edge index);
-//   3. network-simplex descent: move the subtree below one tree edge while that
-//      lowers the objective, entering the non-tree edge at the breakpoint where
-//      the directional slope turns non-negative; stop at the optimal vertex.
-// The directional slope of every subtree comes from per-frame signed weight
-// sums accumulated over the subtree masks, so one pivot costs O(N^2 + M).
-// The returned positions are recomputed by summing measured shifts along the
-// final tree, so the vertex is exact.
-constexpr int L1_NT = 64;
-constexpr int L1_ES = MAXN + 1;          // row stride of the pair -> edge table
-constexpr int L1_DYN_MAX = 160 * 1024 - 20 * 1024;   // LDS left beside the static arrays
+//   3. dual network simplex with bound flipping.  Every non-tree edge e sits at
+//      a flow bound f_e = s_e w_e, s_e = sign(residual) (for a zero residual the
+//      remembered side).  The tree edge above subtree S carries the flow that
+//      balances the bounds crossing S; where that exceeds its weight, S moves
+//      (the directional slope is negative), passing breakpoints of crossing
+//      edges (each flips its side) until the slope turns non-negative; the last
+//      one enters the tree.  When every tree flow fits its weight the flows
+//      certify optimality, also at degenerate vertices.  A long run of
+//      zero-length pivots switches to Bland's rule (smallest edge index,
+//      single breakpoint), which cannot cycle.
+// Edges live in registers (thread t owns edges t + 256 k); frame-level work runs
+// in wave 0 with lane v standing for frame v.  Positions are recomputed at the
+// end by summing measured shifts along the final tree, so the vertex is exact.
+constexpr int L1_NT = 256;
+constexpr int L1_EPT = 8;                 // edges per thread: M <= 2048
+constexpr int L1_LDC = MAXN + 1;          // matrix row stride
+constexpr int L1_BLAND = 3;               // zero-length pivots (x N) before Bland's rule
 
 ZR_DEV double lane_bcast_d(double v, int src) {   // src uniform
   const unsigned long long u = __double_as_longlong(v);
@@ -250,172 +256,174 @@ ZR_DEV double lane_bcast_d(double v, int src) {   // src uniform
   const unsigned hi = __builtin_amdgcn_readlane((unsigned)(u >> 32), src);
   return __longlong_as_double(((unsigned long long)hi << 32) | lo);
 }
-ZR_DEV unsigned long long lane_bcast_u64(unsigned long long v, int src) {
-  const unsigned lo = __builtin_amdgcn_readlane((unsigned)v, src);
-  const unsigned hi = __builtin_amdgcn_readlane((unsigned)(v >> 32), src);
-  return ((unsigned long long)hi << 32) | lo;
-}
 ZR_DEV unsigned long long shfl_u64(unsigned long long v, int src) {
   const unsigned lo = __shfl((unsigned)v, src, 64);
   const unsigned hi = __shfl((unsigned)(v >> 32), src, 64);
   return ((unsigned long long)hi << 32) | lo;
 }
+ZR_DEV bool lex_less(double d1, int e1, double d2, int e2) { return d1 < d2 || (d1 == d2 && e1 < e2); }
 
-// doubles of the shared matrix / sort region
-inline __host__ __device__ int l1_region_doubles(int N, int M) {
-  int p2 = 1;
-  while (p2 < M) p2 <<= 1;
-  const int sortd = (3 * p2 + 1) / 2;     // double key + int value per slot
-  const int mat = N * (N + 1);
-  return mat > sortd ? mat : sortd;
-}
-
-// Subtree masks of the tree given by par (lane = node, -1 at the root and for
-// lanes >= N): sub[v] = {v} + descendants of v.  Ancestor sets by pointer
-// doubling, then one ballot per node.
-ZR_DEV unsigned long long l1_subtree_mask(int par, int N, int lane) {
-  unsigned long long anc = (par >= 0) ? (1ull << par) : 0ull;
-  int jmp = par;
-#pragma unroll
-  for (int s = 0; s < 6; ++s) {
-    const int src = jmp >= 0 ? jmp : lane;
-    const unsigned long long anc_j = shfl_u64(anc, src);
-    const int jmp_j = __shfl(jmp, src, 64);
-    if (jmp >= 0) { anc |= anc_j; jmp = jmp_j; }
-  }
-  unsigned long long mine = 0ull;
-  for (int v = 0; v < N; ++v) {
-    const unsigned long long m = __ballot(lane < N && ((anc >> v) & 1ull));
-    if (lane == v) mine = m | (1ull << v);
-  }
-  return mine;
+// dynamic LDS of k_l1_solve: matrix region (Cholesky / slope sums / sort),
+// then b[M], w[M], breakpoints[M] (double), endpoints[M], tree flags[M],
+// breakpoint edges[M] (int)
+inline __host__ __device__ size_t l1_dyn_bytes(int M) {
+  return sizeof(double) * ((size_t)MAXN * L1_LDC + 3 * (size_t)M) + 3 * sizeof(int) * (size_t)M;
 }
 
 __global__ __launch_bounds__(L1_NT) void k_l1_solve(SolveArgs a) {
-  const int lane = threadIdx.x;
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
   const int axis = blockIdx.x;            // 0: y (index 0), 1: x (index 1)
   const int N = a.N, M = a.M, n1 = N - 1;
-  const int LD = N + 1;
-  __shared__ int eid[MAXN * L1_ES];       // edge of the frame pair {u, v}, -1: none
-  __shared__ double pvs[MAXN];
-  __shared__ int par_s[MAXN], ped_s[MAXN];
-  __shared__ int adj_o[MAXN + 1], adj[2 * MAXN];   // tree adjacency (offsets, edges)
-  // dynamic: region (Cholesky matrix / sort keys+values), b[M], w[M], r[M],
-  // cd[M] (breakpoints), es[M], flag[M], ci[M] (candidate / zero-residual lists)
+  constexpr int LDC = L1_LDC;
   extern __shared__ double dyn[];
   double* Lm = dyn;
-  double* b = Lm + l1_region_doubles(N, M);
-  double* w = b + M;
-  double* r = w + M;
-  double* cd = r + M;
-  int2* es = reinterpret_cast<int2*>(cd + M);
-  int* flag = reinterpret_cast<int*>(es + M);
-  int* ci = flag + M;
+  double* b_s = Lm + MAXN * LDC;
+  double* w_s = b_s + M;
+  double* cd_s = w_s + M;                         // breakpoint list: distances
+  int* es_s = reinterpret_cast<int*>(cd_s + M);   // i | j << 8
+  int* tfl = es_s + M;
+  int* ce_s = tfl + M;                            // breakpoint list: edges
+  __shared__ double pvs[MAXN], idiag[MAXN];
+  __shared__ int t_par[MAXN], t_ped[MAXN];
+  __shared__ unsigned long long t_sub[MAXN];
+  __shared__ int s_nc, s_enter;
+  __shared__ double s_last;
+  __shared__ unsigned long long sub_s[MAXN];
+  __shared__ int par_s[MAXN], ped_s[MAXN], adj_o[MAXN + 1], adj_c[MAXN], adj[2 * MAXN];
+  __shared__ double red_d[2][4], red_w[2][4];
+  __shared__ int red_i[2][4];
+  __shared__ int s_key, s_v, s_used;
+  __shared__ double s_best;
 
-  for (int k = lane; k < N * L1_ES; k += L1_NT) eid[k] = -1;
-  __syncthreads();
-  // ---- data
-  for (int e = lane; e < M; e += L1_NT) {
-    const int2 ed = a.edges[e];
-    es[e] = ed;
-    b[e] = (double)a.shifts[((size_t)ed.x * N + ed.y) * 2 + axis];
-    const double ps = (double)a.peaksig[ed.x * N + ed.y];
-    double we;
-    switch (a.weight_mode) {
-      case 0: we = logistic_d(ps, a.info->logi[0], a.info->logi[1], a.info->logi[2]); break;
-      case 1: we = logistic_d(ps, a.logi_fixed[0], a.logi_fixed[1], a.logi_fixed[2]); break;
-      case 3: we = 1.0; break;
-      default: we = ps; break;
+  // ---- data: each thread keeps its edges in registers
+  int eij[L1_EPT];
+  double eb[L1_EPT], ew[L1_EPT], er[L1_EPT];
+  unsigned tmask = 0u, smask = 0u;      // bit k: tree edge / side +1
+#pragma unroll
+  for (int k = 0; k < L1_EPT; ++k) {
+    const int e = tid + k * L1_NT;
+    eij[k] = -1; eb[k] = 0.0; ew[k] = 0.0; er[k] = 0.0;
+    if (e < M) {
+      const int2 ed = a.edges[e];
+      const double bb = (double)a.shifts[((size_t)ed.x * N + ed.y) * 2 + axis];
+      const double ps = (double)a.peaksig[ed.x * N + ed.y];
+      double we;
+      switch (a.weight_mode) {
+        case 0: we = logistic_d(ps, a.info->logi[0], a.info->logi[1], a.info->logi[2]); break;
+        case 1: we = logistic_d(ps, a.logi_fixed[0], a.logi_fixed[1], a.logi_fixed[2]); break;
+        case 3: we = 1.0; break;
+        default: we = ps; break;
+      }
+      eij[k] = ed.x | (ed.y << 8);
+      eb[k] = bb;
+      ew[k] = we;
     }
-    w[e] = we;
-    flag[e] = 0;
-    eid[ed.x * L1_ES + ed.y] = e;
-    eid[ed.y * L1_ES + ed.x] = e;
   }
-  __syncthreads();
   if (a.weight_mode == 4) {   // 'norm': (ps - min)/(max - min)
     double mn = INFINITY, mx = -INFINITY;
-    for (int e = lane; e < M; e += L1_NT) { mn = fmin(mn, w[e]); mx = fmax(mx, w[e]); }
+#pragma unroll
+    for (int k = 0; k < L1_EPT; ++k)
+      if (eij[k] >= 0) { mn = fmin(mn, ew[k]); mx = fmax(mx, ew[k]); }
     for (int o = 32; o > 0; o >>= 1) { mn = fmin(mn, __shfl_xor(mn, o, 64)); mx = fmax(mx, __shfl_xor(mx, o, 64)); }
-    for (int e = lane; e < M; e += L1_NT) w[e] = (w[e] - mn) / (mx - mn);
+    if (lane == 0) { red_d[0][wid] = mn; red_w[0][wid] = mx; }
     __syncthreads();
+    mn = fmin(fmin(red_d[0][0], red_d[0][1]), fmin(red_d[0][2], red_d[0][3]));
+    mx = fmax(fmax(red_w[0][0], red_w[0][1]), fmax(red_w[0][2], red_w[0][3]));
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < L1_EPT; ++k)
+      if (eij[k] >= 0) ew[k] = (ew[k] - mn) / (mx - mn);
   }
-  if (axis == 0)   // errorDict['weights'] holds the weights clipped to >= 1e-6
-    for (int e = lane; e < M; e += L1_NT) a.weights[e] = fmax(w[e], 1e-6);
-  if (lane < MAXN) pvs[lane] = 0.0;
+#pragma unroll
+  for (int k = 0; k < L1_EPT; ++k) {
+    const int e = tid + k * L1_NT;
+    if (eij[k] >= 0) {
+      es_s[e] = eij[k];
+      b_s[e] = eb[k];
+      w_s[e] = ew[k];
+      tfl[e] = 0;
+      if (axis == 0) a.weights[e] = fmax(ew[k], 1e-6);   // errorDict['weights']: clipped to >= 1e-6
+    }
+  }
+  if (tid < MAXN) pvs[tid] = 0.0;
   __syncthreads();
-  // ---- IRLS warm start on the position Laplacian (node 0 fixed)
+  // ---- IRLS warm start on the position Laplacian (node 0 fixed).  Edge (i < j)
+  // stores -d in the lower triangle (j, i) and -d * b in the upper (i, j).
   double eps = 1.0;
   for (int it = 0; it < IRLS_ITERS; ++it) {
-    for (int k = lane; k < N * LD; k += L1_NT) Lm[k] = 0.0;
+    for (int q = tid; q < N * LDC; q += L1_NT) Lm[q] = 0.0;
     __syncthreads();
-    for (int e = lane; e < M; e += L1_NT) {
-      const int2 ed = es[e];
-      const double re = pvs[ed.y] - pvs[ed.x] - b[e];
-      const double d = fmax(w[e], 1e-12) / fmax(fabs(re), eps);
-      Lm[ed.x * LD + ed.y] = -d;   // one edge per frame pair
-      Lm[ed.y * LD + ed.x] = -d;
+#pragma unroll
+    for (int k = 0; k < L1_EPT; ++k) {
+      if (eij[k] < 0) continue;
+      const int i = eij[k] & 255, j = eij[k] >> 8;
+      const double re = pvs[j] - pvs[i] - eb[k];
+      const double d = fmax(ew[k], 1e-12) / fmax(fabs(re), eps);
+      Lm[j * LDC + i] = -d;
+      Lm[i * LDC + j] = -d * eb[k];
     }
     __syncthreads();
-    // lane k: diagonal = -(row sum of off-diagonals); rhs g_k = sum_j L[k][j] * (signed shift k -> j)
-    double gv = 0.0;
-    if (lane < N) {
+    double gv = 0.0;     // wave 0, lane k: right-hand side of frame k
+    if (tid < N) {
       double sd = 0.0;
       for (int j = 0; j < N; ++j) {
-        if (j == lane) continue;
-        const double l = Lm[lane * LD + j];
-        sd -= l;
-        const int e = eid[lane * L1_ES + j];
-        if (e >= 0) gv += l * (es[e].x == lane ? b[e] : -b[e]);
+        if (j < tid) { sd -= Lm[tid * LDC + j]; gv -= Lm[j * LDC + tid]; }
+        else if (j > tid) { sd -= Lm[j * LDC + tid]; gv += Lm[tid * LDC + j]; }
       }
-      Lm[lane * LD + lane] = sd;
+      Lm[tid * LDC + tid] = sd;
     }
     __syncthreads();
-    // Cholesky (lower) of rows/cols 1..N-1; trailing updates spread over the lanes
+    // Cholesky (lower) of rows/cols 1..N-1; the factor's diagonal goes to diag[]
     for (int k = 1; k < N; ++k) {
-      const double dkk = sqrt(fmax(Lm[k * LD + k], 1e-300));
-      if (lane > k && lane < N) Lm[lane * LD + k] /= dkk;
-      if (lane == k) Lm[k * LD + k] = dkk;
+      const double ik = 1.0 / sqrt(fmax(Lm[k * LDC + k], 1e-300));
+      if (tid == 0) idiag[k] = ik;
+      for (int i = k + 1 + tid; i < N; i += L1_NT) Lm[i * LDC + k] *= ik;
       __syncthreads();
       const int nt = N - 1 - k;
       const int T = nt * (nt + 1) / 2;
-      for (int q = lane; q < T; q += L1_NT) {
+      for (int q = tid; q < T; q += L1_NT) {
         int rr = (int)((sqrtf(8.f * q + 1.f) - 1.f) * 0.5f);
         while ((rr + 1) * (rr + 2) / 2 <= q) ++rr;
         while (rr * (rr + 1) / 2 > q) --rr;
-        const int cc = q - rr * (rr + 1) / 2;
-        const int i = k + 1 + rr, j = k + 1 + cc;
-        Lm[i * LD + j] -= Lm[i * LD + k] * Lm[j * LD + k];
+        const int i = k + 1 + rr, j = k + 1 + (q - rr * (rr + 1) / 2);
+        Lm[i * LDC + j] -= Lm[i * LDC + k] * Lm[j * LDC + k];
       }
       __syncthreads();
     }
-    // L y = g, then L^T p = y (lane i holds y_i, then p_i)
-    for (int j = 1; j < N; ++j) {
-      const double yj = lane_bcast_d(gv, j) / Lm[j * LD + j];
-      if (lane == j) gv = yj;
-      else if (lane > j && lane < N) gv -= Lm[lane * LD + j] * yj;
+    // L y = g, then L^T p = y (wave 0; lane i holds y_i, then p_i)
+    if (wid == 0) {
+      for (int j = 1; j < N; ++j) {
+        const double yj = lane_bcast_d(gv, j) * idiag[j];
+        if (lane == j) gv = yj;
+        else if (lane > j && lane < N) gv -= Lm[lane * LDC + j] * yj;
+      }
+      for (int i = N - 1; i >= 1; --i) {
+        const double pi = lane_bcast_d(gv, i) * idiag[i];
+        if (lane == i) gv = pi;
+        else if (lane >= 1 && lane < i) gv -= Lm[i * LDC + lane] * pi;
+      }
+      if (lane < N) pvs[lane] = (lane == 0) ? 0.0 : gv;
     }
-    for (int i = N - 1; i >= 1; --i) {
-      const double pi = lane_bcast_d(gv, i) / Lm[i * LD + i];
-      if (lane == i) gv = pi;
-      else if (lane >= 1 && lane < i) gv -= Lm[i * LD + lane] * pi;
-    }
-    __syncthreads();
-    if (lane < N) pvs[lane] = (lane == 0) ? 0.0 : gv;
     __syncthreads();
     eps = fmax(eps * 0.5, 1e-9);
   }
+  // initial sides from the warm-start residuals
+#pragma unroll
+  for (int k = 0; k < L1_EPT; ++k) {
+    if (eij[k] < 0) continue;
+    const int i = eij[k] & 255, j = eij[k] >> 8;
+    if (pvs[j] - pvs[i] - eb[k] >= 0.0) smask |= 1u << k;
+  }
   // ---- initial tree: Kruskal over |residual| ascending (ties: edge index)
-  int used = 0;
   {
     int P2 = 1;
     while (P2 < M) P2 <<= 1;
     double* key = Lm;
     int* val = reinterpret_cast<int*>(Lm + P2);
-    for (int i = lane; i < P2; i += L1_NT) {
+    for (int i = tid; i < P2; i += L1_NT) {
       if (i < M) {
-        const int2 ed = es[i];
-        key[i] = fabs(pvs[ed.y] - pvs[ed.x] - b[i]);
+        const int ij = es_s[i];
+        key[i] = fabs(pvs[ij >> 8] - pvs[ij & 255] - b_s[i]);
         val[i] = i;
       } else {
         key[i] = INFINITY;
@@ -425,7 +433,7 @@ __global__ __launch_bounds__(L1_NT) void k_l1_solve(SolveArgs a) {
     __syncthreads();
     for (int kk = 2; kk <= P2; kk <<= 1)
       for (int jj = kk >> 1; jj > 0; jj >>= 1) {
-        for (int q = lane; q < P2 / 2; q += L1_NT) {
+        for (int q = tid; q < P2 / 2; q += L1_NT) {
           const int i = ((q & ~(jj - 1)) << 1) | (q & (jj - 1));
           const int l = i | jj;
           const bool up = (i & kk) == 0;
@@ -436,230 +444,332 @@ __global__ __launch_bounds__(L1_NT) void k_l1_solve(SolveArgs a) {
         }
         __syncthreads();
       }
-    // union-find as per-lane component labels
-    int label = lane;
-    for (int t0 = 0; t0 < M && used < n1; t0 += L1_NT) {
-      const int myv = (t0 + lane < M) ? val[t0 + lane] : 0;
-      const int2 myed = es[myv];
-      const int cnt = min(L1_NT, M - t0);
-      for (int q = 0; q < cnt && used < n1; ++q) {
-        const int e = __builtin_amdgcn_readlane(myv, q);
-        const int x = __builtin_amdgcn_readlane(myed.x, q);
-        const int y = __builtin_amdgcn_readlane(myed.y, q);
-        const int lx = __builtin_amdgcn_readlane(label, x);
-        const int ly = __builtin_amdgcn_readlane(label, y);
-        if (lx != ly) {
-          label = (label == ly) ? lx : label;
-          if (lane == 0) flag[e] = 1;
-          ++used;
+    if (wid == 0) {   // union-find as per-lane component labels
+      int label = lane, used = 0;
+      for (int t0 = 0; t0 < M && used < n1; t0 += 64) {
+        const int myv = (t0 + lane < M) ? val[t0 + lane] : 0;
+        const int myij = es_s[myv];
+        const int cnt = min(64, M - t0);
+        for (int q = 0; q < cnt && used < n1; ++q) {
+          const int e = __builtin_amdgcn_readlane(myv, q);
+          const int ij = __builtin_amdgcn_readlane(myij, q);
+          const int lx = __builtin_amdgcn_readlane(label, ij & 255);
+          const int ly = __builtin_amdgcn_readlane(label, ij >> 8);
+          if (lx != ly) {
+            label = (label == ly) ? lx : label;
+            if (lane == 0) tfl[e] = 1;
+            ++used;
+          }
         }
       }
+      if (lane == 0) s_used = used;
     }
     __syncthreads();
   }
-  // tree from node 0: adjacency, BFS parents and exact positions (tree sums)
+  const bool connected = (s_used == n1);
+#pragma unroll
+  for (int k = 0; k < L1_EPT; ++k)
+    if (eij[k] >= 0 && tfl[tid + k * L1_NT]) tmask |= 1u << k;
+
+  // tree from node 0: adjacency lists, BFS parents, exact positions (tree sums)
   auto build_tree = [&]() {
-    int deg = 0;
-    if (lane < N)
-      for (int u = 0; u < N; ++u) {
-        const int e = eid[lane * L1_ES + u];
-        deg += (e >= 0 && flag[e]) ? 1 : 0;
+    if (tid < MAXN) adj_c[tid] = 0;
+    __syncthreads();
+    for (int e = tid; e < M; e += L1_NT)
+      if (tfl[e]) { atomicAdd(&adj_c[es_s[e] & 255], 1); atomicAdd(&adj_c[es_s[e] >> 8], 1); }
+    __syncthreads();
+    if (wid == 0) {
+      const int deg = (lane < N) ? adj_c[lane] : 0;
+      int off = deg;
+      for (int o = 1; o < 64; o <<= 1) {
+        const int t = __shfl_up(off, o, 64);
+        if (lane >= o) off += t;
       }
-    int off = deg;                                   // inclusive scan over lanes
-    for (int o = 1; o < 64; o <<= 1) {
-      const int t = __shfl_up(off, o, 64);
-      if (lane >= o) off += t;
-    }
-    off -= deg;
-    if (lane < N) {
-      int d = off;
-      for (int u = 0; u < N; ++u) {
-        const int e = eid[lane * L1_ES + u];
-        if (e >= 0 && flag[e] && d < 2 * MAXN) adj[d++] = e;
-      }
-      adj_o[lane] = off;
+      off -= deg;
+      if (lane < N) { adj_o[lane] = off; adj_c[lane] = off; }
       if (lane == N - 1) adj_o[N] = off + deg;
-      par_s[lane] = (lane == 0) ? -1 : -2;
-      ped_s[lane] = -1;
-      if (lane == 0) pvs[0] = 0.0;
     }
     __syncthreads();
-    unsigned long long seen = 1ull;
-    for (int round = 0; round < N; ++round) {
-      bool got = false;
-      int np = -2, ne = -1;
-      double npv = 0.0;
-      if (lane < N && !((seen >> lane) & 1ull)) {
-        const int qe = min(adj_o[lane + 1], 2 * MAXN);
-        for (int q = adj_o[lane]; q < qe; ++q) {
-          const int e = adj[q];
-          const int2 ed = es[e];
-          const int u = (ed.x == lane) ? ed.y : ed.x;
-          if ((seen >> u) & 1ull) { np = u; ne = e; npv = (ed.x == u) ? pvs[u] + b[e] : pvs[u] - b[e]; got = true; break; }
-        }
+    for (int e = tid; e < M; e += L1_NT)
+      if (tfl[e]) {
+        const int p0 = atomicAdd(&adj_c[es_s[e] & 255], 1);
+        if (p0 < 2 * MAXN) adj[p0] = e;
+        const int p1 = atomicAdd(&adj_c[es_s[e] >> 8], 1);
+        if (p1 < 2 * MAXN) adj[p1] = e;
       }
-      const unsigned long long add = __ballot(got);
-      if (got) { par_s[lane] = np; ped_s[lane] = ne; pvs[lane] = npv; }
-      __syncthreads();
-      if (add == 0ull) break;
-      seen |= add;
+    __syncthreads();
+    if (wid == 0) {
+      if (lane < N) { par_s[lane] = -1; ped_s[lane] = -1; }
+      if (lane == 0) pvs[0] = 0.0;
+      unsigned long long seen = 1ull;
+      for (int round = 0; round < N; ++round) {
+        bool got = false;
+        int np = -1, ne = -1;
+        double npv = 0.0;
+        if (lane < N && !((seen >> lane) & 1ull)) {
+          const int qe = min(adj_o[lane + 1], 2 * MAXN);
+          for (int q = adj_o[lane]; q < qe; ++q) {
+            const int e = adj[q];
+            const int ij = es_s[e];
+            const int u = ((ij & 255) == lane) ? (ij >> 8) : (ij & 255);
+            if ((seen >> u) & 1ull) {
+              np = u; ne = e;
+              npv = ((ij & 255) == u) ? pvs[u] + b_s[e] : pvs[u] - b_s[e];
+              got = true;
+              break;
+            }
+          }
+        }
+        const unsigned long long add = __ballot(got);
+        if (got) { par_s[lane] = np; ped_s[lane] = ne; pvs[lane] = npv; }
+        if (add == 0ull) break;
+        seen |= add;
+      }
     }
+    __syncthreads();
+  };
+  // subtree masks sub[v] = {v} + descendants (wave 0): ancestor sets by
+  // pointer doubling, then one ballot per frame
+  auto subtree_masks = [&]() {
+    if (wid == 0) {
+      const int par = (lane < N) ? par_s[lane] : -1;
+      unsigned long long anc = (par >= 0) ? (1ull << par) : 0ull;
+      int jmp = par;
+#pragma unroll
+      for (int s = 0; s < 6; ++s) {
+        const int src = jmp >= 0 ? jmp : lane;
+        const unsigned long long anc_j = shfl_u64(anc, src);
+        const int jmp_j = __shfl(jmp, src, 64);
+        if (jmp >= 0) { anc |= anc_j; jmp = jmp_j; }
+      }
+      for (int v = 0; v < N; ++v) {
+        const unsigned long long m = __ballot(lane < N && ((anc >> v) & 1ull));
+        if (lane == v) sub_s[v] = m | (1ull << v);
+      }
+    }
+    __syncthreads();
   };
   build_tree();
-  if (used < n1) {   // disconnected equation graph
-    if (lane == 0) { if (axis == 0) a.info->status_y = -1; else a.info->status_x = -1; }
-  }
-  int par = (lane < N) ? par_s[lane] : -1;
-  if (par < 0) par = -1;
-  unsigned long long sub = l1_subtree_mask(par, N, lane);
-  // ---- network-simplex descent on spanning trees
-  int piv = 0;
-  bool optimal = false;
+  subtree_masks();
+  if (!connected && tid == 0) { if (axis == 0) a.info->status_y = -1; else a.info->status_x = -1; }
+
+  // ---- dual network simplex
   double wsum = 0.0;
-  for (int e = lane; e < M; e += L1_NT) wsum += w[e];
+#pragma unroll
+  for (int k = 0; k < L1_EPT; ++k) wsum += ew[k];
   wsum = wave_sum_d(wsum);
+  if (lane == 0) red_d[0][wid] = wsum;
+  __syncthreads();
+  wsum = (red_d[0][0] + red_d[0][1]) + (red_d[0][2] + red_d[0][3]);
   const double tol = -1e-12 * (1.0 + wsum);
-  for (; used == n1 && piv <= MAX_PIVOTS; ++piv) {
-    // residuals of the non-tree edges; zero-residual ones are listed in ci
-    int nz = 0;
-    for (int e0 = 0; e0 < M; e0 += L1_NT) {
-      const int e = e0 + lane;
-      bool z = false;
-      if (e < M) {
-        const int2 ed = es[e];
-        const double re = flag[e] ? 0.0 : pvs[ed.y] - pvs[ed.x] - b[e];
-        r[e] = re;
-        z = !flag[e] && re == 0.0;
+  for (int q = tid; q < N * LDC; q += L1_NT) Lm[q] = 0.0;   // frame pairs without an edge stay 0
+  __syncthreads();
+  int piv = 0, streak = 0;
+  bool optimal = false;
+  for (; connected && piv <= MAX_PIVOTS; ++piv) {
+    const bool bland = streak > L1_BLAND * N;
+    // flow bounds of the non-tree edges: C[i][j] = +w s (into j), C[j][i] = -w s
+    if (tid == 0) s_nc = 0;
+#pragma unroll
+    for (int k = 0; k < L1_EPT; ++k) {
+      if (eij[k] < 0) continue;
+      const int i = eij[k] & 255, j = eij[k] >> 8;
+      if ((tmask >> k) & 1u) {
+        er[k] = 0.0;
+        Lm[i * LDC + j] = 0.0;
+        Lm[j * LDC + i] = 0.0;
+      } else {
+        const double r = pvs[j] - pvs[i] - eb[k];
+        er[k] = r;
+        const double s = r > 0.0 ? 1.0 : (r < 0.0 ? -1.0 : (((smask >> k) & 1u) ? 1.0 : -1.0));
+        Lm[i * LDC + j] = ew[k] * s;
+        Lm[j * LDC + i] = -ew[k] * s;
       }
-      const unsigned long long bz = __ballot(z);
-      if (z) ci[nz + __popcll(bz & ((1ull << lane) - 1ull))] = e;
-      nz += __popcll(bz);
     }
     __syncthreads();
-    // per frame: sum over incident non-tree edges of +-w sign(r) (+ at the J end)
-    double av = 0.0;
-    if (lane < N)
+    // slopes of moving each subtree by +-delta: w(tree edge) +- (bound flow into the subtree)
+    if (wid == 0) {
+      double av = 0.0;
+      if (lane < N)
+        for (int x = 0; x < N; ++x) av += Lm[x * LDC + lane];
+      const unsigned long long sb = (lane < N) ? sub_s[lane] : 0ull;
+      double A = 0.0;
       for (int u = 0; u < N; ++u) {
-        const int e = eid[lane * L1_ES + u];
-        if (e < 0 || flag[e]) continue;
-        const double re = r[e];
-        if (re == 0.0) continue;
-        const double c = re > 0.0 ? w[e] : -w[e];
-        av += (es[e].y == lane) ? c : -c;
+        const double au = lane_bcast_d(av, u);
+        if ((sb >> u) & 1ull) A += au;
       }
-    // subtree sums -> directional slopes of moving subtree v by +-delta
-    double A = 0.0;
-    for (int u = 0; u < N; ++u) {
-      const double au = lane_bcast_d(av, u);
-      if ((sub >> u) & 1ull) A += au;
-    }
-    double Z = 0.0;
-    for (int t = 0; t < nz; ++t) {
-      const int e = ci[t];
-      const int2 ed = es[e];
-      if (((sub >> ed.x) & 1ull) != ((sub >> ed.y) & 1ull)) Z += w[e];
-    }
-    double bval = INFINITY;
-    int bkey = 0x7fffffff;     // 2*v + (0: +delta, 1: -delta)
-    if (lane >= 1 && lane < N) {
-      const double wt = w[ped_s[lane]];
-      const double sp = wt + A + Z, sm = wt - A + Z;
-      if (sm < sp) { bval = sm; bkey = 2 * lane + 1; } else { bval = sp; bkey = 2 * lane; }
-      if (!(bval < tol)) { bval = INFINITY; bkey = 0x7fffffff; }
-    }
-    for (int o = 32; o > 0; o >>= 1) {
-      const double ov = __shfl_xor(bval, o, 64);
-      const int ok = __shfl_xor(bkey, o, 64);
-      if (ov < bval || (ov == bval && ok < bkey)) { bval = ov; bkey = ok; }
-    }
-    if (bkey == 0x7fffffff) { optimal = true; break; }
-    if (piv == MAX_PIVOTS) break;
-    const int bv = bkey >> 1;
-    const double bs = (bkey & 1) ? -1.0 : 1.0;
-    const unsigned long long S = lane_bcast_u64(sub, bv);
-    // line search over the breakpoints of crossing non-tree edges whose
-    // residual moves towards zero
-    int nc = 0;
-    for (int e0 = 0; e0 < M; e0 += L1_NT) {
-      const int e = e0 + lane;
-      bool cand = false;
-      double d = 0.0;
-      if (e < M && !flag[e]) {
-        const int2 ed = es[e];
-        const bool iin = (S >> ed.x) & 1ull, jin = (S >> ed.y) & 1ull;
-        if (iin != jin) {
-          const double dir = (jin ? 1.0 : -1.0) * bs;
-          const double re = r[e];
-          if (re != 0.0 && (re > 0) != (dir > 0)) { d = -re / dir; cand = true; }
+      double bval = INFINITY;
+      int bkey = 0x7fffffff;
+      if (lane >= 1 && lane < N) {
+        const int pe = ped_s[lane];
+        const double wt = w_s[pe];
+        const double sp = wt + A, sm = wt - A;
+        const bool up = !(sm < sp);
+        const double v = up ? sp : sm;
+        if (v < tol) {
+          if (bland) { bval = 0.0; bkey = 2 * pe + (up ? 0 : 1); }
+          else { bval = v; bkey = 2 * lane + (up ? 0 : 1); }
         }
       }
-      const unsigned long long bc = __ballot(cand);
-      if (cand) {
-        const int pos = nc + __popcll(bc & ((1ull << lane) - 1ull));
-        cd[pos] = d;
-        ci[pos] = e;
-      }
-      nc += __popcll(bc);
-    }
-    __syncthreads();
-    double slope = bval, last = 0.0;
-    int last_e = -1, enter = -1;
-    while (slope < 0.0) {
-      double md = INFINITY;
-      int me = 0x7fffffff;
-      for (int t = lane; t < nc; t += L1_NT) {
-        const double d = cd[t];
-        const int e = ci[t];
-        if (d < last || (d == last && e <= last_e)) continue;
-        if (d < md || (d == md && e < me)) { md = d; me = e; }
-      }
+      int bvv = lane;
       for (int o = 32; o > 0; o >>= 1) {
-        const double od = __shfl_xor(md, o, 64);
-        const int oe = __shfl_xor(me, o, 64);
-        if (od < md || (od == md && oe < me)) { md = od; me = oe; }
+        const double ov = __shfl_xor(bval, o, 64);
+        const int ok = __shfl_xor(bkey, o, 64);
+        const int ovv = __shfl_xor(bvv, o, 64);
+        if (ov < bval || (ov == bval && ok < bkey)) { bval = ov; bkey = ok; bvv = ovv; }
       }
-      if (me == 0x7fffffff) break;
-      slope += 2.0 * w[me];
-      last = md;
-      last_e = me;
-      enter = me;
-    }
-    if (enter < 0) break;   // cannot improve (numerically flat)
-    // pivot: the subtree moves by bs * last, the entering edge replaces the
-    // parent edge of bv and the path from its inner end up to bv is re-rooted
-    if ((S >> lane) & 1ull) pvs[lane] += bs * last;
-    const int2 ee = es[enter];
-    const int s_in = ((S >> ee.x) & 1ull) ? ee.x : ee.y;
-    const int s_out = (s_in == ee.x) ? ee.y : ee.x;
-    if (lane == 0) {
-      flag[ped_s[bv]] = 0;
-      flag[enter] = 1;
-      int v = s_in, pu = s_out, pe = enter;
-      for (int guard = 0; guard < N; ++guard) {
-        const int nv = par_s[v], ne = ped_s[v];
-        par_s[v] = pu;
-        ped_s[v] = pe;
-        if (v == bv) break;
-        pu = v;
-        pe = ne;
-        v = nv;
+      if (lane == 0) { s_key = bkey; s_v = bvv; }
+      if (bkey != 0x7fffffff && lane == bvv) {   // slope of the chosen move (also under Bland's rule)
+        const double wt = w_s[ped_s[lane]];
+        s_best = (bkey & 1) ? wt - A : wt + A;
       }
     }
     __syncthreads();
-    par = (lane < N) ? par_s[lane] : -1;
-    if (par < 0) par = -1;
-    sub = l1_subtree_mask(par, N, lane);
+    if (s_key == 0x7fffffff) { optimal = true; break; }
+    if (piv == MAX_PIVOTS) break;
+    const int bv = s_v;
+    const double bs = (s_key & 1) ? -1.0 : 1.0;
+    const unsigned long long S = sub_s[bv];
+    const int le = ped_s[bv];
+    // breakpoints: crossing non-tree edges moving towards their other bound
+    {
+      int nmine = 0;
+      double dk[L1_EPT];
+      int ekk[L1_EPT];
+#pragma unroll
+      for (int k = 0; k < L1_EPT; ++k) {
+        if (eij[k] < 0 || ((tmask >> k) & 1u)) continue;
+        const int i = eij[k] & 255, j = eij[k] >> 8;
+        const bool iin = (S >> i) & 1ull, jin = (S >> j) & 1ull;
+        if (iin == jin) continue;
+        const double dir = (jin ? 1.0 : -1.0) * bs;
+        const double r = er[k];
+        const double s = r > 0.0 ? 1.0 : (r < 0.0 ? -1.0 : (((smask >> k) & 1u) ? 1.0 : -1.0));
+        if (dir * s >= 0.0) continue;
+        dk[nmine] = fabs(r);
+        ekk[nmine] = tid + k * L1_NT;
+        ++nmine;
+      }
+      int incl = nmine;                       // wave prefix sum of the counts
+      for (int o = 1; o < 64; o <<= 1) {
+        const int t = __shfl_up(incl, o, 64);
+        if (lane >= o) incl += t;
+      }
+      const int tot = __shfl(incl, 63, 64);
+      int base = 0;
+      if (lane == 0 && tot > 0) base = atomicAdd(&s_nc, tot);
+      base = __shfl(base, 0, 64) + incl - nmine;
+      for (int q = 0; q < nmine; ++q) { cd_s[base + q] = dk[q]; ce_s[base + q] = ekk[q]; }
+    }
+    __syncthreads();
+    // line search (wave 0): pass breakpoints in (distance, edge) order until the
+    // slope turns non-negative; the last one passed enters
+    if (wid == 0) {
+      const int nc = s_nc;
+      double slope = s_best, last = 0.0;
+      int last_e = -1, enter = -1;
+      for (int guard = 0; guard <= nc; ++guard) {
+        double md = INFINITY;
+        int me = 0x7fffffff;
+        for (int t = lane; t < nc; t += 64) {
+          const double d = cd_s[t];
+          const int e = ce_s[t];
+          if (lex_less(last, last_e, d, e) && lex_less(d, e, md, me)) { md = d; me = e; }
+        }
+        for (int o = 32; o > 0; o >>= 1) {
+          const double od = __shfl_xor(md, o, 64);
+          const int oe = __shfl_xor(me, o, 64);
+          if (lex_less(od, oe, md, me)) { md = od; me = oe; }
+        }
+        if (me == 0x7fffffff) break;
+        slope += 2.0 * w_s[me];
+        last = md;
+        last_e = me;
+        enter = me;
+        if (slope >= 0.0 || bland) break;
+      }
+      if (lane == 0) { s_enter = enter; s_last = last; }
+    }
+    __syncthreads();
+    const int enter = s_enter;
+    const double last = s_last;
+    if (enter < 0) break;   // cannot improve (numerically flat)
+    streak = (last == 0.0) ? streak + 1 : 0;
+    // passed breakpoints flip sides; the entering edge joins the tree, the
+    // leaving edge takes the bound on the side the subtree moved to
+#pragma unroll
+    for (int k = 0; k < L1_EPT; ++k) {
+      if (eij[k] < 0) continue;
+      const int e = tid + k * L1_NT;
+      const int i = eij[k] & 255, j = eij[k] >> 8;
+      const bool jin = (S >> j) & 1ull;
+      if (e == le) {
+        tmask &= ~(1u << k);
+        const double sd = jin ? bs : -bs;
+        smask = sd > 0.0 ? (smask | (1u << k)) : (smask & ~(1u << k));
+        continue;
+      }
+      if ((tmask >> k) & 1u) continue;
+      if (e == enter) { tmask |= 1u << k; continue; }
+      const bool iin = (S >> i) & 1ull;
+      if (iin == jin) continue;
+      const double dir = (jin ? 1.0 : -1.0) * bs;
+      const double r = er[k];
+      const double s = r > 0.0 ? 1.0 : (r < 0.0 ? -1.0 : (((smask >> k) & 1u) ? 1.0 : -1.0));
+      if (dir * s >= 0.0) continue;
+      if (!lex_less(last, enter, fabs(r), e)) smask = (s < 0.0) ? (smask | (1u << k)) : (smask & ~(1u << k));
+    }
+    // tree update (wave 0, lane v = frame v).  S moves by bs * last and hangs
+    // from s_out through the entering edge; the path s_in .. bv reverses.
+    //   path node v (ancestor-or-self of s_in inside S): new parent = its child
+    //   on the path (s_out for s_in), new subtree = S minus that child's old subtree;
+    //   outside S: old ancestors of bv lose S, s_out and its ancestors gain S.
+    if (wid == 0) {
+      const int ij = es_s[enter];
+      const int s_in = ((S >> (ij & 255)) & 1ull) ? (ij & 255) : (ij >> 8);
+      const int s_out = (s_in == (ij & 255)) ? (ij >> 8) : (ij & 255);
+      const bool live = lane < N;
+      const int op = live ? par_s[lane] : -1, oe = live ? ped_s[lane] : -1;
+      const unsigned long long osub = live ? sub_s[lane] : 0ull;
+      const bool inS = live && ((S >> lane) & 1ull);
+      const bool onpath = inS && ((osub >> s_in) & 1ull);
+      if (onpath && lane != bv) { t_par[op] = lane; t_ped[op] = oe; t_sub[op] = osub; }
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      int np = op, ne = oe;
+      unsigned long long ns = osub;
+      if (onpath) {
+        if (lane == s_in) { np = s_out; ne = enter; ns = S; }
+        else { np = t_par[lane]; ne = t_ped[lane]; ns = S & ~t_sub[lane]; }
+      } else if (live && !inS) {
+        if ((osub >> bv) & 1ull) ns &= ~S;
+        if ((osub >> s_out) & 1ull) ns |= S;
+      }
+      if (live) {
+        par_s[lane] = np;
+        ped_s[lane] = ne;
+        sub_s[lane] = ns;
+        if (inS) pvs[lane] += bs * last;
+      }
+    }
+    __syncthreads();
   }
   // exact positions by tree sums along the final tree
+#pragma unroll
+  for (int k = 0; k < L1_EPT; ++k)
+    if (eij[k] >= 0) tfl[tid + k * L1_NT] = (tmask >> k) & 1u;
+  __syncthreads();
   build_tree();
-  if (lane == 0) {
+  if (tid == 0) {
     if (axis == 0) { a.info->pivots_y = piv; if (a.info->status_y == 0) a.info->status_y = optimal ? 1 : 2; }
     else { a.info->pivots_x = piv; if (a.info->status_x == 0) a.info->status_x = optimal ? 1 : 2; }
   }
-  for (int k = lane; k < N; k += L1_NT) a.pos[axis * N + k] = pvs[k];
-  for (int e = lane; e < M; e += L1_NT) {
-    const int2 ed = es[e];
-    a.resid[e * 2 + axis] = pvs[ed.y] - pvs[ed.x] - b[e];
+  for (int k = tid; k < N; k += L1_NT) a.pos[axis * N + k] = pvs[k];
+#pragma unroll
+  for (int k = 0; k < L1_EPT; ++k) {
+    if (eij[k] < 0) continue;
+    const int e = tid + k * L1_NT;
+    const int i = eij[k] & 255, j = eij[k] >> 8;
+    a.resid[e * 2 + axis] = pvs[j] - pvs[i] - eb[k];
   }
 }
 
@@ -1621,9 +1731,8 @@ int zr_stage_solve(zr_stage_state* s, const float* shifts, const float* corr, co
   a.logi_fixed[0] = s->p.peaksig_thres; a.logi_fixed[1] = s->p.logistic_k; a.logi_fixed[2] = s->p.logistic_nu;
   a.bound = 0;
   a.weights = s->weights; a.resid = s->resid; a.pos = s->pos; a.info = s->info;
-  const size_t dyn = sizeof(double) * ((size_t)l1_region_doubles(N, M) + 4 * (size_t)M) + sizeof(int2) * M +
-                     2 * sizeof(int) * M;
-  if (dyn > (size_t)L1_DYN_MAX) return serr(ZR_ERR_UNSUPPORTED, "solver workspace %zu B exceeds LDS", dyn);
+  if (M > L1_NT * L1_EPT) return serr(ZR_ERR_UNSUPPORTED, "the solver handles up to %d equations (got %d)", L1_NT * L1_EPT, M);
+  const size_t dyn = l1_dyn_bytes(M);
   static int attr = 0;
   if ((int)dyn > attr) {
     hipFuncSetAttribute((const void*)k_l1_solve, hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn);

@@ -118,6 +118,15 @@ class MoviePlan:
     def solve(self):
         L.check(self.lib.zr_solve(self._mv))
 
+    def set_trimats(self, shiftsTriMat, corrTriMat, peaksigTriMat):
+        """Use caller-supplied triangular matrices as the correlation results
+        (the inputs of the reference's shiftsSolver, ``zorro/zorro.py:1260``)."""
+        n = self.n
+        sh = np.ascontiguousarray(shiftsTriMat, np.float32).reshape(n, n, 2)
+        co = np.ascontiguousarray(corrTriMat, np.float32).reshape(n, n)
+        ps = np.ascontiguousarray(peaksigTriMat, np.float32).reshape(n, n)
+        L.check(self.lib.zr_set_trimats(self._mv, L.fptr(sh), L.fptr(co), L.fptr(ps)))
+
     def apply_shifts(self, aligned_out=None):
         ptr = C.c_void_p(aligned_out.data_ptr()) if aligned_out is not None else C.c_void_p(0)
         L.check(self.lib.zr_apply_shifts(self._mv, ptr))
