@@ -969,13 +969,13 @@ ZR_DEV FrameShift reduce_shift(FrameShift s, int H, int W) {
   return s;
 }
 
-// Shift + sums: each thread forms 4 consecutive aligned pixels of one row for
-// every frame and accumulates imageSum and the even/odd half sums in frame
-// order.  The per-frame shifts are reduced once per block into LDS; a frame
-// with an integer shift takes the identity 3x3 kernel (k = delta), which gives
-// exactly the rolled pixel.  Frames are processed two at a time with every
-// load of the pair issued before the first multiply.
-constexpr int SS_PX = 4, SS_UF = 2;
+// Shift + sums: a thread owns one column x and SS_R consecutive rows; lanes of a
+// wave own consecutive columns, so every load of the 3 x (SS_R + 2) source
+// neighbourhood is one contiguous 256-byte row segment.  Frames are summed in
+// order into imageSum and the even/odd half sums.  The per-frame shifts are
+// reduced once per block into LDS; a frame with an integer shift takes the
+// identity 3x3 kernel (k = delta), which gives exactly the rolled pixel.
+constexpr int SS_R = 8;
 __global__ __launch_bounds__(256) void k_shift_sum(const float* __restrict__ frames, const FrameShift* __restrict__ fs,
                                                   int N, int H, int W, float* __restrict__ img_sum,
                                                   float* __restrict__ even_sum, float* __restrict__ odd_sum,
@@ -992,56 +992,48 @@ __global__ __launch_bounds__(256) void k_shift_sum(const float* __restrict__ fra
     sfs[j] = f;
   }
   __syncthreads();
-  const int x0 = (blockIdx.x * 64 + (threadIdx.x & 63)) * SS_PX;
-  const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
+  const int x = blockIdx.x * 64 + (threadIdx.x & 63);
+  const int y0 = (blockIdx.y * 4 + (threadIdx.x >> 6)) * SS_R;
   double part = 0.0;
-  if (y < H && x0 < W) {
-    float se[SS_PX], so[SS_PX], st[SS_PX];
+  if (x < W && y0 < H) {
+    float st[SS_R], se[SS_R], so[SS_R];
 #pragma unroll
-    for (int i = 0; i < SS_PX; ++i) { se[i] = 0.f; so[i] = 0.f; st[i] = 0.f; }
-    for (int J0 = 0; J0 < N; J0 += SS_UF) {
-      float g[SS_UF][3][SS_PX + 2];
+    for (int i = 0; i < SS_R; ++i) { st[i] = 0.f; se[i] = 0.f; so[i] = 0.f; }
+    for (int J = 0; J < N; ++J) {
+      const FrameShift& s = sfs[J];
+      const float* f = frames + (size_t)J * H * W;
+      // source columns of x+1, x, x-1 (reflected at the frame edge, then rolled)
+      const int c0 = wrap1(refl1(x + 1, W) - s.ix, W);
+      const int c1 = wrap1(x - s.ix, W);
+      const int c2 = wrap1(refl1(x - 1, W) - s.ix, W);
+      float g[SS_R + 2][3];
 #pragma unroll
-      for (int u = 0; u < SS_UF; ++u) {
-        const int J = min(J0 + u, N - 1);
-        const FrameShift& s = sfs[J];
-        const RowSrc rs = row_sources(frames + (size_t)J * H * W, s, y, H, W);
-        // source columns x0-1 .. x0+SS_PX (reflected at the frame edge, then rolled)
-#pragma unroll
-        for (int c = 0; c < SS_PX + 2; ++c) {
-          const int cx = wrap1(refl1(min(x0 - 1 + c, W), W) - s.ix, W);
-#pragma unroll
-          for (int a = 0; a < 3; ++a) g[u][a][c] = rs.r[a][cx];
-        }
+      for (int r = 0; r < SS_R + 2; ++r) {      // source row of output row y0 - 1 + r
+        const int yy = min(y0 - 1 + r, H);
+        const float* row = f + (size_t)wrap1(refl1(yy, H) - s.iy, H) * W;
+        g[r][0] = row[c0];
+        g[r][1] = row[c1];
+        g[r][2] = row[c2];
       }
 #pragma unroll
-      for (int u = 0; u < SS_UF; ++u) {
-        const int J = J0 + u;
-        if (J < N) {
-          const FrameShift& s = sfs[J];
-          float v[SS_PX];
+      for (int i = 0; i < SS_R; ++i) {
+        // out[y] = sum_a sum_b k[a][b] g[y+1-a][x+1-b]; row y+1-a sits at g[i+2-a]
+        float v = 0.f;
 #pragma unroll
-          for (int i = 0; i < SS_PX; ++i) v[i] = 0.f;
-#pragma unroll
-          for (int a = 0; a < 3; ++a)
-#pragma unroll
-            for (int i = 0; i < SS_PX; ++i)      // out[x] = sum_b k[a][b] g[x+1-b]
-              v[i] += s.k[a * 3 + 0] * g[u][a][i + 2] + s.k[a * 3 + 1] * g[u][a][i + 1] + s.k[a * 3 + 2] * g[u][a][i];
-#pragma unroll
-          for (int i = 0; i < SS_PX; ++i) {
-            st[i] += v[i];
-            if (J & 1) so[i] += v[i]; else se[i] += v[i];
-            if (aligned && x0 + i < W) aligned[((size_t)J * H + y) * W + x0 + i] = v[i];
-          }
-        }
+        for (int a = 0; a < 3; ++a)
+          v += s.k[a * 3 + 0] * g[i + 2 - a][0] + s.k[a * 3 + 1] * g[i + 2 - a][1] + s.k[a * 3 + 2] * g[i + 2 - a][2];
+        st[i] += v;
+        if (J & 1) so[i] += v; else se[i] += v;
+        if (aligned && y0 + i < H) aligned[((size_t)J * H + y0 + i) * W + x] = v;
       }
     }
 #pragma unroll
-    for (int i = 0; i < SS_PX; ++i) {
-      if (x0 + i < W) {
-        img_sum[(size_t)y * W + x0 + i] = st[i];
-        even_sum[(size_t)y * W + x0 + i] = se[i];
-        odd_sum[(size_t)y * W + x0 + i] = so[i];
+    for (int i = 0; i < SS_R; ++i) {
+      if (y0 + i < H) {
+        const size_t o = (size_t)(y0 + i) * W + x;
+        img_sum[o] = st[i];
+        even_sum[o] = se[i];
+        odd_sum[o] = so[i];
         part += st[i];
       }
     }
@@ -1754,7 +1746,7 @@ int zr_stage_solve(zr_stage_state* s, const float* shifts, const float* corr, co
 
 int zr_stage_shifts(zr_stage_state* s, const float* frames, float* aligned_out, hipStream_t st) {
   if (!s || !s->have_solution) return serr(ZR_ERR_STATE, "shifts before solve");
-  dim3 grid((s->W + 64 * SS_PX - 1) / (64 * SS_PX), (s->H + 3) / 4);
+  dim3 grid((s->W + 63) / 64, (s->H + 4 * SS_R - 1) / (4 * SS_R));
   hipLaunchKernelGGL(k_shift_sum, grid, dim3(256), 0, st, frames, s->fs, s->N, s->H, s->W, s->img_sum, s->even,
                      s->odd, aligned_out, s->rowsum);
   ZS_HIP(hipGetLastError());
@@ -1804,7 +1796,7 @@ static void fill_dose_args(zr_stage_state* s, DoseArgs& a, const double* rowsum,
 static int dose_spectral(zr_stage_state* s, const float* frames, const float2* S, int Wk, hipStream_t st) {
   const int H = s->H, W = s->W;
   // aligned sums (imageSum, even, odd) and their total for the dose per frame
-  dim3 grid((W + 64 * SS_PX - 1) / (64 * SS_PX), (H + 3) / 4);
+  dim3 grid((W + 63) / 64, (H + 4 * SS_R - 1) / (4 * SS_R));
   hipLaunchKernelGGL(k_shift_sum, grid, dim3(256), 0, st, frames, s->fs, s->N, H, W, s->img_sum, s->even, s->odd,
                      (float*)nullptr, s->rowsum);
   ZS_HIP(hipGetLastError());
