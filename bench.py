@@ -48,6 +48,54 @@ def load_traffic():
     return None
 
 
+PAIR_KERNELS = ("k2_cols", "k2_rows")
+
+
+def pmc_summary(fetch_csv, write_csv, out_json):
+    """Per-kernel HBM bytes from two rocprofv3 --pmc runs (FETCH_SIZE, then
+    WRITE_SIZE, KiB per dispatch, made with --copy-calibration).  The plain
+    1 GiB device copy calibrates both counters; the pair transform's traffic
+    per launch goes to out_json for the roofline line."""
+    import csv
+    import collections
+
+    def load(path, counter):
+        per = collections.defaultdict(list)
+        copies = []
+        for r in csv.DictReader(open(path)):
+            if r["Counter_Name"] != counter:
+                continue
+            name = r["Kernel_Name"].split("(")[0].replace("void ", "")
+            v = float(r["Counter_Value"]) * 1024.0
+            per[name].append(v)
+            if "copyBuffer" in name:
+                copies.append(v)
+        return per, sorted(copies)[-3:]
+
+    F, fc = load(fetch_csv, "FETCH_SIZE")
+    W, wc = load(write_csv, "WRITE_SIZE")
+    gib = float(1 << 30)
+    f_scale = gib / (sum(fc) / len(fc))
+    w_scale = gib / (sum(wc) / len(wc))
+    rows = []
+    for k in sorted(F, key=lambda k: -sum(F[k]) / len(F[k])):
+        f = sum(F[k]) / len(F[k]) * f_scale
+        w = sum(W.get(k, [0.0])) / max(len(W.get(k, [])), 1) * w_scale
+        rows.append((k, len(F[k]), f, w))
+    pair = [r for r in rows if r[0].split("<")[0] in PAIR_KERNELS]
+    traffic = sum(r[2] + r[3] for r in pair)
+    res = dict(traffic_bytes=traffic, fetch_scale=f_scale, write_scale=w_scale,
+               kernels={r[0]: dict(fetch_bytes=r[2], write_bytes=r[3]) for r in pair})
+    with open(out_json, "w") as f:
+        json.dump(res, f, indent=1)
+    print(f"copy calibration: FETCH_SIZE x{f_scale:.3f}, WRITE_SIZE x{w_scale:.3f}")
+    print("| kernel | dispatches | HBM read (MB) | HBM write (MB) |")
+    print("|---|---|---|---|")
+    for k, n, f, w in rows[:20]:
+        print(f"| {k} | {n} | {f / 1e6:.1f} | {w / 1e6:.1f} |")
+    print(f"pair transform traffic per launch: {traffic / 1e9:.3f} GB")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -56,7 +104,12 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run a plain 1 GiB device copy first (reference for rocprofv3 FETCH/WRITE counters)")
+    ap.add_argument("--pmc-summary", nargs=3, metavar=("FETCH_CSV", "WRITE_CSV", "OUT_JSON"),
+                    help="summarise two rocprofv3 --pmc counter_collection.csv files and exit (no GPU)")
     args = ap.parse_args()
+    if args.pmc_summary:
+        pmc_summary(*args.pmc_summary)
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
               "(movies are independent; see DESIGN.md for how the work would shard).", file=sys.stderr)

@@ -133,7 +133,9 @@ int zr_pair_count(zr_movie* mv, int* npairs);
 /* translations f64 [N][2] (after the centroid when originMode='centroid'),
  * relative_est f64 [N-1][2], weights f64 [M], logistic f64 [3] (thr, K, nu),
  * error f64 [M][2] (errorXY), error_stats f64 [2] (mean/std errorNorm),
- * error_xy_frames f64 [N][2] (errorX, errorY).  Any pointer may be NULL. */
+ * error_xy_frames f64 [N][2] (errorX, errorY).  Any pointer may be NULL.
+ * Fails when the equation graph is disconnected or the solver could not prove
+ * optimality. */
 int zr_get_solution(zr_movie* mv, double* translations, double* relative_est, double* weights, double* logistic,
                     double* error, double* error_stats, double* error_xy_frames);
 /* Solver diagnostics: info6 = {positive peak significances, histogram bins,

@@ -25,7 +25,7 @@ constexpr int LOGI_ROUNDS = 30, LOGI_MAX_ITERS = 400;
 constexpr double LOGI_K0 = 0.1, LOGI_K1 = 20.0, LOGI_N0 = 0.05, LOGI_N1 = 5.0;
 constexpr int MAXN = 64;          // frames handled by the one-wave solver
 constexpr int IRLS_ITERS = 2;
-constexpr int MAX_PIVOTS = 400;
+constexpr int MAX_PIVOTS = 4000;
 
 struct FrameShift {
   int iy, ix, conv, pad;
@@ -1880,6 +1880,9 @@ int zr_stage_get_solution(zr_stage_state* s, double* translations, double* relat
   SolveInfo inf;
   ZS_HIP(hipMemcpy(&inf, s->info, sizeof(inf), hipMemcpyDeviceToHost));
   if (inf.status_x < 0 || inf.status_y < 0) return serr(ZR_ERR_ARG, "equation graph is disconnected");
+  if (inf.status_x != 1 || inf.status_y != 1)
+    return serr(ZR_ERR_STATE, "shift solver stopped before proving optimality (status y %d, x %d)", inf.status_y,
+                inf.status_x);
   if (s->p.weight_mode == ZR_W_AUTOLOGISTIC && inf.nbins < 1)
     return serr(ZR_ERR_ARG, "autologistic weights need >= 7 positive peak significances (got %d)", inf.nps);
   const int N = s->N, M = s->M;
