@@ -23,14 +23,18 @@ CASES = {
     "1024sq_10f_nodose": dict(n=10, h=1024, w=1024, seed=9, dose=1.5, box=512, filter=None),
     # non-square frames take the direct dose path (per-frame row FFTs of the aligned frames)
     "512x1024_8f_direct_dose": dict(n=8, h=512, w=1024, seed=13, dose=1.5, box=256),
+    # padded frames: masked correlation, crop back, symmetric-padded dose filter
+    "450x480_pad512_8f_masked": dict(n=8, h=450, w=480, seed=17, dose=1.5, box=256, pad=(512, 512)),
+    "1000x900_pad1024_crop512_10f_masked": dict(n=10, h=1000, w=900, seed=19, dose=1.5, box=512, pad=(1024, 1024),
+                                                crop=(512, 512)),
 }
 
 
 def _kw(c):
     return dict(triMode="tri", maxShift=40, Bmode="conv", Brad=150, subPixReg=16, shiftMethod="lanczos",
-                weightMode="autologistic", originMode="centroid", suppressOrigin=True, fouCrop=(0, 0),
-                shapePadded=(c["h"], c["w"]), filterMode=c.get("filter", "dose"), pixelsize=0.1, voltage=300.0, doLazyFRC=True,
-                frc_box=c["box"])
+                weightMode="autologistic", originMode="centroid", suppressOrigin=True, fouCrop=c.get("crop", (0, 0)),
+                shapePadded=c.get("pad", (c["h"], c["w"])), filterMode=c.get("filter", "dose"), pixelsize=0.1,
+                voltage=300.0, doLazyFRC=True, frc_box=c["box"])
 
 
 @pytest.fixture(scope="module", params=list(CASES))
@@ -55,7 +59,7 @@ def test_solver_matches_oracle_on_same_trimats(run):
     p = O.Params(**run["kw"])
     tm = run["tm"]
     ed = O.shifts_solver(tm["shiftsTriMat"], tm["corrTriMat"], tm["peaksigTriMat"], p,
-                         (run["c"]["h"], run["c"]["w"]))
+                         tuple(run["kw"]["shapePadded"]))
     t_or = ed["translations"] - ed["translations"].mean(axis=0)
     t_gpu = run["sol"]["translations"]
     d = np.max(np.abs(t_gpu - t_or))
@@ -86,7 +90,7 @@ def test_sums_frc_dose_match_oracle_given_translations(run):
     assert run["frc"].shape == frc.shape and dfrc <= 1e-3
     if run["kw"]["filterMode"] != "dose":
         return
-    fs, info = O.dose_filter(aligned, isum, p, (run["c"]["h"], run["c"]["w"]))
+    fs, info = O.dose_filter(aligned, isum, p, tuple(run["kw"]["shapePadded"]))
     rel = np.linalg.norm(g["filtSum"] - fs) / np.linalg.norm(fs)
     print(f"[gpu pipeline] {run['name']}: filtSum rel L2 {rel:.2e}", flush=True)
     assert rel <= 1e-4

@@ -44,7 +44,7 @@ bool smooth235(int n) {
 }
 
 bool supported_len(int n) {
-  static const int ok[] = {256, 512, 1024, 2048, 3072, 3840, 4096};
+  static const int ok[] = {256, 512, 1024, 2048, 3072, 3840, 4096, 5760};
   for (int v : ok)
     if (v == n) return true;
   return false;
@@ -145,8 +145,9 @@ struct zr_movie {
   float2 *twWp2 = nullptr, *twWp = nullptr, *twHp = nullptr, *twHc = nullptr, *twWc = nullptr, *ktab = nullptr;
   float *gy = nullptr, *gx = nullptr;
   int ry = 0, rx = 0;
-  // pairs
-  int P = 0;
+  // pairs: P frame pairs; on the masked path the pair transform also runs the
+  // 4N+1 mask-term pairs (Ptot in all), over nspec = 2N+1 spectra (F, Q, M)
+  int P = 0, Ptot = 0, nspec = 0;
   int4 *pairs = nullptr, *work = nullptr;
   std::vector<int4> pairs_host;
   // workspaces
@@ -206,6 +207,20 @@ static int build_pairs(zr_movie* mv) {
     int ta = (a.x / 8) * 1024 + a.y / 8, tb = (b.x / 8) * 1024 + b.y / 8;
     return ta < tb;
   });
+  mv->Ptot = mv->P;
+  mv->nspec = N;
+  if (!mv->unmasked) {
+    // mask-term pairs over the spectra F_k (slot k), Q_k (slot N+k), M (slot 2N):
+    // Ct_k = (F_k, M), Cb_k = (M, F_k), T_k = (Q_k, M), B_k = (M, Q_k), mp = (M, M)
+    const int P = mv->P, Ms = 2 * N;
+    for (int k = 0; k < N; ++k) wk.push_back(make_int4(k, Ms, P + k, 0));
+    for (int k = 0; k < N; ++k) wk.push_back(make_int4(Ms, k, P + N + k, 0));
+    for (int k = 0; k < N; ++k) wk.push_back(make_int4(N + k, Ms, P + 2 * N + k, 0));
+    for (int k = 0; k < N; ++k) wk.push_back(make_int4(Ms, N + k, P + 3 * N + k, 0));
+    wk.push_back(make_int4(Ms, Ms, P + 4 * N, 0));
+    mv->Ptot = P + 4 * N + 1;
+    mv->nspec = 2 * N + 1;
+  }
   int rc = dev_upload(&mv->pairs, pr);
   if (rc) return rc;
   return dev_upload(&mv->work, wk);
@@ -240,8 +255,9 @@ extern "C" int zr_movie_create(int device, const zr_params* p, int n_frames, int
   mv->Hp = p->shape_padded[0] > 0 ? p->shape_padded[0] : h;
   mv->Wp = p->shape_padded[1] > 0 ? p->shape_padded[1] : w;
   if (mv->Hp < h || mv->Wp < w) { free_movie(mv); return fail(ZR_ERR_ARG, "shapePadded (%d,%d) smaller than frames (%d,%d)", mv->Hp, mv->Wp, h, w); }
+  // padStack (zorro/zorro.py:1908-1958): frames smaller than shapePadded are
+  // zero-padded and correlated under the mask of the frame region
   mv->unmasked = (mv->Hp == h && mv->Wp == w);
-  if (!mv->unmasked) { free_movie(mv); return fail(ZR_ERR_UNSUPPORTED, "padded frames (masked correlation) are not built yet: frames %dx%d, shapePadded %dx%d", h, w, mv->Hp, mv->Wp); }
   mv->crop = (p->fou_crop[0] != 0 || p->fou_crop[1] != 0);
   if (mv->crop) {
     mv->Hc = 2 * (p->fou_crop[0] / 2);
@@ -251,7 +267,7 @@ extern "C" int zr_movie_create(int device, const zr_params* p, int n_frames, int
     mv->Hc = mv->Hp;
     mv->Wc = mv->Wp;
   }
-  if (!supported_len(mv->Wp) || !supported_len(mv->Hp)) { free_movie(mv); return fail(ZR_ERR_UNSUPPORTED, "shapePadded %dx%d: built FFT lengths are 256..4096 (2^a, 3072, 3840)", mv->Hp, mv->Wp); }
+  if (!supported_len(mv->Wp) || !supported_len(mv->Hp)) { free_movie(mv); return fail(ZR_ERR_UNSUPPORTED, "shapePadded %dx%d: built FFT lengths are 256..4096 (2^a), 3072, 3840, 5760", mv->Hp, mv->Wp); }
   {
     const int l = mv->Hc / 64;
     if (mv->Hc % 64 != 0 || !(l == 4 || l == 8 || l == 16 || l == 32 || l == 48 || l == 64)) { free_movie(mv); return fail(ZR_ERR_UNSUPPORTED, "cropped height %d: built sizes are 256, 512, 1024, 2048, 3072, 4096", mv->Hc); }
@@ -321,19 +337,29 @@ extern "C" int zr_movie_create(int device, const zr_params* p, int n_frames, int
   mv->batch = (int)std::min<size_t>(N, 8);
   mv->rb = (mv->Wp >= 3072) ? 4 : 8;   // rows per k1_rows block
   const int nrb = mv->Hp / mv->rb;
-  mv->spectral = p->do_dose && mv->unmasked && mv->H == mv->W;
+  mv->spectral = p->do_dose && mv->unmasked && mv->H == mv->W && mv->Hp <= 4096;
   mv->Wk = mv->spectral ? mv->Wp / 2 + 1 : mv->Wh;
   mv->Hs = mv->spectral ? mv->Hp : mv->Hc;
   mv->hoff = mv->spectral ? mv->Hp - mv->Hc : 0;
   mv->side_in_col = mv->spectral ? mv->Hc / 2 : -1;
+  const size_t nspec = (size_t)mv->nspec;
+  const size_t nrb_sums = (size_t)(mv->H + 7) / 8;   // k1_frame_sums row blocks
   if ((rc = dev_alloc(&mv->T, (size_t)mv->batch * mv->Wk * mv->Hp)) ||
-      (rc = dev_alloc(&mv->S, N * mv->Wk * mv->Hs)) || (rc = dev_alloc(&mv->Sside, mv->spectral ? 0 : N * mv->Wk)) ||
-      (rc = dev_alloc(&mv->Z, (size_t)mv->P * mv->Wh * 2 * mv->m)) || (rc = dev_alloc(&mv->part, N * nrb * 2)) ||
-      (rc = dev_alloc(&mv->stats, N * 2)) || (rc = dev_alloc(&mv->den, (size_t)mv->P)) ||
-      (rc = dev_alloc(&mv->windows, (size_t)mv->P * 4 * mv->m * mv->m)) || (rc = dev_alloc(&mv->shifts, N * N * 2)) ||
+      (rc = dev_alloc(&mv->S, nspec * mv->Wk * mv->Hs)) ||
+      (rc = dev_alloc(&mv->Sside, mv->spectral ? 0 : nspec * mv->Wk)) ||
+      (rc = dev_alloc(&mv->Z, (size_t)mv->Ptot * mv->Wh * 2 * mv->m)) ||
+      (rc = dev_alloc(&mv->part, N * std::max<size_t>(nrb, nrb_sums) * 2)) ||
+      (rc = dev_alloc(&mv->stats, N * 2)) || (rc = dev_alloc(&mv->den, (size_t)mv->Ptot)) ||
+      (rc = dev_alloc(&mv->windows, (size_t)mv->Ptot * 4 * mv->m * mv->m)) || (rc = dev_alloc(&mv->shifts, N * N * 2)) ||
       (rc = dev_alloc(&mv->corr, N * N)) || (rc = dev_alloc(&mv->peaksig, N * N)) || (rc = dev_alloc(&mv->origin, N * N))) {
     free_movie(mv);
     return rc;
+  }
+  if (!mv->unmasked) {
+    // the pair transform's own scale is nc2 / max(1, den): den = 0 keeps the
+    // raw nc2 * Re R[.] windows that the masked combination needs
+    e = hipMemset(mv->den, 0, sizeof(double) * mv->Ptot);
+    if (e != hipSuccess) { free_movie(mv); return fail(ZR_ERR_HIP, "hipMemset: %s", hipGetErrorString(e)); }
   }
   if ((rc = zr_stage_create(&mv->st, mv->p, mv->N, mv->H, mv->W, mv->Hp, mv->Wp, mv->pairs_host, mv->spectral ? 1 : 0))) {
     free_movie(mv);
@@ -371,23 +397,45 @@ extern "C" int zr_xcorr(zr_movie* mv) {
   hipStream_t s = mv->stream;
   int rb = 0;
   hipEvent_t t0 = mv->timer.begin(s);
-  // K1: spectra in batches of frames
-  for (int f0 = 0; f0 < mv->N; f0 += mv->batch) {
-    int nb = std::min(mv->batch, mv->N - f0);
-    ZR_HIP(zr_launch_k1_rows(mv->Wp, mv->frames, mv->H, mv->W, mv->Hp, mv->Wk, f0, nb, mv->twWp2, mv->twWp, mv->T,
-                             mv->part, &rb, s));
-    ZR_HIP(zr_launch_k1_cols(mv->Hp, mv->T, mv->Wk, mv->Hc, f0, nb, mv->twHp, mv->S, mv->Sside, mv->spectral ? 1 : 0, s));
-  }
-  ZR_HIP(zr_launch_k1_stats(mv->part, mv->Hp / rb, mv->N, mv->stats, s));
+  const int full = mv->spectral ? 1 : 0;
   const double nc2 = 1.0 / std::pow((double)mv->Hc * (double)mv->Wc, 2.0);
-  ZR_HIP(zr_launch_k2_den(mv->pairs, mv->P, mv->stats, (double)mv->Hp * (double)mv->Wp, nc2, mv->den, s));
+  if (mv->unmasked) {
+    // K1: spectra in batches of frames; frame sums on the way
+    for (int f0 = 0; f0 < mv->N; f0 += mv->batch) {
+      int nb = std::min(mv->batch, mv->N - f0);
+      ZR_HIP(zr_launch_k1_rows(mv->Wp, mv->frames, mv->H, mv->W, mv->Hp, mv->Wk, f0, nb, mv->twWp2, mv->twWp, mv->T,
+                               mv->part, 0, nullptr, &rb, s));
+      ZR_HIP(zr_launch_k1_cols(mv->Hp, mv->T, mv->Wk, mv->Hc, f0, nb, mv->twHp, mv->S, mv->Sside, full, s));
+    }
+    ZR_HIP(zr_launch_k1_stats(mv->part, mv->Hp / rb, mv->N, mv->stats, s));
+    ZR_HIP(zr_launch_k2_den(mv->pairs, mv->P, mv->stats, (double)mv->Hp * (double)mv->Wp, nc2, mv->den, s));
+  } else {
+    // masked: frame means first, then the spectra of m(f - mu) (slots 0..N),
+    // (m(f - mu))^2 (slots N..2N) and the mask m (slot 2N)
+    int nrb = 0;
+    ZR_HIP(zr_launch_k1_sums(mv->frames, mv->N, mv->H, mv->W, mv->part, &nrb, s));
+    ZR_HIP(zr_launch_k1_stats(mv->part, nrb, mv->N, mv->stats, s));
+    for (int f0 = 0; f0 < mv->N; f0 += mv->batch) {
+      int nb = std::min(mv->batch, mv->N - f0);
+      for (int sq = 0; sq < 2; ++sq) {
+        ZR_HIP(zr_launch_k1_rows(mv->Wp, mv->frames, mv->H, mv->W, mv->Hp, mv->Wk, f0, nb, mv->twWp2, mv->twWp,
+                                 mv->T, mv->part, sq ? 2 : 1, mv->stats, &rb, s));
+        ZR_HIP(zr_launch_k1_cols(mv->Hp, mv->T, mv->Wk, mv->Hc, f0 + sq * mv->N, nb, mv->twHp, mv->S, mv->Sside,
+                                 full, s));
+      }
+    }
+    ZR_HIP(zr_launch_k1_rows(mv->Wp, mv->frames, mv->H, mv->W, mv->Hp, mv->Wk, 0, 1, mv->twWp2, mv->twWp, mv->T,
+                             mv->part, 3, mv->stats, &rb, s));
+    ZR_HIP(zr_launch_k1_cols(mv->Hp, mv->T, mv->Wk, mv->Hc, 2 * mv->N, 1, mv->twHp, mv->S, mv->Sside, full, s));
+  }
   mv->timer.end(ZR_STAGE_SPECTRA, t0, s);
   // K2
   hipEvent_t t1 = mv->timer.begin(s);
-  ZR_HIP(zr_launch_k2_cols(mv->Lper, mv->S, mv->Sside, mv->work, mv->P, mv->Wh, mv->Hc, mv->Wc, mv->m, 8,
+  ZR_HIP(zr_launch_k2_cols(mv->Lper, mv->S, mv->Sside, mv->work, mv->Ptot, mv->Wh, mv->Hc, mv->Wc, mv->m, 8,
                            mv->unmasked ? 1 : 0, mv->twHc, mv->Z, mv->Wk, mv->Hs, mv->hoff, mv->side_in_col, s));
-  ZR_HIP(zr_launch_k2_rows(mv->Z, mv->pairs, mv->P, mv->Wh, mv->Wc, mv->m, mv->twWc, mv->den, (float)nc2,
+  ZR_HIP(zr_launch_k2_rows(mv->Z, mv->pairs, mv->Ptot, mv->Wh, mv->Wc, mv->m, mv->twWc, mv->den, (float)nc2,
                            mv->windows, s));
+  if (!mv->unmasked) ZR_HIP(zr_launch_mnxc_combine(mv->windows, mv->pairs, mv->P, mv->N, mv->m, s));
   mv->timer.end(ZR_STAGE_PAIRS, t1, s);
   // peaks
   hipEvent_t t2 = mv->timer.begin(s);

@@ -29,9 +29,15 @@ inline size_t zr_peaks_lds_bytes(const PeakParamsHost* hp, int* tmp_floats) {
 }
 
 // zr_xcorr.hip
+// mode: 0 frame (sums into part), 1 mean-centred masked frame, 2 its square,
+// 3 the mask (stats: per-frame (sum, sumsq) for modes 1 and 2)
 hipError_t zr_launch_k1_rows(int Wp, const float* frames, int H, int W, int Hp, int Wh, int frame0, int nb,
-                             const float2* twL, const float2* tw2L, float2* T, double* part, int* rb_out,
-                             hipStream_t st);
+                             const float2* twL, const float2* tw2L, float2* T, double* part, int mode,
+                             const double* stats, int* rb_out, hipStream_t st);
+// per-frame FP64 partial sums over H x W; nrb_out = row blocks per frame
+hipError_t zr_launch_k1_sums(const float* frames, int n, int H, int W, double* part, int* nrb_out, hipStream_t st);
+// masked NCC from the pair and mask-term windows (in place on the P pair windows)
+hipError_t zr_launch_mnxc_combine(float* windows, const int4* pairs, int P, int N, int m, hipStream_t st);
 hipError_t zr_launch_k1_cols(int Hp, const float2* T, int Wh, int Hc, int frame0, int nb, const float2* twH,
                              float2* S, float2* Sside, int full, hipStream_t st);
 hipError_t zr_launch_k1_stats(const double* part, int nrb, int n, double* stats, hipStream_t st);

@@ -1309,10 +1309,14 @@ __global__ __launch_bounds__(256) void k_dose_params(DoseArgs a) {
 
 // Column pass: per kx, accumulate FFT_y(T_J[kx]) * exp(-D_J/(2 Nc(q))) * exp(-(q/qc_J)^order)
 // over all frames, then the inverse transform along y.
+// threads of one column block: HP must split evenly over them
+constexpr int dose_cols_nt(int hp) { return hp % 256 == 0 ? 256 : 384; }
 template <int HP>
-__global__ __launch_bounds__(256) void k_dose_cols(const float2* __restrict__ T, DoseArgs a,
-                                                  const float2* __restrict__ twH, float2* __restrict__ G) {
-  constexpr int NT = 256, PER = HP / NT;
+__global__ __launch_bounds__(dose_cols_nt(HP)) void k_dose_cols(const float2* __restrict__ T, DoseArgs a,
+                                                               const float2* __restrict__ twH,
+                                                               float2* __restrict__ G) {
+  constexpr int NT = dose_cols_nt(HP), PER = HP / NT;
+  static_assert(HP % NT == 0, "column length must split evenly over the block");
   __shared__ float2 buf[HP];
   const int tid = threadIdx.x;
   const int kx = xcd_remap(blockIdx.x, gridDim.x);
@@ -1846,7 +1850,7 @@ int zr_stage_dose(zr_stage_state* s, const float* frames, const float2* S, int W
     hipLaunchKernelGGL(k_dose_rows<X>, dim3(nblk), dim3(1024), sh, st, frames, s->fs, s->N, s->H, s->W, Hp, s->twWp2, \
                        s->twWp, s->Tdose, s->img_sum, s->even, s->odd, (float*)nullptr, s->drow);            \
   } break;
-    ZR_DR(256) ZR_DR(512) ZR_DR(1024) ZR_DR(2048) ZR_DR(3072) ZR_DR(3840) ZR_DR(4096)
+    ZR_DR(256) ZR_DR(512) ZR_DR(1024) ZR_DR(2048) ZR_DR(3072) ZR_DR(3840) ZR_DR(4096) ZR_DR(5760)
 #undef ZR_DR
     default: return serr(ZR_ERR_UNSUPPORTED, "dose width %d", Wp);
   }
@@ -1857,15 +1861,15 @@ int zr_stage_dose(zr_stage_state* s, const float* frames, const float2* S, int W
   hipLaunchKernelGGL(k_dose_params, dim3(1), dim3(256), 0, st, a);
   ZS_HIP(hipGetLastError());
   switch (Hp) {
-#define ZR_DC(X) case X: hipLaunchKernelGGL(k_dose_cols<X>, dim3(Wp / 2 + 1), dim3(256), 0, st, s->Tdose, a, s->twHp, s->G); break;
-    ZR_DC(256) ZR_DC(512) ZR_DC(1024) ZR_DC(2048) ZR_DC(3072) ZR_DC(3840) ZR_DC(4096)
+#define ZR_DC(X) case X: hipLaunchKernelGGL(k_dose_cols<X>, dim3(Wp / 2 + 1), dim3(dose_cols_nt(X)), 0, st, s->Tdose, a, s->twHp, s->G); break;
+    ZR_DC(256) ZR_DC(512) ZR_DC(1024) ZR_DC(2048) ZR_DC(3072) ZR_DC(3840) ZR_DC(4096) ZR_DC(5760)
 #undef ZR_DC
     default: return serr(ZR_ERR_UNSUPPORTED, "dose height %d", Hp);
   }
   ZS_HIP(hipGetLastError());
   switch (Wp) {
 #define ZR_DO(X) case X: hipLaunchKernelGGL(k_dose_out<X>, dim3(s->H), dim3(256), 0, st, s->G, s->H, s->W, s->twWp2, s->twWp, s->filt); break;
-    ZR_DO(256) ZR_DO(512) ZR_DO(1024) ZR_DO(2048) ZR_DO(3072) ZR_DO(3840) ZR_DO(4096)
+    ZR_DO(256) ZR_DO(512) ZR_DO(1024) ZR_DO(2048) ZR_DO(3072) ZR_DO(3840) ZR_DO(4096) ZR_DO(5760)
 #undef ZR_DO
     default: return serr(ZR_ERR_UNSUPPORTED, "dose width %d", Wp);
   }

@@ -23,11 +23,24 @@
 // transformed by its own group of TPR threads (the groups share the block's
 // barriers, so the RB transforms proceed together).  Every thread issues all of
 // its row loads before the first use.
+//
+// The transformed image is selected by `mode` (frames zero-padded to Hp x WP,
+// mask m = 1 on the H x W frame region):
+//   K1_FRAME   f                      (frame sums go to `part`)
+//   K1_CENTRED m (f - mu_f)           mu_f = stats[2f] / (H W)
+//   K1_SQUARED (m (f - mu_f))^2
+//   K1_MASK    m
+// Subtracting the frame mean inside the mask leaves the masked normalised
+// cross-correlation unchanged (m^2 = m makes every mask term shift with the
+// mean in a way that cancels exactly in numerator and denominator) and keeps
+// the float32 spectra free of the large DC term.
+enum { K1_FRAME = 0, K1_CENTRED = 1, K1_SQUARED = 2, K1_MASK = 3 };
 template <int WP, int RB, int TPR>
 __global__ __launch_bounds__(RB * TPR) void k1_rows(const float* __restrict__ frames, int H, int W, int Hp, int Wh,
                                                     int frame0, const float2* __restrict__ twL,
                                                     const float2* __restrict__ tw2L, float2* __restrict__ T,
-                                                    double* __restrict__ part) {
+                                                    double* __restrict__ part, int mode,
+                                                    const double* __restrict__ stats) {
   constexpr int L = WP / 2, NT = RB * TPR, PER = L / TPR;
   static_assert(L % TPR == 0, "row length must split evenly over the row group");
   extern __shared__ float2 k1_lds[];
@@ -43,7 +56,14 @@ __global__ __launch_bounds__(RB * TPR) void k1_rows(const float* __restrict__ fr
   const int y = y0 + g;
   const float* row = frames + ((size_t)f * H + y) * W;
   float2 v[PER];
-  if (y < H && W == WP) {
+  if (mode == K1_MASK) {
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+      const int x = 2 * (gt + i * TPR);
+      v[i].x = (y < H && x < W) ? 1.f : 0.f;
+      v[i].y = (y < H && x + 1 < W) ? 1.f : 0.f;
+    }
+  } else if (y < H && W == WP) {
     const float2* row2 = reinterpret_cast<const float2*>(row);
 #pragma unroll
     for (int i = 0; i < PER; ++i) v[i] = row2[gt + i * TPR];
@@ -53,6 +73,18 @@ __global__ __launch_bounds__(RB * TPR) void k1_rows(const float* __restrict__ fr
       const int x = 2 * (gt + i * TPR);
       v[i].x = (y < H && x < W) ? row[x] : 0.f;
       v[i].y = (y < H && x + 1 < W) ? row[x + 1] : 0.f;
+    }
+  }
+  if (mode == K1_CENTRED || mode == K1_SQUARED) {
+    const float mu = (float)(stats[2 * f] / ((double)H * (double)W));
+    const bool sq = (mode == K1_SQUARED);
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+      const int x = 2 * (gt + i * TPR);
+      const float a = (y < H && x < W) ? v[i].x - mu : 0.f;
+      const float b = (y < H && x + 1 < W) ? v[i].y - mu : 0.f;
+      v[i].x = sq ? a * a : a;
+      v[i].y = sq ? b * b : b;
     }
   }
   double s1 = 0.0, s2 = 0.0;
@@ -83,6 +115,7 @@ __global__ __launch_bounds__(RB * TPR) void k1_rows(const float* __restrict__ fr
 #pragma unroll
     for (int r = 0; r < RB; r += 2) dst[r / 2] = make_float4(out[r].x, out[r].y, out[r + 1].x, out[r + 1].y);
   }
+  if (mode != K1_FRAME) return;   // block-uniform
   // frame sums (for the intensity normalisation)
   s1 = wave_sum_d(s1);
   s2 = wave_sum_d(s2);
@@ -146,6 +179,62 @@ __global__ __launch_bounds__(256) void k1_frame_stats(const double* __restrict__
   if (tid == 0) {
     stats[2 * f] = (ra[0] + ra[1]) + (ra[2] + ra[3]);
     stats[2 * f + 1] = (rb[0] + rb[1]) + (rb[2] + rb[3]);
+  }
+}
+
+// Frame sums ahead of the masked row pass (which needs each frame's mean):
+// block = K1S_RB rows of one frame, FP64 partials in the layout k1_frame_stats
+// reduces (part[(f * nrb + row block) * 2 + {0, 1}]).
+constexpr int K1S_RB = 8;
+__global__ __launch_bounds__(256) void k1_frame_sums(const float* __restrict__ frames, int H, int W,
+                                                     double* __restrict__ part) {
+  __shared__ double ra[4], rb[4];
+  const int nrb = (H + K1S_RB - 1) / K1S_RB;
+  const int f = blockIdx.x / nrb, r = blockIdx.x % nrb;
+  const int tid = threadIdx.x;
+  double a = 0.0, b = 0.0;
+  const int y1 = min(H, (r + 1) * K1S_RB);
+  for (int y = r * K1S_RB; y < y1; ++y) {
+    const float* row = frames + ((size_t)f * H + y) * W;
+    for (int x = tid; x < W; x += 256) {
+      const double v = row[x];
+      a += v;
+      b += v * v;
+    }
+  }
+  a = wave_sum_d(a);
+  b = wave_sum_d(b);
+  if ((tid & 63) == 0) { ra[tid >> 6] = a; rb[tid >> 6] = b; }
+  __syncthreads();
+  if (tid == 0) {
+    part[((size_t)f * nrb + r) * 2 + 0] = (ra[0] + ra[1]) + (ra[2] + ra[3]);
+    part[((size_t)f * nrb + r) * 2 + 1] = (rb[0] + rb[1]) + (rb[2] + rb[3]);
+  }
+}
+
+// Masked normalised cross-correlation from its lag-window terms
+// (zorro/zorro.py:1031-1074).  Every term is nc2 * Re R[.] on the 2m x 2m lag
+// window, produced by the pair transform for spectrum pairs (a, b) -> conj(S_a) S_b:
+//   X  (F_I, F_J)            the pair itself, window p
+//   Ct (F_I, M)   Cb (M, F_J)    T (Q_I, M)   B (M, Q_J)   mp (M, M)
+// laid out after the P pair windows as [Ct 0..N) [Cb 0..N) [T 0..N) [B 0..N) [mp].
+//   C = (X - Ct Cb / mp) / clip(sqrt((B - Cb^2/mp) (T - Ct^2/mp)), 1, inf)
+// evaluated in FP64; as with numpy's clip, a NaN denominator stays NaN.
+__global__ __launch_bounds__(256) void k_mnxc_combine(float* __restrict__ windows, const int4* __restrict__ pairs,
+                                                      int P, int N, int nn) {
+  const int4 pr = pairs[blockIdx.x];
+  const int I = pr.x, J = pr.y;
+  float* X = windows + (size_t)pr.z * nn;
+  const float* Ct = windows + (size_t)(P + I) * nn;
+  const float* Cb = windows + (size_t)(P + N + J) * nn;
+  const float* Tt = windows + (size_t)(P + 2 * N + I) * nn;
+  const float* Bt = windows + (size_t)(P + 3 * N + J) * nn;
+  const float* mp = windows + (size_t)(P + 4 * N) * nn;
+  for (int e = threadIdx.x; e < nn; e += 256) {
+    const double ct = Ct[e], cb = Cb[e], mpe = mp[e];
+    double den = sqrt((Bt[e] - cb * (cb / mpe)) * (Tt[e] - ct * (ct / mpe)));
+    den = (den < 1.0) ? 1.0 : den;
+    X[e] = (float)(((double)X[e] - ct * cb / mpe) / den);
   }
 }
 
@@ -599,12 +688,18 @@ __global__ __launch_bounds__(256) void k_peaks(const float* __restrict__ windows
 // rows per k1_rows block for a padded width (also the granularity of the
 // frame-sum partials)
 constexpr int k1_rows_per_block(int wp) { return wp >= 3072 ? 4 : 8; }
+// threads per row: the half-length L = WP/2 must split evenly over them
+constexpr int k1_threads_per_row(int wp) {
+  return (wp >= 2048 && (wp / 2) % 128 == 0) ? 128 : ((wp / 2) % 192 == 0 ? 192 : 64);
+}
 
 template <int WP>
 static hipError_t launch_k1_rows_t(const float* frames, int H, int W, int Hp, int Wh, int frame0, int nb,
-                                   const float2* twL, const float2* tw2L, float2* T, double* part, hipStream_t st) {
+                                   const float2* twL, const float2* tw2L, float2* T, double* part, int mode,
+                                   const double* stats, hipStream_t st) {
   constexpr int RB = k1_rows_per_block(WP);
-  constexpr int TPR = (WP >= 2048) ? 128 : 64;
+  constexpr int TPR = k1_threads_per_row(WP);
+  if (Hp % RB != 0) return hipErrorInvalidValue;
   int grid = nb * (Hp / RB);
   size_t sh = (size_t)RB * (WP / 2) * sizeof(float2);
   static bool attr = false;
@@ -613,16 +708,16 @@ static hipError_t launch_k1_rows_t(const float* frames, int H, int W, int Hp, in
     attr = true;
   }
   hipLaunchKernelGGL((k1_rows<WP, RB, TPR>), dim3(grid), dim3(RB * TPR), sh, st, frames, H, W, Hp, Wh, frame0, twL,
-                     tw2L, T, part);
+                     tw2L, T, part, mode, stats);
   return hipGetLastError();
 }
 
 hipError_t zr_launch_k1_rows(int Wp, const float* frames, int H, int W, int Hp, int Wh, int frame0, int nb,
-                             const float2* twL, const float2* tw2L, float2* T, double* part, int* rb_out,
-                             hipStream_t st) {
+                             const float2* twL, const float2* tw2L, float2* T, double* part, int mode,
+                             const double* stats, int* rb_out, hipStream_t st) {
   switch (Wp) {
-#define ZR_CASE(X) case X: *rb_out = k1_rows_per_block(X); return launch_k1_rows_t<X>(frames, H, W, Hp, Wh, frame0, nb, twL, tw2L, T, part, st);
-    ZR_CASE(256) ZR_CASE(512) ZR_CASE(1024) ZR_CASE(2048) ZR_CASE(3072) ZR_CASE(3840) ZR_CASE(4096)
+#define ZR_CASE(X) case X: *rb_out = k1_rows_per_block(X); return launch_k1_rows_t<X>(frames, H, W, Hp, Wh, frame0, nb, twL, tw2L, T, part, mode, stats, st);
+    ZR_CASE(256) ZR_CASE(512) ZR_CASE(1024) ZR_CASE(2048) ZR_CASE(3072) ZR_CASE(3840) ZR_CASE(4096) ZR_CASE(5760)
 #undef ZR_CASE
     default: return hipErrorInvalidValue;
   }
@@ -633,10 +728,22 @@ hipError_t zr_launch_k1_cols(int Hp, const float2* T, int Wh, int Hc, int frame0
   int grid = nb * Wh;
   switch (Hp) {
 #define ZR_CASE(X) case X: hipLaunchKernelGGL((k1_cols<X, (X >= 2048 ? 512 : 256)>), dim3(grid), dim3(X >= 2048 ? 512 : 256), 0, st, T, Wh, Hc, frame0, twH, S, Sside, full); return hipGetLastError();
-    ZR_CASE(256) ZR_CASE(512) ZR_CASE(1024) ZR_CASE(2048) ZR_CASE(3072) ZR_CASE(3840) ZR_CASE(4096)
+    ZR_CASE(256) ZR_CASE(512) ZR_CASE(1024) ZR_CASE(2048) ZR_CASE(3072) ZR_CASE(3840) ZR_CASE(4096) ZR_CASE(5760)
 #undef ZR_CASE
     default: return hipErrorInvalidValue;
   }
+}
+
+hipError_t zr_launch_k1_sums(const float* frames, int n, int H, int W, double* part, int* nrb_out, hipStream_t st) {
+  const int nrb = (H + K1S_RB - 1) / K1S_RB;
+  *nrb_out = nrb;
+  hipLaunchKernelGGL(k1_frame_sums, dim3(n * nrb), dim3(256), 0, st, frames, H, W, part);
+  return hipGetLastError();
+}
+
+hipError_t zr_launch_mnxc_combine(float* windows, const int4* pairs, int P, int N, int m, hipStream_t st) {
+  hipLaunchKernelGGL(k_mnxc_combine, dim3(P), dim3(256), 0, st, windows, pairs, P, N, 4 * m * m);
+  return hipGetLastError();
 }
 
 hipError_t zr_launch_k1_stats(const double* part, int nrb, int n, double* stats, hipStream_t st) {
