@@ -5,7 +5,12 @@ One step = one movie through the whole alignImageStack path (forward spectra,
 Lanczos shift + sums, even/odd FRC, dose filter), with the frames already
 resident in HBM.  Prints ONE JSON line (rank 0).
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline] [--inflight K]
+
+Other workloads (each prints its own JSON line; the line above is the default):
+
+    python bench.py --workload served    # Poisson arrivals of a 3-class movie mix, p50/p99 vs rate
+    python bench.py --workload batched   # 48 uneven movies (20-60 frames, 4 detector shapes) at once
 """
 from __future__ import annotations
 
@@ -106,6 +111,10 @@ def main():
                     help="run a plain 1 GiB device copy first (reference for rocprofv3 FETCH/WRITE counters)")
     ap.add_argument("--pmc-summary", nargs=3, metavar=("FETCH_CSV", "WRITE_CSV", "OUT_JSON"),
                     help="summarise two rocprofv3 --pmc counter_collection.csv files and exit (no GPU)")
+    ap.add_argument("--workload", default="c2", choices=["c2", "served", "batched"])
+    ap.add_argument("--inflight", type=int, default=1,
+                    help="C2 movies in flight at once (one plan and stream each)")
+    ap.add_argument("--seconds", type=float, default=6.0, help="served: arrival window per rate")
     args = ap.parse_args()
     if args.pmc_summary:
         pmc_summary(*args.pmc_summary)
@@ -116,12 +125,21 @@ def main():
         sys.exit(2)
 
     import torch
-    from zorro_amd.engine import MoviePlan, make_params
-    from zorro_amd.synth import make_movie_torch
-
     if not torch.cuda.is_available():
         print("no GPU visible", file=sys.stderr)
         sys.exit(1)
+    if args.workload == "served":
+        return bench_served(args)
+    if args.workload == "batched":
+        return bench_batched(args)
+    return bench_c2(args)
+
+
+def bench_c2(args):
+    import torch
+    from zorro_amd.engine import MoviePlan, make_params
+    from zorro_amd.synth import make_movie_torch
+
     dev = torch.device("cuda:0")
     c = WORKLOAD
     print(f"[bench] generating {c['n']}x{c['h']}x{c['w']} movie on {dev}", file=sys.stderr, flush=True)
@@ -135,26 +153,39 @@ def main():
             dst.copy_(src)
         torch.cuda.synchronize()
         del src, dst
-    plan = MoviePlan(make_params(**PARAMS), c["n"], c["h"], c["w"])
-    plan.set_frames(frames)
+    K = max(1, args.inflight)
+    plans = [MoviePlan(make_params(**PARAMS), c["n"], c["h"], c["w"]) for _ in range(K)]
+    for p in plans:
+        p.set_frames(frames)
+    plan = plans[0]
     info = plan.info()
 
-    print(f"[bench] plan ready {info}; warmup {args.warmup}", file=sys.stderr, flush=True)
-    for _ in range(args.warmup):
-        plan.align()
-    plan.synchronize()
+    print(f"[bench] plan ready {info}; {K} in flight; warmup {args.warmup}", file=sys.stderr, flush=True)
+    for i in range(args.warmup):
+        plans[i % K].align()
+    for p in plans:
+        p.synchronize()
     torch.cuda.synchronize()
 
     plan.set_timing(True)
     plan.stage_times(reset=True)
     lat = []
+    start = [0.0] * K
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    for _ in range(args.steps):
-        ts = time.perf_counter()
-        plan.align()
-        plan.synchronize()
-        lat.append((time.perf_counter() - ts) * 1e3)
+    # K plans on K streams: movie s runs on plan s % K once that plan's previous
+    # movie has finished; latency = enqueue -> finish of each movie
+    for s in range(args.steps):
+        k = s % K
+        if s >= K:
+            plans[k].synchronize()
+            lat.append((time.perf_counter() - start[k]) * 1e3)
+        start[k] = time.perf_counter()
+        plans[k].align()
+    for s in range(max(0, args.steps - K), args.steps):
+        k = s % K
+        plans[k].synchronize()
+        lat.append((time.perf_counter() - start[k]) * 1e3)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     st = plan.stage_times()
@@ -185,6 +216,7 @@ def main():
                                     "L1 shifts solver, Lanczos shift + sum, even/odd FRC (512 tiles), "
                                     "dose filter (1 A/px, 300 kV)", frames=c["n"], shape=[c["h"], c["w"]],
                            pairs=info["pairs"], fouCrop=[info["Hc"], info["Wc"]], inputs="resident in HBM"),
+               inflight=K,
                frames_per_s=round(c["n"] * 1000.0 / ms_step, 1),
                latency_ms=dict(p50=round(float(np.percentile(lat, 50)), 3),
                                p99=round(float(np.percentile(lat, 99)), 3)),
@@ -199,7 +231,120 @@ def main():
         out["cpu_baseline"] = dict(value=round(v, 5), unit="movies/min", cores=d["cores"], kind="port",
                                    sample=d["sample"], stage_s={k: round(x, 2) for k, x in d["stage_s"].items()})
         out["speedup_vs_cpu"] = round(movies_min / v, 1)
-    plan.close()
+    for p in plans:
+        p.close()
+    print(json.dumps(out), flush=True)
+
+
+# ---------------------------------------------------------------- serving
+SERVE_BASE = dict(triMode="tri", maxShift=40, Bmode="conv", Brad=150, subPixReg=16, shiftMethod="lanczos",
+                  weightMode="autologistic", originMode="centroid", suppressOrigin=True, filterMode="dose",
+                  pixelsize=0.1, voltage=300.0, doLazyFRC=True, frc_box=512, frc_overlap=0.5)
+# detector shape -> (padded shape, fouCrop): padded to the smallest square 2^a3^b5^c size as
+# findValidFFTWDim does (zorro/zorro_util.py:1068-1100), cropped to 3072 when larger
+SHAPES = {(2048, 2048): ((2048, 2048), (0, 0)), (3838, 3710): ((3840, 3840), (3072, 3072)),
+          (4096, 4096): ((4096, 4096), (3072, 3072)), (5760, 4092): ((5760, 5760), (3072, 3072))}
+
+
+def _movie_pool(specs, seed0=100):
+    """Device-resident synthetic movies, one per (n, h, w, dose) spec."""
+    import torch
+    from zorro_amd.synth import make_movie_torch
+    pool = []
+    for i, (n, h, w, dose) in enumerate(specs):
+        f, d = make_movie_torch(n, h, w, seed=seed0 + i, dose=dose, device="cuda:0")
+        pad, crop = SHAPES[(h, w)]
+        pool.append((f, d, dict(SERVE_BASE, shapePadded=pad, fouCrop=crop)))
+    torch.cuda.synchronize()
+    return pool
+
+
+def _rms_truth(res, drift):
+    truth = -(drift - drift.mean(axis=0))
+    return float(np.sqrt(np.mean((res.translations - truth) ** 2)))
+
+
+def bench_served(args):
+    """Serving stream: Poisson arrivals of a 30/40/30 mix of 2048^2 x 20,
+    4096^2 x 40 and 5760x4092 x 60 movies at 0.2-2x the measured saturation
+    rate; movies/min, p50/p99 latency and the error against the true drift at
+    each rate."""
+    from zorro_amd.serve import MovieServer, latency_summary
+    classes = [((20, 2048, 2048, 1.0), 0.3), ((40, 4096, 4096, 1.0), 0.4), ((60, 5760, 4092, 1.0), 0.3)]
+    print("[bench served] generating the movie pool", file=sys.stderr, flush=True)
+    pool = _movie_pool([c for c, _ in classes])
+    probs = np.array([p for _, p in classes])
+    rng = np.random.default_rng(7)
+    out = dict(metric="served movies/min and p50/p99 latency (30% 2048^2x20, 40% 4096^2x40, 30% 5760x4092x60)",
+               unit="movies/min", n_gpus=1, data="synthetic", inflight=args.inflight, rates=[])
+    with MovieServer(max_inflight=max(2, args.inflight), return_sums="device") as srv:
+        # warm-up: one of each class creates the plans
+        for f, d, kw in pool:
+            srv.submit(f, **kw).result()
+        # saturation: a closed backlog in the mix proportions
+        seq = rng.choice(len(pool), size=30, p=probs)
+        t0 = time.perf_counter()
+        futs = [srv.submit(pool[k][0], **pool[k][2]) for k in seq]
+        res = [f.result() for f in futs]
+        sat = len(seq) / (time.perf_counter() - t0)          # movies/s
+        out["saturation_movies_min"] = round(sat * 60, 1)
+        print(f"[bench served] saturation {sat * 60:.1f} movies/min", file=sys.stderr, flush=True)
+        for frac in (0.2, 0.5, 0.8, 1.0, 1.5, 2.0):
+            lam = sat * frac
+            n = max(8, int(lam * args.seconds))
+            gaps = rng.exponential(1.0 / lam, size=n)
+            kinds = rng.choice(len(pool), size=n, p=probs)
+            futs = []
+            t0 = time.perf_counter()
+            tnext = t0
+            for g, k in zip(gaps, kinds):
+                tnext += g
+                while time.perf_counter() < tnext:
+                    time.sleep(min(0.0005, max(0.0, tnext - time.perf_counter())))
+                futs.append((k, srv.submit(pool[k][0], arrival=tnext, **pool[k][2])))
+            res = [(k, f.result()) for k, f in futs]
+            wall = max(r.done for _, r in res) - t0
+            lat = latency_summary([r for _, r in res])
+            err = max(_rms_truth(r, pool[k][1]) for k, r in res)
+            row = dict(rate_frac=frac, offered_movies_min=round(lam * 60, 1), movies=n,
+                       achieved_movies_min=round(n / wall * 60, 1), p50_ms=round(lat["p50_ms"], 2),
+                       p99_ms=round(lat["p99_ms"], 2), p99_over_p50=round(lat["p99_ms"] / lat["p50_ms"], 3),
+                       max_rms_vs_true_drift_px=round(err, 4))
+            print(f"[bench served] {row}", file=sys.stderr, flush=True)
+            out["rates"].append(row)
+        out["server"] = dict(srv.stats)
+    print(json.dumps(out), flush=True)
+
+
+def bench_batched(args):
+    """48 uneven movies (frame counts U{20..60}, shapes cycling over 2048^2,
+    3838x3710, 4096^2, 5760x4092, doses U(0.5, 2.0)) submitted at once."""
+    from zorro_amd.serve import MovieServer, latency_summary
+    rng = np.random.default_rng(11)
+    shapes = list(SHAPES)
+    specs = [(int(rng.integers(20, 61)), *shapes[i % 4], float(rng.uniform(0.5, 2.0))) for i in range(48)]
+    # movies are generated per shape class in groups to bound memory: 12 per class
+    out = dict(metric="batched movies/min (48 movies, 20-60 frames, 4 detector shapes)", unit="movies/min",
+               n_gpus=1, data="synthetic", inflight=max(2, args.inflight))
+    total_frames = sum(s[0] for s in specs)
+    results = []
+    t_all = 0.0
+    with MovieServer(max_inflight=max(2, args.inflight), return_sums="device") as srv:
+        for cls in range(4):
+            group = [s for i, s in enumerate(specs) if i % 4 == cls]
+            pool = _movie_pool(group, seed0=300 + 20 * cls)
+            srv.submit(pool[0][0], **pool[0][2]).result()                  # plan warm-up
+            t0 = time.perf_counter()
+            futs = [(d, srv.submit(f, arrival=t0, **kw)) for f, d, kw in pool]
+            res = [(d, f.result()) for d, f in futs]
+            t_all += time.perf_counter() - t0
+            results += res
+            print(f"[bench batched] class {shapes[cls]}: {len(pool)} movies", file=sys.stderr, flush=True)
+            del pool
+        out["server"] = dict(srv.stats)
+    out.update(value=round(len(specs) / t_all * 60, 1), frames_per_s=round(total_frames / t_all, 1),
+               seconds=round(t_all, 3), max_rms_vs_true_drift_px=round(max(_rms_truth(r, d) for d, r in results), 4),
+               latency=latency_summary([r for _, r in results]))
     print(json.dumps(out), flush=True)
 
 

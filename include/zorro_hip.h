@@ -35,8 +35,13 @@ enum {
   ZR_ERR_NOMEM = -5         /* device allocation failed */
 };
 
-/* triMode (zorro/zorro.py:207, :559-581) */
-enum { ZR_TRI_TRI = 0, ZR_TRI_DIAG = 1, ZR_TRI_AUTO = 2 };
+/* triMode (zorro/zorro.py:207, :559-581): 'tri' all pairs; 'diag' the band
+ * diagStart < J-I <= diagWidth + diagStart; 'auto' each row stops after the
+ * first pair whose peak significance is <= peaksigThres or J-I >= autoMax
+ * (:992-996); 'first' correlates frame 0 with every other frame (no solver,
+ * :1628-1635); 'refine' correlates every frame with the sum of the others
+ * (no solver, :934-956, :1636-1647). */
+enum { ZR_TRI_TRI = 0, ZR_TRI_DIAG = 1, ZR_TRI_AUTO = 2, ZR_TRI_FIRST = 3, ZR_TRI_REFINE = 4 };
 /* Bmode (zorro/zorro.py:117, :1104-1141) */
 enum { ZR_B_NONE = 0, ZR_B_CONV = 1, ZR_B_OPTI = 2 };
 /* shiftMethod (zorro/zorro.py:190, :1734-1808) */
@@ -65,6 +70,7 @@ typedef struct zr_params {
   int do_frc;                             /* doLazyFRC */
   int frc_box;                            /* tiledFRC box */
   double frc_overlap;                     /* tiledFRC overlap */
+  int auto_max;                           /* autoMax (triMode 'auto') */
 } zr_params;
 
 typedef struct zr_movie zr_movie;
@@ -127,9 +133,16 @@ int zr_synchronize(zr_movie* mv);
 /* Result read-back (host buffers, synchronising).
  * shifts f32 [N][N][2], corr/peaksig/origin f32 [N][N] (errorDict trimats). */
 int zr_get_trimats(zr_movie* mv, float* shifts, float* corr, float* peaksig, float* origin);
+/* Bmode='opti': the Gaussian width chosen for each pair, f32 [N][N] (the
+ * reference only prints it, zorro/zorro.py:1123-1124). */
+int zr_get_peak_sigmas(zr_movie* mv, float* sigma);
 /* windows f32 [P][2m][2m] in canonical pair order (row-major over I<J). */
 int zr_get_windows(zr_movie* mv, float* windows);
 int zr_pair_count(zr_movie* mv, int* npairs);
+/* Equations of the last zr_solve (the rows of shiftsSolver's Acoeff,
+ * zorro/zorro.py:1296-1311): the length of weights and errorXY.  0 for the
+ * triModes without a solver. */
+int zr_equation_count(zr_movie* mv, int* m);
 /* translations f64 [N][2] (after the centroid when originMode='centroid'),
  * relative_est f64 [N-1][2], weights f64 [M], logistic f64 [3] (thr, K, nu),
  * error f64 [M][2] (errorXY), error_stats f64 [2] (mean/std errorNorm),
@@ -164,6 +177,17 @@ int zr_set_timing(zr_movie* mv, int enable);
 int zr_get_stage_times(zr_movie* mv, double* ms, long* counts, int reset);
 /* dims[12] = N, H, W, Hp, Wp, Hc, Wc, Wh, maxShift, pairs, unmasked, K1 batch. */
 int zr_movie_info(zr_movie* mv, int* dims);
+
+/* Device memory the movie holds (workspaces, tables, frames copied in by
+ * zr_movie_set_frames): what a server admitting movies by HBM budget charges
+ * for it.  No reference counterpart (the reference holds one movie per process). */
+int zr_movie_device_bytes(zr_movie* mv, size_t* bytes);
+
+/* Asynchronous device-to-device copies of imageSum, filtSum and the even/odd
+ * half sums (f32 [H][W] device buffers, any may be NULL) on the movie's stream,
+ * so that a reused movie plan can hand its results to a caller-owned buffer.
+ * The results of applyShifts/doseFilter (zorro/zorro.py:1804, :2487-2494). */
+int zr_copy_sums_device(zr_movie* mv, float* image_sum, float* filt_sum, float* even_sum, float* odd_sum);
 
 /* Device pointers of results for zero-copy consumers (valid until destroy). */
 int zr_device_results(zr_movie* mv, const float** image_sum, const float** filt_sum, const double** translations);

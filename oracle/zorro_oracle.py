@@ -81,6 +81,7 @@ class Params:
     triMode: str = "diag"                # :207
     diagWidth: int = 5                   # :211
     diagStart: int = 0                   # :210
+    autoMax: int = 10                    # :212
     maxShift: int = 100                  # :191
     preShift: bool = False               # :194
     Bmode: str = "opti"                  # :117
@@ -323,7 +324,7 @@ def xcorrnm2_tri(images: np.ndarray, mask: np.ndarray, p: Params, return_windows
     shapeP = images.shape[1:]
     shapeC = shape_cropped(p.fouCrop, shapeP)
     tri = tri_indices(n, p.triMode, p.diagWidth, p.diagStart)
-    if p.triMode not in ("tri", "diag", "auto"):
+    if p.triMode not in ("tri", "diag", "auto", "first", "refine"):
         raise NotImplementedError("triMode %r is outside the supported path" % p.triMode)
     unmasked = bool(mask.all())
     F, Q = frame_spectra(images, mask[0], shapeC)
@@ -337,7 +338,27 @@ def xcorrnm2_tri(images: np.ndarray, mask: np.ndarray, p: Params, return_windows
     peaksig = np.zeros([n, n], dtype=np.float32)
     origin = np.zeros([n, n], dtype=np.float32)
     windows = {}
-    for I in range(n):
+    if p.triMode == "refine":
+        # template of frame I: (sum_k F_k - F_I)/N and (sum_k Q_k - Q_I)/N, base frame I,
+        # for I < N-1 (zorro/zorro.py:934-956)
+        sF, sQ = F.sum(axis=0), Q.sum(axis=0)
+        for I in range(n - 1):
+            TF, TQ = (sF - F[I]) / n, (sQ - Q[I]) / n
+            if unmasked:
+                s_t, q_t = (sums.sum() - sums[I]) / n, (sumsq.sum() - sumsq[I]) / n
+                W, o = mnxc2_window_unmasked(TF, F[I], s_t, q_t, sums[I], sumsq[I], shapeP, shapeC, p.maxShift,
+                                             p.suppressOrigin)
+            else:
+                W, o = mnxc2_window(TF, TQ, F[I], Q[I], M, mp, shapeC, p.maxShift, p.suppressOrigin)
+            origin[I, I] = o
+            s, c, ps, _ = locate_peak(W, p, shapeP)
+            shifts[I, I] = s
+            corr[I, I] = c
+            peaksig[I, I] = ps
+            if return_windows:
+                windows[(I, I)] = W
+        tri = np.zeros((0, n), bool)
+    for I in range(tri.shape[0]):          # one row for triMode first
         for J in np.nonzero(tri[I])[0]:
             if unmasked:
                 W, o = mnxc2_window_unmasked(F[I], F[J], sums[I], sumsq[I], sums[J], sumsq[J],
@@ -351,7 +372,7 @@ def xcorrnm2_tri(images: np.ndarray, mask: np.ndarray, p: Params, return_windows
             peaksig[I, J] = ps
             if return_windows:
                 windows[(I, J)] = W
-            if p.triMode == "auto" and (ps <= p.peaksigThres):
+            if p.triMode == "auto" and (ps <= p.peaksigThres or J - I >= p.autoMax):   # :992-996
                 break
     if np.any(p.fouCrop):                                                  # :1004-1006 (float32 multiply)
         shifts[:, :, 0] *= np.float32(shapeP[0] / shapeC[0])
@@ -756,7 +777,12 @@ def align_image_stack(images: np.ndarray, p: Params):
     n, h, w = images.shape
     padded, mask = pad_stack(images, p.shapePadded)
     xc = xcorrnm2_tri(padded, mask, p)
-    ed = shifts_solver(xc["shiftsTriMat"], xc["corrTriMat"], xc["peaksigTriMat"], p, padded.shape[1:])
+    if p.triMode in ("first", "refine"):                                 # :1626-1647, no solver
+        st = xc["shiftsTriMat"]
+        t = -(st[0, :] if p.triMode == "first" else np.stack([st[k, k] for k in range(n)])).astype(np.float64)
+        ed = dict(shiftsTriMat=st, corrTriMat=xc["corrTriMat"], peaksigTriMat=xc["peaksigTriMat"], translations=t)
+    else:
+        ed = shifts_solver(xc["shiftsTriMat"], xc["corrTriMat"], xc["peaksigTriMat"], p, padded.shape[1:])
     ed["originTriMat"] = xc["originTriMat"]
     aligned, image_sum, trans = apply_shifts(images, ed["translations"], p)
     res = dict(xcorr=xc, errorDict=ed, translations=trans, imageSum=image_sum, aligned=aligned)

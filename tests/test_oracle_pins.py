@@ -151,6 +151,29 @@ def test_b_filter_conv_equals_explicit_reflect_gaussian():
     assert np.allclose(got, expect, atol=1e-12)
 
 
+def test_b_filter_opti_minimises_peak_contrast_objective():
+    """Bmode='opti' (zorro/zorro.py:1105-1124): the width Brent's bounded search
+    returns is the minimiser of std/(max - mean) over [0, 7] to within a fine
+    grid's resolution, and the window comes back filtered with it."""
+    yy, xx = np.mgrid[0:80, 0:80].astype(float)
+    for seed, (y0, x0, w) in enumerate([(43.2, 37.9, 2.0), (20.0, 55.5, 3.5), (60.7, 21.3, 1.2)]):
+        rng = np.random.default_rng(100 + seed)
+        C = np.exp(-((yy - y0) ** 2 + (xx - x0) ** 2) / (2 * w * w)) + 0.3 * rng.standard_normal((80, 80))
+
+        def f(s):
+            cf = scipy.ndimage.gaussian_filter(C, s)
+            return np.std(cf) / (np.max(cf) - np.mean(cf))
+        grid = np.linspace(0.0, 7.0, 701)
+        g = grid[int(np.argmin([f(s) for s in grid]))]
+        got = O.b_filter(C, "opti", 512, (1024, 1024))
+        # recover the width from the returned window: it is gaussian_filter(C, s*) for the s* found
+        s_star = scipy.optimize.minimize_scalar(f, bounds=[0.0, 7.0], method="bounded",
+                                                options={"maxiter": 15, "xatol": 0.01}).x
+        assert np.allclose(got, scipy.ndimage.gaussian_filter(C, s_star), atol=1e-12)
+        assert abs(s_star - g) <= 0.05, (s_star, g)
+        assert f(s_star) <= f(g) + 1e-3 * abs(f(g))
+
+
 def _brute_subpixel(patch, P=16):
     n = 16 * P
     f = np.arange(-8, 8)
@@ -220,6 +243,79 @@ def test_build_equations_tri_and_diag_intervals():
             expect = np.zeros(5)
             expect[I:j + 1] = 1.0             # pair (I, J=j+1) sums x_I..x_{J-1}
             assert np.array_equal(A[r], expect)
+
+
+def test_build_equations_auto_pattern_known_answer():
+    """triMode 'auto' rows of different lengths: the rot90 construction
+    (zorro/zorro.py:1296-1311) pairs a measurement with the reversed row of a
+    LOWER row of the block, worked out by hand for 5 frames with pairs
+    (0,1) (0,2) (0,3) | (1,2) | (2,3) (2,4) | (3,4)."""
+    pairs = [(0, 1), (0, 2), (0, 3), (1, 2), (2, 3), (2, 4), (3, 4)]
+    tri = np.zeros((4, 4), bool)
+    for I, J in pairs:
+        tri[I, J - 1] = True
+    A = O.build_equations(tri)
+    expect = np.array([[1, 0, 0, 0],     # b(0,1): x0
+                       [0, 1, 0, 0],     # b(0,2): x1 (row 1 of the block, reversed), not x0 + x1
+                       [1, 1, 1, 0],     # b(0,3)
+                       [0, 1, 0, 0],     # b(1,2)
+                       [0, 0, 1, 0],     # b(2,3)
+                       [0, 0, 1, 1],     # b(2,4)
+                       [0, 0, 0, 1]])    # b(3,4)
+    assert np.array_equal(A, expect)
+
+
+def test_auto_stops_each_row_after_low_significance_or_automax():
+    """xcorrnm2_tri's 'auto' loop (zorro/zorro.py:992-996) on a movie whose frames
+    decorrelate: every row is a contiguous run from J = I+1, its last pair is the
+    first with peaksig <= peaksigThres or J - I >= autoMax."""
+    from zorro_amd.synth import make_movie
+    n = 8
+    frames, _ = make_movie(n, 256, 256, seed=5, dose=4.0)
+    p = O.Params(triMode="auto", autoMax=5, peaksigThres=3.115, maxShift=20, Bmode="conv", Brad=150, subPixReg=1,
+                 fouCrop=(0, 0), shapePadded=(256, 256))
+    padded, mask = O.pad_stack(frames.astype(np.float64), (256, 256))
+    xc = O.xcorrnm2_tri(padded, mask, p)
+    ps = xc["peaksigTriMat"]
+    lengths = []
+    for I in range(n - 1):
+        row = [J for J in range(I + 1, n) if xc["corrTriMat"][I, J] != 0]
+        assert row == list(range(I + 1, I + 1 + len(row)))
+        last = row[-1]
+        stop = ps[I, last] <= p.peaksigThres or last - I >= p.autoMax
+        assert stop or last == n - 1
+        assert all(ps[I, J] > p.peaksigThres and J - I < p.autoMax for J in row[:-1])
+        lengths.append(len(row))
+    assert min(lengths) == 1 and max(lengths) >= 3        # rows of different lengths occur
+
+
+def test_refine_known_answer_against_scaled_template():
+    """triMode 'refine' (zorro/zorro.py:934-956): with frames [roll(f, d), f, f, f]
+    the template of frame 0 is (sum F - F_0)/4 = 3/4 F and its squared term
+    3/4 Q, so the peak sits at lag d with
+    C = (3/4)(Np SQ - S^2) / sqrt((Np SQ - S^2)(3/4 Np SQ - 9/16 S^2))."""
+    H = W = 64
+    f = 10 + 4 * RNG.standard_normal((H, W))
+    d = (3, -5)
+    frames = np.stack([np.roll(np.roll(f, d[0], 0), d[1], 1), f, f, f])
+    p = O.Params(triMode="refine", maxShift=16, Bmode="none", subPixReg=1, suppressOrigin=False, fouCrop=(0, 0),
+                 shapePadded=(H, W))
+    padded, mask = O.pad_stack(frames, (H, W))
+    xc = O.xcorrnm2_tri(padded, mask, p, return_windows=True)
+    assert np.allclose(xc["shiftsTriMat"][0, 0], d)
+    Np, S, SQ = H * W, f.sum(), (f * f).sum()
+    expect = 0.75 * (Np * SQ - S * S) / math.sqrt((Np * SQ - S * S) * (0.75 * Np * SQ - 0.5625 * S * S))
+    win = xc["windows"][(0, 0)]
+    assert np.isclose(win[16 + d[0], 16 + d[1]], expect, rtol=1e-9)
+    assert np.all(xc["corrTriMat"][3] == 0)               # the last frame is never the base
+    # the general five-term form with an all-ones mask gives the same window
+    # (the closed form holds for the refine template, whose squared term is
+    # not the square of its image)
+    F, Q = O.frame_spectra(frames, np.ones((H, W)), (H, W))
+    M, mp = O.mask_spectrum(np.ones((H, W), bool), (H, W))
+    TF, TQ = (F.sum(0) - F[0]) / 4, (Q.sum(0) - Q[0]) / 4
+    gen, _ = O.mnxc2_window(TF, TQ, F[0], Q[0], M, mp, (H, W), 16, suppressOrigin=False)
+    assert np.allclose(gen, win, rtol=1e-7, atol=1e-9)
 
 
 def test_l1_solver_exact_on_consistent_system_and_optimal_otherwise():

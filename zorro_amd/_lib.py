@@ -13,7 +13,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libzorro_hip.so")
 
 ZR_OK, ZR_ERR_ARG, ZR_ERR_HIP, ZR_ERR_UNSUPPORTED, ZR_ERR_STATE, ZR_ERR_NOMEM = 0, -1, -2, -3, -4, -5
-ZR_TRI_TRI, ZR_TRI_DIAG, ZR_TRI_AUTO = 0, 1, 2
+ZR_TRI_TRI, ZR_TRI_DIAG, ZR_TRI_AUTO, ZR_TRI_FIRST, ZR_TRI_REFINE = 0, 1, 2, 3, 4
 ZR_B_NONE, ZR_B_CONV, ZR_B_OPTI = 0, 1, 2
 ZR_SHIFT_INTEGER, ZR_SHIFT_LANCZOS = 0, 1
 ZR_W_AUTOLOGISTIC, ZR_W_LOGISTIC, ZR_W_CORR, ZR_W_UNWEIGHTED, ZR_W_NORM = 0, 1, 2, 3, 4
@@ -34,6 +34,7 @@ class ZrParams(C.Structure):
         ("dose_order", C.c_double), ("dose_missing", C.c_double),
         ("pixelsize_nm", C.c_double), ("voltage_kv", C.c_double),
         ("do_frc", C.c_int), ("frc_box", C.c_int), ("frc_overlap", C.c_double),
+        ("auto_max", C.c_int),
     ]
 
 
@@ -66,7 +67,9 @@ SIGNATURES = {
     "zr_synchronize": (C.c_int, [_P]),
     "zr_get_trimats": (C.c_int, [_P, _F, _F, _F, _F]),
     "zr_get_windows": (C.c_int, [_P, _F]),
+    "zr_get_peak_sigmas": (C.c_int, [_P, _F]),
     "zr_pair_count": (C.c_int, [_P, _I]),
+    "zr_equation_count": (C.c_int, [_P, _I]),
     "zr_get_solution": (C.c_int, [_P, _D, _D, _D, _D, _D, _D, _D]),
     "zr_get_sums": (C.c_int, [_P, _F, _F, _F, _F]),
     "zr_get_frc": (C.c_int, [_P, _D, _I]),
@@ -75,6 +78,8 @@ SIGNATURES = {
     "zr_set_timing": (C.c_int, [_P, C.c_int]),
     "zr_get_stage_times": (C.c_int, [_P, _D, C.POINTER(C.c_long), C.c_int]),
     "zr_movie_info": (C.c_int, [_P, _I]),
+    "zr_movie_device_bytes": (C.c_int, [_P, C.POINTER(C.c_size_t)]),
+    "zr_copy_sums_device": (C.c_int, [_P, _P, _P, _P, _P]),
 }
 
 STAGES = ["spectra", "pairs", "peaks", "solve", "shifts", "dose", "frc"]
@@ -98,6 +103,14 @@ def load(path: str = LIB_PATH):
     if not os.path.exists(path):
         raise ImportError(f"{path} is missing: build it with `python -m zorro_amd.build` "
                           "(there is no CPU fallback for the HIP path)")
+    # torch ships its own HIP runtime (same soname libamdhip64.so.7).  Loading
+    # torch first makes the library bind to that one, so a process that moves
+    # tensors with torch and runs these kernels has a single HIP runtime; the
+    # other order loads two runtimes and torch then finds no device.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     lib = C.CDLL(path)
     for name, (res, args) in SIGNATURES.items():
         fn = getattr(lib, name)

@@ -17,6 +17,7 @@
 namespace {
 
 thread_local std::string g_err;
+thread_local size_t g_alloc_bytes = 0;   // device bytes allocated by this thread's current zr_movie_create
 
 int fail(int code, const char* fmt, ...) {
   char buf[512];
@@ -66,6 +67,7 @@ int dev_alloc(T** p, size_t count) {
   if (count == 0) return ZR_OK;
   hipError_t e = hipMalloc((void**)p, count * sizeof(T));
   if (e != hipSuccess) return fail(ZR_ERR_NOMEM, "hipMalloc of %zu bytes failed: %s", count * sizeof(T), hipGetErrorString(e));
+  g_alloc_bytes += count * sizeof(T);
   return ZR_OK;
 }
 
@@ -148,14 +150,16 @@ struct zr_movie {
   // pairs: P frame pairs; on the masked path the pair transform also runs the
   // 4N+1 mask-term pairs (Ptot in all), over nspec = 2N+1 spectra (F, Q, M)
   int P = 0, Ptot = 0, nspec = 0;
+  int win_out0 = 0, nwin = 0;      // first window of the pairs' results, windows allocated
   int4 *pairs = nullptr, *work = nullptr;
   std::vector<int4> pairs_host;
   // workspaces
   float2 *T = nullptr, *S = nullptr, *Sside = nullptr, *Z = nullptr;
-  double *part = nullptr, *stats = nullptr, *den = nullptr;
+  double *part = nullptr, *stats = nullptr, *den = nullptr, *stats_c = nullptr;
   float* windows = nullptr;
-  float *shifts = nullptr, *corr = nullptr, *peaksig = nullptr, *origin = nullptr;
+  float *shifts = nullptr, *corr = nullptr, *peaksig = nullptr, *origin = nullptr, *bsigma = nullptr;
   bool have_xcorr = false;
+  size_t bytes = 0;                // device memory held (workspaces, tables, owned frames)
   zr_stage_state* st = nullptr;   // solver / shift / FRC / dose state (zr_stages.h)
   StageTimer timer;
 };
@@ -172,8 +176,8 @@ static void free_movie(zr_movie* mv) {
   if (!mv) return;
   hipSetDevice(mv->device);
   void* ptrs[] = {mv->frames_owned, mv->twWp2, mv->twWp, mv->twHp, mv->twHc, mv->twWc, mv->ktab, mv->gy, mv->gx,
-                  mv->pairs, mv->work, mv->T, mv->S, mv->Sside, mv->Z, mv->part, mv->stats, mv->den,
-                  mv->windows, mv->shifts, mv->corr, mv->peaksig, mv->origin};
+                  mv->pairs, mv->work, mv->T, mv->S, mv->Sside, mv->Z, mv->part, mv->stats, mv->den, mv->stats_c,
+                  mv->windows, mv->shifts, mv->corr, mv->peaksig, mv->origin, mv->bsigma};
   for (void* q : ptrs)
     if (q) hipFree(q);
   zr_stage_free(mv->st);
@@ -183,10 +187,51 @@ static void free_movie(zr_movie* mv) {
 
 extern "C" void zr_movie_destroy(zr_movie* mv) { free_movie(mv); }
 
+// triMode 'refine' (zorro/zorro.py:934-956): frame I (I < N-1) against the
+// template (sum_k F_k - F_I) / N.  By linearity its raw window is
+// (W[sum, I] - W[I, I]) / N, so the pair transform runs the pairs (sum, I) and
+// (I, I) (windows 2I, 2I+1), plus on the masked path the mask terms of a pair
+// and the template's Ct, T from the sums (k_refine_combine's layout).  Results
+// go to windows win_out0 + I, read by the peak stage as the pair (I, I).
+static int build_refine_pairs(zr_movie* mv) {
+  const int N = mv->N;
+  std::vector<int4> wk, pr;
+  const int sumF = mv->unmasked ? N : 2 * N + 1;
+  for (int k = 0; k < N - 1; ++k) {
+    wk.push_back(make_int4(sumF, k, 2 * k, 0));
+    wk.push_back(make_int4(k, k, 2 * k + 1, 0));
+  }
+  if (mv->unmasked) {
+    mv->nspec = N + 1;
+  } else {
+    const int b = 2 * (N - 1), Ms = 2 * N;
+    for (int k = 0; k < N; ++k) wk.push_back(make_int4(k, Ms, b + k, 0));
+    for (int k = 0; k < N; ++k) wk.push_back(make_int4(Ms, k, b + N + k, 0));
+    for (int k = 0; k < N; ++k) wk.push_back(make_int4(N + k, Ms, b + 2 * N + k, 0));
+    for (int k = 0; k < N; ++k) wk.push_back(make_int4(Ms, N + k, b + 3 * N + k, 0));
+    wk.push_back(make_int4(Ms, Ms, b + 4 * N, 0));
+    wk.push_back(make_int4(2 * N + 1, Ms, b + 4 * N + 1, 0));
+    wk.push_back(make_int4(2 * N + 2, Ms, b + 4 * N + 2, 0));
+    mv->nspec = 2 * N + 3;
+  }
+  mv->Ptot = (int)wk.size();
+  mv->win_out0 = mv->Ptot;
+  for (int k = 0; k < N - 1; ++k) pr.push_back(make_int4(k, k, mv->win_out0 + k, 0));
+  mv->P = (int)pr.size();
+  mv->nwin = mv->Ptot + mv->P;
+  mv->pairs_host = pr;
+  int rc = dev_upload(&mv->pairs, pr);
+  if (rc) return rc;
+  return dev_upload(&mv->work, wk);
+}
+
 static int build_pairs(zr_movie* mv) {
   const zr_params& p = mv->p;
   const int N = mv->N;
   std::vector<int4> pr;
+  // triIndices of __init_xcorrnm2 (zorro/zorro.py:559-581); 'auto' correlates
+  // the band J - I <= autoMax, beyond which every row has stopped (:992-996)
+  if (p.tri_mode == ZR_TRI_AUTO && p.auto_max < 1) return fail(ZR_ERR_ARG, "autoMax must be >= 1");
   for (int I = 0; I < N; ++I)
     for (int J = I + 1; J < N; ++J) {
       int d = J - I;
@@ -195,9 +240,13 @@ static int build_pairs(zr_movie* mv) {
       else if (p.tri_mode == ZR_TRI_DIAG) {
         int wdt = p.diag_width < 0 ? N : p.diag_width;
         take = (d <= wdt + p.diag_start) && (d > p.diag_start);
-      } else return fail(ZR_ERR_UNSUPPORTED, "triMode %d is not built (tri, diag only)", p.tri_mode);
+      } else if (p.tri_mode == ZR_TRI_AUTO) take = d <= p.auto_max;
+      else if (p.tri_mode == ZR_TRI_FIRST) take = (I == 0);
+      else if (p.tri_mode == ZR_TRI_REFINE) take = false;
+      else return fail(ZR_ERR_UNSUPPORTED, "triMode %d is not built (tri, diag, auto, first, refine)", p.tri_mode);
       if (take) pr.push_back(make_int4(I, J, (int)pr.size(), 0));
     }
+  if (p.tri_mode == ZR_TRI_REFINE) return build_refine_pairs(mv);
   if (pr.empty()) return fail(ZR_ERR_ARG, "no frame pairs to correlate");
   mv->P = (int)pr.size();
   mv->pairs_host = pr;
@@ -221,6 +270,7 @@ static int build_pairs(zr_movie* mv) {
     mv->Ptot = P + 4 * N + 1;
     mv->nspec = 2 * N + 1;
   }
+  mv->nwin = mv->Ptot;
   int rc = dev_upload(&mv->pairs, pr);
   if (rc) return rc;
   return dev_upload(&mv->work, wk);
@@ -231,8 +281,10 @@ static PeakParamsHost peak_params(const zr_movie* mv) {
   hp.m = mv->m;
   hp.suppress = mv->p.suppress_origin ? 1 : 0;
   hp.bconv = (mv->p.b_mode == ZR_B_CONV && mv->p.b_rad > 0) ? 1 : 0;
-  hp.ry = mv->ry;
-  hp.rx = mv->rx;
+  hp.bopti = (mv->p.b_mode == ZR_B_OPTI) ? 1 : 0;
+  // 'opti' searches sigma in [0, 7]: radius up to int(4 * 7 + 0.5) = 28
+  hp.ry = hp.bopti ? 28 : mv->ry;
+  hp.rx = hp.bopti ? 28 : mv->rx;
   hp.subpix = mv->p.sub_pix_reg;
   hp.scale_y = mv->crop ? (float)((double)mv->Hp / (double)mv->Hc) : 1.f;
   hp.scale_x = mv->crop ? (float)((double)mv->Wp / (double)mv->Wc) : 1.f;
@@ -245,6 +297,7 @@ extern "C" int zr_movie_create(int device, const zr_params* p, int n_frames, int
   *out = nullptr;
   if (n_frames < 2 || h < 16 || w < 16) return fail(ZR_ERR_ARG, "need >= 2 frames of >= 16x16 (got %d x %d x %d)", n_frames, h, w);
   zr_movie* mv = new zr_movie();
+  g_alloc_bytes = 0;
   mv->device = device;
   mv->p = *p;
   mv->N = n_frames;
@@ -277,7 +330,6 @@ extern "C" int zr_movie_create(int device, const zr_params* p, int n_frames, int
   mv->Wh = mv->Wc / 2 + 1;
   mv->m = p->max_shift;
   if (mv->m <= 0 || 2 * mv->m > 128 || 2 * mv->m > mv->Hc || 2 * mv->m > mv->Wc) { free_movie(mv); return fail(ZR_ERR_UNSUPPORTED, "maxShift %d: built range is 1..64 (and 2*maxShift <= cropped size)", mv->m); }
-  if (p->b_mode == ZR_B_OPTI) { free_movie(mv); return fail(ZR_ERR_UNSUPPORTED, "Bmode='opti' is not built yet; use 'conv' or none"); }
   if (p->sub_pix_reg > 1 && (p->sub_pix_reg & (p->sub_pix_reg - 1)) != 0) { free_movie(mv); return fail(ZR_ERR_UNSUPPORTED, "subPixReg must be a power of two"); }
   if (p->sub_pix_reg > 32) { free_movie(mv); return fail(ZR_ERR_UNSUPPORTED, "subPixReg > 32 is not built"); }
   int rc;
@@ -349,9 +401,11 @@ extern "C" int zr_movie_create(int device, const zr_params* p, int n_frames, int
       (rc = dev_alloc(&mv->Sside, mv->spectral ? 0 : nspec * mv->Wk)) ||
       (rc = dev_alloc(&mv->Z, (size_t)mv->Ptot * mv->Wh * 2 * mv->m)) ||
       (rc = dev_alloc(&mv->part, N * std::max<size_t>(nrb, nrb_sums) * 2)) ||
-      (rc = dev_alloc(&mv->stats, N * 2)) || (rc = dev_alloc(&mv->den, (size_t)mv->Ptot)) ||
-      (rc = dev_alloc(&mv->windows, (size_t)mv->Ptot * 4 * mv->m * mv->m)) || (rc = dev_alloc(&mv->shifts, N * N * 2)) ||
-      (rc = dev_alloc(&mv->corr, N * N)) || (rc = dev_alloc(&mv->peaksig, N * N)) || (rc = dev_alloc(&mv->origin, N * N))) {
+      (rc = dev_alloc(&mv->stats, N * 2)) || (rc = dev_alloc(&mv->stats_c, N * 2)) ||
+      (rc = dev_alloc(&mv->den, (size_t)mv->Ptot)) ||
+      (rc = dev_alloc(&mv->windows, (size_t)mv->nwin * 4 * mv->m * mv->m)) || (rc = dev_alloc(&mv->shifts, N * N * 2)) ||
+      (rc = dev_alloc(&mv->corr, N * N)) || (rc = dev_alloc(&mv->peaksig, N * N)) || (rc = dev_alloc(&mv->origin, N * N)) ||
+      (rc = dev_alloc(&mv->bsigma, N * N))) {
     free_movie(mv);
     return rc;
   }
@@ -365,8 +419,22 @@ extern "C" int zr_movie_create(int device, const zr_params* p, int n_frames, int
     free_movie(mv);
     return fail(rc, "%s", zr_stage_error());
   }
+  mv->bytes = g_alloc_bytes + zr_stage_device_bytes(mv->st);
   *out = mv;
   return ZR_OK;
+}
+
+extern "C" int zr_movie_device_bytes(zr_movie* mv, size_t* bytes) {
+  if (!mv || !bytes) return fail(ZR_ERR_ARG, "NULL argument");
+  *bytes = mv->bytes;
+  return ZR_OK;
+}
+
+extern "C" int zr_copy_sums_device(zr_movie* mv, float* image_sum, float* filt_sum, float* even_sum, float* odd_sum) {
+  if (!mv) return fail(ZR_ERR_ARG, "NULL movie");
+  ZR_HIP(hipSetDevice(mv->device));
+  int rc = zr_stage_copy_sums_device(mv->st, image_sum, filt_sum, even_sum, odd_sum, mv->stream);
+  return rc ? fail(rc, "%s", zr_stage_error()) : ZR_OK;
 }
 
 extern "C" int zr_movie_set_stream(zr_movie* mv, void* s) {
@@ -382,7 +450,10 @@ extern "C" int zr_movie_set_frames(zr_movie* mv, const float* frames, int on_dev
     mv->frames = frames;
   } else {
     size_t bytes = (size_t)mv->N * mv->H * mv->W * sizeof(float);
-    if (!mv->frames_owned) ZR_HIP(hipMalloc((void**)&mv->frames_owned, bytes));
+    if (!mv->frames_owned) {
+      ZR_HIP(hipMalloc((void**)&mv->frames_owned, bytes));
+      mv->bytes += bytes;
+    }
     ZR_HIP(hipMemcpyAsync(mv->frames_owned, frames, bytes, hipMemcpyHostToDevice, mv->stream));
     mv->frames = mv->frames_owned;
   }
@@ -399,6 +470,8 @@ extern "C" int zr_xcorr(zr_movie* mv) {
   hipEvent_t t0 = mv->timer.begin(s);
   const int full = mv->spectral ? 1 : 0;
   const double nc2 = 1.0 / std::pow((double)mv->Hc * (double)mv->Wc, 2.0);
+  const bool refine = mv->p.tri_mode == ZR_TRI_REFINE;
+  const size_t slot = (size_t)mv->Wk * mv->Hs;
   if (mv->unmasked) {
     // K1: spectra in batches of frames; frame sums on the way
     for (int f0 = 0; f0 < mv->N; f0 += mv->batch) {
@@ -408,18 +481,30 @@ extern "C" int zr_xcorr(zr_movie* mv) {
       ZR_HIP(zr_launch_k1_cols(mv->Hp, mv->T, mv->Wk, mv->Hc, f0, nb, mv->twHp, mv->S, mv->Sside, full, s));
     }
     ZR_HIP(zr_launch_k1_stats(mv->part, mv->Hp / rb, mv->N, mv->stats, s));
-    ZR_HIP(zr_launch_k2_den(mv->pairs, mv->P, mv->stats, (double)mv->Hp * (double)mv->Wp, nc2, mv->den, s));
+    if (refine) {
+      ZR_HIP(zr_launch_spec_sum(mv->S, slot, 0, mv->N, mv->N, s));
+      if (!full) ZR_HIP(zr_launch_spec_sum(mv->Sside, (size_t)mv->Wk, 0, mv->N, mv->N, s));
+      ZR_HIP(zr_launch_refine_den(mv->stats, mv->N, (double)mv->Hp * (double)mv->Wp, nc2, mv->den, s));
+    } else {
+      ZR_HIP(zr_launch_k2_den(mv->pairs, mv->P, mv->stats, (double)mv->Hp * (double)mv->Wp, nc2, mv->den, s));
+    }
   } else {
     // masked: frame means first, then the spectra of m(f - mu) (slots 0..N),
-    // (m(f - mu))^2 (slots N..2N) and the mask m (slot 2N)
+    // (m(f - mu))^2 (slots N..2N) and the mask m (slot 2N).  'refine' centres
+    // every frame by the mean over all frames instead (k_refine_combine).
     int nrb = 0;
     ZR_HIP(zr_launch_k1_sums(mv->frames, mv->N, mv->H, mv->W, mv->part, &nrb, s));
     ZR_HIP(zr_launch_k1_stats(mv->part, nrb, mv->N, mv->stats, s));
+    const double* mu = mv->stats;
+    if (refine) {
+      ZR_HIP(zr_launch_common_mean(mv->stats, mv->N, mv->stats_c, s));
+      mu = mv->stats_c;
+    }
     for (int f0 = 0; f0 < mv->N; f0 += mv->batch) {
       int nb = std::min(mv->batch, mv->N - f0);
       for (int sq = 0; sq < 2; ++sq) {
         ZR_HIP(zr_launch_k1_rows(mv->Wp, mv->frames, mv->H, mv->W, mv->Hp, mv->Wk, f0, nb, mv->twWp2, mv->twWp,
-                                 mv->T, mv->part, sq ? 2 : 1, mv->stats, &rb, s));
+                                 mv->T, mv->part, sq ? 2 : 1, mu, &rb, s));
         ZR_HIP(zr_launch_k1_cols(mv->Hp, mv->T, mv->Wk, mv->Hc, f0 + sq * mv->N, nb, mv->twHp, mv->S, mv->Sside,
                                  full, s));
       }
@@ -427,6 +512,12 @@ extern "C" int zr_xcorr(zr_movie* mv) {
     ZR_HIP(zr_launch_k1_rows(mv->Wp, mv->frames, mv->H, mv->W, mv->Hp, mv->Wk, 0, 1, mv->twWp2, mv->twWp, mv->T,
                              mv->part, 3, mv->stats, &rb, s));
     ZR_HIP(zr_launch_k1_cols(mv->Hp, mv->T, mv->Wk, mv->Hc, 2 * mv->N, 1, mv->twHp, mv->S, mv->Sside, full, s));
+    if (refine) {
+      for (int q = 0; q < 2; ++q) {
+        ZR_HIP(zr_launch_spec_sum(mv->S, slot, q * mv->N, mv->N, 2 * mv->N + 1 + q, s));
+        ZR_HIP(zr_launch_spec_sum(mv->Sside, (size_t)mv->Wk, q * mv->N, mv->N, 2 * mv->N + 1 + q, s));
+      }
+    }
   }
   mv->timer.end(ZR_STAGE_SPECTRA, t0, s);
   // K2
@@ -435,7 +526,10 @@ extern "C" int zr_xcorr(zr_movie* mv) {
                            mv->unmasked ? 1 : 0, mv->twHc, mv->Z, mv->Wk, mv->Hs, mv->hoff, mv->side_in_col, s));
   ZR_HIP(zr_launch_k2_rows(mv->Z, mv->pairs, mv->Ptot, mv->Wh, mv->Wc, mv->m, mv->twWc, mv->den, (float)nc2,
                            mv->windows, s));
-  if (!mv->unmasked) ZR_HIP(zr_launch_mnxc_combine(mv->windows, mv->pairs, mv->P, mv->N, mv->m, s));
+  if (refine)
+    ZR_HIP(zr_launch_refine_combine(mv->windows, mv->N, mv->m, mv->unmasked ? 0 : 1, mv->win_out0, mv->stats_c,
+                                    (double)mv->H * (double)mv->W, s));
+  else if (!mv->unmasked) ZR_HIP(zr_launch_mnxc_combine(mv->windows, mv->pairs, mv->P, mv->N, mv->m, s));
   mv->timer.end(ZR_STAGE_PAIRS, t1, s);
   // peaks
   hipEvent_t t2 = mv->timer.begin(s);
@@ -445,7 +539,10 @@ extern "C" int zr_xcorr(zr_movie* mv) {
   ZR_HIP(hipMemsetAsync(mv->origin, 0, sizeof(float) * mv->N * mv->N, s));
   const PeakParamsHost hp = peak_params(mv);
   ZR_HIP(zr_launch_peaks(mv->windows, mv->pairs, mv->P, &hp, mv->gy, mv->gx, mv->ktab, mv->shifts, mv->corr,
-                         mv->peaksig, mv->origin, s));
+                         mv->peaksig, mv->origin, mv->bsigma, s));
+  if (mv->p.tri_mode == ZR_TRI_AUTO)
+    ZR_HIP(zr_launch_auto_prune(mv->shifts, mv->corr, mv->peaksig, mv->origin, mv->N, mv->p.auto_max,
+                                mv->p.peaksig_thres, s));
   mv->timer.end(ZR_STAGE_PEAKS, t2, s);
   mv->have_xcorr = true;
   return ZR_OK;
@@ -455,6 +552,19 @@ extern "C" int zr_solve(zr_movie* mv) {
   if (!mv) return fail(ZR_ERR_ARG, "NULL movie");
   if (!mv->have_xcorr) return fail(ZR_ERR_STATE, "zr_solve before zr_xcorr");
   ZR_HIP(hipSetDevice(mv->device));
+  if (mv->p.tri_mode == ZR_TRI_AUTO) {
+    // the equations follow the pairs that survived the row-wise stop:
+    // triIndices = corrTriMat[:-1, 1:] != 0 (zorro/zorro.py:1282)
+    std::vector<float> corr((size_t)mv->N * mv->N);
+    ZR_HIP(hipMemcpyAsync(corr.data(), mv->corr, corr.size() * sizeof(float), hipMemcpyDeviceToHost, mv->stream));
+    ZR_HIP(hipStreamSynchronize(mv->stream));
+    std::vector<int4> pr;
+    for (int I = 0; I < mv->N - 1; ++I)
+      for (int J = I + 1; J < mv->N; ++J)
+        if (corr[(size_t)I * mv->N + J] != 0.f) pr.push_back(make_int4(I, J, (int)pr.size(), 0));
+    int rc = zr_stage_set_pairs(mv->st, pr);
+    if (rc) return fail(rc, "%s", zr_stage_error());
+  }
   hipEvent_t t0 = mv->timer.begin(mv->stream);
   int rc = zr_stage_solve(mv->st, mv->shifts, mv->corr, mv->peaksig, mv->stream);
   mv->timer.end(ZR_STAGE_SOLVE, t0, mv->stream);
@@ -524,6 +634,12 @@ extern "C" int zr_synchronize(zr_movie* mv) {
   return ZR_OK;
 }
 
+extern "C" int zr_equation_count(zr_movie* mv, int* m) {
+  if (!mv || !m) return fail(ZR_ERR_ARG, "NULL argument");
+  *m = zr_stage_equation_count(mv->st);
+  return ZR_OK;
+}
+
 extern "C" int zr_pair_count(zr_movie* mv, int* npairs) {
   if (!mv || !npairs) return fail(ZR_ERR_ARG, "NULL argument");
   *npairs = mv->P;
@@ -543,12 +659,23 @@ extern "C" int zr_get_trimats(zr_movie* mv, float* shifts, float* corr, float* p
   return ZR_OK;
 }
 
+extern "C" int zr_get_peak_sigmas(zr_movie* mv, float* sigma) {
+  if (!mv || !sigma) return fail(ZR_ERR_ARG, "NULL argument");
+  if (!mv->have_xcorr) return fail(ZR_ERR_STATE, "no correlation results");
+  if (mv->p.b_mode != ZR_B_OPTI) return fail(ZR_ERR_STATE, "per-pair sigmas exist only for Bmode='opti'");
+  ZR_HIP(hipSetDevice(mv->device));
+  ZR_HIP(hipStreamSynchronize(mv->stream));
+  ZR_HIP(hipMemcpy(sigma, mv->bsigma, (size_t)mv->N * mv->N * sizeof(float), hipMemcpyDeviceToHost));
+  return ZR_OK;
+}
+
 extern "C" int zr_get_windows(zr_movie* mv, float* windows) {
   if (!mv || !windows) return fail(ZR_ERR_ARG, "NULL argument");
   if (!mv->have_xcorr) return fail(ZR_ERR_STATE, "no correlation results");
   ZR_HIP(hipSetDevice(mv->device));
   ZR_HIP(hipStreamSynchronize(mv->stream));
-  ZR_HIP(hipMemcpy(windows, mv->windows, (size_t)mv->P * 4 * mv->m * mv->m * sizeof(float), hipMemcpyDeviceToHost));
+  const size_t nn = 4 * (size_t)mv->m * mv->m;
+  ZR_HIP(hipMemcpy(windows, mv->windows + mv->win_out0 * nn, (size_t)mv->P * nn * sizeof(float), hipMemcpyDeviceToHost));
   return ZR_OK;
 }
 

@@ -6,6 +6,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <mutex>
+
 #define ZR_DEV __device__ __forceinline__
 
 // ---------------------------------------------------------------- complex
@@ -279,4 +281,20 @@ ZR_DEV double wave_sum_d(double v) {
 ZR_DEV float wave_sum_f(float v) {
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
   return v;
+}
+
+// ------------------------------------------------------ host: LDS opt-in
+// Let a kernel take all of a workgroup's LDS (160 KiB on gfx950, minus its
+// static LDS) as dynamic memory, once per process.  The dynamic size a launch
+// asks for varies with the movie's shape (maxShift, Gaussian radius, frame
+// count), and movies may launch from several threads; one maximal opt-in
+// avoids racing per-size updates.
+inline void zr_allow_full_lds(const void* fn, std::once_flag& flag) {
+  std::call_once(flag, [fn] {
+    hipFuncAttributes fa{};
+    size_t stat = 0;
+    if (hipFuncGetAttributes(&fa, fn) == hipSuccess) stat = fa.sharedSizeBytes;
+    const size_t lim = 160 * 1024;
+    hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)(lim > stat ? lim - stat : 0));
+  });
 }

@@ -3,7 +3,7 @@
 #include <hip/hip_runtime.h>
 
 struct PeakParamsHost {
-  int m, suppress, bconv, ry, rx, subpix;
+  int m, suppress, bconv, bopti, ry, rx, subpix;
   float scale_y, scale_x;
   int n;
 };
@@ -18,7 +18,7 @@ inline size_t zr_peaks_lds_bytes(const PeakParamsHost* hp, int* tmp_floats) {
   const size_t n2 = 2 * (size_t)hp->m, nn = n2 * n2;
   const size_t G = 16 * (size_t)(hp->subpix > 1 ? hp->subpix : 1);
   size_t t = nn > 34 * G ? nn : 34 * G;
-  if (hp->bconv) {
+  if (hp->bconv || hp->bopti) {
     const size_t r = (size_t)(hp->ry > hp->rx ? hp->ry : hp->rx);
     const size_t padded = (n2 + 2 * r) * n2;
     t = t > padded ? t : padded;
@@ -50,4 +50,13 @@ hipError_t zr_launch_k2_den(const int4* pairs, int npairs, const double* stats, 
                             hipStream_t st);
 hipError_t zr_launch_peaks(const float* windows, const int4* pairs, int npairs, const PeakParamsHost* hp,
                            const float* gy, const float* gx, const float2* ktab, float* shifts, float* corr,
-                           float* peaksig, float* origin, hipStream_t st);
+                           float* peaksig, float* origin, float* bsigma, hipStream_t st);
+// triMode 'auto': zero the trimat entries after each row's stopping pair
+hipError_t zr_launch_auto_prune(float* shifts, float* corr, float* peaksig, float* origin, int N, int auto_max,
+                                double thres, hipStream_t st);
+// triMode 'refine': sums of spectra slots, closed-form denominators, window combination
+hipError_t zr_launch_spec_sum(float2* S, size_t slot_elems, int slot0, int count, int out_slot, hipStream_t st);
+hipError_t zr_launch_refine_den(const double* stats, int N, double npix, double nc2, double* den, hipStream_t st);
+hipError_t zr_launch_refine_combine(float* windows, int N, int m, int masked, int out0, const double* cm,
+                                    double npix, hipStream_t st);
+hipError_t zr_launch_common_mean(const double* stats, int N, double* out, hipStream_t st);

@@ -13,6 +13,14 @@ const char* zr_stage_error();
 int zr_stage_create(zr_stage_state** out, const zr_params& p, int N, int H, int W, int Hp, int Wp,
                     const std::vector<int4>& pairs, int spectral);
 void zr_stage_free(zr_stage_state* s);
+// Replace the solver's equations with those of a new pair set (triMode 'auto'
+// after its row-wise early stop); at most the pair count given at creation.
+int zr_stage_set_pairs(zr_stage_state* s, const std::vector<int4>& pairs);
+int zr_stage_equation_count(zr_stage_state* s);
+size_t zr_stage_device_bytes(const zr_stage_state* s);
+// asynchronous device-to-device copies of the sums (any pointer may be NULL)
+int zr_stage_copy_sums_device(zr_stage_state* s, float* image_sum, float* filt_sum, float* even_sum, float* odd_sum,
+                              hipStream_t st);
 int zr_stage_solve(zr_stage_state* s, const float* shifts, const float* corr, const float* peaksig, hipStream_t st);
 int zr_stage_shifts(zr_stage_state* s, const float* frames, float* aligned_out, hipStream_t st);
 int zr_stage_frc(zr_stage_state* s, hipStream_t st);

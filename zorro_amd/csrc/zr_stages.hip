@@ -216,7 +216,8 @@ struct SolveArgs {
   const float* shifts;    // [N][N][2]
   const float* corr;      // [N][N]
   const float* peaksig;   // [N][N]
-  const int2* edges;      // [M] (I, J), canonical order
+  const int2* edges;      // [M] graph edge (i, j): the equation p_j - p_i = b
+  const int2* meas;       // [M] trimat entry (I, J) holding b and the peak significance
   int N, M;
   int weight_mode;
   double logi_fixed[3];
@@ -304,9 +305,9 @@ __global__ __launch_bounds__(L1_NT) void k_l1_solve(SolveArgs a) {
     const int e = tid + k * L1_NT;
     eij[k] = -1; eb[k] = 0.0; ew[k] = 0.0; er[k] = 0.0;
     if (e < M) {
-      const int2 ed = a.edges[e];
-      const double bb = (double)a.shifts[((size_t)ed.x * N + ed.y) * 2 + axis];
-      const double ps = (double)a.peaksig[ed.x * N + ed.y];
+      const int2 ed = a.edges[e], ms = a.meas[e];
+      const double bb = (double)a.shifts[((size_t)ms.x * N + ms.y) * 2 + axis];
+      const double ps = (double)a.peaksig[ms.x * N + ms.y];
       double we;
       switch (a.weight_mode) {
         case 0: we = logistic_d(ps, a.info->logi[0], a.info->logi[1], a.info->logi[2]); break;
@@ -783,7 +784,7 @@ struct PostArgs {
   const double* pos;      // [2][N]
   const double* weights;  // [M]
   const double* resid;    // [M][2]
-  const int2* edges;
+  const int2* meas;       // [M] trimat entry of each equation
   int N, M;
   int centroid, lanczos, subpix;
   int H, W, box;
@@ -796,6 +797,19 @@ struct PostArgs {
   FrameShift* fs;
   SolveInfo* info;
 };
+
+// Modes without a solver: triMode 'first' takes translations = -shiftsTriMat[0, :]
+// (zorro/zorro.py:1628-1629), 'refine' -shiftsTriMat[K, K] (:1639-1643).
+// Positions p = -translations.
+__global__ __launch_bounds__(64) void k_direct_positions(const float* __restrict__ shifts, int N, int refine,
+                                                         double* __restrict__ pos, SolveInfo* info) {
+  for (int k = threadIdx.x; k < N; k += 64) {
+    const size_t e = refine ? ((size_t)k * N + k) : (size_t)k;
+    pos[k] = (double)shifts[e * 2 + 0];
+    pos[N + k] = (double)shifts[e * 2 + 1];
+  }
+  if (threadIdx.x == 0) { info->status_x = 1; info->status_y = 1; }
+}
 
 __global__ __launch_bounds__(64) void k_solution_post(PostArgs a) {
   const int lane = threadIdx.x;
@@ -827,7 +841,7 @@ __global__ __launch_bounds__(64) void k_solution_post(PostArgs a) {
       sw += w; swe += w * en; swe2 += w * en * en;
     }
     sw = wave_sum_d(sw); swe = wave_sum_d(swe); swe2 = wave_sum_d(swe2);
-    if (lane == 0) {
+    if (lane == 0 && M > 0) {
       double mean = swe / sw;
       a.err_stats[0] = mean;
       a.err_stats[1] = sqrt(swe2 / sw - mean * mean);
@@ -873,7 +887,7 @@ __global__ __launch_bounds__(64) void k_solution_post(PostArgs a) {
     // padded[i][j] holds the equation (i, j) for frame pair (i, j), i < j
     double nx = 0, ny = 0, dn = 0;
     for (int e = 0; e < M; ++e) {
-      int2 ed = a.edges[e];
+      int2 ed = a.meas[e];
       double w = fmax(a.weights[e], 1e-6);
       bool row = (ed.x == k);                           // padded[k][:]
       bool col = (ed.y == ((k - 1 + N) % N));           // padded[:][k-1] (k=0 -> last column)
@@ -884,8 +898,8 @@ __global__ __launch_bounds__(64) void k_solution_post(PostArgs a) {
         dn += c * w;
       }
     }
-    a.err_frames[2 * k] = ny / dn;
-    a.err_frames[2 * k + 1] = nx / dn;
+    a.err_frames[2 * k] = dn > 0.0 ? ny / dn : 0.0;
+    a.err_frames[2 * k + 1] = dn > 0.0 ? nx / dn : 0.0;
   }
   __syncthreads();
   if (lane == 0) {
@@ -1591,12 +1605,61 @@ hipError_t upload(T** p, const std::vector<T>& v) {
   if (e != hipSuccess) return e;
   return hipMemcpy(*p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice);
 }
+
+// The equations of shiftsSolver (zorro/zorro.py:1290-1320) for a pair set:
+// tri[I][J-1] marks the pair (I, J); row I contributes the rows of
+// rot90(tri[I:lc, I:lc], 2), lc = last marked column of row I + 1, and the
+// measured shifts are the marked entries in row-major order.  Every equation
+// row is a contiguous run of unknowns x_a..x_b (x_j = p_{j+1} - p_j), i.e. the
+// graph edge (a, b+1) in position space; meas[e] is the trimat entry that
+// holds its measurement.  For tri and diag patterns the edge of each equation
+// is its own pair; other patterns can give an equation whose run differs from
+// the pair whose shift it is matched with, exactly as in the reference.
+int build_equations(int N, const std::vector<int4>& pairs, std::vector<int2>& graph, std::vector<int2>& meas) {
+  const int n = N - 1;
+  std::vector<unsigned char> tri((size_t)n * n, 0);
+  for (const int4& q : pairs) {
+    if (q.x < 0 || q.y <= q.x || q.y > n) return -1;
+    tri[(size_t)q.x * n + (q.y - 1)] = 1;
+  }
+  graph.clear();
+  meas.clear();
+  for (int I = 0; I < n; ++I)
+    for (int c = 0; c < n; ++c)
+      if (tri[(size_t)I * n + c]) meas.push_back(make_int2(I, c + 1));
+  for (int I = 0; I < n; ++I) {
+    int lc = -1;
+    for (int c = 0; c < n; ++c)
+      if (tri[(size_t)I * n + c]) lc = c + 1;
+    if (lc < 0) continue;
+    const int k = lc - I;
+    if (k <= 0) return -1;
+    for (int a = 0; a < k; ++a) {
+      // rot[a][b] = tri[I + k-1-a][I + k-1-b]
+      const int row = I + k - 1 - a;
+      int c0 = -1, c1 = -1;
+      bool gap = false;
+      for (int b = 0; b < k; ++b) {
+        const bool on = tri[(size_t)row * n + (I + k - 1 - b)] != 0;
+        if (on) {
+          if (c0 < 0) c0 = b;
+          else if (c1 != b - 1) gap = true;
+          c1 = b;
+        }
+      }
+      if (c0 < 0 || gap) return -2;
+      graph.push_back(make_int2(I + c0, I + c1 + 1));
+    }
+  }
+  return graph.size() == meas.size() ? 0 : -3;
+}
 }  // namespace
 
 struct zr_stage_state {
   zr_params p{};
-  int N = 0, H = 0, W = 0, Hp = 0, Wp = 0, M = 0;
-  int2* edges = nullptr;
+  int N = 0, H = 0, W = 0, Hp = 0, Wp = 0, M = 0, Mcap = 0;
+  size_t bytes = 0;            // device memory held
+  int2 *edges = nullptr, *meas = nullptr;
   float *hs = nullptr, *cdf = nullptr;
   double* obj = nullptr;
   SolveInfo* info = nullptr;
@@ -1619,9 +1682,37 @@ struct zr_stage_state {
 
 const char* zr_stage_error() { return s_err.c_str(); }
 
+// device allocations of a stage state, counted for the movie's memory footprint
+static hipError_t smalloc(zr_stage_state* s, void** p, size_t bytes) {
+  hipError_t e = hipMalloc(p, bytes);
+  if (e == hipSuccess) s->bytes += bytes;
+  return e;
+}
+template <typename T>
+static hipError_t supload(zr_stage_state* s, T** p, const std::vector<T>& v) {
+  hipError_t e = upload(p, v);
+  if (e == hipSuccess) s->bytes += v.size() * sizeof(T);
+  return e;
+}
+size_t zr_stage_device_bytes(const zr_stage_state* s) { return s ? s->bytes : 0; }
+
+int zr_stage_copy_sums_device(zr_stage_state* s, float* image_sum, float* filt_sum, float* even_sum, float* odd_sum,
+                              hipStream_t st) {
+  if (!s || !s->have_sums) return serr(ZR_ERR_STATE, "no sums");
+  const size_t b = sizeof(float) * s->H * s->W;
+  if (image_sum) ZS_HIP(hipMemcpyAsync(image_sum, s->img_sum, b, hipMemcpyDeviceToDevice, st));
+  if (even_sum) ZS_HIP(hipMemcpyAsync(even_sum, s->even, b, hipMemcpyDeviceToDevice, st));
+  if (odd_sum) ZS_HIP(hipMemcpyAsync(odd_sum, s->odd, b, hipMemcpyDeviceToDevice, st));
+  if (filt_sum) {
+    if (!s->have_dose) return serr(ZR_ERR_STATE, "no dose-filtered sum");
+    ZS_HIP(hipMemcpyAsync(filt_sum, s->filt, b, hipMemcpyDeviceToDevice, st));
+  }
+  return ZR_OK;
+}
+
 void zr_stage_free(zr_stage_state* s) {
   if (!s) return;
-  void* ptrs[] = {s->edges, s->hs, s->cdf, s->obj, s->info, s->weights, s->resid, s->pos, s->trans_raw, s->trans,
+  void* ptrs[] = {s->edges, s->meas, s->hs, s->cdf, s->obj, s->info, s->weights, s->resid, s->pos, s->trans_raw, s->trans,
                   s->rel, s->err_stats, s->err_frames, s->fs, s->img_sum, s->even, s->odd, s->filt, s->rowsum,
                   s->twB, s->twB2, s->Tfrc, s->frc2d, s->frc, s->twHp, s->twWp, s->twWp2, s->Tdose, s->G,
                   s->accum, s->thresq, s->drow, s->R0, s->RH, s->C0, s->CW, s->Py, s->Px};
@@ -1637,34 +1728,41 @@ int zr_stage_create(zr_stage_state** out, const zr_params& p, int N, int H, int 
   zr_stage_state* s = new zr_stage_state();
   s->p = p;
   s->N = N; s->H = H; s->W = W; s->Hp = Hp; s->Wp = Wp;
-  s->M = (int)pairs.size();
+  s->M = s->Mcap = (int)pairs.size();
   if (s->M > 2016) { zr_stage_free(s); return serr(ZR_ERR_UNSUPPORTED, "too many equations"); }
-  std::vector<int2> ed(s->M);
-  for (int i = 0; i < s->M; ++i) ed[i] = make_int2(pairs[i].x, pairs[i].y);
-  // the equations are the pair intervals [I, J) (rot90 construction of the
-  // reference for tri/diag patterns): edge I-J in position space
+  std::vector<int2> ed, ms;
+  if (p.tri_mode == ZR_TRI_FIRST || p.tri_mode == ZR_TRI_REFINE) {
+    // no solver: translations come straight from the trimats
+    ed.assign(1, make_int2(0, 0));
+    ms = ed;
+    s->M = 0;
+  } else if (build_equations(N, pairs, ed, ms) != 0) {
+    zr_stage_free(s);
+    return serr(ZR_ERR_UNSUPPORTED, "pair pattern gives equations the solver cannot represent");
+  }
 #define ZC(x) do { hipError_t e_ = (x); if (e_ != hipSuccess) { zr_stage_free(s); return serr(ZR_ERR_NOMEM, "%s: %s", #x, hipGetErrorString(e_)); } } while (0)
-  ZC(upload(&s->edges, ed));
-  const int maxbins = std::max(1, s->M / 7 + 1);
-  ZC(hipMalloc((void**)&s->hs, sizeof(float) * maxbins));
-  ZC(hipMalloc((void**)&s->cdf, sizeof(float) * maxbins));
-  ZC(hipMalloc((void**)&s->obj, sizeof(double) * LOGI_G0 * LOGI_G1 * LOGI_G2));
-  ZC(hipMalloc((void**)&s->info, sizeof(SolveInfo)));
-  ZC(hipMalloc((void**)&s->weights, sizeof(double) * s->M));
-  ZC(hipMalloc((void**)&s->resid, sizeof(double) * s->M * 2));
-  ZC(hipMalloc((void**)&s->pos, sizeof(double) * N * 2));
-  ZC(hipMalloc((void**)&s->trans_raw, sizeof(double) * N * 2));
-  ZC(hipMalloc((void**)&s->trans, sizeof(double) * N * 2));
-  ZC(hipMalloc((void**)&s->rel, sizeof(double) * N * 2));
-  ZC(hipMalloc((void**)&s->err_stats, sizeof(double) * 2));
-  ZC(hipMalloc((void**)&s->err_frames, sizeof(double) * N * 2));
-  ZC(hipMalloc((void**)&s->fs, sizeof(FrameShift) * N));
-  ZC(hipMalloc((void**)&s->img_sum, sizeof(float) * H * W));
-  ZC(hipMalloc((void**)&s->even, sizeof(float) * H * W));
-  ZC(hipMalloc((void**)&s->odd, sizeof(float) * H * W));
-  ZC(hipMalloc((void**)&s->filt, sizeof(float) * H * W));
+  ZC(supload(s, &s->edges, ed));
+  ZC(supload(s, &s->meas, ms));
+  const int maxbins = std::max(1, s->Mcap / 7 + 1);
+  ZC(smalloc(s, (void**)&s->hs, sizeof(float) * maxbins));
+  ZC(smalloc(s, (void**)&s->cdf, sizeof(float) * maxbins));
+  ZC(smalloc(s, (void**)&s->obj, sizeof(double) * LOGI_G0 * LOGI_G1 * LOGI_G2));
+  ZC(smalloc(s, (void**)&s->info, sizeof(SolveInfo)));
+  ZC(smalloc(s, (void**)&s->weights, sizeof(double) * std::max(1, s->Mcap)));
+  ZC(smalloc(s, (void**)&s->resid, sizeof(double) * std::max(1, s->Mcap) * 2));
+  ZC(smalloc(s, (void**)&s->pos, sizeof(double) * N * 2));
+  ZC(smalloc(s, (void**)&s->trans_raw, sizeof(double) * N * 2));
+  ZC(smalloc(s, (void**)&s->trans, sizeof(double) * N * 2));
+  ZC(smalloc(s, (void**)&s->rel, sizeof(double) * N * 2));
+  ZC(smalloc(s, (void**)&s->err_stats, sizeof(double) * 2));
+  ZC(smalloc(s, (void**)&s->err_frames, sizeof(double) * N * 2));
+  ZC(smalloc(s, (void**)&s->fs, sizeof(FrameShift) * N));
+  ZC(smalloc(s, (void**)&s->img_sum, sizeof(float) * H * W));
+  ZC(smalloc(s, (void**)&s->even, sizeof(float) * H * W));
+  ZC(smalloc(s, (void**)&s->odd, sizeof(float) * H * W));
+  ZC(smalloc(s, (void**)&s->filt, sizeof(float) * H * W));
   const int nrow_blocks = ((W + 63) / 64) * ((H + 3) / 4);
-  ZC(hipMalloc((void**)&s->rowsum, sizeof(double) * std::max(nrow_blocks, Hp / 4 + 1)));
+  ZC(smalloc(s, (void**)&s->rowsum, sizeof(double) * std::max(nrow_blocks, Hp / 4 + 1)));
   if (p.do_frc) {
     s->box = p.frc_box;
     if (s->box != 512 && s->box != 256 && s->box != 1024) { zr_stage_free(s); return serr(ZR_ERR_UNSUPPORTED, "FRC box %d: built sizes 256, 512, 1024", s->box); }
@@ -1672,47 +1770,80 @@ int zr_stage_create(zr_stage_state** out, const zr_params& p, int N, int H, int 
     s->tmax_y = std::max(0, (int)std::floor((double)(H - s->box) / s->box / (1.0 - ov)));
     s->tmax_x = std::max(0, (int)std::floor((double)(W - s->box) / s->box / (1.0 - ov)));
     size_t nt = (size_t)std::max(1, s->tmax_y * s->tmax_x);
-    ZC(upload(&s->twB, fwd_table(s->box)));
-    ZC(upload(&s->twB2, fwd_table(s->box / 2)));
-    ZC(hipMalloc((void**)&s->Tfrc, sizeof(float2) * nt * 2 * (s->box / 2 + 1) * s->box));
-    ZC(hipMalloc((void**)&s->frc2d, sizeof(double) * s->box * (s->box / 2 + 1)));
-    ZC(hipMalloc((void**)&s->frc, sizeof(double) * s->box));
+    ZC(supload(s, &s->twB, fwd_table(s->box)));
+    ZC(supload(s, &s->twB2, fwd_table(s->box / 2)));
+    ZC(smalloc(s, (void**)&s->Tfrc, sizeof(float2) * nt * 2 * (s->box / 2 + 1) * s->box));
+    ZC(smalloc(s, (void**)&s->frc2d, sizeof(double) * s->box * (s->box / 2 + 1)));
+    ZC(smalloc(s, (void**)&s->frc, sizeof(double) * s->box));
   }
   if (p.do_dose) {
     if (!(p.pixelsize_nm > 0)) { zr_stage_free(s); return serr(ZR_ERR_ARG, "dose filter needs pixelsize"); }
     if (!(p.voltage_kv > 0)) { zr_stage_free(s); return serr(ZR_ERR_ARG, "Accelerating voltage not set in advance for dose filtering"); }
-    ZC(upload(&s->twHp, fwd_table(Hp)));
-    ZC(upload(&s->twWp, fwd_table(Wp)));
-    ZC(upload(&s->twWp2, fwd_table(Wp / 2)));
+    ZC(supload(s, &s->twHp, fwd_table(Hp)));
+    ZC(supload(s, &s->twWp, fwd_table(Wp)));
+    ZC(supload(s, &s->twWp2, fwd_table(Wp / 2)));
     s->spectral = spectral != 0;
     if (s->spectral) {
       const size_t wh = (size_t)N * (Wp / 2 + 1), hh = (size_t)N * Hp;
-      ZC(hipMalloc((void**)&s->R0, sizeof(float2) * wh));
-      ZC(hipMalloc((void**)&s->RH, sizeof(float2) * wh));
-      ZC(hipMalloc((void**)&s->Px, sizeof(float2) * wh));
-      ZC(hipMalloc((void**)&s->C0, sizeof(float2) * hh));
-      ZC(hipMalloc((void**)&s->CW, sizeof(float2) * hh));
-      ZC(hipMalloc((void**)&s->Py, sizeof(float2) * hh));
+      ZC(smalloc(s, (void**)&s->R0, sizeof(float2) * wh));
+      ZC(smalloc(s, (void**)&s->RH, sizeof(float2) * wh));
+      ZC(smalloc(s, (void**)&s->Px, sizeof(float2) * wh));
+      ZC(smalloc(s, (void**)&s->C0, sizeof(float2) * hh));
+      ZC(smalloc(s, (void**)&s->CW, sizeof(float2) * hh));
+      ZC(smalloc(s, (void**)&s->Py, sizeof(float2) * hh));
     } else {
-      ZC(hipMalloc((void**)&s->Tdose, sizeof(float2) * (size_t)N * (Wp / 2 + 1) * Hp));
+      ZC(smalloc(s, (void**)&s->Tdose, sizeof(float2) * (size_t)N * (Wp / 2 + 1) * Hp));
     }
-    ZC(hipMalloc((void**)&s->G, sizeof(float2) * (size_t)(Wp / 2 + 1) * H));
-    ZC(hipMalloc((void**)&s->accum, sizeof(double) * (N + 1)));
-    ZC(hipMalloc((void**)&s->thresq, sizeof(double) * N));
-    ZC(hipMalloc((void**)&s->drow, sizeof(double) * (Hp / 4 + 1)));
+    ZC(smalloc(s, (void**)&s->G, sizeof(float2) * (size_t)(Wp / 2 + 1) * H));
+    ZC(smalloc(s, (void**)&s->accum, sizeof(double) * (N + 1)));
+    ZC(smalloc(s, (void**)&s->thresq, sizeof(double) * N));
+    ZC(smalloc(s, (void**)&s->drow, sizeof(double) * (Hp / 4 + 1)));
   }
 #undef ZC
   *out = s;
   return ZR_OK;
 }
 
+int zr_stage_set_pairs(zr_stage_state* s, const std::vector<int4>& pairs) {
+  if (!s) return serr(ZR_ERR_STATE, "no stage state");
+  std::vector<int2> ed, ms;
+  if (build_equations(s->N, pairs, ed, ms) != 0)
+    return serr(ZR_ERR_UNSUPPORTED, "pair pattern gives equations the solver cannot represent");
+  if ((int)ed.size() > s->Mcap || ed.empty()) return serr(ZR_ERR_ARG, "equation count %zu outside 1..%d", ed.size(), s->Mcap);
+  ZS_HIP(hipMemcpy(s->edges, ed.data(), ed.size() * sizeof(int2), hipMemcpyHostToDevice));
+  ZS_HIP(hipMemcpy(s->meas, ms.data(), ms.size() * sizeof(int2), hipMemcpyHostToDevice));
+  s->M = (int)ed.size();
+  return ZR_OK;
+}
+
+int zr_stage_equation_count(zr_stage_state* s) { return s ? s->M : 0; }
+
+static void fill_post_args(zr_stage_state* s, PostArgs& q) {
+  q.pos = s->pos; q.weights = s->weights; q.resid = s->resid; q.meas = s->meas;
+  q.N = s->N; q.M = s->M; q.centroid = s->p.origin_centroid; q.lanczos = s->p.shift_method == ZR_SHIFT_LANCZOS;
+  q.subpix = s->p.sub_pix_reg; q.H = s->H; q.W = s->W; q.box = s->box; q.overlap = s->p.frc_overlap;
+  q.trans_raw = s->trans_raw; q.trans = s->trans; q.rel = s->rel; q.err_stats = s->err_stats;
+  q.err_frames = s->err_frames; q.fs = s->fs; q.info = s->info;
+}
+
 int zr_stage_solve(zr_stage_state* s, const float* shifts, const float* corr, const float* peaksig, hipStream_t st) {
   if (!s) return serr(ZR_ERR_STATE, "no stage state");
   const int N = s->N, M = s->M;
   ZS_HIP(hipMemsetAsync(s->info, 0, sizeof(SolveInfo), st));
+  if (s->p.tri_mode == ZR_TRI_FIRST || s->p.tri_mode == ZR_TRI_REFINE) {
+    hipLaunchKernelGGL(k_direct_positions, dim3(1), dim3(64), 0, st, shifts, N,
+                       s->p.tri_mode == ZR_TRI_REFINE ? 1 : 0, s->pos, s->info);
+    ZS_HIP(hipGetLastError());
+    PostArgs q;
+    fill_post_args(s, q);
+    hipLaunchKernelGGL(k_solution_post, dim3(1), dim3(64), 0, st, q);
+    ZS_HIP(hipGetLastError());
+    s->have_solution = true;
+    return ZR_OK;
+  }
   const int wm = s->p.weight_mode;
   if (wm == ZR_W_AUTOLOGISTIC) {
-    const int maxbins = std::max(1, M / 7 + 1);
+    const int maxbins = std::max(1, s->Mcap / 7 + 1);
     hipLaunchKernelGGL(k_ps_hist, dim3(1), dim3(256), 0, st, peaksig, N * N, s->hs, s->cdf, s->info, maxbins);
     ZS_HIP(hipGetLastError());
     const int total = LOGI_G0 * LOGI_G1 * LOGI_G2;
@@ -1722,26 +1853,19 @@ int zr_stage_solve(zr_stage_state* s, const float* shifts, const float* corr, co
     ZS_HIP(hipGetLastError());
   }
   SolveArgs a;
-  a.shifts = shifts; a.corr = corr; a.peaksig = peaksig; a.edges = s->edges;
+  a.shifts = shifts; a.corr = corr; a.peaksig = peaksig; a.edges = s->edges; a.meas = s->meas;
   a.N = N; a.M = M; a.weight_mode = wm;
   a.logi_fixed[0] = s->p.peaksig_thres; a.logi_fixed[1] = s->p.logistic_k; a.logi_fixed[2] = s->p.logistic_nu;
   a.bound = 0;
   a.weights = s->weights; a.resid = s->resid; a.pos = s->pos; a.info = s->info;
   if (M > L1_NT * L1_EPT) return serr(ZR_ERR_UNSUPPORTED, "the solver handles up to %d equations (got %d)", L1_NT * L1_EPT, M);
   const size_t dyn = l1_dyn_bytes(M);
-  static int attr = 0;
-  if ((int)dyn > attr) {
-    hipFuncSetAttribute((const void*)k_l1_solve, hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn);
-    attr = (int)dyn;
-  }
+  static std::once_flag lds_once;
+  zr_allow_full_lds((const void*)k_l1_solve, lds_once);
   hipLaunchKernelGGL(k_l1_solve, dim3(2), dim3(L1_NT), dyn, st, a);
   ZS_HIP(hipGetLastError());
   PostArgs q;
-  q.pos = s->pos; q.weights = s->weights; q.resid = s->resid; q.edges = s->edges;
-  q.N = N; q.M = M; q.centroid = s->p.origin_centroid; q.lanczos = s->p.shift_method == ZR_SHIFT_LANCZOS;
-  q.subpix = s->p.sub_pix_reg; q.H = s->H; q.W = s->W; q.box = s->box; q.overlap = s->p.frc_overlap;
-  q.trans_raw = s->trans_raw; q.trans = s->trans; q.rel = s->rel; q.err_stats = s->err_stats;
-  q.err_frames = s->err_frames; q.fs = s->fs; q.info = s->info;
+  fill_post_args(s, q);
   hipLaunchKernelGGL(k_solution_post, dim3(1), dim3(64), 0, st, q);
   ZS_HIP(hipGetLastError());
   s->have_solution = true;
@@ -1845,8 +1969,8 @@ int zr_stage_dose(zr_stage_state* s, const float* frames, const float2* S, int W
   switch (Wp) {
 #define ZR_DR(X)                                                                                              \
   case X: {                                                                                                   \
-    static bool attr = false;                                                                                 \
-    if (!attr) { hipFuncSetAttribute((const void*)k_dose_rows<X>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh); attr = true; } \
+    static std::once_flag lds_once;                                                                           \
+    zr_allow_full_lds((const void*)k_dose_rows<X>, lds_once);                                                 \
     hipLaunchKernelGGL(k_dose_rows<X>, dim3(nblk), dim3(1024), sh, st, frames, s->fs, s->N, s->H, s->W, Hp, s->twWp2, \
                        s->twWp, s->Tdose, s->img_sum, s->even, s->odd, (float*)nullptr, s->drow);            \
   } break;
@@ -1887,14 +2011,15 @@ int zr_stage_get_solution(zr_stage_state* s, double* translations, double* relat
   if (inf.status_x != 1 || inf.status_y != 1)
     return serr(ZR_ERR_STATE, "shift solver stopped before proving optimality (status y %d, x %d)", inf.status_y,
                 inf.status_x);
-  if (s->p.weight_mode == ZR_W_AUTOLOGISTIC && inf.nbins < 1)
+  const bool solver = !(s->p.tri_mode == ZR_TRI_FIRST || s->p.tri_mode == ZR_TRI_REFINE);
+  if (solver && s->p.weight_mode == ZR_W_AUTOLOGISTIC && inf.nbins < 1)
     return serr(ZR_ERR_ARG, "autologistic weights need >= 7 positive peak significances (got %d)", inf.nps);
   const int N = s->N, M = s->M;
   if (translations) ZS_HIP(hipMemcpy(translations, s->trans, sizeof(double) * 2 * N, hipMemcpyDeviceToHost));
   if (relative_est) ZS_HIP(hipMemcpy(relative_est, s->rel, sizeof(double) * 2 * (N - 1), hipMemcpyDeviceToHost));
   if (weights) ZS_HIP(hipMemcpy(weights, s->weights, sizeof(double) * M, hipMemcpyDeviceToHost));
   if (logistic) {
-    if (s->p.weight_mode == ZR_W_AUTOLOGISTIC) for (int i = 0; i < 3; ++i) logistic[i] = inf.logi[i];
+    if (solver && s->p.weight_mode == ZR_W_AUTOLOGISTIC) for (int i = 0; i < 3; ++i) logistic[i] = inf.logi[i];
     else { logistic[0] = s->p.peaksig_thres; logistic[1] = s->p.logistic_k; logistic[2] = s->p.logistic_nu; }
   }
   if (error) ZS_HIP(hipMemcpy(error, s->resid, sizeof(double) * 2 * M, hipMemcpyDeviceToHost));

@@ -76,7 +76,8 @@ __global__ __launch_bounds__(RB * TPR) void k1_rows(const float* __restrict__ fr
     }
   }
   if (mode == K1_CENTRED || mode == K1_SQUARED) {
-    const float mu = (float)(stats[2 * f] / ((double)H * (double)W));
+    // stats == NULL: no centring (m f and (m f)^2 as they are)
+    const float mu = stats ? (float)(stats[2 * f] / ((double)H * (double)W)) : 0.f;
     const bool sq = (mode == K1_SQUARED);
 #pragma unroll
     for (int i = 0; i < PER; ++i) {
@@ -235,6 +236,102 @@ __global__ __launch_bounds__(256) void k_mnxc_combine(float* __restrict__ window
     double den = sqrt((Bt[e] - cb * (cb / mpe)) * (Tt[e] - ct * (ct / mpe)));
     den = (den < 1.0) ? 1.0 : den;
     X[e] = (float)(((double)X[e] - ct * cb / mpe) / den);
+  }
+}
+
+// triMode 'refine' (zorro/zorro.py:934-956): the template of frame I is the sum
+// of all spectra minus its own, so the sums of the spectra are kept as extra
+// slots.  dst[e] = sum_k src[slot0 + k][e] over `count` slots, frame order.
+__global__ __launch_bounds__(256) void k_spec_sum(float2* __restrict__ S, size_t slot_elems, int slot0, int count,
+                                                  int out_slot) {
+  const size_t e = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (e >= slot_elems) return;
+  float2 acc = make_float2(0.f, 0.f);
+  for (int k = 0; k < count; ++k) acc = c_add(acc, S[(size_t)(slot0 + k) * slot_elems + e]);
+  S[(size_t)out_slot * slot_elems + e] = acc;
+}
+
+// Unmasked 'refine' normalisation: the closed form with the template's sums
+// S_T = (S_tot - S_I)/N and SQ_T = (SQ_tot - SQ_I)/N (the template's squared
+// spectrum is (sum Q - Q_I)/N, zorro/zorro.py:945-946), both windows of the
+// pair (sum, I) and (I, I) take the same scale.
+__global__ __launch_bounds__(64) void k_refine_den(const double* __restrict__ stats, int N, double npix, double nc2,
+                                                   double* __restrict__ den) {
+  __shared__ double tot[2];
+  if (threadIdx.x == 0) {
+    double a = 0.0, b = 0.0;
+    for (int f = 0; f < N; ++f) { a += stats[2 * f]; b += stats[2 * f + 1]; }
+    tot[0] = a;
+    tot[1] = b;
+  }
+  __syncthreads();
+  for (int k = threadIdx.x; k < N - 1; k += 64) {
+    const double sI = stats[2 * k], qI = stats[2 * k + 1];
+    const double sT = (tot[0] - sI) / N, qT = (tot[1] - qI) / N;
+    const double a = fmax(npix * qI - sI * sI, 0.0), b = fmax(npix * qT - sT * sT, 0.0);
+    den[2 * k] = den[2 * k + 1] = nc2 * sqrt(a * b);
+  }
+}
+
+// Masked 'refine' transforms every frame centred by one common value c (the
+// mean over all frames): unlike a pair, the refine template's squared term is a
+// mean of squares, so the formula is not invariant to the frames' means.  Every
+// frame gets c through the per-frame sums k1_rows reads: out[2f] = sum_k S_k / N.
+__global__ __launch_bounds__(64) void k_common_mean(const double* __restrict__ stats, int N,
+                                                    double* __restrict__ out) {
+  __shared__ double tot;
+  if (threadIdx.x == 0) {
+    double a = 0.0;
+    for (int f = 0; f < N; ++f) a += stats[2 * f];
+    tot = a / N;
+  }
+  __syncthreads();
+  for (int f = threadIdx.x; f < N; f += 64) { out[2 * f] = tot; out[2 * f + 1] = 0.0; }
+}
+
+// 'refine' windows from the pair-transform windows (in units of nc2 Re R[.]):
+// X = (W[sum, I] - W[I, I]) / N; masked: Ct and T likewise from their sum terms,
+// Cb, B and mp as for a pair, then the five-term combination.  Writes window
+// out0 + I for I < N-1 (one block each).  Layout of the masked terms after the
+// 2(N-1) X windows, base b: Ct [b, b+N), Cb [b+N, ..), T [b+2N, ..), B [b+3N, ..),
+// mp b+4N, Ct_sum b+4N+1, T_sum b+4N+2.
+// The masked spectra are of frames centred by c (cm: c * H * W as summed by
+// k_common_mean, npix = H W); the reference's uncentred terms follow exactly
+// (F = F' + cM, Q = Q' + 2cF' + c^2 M, k1 = (N-1)/N):
+//   Ct_T = Ct'_T + c mp k1          T_T = T'_T + 2c Ct'_T + c^2 mp k1
+//   Cb   = Cb'   + c mp             B   = B'   + 2c Cb'    + c^2 mp
+//   X    = X'_T  + c Cb' k1 + c Ct'_T + c^2 mp k1
+// and the combination runs in FP64 on them.
+__global__ __launch_bounds__(256) void k_refine_combine(float* __restrict__ windows, int N, int nn, int masked,
+                                                        int out0, const double* __restrict__ cm, double npix) {
+  const int I = blockIdx.x;
+  const float invN = 1.f / (float)N;
+  const float* Xs = windows + (size_t)(2 * I) * nn;
+  const float* Xi = windows + (size_t)(2 * I + 1) * nn;
+  float* out = windows + (size_t)(out0 + I) * nn;
+  const int b = 2 * (N - 1);
+  const double c = masked ? cm[0] / npix : 0.0;
+  const double k1 = (double)(N - 1) / (double)N, dN = (double)N;
+  for (int e = threadIdx.x; e < nn; e += 256) {
+    if (!masked) {
+      out[e] = (Xs[e] - Xi[e]) * invN;
+      continue;
+    }
+    const double xc = ((double)Xs[e] - (double)Xi[e]) / dN;
+    const double ctc = ((double)windows[(size_t)(b + 4 * N + 1) * nn + e] - (double)windows[(size_t)(b + I) * nn + e]) / dN;
+    const double ttc = ((double)windows[(size_t)(b + 4 * N + 2) * nn + e] -
+                        (double)windows[(size_t)(b + 2 * N + I) * nn + e]) / dN;
+    const double cbc = windows[(size_t)(b + N + I) * nn + e];
+    const double bbc = windows[(size_t)(b + 3 * N + I) * nn + e];
+    const double mp = windows[(size_t)(b + 4 * N) * nn + e];
+    const double ct = ctc + c * mp * k1;
+    const double tt = ttc + 2.0 * c * ctc + c * c * mp * k1;
+    const double cb = cbc + c * mp;
+    const double bb = bbc + 2.0 * c * cbc + c * c * mp;
+    const double x = xc + c * cbc * k1 + c * ctc + c * c * mp * k1;
+    double den = sqrt((bb - cb * (cb / mp)) * (tt - ct * (ct / mp)));
+    den = (den < 1.0) ? 1.0 : den;
+    out[e] = (float)((x - ct * cb / mp) / den);
   }
 }
 
@@ -465,7 +562,8 @@ __global__ void k2_denominators(const int4* __restrict__ pairs, int npairs, cons
 struct PeakParams {
   int m;             // maxShift
   int suppress;      // suppressOrigin
-  int bconv;         // 1: Gaussian on the window
+  int bconv;         // 1: Gaussian on the window (Bmode 'conv')
+  int bopti;         // 1: Gaussian width chosen per pair (Bmode 'opti')
   int ry, rx;        // Gaussian radii
   int subpix;        // subPixReg (<=1: integer)
   float scale_y, scale_x;   // Hp/Hc, Wp/Wc (1 when no crop)
@@ -480,12 +578,176 @@ ZR_DEV int reflect_idx(int i, int n) {
   return i < n ? i : p - 1 - i;
 }
 
+// scipy.ndimage.gaussian_filter of the n2 x n2 window (mode 'reflect'): axis 0
+// then axis 1 through reflect-padded copies in P.  The window value at index
+// cidx is replaced by sup (origin suppression).  Ends synchronised.
+ZR_DEV void gauss_window(const float* __restrict__ win, int cidx, float sup, int n2, const float* wy, int ry,
+                         const float* wx, int rx, float* P, float* Cf, int tid) {
+  const int nn = n2 * n2;
+  const int prows = n2 + 2 * ry;
+  for (int e = tid; e < prows * n2; e += 256) {
+    const int i = e / n2 - ry, j = e - (e / n2) * n2;
+    const int src = reflect_idx(i, n2) * n2 + j;
+    P[e] = src == cidx ? sup : win[src];
+  }
+  __syncthreads();
+  for (int e = tid; e < nn; e += 256) {
+    const int i = e / n2, j = e - i * n2;
+    const float* col = P + i * n2 + j;
+    float acc = 0.f;
+    for (int k = 0; k <= 2 * ry; ++k) acc += wy[k] * col[k * n2];
+    Cf[e] = acc;
+  }
+  __syncthreads();
+  const int pc = n2 + 2 * rx;
+  for (int e = tid; e < n2 * pc; e += 256) {
+    const int i = e / pc, j = e - i * pc - rx;
+    P[e] = Cf[i * n2 + reflect_idx(j, n2)];
+  }
+  __syncthreads();
+  for (int e = tid; e < nn; e += 256) {
+    const int i = e / n2, j = e - i * n2;
+    const float* row = P + i * pc + j;
+    float acc = 0.f;
+    for (int k = 0; k <= 2 * rx; ++k) acc += wx[k] * row[k];
+    Cf[e] = acc;
+  }
+  __syncthreads();
+}
+
+// Bmode 'opti' (zorro/zorro.py:1105-1124): the Gaussian width sigma in [0, 7]
+// minimising std(Cf) / (max(Cf) - mean(Cf)) over the whole window, found by
+// bounded Brent minimisation with maxiter 15 and xatol 0.01.  The steps are
+// those of Brent's fminbound (Brent 1973, ch. 5; the form scipy's
+// minimize_scalar(method='bounded') implements): golden-section steps,
+// parabolic steps when acceptable, tolerance sqrt(2.2e-16)|x| + xatol/3.
+// All threads run the scalar logic redundantly; each objective evaluation is
+// a block-wide filter + FP64 reductions.  Returns the chosen sigma with Cf
+// holding the window filtered by it.
+constexpr int OPTI_RMAX = 28;        // int(4 * 7 + 0.5)
+ZR_DEV double opti_objective(const float* __restrict__ win, int cidx, float sup, int n2, double sigma, float* wts,
+                             float* P, float* Cf, double* red, int tid) {
+  const int nn = n2 * n2;
+  __syncthreads();   // previous users of wts / Cf are done
+  if (sigma > 1e-15) {
+    const int r = (int)(4.0 * sigma + 0.5);
+    if (tid <= 2 * r) {
+      double sum = 0.0;
+      for (int k = -r; k <= r; ++k) sum += exp(-0.5 / (sigma * sigma) * (double)(k * k));
+      const int k = tid - r;
+      wts[tid] = (float)(exp(-0.5 / (sigma * sigma) * (double)(k * k)) / sum);
+    }
+    __syncthreads();
+    gauss_window(win, cidx, sup, n2, wts, r, wts, r, P, Cf, tid);
+  } else {
+    for (int e = tid; e < nn; e += 256) Cf[e] = e == cidx ? sup : win[e];
+    __syncthreads();
+  }
+  double s1 = 0.0, mx = -INFINITY;
+  for (int e = tid; e < nn; e += 256) {
+    const double v = Cf[e];
+    s1 += v;
+    mx = fmax(mx, v);
+  }
+  s1 = wave_sum_d(s1);
+  for (int o = 32; o > 0; o >>= 1) mx = fmax(mx, __shfl_xor(mx, o, 64));
+  if ((tid & 63) == 0) { red[tid >> 6] = s1; red[4 + (tid >> 6)] = mx; }
+  __syncthreads();
+  const double mean = (red[0] + red[1] + red[2] + red[3]) / nn;
+  const double vmax = fmax(fmax(red[4], red[5]), fmax(red[6], red[7]));
+  double s2 = 0.0;
+  for (int e = tid; e < nn; e += 256) {
+    const double d = (double)Cf[e] - mean;
+    s2 += d * d;
+  }
+  s2 = wave_sum_d(s2);
+  __syncthreads();
+  if ((tid & 63) == 0) red[8 + (tid >> 6)] = s2;
+  __syncthreads();
+  const double sd = sqrt((red[8] + red[9] + red[10] + red[11]) / nn);
+  return sd / (vmax - mean);
+}
+
+ZR_DEV double opti_sigma(const float* __restrict__ win, int cidx, float sup, int n2, float* wts, float* P, float* Cf,
+                         double* red, int tid) {
+  const double lo = 0.0, hi = 7.0, xatol = 0.01;
+  const int maxfun = 15;
+  const double sqrt_eps = sqrt(2.2e-16);
+  const double golden_mean = 0.5 * (3.0 - sqrt(5.0));
+  double a = lo, b = hi;
+  double fulc = a + golden_mean * (b - a);
+  double nfc = fulc, xf = fulc;
+  double rat = 0.0, e = 0.0;
+  double x = xf;
+  double fx = opti_objective(win, cidx, sup, n2, x, wts, P, Cf, red, tid);
+  int num = 1;
+  double ffulc = fx, fnfc = fx;
+  double xm = 0.5 * (a + b);
+  double tol1 = sqrt_eps * fabs(xf) + xatol / 3.0;
+  double tol2 = 2.0 * tol1;
+  while (fabs(xf - xm) > (tol2 - 0.5 * (b - a))) {
+    bool golden = true;
+    if (fabs(e) > tol1) {
+      golden = false;
+      double r = (xf - nfc) * (fx - ffulc);
+      double q = (xf - fulc) * (fx - fnfc);
+      double p = (xf - fulc) * q - (xf - nfc) * r;
+      q = 2.0 * (q - r);
+      if (q > 0.0) p = -p;
+      q = fabs(q);
+      r = e;
+      e = rat;
+      if ((fabs(p) < fabs(0.5 * q * r)) && (p > q * (a - xf)) && (p < q * (b - xf))) {
+        rat = (p + 0.0) / q;
+        x = xf + rat;
+        if (((x - a) < tol2) || ((b - x) < tol2)) {
+          const double dxm = xm - xf;
+          const double si = (dxm > 0.0) - (dxm < 0.0) + (dxm == 0.0);
+          rat = tol1 * si;
+        }
+      } else {
+        golden = true;
+      }
+    }
+    if (golden) {
+      e = (xf >= xm) ? a - xf : b - xf;
+      rat = golden_mean * e;
+    }
+    const double si = (rat > 0.0) - (rat < 0.0) + (rat == 0.0);
+    x = xf + si * fmax(fabs(rat), tol1);
+    const double fu = opti_objective(win, cidx, sup, n2, x, wts, P, Cf, red, tid);
+    ++num;
+    if (fu <= fx) {
+      if (x >= xf) a = xf; else b = xf;
+      fulc = nfc; ffulc = fnfc;
+      nfc = xf; fnfc = fx;
+      xf = x; fx = fu;
+    } else {
+      if (x < xf) a = x; else b = x;
+      if ((fu <= fnfc) || (nfc == xf)) {
+        fulc = nfc; ffulc = fnfc;
+        nfc = x; fnfc = fu;
+      } else if ((fu <= ffulc) || (fulc == xf) || (fulc == nfc)) {
+        fulc = x; ffulc = fu;
+      }
+    }
+    xm = 0.5 * (a + b);
+    tol1 = sqrt_eps * fabs(xf) + xatol / 3.0;
+    tol2 = 2.0 * tol1;
+    if (num >= maxfun) break;
+  }
+  // leave Cf filtered with the chosen width
+  opti_objective(win, cidx, sup, n2, xf, wts, P, Cf, red, tid);
+  return xf;
+}
+
 // One block (256 threads) per pair.
 __global__ __launch_bounds__(256) void k_peaks(const float* __restrict__ windows, const int4* __restrict__ pairs,
                                               PeakParams pp, const float* __restrict__ gy,
                                               const float* __restrict__ gx, const float2* __restrict__ ktab,
                                               float* __restrict__ shifts, float* __restrict__ corr,
-                                              float* __restrict__ peaksig, float* __restrict__ origin) {
+                                              float* __restrict__ peaksig, float* __restrict__ origin,
+                                              float* __restrict__ bsigma) {
   // LDS: [tmp: max(2m*2m, reflect-padded window, 34*16*subpix)] [Cf: 2m*2m]
   extern __shared__ float smem[];
   const int m = pp.m, n2 = 2 * pp.m, nn = n2 * n2;
@@ -512,39 +774,13 @@ __global__ __launch_bounds__(256) void k_peaks(const float* __restrict__ windows
   }
   __syncthreads();
   const float sup = s_sup;   // window value at zero lag after suppression
-  if (pp.bconv) {
-    // separable Gaussian, axis 0 then axis 1, through reflect-padded copies:
-    // P[(i+ry)][j] = C[refl(i)][j] for i in [-ry, 2m+ry)
-    float* P = tmp;
-    const int prows = n2 + 2 * pp.ry;
-    for (int e = tid; e < prows * n2; e += 256) {
-      const int i = e / n2 - pp.ry, j = e - (e / n2) * n2;
-      const int src = reflect_idx(i, n2) * n2 + j;
-      P[e] = src == cidx ? sup : win[src];
-    }
-    __syncthreads();
-    for (int e = tid; e < nn; e += 256) {
-      const int i = e / n2, j = e - i * n2;
-      const float* col = P + i * n2 + j;
-      float s = 0.f;
-      for (int k = 0; k <= 2 * pp.ry; ++k) s += gy[k] * col[k * n2];
-      Cf[e] = s;
-    }
-    __syncthreads();
-    // rows: Q[i][j+rx] = Cf[i][refl(j)]
-    const int pc = n2 + 2 * pp.rx;
-    for (int e = tid; e < n2 * pc; e += 256) {
-      const int i = e / pc, j = e - i * pc - pp.rx;
-      P[e] = Cf[i * n2 + reflect_idx(j, n2)];
-    }
-    __syncthreads();
-    for (int e = tid; e < nn; e += 256) {
-      const int i = e / n2, j = e - i * n2;
-      const float* row = P + i * pc + j;
-      float s = 0.f;
-      for (int k = 0; k <= 2 * pp.rx; ++k) s += gx[k] * row[k];
-      Cf[e] = s;
-    }
+  if (pp.bopti) {
+    __shared__ float wts[2 * OPTI_RMAX + 1];
+    __shared__ double ored[12];
+    const double sg = opti_sigma(win, cidx, sup, n2, wts, tmp, Cf, ored, tid);
+    if (tid == 0 && bsigma) bsigma[pr.x * pp.n + pr.y] = (float)sg;
+  } else if (pp.bconv) {
+    gauss_window(win, cidx, sup, n2, gy, pp.ry, gx, pp.rx, tmp, Cf, tid);
   } else {
     for (int e = tid; e < nn; e += 256) Cf[e] = e == cidx ? sup : win[e];
   }
@@ -679,6 +915,30 @@ __global__ __launch_bounds__(256) void k_peaks(const float* __restrict__ windows
   }
 }
 
+// triMode 'auto' (zorro/zorro.py:992-996): row I visits J = I+1, I+2, ... and
+// stops after the first pair whose peak significance is <= peaksigThres or whose
+// J - I reaches autoMax.  The pairs after the stop are never correlated in the
+// reference, so their trimat entries return to zero.  One thread per row.
+__global__ __launch_bounds__(64) void k_auto_prune(float* __restrict__ shifts, float* __restrict__ corr,
+                                                   float* __restrict__ peaksig, float* __restrict__ origin, int N,
+                                                   int auto_max, double thres) {
+  for (int I = threadIdx.x; I < N; I += 64) {
+    bool stopped = false;
+    for (int J = I + 1; J < N; ++J) {
+      const size_t e = (size_t)I * N + J;
+      if (stopped) {
+        shifts[2 * e] = 0.f;
+        shifts[2 * e + 1] = 0.f;
+        corr[e] = 0.f;
+        peaksig[e] = 0.f;
+        origin[e] = 0.f;
+      } else if ((double)peaksig[e] <= thres || J - I >= auto_max) {
+        stopped = true;
+      }
+    }
+  }
+}
+
 // ----------------------------------------------------------- launchers
 // Host-side launch helpers (instantiated for the supported sizes).
 #include "zr_launch.h"
@@ -702,11 +962,8 @@ static hipError_t launch_k1_rows_t(const float* frames, int H, int W, int Hp, in
   if (Hp % RB != 0) return hipErrorInvalidValue;
   int grid = nb * (Hp / RB);
   size_t sh = (size_t)RB * (WP / 2) * sizeof(float2);
-  static bool attr = false;
-  if (!attr) {
-    hipFuncSetAttribute((const void*)k1_rows<WP, RB, TPR>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh);
-    attr = true;
-  }
+  static std::once_flag lds_once;
+  zr_allow_full_lds((const void*)k1_rows<WP, RB, TPR>, lds_once);
   hipLaunchKernelGGL((k1_rows<WP, RB, TPR>), dim3(grid), dim3(RB * TPR), sh, st, frames, H, W, Hp, Wh, frame0, twL,
                      tw2L, T, part, mode, stats);
   return hipGetLastError();
@@ -759,8 +1016,8 @@ hipError_t zr_launch_k2_cols(int Lper, const float2* S, const float2* Sside, con
   int ngroups = (Wh + cols_per_block - 1) / cols_per_block;
   int grid = ngroups * npairs;
   switch (Lper) {
-#define ZR_CASE(X) case X: { size_t sh = ((size_t)K2C_WPB * (X / 2) * 65 + (size_t)2 * 2 * m * 8) * sizeof(float2); static bool attr = false; \
-      if (!attr) { hipFuncSetAttribute((const void*)k2_cols<X, K2C_WPB>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh); attr = true; } \
+#define ZR_CASE(X) case X: { size_t sh = ((size_t)K2C_WPB * (X / 2) * 65 + (size_t)2 * 2 * m * 8) * sizeof(float2); static std::once_flag lds_once; \
+      zr_allow_full_lds((const void*)k2_cols<X, K2C_WPB>, lds_once); \
       hipLaunchKernelGGL((k2_cols<X, K2C_WPB>), dim3(grid), dim3(64 * K2C_WPB), sh, st, S, Sside, work, npairs, Wh, Hc, Wc, m, cols_per_wave, dc_zero, twHc, Z, Wk, Hs, hoff, side_in_col); return hipGetLastError(); }
     ZR_CASE(4) ZR_CASE(8) ZR_CASE(16) ZR_CASE(32) ZR_CASE(48) ZR_CASE(64)
 #undef ZR_CASE
@@ -789,18 +1046,44 @@ hipError_t zr_launch_k2_den(const int4* pairs, int npairs, const double* stats, 
 
 hipError_t zr_launch_peaks(const float* windows, const int4* pairs, int npairs, const PeakParamsHost* hp,
                            const float* gy, const float* gx, const float2* ktab, float* shifts, float* corr,
-                           float* peaksig, float* origin, hipStream_t st) {
+                           float* peaksig, float* origin, float* bsigma, hipStream_t st) {
   PeakParams pp;
-  pp.m = hp->m; pp.suppress = hp->suppress; pp.bconv = hp->bconv; pp.ry = hp->ry; pp.rx = hp->rx;
+  pp.m = hp->m; pp.suppress = hp->suppress; pp.bconv = hp->bconv; pp.bopti = hp->bopti; pp.ry = hp->ry;
+  pp.rx = hp->rx;
   pp.subpix = hp->subpix; pp.scale_y = hp->scale_y; pp.scale_x = hp->scale_x; pp.n = hp->n;
   const size_t sh = zr_peaks_lds_bytes(hp, &pp.tmp_floats);
   if (sh > (size_t)ZR_LDS_BYTES) return hipErrorInvalidConfiguration;
-  static int attr = 0;
-  if ((int)sh > attr) {
-    hipFuncSetAttribute((const void*)k_peaks, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh);
-    attr = (int)sh;
-  }
+  static std::once_flag lds_once;
+  zr_allow_full_lds((const void*)k_peaks, lds_once);
   hipLaunchKernelGGL(k_peaks, dim3(npairs), dim3(256), sh, st, windows, pairs, pp, gy, gx, ktab, shifts, corr,
-                     peaksig, origin);
+                     peaksig, origin, bsigma);
+  return hipGetLastError();
+}
+
+hipError_t zr_launch_auto_prune(float* shifts, float* corr, float* peaksig, float* origin, int N, int auto_max,
+                                double thres, hipStream_t st) {
+  hipLaunchKernelGGL(k_auto_prune, dim3(1), dim3(64), 0, st, shifts, corr, peaksig, origin, N, auto_max, thres);
+  return hipGetLastError();
+}
+
+hipError_t zr_launch_spec_sum(float2* S, size_t slot_elems, int slot0, int count, int out_slot, hipStream_t st) {
+  hipLaunchKernelGGL(k_spec_sum, dim3((unsigned)((slot_elems + 255) / 256)), dim3(256), 0, st, S, slot_elems, slot0,
+                     count, out_slot);
+  return hipGetLastError();
+}
+
+hipError_t zr_launch_refine_den(const double* stats, int N, double npix, double nc2, double* den, hipStream_t st) {
+  hipLaunchKernelGGL(k_refine_den, dim3(1), dim3(64), 0, st, stats, N, npix, nc2, den);
+  return hipGetLastError();
+}
+
+hipError_t zr_launch_refine_combine(float* windows, int N, int m, int masked, int out0, const double* cm,
+                                    double npix, hipStream_t st) {
+  hipLaunchKernelGGL(k_refine_combine, dim3(N - 1), dim3(256), 0, st, windows, N, 4 * m * m, masked, out0, cm, npix);
+  return hipGetLastError();
+}
+
+hipError_t zr_launch_common_mean(const double* stats, int N, double* out, hipStream_t st) {
+  hipLaunchKernelGGL(k_common_mean, dim3(1), dim3(64), 0, st, stats, N, out);
   return hipGetLastError();
 }

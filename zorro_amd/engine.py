@@ -14,7 +14,8 @@ import numpy as np
 
 from . import _lib as L
 
-TRI = {"tri": L.ZR_TRI_TRI, "diag": L.ZR_TRI_DIAG, "auto": L.ZR_TRI_AUTO}
+TRI = {"tri": L.ZR_TRI_TRI, "diag": L.ZR_TRI_DIAG, "auto": L.ZR_TRI_AUTO, "first": L.ZR_TRI_FIRST,
+       "refine": L.ZR_TRI_REFINE}
 BMODE = {None: L.ZR_B_NONE, "none": L.ZR_B_NONE, "conv": L.ZR_B_CONV, "convolution": L.ZR_B_CONV,
          "opti": L.ZR_B_OPTI}
 WMODE = {"autologistic": L.ZR_W_AUTOLOGISTIC, "logistic": L.ZR_W_LOGISTIC, "corr": L.ZR_W_CORR, None: L.ZR_W_CORR,
@@ -25,7 +26,8 @@ def make_params(triMode="diag", diagWidth=5, diagStart=0, maxShift=100, Bmode="o
                 shiftMethod="lanczos", weightMode="autologistic", peaksigThres=6.0, logisticK=5.0,
                 logisticNu=0.15, originMode="centroid", suppressOrigin=True, fouCrop=(3072, 3072),
                 shapePadded=(4096, 4096), filterMode=None, doseFiltParam=(None, 0.24499, -1.6649, 2.8141, 32, 0),
-                pixelsize=None, voltage=300.0, doLazyFRC=True, frc_box=512, frc_overlap=0.5) -> L.ZrParams:
+                pixelsize=None, voltage=300.0, doLazyFRC=True, frc_box=512, frc_overlap=0.5,
+                autoMax=10) -> L.ZrParams:
     """ZrParams from ImageRegistrator attribute values (defaults of zorro/zorro.py:82-243)."""
     p = L.ZrParams()
     if triMode not in TRI:
@@ -33,6 +35,7 @@ def make_params(triMode="diag", diagWidth=5, diagStart=0, maxShift=100, Bmode="o
     p.tri_mode = TRI[triMode]
     p.diag_width = -1 if diagWidth is None else int(diagWidth)
     p.diag_start = int(diagStart)
+    p.auto_max = int(autoMax)
     if not maxShift:
         raise NotImplementedError("maxShift=None is not built")
     p.max_shift = int(maxShift)
@@ -169,14 +172,25 @@ class MoviePlan:
         L.check(self.lib.zr_get_trimats(self._mv, L.fptr(sh), L.fptr(co), L.fptr(ps), L.fptr(og)))
         return dict(shiftsTriMat=sh, corrTriMat=co, peaksigTriMat=ps, originTriMat=og)
 
+    def peak_sigmas(self):
+        """Bmode='opti': Gaussian width chosen per pair, [N, N]."""
+        sg = np.zeros((self.n, self.n), np.float32)
+        L.check(self.lib.zr_get_peak_sigmas(self._mv, L.fptr(sg)))
+        return sg
+
     def windows(self):
         w = np.zeros((self.npairs, 2 * self.m, 2 * self.m), np.float32)
         L.check(self.lib.zr_get_windows(self._mv, L.fptr(w)))
         return w
 
+    def equation_count(self):
+        m = C.c_int()
+        L.check(self.lib.zr_equation_count(self._mv, C.byref(m)))
+        return m.value
+
     def solution(self):
         n = self.n
-        M = self.npairs
+        M = max(self.equation_count(), 1)
         t = np.zeros((n, 2))
         rel = np.zeros((n - 1, 2))
         w = np.zeros(M)
@@ -186,6 +200,8 @@ class MoviePlan:
         exy = np.zeros((n, 2))
         L.check(self.lib.zr_get_solution(self._mv, L.dptr(t), L.dptr(rel), L.dptr(w), L.dptr(lg), L.dptr(err),
                                          L.dptr(es), L.dptr(exy)))
+        M = self.equation_count()
+        w, err = w[:M], err[:M]
         return dict(translations=t, relativeEst=rel, weights=w, logistic=lg, errorXY=err,
                     mean_errorNorm=es[0], std_errorNorm=es[1], errorX=exy[:, 1], errorY=exy[:, 0])
 

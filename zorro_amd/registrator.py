@@ -94,6 +94,7 @@ class ImageRegistrator:
         if self.filterMode and "hot" in self.filterMode.lower():
             raise NotImplementedError("hotpixFilter is not built")
         return make_params(triMode=self.triMode, diagWidth=self.diagWidth, diagStart=self.diagStart,
+                           autoMax=self.autoMax,
                            maxShift=self.maxShift, Bmode=self.Bmode, Brad=self.Brad, subPixReg=self.subPixReg,
                            shiftMethod=self.shiftMethod, weightMode=self.weightMode,
                            peaksigThres=self.peaksigThres, logisticK=self.logisticK, logisticNu=self.logisticNu,
@@ -129,8 +130,18 @@ class ImageRegistrator:
         self.bench["solve1"] = time.time()
         tm = plan.trimats()
         sol = plan.solution()
+        if self.triMode in ("first", "refine"):
+            # no solver (zorro/zorro.py:1626-1647): the errorDict holds the trimats
+            # and the translations taken from them
+            ed = dict(tm)
+            if self.triMode == "first":
+                ed["translations"] = -tm["shiftsTriMat"][0, :].astype(np.float64)
+            else:
+                ed["translations"] = -np.stack([tm["shiftsTriMat"][k, k] for k in range(n)]).astype(np.float64)
+            self.errorDictList.append(ed)
+            return self._finish(plan, sol)
         info = plan.solver_info()
-        pairs = [(I, J) for I in range(n) for J in range(I + 1, n) if tm["corrTriMat"][I, J] != 0]
+        pairs = [(I, J) for I in range(n - 1) for J in range(I + 1, n) if tm["corrTriMat"][I, J] != 0]
         ed = dict(tm)
         ed.update(translations=sol["translations_raw"] if "translations_raw" in sol else None)
         # errorDict keys of shiftsSolver (zorro/zorro.py:1550-1565)
@@ -146,9 +157,12 @@ class ImageRegistrator:
             thr, K, nu = sol["logistic"]
             self.peaksigThres, self.logisticK, self.logisticNu = thr, K, nu
             ed.update(peaksigThres=thr, logisticK=K, logisticNu=nu)
-        trans_c = sol["translations"]
         ed["translations"] = -np.vstack((np.zeros([1, 2]), np.cumsum(sol["relativeEst"], axis=0)))
         self.errorDictList.append(ed)
+        return self._finish(plan, sol)
+
+    def _finish(self, plan, sol):
+        """applyShifts, FRC and the dose filter (zorro/zorro.py:1672-1712)."""
         self.bench["shifts0"] = time.time()
         if self.filterMode and "dose" in self.filterMode.lower():
             self.bench["dose0"] = time.time()
@@ -161,7 +175,7 @@ class ImageRegistrator:
         plan.synchronize()
         self.bench["shifts1"] = self.bench["frc1"] = self.bench["dose1"] = time.time()
         sums = plan.sums()
-        self.translations = trans_c.copy()
+        self.translations = sol["translations"].copy()
         self.imageSum = sums["imageSum"]
         self.filtSum = sums.get("filtSum")
         if self.doLazyFRC:
