@@ -71,7 +71,11 @@ typedef struct zr_params {
   int frc_box;                            /* tiledFRC box */
   double frc_overlap;                     /* tiledFRC overlap */
   int auto_max;                           /* autoMax (triMode 'auto') */
+  int shape_binned[2];                    /* shapeBinned; {0,0} = no binning (binStack 'fourier') */
 } zr_params;
+
+/* Element types of raw frames (MRC modes: uint8 counts, uint16, float32). */
+enum { ZR_DTYPE_F32 = 0, ZR_DTYPE_U8 = 1, ZR_DTYPE_U16 = 2 };
 
 typedef struct zr_movie zr_movie;
 
@@ -93,6 +97,16 @@ int zr_movie_set_stream(zr_movie* mv, void* hip_stream);
  * caller keeps alive (no copy); on_device=0: host memory, copied.
  * Replaces the assignment of ImageRegistrator.images (zorro/zorro.py:116). */
 int zr_movie_set_frames(zr_movie* mv, const float* frames, int on_device);
+
+/* Provide raw frames [N][h][w] (h, w as given to zr_movie_create) of type
+ * dtype, optionally times a gain reference f32 [h][w] (already rotated/flipped
+ * per gainInfo) and divided by gain_scalar (0 or 1: none), then Fourier-binned
+ * to shape_binned when set.  Replaces loadData's gain product
+ * (zorro/zorro.py:3416-3419, :3455-3482) and binStack('fourier') (:1826-1904).
+ * Host inputs (on_device / gain_on_device = 0) are copied; the call returns
+ * when they may be reused. */
+int zr_movie_set_frames_raw(zr_movie* mv, const void* frames, int dtype, int on_device, const float* gain,
+                            int gain_on_device, double gain_scalar);
 
 /* All-pairs masked normalised cross-correlation, peak location, significance
  * and sub-pixel shifts into device trimats.
@@ -170,7 +184,8 @@ enum {
   ZR_STAGE_SHIFTS = 4,    /* shift + sums without the dose filter (shifts0..1) */
   ZR_STAGE_DOSE = 5,      /* shift + sums + dose filter (shifts, dose0..1) */
   ZR_STAGE_FRC = 6,       /* even/odd FRC (frc0..1) */
-  ZR_NSTAGES = 7
+  ZR_STAGE_INPUT = 7,     /* gain product and Fourier binning (bin0..1) */
+  ZR_NSTAGES = 8
 };
 int zr_set_timing(zr_movie* mv, int enable);
 /* ms[ZR_NSTAGES] accumulated milliseconds, counts[ZR_NSTAGES] calls; reset!=0 clears. */

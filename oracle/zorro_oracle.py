@@ -49,7 +49,7 @@ __all__ = [
     "logistic_fit", "cdf_logistic_curve", "l1_solve", "exact_vertex",
     "shifts_solver", "lanczos_kernel", "apply_shifts", "rotmean",
     "hann_apodization", "frc_crop_limits", "tiled_frc", "lazy_frc",
-    "dose_filter", "align_image_stack", "PINS",
+    "dose_filter", "orient_gain", "apply_gain", "bin_stack", "align_image_stack", "PINS",
 ]
 
 # Which oracle function is pinned by what (mirrored in tests/test_oracle_pins.py).
@@ -67,6 +67,8 @@ PINS = {
     "rotmean": "radially symmetric input returns its profile; constant -> constant",
     "tiled_frc": "identical halves -> FRC == 1; independent noise -> ~0",
     "dose_filter": "N=1 closed form per frequency; huge critical dose -> filtSum == |image| (scaled)",
+    "bin_stack": "band-limited image -> 4 x its even samples (2x crop); constant c -> 4c; brute-force DFT",
+    "orient_gain/apply_gain": "numpy rot90/flip identities on an index ramp; element-wise product",
 }
 
 
@@ -95,6 +97,7 @@ class Params:
     originMode: str = "centroid"         # :203
     suppressOrigin: bool = True          # :204
     fouCrop: tuple = (3072, 3072)        # :120
+    shapeBinned: Optional[tuple] = None  # :185
     shapePadded: tuple = (4096, 4096)    # :184
     filterMode: Optional[str] = None     # :135
     doseFiltParam: tuple = (None, 0.24499, -1.6649, 2.8141, 32, 0)  # :137
@@ -768,12 +771,72 @@ def dose_filter(aligned, imageSum, p: Params, shapePadded):
 
 
 # ---------------------------------------------------------------------------
+# Gain reference and Fourier binning (the C4 input path)
+# ---------------------------------------------------------------------------
+
+def orient_gain(gainRef: np.ndarray, gainInfo: dict | None):
+    """Rotations/flips of a loaded gain reference.  ``zorro/zorro.py:3486-3503``:
+    'Diagonal' -> rot90 (k=1); then 'Horizontal' and 'Vertical' together ->
+    rot90 (k=2), else 'Horizontal' -> fliplr, 'Vertical' -> flipud."""
+    g = np.asarray(gainRef)
+    info = gainInfo or {}
+    if info.get("Diagonal"):
+        g = np.rot90(g, k=1)
+    if info.get("Horizontal") and info.get("Vertical"):
+        g = np.rot90(g, k=2)
+    elif info.get("Horizontal"):
+        g = np.fliplr(g)
+    elif info.get("Vertical"):
+        g = np.flipud(g)
+    return g
+
+
+def apply_gain(raw: np.ndarray, gainRef: np.ndarray | None, gain: float | None = None):
+    """``loadData`` for a stack (``zorro/zorro.py:3416-3419``, ``:3455-3482``):
+    integer data cast to float32, times the (oriented) gain reference, divided
+    by the scalar gain when it is set and not 1.  float32 arithmetic."""
+    t = np.asarray(raw).astype(np.float32)
+    if gainRef is not None:
+        t = np.asarray(gainRef, np.float32) * t
+    if gain and not np.isclose(gain, 1.0):
+        t = t / np.float32(gain)
+    return t
+
+
+def bin_stack(images: np.ndarray, shapeBinned):
+    """``binStack(binKernel='fourier')`` (``zorro/zorro.py:1866-1901``): per frame a
+    forward transform at the full size, the four-quadrant crop to ``shapeBinned``
+    (corner sizes shapeBinned // 2), scaling by 1/(Hb Wb), the raw inverse at the
+    binned size and the real part.  Returns (binned, binScale)."""
+    n, h, w = images.shape
+    hb, wb = int(shapeBinned[0]), int(shapeBinned[1])
+    b2 = (hb // 2, wb // 2)
+    out = np.empty((n, hb, wb), dtype=np.float64)
+    for J in range(n):
+        F = np.fft.fft2(np.asarray(images[J], np.float64))
+        Fb = np.zeros((hb, wb), dtype=np.complex128)
+        Fb[:b2[0], :b2[1]] = F[:b2[0], :b2[1]]
+        Fb[:b2[0], -b2[1]:] = F[:b2[0], -b2[1]:]
+        Fb[-b2[0]:, :b2[1]] = F[-b2[0]:, :b2[1]]
+        Fb[-b2[0]:, -b2[1]:] = F[-b2[0]:, -b2[1]:]
+        Fb *= 1.0 / (hb * wb)
+        out[J] = np.real(raw_ifft2(Fb))
+    return out, np.array([h, w], float) / np.array([hb, wb], float)
+
+
+# ---------------------------------------------------------------------------
 # Orchestration
 # ---------------------------------------------------------------------------
 
 def align_image_stack(images: np.ndarray, p: Params):
-    """``alignImageStack`` for ``xcorrMode='zorro'``.  ``zorro/zorro.py:1569-1716``."""
+    """``alignImageStack`` for ``xcorrMode='zorro'``.  ``zorro/zorro.py:1569-1716``.
+    With ``shapeBinned`` set the stack is Fourier-binned first and the pixel
+    size scaled by the mean bin factor (``:1591-1592``, ``:1835-1836``)."""
     images = np.asarray(images, dtype=np.float64)
+    if p.shapeBinned is not None and np.any(p.shapeBinned):
+        images, scale = bin_stack(images, p.shapeBinned)
+        if p.pixelsize:
+            p = Params(**{**p.__dict__, "pixelsize": p.pixelsize * float(np.mean(scale))})
     n, h, w = images.shape
     padded, mask = pad_stack(images, p.shapePadded)
     xc = xcorrnm2_tri(padded, mask, p)

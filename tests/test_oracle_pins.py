@@ -449,3 +449,50 @@ def test_dose_filter_single_cosine_follows_published_exposure_curve():
     g = math.exp(-0.5 * dose / crit) * math.exp(-(q / tq) ** 32)
     expect = np.abs(10.0 + g * np.cos(2 * np.pi * 5 * xx / w)) * h * w
     assert np.allclose(fs, expect, rtol=1e-10)
+
+
+# ------------------------------------------------------------- gain + binning
+def test_bin_stack_band_limited_image_is_decimated_and_constant_scaled():
+    """binStack('fourier') (zorro/zorro.py:1866-1901) by exactly 2 on non-smooth
+    sizes: an image whose spectrum lies inside the kept box comes back as 4 x its
+    even samples; a constant c comes back as 4c (the raw inverse at the binned size)."""
+    H, W = 2 * 19, 2 * 21
+    F = np.zeros((H, W), complex)
+    k = RNG.standard_normal((9, 9)) + 1j * RNG.standard_normal((9, 9))
+    F[:5, :5] = k[:5, :5]
+    F[-4:, :5] = k[5:, :5]
+    F[:5, -4:] = k[:5, 5:]
+    F[-4:, -4:] = k[5:, 5:]
+    img = np.real(np.fft.ifft2(F))
+    out, scale = O.bin_stack(img[None], (H // 2, W // 2))
+    assert np.allclose(out[0], 4 * img[::2, ::2], atol=1e-12) and np.allclose(scale, [2, 2])
+    out, _ = O.bin_stack(np.full((1, H, W), 2.5), (H // 2, W // 2))
+    assert np.allclose(out, 10.0)
+
+
+def test_bin_stack_matches_bruteforce_dft_crop():
+    H, W, hb, wb = 14, 10, 6, 4
+    img = RNG.standard_normal((H, W))
+    out, _ = O.bin_stack(img[None], (hb, wb))
+    ky = np.r_[np.arange(hb // 2), np.arange(H - hb // 2, H)]
+    kx = np.r_[np.arange(wb // 2), np.arange(W - wb // 2, W)]
+    y, x = np.arange(H), np.arange(W)
+    Fk = np.exp(-2j * np.pi * np.outer(ky, y) / H) @ img @ np.exp(-2j * np.pi * np.outer(x, kx) / W)
+    yb, xb = np.arange(hb), np.arange(wb)
+    # the cropped grid: index i of the binned spectrum carries frequency ky[i]; inverse at the binned size
+    fy = np.r_[np.arange(hb // 2), np.arange(-(hb // 2), 0)]
+    fx = np.r_[np.arange(wb // 2), np.arange(-(wb // 2), 0)]
+    brute = np.real(np.exp(2j * np.pi * np.outer(yb, fy) / hb) @ Fk @ np.exp(2j * np.pi * np.outer(fx, xb) / wb))
+    assert np.allclose(out[0], brute / (hb * wb), atol=1e-12)
+
+
+def test_gain_orientation_and_product():
+    g = np.arange(12, dtype=float).reshape(3, 4)
+    assert np.array_equal(O.orient_gain(g, {"Diagonal": True}), np.flipud(g.T))
+    assert np.array_equal(O.orient_gain(g, {"Horizontal": True, "Vertical": True}), g[::-1, ::-1])
+    assert np.array_equal(O.orient_gain(g, {"Horizontal": True}), g[:, ::-1])
+    assert np.array_equal(O.orient_gain(g, {"Vertical": True}), g[::-1, :])
+    assert np.array_equal(O.orient_gain(g, {"Diagonal": True, "Vertical": True}), np.flipud(g.T)[::-1, :])
+    raw = np.array([[[0, 1, 2, 3]] * 3], np.uint8)
+    gr = np.full((3, 4), 1.5, np.float32)
+    assert np.array_equal(O.apply_gain(raw, gr, 2.0), raw.astype(np.float32) * 1.5 / 2)

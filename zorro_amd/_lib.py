@@ -17,6 +17,7 @@ ZR_TRI_TRI, ZR_TRI_DIAG, ZR_TRI_AUTO, ZR_TRI_FIRST, ZR_TRI_REFINE = 0, 1, 2, 3, 
 ZR_B_NONE, ZR_B_CONV, ZR_B_OPTI = 0, 1, 2
 ZR_SHIFT_INTEGER, ZR_SHIFT_LANCZOS = 0, 1
 ZR_W_AUTOLOGISTIC, ZR_W_LOGISTIC, ZR_W_CORR, ZR_W_UNWEIGHTED, ZR_W_NORM = 0, 1, 2, 3, 4
+ZR_DTYPE_F32, ZR_DTYPE_U8, ZR_DTYPE_U16 = 0, 1, 2
 
 
 class ZrParams(C.Structure):
@@ -35,6 +36,7 @@ class ZrParams(C.Structure):
         ("pixelsize_nm", C.c_double), ("voltage_kv", C.c_double),
         ("do_frc", C.c_int), ("frc_box", C.c_int), ("frc_overlap", C.c_double),
         ("auto_max", C.c_int),
+        ("shape_binned", C.c_int * 2),
     ]
 
 
@@ -57,6 +59,7 @@ SIGNATURES = {
     "zr_movie_destroy": (None, [_P]),
     "zr_movie_set_stream": (C.c_int, [_P, _P]),
     "zr_movie_set_frames": (C.c_int, [_P, _P, C.c_int]),
+    "zr_movie_set_frames_raw": (C.c_int, [_P, _P, C.c_int, C.c_int, _P, C.c_int, C.c_double]),
     "zr_xcorr": (C.c_int, [_P]),
     "zr_solve": (C.c_int, [_P]),
     "zr_set_trimats": (C.c_int, [_P, _F, _F, _F]),
@@ -82,7 +85,7 @@ SIGNATURES = {
     "zr_copy_sums_device": (C.c_int, [_P, _P, _P, _P, _P]),
 }
 
-STAGES = ["spectra", "pairs", "peaks", "solve", "shifts", "dose", "frc"]
+STAGES = ["spectra", "pairs", "peaks", "solve", "shifts", "dose", "frc", "input"]
 
 _lib = None
 

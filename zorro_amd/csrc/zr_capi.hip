@@ -161,6 +161,18 @@ struct zr_movie {
   bool have_xcorr = false;
   size_t bytes = 0;                // device memory held (workspaces, tables, owned frames)
   zr_stage_state* st = nullptr;   // solver / shift / FRC / dose state (zr_stages.h)
+  // input stage (zr_bin.hip): raw frames Hr x Wr, Fourier-binned to H x W when
+  // shape_binned is set; gain product fused with the conversion
+  int Hr = 0, Wr = 0, nbb = 1;
+  bool binned = false;
+  int Mrow = 0, Mcol = 0, Mout = 0;
+  float2 *twMrow = nullptr, *twMcol = nullptr, *twMout = nullptr;
+  float2 *chW_c = nullptr, *chW_B = nullptr, *chH_c = nullptr, *chH_B = nullptr;
+  float2 *chHb_c = nullptr, *chHb_B = nullptr, *chWb_c = nullptr, *chWb_B = nullptr;
+  float2 *binA = nullptr, *binB = nullptr;   // [nbb][Hr][W] / [nbb][W][Hr] (then Gc/Gr [nbb][W][H], [nbb][H][W])
+  void* raw_owned = nullptr;
+  size_t raw_owned_bytes = 0;
+  float* gain_owned = nullptr;
   StageTimer timer;
 };
 
@@ -177,7 +189,9 @@ static void free_movie(zr_movie* mv) {
   hipSetDevice(mv->device);
   void* ptrs[] = {mv->frames_owned, mv->twWp2, mv->twWp, mv->twHp, mv->twHc, mv->twWc, mv->ktab, mv->gy, mv->gx,
                   mv->pairs, mv->work, mv->T, mv->S, mv->Sside, mv->Z, mv->part, mv->stats, mv->den, mv->stats_c,
-                  mv->windows, mv->shifts, mv->corr, mv->peaksig, mv->origin, mv->bsigma};
+                  mv->windows, mv->shifts, mv->corr, mv->peaksig, mv->origin, mv->bsigma, mv->twMrow, mv->twMcol,
+                  mv->twMout, mv->chW_c, mv->chW_B, mv->chH_c, mv->chH_B, mv->chHb_c, mv->chHb_B, mv->chWb_c,
+                  mv->chWb_B, mv->binA, mv->binB, mv->raw_owned, mv->gain_owned};
   for (void* q : ptrs)
     if (q) hipFree(q);
   zr_stage_free(mv->st);
@@ -301,10 +315,27 @@ extern "C" int zr_movie_create(int device, const zr_params* p, int n_frames, int
   mv->device = device;
   mv->p = *p;
   mv->N = n_frames;
-  mv->H = h;
-  mv->W = w;
+  mv->Hr = h;
+  mv->Wr = w;
   hipError_t e = hipSetDevice(device);
   if (e != hipSuccess) { free_movie(mv); return fail(ZR_ERR_HIP, "hipSetDevice(%d): %s", device, hipGetErrorString(e)); }
+  // binStack (zorro/zorro.py:1591-1592, :1826-1904): everything after it sees
+  // the binned frames, and the pixel size grows by the mean bin factor
+  mv->binned = p->shape_binned[0] > 0 || p->shape_binned[1] > 0;
+  if (mv->binned) {
+    const int hb = p->shape_binned[0], wb = p->shape_binned[1];
+    if (hb <= 0 || wb <= 0 || hb > h || wb > w || (hb & 1) || (wb & 1)) {
+      free_movie(mv);
+      return fail(ZR_ERR_ARG, "shapeBinned (%d,%d) must be even and within the frames (%d,%d)", hb, wb, h, w);
+    }
+    if (h > 8192 || w > 8192) { free_movie(mv); return fail(ZR_ERR_UNSUPPORTED, "binning built for frames up to 8192 (got %dx%d)", h, w); }
+    h = hb;
+    w = wb;
+    mv->p.pixelsize_nm = p->pixelsize_nm * 0.5 * ((double)mv->Hr / hb + (double)mv->Wr / wb);
+  }
+  mv->H = h;
+  mv->W = w;
+  if (h < 16 || w < 16) { free_movie(mv); return fail(ZR_ERR_ARG, "binned frames below 16x16"); }
   mv->Hp = p->shape_padded[0] > 0 ? p->shape_padded[0] : h;
   mv->Wp = p->shape_padded[1] > 0 ? p->shape_padded[1] : w;
   if (mv->Hp < h || mv->Wp < w) { free_movie(mv); return fail(ZR_ERR_ARG, "shapePadded (%d,%d) smaller than frames (%d,%d)", mv->Hp, mv->Wp, h, w); }
@@ -415,6 +446,33 @@ extern "C" int zr_movie_create(int device, const zr_params* p, int n_frames, int
     e = hipMemset(mv->den, 0, sizeof(double) * mv->Ptot);
     if (e != hipSuccess) { free_movie(mv); return fail(ZR_ERR_HIP, "hipMemset: %s", hipGetErrorString(e)); }
   }
+  if (mv->binned) {
+    // Bluestein tables: rows at the raw width, columns at the raw height with the
+    // inverse at the binned height (same convolution size), out rows at the binned width
+    mv->Mrow = zr_bin_conv_size(mv->Wr);
+    mv->Mcol = zr_bin_conv_size(mv->Hr);
+    mv->Mout = zr_bin_conv_size(mv->W);
+    std::vector<float2> c, B;
+    auto tables = [&](int L, int M, float2** dc, float2** dB) {
+      zr_bin_chirp_tables(L, M, c, B);
+      int r = dev_upload(dc, c);
+      return r ? r : dev_upload(dB, B);
+    };
+    if ((rc = tables(mv->Wr, mv->Mrow, &mv->chW_c, &mv->chW_B)) || (rc = tables(mv->Hr, mv->Mcol, &mv->chH_c, &mv->chH_B)) ||
+        (rc = tables(mv->H, mv->Mcol, &mv->chHb_c, &mv->chHb_B)) || (rc = tables(mv->W, mv->Mout, &mv->chWb_c, &mv->chWb_B)) ||
+        (rc = dev_upload(&mv->twMrow, fwd_table(mv->Mrow))) || (rc = dev_upload(&mv->twMcol, fwd_table(mv->Mcol))) ||
+        (rc = dev_upload(&mv->twMout, fwd_table(mv->Mout)))) {
+      free_movie(mv);
+      return rc;
+    }
+    mv->nbb = std::min(mv->N, 4);
+    const size_t per = (size_t)mv->Hr * mv->W;   // >= W * H of the column pass
+    if ((rc = dev_alloc(&mv->binA, per * mv->nbb)) || (rc = dev_alloc(&mv->binB, per * mv->nbb)) ||
+        (rc = dev_alloc(&mv->frames_owned, (size_t)mv->N * mv->H * mv->W))) {
+      free_movie(mv);
+      return rc;
+    }
+  }
   if ((rc = zr_stage_create(&mv->st, mv->p, mv->N, mv->H, mv->W, mv->Hp, mv->Wp, mv->pairs_host, mv->spectral ? 1 : 0))) {
     free_movie(mv);
     return fail(rc, "%s", zr_stage_error());
@@ -443,8 +501,80 @@ extern "C" int zr_movie_set_stream(zr_movie* mv, void* s) {
   return ZR_OK;
 }
 
+// Input stage: raw frames (device) -> frames_owned: Fourier binning with the
+// gain product fused into its row pass, or the gain product alone.
+static int run_input_stage(zr_movie* mv, const void* raw, int dtype, const float* gain, double gain_scalar) {
+  hipStream_t s = mv->stream;
+  const int use_scale = (gain_scalar != 0.0 && std::fabs(gain_scalar - 1.0) > 1e-8) ? 1 : 0;
+  const float gs = use_scale ? (float)gain_scalar : 1.f;
+  hipEvent_t t0 = mv->timer.begin(s);
+  if (!mv->binned) {
+    const size_t npix = (size_t)mv->H * mv->W;
+    ZR_HIP(zr_launch_gain_convert(raw, dtype, npix, npix * mv->N, gain, gs, use_scale, mv->frames_owned, s));
+  } else {
+    const int Hr = mv->Hr, Wr = mv->Wr, H = mv->H, W = mv->W;
+    const BinChirp cw{mv->chW_c, mv->chW_B, Wr}, ch{mv->chH_c, mv->chH_B, Hr}, chb{mv->chHb_c, mv->chHb_B, H},
+        cwb{mv->chWb_c, mv->chWb_B, W};
+    const float scale = (float)(1.0 / ((double)H * (double)W));
+    for (int f0 = 0; f0 < mv->N; f0 += mv->nbb) {
+      const int nb = std::min(mv->nbb, mv->N - f0);
+      ZR_HIP(zr_launch_bin_rows(mv->Mrow, raw, dtype, f0, nb, Hr, Wr, gain, gs, use_scale, cw, mv->twMrow, W, mv->binA, s));
+      ZR_HIP(zr_launch_bin_transpose(mv->binA, nb, Hr, W, mv->binB, s));
+      ZR_HIP(zr_launch_bin_cols(mv->Mcol, mv->binB, nb, Hr, W, H, ch, chb, mv->twMcol, scale, mv->binA, s));
+      ZR_HIP(zr_launch_bin_transpose(mv->binA, nb, W, H, mv->binB, s));
+      ZR_HIP(zr_launch_bin_out(mv->Mout, mv->binB, nb, H, W, cwb, mv->twMout, f0, mv->frames_owned, s));
+    }
+  }
+  mv->timer.end(ZR_STAGE_INPUT, t0, s);
+  mv->frames = mv->frames_owned;
+  mv->have_xcorr = false;
+  return ZR_OK;
+}
+
+extern "C" int zr_movie_set_frames_raw(zr_movie* mv, const void* frames, int dtype, int on_device, const float* gain,
+                                       int gain_on_device, double gain_scalar) {
+  if (!mv || !frames) return fail(ZR_ERR_ARG, "NULL argument");
+  if (dtype != ZR_DTYPE_F32 && dtype != ZR_DTYPE_U8 && dtype != ZR_DTYPE_U16) return fail(ZR_ERR_ARG, "dtype %d", dtype);
+  ZR_HIP(hipSetDevice(mv->device));
+  const size_t esz = dtype == ZR_DTYPE_F32 ? 4 : (dtype == ZR_DTYPE_U8 ? 1 : 2);
+  const size_t raw_bytes = (size_t)mv->N * mv->Hr * mv->Wr * esz;
+  const void* rawd = frames;
+  if (!on_device) {
+    if (mv->raw_owned_bytes < raw_bytes) {
+      if (mv->raw_owned) { hipFree(mv->raw_owned); mv->bytes -= mv->raw_owned_bytes; }
+      mv->raw_owned = nullptr;
+      mv->raw_owned_bytes = 0;
+      ZR_HIP(hipMalloc(&mv->raw_owned, raw_bytes));
+      mv->raw_owned_bytes = raw_bytes;
+      mv->bytes += raw_bytes;
+    }
+    ZR_HIP(hipMemcpyAsync(mv->raw_owned, frames, raw_bytes, hipMemcpyHostToDevice, mv->stream));
+    rawd = mv->raw_owned;
+  }
+  const float* gd = gain;
+  if (gain && !gain_on_device) {
+    const size_t gb = (size_t)mv->Hr * mv->Wr * sizeof(float);
+    if (!mv->gain_owned) {
+      ZR_HIP(hipMalloc((void**)&mv->gain_owned, gb));
+      mv->bytes += gb;
+    }
+    ZR_HIP(hipMemcpyAsync(mv->gain_owned, gain, gb, hipMemcpyHostToDevice, mv->stream));
+    gd = mv->gain_owned;
+  }
+  if (!mv->frames_owned) {
+    const size_t fb = (size_t)mv->N * mv->H * mv->W * sizeof(float);
+    ZR_HIP(hipMalloc((void**)&mv->frames_owned, fb));
+    mv->bytes += fb;
+  }
+  int rc = run_input_stage(mv, rawd, dtype, gd, gain_scalar);
+  if (rc) return rc;
+  if (!on_device || (gain && !gain_on_device)) ZR_HIP(hipStreamSynchronize(mv->stream));   // host buffers may go away
+  return ZR_OK;
+}
+
 extern "C" int zr_movie_set_frames(zr_movie* mv, const float* frames, int on_device) {
   if (!mv || !frames) return fail(ZR_ERR_ARG, "NULL argument");
+  if (mv->binned) return zr_movie_set_frames_raw(mv, frames, ZR_DTYPE_F32, on_device, nullptr, 0, 1.0);
   ZR_HIP(hipSetDevice(mv->device));
   if (on_device) {
     mv->frames = frames;

@@ -60,3 +60,24 @@ hipError_t zr_launch_refine_den(const double* stats, int N, double npix, double 
 hipError_t zr_launch_refine_combine(float* windows, int N, int m, int masked, int out0, const double* cm,
                                     double npix, hipStream_t st);
 hipError_t zr_launch_common_mean(const double* stats, int N, double* out, hipStream_t st);
+
+// zr_bin.hip: gain product and Fourier binning through Bluestein transforms.
+// A chirp table set for length L: c[L] = e^{-i pi n^2/L}, B[M] = FFT_M(conj chirp)/M.
+struct BinChirp {
+  const float2* c;
+  const float2* B;
+  int L;
+};
+#include <vector>
+int zr_bin_conv_size(int L);   // smallest power of two >= 2L-1
+void zr_bin_chirp_tables(int L, int M, std::vector<float2>& c, std::vector<float2>& B);
+hipError_t zr_launch_bin_rows(int M, const void* raw, int dtype, int frame0, int nb, int H, int W, const float* gain,
+                              float gscale, int use_scale, const BinChirp& cw, const float2* twM, int Wb, float2* Tr,
+                              hipStream_t st);
+hipError_t zr_launch_bin_cols(int M, const float2* Tc, int nb, int H, int Wb, int Hb, const BinChirp& ch,
+                              const BinChirp& chb, const float2* twM, float scale, float2* Gc, hipStream_t st);
+hipError_t zr_launch_bin_out(int M, const float2* Gr, int nb, int Hb, int Wb, const BinChirp& cwb, const float2* twM,
+                             int frame0, float* frames, hipStream_t st);
+hipError_t zr_launch_bin_transpose(const float2* in, int nb, int R, int C, float2* out, hipStream_t st);
+hipError_t zr_launch_gain_convert(const void* raw, int dtype, size_t npix_frame, size_t total, const float* gain,
+                                  float gscale, int use_scale, float* frames, hipStream_t st);

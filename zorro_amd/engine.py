@@ -27,7 +27,7 @@ def make_params(triMode="diag", diagWidth=5, diagStart=0, maxShift=100, Bmode="o
                 logisticNu=0.15, originMode="centroid", suppressOrigin=True, fouCrop=(3072, 3072),
                 shapePadded=(4096, 4096), filterMode=None, doseFiltParam=(None, 0.24499, -1.6649, 2.8141, 32, 0),
                 pixelsize=None, voltage=300.0, doLazyFRC=True, frc_box=512, frc_overlap=0.5,
-                autoMax=10) -> L.ZrParams:
+                autoMax=10, shapeBinned=None) -> L.ZrParams:
     """ZrParams from ImageRegistrator attribute values (defaults of zorro/zorro.py:82-243)."""
     p = L.ZrParams()
     if triMode not in TRI:
@@ -66,11 +66,15 @@ def make_params(triMode="diag", diagWidth=5, diagStart=0, maxShift=100, Bmode="o
     p.do_frc = 1 if doLazyFRC else 0
     p.frc_box = int(frc_box)
     p.frc_overlap = float(frc_overlap)
+    if shapeBinned is not None and np.any(shapeBinned):
+        p.shape_binned[0], p.shape_binned[1] = (int(shapeBinned[0]), int(shapeBinned[1]))
     return p
 
 
 class MoviePlan:
-    """Device plan for one movie of ``n`` frames of ``h`` x ``w``."""
+    """Device plan for one movie of ``n`` frames of ``h`` x ``w`` (the raw size:
+    with ``shapeBinned`` the frames are binned on input and every result has the
+    binned size ``hb`` x ``wb``)."""
 
     def __init__(self, params: L.ZrParams, n: int, h: int, w: int, device: int = 0):
         self.lib = L.load()
@@ -83,6 +87,8 @@ class MoviePlan:
         self.npairs = npairs.value
         self.m = params.max_shift
         self._frames_ref = None
+        inf = self.info()
+        self.hb, self.wb = inf["H"], inf["W"]
 
     def close(self):
         if self._mv:
@@ -113,6 +119,36 @@ class MoviePlan:
                 raise ValueError(f"frames shape {a.shape} != plan {(self.n, self.h, self.w)}")
             self._frames_ref = a
             L.check(self.lib.zr_movie_set_frames(self._mv, C.c_void_p(a.ctypes.data), 0))
+
+    def set_frames_raw(self, frames, gain=None, gain_scalar=None):
+        """Raw frames (uint8, uint16 or float32; numpy or a CUDA tensor) and an
+        optional gain reference f32 [h, w] already oriented as loadData does
+        (``zorro/zorro.py:3486-3503``); gain product and binning run on the GPU."""
+        dt = {"uint8": L.ZR_DTYPE_U8, "uint16": L.ZR_DTYPE_U16, "float32": L.ZR_DTYPE_F32}
+        def ptr(a, kind):
+            if a is None:
+                return C.c_void_p(0), 0, None
+            if hasattr(a, "data_ptr") and getattr(a, "is_cuda", False):
+                if not a.is_contiguous():
+                    raise ValueError(f"{kind} must be contiguous")
+                return C.c_void_p(a.data_ptr()), 1, a
+            a = np.ascontiguousarray(a)
+            return C.c_void_p(a.ctypes.data), 0, a
+        name = str(frames.dtype).replace("torch.", "")
+        if name not in dt:
+            raise ValueError(f"raw frames must be uint8, uint16 or float32, not {frames.dtype}")
+        if tuple(frames.shape) != (self.n, self.h, self.w):
+            raise ValueError(f"frames shape {tuple(frames.shape)} != plan {(self.n, self.h, self.w)}")
+        if gain is not None:
+            if not hasattr(gain, "data_ptr"):
+                gain = np.asarray(gain, np.float32)
+            if tuple(gain.shape) != (self.h, self.w):
+                raise ValueError(f"gain shape {tuple(gain.shape)} != frame shape {(self.h, self.w)}")
+        fp, fdev, fref = ptr(frames, "frames")
+        gp, gdev, gref = ptr(gain, "gain")
+        self._frames_ref = (fref, gref)
+        L.check(self.lib.zr_movie_set_frames_raw(self._mv, fp, dt[name], fdev, gp, gdev,
+                                                 float(gain_scalar) if gain_scalar else 0.0))
 
     # ---- stages
     def xcorr(self):
@@ -212,7 +248,7 @@ class MoviePlan:
 
     def sums(self):
         """imageSum, evenSum, oddSum and (when the dose filter ran) filtSum."""
-        h, w = self.h, self.w
+        h, w = self.hb, self.wb
         out = [np.zeros((h, w), np.float32) for _ in range(4)]
         ptrs = [L.fptr(a) for a in out]
         if not self.params.do_dose:
