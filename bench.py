@@ -115,6 +115,8 @@ def main():
     ap.add_argument("--inflight", type=int, default=1,
                     help="C2 movies in flight at once (one plan and stream each)")
     ap.add_argument("--seconds", type=float, default=6.0, help="served: arrival window per rate")
+    ap.add_argument("--graph", type=int, default=1,
+                    help="replay each movie's stage sequence as one captured HIP graph (1) or launch eagerly (0)")
     args = ap.parse_args()
     if args.pmc_summary:
         pmc_summary(*args.pmc_summary)
@@ -157,17 +159,21 @@ def bench_c2(args):
     plans = [MoviePlan(make_params(**PARAMS), c["n"], c["h"], c["w"]) for _ in range(K)]
     for p in plans:
         p.set_frames(frames)
+        if args.graph:
+            p.use_graph(True)
     plan = plans[0]
     info = plan.info()
+    # stage times from HIP events on plan 0's stream (inside its graph when graphs are on)
+    plan.set_timing(True)
 
-    print(f"[bench] plan ready {info}; {K} in flight; warmup {args.warmup}", file=sys.stderr, flush=True)
-    for i in range(args.warmup):
+    print(f"[bench] plan ready {info}; {K} in flight; graph {args.graph}; warmup {args.warmup}", file=sys.stderr,
+          flush=True)
+    for i in range(max(args.warmup, K if args.graph else 0)):
         plans[i % K].align()
     for p in plans:
         p.synchronize()
     torch.cuda.synchronize()
 
-    plan.set_timing(True)
     plan.stage_times(reset=True)
     lat = []
     start = [0.0] * K
@@ -216,7 +222,7 @@ def bench_c2(args):
                                     "L1 shifts solver, Lanczos shift + sum, even/odd FRC (512 tiles), "
                                     "dose filter (1 A/px, 300 kV)", frames=c["n"], shape=[c["h"], c["w"]],
                            pairs=info["pairs"], fouCrop=[info["Hc"], info["Wc"]], inputs="resident in HBM"),
-               inflight=K,
+               inflight=K, hip_graph=bool(args.graph),
                frames_per_s=round(c["n"] * 1000.0 / ms_step, 1),
                latency_ms=dict(p50=round(float(np.percentile(lat, 50)), 3),
                                p99=round(float(np.percentile(lat, 99)), 3)),

@@ -144,9 +144,21 @@ int zr_align(zr_movie* mv);
 
 int zr_synchronize(zr_movie* mv);
 
+/* enable != 0: zr_align captures its whole kernel sequence on the movie's
+ * stream into a HIP graph on its next call and replays that graph from then on
+ * (one launch per movie).  The graph holds the frames pointer: a different
+ * device pointer in zr_movie_set_frames, a new stream or a change of timing
+ * re-captures it.  Not available for triMode 'auto', whose solver reads the
+ * pair list back to the host.  No reference counterpart (launch batching). */
+int zr_movie_use_graph(zr_movie* mv, int enable);
+
 /* Result read-back (host buffers, synchronising).
  * shifts f32 [N][N][2], corr/peaksig/origin f32 [N][N] (errorDict trimats). */
 int zr_get_trimats(zr_movie* mv, float* shifts, float* corr, float* peaksig, float* origin);
+/* The frames the registration works on, f32 [N][H][W] (after the gain product
+ * and binning of zr_movie_set_frames_raw: self.images after binStack,
+ * zorro/zorro.py:1903). */
+int zr_get_frames(zr_movie* mv, float* frames);
 /* Bmode='opti': the Gaussian width chosen for each pair, f32 [N][N] (the
  * reference only prints it, zorro/zorro.py:1123-1124). */
 int zr_get_peak_sigmas(zr_movie* mv, float* sigma);

@@ -68,9 +68,11 @@ SIGNATURES = {
     "zr_dose_filter": (C.c_int, [_P]),
     "zr_align": (C.c_int, [_P]),
     "zr_synchronize": (C.c_int, [_P]),
+    "zr_movie_use_graph": (C.c_int, [_P, C.c_int]),
     "zr_get_trimats": (C.c_int, [_P, _F, _F, _F, _F]),
     "zr_get_windows": (C.c_int, [_P, _F]),
     "zr_get_peak_sigmas": (C.c_int, [_P, _F]),
+    "zr_get_frames": (C.c_int, [_P, _F]),
     "zr_pair_count": (C.c_int, [_P, _I]),
     "zr_equation_count": (C.c_int, [_P, _I]),
     "zr_get_solution": (C.c_int, [_P, _D, _D, _D, _D, _D, _D, _D]),

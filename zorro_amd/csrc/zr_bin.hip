@@ -206,7 +206,7 @@ std::complex<double> chirp_phase(long n, long L, int sign) {
 }  // namespace
 
 int zr_bin_conv_size(int L) {
-  int M = 1;
+  int M = 4096;             // smallest built transform; any M >= 2L-1 gives the same DFT
   while (M < 2 * L - 1) M <<= 1;
   return M;
 }

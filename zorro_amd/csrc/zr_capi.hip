@@ -87,9 +87,11 @@ struct StageTimer {
   struct Pending { int stage; hipEvent_t a, b; };
   std::vector<Pending> pending;
   std::vector<hipEvent_t> pool;
+  std::vector<Pending> graph_pending;   // event pairs recorded inside a captured graph
   double ms[ZR_NSTAGES] = {0};
   long count[ZR_NSTAGES] = {0};
   bool on = false;
+  bool capturing = false;
   hipEvent_t get() {
     if (!pool.empty()) { hipEvent_t e = pool.back(); pool.pop_back(); return e; }
     hipEvent_t e = nullptr;
@@ -106,7 +108,21 @@ struct StageTimer {
     if (!on || !a) return;
     hipEvent_t b = get();
     hipEventRecord(b, s);
-    pending.push_back({stage, a, b});
+    (capturing ? graph_pending : pending).push_back({stage, a, b});
+  }
+  // after a replay of the captured graph has finished: its event nodes hold
+  // this replay's stage times
+  void harvest_graph() {
+    for (auto& p : graph_pending) {
+      float t = 0.f;
+      hipEventElapsedTime(&t, p.a, p.b);
+      ms[p.stage] += t;
+      count[p.stage] += 1;
+    }
+  }
+  void drop_graph() {
+    for (auto& p : graph_pending) { pool.push_back(p.a); pool.push_back(p.b); }
+    graph_pending.clear();
   }
   void harvest() {
     for (auto& p : pending) {
@@ -122,6 +138,7 @@ struct StageTimer {
   }
   ~StageTimer() {
     for (auto& p : pending) { hipEventDestroy(p.a); hipEventDestroy(p.b); }
+    for (auto& p : graph_pending) { hipEventDestroy(p.a); hipEventDestroy(p.b); }
     for (auto e : pool) hipEventDestroy(e);
   }
 };
@@ -145,6 +162,7 @@ struct zr_movie {
   float* frames_owned = nullptr;
   // tables
   float2 *twWp2 = nullptr, *twWp = nullptr, *twHp = nullptr, *twHc = nullptr, *twWc = nullptr, *ktab = nullptr;
+  float2* k2btab = nullptr;        // [Wh][2m] twiddles of the matrix-core row pass (NULL: vector pass)
   float *gy = nullptr, *gx = nullptr;
   int ry = 0, rx = 0;
   // pairs: P frame pairs; on the masked path the pair transform also runs the
@@ -173,6 +191,10 @@ struct zr_movie {
   void* raw_owned = nullptr;
   size_t raw_owned_bytes = 0;
   float* gain_owned = nullptr;
+  // zr_align as one HIP graph (zr_movie_use_graph)
+  bool use_graph = false;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t graph_exec = nullptr;
   StageTimer timer;
 };
 
@@ -184,10 +206,19 @@ extern "C" int zr_device_count(int* n) {
   return ZR_OK;
 }
 
+static void drop_graph(zr_movie* mv) {
+  if (mv->graph_exec) hipGraphExecDestroy(mv->graph_exec);
+  if (mv->graph) hipGraphDestroy(mv->graph);
+  mv->graph_exec = nullptr;
+  mv->graph = nullptr;
+  mv->timer.drop_graph();
+}
+
 static void free_movie(zr_movie* mv) {
   if (!mv) return;
   hipSetDevice(mv->device);
-  void* ptrs[] = {mv->frames_owned, mv->twWp2, mv->twWp, mv->twHp, mv->twHc, mv->twWc, mv->ktab, mv->gy, mv->gx,
+  drop_graph(mv);
+  void* ptrs[] = {mv->frames_owned, mv->k2btab, mv->twWp2, mv->twWp, mv->twHp, mv->twHc, mv->twWc, mv->ktab, mv->gy, mv->gx,
                   mv->pairs, mv->work, mv->T, mv->S, mv->Sside, mv->Z, mv->part, mv->stats, mv->den, mv->stats_c,
                   mv->windows, mv->shifts, mv->corr, mv->peaksig, mv->origin, mv->bsigma, mv->twMrow, mv->twMcol,
                   mv->twMout, mv->chW_c, mv->chW_B, mv->chH_c, mv->chH_B, mv->chHb_c, mv->chHb_B, mv->chWb_c,
@@ -440,6 +471,12 @@ extern "C" int zr_movie_create(int device, const zr_params* p, int n_frames, int
     free_movie(mv);
     return rc;
   }
+  if ((2 * mv->m) % 16 == 0) {
+    if ((rc = dev_alloc(&mv->k2btab, (size_t)mv->Wh * 2 * mv->m))) { free_movie(mv); return rc; }
+    e = zr_launch_k2_twiddles(mv->Wh, mv->Wc, mv->m, mv->k2btab, mv->own_stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(mv->own_stream);
+    if (e != hipSuccess) { free_movie(mv); return fail(ZR_ERR_HIP, "k2 twiddles: %s", hipGetErrorString(e)); }
+  }
   if (!mv->unmasked) {
     // the pair transform's own scale is nc2 / max(1, den): den = 0 keeps the
     // raw nc2 * Re R[.] windows that the masked combination needs
@@ -497,6 +534,7 @@ extern "C" int zr_copy_sums_device(zr_movie* mv, float* image_sum, float* filt_s
 
 extern "C" int zr_movie_set_stream(zr_movie* mv, void* s) {
   if (!mv) return fail(ZR_ERR_ARG, "NULL movie");
+  drop_graph(mv);
   mv->stream = s ? (hipStream_t)s : mv->own_stream;
   return ZR_OK;
 }
@@ -577,6 +615,7 @@ extern "C" int zr_movie_set_frames(zr_movie* mv, const float* frames, int on_dev
   if (mv->binned) return zr_movie_set_frames_raw(mv, frames, ZR_DTYPE_F32, on_device, nullptr, 0, 1.0);
   ZR_HIP(hipSetDevice(mv->device));
   if (on_device) {
+    if (frames != mv->frames) drop_graph(mv);   // the graph holds the frames pointer
     mv->frames = frames;
   } else {
     size_t bytes = (size_t)mv->N * mv->H * mv->W * sizeof(float);
@@ -654,8 +693,8 @@ extern "C" int zr_xcorr(zr_movie* mv) {
   hipEvent_t t1 = mv->timer.begin(s);
   ZR_HIP(zr_launch_k2_cols(mv->Lper, mv->S, mv->Sside, mv->work, mv->Ptot, mv->Wh, mv->Hc, mv->Wc, mv->m, 8,
                            mv->unmasked ? 1 : 0, mv->twHc, mv->Z, mv->Wk, mv->Hs, mv->hoff, mv->side_in_col, s));
-  ZR_HIP(zr_launch_k2_rows(mv->Z, mv->pairs, mv->Ptot, mv->Wh, mv->Wc, mv->m, mv->twWc, mv->den, (float)nc2,
-                           mv->windows, s));
+  ZR_HIP(zr_launch_k2_rows(mv->Z, mv->pairs, mv->Ptot, mv->Wh, mv->Wc, mv->m, mv->twWc, mv->k2btab, mv->den,
+                           (float)nc2, mv->windows, s));
   if (refine)
     ZR_HIP(zr_launch_refine_combine(mv->windows, mv->N, mv->m, mv->unmasked ? 0 : 1, mv->win_out0, mv->stats_c,
                                     (double)mv->H * (double)mv->W, s));
@@ -663,10 +702,11 @@ extern "C" int zr_xcorr(zr_movie* mv) {
   mv->timer.end(ZR_STAGE_PAIRS, t1, s);
   // peaks
   hipEvent_t t2 = mv->timer.begin(s);
-  ZR_HIP(hipMemsetAsync(mv->shifts, 0, sizeof(float) * 2 * mv->N * mv->N, s));
-  ZR_HIP(hipMemsetAsync(mv->corr, 0, sizeof(float) * mv->N * mv->N, s));
-  ZR_HIP(hipMemsetAsync(mv->peaksig, 0, sizeof(float) * mv->N * mv->N, s));
-  ZR_HIP(hipMemsetAsync(mv->origin, 0, sizeof(float) * mv->N * mv->N, s));
+  // cleared by a kernel (not hipMemsetAsync) so that the sequence can be captured in a graph
+  ZR_HIP(zr_launch_fill_zero(mv->shifts, sizeof(float) * 2 * mv->N * mv->N, s));
+  ZR_HIP(zr_launch_fill_zero(mv->corr, sizeof(float) * mv->N * mv->N, s));
+  ZR_HIP(zr_launch_fill_zero(mv->peaksig, sizeof(float) * mv->N * mv->N, s));
+  ZR_HIP(zr_launch_fill_zero(mv->origin, sizeof(float) * mv->N * mv->N, s));
   const PeakParamsHost hp = peak_params(mv);
   ZR_HIP(zr_launch_peaks(mv->windows, mv->pairs, mv->P, &hp, mv->gy, mv->gx, mv->ktab, mv->shifts, mv->corr,
                          mv->peaksig, mv->origin, mv->bsigma, s));
@@ -743,7 +783,7 @@ extern "C" int zr_dose_filter(zr_movie* mv) {
   return rc ? fail(rc, "%s", zr_stage_error()) : ZR_OK;
 }
 
-extern "C" int zr_align(zr_movie* mv) {
+static int align_sequence(zr_movie* mv) {
   int rc;
   if ((rc = zr_xcorr(mv))) return rc;
   if ((rc = zr_solve(mv))) return rc;
@@ -754,6 +794,48 @@ extern "C" int zr_align(zr_movie* mv) {
     return rc;
   }
   if (mv->p.do_frc && (rc = zr_frc(mv))) return rc;
+  return ZR_OK;
+}
+
+extern "C" int zr_movie_use_graph(zr_movie* mv, int enable) {
+  if (!mv) return fail(ZR_ERR_ARG, "NULL movie");
+  if (enable && mv->p.tri_mode == ZR_TRI_AUTO)
+    return fail(ZR_ERR_UNSUPPORTED, "triMode 'auto' reads the pair list back to the host between stages: no graph");
+  if (!enable) drop_graph(mv);
+  mv->use_graph = enable != 0;
+  return ZR_OK;
+}
+
+extern "C" int zr_align(zr_movie* mv) {
+  if (!mv) return fail(ZR_ERR_ARG, "NULL movie");
+  if (!mv->use_graph) return align_sequence(mv);
+  if (!mv->frames) return fail(ZR_ERR_STATE, "frames not set");
+  ZR_HIP(hipSetDevice(mv->device));
+  if (!mv->graph_exec) {
+    // capture the whole stage sequence on the movie's stream once; replays
+    // re-run it on whatever the frames buffer holds
+    mv->timer.drop_graph();
+    ZR_HIP(hipStreamBeginCapture(mv->stream, hipStreamCaptureModeRelaxed));
+    mv->timer.capturing = true;
+    int rc = align_sequence(mv);
+    mv->timer.capturing = false;
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(mv->stream, &g);
+    if (rc) {
+      if (g) hipGraphDestroy(g);
+      mv->timer.drop_graph();
+      return rc;
+    }
+    if (e != hipSuccess) return fail(ZR_ERR_HIP, "hipStreamEndCapture: %s", hipGetErrorString(e));
+    mv->graph = g;
+    e = hipGraphInstantiate(&mv->graph_exec, g, nullptr, nullptr, 0);
+    if (e != hipSuccess) { drop_graph(mv); return fail(ZR_ERR_HIP, "hipGraphInstantiate: %s", hipGetErrorString(e)); }
+  }
+  ZR_HIP(hipGraphLaunch(mv->graph_exec, mv->stream));
+  if (mv->timer.on && !mv->timer.graph_pending.empty()) {
+    ZR_HIP(hipStreamSynchronize(mv->stream));
+    mv->timer.harvest_graph();
+  }
   return ZR_OK;
 }
 
@@ -786,6 +868,15 @@ extern "C" int zr_get_trimats(zr_movie* mv, float* shifts, float* corr, float* p
   if (corr) ZR_HIP(hipMemcpy(corr, mv->corr, nn * sizeof(float), hipMemcpyDeviceToHost));
   if (peaksig) ZR_HIP(hipMemcpy(peaksig, mv->peaksig, nn * sizeof(float), hipMemcpyDeviceToHost));
   if (origin) ZR_HIP(hipMemcpy(origin, mv->origin, nn * sizeof(float), hipMemcpyDeviceToHost));
+  return ZR_OK;
+}
+
+extern "C" int zr_get_frames(zr_movie* mv, float* frames) {
+  if (!mv || !frames) return fail(ZR_ERR_ARG, "NULL argument");
+  if (!mv->frames) return fail(ZR_ERR_STATE, "frames not set");
+  ZR_HIP(hipSetDevice(mv->device));
+  ZR_HIP(hipStreamSynchronize(mv->stream));
+  ZR_HIP(hipMemcpy(frames, mv->frames, (size_t)mv->N * mv->H * mv->W * sizeof(float), hipMemcpyDeviceToHost));
   return ZR_OK;
 }
 
@@ -852,6 +943,7 @@ extern "C" int zr_device_results(zr_movie* mv, const float** image_sum, const fl
 
 extern "C" int zr_set_timing(zr_movie* mv, int enable) {
   if (!mv) return fail(ZR_ERR_ARG, "NULL movie");
+  if ((enable != 0) != mv->timer.on) drop_graph(mv);   // a graph carries its timing events or not
   mv->timer.on = enable != 0;
   return ZR_OK;
 }

@@ -44,8 +44,12 @@ hipError_t zr_launch_k1_stats(const double* part, int nrb, int n, double* stats,
 hipError_t zr_launch_k2_cols(int Lper, const float2* S, const float2* Sside, const int4* work, int npairs, int Wh,
                              int Hc, int Wc, int m, int cols_per_wave, int dc_zero, const float2* twHc, float2* Z,
                              int Wk, int Hs, int hoff, int side_in_col, hipStream_t st);
+// btab (non-NULL, 2m a multiple of 16): the matrix-core pass with the table of
+// zr_launch_k2_twiddles; otherwise the vector-ALU pass
 hipError_t zr_launch_k2_rows(const float2* Z, const int4* pairs, int npairs, int Wh, int Wc, int m,
-                             const float2* twWc, const double* den, float nc2, float* windows, hipStream_t st);
+                             const float2* twWc, const float2* btab, const double* den, float nc2, float* windows,
+                             hipStream_t st);
+hipError_t zr_launch_k2_twiddles(int Wh, int Wc, int m, float2* btab, hipStream_t st);
 hipError_t zr_launch_k2_den(const int4* pairs, int npairs, const double* stats, double npix, double nc2, double* den,
                             hipStream_t st);
 hipError_t zr_launch_peaks(const float* windows, const int4* pairs, int npairs, const PeakParamsHost* hp,
@@ -69,7 +73,7 @@ struct BinChirp {
   int L;
 };
 #include <vector>
-int zr_bin_conv_size(int L);   // smallest power of two >= 2L-1
+int zr_bin_conv_size(int L);   // power of two >= max(4096, 2L-1)
 void zr_bin_chirp_tables(int L, int M, std::vector<float2>& c, std::vector<float2>& B);
 hipError_t zr_launch_bin_rows(int M, const void* raw, int dtype, int frame0, int nb, int H, int W, const float* gain,
                               float gscale, int use_scale, const BinChirp& cw, const float2* twM, int Wb, float2* Tr,
@@ -81,3 +85,5 @@ hipError_t zr_launch_bin_out(int M, const float2* Gr, int nb, int Hb, int Wb, co
 hipError_t zr_launch_bin_transpose(const float2* in, int nb, int R, int C, float2* out, hipStream_t st);
 hipError_t zr_launch_gain_convert(const void* raw, int dtype, size_t npix_frame, size_t total, const float* gain,
                                   float gscale, int use_scale, float* frames, hipStream_t st);
+// zero fill by a kernel (graph-capturable); bytes multiple of 4
+hipError_t zr_launch_fill_zero(void* p, size_t bytes, hipStream_t st);

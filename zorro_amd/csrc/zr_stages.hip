@@ -16,6 +16,7 @@
 #include <vector>
 
 #include "zr_common.h"
+#include "zr_launch.h"
 #include "zr_stages.h"
 
 // ---------------------------------------------------------------- constants
@@ -1829,7 +1830,7 @@ static void fill_post_args(zr_stage_state* s, PostArgs& q) {
 int zr_stage_solve(zr_stage_state* s, const float* shifts, const float* corr, const float* peaksig, hipStream_t st) {
   if (!s) return serr(ZR_ERR_STATE, "no stage state");
   const int N = s->N, M = s->M;
-  ZS_HIP(hipMemsetAsync(s->info, 0, sizeof(SolveInfo), st));
+  ZS_HIP(zr_launch_fill_zero(s->info, sizeof(SolveInfo), st));   // a kernel, so the solve can be graph-captured
   if (s->p.tri_mode == ZR_TRI_FIRST || s->p.tri_mode == ZR_TRI_REFINE) {
     hipLaunchKernelGGL(k_direct_positions, dim3(1), dim3(64), 0, st, shifts, N,
                        s->p.tri_mode == ZR_TRI_REFINE ? 1 : 0, s->pos, s->info);

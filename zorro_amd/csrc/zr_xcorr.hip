@@ -545,6 +545,91 @@ __global__ __launch_bounds__(1024) void k2_rows(const float2* __restrict__ Z, co
   }
 }
 
+// Pass 2 on the matrix cores (2m a multiple of 16).  The row DFT over kx is a
+// real GEMM per pair with K = 2 Wh (real and imaginary parts along kx):
+//   A[(kx, c)][dy] = Z[kx][dy].c,  B[(kx, c)][dx] = (cos, -sin)(2 pi kx (dx - m) / Wc)
+//   W[dy][dx] = sum_K A B = sum_kx Re(Z[kx][dy] e^{+2 pi i kx (dx - m) / Wc}),
+// then C = nc2 W / max(1, den).  One block per pair; wave w owns the 16-row tile
+// strip dy in [16w, 16w + 16) and all 2m/16 column tiles; K in chunks of
+// K2M_KX columns kx staged through LDS (the twiddle table B is shared by all
+// pairs and stays in L2).  v_mfma_f32_16x16x4f32: lane l supplies A[l % 16][l / 16]
+// and B[l / 16][l % 16] of a 16x4 / 4x16 step and holds D[4 (l / 16) + v][l % 16].
+typedef float zr_v4f __attribute__((ext_vector_type(4)));
+constexpr int K2M_KX = 32;
+constexpr int K2M_MAXT = 8;          // 2m / 16 <= 8 (maxShift <= 64)
+__global__ __launch_bounds__(512) void k2_rows_mfma(const float2* __restrict__ Z, int Wh, int m,
+                                                    const float2* __restrict__ Btab, const double* __restrict__ den,
+                                                    float nc2, float* __restrict__ windows) {
+  const int twoM = 2 * m, nt = twoM / 16;
+  __shared__ float As[K2M_KX][2 * 128];   // [kx][dy][c]
+  __shared__ float Bs[K2M_KX][2 * 128];   // [kx][dx][c]
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int p = blockIdx.x;
+  const float2* zp = Z + (size_t)p * Wh * twoM;
+  zr_v4f acc[K2M_MAXT];
+#pragma unroll
+  for (int t = 0; t < K2M_MAXT; ++t) acc[t] = zr_v4f{0.f, 0.f, 0.f, 0.f};
+  const int i16 = lane & 15, kq = lane >> 4;        // row/col within a tile, K slot 0..3
+  for (int kx0 = 0; kx0 < Wh; kx0 += K2M_KX) {
+    const int kc = min(K2M_KX, Wh - kx0);
+    __syncthreads();
+    for (int e = tid; e < K2M_KX * twoM; e += blockDim.x) {
+      const int kl = e / twoM, j = e - kl * twoM;
+      float2 a = make_float2(0.f, 0.f), b = make_float2(0.f, 0.f);
+      if (kl < kc) {
+        a = zp[(size_t)(kx0 + kl) * twoM + j];
+        b = Btab[(size_t)(kx0 + kl) * twoM + j];
+      }
+      As[kl][2 * j] = a.x;
+      As[kl][2 * j + 1] = a.y;
+      Bs[kl][2 * j] = b.x;
+      Bs[kl][2 * j + 1] = b.y;
+    }
+    __syncthreads();
+    if (w < nt) {
+      // K index k = 2 kl + c; a step of 4 covers kl = 2s, 2s+1
+      for (int s2 = 0; s2 < K2M_KX / 2; ++s2) {
+        const int kl = 2 * s2 + (kq >> 1), c = kq & 1;
+        const float a = As[kl][2 * (16 * w + i16) + c];
+#pragma unroll
+        for (int t = 0; t < K2M_MAXT; ++t) {
+          if (t < nt) {
+            const float b = Bs[kl][2 * (16 * t + i16) + c];
+            acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, acc[t], 0, 0, 0);
+          }
+        }
+      }
+    }
+  }
+  if (w < nt) {
+    const float sc = (float)((double)nc2 / fmax(1.0, den[p]));
+    float* wdst = windows + (size_t)p * twoM * twoM;
+#pragma unroll
+    for (int t = 0; t < K2M_MAXT; ++t) {
+      if (t < nt) {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int dy = 16 * w + 4 * kq + v, dx = 16 * t + i16;
+          wdst[dy * twoM + dx] = acc[t][v] * sc;
+        }
+      }
+    }
+  }
+}
+
+// Twiddles of the matrix-core row pass: Btab[kx][dx] = (cos, -sin)(2 pi kx (dx - m) / Wc),
+// phase index kx (dx - m) mod Wc taken exactly in integers, FP64 sin/cos.
+__global__ __launch_bounds__(256) void k2_rows_twiddles(int Wh, int Wc, int m, float2* __restrict__ Btab) {
+  const int twoM = 2 * m;
+  for (int e = blockIdx.x * 256 + threadIdx.x; e < Wh * twoM; e += gridDim.x * 256) {
+    const int kx = e / twoM, dx = e - kx * twoM;
+    const long d = (((long)(dx - m) % Wc) + Wc) % Wc;
+    const long ph = ((long)kx * d) % Wc;
+    const double a = 2.0 * 3.14159265358979323846 * (double)ph / (double)Wc;
+    Btab[e] = make_float2((float)cos(a), (float)(-sin(a)));
+  }
+}
+
 // Per-pair normalisation denominator on the unmasked path (FP64):
 // D = nc2 * sqrt((Np*SQ_J - S_J^2)(Np*SQ_I - S_I^2))
 __global__ void k2_denominators(const int4* __restrict__ pairs, int npairs, const double* __restrict__ stats,
@@ -939,6 +1024,12 @@ __global__ __launch_bounds__(64) void k_auto_prune(float* __restrict__ shifts, f
   }
 }
 
+// Zero fill (a kernel rather than hipMemsetAsync, so that captured graphs hold
+// only kernel nodes); bytes must be a multiple of 4.
+__global__ __launch_bounds__(256) void k_fill_zero(uint32_t* __restrict__ p, size_t words) {
+  for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < words; i += (size_t)gridDim.x * 256) p[i] = 0u;
+}
+
 // ----------------------------------------------------------- launchers
 // Host-side launch helpers (instantiated for the supported sizes).
 #include "zr_launch.h"
@@ -1026,8 +1117,13 @@ hipError_t zr_launch_k2_cols(int Lper, const float2* S, const float2* Sside, con
 }
 
 hipError_t zr_launch_k2_rows(const float2* Z, const int4* pairs, int npairs, int Wh, int Wc, int m,
-                             const float2* twWc, const double* den, float nc2, float* windows, hipStream_t st) {
+                             const float2* twWc, const float2* btab, const double* den, float nc2, float* windows,
+                             hipStream_t st) {
   int twoM = 2 * m;
+  if (btab && twoM % 16 == 0 && twoM <= 128) {
+    hipLaunchKernelGGL(k2_rows_mfma, dim3(npairs), dim3(64 * (twoM / 16)), 0, st, Z, Wh, m, btab, den, nc2, windows);
+    return hipGetLastError();
+  }
   int ndxg = (twoM + K2R_DX - 1) / K2R_DX, ndyg = (twoM + K2R_DY - 1) / K2R_DY;
   int nt = ndxg * ndyg;
   if (nt > 1024) return hipErrorInvalidValue;
@@ -1085,5 +1181,19 @@ hipError_t zr_launch_refine_combine(float* windows, int N, int m, int masked, in
 
 hipError_t zr_launch_common_mean(const double* stats, int N, double* out, hipStream_t st) {
   hipLaunchKernelGGL(k_common_mean, dim3(1), dim3(64), 0, st, stats, N, out);
+  return hipGetLastError();
+}
+
+hipError_t zr_launch_fill_zero(void* p, size_t bytes, hipStream_t st) {
+  const size_t words = bytes / 4;
+  const size_t blocks = std::min<size_t>((words + 255) / 256, 4096);
+  if (blocks == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_fill_zero, dim3((unsigned)blocks), dim3(256), 0, st, (uint32_t*)p, words);
+  return hipGetLastError();
+}
+
+hipError_t zr_launch_k2_twiddles(int Wh, int Wc, int m, float2* btab, hipStream_t st) {
+  const int n = Wh * 2 * m;
+  hipLaunchKernelGGL(k2_rows_twiddles, dim3((n + 255) / 256), dim3(256), 0, st, Wh, Wc, m, btab);
   return hipGetLastError();
 }

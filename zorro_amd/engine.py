@@ -179,6 +179,10 @@ class MoviePlan:
     def align(self):
         L.check(self.lib.zr_align(self._mv))
 
+    def use_graph(self, on: bool = True):
+        """Replay zr_align as one captured HIP graph (see zr_movie_use_graph)."""
+        L.check(self.lib.zr_movie_use_graph(self._mv, 1 if on else 0))
+
     def synchronize(self):
         L.check(self.lib.zr_synchronize(self._mv))
 
@@ -207,6 +211,12 @@ class MoviePlan:
         og = np.zeros((n, n), np.float32)
         L.check(self.lib.zr_get_trimats(self._mv, L.fptr(sh), L.fptr(co), L.fptr(ps), L.fptr(og)))
         return dict(shiftsTriMat=sh, corrTriMat=co, peaksigTriMat=ps, originTriMat=og)
+
+    def frames(self):
+        """The working frames [n, hb, wb] (after gain and binning)."""
+        f = np.zeros((self.n, self.hb, self.wb), np.float32)
+        L.check(self.lib.zr_get_frames(self._mv, L.fptr(f)))
+        return f
 
     def peak_sigmas(self):
         """Bmode='opti': Gaussian width chosen per pair, [N, N]."""

@@ -56,6 +56,9 @@ class ImageRegistrator:
         self.shapePadded = [4096, 4096]
         self.shapeOriginal = None
         self.shapeBinned = None
+        self.gainRef = None
+        self.gainInfo = {}
+        self.gain = None
         self.subPixReg = 16
         self.shiftMethod = "lanczos"
         self.maxShift = 100
@@ -89,8 +92,6 @@ class ImageRegistrator:
             raise NotImplementedError("user masks are not built (the single automatic mask is)")
         if self.doEvenOddFRC:
             raise NotImplementedError("evenOddFouRingCorr is broken in the reference (zorro/zorro.py:2099-2224)")
-        if np.any(self.shapeBinned):
-            raise NotImplementedError("binStack is not built")
         if self.filterMode and "hot" in self.filterMode.lower():
             raise NotImplementedError("hotpixFilter is not built")
         return make_params(triMode=self.triMode, diagWidth=self.diagWidth, diagStart=self.diagStart,
@@ -101,7 +102,7 @@ class ImageRegistrator:
                            originMode=self.originMode, suppressOrigin=self.suppressOrigin, fouCrop=self.fouCrop,
                            shapePadded=self.shapePadded, filterMode=self.filterMode,
                            doseFiltParam=self.doseFiltParam, pixelsize=self.pixelsize, voltage=self.voltage,
-                           doLazyFRC=self.doLazyFRC)
+                           doLazyFRC=self.doLazyFRC, shapeBinned=self.shapeBinned)
 
     def _plan_for(self, n, h, w):
         p = self._params()
@@ -113,6 +114,32 @@ class ImageRegistrator:
             self._plan_key = key
         return self._plan
 
+    def loadStack(self, stack, gainRef=None, gainInfo=None, gain=None):
+        """The data path of ``loadData`` for a movie stack (``zorro/zorro.py:3413-3482``):
+        integer frames (uint8/uint16 counts, or float32) times the gain reference
+        rotated/flipped per ``gainInfo`` (``:3486-3503``), divided by ``gain``.
+        The product is formed on the GPU by ``alignImageStack`` (fused with the
+        Fourier binning when ``shapeBinned`` is set); file reading stays with
+        the caller."""
+        if gainRef is not None:
+            g = np.asarray(gainRef, np.float32)
+            info = gainInfo if gainInfo is not None else self.gainInfo
+            if info.get("Diagonal"):
+                g = np.rot90(g, k=1)
+            if info.get("Horizontal") and info.get("Vertical"):
+                g = np.rot90(g, k=2)
+            elif info.get("Horizontal"):
+                g = np.fliplr(g)
+            elif info.get("Vertical"):
+                g = np.flipud(g)
+            self.gainRef = np.ascontiguousarray(g)
+            if gainInfo is not None:
+                self.gainInfo = gainInfo
+        if gain is not None:
+            self.gain = gain
+        self.images = stack
+        return self
+
     def alignImageStack(self):
         """Register, sum, FRC and dose-filter ``self.images`` (``zorro/zorro.py:1569-1716``)."""
         if self.images is None or np.ndim(self.images) != 3:
@@ -120,7 +147,20 @@ class ImageRegistrator:
         self.bench["total0"] = time.time()
         n, h, w = (int(v) for v in self.images.shape)
         plan = self._plan_for(n, h, w)
-        plan.set_frames(self.images)
+        raw_input = (self.gainRef is not None or np.any(self.shapeBinned) or
+                     str(getattr(self.images, "dtype", "float32")).replace("torch.", "") != "float32"
+                     or (self.gain and not np.isclose(self.gain, 1.0)))
+        if raw_input:
+            plan.set_frames_raw(self.images, gain=self.gainRef, gain_scalar=self.gain)
+            if np.any(self.shapeBinned):
+                # binStack replaces the stack by the binned one and scales the pixel
+                # size by the mean bin factor (zorro/zorro.py:1835-1836, :1903)
+                if self.pixelsize:
+                    self.pixelsize = self.pixelsize * float(np.mean(np.array([h, w]) / np.array(self.shapeBinned)))
+                self.images = plan.frames()
+                self.gainRef = None
+        else:
+            plan.set_frames(self.images)
         self.bench["xcorr0"] = time.time()
         plan.xcorr()
         plan.synchronize()
