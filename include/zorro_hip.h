@@ -146,7 +146,8 @@ int zr_synchronize(zr_movie* mv);
 
 /* enable != 0: zr_align captures its whole kernel sequence on the movie's
  * stream into a HIP graph on its next call and replays that graph from then on
- * (one launch per movie).  The graph holds the frames pointer: a different
+ * (three graph launches per movie: spectra, pair transform, the rest, so that
+ * stage timing can bracket the pair transform).  The graphs hold the frames pointer: a different
  * device pointer in zr_movie_set_frames, a new stream or a change of timing
  * re-captures it.  Not available for triMode 'auto', whose solver reads the
  * pair list back to the host.  No reference counterpart (launch batching). */
@@ -197,7 +198,8 @@ enum {
   ZR_STAGE_DOSE = 5,      /* shift + sums + dose filter (shifts, dose0..1) */
   ZR_STAGE_FRC = 6,       /* even/odd FRC (frc0..1) */
   ZR_STAGE_INPUT = 7,     /* gain product and Fourier binning (bin0..1) */
-  ZR_NSTAGES = 8
+  ZR_STAGE_TAIL = 8,      /* graph replay: peaks + solve + shifts/dose + FRC as one graph */
+  ZR_NSTAGES = 9
 };
 int zr_set_timing(zr_movie* mv, int enable);
 /* ms[ZR_NSTAGES] accumulated milliseconds, counts[ZR_NSTAGES] calls; reset!=0 clears. */

@@ -87,7 +87,7 @@ SIGNATURES = {
     "zr_copy_sums_device": (C.c_int, [_P, _P, _P, _P, _P]),
 }
 
-STAGES = ["spectra", "pairs", "peaks", "solve", "shifts", "dose", "frc", "input"]
+STAGES = ["spectra", "pairs", "peaks", "solve", "shifts", "dose", "frc", "input", "tail"]
 
 _lib = None
 

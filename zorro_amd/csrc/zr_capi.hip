@@ -87,11 +87,9 @@ struct StageTimer {
   struct Pending { int stage; hipEvent_t a, b; };
   std::vector<Pending> pending;
   std::vector<hipEvent_t> pool;
-  std::vector<Pending> graph_pending;   // event pairs recorded inside a captured graph
   double ms[ZR_NSTAGES] = {0};
   long count[ZR_NSTAGES] = {0};
   bool on = false;
-  bool capturing = false;
   hipEvent_t get() {
     if (!pool.empty()) { hipEvent_t e = pool.back(); pool.pop_back(); return e; }
     hipEvent_t e = nullptr;
@@ -108,21 +106,7 @@ struct StageTimer {
     if (!on || !a) return;
     hipEvent_t b = get();
     hipEventRecord(b, s);
-    (capturing ? graph_pending : pending).push_back({stage, a, b});
-  }
-  // after a replay of the captured graph has finished: its event nodes hold
-  // this replay's stage times
-  void harvest_graph() {
-    for (auto& p : graph_pending) {
-      float t = 0.f;
-      hipEventElapsedTime(&t, p.a, p.b);
-      ms[p.stage] += t;
-      count[p.stage] += 1;
-    }
-  }
-  void drop_graph() {
-    for (auto& p : graph_pending) { pool.push_back(p.a); pool.push_back(p.b); }
-    graph_pending.clear();
+    pending.push_back({stage, a, b});
   }
   void harvest() {
     for (auto& p : pending) {
@@ -138,7 +122,6 @@ struct StageTimer {
   }
   ~StageTimer() {
     for (auto& p : pending) { hipEventDestroy(p.a); hipEventDestroy(p.b); }
-    for (auto& p : graph_pending) { hipEventDestroy(p.a); hipEventDestroy(p.b); }
     for (auto e : pool) hipEventDestroy(e);
   }
 };
@@ -193,8 +176,8 @@ struct zr_movie {
   float* gain_owned = nullptr;
   // zr_align as one HIP graph (zr_movie_use_graph)
   bool use_graph = false;
-  hipGraph_t graph = nullptr;
-  hipGraphExec_t graph_exec = nullptr;
+  hipGraph_t graph[3] = {nullptr, nullptr, nullptr};          // spectra | pair transform | the rest
+  hipGraphExec_t graph_exec[3] = {nullptr, nullptr, nullptr};
   StageTimer timer;
 };
 
@@ -207,11 +190,12 @@ extern "C" int zr_device_count(int* n) {
 }
 
 static void drop_graph(zr_movie* mv) {
-  if (mv->graph_exec) hipGraphExecDestroy(mv->graph_exec);
-  if (mv->graph) hipGraphDestroy(mv->graph);
-  mv->graph_exec = nullptr;
-  mv->graph = nullptr;
-  mv->timer.drop_graph();
+  for (int i = 0; i < 3; ++i) {
+    if (mv->graph_exec[i]) hipGraphExecDestroy(mv->graph_exec[i]);
+    if (mv->graph[i]) hipGraphDestroy(mv->graph[i]);
+    mv->graph_exec[i] = nullptr;
+    mv->graph[i] = nullptr;
+  }
 }
 
 static void free_movie(zr_movie* mv) {
@@ -630,16 +614,21 @@ extern "C" int zr_movie_set_frames(zr_movie* mv, const float* frames, int on_dev
   return ZR_OK;
 }
 
-extern "C" int zr_xcorr(zr_movie* mv) {
-  if (!mv) return fail(ZR_ERR_ARG, "NULL movie");
-  if (!mv->frames) return fail(ZR_ERR_STATE, "frames not set");
-  ZR_HIP(hipSetDevice(mv->device));
+static int xcorr_pairs(zr_movie* mv, double nc2, bool refine);
+static int xcorr_peaks(zr_movie* mv);
+
+// zr_xcorr in three parts (spectra, pair transform, peaks); a graph-replayed
+// movie captures the first two as their own graphs so that the pair transform
+// can be timed by events recorded between graph launches.
+static int xcorr_part(zr_movie* mv, int part) {
   hipStream_t s = mv->stream;
   int rb = 0;
-  hipEvent_t t0 = mv->timer.begin(s);
-  const int full = mv->spectral ? 1 : 0;
   const double nc2 = 1.0 / std::pow((double)mv->Hc * (double)mv->Wc, 2.0);
   const bool refine = mv->p.tri_mode == ZR_TRI_REFINE;
+  if (part == 1) return xcorr_pairs(mv, nc2, refine);
+  if (part == 2) return xcorr_peaks(mv);
+  hipEvent_t t0 = mv->timer.begin(s);
+  const int full = mv->spectral ? 1 : 0;
   const size_t slot = (size_t)mv->Wk * mv->Hs;
   if (mv->unmasked) {
     // K1: spectra in batches of frames; frame sums on the way
@@ -689,7 +678,11 @@ extern "C" int zr_xcorr(zr_movie* mv) {
     }
   }
   mv->timer.end(ZR_STAGE_SPECTRA, t0, s);
-  // K2
+  return ZR_OK;
+}
+
+static int xcorr_pairs(zr_movie* mv, double nc2, bool refine) {
+  hipStream_t s = mv->stream;
   hipEvent_t t1 = mv->timer.begin(s);
   ZR_HIP(zr_launch_k2_cols(mv->Lper, mv->S, mv->Sside, mv->work, mv->Ptot, mv->Wh, mv->Hc, mv->Wc, mv->m, 8,
                            mv->unmasked ? 1 : 0, mv->twHc, mv->Z, mv->Wk, mv->Hs, mv->hoff, mv->side_in_col, s));
@@ -700,7 +693,11 @@ extern "C" int zr_xcorr(zr_movie* mv) {
                                     (double)mv->H * (double)mv->W, s));
   else if (!mv->unmasked) ZR_HIP(zr_launch_mnxc_combine(mv->windows, mv->pairs, mv->P, mv->N, mv->m, s));
   mv->timer.end(ZR_STAGE_PAIRS, t1, s);
-  // peaks
+  return ZR_OK;
+}
+
+static int xcorr_peaks(zr_movie* mv) {
+  hipStream_t s = mv->stream;
   hipEvent_t t2 = mv->timer.begin(s);
   // cleared by a kernel (not hipMemsetAsync) so that the sequence can be captured in a graph
   ZR_HIP(zr_launch_fill_zero(mv->shifts, sizeof(float) * 2 * mv->N * mv->N, s));
@@ -715,6 +712,16 @@ extern "C" int zr_xcorr(zr_movie* mv) {
                                 mv->p.peaksig_thres, s));
   mv->timer.end(ZR_STAGE_PEAKS, t2, s);
   mv->have_xcorr = true;
+  return ZR_OK;
+}
+
+extern "C" int zr_xcorr(zr_movie* mv) {
+  if (!mv) return fail(ZR_ERR_ARG, "NULL movie");
+  if (!mv->frames) return fail(ZR_ERR_STATE, "frames not set");
+  ZR_HIP(hipSetDevice(mv->device));
+  int rc;
+  for (int part = 0; part < 3; ++part)
+    if ((rc = xcorr_part(mv, part))) return rc;
   return ZR_OK;
 }
 
@@ -783,6 +790,20 @@ extern "C" int zr_dose_filter(zr_movie* mv) {
   return rc ? fail(rc, "%s", zr_stage_error()) : ZR_OK;
 }
 
+// the stages after the pair transform (peaks, solve, shifts/dose, FRC)
+static int align_tail(zr_movie* mv) {
+  int rc;
+  if ((rc = xcorr_part(mv, 2))) return rc;
+  if ((rc = zr_solve(mv))) return rc;
+  if (mv->p.do_dose) {
+    if ((rc = zr_dose_filter(mv))) return rc;
+  } else if ((rc = zr_apply_shifts(mv, nullptr))) {
+    return rc;
+  }
+  if (mv->p.do_frc && (rc = zr_frc(mv))) return rc;
+  return ZR_OK;
+}
+
 static int align_sequence(zr_movie* mv) {
   int rc;
   if ((rc = zr_xcorr(mv))) return rc;
@@ -811,30 +832,33 @@ extern "C" int zr_align(zr_movie* mv) {
   if (!mv->use_graph) return align_sequence(mv);
   if (!mv->frames) return fail(ZR_ERR_STATE, "frames not set");
   ZR_HIP(hipSetDevice(mv->device));
-  if (!mv->graph_exec) {
-    // capture the whole stage sequence on the movie's stream once; replays
-    // re-run it on whatever the frames buffer holds
-    mv->timer.drop_graph();
-    ZR_HIP(hipStreamBeginCapture(mv->stream, hipStreamCaptureModeRelaxed));
-    mv->timer.capturing = true;
-    int rc = align_sequence(mv);
-    mv->timer.capturing = false;
-    hipGraph_t g = nullptr;
-    hipError_t e = hipStreamEndCapture(mv->stream, &g);
-    if (rc) {
-      if (g) hipGraphDestroy(g);
-      mv->timer.drop_graph();
-      return rc;
+  if (!mv->graph_exec[0]) {
+    // capture the stage sequence on the movie's stream once, as three graphs
+    // (spectra | pair transform | the rest); replays re-run them on whatever
+    // the frames buffer holds.  Timing events stay outside the graphs.
+    const bool timing = mv->timer.on;
+    mv->timer.on = false;
+    int rc = ZR_OK;
+    for (int seg = 0; seg < 3 && rc == ZR_OK; ++seg) {
+      hipError_t e = hipStreamBeginCapture(mv->stream, hipStreamCaptureModeRelaxed);
+      if (e != hipSuccess) { rc = fail(ZR_ERR_HIP, "hipStreamBeginCapture: %s", hipGetErrorString(e)); break; }
+      rc = seg < 2 ? xcorr_part(mv, seg) : align_tail(mv);
+      hipGraph_t g = nullptr;
+      e = hipStreamEndCapture(mv->stream, &g);
+      if (rc == ZR_OK && e != hipSuccess) rc = fail(ZR_ERR_HIP, "hipStreamEndCapture: %s", hipGetErrorString(e));
+      if (rc != ZR_OK) { if (g) hipGraphDestroy(g); break; }
+      mv->graph[seg] = g;
+      e = hipGraphInstantiate(&mv->graph_exec[seg], g, nullptr, nullptr, 0);
+      if (e != hipSuccess) rc = fail(ZR_ERR_HIP, "hipGraphInstantiate: %s", hipGetErrorString(e));
     }
-    if (e != hipSuccess) return fail(ZR_ERR_HIP, "hipStreamEndCapture: %s", hipGetErrorString(e));
-    mv->graph = g;
-    e = hipGraphInstantiate(&mv->graph_exec, g, nullptr, nullptr, 0);
-    if (e != hipSuccess) { drop_graph(mv); return fail(ZR_ERR_HIP, "hipGraphInstantiate: %s", hipGetErrorString(e)); }
+    mv->timer.on = timing;
+    if (rc != ZR_OK) { drop_graph(mv); return rc; }
   }
-  ZR_HIP(hipGraphLaunch(mv->graph_exec, mv->stream));
-  if (mv->timer.on && !mv->timer.graph_pending.empty()) {
-    ZR_HIP(hipStreamSynchronize(mv->stream));
-    mv->timer.harvest_graph();
+  static const int seg_stage[3] = {ZR_STAGE_SPECTRA, ZR_STAGE_PAIRS, ZR_STAGE_TAIL};
+  for (int seg = 0; seg < 3; ++seg) {
+    hipEvent_t t = mv->timer.begin(mv->stream);
+    ZR_HIP(hipGraphLaunch(mv->graph_exec[seg], mv->stream));
+    mv->timer.end(seg_stage[seg], t, mv->stream);
   }
   return ZR_OK;
 }
@@ -943,7 +967,6 @@ extern "C" int zr_device_results(zr_movie* mv, const float** image_sum, const fl
 
 extern "C" int zr_set_timing(zr_movie* mv, int enable) {
   if (!mv) return fail(ZR_ERR_ARG, "NULL movie");
-  if ((enable != 0) != mv->timer.on) drop_graph(mv);   // a graph carries its timing events or not
   mv->timer.on = enable != 0;
   return ZR_OK;
 }

@@ -1112,20 +1112,22 @@ __global__ __launch_bounds__(256) void k_frc_rows(const float* __restrict__ even
   }
 }
 
-// Column pass: per kx, FFT the even and odd columns of every tile and accumulate
-// Re(E conj(O)) / sqrt(|O|^2 |E|^2) in FP64.
+// Column pass: per kx, FFT the even and odd columns of the tiles of one tile
+// group and accumulate Re(E conj(O)) / sqrt(|O|^2 |E|^2) in FP64 into that
+// group's partial plane (k_frc_reduce adds the groups in a fixed order).
+constexpr int FRC_GROUPS = 16;
 template <int BOX>
 __global__ __launch_bounds__(256) void k_frc_cols(const float2* __restrict__ T, const SolveInfo* __restrict__ info,
                                                  int tmax_x, int ntiles_max, const float2* __restrict__ twB,
-                                                 double* __restrict__ frc2d) {
+                                                 double* __restrict__ part) {
   constexpr int NT = 256, PER = BOX / NT;
   __shared__ float2 be[BOX], bo[BOX];
   const int tid = threadIdx.x;
-  const int k = blockIdx.x;
   const int Wh = BOX / 2 + 1;
+  const int k = blockIdx.x % Wh, g = blockIdx.x / Wh;
   double acc[PER];
   for (int i = 0; i < PER; ++i) acc[i] = 0.0;
-  for (int tile = 0; tile < ntiles_max; ++tile) {
+  for (int tile = g; tile < ntiles_max; tile += FRC_GROUPS) {
     const int ty = tile / tmax_x, tx = tile % tmax_x;
     if (ty >= info->tiles_y || tx >= info->tiles_x) continue;
     const float2* ce = T + (((size_t)tile * 2 + 0) * Wh + k) * BOX;
@@ -1143,7 +1145,17 @@ __global__ __launch_bounds__(256) void k_frc_cols(const float2* __restrict__ T, 
     }
     __syncthreads();
   }
-  for (int i = 0; i < PER; ++i) frc2d[(size_t)(tid + i * NT) * Wh + k] = acc[i];
+  double* dst = part + (size_t)g * BOX * Wh;
+  for (int i = 0; i < PER; ++i) dst[(size_t)(tid + i * NT) * Wh + k] = acc[i];
+}
+
+__global__ __launch_bounds__(256) void k_frc_reduce(const double* __restrict__ part, size_t plane,
+                                                    double* __restrict__ frc2d) {
+  for (size_t e = (size_t)blockIdx.x * 256 + threadIdx.x; e < plane; e += (size_t)gridDim.x * 256) {
+    double a = 0.0;
+    for (int g = 0; g < FRC_GROUPS; ++g) a += part[(size_t)g * plane + e];
+    frc2d[e] = a;
+  }
 }
 
 // Ring means of the normalised, fftshifted 2D FRC (bilinear radial weights).
@@ -1153,26 +1165,38 @@ __global__ __launch_bounds__(256) void k_frc_rings(const double* __restrict__ fr
   const int half = box / 2, Wh = half + 1;
   const double norm = frc2d[0];
   double num = 0.0, den = 0.0;
-  const int lo = -(k + 1), hi = k + 1;
-  const int span = hi - lo + 1;
-  for (int t = threadIdx.x; t < span * span; t += 256) {
-    int yy = lo + t / span, xx = lo + t % span;
-    if (yy < -half || yy >= half || xx < -half || xx >= half) continue;
-    double r = sqrt((double)yy * yy + (double)xx * xx);
-    double rf = floor(r);
-    double rem = r - rf;
-    int ri = (int)rf;
-    double wgt = 0.0;
-    if (ri == k) wgt += 1.0 - rem;
-    if (ri + 1 == k) wgt += rem;
-    if (wgt == 0.0) continue;
-    // fftshifted index (yy+half, xx+half) <-> FFT frequency (yy, xx)
-    int fy = (yy + box) % box, fx = (xx + box) % box;
-    double v;
-    if (fx <= half) v = frc2d[(size_t)fy * Wh + fx];
-    else v = frc2d[(size_t)((box - fy) % box) * Wh + (box - fx)];
-    num += wgt * (v / norm);
-    den += wgt;
+  // only pixels with k - 1 <= r < k + 1 carry weight to ring k: per row yy the
+  // columns |xx| in [sqrt(max(0, (k-1)^2 - yy^2)), sqrt((k+1)^2 - yy^2)], widened
+  // by one on each side and tested exactly
+  for (int yy = -(k + 1) + (int)threadIdx.x; yy <= k + 1; yy += 256) {
+    if (yy < -half || yy >= half) continue;
+    const double y2 = (double)yy * yy;
+    const double lo2 = (double)(k - 1) * (k - 1) - y2, hi2 = (double)(k + 1) * (k + 1) - y2;
+    if (hi2 < 0.0) continue;
+    const int xlo = lo2 > 0.0 ? max(0, (int)floor(sqrt(lo2)) - 1) : 0;
+    const int xhi = (int)ceil(sqrt(hi2)) + 1;
+    for (int ax = xlo; ax <= xhi; ++ax) {
+      for (int sgn = 0; sgn < 2; ++sgn) {
+        if (sgn == 1 && ax == 0) continue;
+        const int xx = sgn ? -ax : ax;
+        if (xx < -half || xx >= half) continue;
+        double r = sqrt(y2 + (double)xx * xx);
+        double rf = floor(r);
+        double rem = r - rf;
+        int ri = (int)rf;
+        double wgt = 0.0;
+        if (ri == k) wgt += 1.0 - rem;
+        if (ri + 1 == k) wgt += rem;
+        if (wgt == 0.0) continue;
+        // fftshifted index (yy+half, xx+half) <-> FFT frequency (yy, xx)
+        int fy = (yy + box) % box, fx = (xx + box) % box;
+        double v;
+        if (fx <= half) v = frc2d[(size_t)fy * Wh + fx];
+        else v = frc2d[(size_t)((box - fy) % box) * Wh + (box - fx)];
+        num += wgt * (v / norm);
+        den += wgt;
+      }
+    }
   }
   __shared__ double sn[4], sd[4];
   num = wave_sum_d(num);
@@ -1673,7 +1697,7 @@ struct zr_stage_state {
   // FRC
   int box = 512, tmax_y = 0, tmax_x = 0;
   float2 *twB = nullptr, *twB2 = nullptr, *Tfrc = nullptr;
-  double *frc2d = nullptr, *frc = nullptr;
+  double *frc2d = nullptr, *frc = nullptr, *frc_part = nullptr;
   // dose
   float2 *twHp = nullptr, *twWp = nullptr, *twWp2 = nullptr, *Tdose = nullptr, *G = nullptr;
   double *accum = nullptr, *thresq = nullptr, *drow = nullptr;
@@ -1715,7 +1739,7 @@ void zr_stage_free(zr_stage_state* s) {
   if (!s) return;
   void* ptrs[] = {s->edges, s->meas, s->hs, s->cdf, s->obj, s->info, s->weights, s->resid, s->pos, s->trans_raw, s->trans,
                   s->rel, s->err_stats, s->err_frames, s->fs, s->img_sum, s->even, s->odd, s->filt, s->rowsum,
-                  s->twB, s->twB2, s->Tfrc, s->frc2d, s->frc, s->twHp, s->twWp, s->twWp2, s->Tdose, s->G,
+                  s->twB, s->twB2, s->Tfrc, s->frc2d, s->frc, s->frc_part, s->twHp, s->twWp, s->twWp2, s->Tdose, s->G,
                   s->accum, s->thresq, s->drow, s->R0, s->RH, s->C0, s->CW, s->Py, s->Px};
   for (void* q : ptrs)
     if (q) hipFree(q);
@@ -1775,6 +1799,7 @@ int zr_stage_create(zr_stage_state** out, const zr_params& p, int N, int H, int 
     ZC(supload(s, &s->twB2, fwd_table(s->box / 2)));
     ZC(smalloc(s, (void**)&s->Tfrc, sizeof(float2) * nt * 2 * (s->box / 2 + 1) * s->box));
     ZC(smalloc(s, (void**)&s->frc2d, sizeof(double) * s->box * (s->box / 2 + 1)));
+    ZC(smalloc(s, (void**)&s->frc_part, sizeof(double) * FRC_GROUPS * s->box * (s->box / 2 + 1)));
     ZC(smalloc(s, (void**)&s->frc, sizeof(double) * s->box));
   }
   if (p.do_dose) {
@@ -1896,7 +1921,10 @@ int zr_stage_frc(zr_stage_state* s, hipStream_t st) {
     hipLaunchKernelGGL(k_frc_rows<BX>, dim3(nt * 2 * (BX / 4)), dim3(256), 0, st, s->even, s->odd, s->info, s->W, \
                        s->tmax_x, step, s->twB2, s->twB, s->Tfrc);                                            \
     ZS_HIP(hipGetLastError());                                                                                \
-    hipLaunchKernelGGL(k_frc_cols<BX>, dim3(BX / 2 + 1), dim3(256), 0, st, s->Tfrc, s->info, s->tmax_x, nt, s->twB, \
+    hipLaunchKernelGGL(k_frc_cols<BX>, dim3((BX / 2 + 1) * FRC_GROUPS), dim3(256), 0, st, s->Tfrc, s->info,     \
+                       s->tmax_x, nt, s->twB, s->frc_part);                                                  \
+    ZS_HIP(hipGetLastError());                                                                                \
+    hipLaunchKernelGGL(k_frc_reduce, dim3(256), dim3(256), 0, st, s->frc_part, (size_t)BX * (BX / 2 + 1),      \
                        s->frc2d);                                                                             \
     ZS_HIP(hipGetLastError());                                                                                \
     break;
