@@ -44,6 +44,17 @@ def k2_algorithmic_bytes(info):
     return info["pairs"] * (2 * hs + 4 * (2 * m) ** 2 + 16)
 
 
+def k2_flops(info):
+    """FFT-equivalent FLOPs of the pair transform per launch (the scope table's
+    figure: 6 hs/8 for the conjugate products + 2.5 Hc Wc log2(Hc Wc) for the
+    inverse transform, per pair)."""
+    hc, wc = info["Hc"], info["Wc"]
+    return info["pairs"] * (6.0 * hc * (wc // 2 + 1) + 2.5 * hc * wc * math.log2(hc * wc))
+
+
+FP32_PEAK_TFLOPS = 157.3
+
+
 def load_traffic():
     """HBM bytes per K2 launch from the committed rocprofv3 PMC summary, if any."""
     p = os.path.join(HERE, "profiles", "k2_pairs_traffic.json")
@@ -156,7 +167,7 @@ def bench_c2(args):
         torch.cuda.synchronize()
         del src, dst
     K = max(1, args.inflight)
-    plans = [MoviePlan(make_params(**PARAMS), c["n"], c["h"], c["w"]) for _ in range(K)]
+    plans = [MoviePlan(make_params(**PARAMS), c["n"], c["h"], c["w"]) for _ in range(max(K, 2))]
     for p in plans:
         p.set_frames(frames)
         if args.graph:
@@ -168,37 +179,48 @@ def bench_c2(args):
 
     print(f"[bench] plan ready {info}; {K} in flight; graph {args.graph}; warmup {args.warmup}", file=sys.stderr,
           flush=True)
-    for i in range(max(args.warmup, K if args.graph else 0)):
-        plans[i % K].align()
+    for i in range(max(args.warmup, len(plans) if args.graph else 0)):
+        plans[i % len(plans)].align()
     for p in plans:
         p.synchronize()
     torch.cuda.synchronize()
 
-    plan.stage_times(reset=True)
-    lat = []
-    start = [0.0] * K
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    # K plans on K streams: movie s runs on plan s % K once that plan's previous
-    # movie has finished; latency = enqueue -> finish of each movie
-    for s in range(args.steps):
-        k = s % K
-        if s >= K:
+    def timed(kin, steps):
+        """kin plans on kin streams: movie s runs on plan s % kin once that
+        plan's previous movie has finished; latency = enqueue -> finish."""
+        lat = []
+        start = [0.0] * kin
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for s in range(steps):
+            k = s % kin
+            if s >= kin:
+                plans[k].synchronize()
+                lat.append((time.perf_counter() - start[k]) * 1e3)
+            start[k] = time.perf_counter()
+            plans[k].align()
+        for s in range(max(0, steps - kin), steps):
+            k = s % kin
             plans[k].synchronize()
             lat.append((time.perf_counter() - start[k]) * 1e3)
-        start[k] = time.perf_counter()
-        plans[k].align()
-    for s in range(max(0, args.steps - K), args.steps):
-        k = s % K
-        plans[k].synchronize()
-        lat.append((time.perf_counter() - start[k]) * 1e3)
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) * 1e3 / steps, lat
+
+    plan.stage_times(reset=True)
+    ms_step, lat = timed(K, args.steps)
     st = plan.stage_times()
     plan.set_timing(False)
-    print(f"[bench] {args.steps} steps: {(t1 - t0) * 1e3 / args.steps:.2f} ms/step", file=sys.stderr, flush=True)
+    print(f"[bench] {args.steps} steps: {ms_step:.2f} ms/step", file=sys.stderr, flush=True)
+    conc = None
+    if K == 1:
+        # the same movies two at a time (serving mode): one movie's latency-bound
+        # solver stages overlap the other's bandwidth-bound ones
+        ms2, lat2 = timed(2, max(args.steps, 4))
+        conc = dict(inflight=2, value=round(60000.0 / ms2, 2), ms_per_step=round(ms2, 3),
+                    latency_ms=dict(p50=round(float(np.percentile(lat2, 50)), 3),
+                                    p99=round(float(np.percentile(lat2, 99)), 3)))
+        print(f"[bench] 2 in flight: {ms2:.2f} ms/movie", file=sys.stderr, flush=True)
 
-    ms_step = (t1 - t0) * 1e3 / args.steps
     movies_min = 60000.0 / ms_step
     sol = plan.solution()
     truth = -(drift - drift.mean(axis=0))
@@ -208,10 +230,14 @@ def bench_c2(args):
     alg = k2_algorithmic_bytes(info)
     achieved = alg / (k2_ms * 1e-3) / 1e9
     traffic = load_traffic()
+    flops = k2_flops(info)
     roofline = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
                     frac=round(achieved / HBM_PEAK_GBS, 4), traffic=traffic,
-                    kernel="pair transform (k2_cols + k2_rows)", launch_ms=round(k2_ms, 4),
-                    algorithmic_bytes=alg)
+                    kernel="pair transform (k2_cols + k2_rows_mfma)", launch_ms=round(k2_ms, 4),
+                    algorithmic_bytes=alg,
+                    # a reuse implementation above 100% also reports its FLOP fraction
+                    flop_tflops=round(flops / (k2_ms * 1e-3) / 1e12, 2),
+                    flop_frac=round(flops / (k2_ms * 1e-3) / (FP32_PEAK_TFLOPS * 1e12), 4))
 
     out = dict(metric=METRIC, value=round(movies_min, 2), unit="movies/min", n_gpus=1, steps=args.steps,
                warmup=args.warmup, ms_per_step=round(ms_step, 3), higher_is_better=True, scaling="weak",
@@ -229,6 +255,8 @@ def bench_c2(args):
                stages_ms={k: round(v[0] / max(v[1], 1), 4) for k, v in st.items() if v[1]},
                roofline=roofline,
                check=dict(rms_vs_true_drift_px=round(rms, 4), solver=plan.solver_info()))
+    if conc:
+        out["concurrent"] = conc
 
     if not args.no_cpu_baseline:
         print("[bench] timing the CPU baseline sample", file=sys.stderr, flush=True)

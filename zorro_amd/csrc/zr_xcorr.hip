@@ -555,66 +555,68 @@ __global__ __launch_bounds__(1024) void k2_rows(const float2* __restrict__ Z, co
 // pairs and stays in L2).  v_mfma_f32_16x16x4f32: lane l supplies A[l % 16][l / 16]
 // and B[l / 16][l % 16] of a 16x4 / 4x16 step and holds D[4 (l / 16) + v][l % 16].
 typedef float zr_v4f __attribute__((ext_vector_type(4)));
-constexpr int K2M_KX = 32;
-constexpr int K2M_MAXT = 8;          // 2m / 16 <= 8 (maxShift <= 64)
-__global__ __launch_bounds__(512) void k2_rows_mfma(const float2* __restrict__ Z, int Wh, int m,
-                                                    const float2* __restrict__ Btab, const double* __restrict__ den,
-                                                    float nc2, float* __restrict__ windows) {
-  const int twoM = 2 * m, nt = twoM / 16;
-  __shared__ float As[K2M_KX][2 * 128];   // [kx][dy][c]
-  __shared__ float Bs[K2M_KX][2 * 128];   // [kx][dx][c]
+constexpr int K2M_KX = 32;           // kx columns per LDS chunk (K = 64)
+// NT = 2m/16 tile strips (one wave each).  Per chunk every thread holds
+// K2M_KX*2m/(64 NT) = 8 elements of Z and of the twiddles; the next chunk's
+// loads are in flight while the current chunk's MFMAs run.
+template <int NT>
+__global__ __launch_bounds__(64 * NT) void k2_rows_mfma(const float2* __restrict__ Z, int Wh,
+                                                        const float2* __restrict__ Btab,
+                                                        const double* __restrict__ den, float nc2,
+                                                        float* __restrict__ windows) {
+  constexpr int TWOM = 16 * NT, NTH = 64 * NT;
+  constexpr int PER = K2M_KX * TWOM / NTH;        // = 8
+  constexpr int ROW = 2 * TWOM + 4;               // floats per kx row in LDS (padded)
+  __shared__ float As[K2M_KX * ROW];               // [kx][dy][c]
+  __shared__ float Bs[K2M_KX * ROW];               // [kx][dx][c]
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int p = blockIdx.x;
-  const float2* zp = Z + (size_t)p * Wh * twoM;
-  zr_v4f acc[K2M_MAXT];
+  const float2* zp = Z + (size_t)p * Wh * TWOM;
+  const int i16 = lane & 15, kq = lane >> 4;
+  zr_v4f acc[NT];
 #pragma unroll
-  for (int t = 0; t < K2M_MAXT; ++t) acc[t] = zr_v4f{0.f, 0.f, 0.f, 0.f};
-  const int i16 = lane & 15, kq = lane >> 4;        // row/col within a tile, K slot 0..3
+  for (int t = 0; t < NT; ++t) acc[t] = zr_v4f{0.f, 0.f, 0.f, 0.f};
+  float2 ra[PER], rb[PER];
+  auto load = [&](int kx0) {
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+      const int e = tid + i * NTH;                 // e = kl * TWOM + j
+      const int kx = kx0 + e / TWOM;
+      const bool ok = kx < Wh;
+      ra[i] = ok ? zp[(size_t)kx0 * TWOM + e] : make_float2(0.f, 0.f);
+      rb[i] = ok ? Btab[(size_t)kx0 * TWOM + e] : make_float2(0.f, 0.f);
+    }
+  };
+  load(0);
   for (int kx0 = 0; kx0 < Wh; kx0 += K2M_KX) {
-    const int kc = min(K2M_KX, Wh - kx0);
     __syncthreads();
-    for (int e = tid; e < K2M_KX * twoM; e += blockDim.x) {
-      const int kl = e / twoM, j = e - kl * twoM;
-      float2 a = make_float2(0.f, 0.f), b = make_float2(0.f, 0.f);
-      if (kl < kc) {
-        a = zp[(size_t)(kx0 + kl) * twoM + j];
-        b = Btab[(size_t)(kx0 + kl) * twoM + j];
-      }
-      As[kl][2 * j] = a.x;
-      As[kl][2 * j + 1] = a.y;
-      Bs[kl][2 * j] = b.x;
-      Bs[kl][2 * j + 1] = b.y;
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+      const int e = tid + i * NTH;
+      const int kl = e / TWOM, j = e - kl * TWOM;
+      *reinterpret_cast<float2*>(&As[kl * ROW + 2 * j]) = ra[i];
+      *reinterpret_cast<float2*>(&Bs[kl * ROW + 2 * j]) = rb[i];
     }
     __syncthreads();
-    if (w < nt) {
-      // K index k = 2 kl + c; a step of 4 covers kl = 2s, 2s+1
-      for (int s2 = 0; s2 < K2M_KX / 2; ++s2) {
-        const int kl = 2 * s2 + (kq >> 1), c = kq & 1;
-        const float a = As[kl][2 * (16 * w + i16) + c];
+    if (kx0 + K2M_KX < Wh) load(kx0 + K2M_KX);
 #pragma unroll
-        for (int t = 0; t < K2M_MAXT; ++t) {
-          if (t < nt) {
-            const float b = Bs[kl][2 * (16 * t + i16) + c];
-            acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, acc[t], 0, 0, 0);
-          }
-        }
-      }
+    for (int s2 = 0; s2 < K2M_KX / 2; ++s2) {
+      // K index k = 2 kl + c: this step covers kl = 2 s2, 2 s2 + 1
+      const int kl = 2 * s2 + (kq >> 1), c = kq & 1;
+      const float a = As[kl * ROW + 2 * (16 * w + i16) + c];
+      float b[NT];
+#pragma unroll
+      for (int t = 0; t < NT; ++t) b[t] = Bs[kl * ROW + 2 * (16 * t + i16) + c];
+#pragma unroll
+      for (int t = 0; t < NT; ++t) acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b[t], acc[t], 0, 0, 0);
     }
   }
-  if (w < nt) {
-    const float sc = (float)((double)nc2 / fmax(1.0, den[p]));
-    float* wdst = windows + (size_t)p * twoM * twoM;
+  const float sc = (float)((double)nc2 / fmax(1.0, den[p]));
+  float* wdst = windows + (size_t)p * TWOM * TWOM;
 #pragma unroll
-    for (int t = 0; t < K2M_MAXT; ++t) {
-      if (t < nt) {
+  for (int t = 0; t < NT; ++t)
 #pragma unroll
-        for (int v = 0; v < 4; ++v) {
-          const int dy = 16 * w + 4 * kq + v, dx = 16 * t + i16;
-          wdst[dy * twoM + dx] = acc[t][v] * sc;
-        }
-      }
-    }
-  }
+    for (int v = 0; v < 4; ++v) wdst[(16 * w + 4 * kq + v) * TWOM + 16 * t + i16] = acc[t][v] * sc;
 }
 
 // Twiddles of the matrix-core row pass: Btab[kx][dx] = (cos, -sin)(2 pi kx (dx - m) / Wc),
@@ -1121,8 +1123,12 @@ hipError_t zr_launch_k2_rows(const float2* Z, const int4* pairs, int npairs, int
                              hipStream_t st) {
   int twoM = 2 * m;
   if (btab && twoM % 16 == 0 && twoM <= 128) {
-    hipLaunchKernelGGL(k2_rows_mfma, dim3(npairs), dim3(64 * (twoM / 16)), 0, st, Z, Wh, m, btab, den, nc2, windows);
-    return hipGetLastError();
+    switch (twoM / 16) {
+#define ZR_MF(T) case T: hipLaunchKernelGGL(k2_rows_mfma<T>, dim3(npairs), dim3(64 * T), 0, st, Z, Wh, btab, den, nc2, windows); return hipGetLastError();
+      ZR_MF(1) ZR_MF(2) ZR_MF(3) ZR_MF(4) ZR_MF(5) ZR_MF(6) ZR_MF(7) ZR_MF(8)
+#undef ZR_MF
+      default: break;
+    }
   }
   int ndxg = (twoM + K2R_DX - 1) / K2R_DX, ndyg = (twoM + K2R_DY - 1) / K2R_DY;
   int nt = ndxg * ndyg;
