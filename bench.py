@@ -126,6 +126,8 @@ def main():
     ap.add_argument("--inflight", type=int, default=1,
                     help="C2 movies in flight at once (one plan and stream each)")
     ap.add_argument("--seconds", type=float, default=6.0, help="served: arrival window per rate")
+    ap.add_argument("--no-concurrent", action="store_true",
+                    help="skip the two-in-flight phase (profiling runs: one movie on the GPU at a time)")
     ap.add_argument("--graph", type=int, default=1,
                     help="replay each movie's stage sequence as one captured HIP graph (1) or launch eagerly (0)")
     args = ap.parse_args()
@@ -212,7 +214,7 @@ def bench_c2(args):
     plan.set_timing(False)
     print(f"[bench] {args.steps} steps: {ms_step:.2f} ms/step", file=sys.stderr, flush=True)
     conc = None
-    if K == 1:
+    if K == 1 and not args.no_concurrent:
         # the same movies two at a time (serving mode): one movie's latency-bound
         # solver stages overlap the other's bandwidth-bound ones
         ms2, lat2 = timed(2, max(args.steps, 4))

@@ -25,6 +25,11 @@ CASES = [
     ("1024sq_C1", 20, 1024, 1024, (0, 0), 40, 150, 1),
     ("2048sq_crop1024", 6, 2048, 2048, (1024, 1024), 40, 150, 5),
     ("4096sq_crop3072", 4, 4096, 4096, (3072, 3072), 40, 150, 2),
+    # column pass with two outputs per residue at its limits: L = 64 (uncropped
+    # 4096), L = 32 with maxShift = L, L = 48 with maxShift = L
+    ("4096sq_nocrop_L64", 3, 4096, 4096, (0, 0), 40, 150, 7),
+    ("2048sq_nocrop_m32", 8, 2048, 2048, (0, 0), 32, 150, 9),
+    ("4096sq_crop3072_m48", 3, 4096, 4096, (3072, 3072), 48, 150, 4),
     # largest built window: 128x128 lags with a 51-tap Gaussian (LDS-bound peak stage)
     ("4096sq_crop3072_m64", 3, 4096, 4096, (3072, 3072), 64, 150, 3),
 ]

@@ -146,6 +146,7 @@ struct zr_movie {
   // tables
   float2 *twWp2 = nullptr, *twWp = nullptr, *twHp = nullptr, *twHc = nullptr, *twWc = nullptr, *ktab = nullptr;
   float2* k2btab = nullptr;        // [Wh][2m] twiddles of the matrix-core row pass (NULL: vector pass)
+  float2* k2ltw = nullptr;         // [L+1][64] lane twiddles of the m <= L column pass (NULL: general pass)
   float *gy = nullptr, *gx = nullptr;
   int ry = 0, rx = 0;
   // pairs: P frame pairs; on the masked path the pair transform also runs the
@@ -202,7 +203,7 @@ static void free_movie(zr_movie* mv) {
   if (!mv) return;
   hipSetDevice(mv->device);
   drop_graph(mv);
-  void* ptrs[] = {mv->frames_owned, mv->k2btab, mv->twWp2, mv->twWp, mv->twHp, mv->twHc, mv->twWc, mv->ktab, mv->gy, mv->gx,
+  void* ptrs[] = {mv->frames_owned, mv->k2btab, mv->k2ltw, mv->twWp2, mv->twWp, mv->twHp, mv->twHc, mv->twWc, mv->ktab, mv->gy, mv->gx,
                   mv->pairs, mv->work, mv->T, mv->S, mv->Sside, mv->Z, mv->part, mv->stats, mv->den, mv->stats_c,
                   mv->windows, mv->shifts, mv->corr, mv->peaksig, mv->origin, mv->bsigma, mv->twMrow, mv->twMcol,
                   mv->twMout, mv->chW_c, mv->chW_B, mv->chH_c, mv->chH_B, mv->chHb_c, mv->chHb_B, mv->chWb_c,
@@ -442,7 +443,8 @@ extern "C" int zr_movie_create(int device, const zr_params* p, int n_frames, int
   mv->side_in_col = mv->spectral ? mv->Hc / 2 : -1;
   const size_t nspec = (size_t)mv->nspec;
   const size_t nrb_sums = (size_t)(mv->H + 7) / 8;   // k1_frame_sums row blocks
-  if ((rc = dev_alloc(&mv->T, (size_t)mv->batch * mv->Wk * mv->Hp)) ||
+  // K1 intermediate: column pairs, an odd Wk padded by one zero column
+  if ((rc = dev_alloc(&mv->T, (size_t)mv->batch * ((mv->Wk + 1) / 2) * 2 * mv->Hp)) ||
       (rc = dev_alloc(&mv->S, nspec * mv->Wk * mv->Hs)) ||
       (rc = dev_alloc(&mv->Sside, mv->spectral ? 0 : nspec * mv->Wk)) ||
       (rc = dev_alloc(&mv->Z, (size_t)mv->Ptot * mv->Wh * 2 * mv->m)) ||
@@ -460,6 +462,12 @@ extern "C" int zr_movie_create(int device, const zr_params* p, int n_frames, int
     e = zr_launch_k2_twiddles(mv->Wh, mv->Wc, mv->m, mv->k2btab, mv->own_stream);
     if (e == hipSuccess) e = hipStreamSynchronize(mv->own_stream);
     if (e != hipSuccess) { free_movie(mv); return fail(ZR_ERR_HIP, "k2 twiddles: %s", hipGetErrorString(e)); }
+  }
+  if (zr_k2_cols_r2_ok(mv->Lper, mv->m) && mv->Hc == 64 * mv->Lper) {
+    if ((rc = dev_alloc(&mv->k2ltw, (size_t)(mv->Lper + 1) * 64))) { free_movie(mv); return rc; }
+    e = zr_launch_k2_lane_twiddles(mv->Lper, mv->Hc, mv->k2ltw, mv->own_stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(mv->own_stream);
+    if (e != hipSuccess) { free_movie(mv); return fail(ZR_ERR_HIP, "k2 lane twiddles: %s", hipGetErrorString(e)); }
   }
   if (!mv->unmasked) {
     // the pair transform's own scale is nc2 / max(1, den): den = 0 keeps the
@@ -684,8 +692,12 @@ static int xcorr_part(zr_movie* mv, int part) {
 static int xcorr_pairs(zr_movie* mv, double nc2, bool refine) {
   hipStream_t s = mv->stream;
   hipEvent_t t1 = mv->timer.begin(s);
-  ZR_HIP(zr_launch_k2_cols(mv->Lper, mv->S, mv->Sside, mv->work, mv->Ptot, mv->Wh, mv->Hc, mv->Wc, mv->m, 8,
-                           mv->unmasked ? 1 : 0, mv->twHc, mv->Z, mv->Wk, mv->Hs, mv->hoff, mv->side_in_col, s));
+  if (mv->k2ltw)
+    ZR_HIP(zr_launch_k2_cols_r2(mv->Lper, mv->S, mv->Sside, mv->work, mv->Ptot, mv->Wh, mv->Hc, mv->Wc, mv->m, 8,
+                                mv->unmasked ? 1 : 0, mv->k2ltw, mv->Z, mv->Wk, mv->Hs, mv->hoff, mv->side_in_col, s));
+  else
+    ZR_HIP(zr_launch_k2_cols(mv->Lper, mv->S, mv->Sside, mv->work, mv->Ptot, mv->Wh, mv->Hc, mv->Wc, mv->m, 8,
+                             mv->unmasked ? 1 : 0, mv->twHc, mv->Z, mv->Wk, mv->Hs, mv->hoff, mv->side_in_col, s));
   ZR_HIP(zr_launch_k2_rows(mv->Z, mv->pairs, mv->Ptot, mv->Wh, mv->Wc, mv->m, mv->twWc, mv->k2btab, mv->den,
                            (float)nc2, mv->windows, s));
   if (refine)

@@ -44,6 +44,13 @@ hipError_t zr_launch_k1_stats(const double* part, int nrb, int n, double* stats,
 hipError_t zr_launch_k2_cols(int Lper, const float2* S, const float2* Sside, const int4* work, int npairs, int Wh,
                              int Hc, int Wc, int m, int cols_per_wave, int dc_zero, const float2* twHc, float2* Z,
                              int Wk, int Hs, int hoff, int side_in_col, hipStream_t st);
+// column pass for m <= L (L = Hc/64 in {32, 48, 64}): sums over lanes of
+// pre-twiddled values with the [L+1][64] table of zr_launch_k2_lane_twiddles
+bool zr_k2_cols_r2_ok(int Lper, int m);
+hipError_t zr_launch_k2_lane_twiddles(int Lper, int Hc, float2* ltw, hipStream_t st);
+hipError_t zr_launch_k2_cols_r2(int Lper, const float2* S, const float2* Sside, const int4* work, int npairs, int Wh,
+                                int Hc, int Wc, int m, int cols_per_wave, int dc_zero, const float2* ltw, float2* Z,
+                                int Wk, int Hs, int hoff, int side_in_col, hipStream_t st);
 // btab (non-NULL, 2m a multiple of 16): the matrix-core pass with the table of
 // zr_launch_k2_twiddles; otherwise the vector-ALU pass
 hipError_t zr_launch_k2_rows(const float2* Z, const int4* pairs, int npairs, int Wh, int Wc, int m,

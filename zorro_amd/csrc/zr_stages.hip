@@ -132,20 +132,22 @@ __global__ __launch_bounds__(256) void k_logi_grid(const float* __restrict__ hs,
 }
 
 // argmin of the grid (first index) then the 3x3x3 pattern search.  One block of
-// 256: every (candidate, bin) term is evaluated in parallel, each candidate's
-// objective is then summed over bins in index order.
-__global__ __launch_bounds__(256) void k_logi_refine(const float* __restrict__ hs, const float* __restrict__ cdf,
-                                                    const double* __restrict__ obj, SolveInfo* info) {
+// LOGI_NT: every (candidate, bin) term is evaluated in parallel (K = e^lk and
+// nu = e^ln once per candidate), each candidate's objective is then summed over
+// bins in index order.
+constexpr int LOGI_NT = 1024;
+__global__ __launch_bounds__(LOGI_NT) void k_logi_refine(const float* __restrict__ hs, const float* __restrict__ cdf,
+                                                        const double* __restrict__ obj, SolveInfo* info) {
   __shared__ double terms[27 * 290];
-  __shared__ double cval[27], ccand[27][3];
-  __shared__ double rv[4];
-  __shared__ int ri[4];
+  __shared__ double cval[27], ccand[27][5];
+  __shared__ double rv[LOGI_NT / 64];
+  __shared__ int ri[LOGI_NT / 64];
   const int tid = threadIdx.x;
   if (info->nbins < 1) return;
   const int total = LOGI_G0 * LOGI_G1 * LOGI_G2;
   double bv = INFINITY;
   int bi = 0x7fffffff;
-  for (int i = tid; i < total; i += 256) {
+  for (int i = tid; i < total; i += LOGI_NT) {
     double v = obj[i];
     if (v < bv || (v == bv && i < bi)) { bv = v; bi = i; }
   }
@@ -157,7 +159,7 @@ __global__ __launch_bounds__(256) void k_logi_refine(const float* __restrict__ h
   if ((tid & 63) == 0) { rv[tid >> 6] = bv; ri[tid >> 6] = bi; }
   __syncthreads();
   bv = rv[0]; bi = ri[0];
-  for (int k = 1; k < 4; ++k)
+  for (int k = 1; k < LOGI_NT / 64; ++k)
     if (rv[k] < bv || (rv[k] == bv && ri[k] < bi)) { bv = rv[k]; bi = ri[k]; }
   const double lo = info->logi[0], hi = info->logi[1];
   const double lk0 = log(LOGI_K0), lk1 = log(LOGI_K1), ln0 = log(LOGI_N0), ln1 = log(LOGI_N1);
@@ -179,11 +181,13 @@ __global__ __launch_bounds__(256) void k_logi_refine(const float* __restrict__ h
       ccand[tid][0] = fmin(fmax(cur[0] + o0 * step[0], lob[0]), hib[0]);
       ccand[tid][1] = fmin(fmax(cur[1] + o1 * step[1], lob[1]), hib[1]);
       ccand[tid][2] = fmin(fmax(cur[2] + o2 * step[2], lob[2]), hib[2]);
+      ccand[tid][3] = exp(ccand[tid][1]);
+      ccand[tid][4] = exp(ccand[tid][2]);
     }
     __syncthreads();
-    for (int e = tid; e < 27 * nb; e += 256) {
+    for (int e = tid; e < 27 * nb; e += LOGI_NT) {
       int c = e / nb, i = e % nb;
-      terms[e] = fabs((double)cdf[i] - logistic_d((double)hs[i], ccand[c][0], exp(ccand[c][1]), exp(ccand[c][2])));
+      terms[e] = fabs((double)cdf[i] - logistic_d((double)hs[i], ccand[c][0], ccand[c][3], ccand[c][4]));
     }
     __syncthreads();
     if (tid < 27) {
@@ -994,7 +998,8 @@ constexpr int SS_R = 8;
 __global__ __launch_bounds__(256) void k_shift_sum(const float* __restrict__ frames, const FrameShift* __restrict__ fs,
                                                   int N, int H, int W, float* __restrict__ img_sum,
                                                   float* __restrict__ even_sum, float* __restrict__ odd_sum,
-                                                  float* __restrict__ aligned, double* __restrict__ rowsum) {
+                                                  float* __restrict__ aligned, double* __restrict__ rowsum,
+                                                  int gx) {
   __shared__ FrameShift sfs[MAXN];
   __shared__ double red[4];
   for (int j = threadIdx.x; j < N; j += 256) {
@@ -1007,8 +1012,12 @@ __global__ __launch_bounds__(256) void k_shift_sum(const float* __restrict__ fra
     sfs[j] = f;
   }
   __syncthreads();
-  const int x = blockIdx.x * 64 + (threadIdx.x & 63);
-  const int y0 = (blockIdx.y * 4 + (threadIdx.x >> 6)) * SS_R;
+  // 1-D grid of gx x gy tiles, remapped so that neighbouring tiles (which share
+  // halo rows and columns) run on the same XCD and meet in its L2
+  const int lin = xcd_remap(blockIdx.x, gridDim.x);
+  const int bx = lin % gx, by = lin / gx;
+  const int x = bx * 64 + (threadIdx.x & 63);
+  const int y0 = (by * 4 + (threadIdx.x >> 6)) * SS_R;
   double part = 0.0;
   if (x < W && y0 < H) {
     float st[SS_R], se[SS_R], so[SS_R];
@@ -1056,16 +1065,27 @@ __global__ __launch_bounds__(256) void k_shift_sum(const float* __restrict__ fra
   part = wave_sum_d(part);
   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = part;
   __syncthreads();
-  if (threadIdx.x == 0) rowsum[blockIdx.y * gridDim.x + blockIdx.x] = red[0] + red[1] + red[2] + red[3];
+  if (threadIdx.x == 0) rowsum[lin] = red[0] + red[1] + red[2] + red[3];
 }
 
 // ------------------------------------------------------------------ K4
 // Row pass over FRC tiles: Hann-windowed tile rows -> real FFT (box) -> kx-major.
+// Hann apodisation of a box (zorro/zorro_util.py:62-104), r <= 1 inside the
+// inscribed ellipse: hann[y][x] = 0.5 (1 + cos(pi r)), tabulated once per plan.
+__global__ __launch_bounds__(256) void k_frc_hann(int box, float* __restrict__ hann) {
+  const int e = blockIdx.x * 256 + threadIdx.x;
+  if (e >= box * box) return;
+  const int yl = e / box, xl = e % box;
+  const double yv = (double)(yl - box / 2) / (box / 2), xv = (double)(xl - box / 2) / (box / 2);
+  const double rr = xv * xv + yv * yv;
+  hann[e] = rr <= 1.0 ? (float)(0.5 * (1.0 + cos(M_PI * sqrt(rr)))) : 0.f;
+}
+
 template <int BOX>
 __global__ __launch_bounds__(256) void k_frc_rows(const float* __restrict__ even, const float* __restrict__ odd,
                                                  const SolveInfo* __restrict__ info, int W, int tmax_x, double step,
                                                  const float2* __restrict__ twL, const float2* __restrict__ tw2L,
-                                                 float2* __restrict__ T) {
+                                                 const float* __restrict__ hann, float2* __restrict__ T) {
   constexpr int L = BOX / 2, RB = 4, NT = 256;
   __shared__ float2 buf[RB][L];
   const int tid = threadIdx.x;
@@ -1083,17 +1103,10 @@ __global__ __launch_bounds__(256) void k_frc_rows(const float* __restrict__ even
   for (int r = 0; r < RB; ++r) {
     const int yl = rb * RB + r;
     const float* row = img + (size_t)(oy + yl) * W + ox;
-    const double yv = (double)(yl - BOX / 2) / (BOX / 2);
+    const float* hw = hann + (size_t)yl * BOX;
     for (int n = tid; n < L; n += NT) {
-      float v2[2];
-      for (int q = 0; q < 2; ++q) {
-        int xl = 2 * n + q;
-        double xv = (double)(xl - BOX / 2) / (BOX / 2);
-        double rr = xv * xv + yv * yv;
-        float hw = rr <= 1.0 ? (float)(0.5 * (1.0 + cos(M_PI * sqrt(rr)))) : 0.f;
-        v2[q] = hw * row[xl];
-      }
-      buf[r][n] = make_float2(v2[0], v2[1]);
+      const int xl = 2 * n;
+      buf[r][n] = make_float2(hw[xl] * row[xl], hw[xl + 1] * row[xl + 1]);
     }
   }
   __syncthreads();
@@ -1564,21 +1577,30 @@ __global__ __launch_bounds__(NT) void k_dose_cols_spec(const float2* __restrict_
   }
   const float2 ex = twW[(int)(((long)kx * (a.W - 1)) % a.W)];  // e^{-2 pi i kx (W-1)/W}
   const float ord = (float)a.order;
-  for (int J = 0; J < a.N; ++J) {
+  // frame J's column (the HBM stream) is loaded one frame ahead; the border
+  // terms py, c0, cw (one column per frame, shared by every block) come from L2
+  float2 F[PER];
+  auto load = [&](int J, float2* f_) {
     const float2* col = S + ((size_t)J * Wk + kx) * HP;
-    const float2 px = Px[(size_t)J * Whalf + kx];
-    const float2 r0 = R0[(size_t)J * Whalf + kx], rh = RH[(size_t)J * Whalf + kx];
-    const float mhd = (float)(-0.5 * a.accum[J + 1]);
-    const float itq = (float)(1.0 / a.thresq[J]);
-    float2 F[PER], py[PER], c0[PER], cw[PER];
+#pragma unroll
+    for (int i = 0; i < PER; ++i) f_[i] = col[tid + i * NT];
+  };
+  if (a.N > 0) load(0, F);
+  for (int J = 0; J < a.N; ++J) {
+    float2 Fn[PER];
+    if (J + 1 < a.N) load(J + 1, Fn);
+    float2 py[PER], c0[PER], cw[PER];
 #pragma unroll
     for (int i = 0; i < PER; ++i) {
       const int ky = tid + i * NT;
-      F[i] = col[ky];
       py[i] = Py[(size_t)J * a.H + ky];
       c0[i] = C0[(size_t)J * a.H + ky];
       cw[i] = CW[(size_t)J * a.H + ky];
     }
+    const float2 px = Px[(size_t)J * Whalf + kx];
+    const float2 r0 = R0[(size_t)J * Whalf + kx], rh = RH[(size_t)J * Whalf + kx];
+    const float mhd = (float)(-0.5 * a.accum[J + 1]);
+    const float itq = (float)(1.0 / a.thresq[J]);
 #pragma unroll
     for (int i = 0; i < PER; ++i) {
       float2 sp = c_mul(c_mul(F[i], py[i]), px);
@@ -1587,6 +1609,10 @@ __global__ __launch_bounds__(NT) void k_dose_cols_spec(const float2* __restrict_
       const float gval = expf(-cutoff_pow(q[i] * itq, ord, a.order_sq)) * expf(mhd * invc[i]);
       acc[i].x += sp.x * gval;
       acc[i].y += sp.y * gval;
+    }
+    if (J + 1 < a.N) {
+#pragma unroll
+      for (int i = 0; i < PER; ++i) F[i] = Fn[i];
     }
   }
 #pragma unroll
@@ -1698,6 +1724,7 @@ struct zr_stage_state {
   int box = 512, tmax_y = 0, tmax_x = 0;
   float2 *twB = nullptr, *twB2 = nullptr, *Tfrc = nullptr;
   double *frc2d = nullptr, *frc = nullptr, *frc_part = nullptr;
+  float* hann = nullptr;
   // dose
   float2 *twHp = nullptr, *twWp = nullptr, *twWp2 = nullptr, *Tdose = nullptr, *G = nullptr;
   double *accum = nullptr, *thresq = nullptr, *drow = nullptr;
@@ -1739,7 +1766,7 @@ void zr_stage_free(zr_stage_state* s) {
   if (!s) return;
   void* ptrs[] = {s->edges, s->meas, s->hs, s->cdf, s->obj, s->info, s->weights, s->resid, s->pos, s->trans_raw, s->trans,
                   s->rel, s->err_stats, s->err_frames, s->fs, s->img_sum, s->even, s->odd, s->filt, s->rowsum,
-                  s->twB, s->twB2, s->Tfrc, s->frc2d, s->frc, s->frc_part, s->twHp, s->twWp, s->twWp2, s->Tdose, s->G,
+                  s->twB, s->twB2, s->Tfrc, s->frc2d, s->frc, s->frc_part, s->hann, s->twHp, s->twWp, s->twWp2, s->Tdose, s->G,
                   s->accum, s->thresq, s->drow, s->R0, s->RH, s->C0, s->CW, s->Py, s->Px};
   for (void* q : ptrs)
     if (q) hipFree(q);
@@ -1797,6 +1824,11 @@ int zr_stage_create(zr_stage_state** out, const zr_params& p, int N, int H, int 
     size_t nt = (size_t)std::max(1, s->tmax_y * s->tmax_x);
     ZC(supload(s, &s->twB, fwd_table(s->box)));
     ZC(supload(s, &s->twB2, fwd_table(s->box / 2)));
+    ZC(smalloc(s, (void**)&s->hann, sizeof(float) * s->box * s->box));
+    // on the null stream, waited for alone (other movies' streams keep running)
+    hipLaunchKernelGGL(k_frc_hann, dim3((s->box * s->box + 255) / 256), dim3(256), 0, 0, s->box, s->hann);
+    ZC(hipGetLastError());
+    ZC(hipStreamSynchronize(nullptr));
     ZC(smalloc(s, (void**)&s->Tfrc, sizeof(float2) * nt * 2 * (s->box / 2 + 1) * s->box));
     ZC(smalloc(s, (void**)&s->frc2d, sizeof(double) * s->box * (s->box / 2 + 1)));
     ZC(smalloc(s, (void**)&s->frc_part, sizeof(double) * FRC_GROUPS * s->box * (s->box / 2 + 1)));
@@ -1875,7 +1907,7 @@ int zr_stage_solve(zr_stage_state* s, const float* shifts, const float* corr, co
     const int total = LOGI_G0 * LOGI_G1 * LOGI_G2;
     hipLaunchKernelGGL(k_logi_grid, dim3((total + 255) / 256), dim3(256), 0, st, s->hs, s->cdf, s->info, s->obj);
     ZS_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_logi_refine, dim3(1), dim3(256), 0, st, s->hs, s->cdf, s->obj, s->info);
+    hipLaunchKernelGGL(k_logi_refine, dim3(1), dim3(LOGI_NT), 0, st, s->hs, s->cdf, s->obj, s->info);
     ZS_HIP(hipGetLastError());
   }
   SolveArgs a;
@@ -1900,9 +1932,9 @@ int zr_stage_solve(zr_stage_state* s, const float* shifts, const float* corr, co
 
 int zr_stage_shifts(zr_stage_state* s, const float* frames, float* aligned_out, hipStream_t st) {
   if (!s || !s->have_solution) return serr(ZR_ERR_STATE, "shifts before solve");
-  dim3 grid((s->W + 63) / 64, (s->H + 4 * SS_R - 1) / (4 * SS_R));
-  hipLaunchKernelGGL(k_shift_sum, grid, dim3(256), 0, st, frames, s->fs, s->N, s->H, s->W, s->img_sum, s->even,
-                     s->odd, aligned_out, s->rowsum);
+  const int gx = (s->W + 63) / 64, gy = (s->H + 4 * SS_R - 1) / (4 * SS_R);
+  hipLaunchKernelGGL(k_shift_sum, dim3(gx * gy), dim3(256), 0, st, frames, s->fs, s->N, s->H, s->W, s->img_sum,
+                     s->even, s->odd, aligned_out, s->rowsum, gx);
   ZS_HIP(hipGetLastError());
   s->have_sums = true;
   return ZR_OK;
@@ -1919,7 +1951,7 @@ int zr_stage_frc(zr_stage_state* s, hipStream_t st) {
 #define ZR_FRC(BX)                                                                                            \
   case BX:                                                                                                    \
     hipLaunchKernelGGL(k_frc_rows<BX>, dim3(nt * 2 * (BX / 4)), dim3(256), 0, st, s->even, s->odd, s->info, s->W, \
-                       s->tmax_x, step, s->twB2, s->twB, s->Tfrc);                                            \
+                       s->tmax_x, step, s->twB2, s->twB, s->hann, s->Tfrc);                                   \
     ZS_HIP(hipGetLastError());                                                                                \
     hipLaunchKernelGGL(k_frc_cols<BX>, dim3((BX / 2 + 1) * FRC_GROUPS), dim3(256), 0, st, s->Tfrc, s->info,     \
                        s->tmax_x, nt, s->twB, s->frc_part);                                                  \
@@ -1953,13 +1985,13 @@ static void fill_dose_args(zr_stage_state* s, DoseArgs& a, const double* rowsum,
 static int dose_spectral(zr_stage_state* s, const float* frames, const float2* S, int Wk, hipStream_t st) {
   const int H = s->H, W = s->W;
   // aligned sums (imageSum, even, odd) and their total for the dose per frame
-  dim3 grid((W + 63) / 64, (H + 4 * SS_R - 1) / (4 * SS_R));
-  hipLaunchKernelGGL(k_shift_sum, grid, dim3(256), 0, st, frames, s->fs, s->N, H, W, s->img_sum, s->even, s->odd,
-                     (float*)nullptr, s->rowsum);
+  const int gx = (W + 63) / 64, gy = (H + 4 * SS_R - 1) / (4 * SS_R);
+  hipLaunchKernelGGL(k_shift_sum, dim3(gx * gy), dim3(256), 0, st, frames, s->fs, s->N, H, W, s->img_sum, s->even,
+                     s->odd, (float*)nullptr, s->rowsum, gx);
   ZS_HIP(hipGetLastError());
   s->have_sums = true;
   DoseArgs a;
-  fill_dose_args(s, a, s->rowsum, (int)(grid.x * grid.y));
+  fill_dose_args(s, a, s->rowsum, gx * gy);
   hipLaunchKernelGGL(k_dose_params, dim3(1), dim3(256), 0, st, a);
   ZS_HIP(hipGetLastError());
   if (H != W) return serr(ZR_ERR_UNSUPPORTED, "spectral dose path needs square frames");

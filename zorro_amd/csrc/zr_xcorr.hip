@@ -98,23 +98,32 @@ __global__ __launch_bounds__(RB * TPR) void k1_rows(const float* __restrict__ fr
   }
   __syncthreads();
   wg_fft<L, -1, TPR>(buf[g], twL, gt);
-  // post-process and write transposed: T[fb][k][y0..y0+RB)
-  for (int k = tid; k < Wh; k += NT) {
-    const float2 w = tw2L[k];   // exp(-2*pi*i*k/WP)
-    float2 out[RB];
+  // post-process and write transposed in column pairs: T[fb][k/2][y][k%2], so a
+  // thread writes RB rows x 2 columns = 16 RB contiguous bytes (the odd pad
+  // column of an odd Wh is written as zero)
+  const int KP = (Wh + 1) / 2;
+  for (int kp = tid; kp < KP; kp += NT) {
+    float2 out[2][RB];
 #pragma unroll
-    for (int r = 0; r < RB; ++r) {
-      const float2 zk = buf[r][k % L];
-      const float2 zn = c_conj(buf[r][(L - k) % L]);
-      const float2 e = c_scale(c_add(zk, zn), 0.5f);
-      const float2 o = c_scale(c_sub(zk, zn), 0.5f);           // = i * O
-      // X = E + w * O,  O = -i * o
-      const float2 Ok = make_float2(o.y, -o.x);
-      out[r] = c_add(e, c_mul(w, Ok));
+    for (int h = 0; h < 2; ++h) {
+      const int k = 2 * kp + h;
+      const bool live = k < Wh;
+      const int kk = live ? k : 0;
+      const float2 w = tw2L[kk];   // exp(-2*pi*i*k/WP)
+#pragma unroll
+      for (int r = 0; r < RB; ++r) {
+        const float2 zk = buf[r][kk % L];
+        const float2 zn = c_conj(buf[r][(L - kk) % L]);
+        const float2 e = c_scale(c_add(zk, zn), 0.5f);
+        const float2 o = c_scale(c_sub(zk, zn), 0.5f);           // = i * O
+        // X = E + w * O,  O = -i * o
+        const float2 Ok = make_float2(o.y, -o.x);
+        out[h][r] = live ? c_add(e, c_mul(w, Ok)) : make_float2(0.f, 0.f);
+      }
     }
-    float4* dst = reinterpret_cast<float4*>(T + ((size_t)fb * Wh + k) * Hp + y0);
+    float4* dst = reinterpret_cast<float4*>(T + (((size_t)fb * KP + kp) * Hp + y0) * 2);
 #pragma unroll
-    for (int r = 0; r < RB; r += 2) dst[r / 2] = make_float4(out[r].x, out[r].y, out[r + 1].x, out[r + 1].y);
+    for (int r = 0; r < RB; ++r) dst[r] = make_float4(out[0][r].x, out[0][r].y, out[1][r].x, out[1][r].y);
   }
   if (mode != K1_FRAME) return;   // block-uniform
   // frame sums (for the intensity normalisation)
@@ -130,40 +139,54 @@ __global__ __launch_bounds__(RB * TPR) void k1_rows(const float* __restrict__ fr
   }
 }
 
-// Column pass: complex FFT of length HP along y for each kept kx, then the
+// Column pass: complex FFT of length HP along y for a pair of kept columns
+// (kx = 2 kp, 2 kp + 1: one contiguous 16-byte-per-row stream of T), then the
 // four-quadrant crop to Hc rows and the side row +Hc/2 (or all HP rows when
-// full != 0).  Loads are staged in registers before the LDS fill.
-template <int HP, int NT>
-__global__ __launch_bounds__(NT) void k1_cols(const float2* __restrict__ T, int Wh, int Hc, int frame0,
-                                              const float2* __restrict__ twH, float2* __restrict__ S,
-                                              float2* __restrict__ Sside, int full) {
-  constexpr int PER = (HP + NT - 1) / NT;
-  __shared__ float2 buf[HP];
+// full != 0).  Each column is transformed by its own group of NTC threads (the
+// groups share the block's barriers).  Loads are staged in registers before
+// the LDS fill.
+template <int HP, int NTC>
+__global__ __launch_bounds__(2 * NTC) void k1_cols(const float2* __restrict__ T, int Wh, int Hc, int frame0,
+                                                   const float2* __restrict__ twH, float2* __restrict__ S,
+                                                   float2* __restrict__ Sside, int full) {
+  constexpr int NT = 2 * NTC;
+  constexpr int PER = (HP + NT - 1) / NT;        // rows per thread in the load
+  constexpr int PERC = (HP + NTC - 1) / NTC;     // rows per thread in the store
+  extern __shared__ float2 k1c_lds[];
+  float2 (*buf)[HP] = reinterpret_cast<float2 (*)[HP]>(k1c_lds);
   const int tid = threadIdx.x;
+  const int KP = (Wh + 1) / 2;
   const int blk = xcd_remap(blockIdx.x, gridDim.x);
-  const int fb = blk / Wh, k = blk % Wh;
-  const float2* src = T + ((size_t)fb * Wh + k) * HP;
-  float2 v[PER];
+  const int fb = blk / KP, kp = blk % KP;
+  const float4* src = reinterpret_cast<const float4*>(T + ((size_t)fb * KP + kp) * HP * 2);
+  float4 v[PER];
 #pragma unroll
   for (int i = 0; i < PER; ++i)
     if (HP % NT == 0 || tid + i * NT < HP) v[i] = src[tid + i * NT];
 #pragma unroll
   for (int i = 0; i < PER; ++i)
-    if (HP % NT == 0 || tid + i * NT < HP) buf[tid + i * NT] = v[i];
+    if (HP % NT == 0 || tid + i * NT < HP) {
+      buf[0][tid + i * NT] = make_float2(v[i].x, v[i].y);
+      buf[1][tid + i * NT] = make_float2(v[i].z, v[i].w);
+    }
   __syncthreads();
-  wg_fft<HP, -1, NT>(buf, twH, tid);
+  const int g = tid / NTC, gt = tid - g * NTC;
+  wg_fft<HP, -1, NTC>(buf[g], twH, gt);
+  const int k = 2 * kp + g;
+  if (k >= Wh) return;                            // pad column of an odd Wh (after the last barrier)
   const int f = frame0 + fb;
+  const float2* col = buf[g];
   if (full) {
     float2* dst = S + ((size_t)f * Wh + k) * HP;
 #pragma unroll
-    for (int i = 0; i < PER; ++i)
-      if (HP % NT == 0 || tid + i * NT < HP) dst[tid + i * NT] = buf[tid + i * NT];
+    for (int i = 0; i < PERC; ++i)
+      if (HP % NTC == 0 || gt + i * NTC < HP) dst[gt + i * NTC] = col[gt + i * NTC];
     return;
   }
   float2* dst = S + ((size_t)f * Wh + k) * Hc;
   const int h2 = Hc / 2;
-  for (int n = tid; n < Hc; n += NT) dst[n] = buf[n < h2 ? n : n + HP - Hc];
-  if (tid == 0) Sside[(size_t)f * Wh + k] = buf[h2];
+  for (int n = gt; n < Hc; n += NTC) dst[n] = col[n < h2 ? n : n + HP - Hc];
+  if (gt == 0) Sside[(size_t)f * Wh + k] = col[h2];
 }
 
 __global__ __launch_bounds__(256) void k1_frame_stats(const double* __restrict__ part, int nrb, int n,
@@ -477,6 +500,142 @@ __global__ __launch_bounds__(64 * WPB, 2) void k2_cols(const float2* __restrict_
       __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     }
   }
+}
+
+// Pass 1 for m <= L (every residue r = dy mod L has at most two outputs:
+// dy = r and dy = r - L).  With Hc = 64 L and dy = r + L j,
+//   Z[dy] = sum_a e^{+2 pi i a r / Hc} e^{+2 pi i a j / 64} Y_a[r],
+// so lane a multiplies its Y_a[r] by the twiddle e^{+2 pi i a r/Hc} (output
+// dy = r) and that product by e^{-2 pi i a/64} (output dy = r - L): every
+// output is then a plain sum over the 64 lanes, taken through LDS, K2R2_RC
+// residues (two rows each) at a time; each row is summed by four lanes (16
+// terms each) and the four partial sums are added by two lane swaps.  The
+// twiddle table ([L+1][64], row L = e^{-2 pi i a/64}) is copied to LDS once per
+// block.  The column weight c(kx) is applied to the sums (a power of two: the
+// same result as scaling every product).  Loads: the column's rows in two
+// batches (K2R2_B1 rows of both frames, then the rest), each issued at once.
+constexpr int K2R2_RC = 8, K2R2_B1 = 32, K2R2_WPB = 4, K2R2_LP = 65, K2R2_OCC = 2;
+constexpr size_t k2r2_lds_bytes(int L) {
+  return ((size_t)K2R2_WPB * 2 * K2R2_RC * K2R2_LP + (size_t)(L + 1) * 64) * sizeof(float2);
+}
+template <int L, int WPB>
+__global__ __launch_bounds__(64 * WPB, K2R2_OCC) void k2_cols_r2(const float2* __restrict__ S, const float2* __restrict__ Sside,
+                                                           const int4* __restrict__ work, int nwork_pairs, int Wh,
+                                                           int Hc, int Wc, int m, int cols_per_wave, int dc_zero,
+                                                           const float2* __restrict__ ltw, float2* __restrict__ Z,
+                                                           int Wk, int Hs, int hoff, int side_in_col) {
+  static_assert(L % K2R2_RC == 0, "residues must split into whole chunks");
+  constexpr int LP = K2R2_LP;             // padded LDS row: rows 2 dwords apart in the banks
+  constexpr int NCH = L / K2R2_RC;
+  constexpr int ROWS = 2 * K2R2_RC;
+  extern __shared__ float2 k2_lds[];
+  const int lane = threadIdx.x & 63;
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);   // wave index (uniform)
+  float2* my = k2_lds + (size_t)w * ROWS * LP;
+  float2* tab = k2_lds + (size_t)WPB * ROWS * LP;                    // [L+1][64]
+  const int blk = xcd_remap(blockIdx.x, gridDim.x);
+  const int kgroup = blk / nwork_pairs;
+  const int4 wk = work[blk % nwork_pairs];
+  const int I = wk.x, J = wk.y, p = wk.z;
+  const int twoM = 2 * m;
+  for (int e = threadIdx.x; e < (L + 1) * 64; e += 64 * WPB) tab[e] = ltw[e];
+  __syncthreads();
+  const float2 wlane = tab[L * 64 + lane];    // e^{-2 pi i lane / 64}
+  // reader role: row slot (residue r0 + slot/2, kind slot%2), quarter of the lanes
+  const int slot = lane & (ROWS - 1), part = lane / ROWS;
+  static_assert(ROWS * 4 == 64, "four readers per row");
+  const int kbase = kgroup * WPB * cols_per_wave;
+  for (int c = 0; c < cols_per_wave; ++c) {
+    const int kx = kbase + c * WPB + w;
+    if (kx >= Wh) break;
+    const float2* sI = S + ((size_t)I * Wk + kx) * Hs;
+    const float2* sJ = S + ((size_t)J * Wk + kx) * Hs;
+    const float cw = (kx == 0 || 2 * kx == Wc) ? 1.f : 2.f;
+    float2 nv;
+    {
+      const float2 nm = c_cjmul(sI[Hc / 2 + hoff], sJ[Hc / 2 + hoff]);
+      const float2 ns = side_in_col >= 0 ? c_cjmul(sI[side_in_col], sJ[side_in_col])
+                                         : c_cjmul(Sside[(size_t)I * Wk + kx], Sside[(size_t)J * Wk + kx]);
+      nv = (kx == 0) ? nm : ((2 * kx == Wc) ? ns : c_add(nm, ns));
+      nv = c_scale(nv, 1.f / cw);          // exact: the sums are scaled by cw below
+    }
+    // column loads through buffer resources: per-row offsets stay scalar
+    const __amdgpu_buffer_rsrc_t rI = __builtin_amdgcn_make_buffer_rsrc((void*)sI, (short)0, Hs * 8, 0x00020000);
+    const __amdgpu_buffer_rsrc_t rJ = __builtin_amdgcn_make_buffer_rsrc((void*)sJ, (short)0, Hs * 8, 0x00020000);
+    float2 v[L];
+    auto ld = [&](const __amdgpu_buffer_rsrc_t& rs, int b) -> float2 {
+      const int soff = (64 * b + ((b >= L / 2) ? hoff : 0)) * 8;
+      return __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(rs, lane * 8, soff, 0));
+    };
+    constexpr int B1 = L < K2R2_B1 ? L : K2R2_B1;
+    {
+      float2 t[B1];
+#pragma unroll
+      for (int b = 0; b < B1; ++b) { v[b] = ld(rI, b); t[b] = ld(rJ, b); }
+      __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int b = 0; b < B1; ++b) v[b] = c_cjmul(v[b], t[b]);
+    }
+    if constexpr (L > B1) {
+      float2 t[L - B1];
+#pragma unroll
+      for (int b = B1; b < L; ++b) { v[b] = ld(rI, b); t[b - B1] = ld(rJ, b); }
+      __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int b = B1; b < L; ++b) v[b] = c_cjmul(v[b], t[b - B1]);
+    }
+    {
+      const bool l0 = (lane == 0);
+      const bool dc = l0 && dc_zero && kx == 0;
+      v[0] = dc ? make_float2(0.f, 0.f) : v[0];
+      v[L / 2] = l0 ? nv : v[L / 2];
+    }
+    Dft<L, 1>::run(v);
+    float2* zdst = Z + ((size_t)p * Wh + kx) * twoM;
+#pragma unroll
+    for (int ch = 0; ch < NCH; ++ch) {
+#pragma unroll
+      for (int i = 0; i < K2R2_RC; ++i) {
+        const int r = ch * K2R2_RC + i;
+        // rows of outputs outside [-m, m) are written too and never read
+        const float2 u = c_mul(v[r], tab[r * 64 + lane]);
+        my[(2 * i) * LP + lane] = u;                     // dy = r
+        my[(2 * i + 1) * LP + lane] = c_mul(u, wlane);   // dy = r - L
+      }
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      const int r = ch * K2R2_RC + (slot >> 1);
+      const int kind = slot & 1;
+      const bool valid = kind == 0 ? (r <= m - 1) : (r >= L - m);
+      float2 acc = make_float2(0.f, 0.f);
+      {
+        const float2* row = my + slot * LP + 16 * part;
+#pragma unroll
+        for (int i = 0; i < 16; ++i) acc = c_add(acc, row[i]);
+      }
+      acc.x += __shfl_xor(acc.x, 16, 64);
+      acc.y += __shfl_xor(acc.y, 16, 64);
+      acc.x += __shfl_xor(acc.x, 32, 64);
+      acc.y += __shfl_xor(acc.y, 32, 64);
+      if (valid && part == 0) {
+        const int dy = kind == 0 ? r : r - L;
+        zdst[dy + m] = c_scale(acc, cw);
+      }
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    }
+  }
+}
+
+// Lane twiddles of k2_cols_r2, row-major [L + 1][64], from the angle in FP64:
+// rows r < L hold e^{+2 pi i a r / Hc}, row L holds e^{-2 pi i a / 64}.
+__global__ __launch_bounds__(256) void k2_lane_twiddles(int L, int Hc, float2* __restrict__ ltw) {
+  const int e = blockIdx.x * 256 + threadIdx.x;
+  if (e >= (L + 1) * 64) return;
+  const int r = e / 64, a = e % 64;
+  const long ph = r < L ? (long)a * r % Hc : (long)(Hc - a * L) % Hc;
+  const double ang = 2.0 * 3.14159265358979323846 * (double)ph / (double)Hc;
+  ltw[e] = make_float2((float)cos(ang), (float)sin(ang));
 }
 
 // Pass 2.  One block per pair: W(dy,dx) = sum_kx Re(Z[kx][dy] e^{+2 pi i kx dx/Wc}),
@@ -1075,9 +1234,11 @@ hipError_t zr_launch_k1_rows(int Wp, const float* frames, int H, int W, int Hp, 
 
 hipError_t zr_launch_k1_cols(int Hp, const float2* T, int Wh, int Hc, int frame0, int nb, const float2* twH,
                              float2* S, float2* Sside, int full, hipStream_t st) {
-  int grid = nb * Wh;
+  const int grid = nb * ((Wh + 1) / 2);
   switch (Hp) {
-#define ZR_CASE(X) case X: hipLaunchKernelGGL((k1_cols<X, (X >= 2048 ? 512 : 256)>), dim3(grid), dim3(X >= 2048 ? 512 : 256), 0, st, T, Wh, Hc, frame0, twH, S, Sside, full); return hipGetLastError();
+#define ZR_CASE(X) case X: { constexpr int NTC = X >= 2048 ? 512 : 256; static std::once_flag lds_once; \
+      zr_allow_full_lds((const void*)k1_cols<X, NTC>, lds_once); \
+      hipLaunchKernelGGL((k1_cols<X, NTC>), dim3(grid), dim3(2 * NTC), 2 * X * sizeof(float2), st, T, Wh, Hc, frame0, twH, S, Sside, full); return hipGetLastError(); }
     ZR_CASE(256) ZR_CASE(512) ZR_CASE(1024) ZR_CASE(2048) ZR_CASE(3072) ZR_CASE(3840) ZR_CASE(4096) ZR_CASE(5760)
 #undef ZR_CASE
     default: return hipErrorInvalidValue;
@@ -1113,6 +1274,32 @@ hipError_t zr_launch_k2_cols(int Lper, const float2* S, const float2* Sside, con
       zr_allow_full_lds((const void*)k2_cols<X, K2C_WPB>, lds_once); \
       hipLaunchKernelGGL((k2_cols<X, K2C_WPB>), dim3(grid), dim3(64 * K2C_WPB), sh, st, S, Sside, work, npairs, Wh, Hc, Wc, m, cols_per_wave, dc_zero, twHc, Z, Wk, Hs, hoff, side_in_col); return hipGetLastError(); }
     ZR_CASE(4) ZR_CASE(8) ZR_CASE(16) ZR_CASE(32) ZR_CASE(48) ZR_CASE(64)
+#undef ZR_CASE
+    default: return hipErrorInvalidValue;
+  }
+}
+
+bool zr_k2_cols_r2_ok(int Lper, int m) { return (Lper == 32 || Lper == 48 || Lper == 64) && m >= 1 && m <= Lper; }
+
+hipError_t zr_launch_k2_lane_twiddles(int Lper, int Hc, float2* ltw, hipStream_t st) {
+  const int n = (Lper + 1) * 64;
+  hipLaunchKernelGGL(k2_lane_twiddles, dim3((n + 255) / 256), dim3(256), 0, st, Lper, Hc, ltw);
+  return hipGetLastError();
+}
+
+hipError_t zr_launch_k2_cols_r2(int Lper, const float2* S, const float2* Sside, const int4* work, int npairs, int Wh,
+                                int Hc, int Wc, int m, int cols_per_wave, int dc_zero, const float2* ltw, float2* Z,
+                                int Wk, int Hs, int hoff, int side_in_col, hipStream_t st) {
+  if (!zr_k2_cols_r2_ok(Lper, m) || Hc != 64 * Lper) return hipErrorInvalidValue;
+  const int cols_per_block = K2R2_WPB * cols_per_wave;
+  const int ngroups = (Wh + cols_per_block - 1) / cols_per_block;
+  const int grid = ngroups * npairs;
+  const size_t sh = k2r2_lds_bytes(Lper);
+  switch (Lper) {
+#define ZR_CASE(X) case X: { static std::once_flag lds_once; \
+      zr_allow_full_lds((const void*)k2_cols_r2<X, K2R2_WPB>, lds_once); \
+      hipLaunchKernelGGL((k2_cols_r2<X, K2R2_WPB>), dim3(grid), dim3(64 * K2R2_WPB), sh, st, S, Sside, work, npairs, Wh, Hc, Wc, m, cols_per_wave, dc_zero, ltw, Z, Wk, Hs, hoff, side_in_col); return hipGetLastError(); }
+    ZR_CASE(32) ZR_CASE(48) ZR_CASE(64)
 #undef ZR_CASE
     default: return hipErrorInvalidValue;
   }
