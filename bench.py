@@ -64,7 +64,7 @@ def load_traffic():
     return None
 
 
-PAIR_KERNELS = ("k2_cols", "k2_rows")
+PAIR_KERNELS = ("k2_cols", "k2_cols_r2", "k2_rows", "k2_rows_mfma")
 
 
 def pmc_summary(fetch_csv, write_csv, out_json):

@@ -72,6 +72,16 @@ typedef struct zr_params {
   double frc_overlap;                     /* tiledFRC overlap */
   int auto_max;                           /* autoMax (triMode 'auto') */
   int shape_binned[2];                    /* shapeBinned; {0,0} = no binning (binStack 'fourier') */
+  /* 'hot' in filterMode: hotpixFilter (zorro/zorro.py:2497-2662) with hotpixInfo
+   * (:139-141); decorrOutliers (random noise) is not built */
+  int do_hot;
+  double hot_logistic_k, hot_relax, hot_max_sigma;  /* logisticK, relax, maxSigma */
+  int hot_psf_k2;                                   /* psf == 'K2' (else the uniform 3x3 kernel) */
+  double hot_cutoff_lower, hot_cutoff_upper;        /* cut-offs used when a line fit fails */
+  int hot_fixed_cutoffs;                            /* 1: use the two cut-offs as given, no fits
+                                                       (hotpixFilter(cutoffLower, cutoffUpper)) */
+  double hot_neighbour_thres;                       /* neighbourThres */
+  double hot_mu_sigma[2];                           /* both > 0: fixed normal-CDF (mu, sigma) instead of the fit */
 } zr_params;
 
 /* Element types of raw frames (MRC modes: uint8 counts, uint16, float32). */
@@ -110,7 +120,10 @@ int zr_movie_set_frames_raw(zr_movie* mv, const void* frames, int dtype, int on_
 
 /* All-pairs masked normalised cross-correlation, peak location, significance
  * and sub-pixel shifts into device trimats.
- * Replaces xcorrnm2_tri + mnxc2 + locatePeak (zorro/zorro.py:773-1231). */
+ * Replaces xcorrnm2_tri + mnxc2 + locatePeak (zorro/zorro.py:773-1231).
+ * With do_hot set, the outlier-pixel filter runs first and rewrites the frames
+ * in place (hotpixFilter replaces self.images, zorro/zorro.py:1595-1596,
+ * :2647); frames given by device pointer are modified. */
 int zr_xcorr(zr_movie* mv);
 
 /* Weights and the weighted-L1 shift solution (FP64), translations.
@@ -160,6 +173,12 @@ int zr_get_trimats(zr_movie* mv, float* shifts, float* corr, float* peaksig, flo
  * and binning of zr_movie_set_frames_raw: self.images after binStack,
  * zorro/zorro.py:1903). */
 int zr_get_frames(zr_movie* mv, float* frames);
+/* hotpixFilter results (hotpixInfo, zorro/zorro.py:2578-2602, :2637):
+ * info10 = {mu, sigma (normal-CDF fit of the unaligned-sum histogram),
+ * cutoffLower, cutoffUpper, guessDeadpix, guessHotpix, neighborPix, frameMean,
+ * frameStd, fit objective}.  Fails when the filter has not run or the
+ * histogram range was outside 2..65536 unit bins. */
+int zr_get_hotpix_info(zr_movie* mv, double* info10);
 /* Bmode='opti': the Gaussian width chosen for each pair, f32 [N][N] (the
  * reference only prints it, zorro/zorro.py:1123-1124). */
 int zr_get_peak_sigmas(zr_movie* mv, float* sigma);

@@ -50,6 +50,7 @@ __all__ = [
     "shifts_solver", "lanczos_kernel", "apply_shifts", "rotmean",
     "hann_apodization", "frc_crop_limits", "tiled_frc", "lazy_frc",
     "dose_filter", "orient_gain", "apply_gain", "bin_stack", "align_image_stack", "PINS",
+    "HOTPIX_DEFAULTS", "hotpix_psf_kernel", "hotpix_norm_objective", "hotpix_stats", "hotpix_filter",
 ]
 
 # Which oracle function is pinned by what (mirrored in tests/test_oracle_pins.py).
@@ -106,6 +107,8 @@ class Params:
     doLazyFRC: bool = True               # :164
     frc_box: int = 512                   # :2226
     frc_overlap: float = 0.5             # :2226
+    hotpixInfo: Optional[dict] = None    # :139-141 ('hot' in filterMode)
+    hotMuSigma: Optional[tuple] = None   # fixed normal-CDF fit of hotpixFilter (test override)
 
 
 # ---------------------------------------------------------------------------
@@ -202,9 +205,10 @@ def frame_spectra(images: np.ndarray, mask: np.ndarray, shapeC):
     return F, Q
 
 
-def window_from_full(C: np.ndarray, m: int, suppressOrigin: bool = True):
+def window_from_full(C: np.ndarray, m, suppressOrigin: bool = True):
     """Origin value, Q3 origin suppression, ifftshift and crop to the 2m x 2m lag
-    window (index i <-> lag i-m).  ``zorro/zorro.py:1078-1092``."""
+    window (index i <-> lag i-m); m=None keeps the whole shifted map (preShift,
+    ``:1089-1092``).  ``zorro/zorro.py:1078-1092``."""
     C = C.copy()
     origin = C[0, 0]
     if suppressOrigin:
@@ -212,6 +216,8 @@ def window_from_full(C: np.ndarray, m: int, suppressOrigin: bool = True):
         C[0, 0] = 0.125 * (C[1, 0] + C[0, 1] + C[-1, 0] + C[-1, 0] +
                            C[1, 1] + C[-1, 1] + C[-1, 1] + C[-1, -1])
     C = np.fft.ifftshift(C)
+    if m is None:
+        return C, origin
     h2, w2 = C.shape[0] // 2, C.shape[1] // 2
     return C[h2 - m:h2 + m, w2 - m:w2 + m].copy(), origin
 
@@ -278,15 +284,22 @@ def _py_slice(c, half, n):
     return s, max(0, e - s)
 
 
-def locate_peak(C: np.ndarray, p: Params, shapePadded):
+def locate_peak(C: np.ndarray, p: Params, shapePadded, roll=None):
     """Peak, significance and Fourier-interpolated sub-pixel shift.
-    ``zorro/zorro.py:1099-1231``.  Returns (shift_yx, corr, peaksig, cmaxpos)."""
+    ``zorro/zorro.py:1099-1231``.  Returns (shift_yx, corr, peaksig, cmaxpos).
+    ``roll`` (preShift): C is the whole shifted map and the maxShift disc,
+    centred on the map, is rolled by ``roll`` = (dy, dx) (``:1148-1163``)."""
     Cf = b_filter(C, p.Bmode, p.Brad, shapePadded)
     m = p.maxShift
     if not m:
         raise NotImplementedError("maxShift=None is outside the supported path")
-    xm, ym = np.meshgrid(np.arange(-m, m), np.arange(-m, m))
-    disc = (xm * xm + ym * ym) < m ** 2                                 # :841-851
+    if roll is None:
+        xm, ym = np.meshgrid(np.arange(-m, m), np.arange(-m, m))
+        disc = (xm * xm + ym * ym) < m ** 2                             # :841-851
+    else:
+        h2, w2 = C.shape[0] // 2, C.shape[1] // 2
+        xm, ym = np.meshgrid(np.arange(-h2, h2), np.arange(-w2, w2))      # :842-844 (square maps)
+        disc = np.roll(np.roll((xm * xm + ym * ym) < m ** 2, int(roll[0]), axis=0), int(roll[1]), axis=1)
     Cm = Cf * disc
     cmax = np.unravel_index(np.argmax(Cm), Cm.shape)                     # :1167
     vals = Cf[disc]
@@ -361,15 +374,22 @@ def xcorrnm2_tri(images: np.ndarray, mask: np.ndarray, p: Params, return_windows
             if return_windows:
                 windows[(I, I)] = W
         tri = np.zeros((0, n), bool)
+    if p.preShift and (p.triMode == "refine" or shapeC[0] != shapeC[1]):
+        raise NotImplementedError("preShift: 'refine' is not working in the reference (:1150) and non-square maps "
+                                  "do not broadcast against its mask (:842-844)")
+    mwin = None if p.preShift else p.maxShift
     for I in range(tri.shape[0]):          # one row for triMode first
         for J in np.nonzero(tri[I])[0]:
             if unmasked:
                 W, o = mnxc2_window_unmasked(F[I], F[J], sums[I], sumsq[I], sums[J], sumsq[J],
-                                             shapeP, shapeC, p.maxShift, p.suppressOrigin)
+                                             shapeP, shapeC, mwin, p.suppressOrigin)
             else:
-                W, o = mnxc2_window(F[I], Q[I], F[J], Q[J], M, mp, shapeC, p.maxShift, p.suppressOrigin)
+                W, o = mnxc2_window(F[I], Q[I], F[J], Q[J], M, mp, shapeC, mwin, p.suppressOrigin)
             origin[I, J] = o
-            s, c, ps, _ = locate_peak(W, p, shapeP)
+            # preShift: the disc follows the previous pair of the row (:1151-1153);
+            # shiftsTriMat still holds cropped pixels here (rescaled after the loop)
+            roll = np.round(shifts[I, J - 1]).astype(int) if p.preShift else None
+            s, c, ps, _ = locate_peak(W, p, shapeP, roll)
             shifts[I, J] = s
             corr[I, J] = c
             peaksig[I, J] = ps
@@ -825,6 +845,137 @@ def bin_stack(images: np.ndarray, shapeBinned):
 
 
 # ---------------------------------------------------------------------------
+# Outlier-pixel filter (filterMode containing 'hot')
+# ---------------------------------------------------------------------------
+
+HOTPIX_DEFAULTS = {"logisticK": 6.0, "relax": 0.925, "maxSigma": 8.0, "psf": "K2",
+                   "cutoffLower": -4.0, "cutoffUpper": 3.25}          # zorro/zorro.py:139-141
+
+
+def hotpix_psf_kernel(psf: str = "K2") -> np.ndarray:
+    """3x3 point-spread kernel with a zero centre, normalised to sum 1
+    (``zorro/zorro.py:2510-2519``)."""
+    p1 = np.float32(0.173235968) if psf == "K2" else np.float32(1.0)
+    k = np.array([[p1 * p1, p1, p1 * p1], [p1, 0.0, p1], [p1 * p1, p1, p1 * p1]], dtype=np.float32)
+    return k / np.sum(k)
+
+
+def hotpix_norm_objective(params, counts, cdf):
+    """sum |cdf - Phi((c - mu)/sigma)| (``zorro/zorro.py:2550-2552``)."""
+    import scipy.stats
+    return float(np.sum(np.abs(cdf - scipy.stats.norm.cdf(counts, loc=params[0], scale=params[1]))))
+
+
+def hotpix_stats(unaligned_sum: np.ndarray, info: dict, cutoffLower=None, cutoffUpper=None):
+    """Histogram of the unaligned sum, its CDF, the bounded normal-CDF fit and the
+    two sigma cut-offs (``zorro/zorro.py:2529-2596``).  ``info['mu_sigma']``, when
+    present, replaces the L-BFGS-B fit (the GPU path takes the same override).
+    Returns a dict with edges, hist, counts, cdf, mu, sigma, cutoffLower,
+    cutoffUpper and the objective at (mu, sigma)."""
+    import scipy.special
+    s = np.asarray(unaligned_sum, np.float32)
+    sum_mean = float(np.mean(s, dtype=np.float64))
+    pstd = math.sqrt(sum_mean)
+    max_sigma = float(info.get("maxSigma", 8.0))
+    edges = np.arange(math.floor(sum_mean - max_sigma * pstd) - 0.5, math.ceil(sum_mean + max_sigma * pstd) + 0.5, 1)
+    hist, edges = np.histogram(s, edges)
+    hist = hist.astype(np.float32)
+    counts = edges[:-1].astype(np.float32)
+    counts += np.float32(0.5) * (counts[1] - counts[0])
+    cdf = np.cumsum(hist, dtype=np.float32)
+    cdf /= cdf[-1]
+    if info.get("mu_sigma") is not None:
+        mu, sigma = (float(v) for v in info["mu_sigma"])
+    else:
+        res = scipy.optimize.minimize(hotpix_norm_objective, (sum_mean, pstd), args=(counts, cdf), method="L-BFGS-B",
+                                      bounds=((sum_mean - 0.5 * pstd, sum_mean + 0.5 * pstd),
+                                              (0.7 * pstd, 1.3 * pstd)))
+        mu, sigma = float(res.x[0]), float(res.x[1])
+    sig_cdf = (np.float32(math.sqrt(2.0)) * scipy.special.erfinv(np.float32(2.0) * cdf - np.float32(1.0))).astype(np.float32)
+    normal_sigma = ((counts - np.float32(mu)) / np.float32(sigma)).astype(np.float32)
+    err = (normal_sigma - sig_cdf).astype(np.float32)
+    keep = ~np.isinf(err)
+    err, ns = err[keep], normal_sigma[keep]
+
+    def fit(xs, ys):
+        if xs.size == 0:
+            raise ValueError("empty fit")
+        A = np.stack([xs.astype(np.float64), np.ones(xs.size)]).T
+        sol = np.linalg.lstsq(A, ys.astype(np.float64), rcond=None)[0]
+        return np.float32(-sol[1] / sol[0])
+
+    # given cut-offs skip their fits (hotpixFilter(cutoffLower, cutoffUpper), :2570, :2583);
+    # an empty fit raised LinAlgError in the numpy of the reference's time, which
+    # the reference catches and replaces by the default (:2579-2581, :2590-2592)
+    if cutoffLower:
+        cut_lo = np.float32(cutoffLower)
+    else:
+        try:
+            lower_index = np.where(err > -0.5)[0][0]
+            cut_lo = fit(ns[:lower_index], err[:lower_index])
+        except (IndexError, ValueError, np.linalg.LinAlgError):
+            cut_lo = np.float32(info.get("cutoffLower", -4.0))
+    if cutoffUpper:
+        cut_hi = np.float32(cutoffUpper)
+    else:
+        try:
+            upper_index = np.where(err < 0.5)[0][-1]
+            cut_hi = fit(ns[upper_index:], err[upper_index:])
+        except (IndexError, ValueError, np.linalg.LinAlgError):
+            cut_hi = np.float32(info.get("cutoffUpper", 3.25))
+    return dict(edges=edges, hist=hist, counts=counts, cdf=cdf, mu=mu, sigma=sigma, cutoffLower=float(cut_lo),
+                cutoffUpper=float(cut_hi), objective=hotpix_norm_objective((mu, sigma), counts, cdf),
+                sumMean=sum_mean)
+
+
+def hotpix_filter(images: np.ndarray, info: dict | None = None, neighbourThres: float = 0.01, cutoffLower=None,
+                  cutoffUpper=None):
+    """``hotpixFilter`` (``zorro/zorro.py:2497-2662``): logistic weights from the
+    sigma of each pixel of the unaligned sum, singleton outliers replaced by the
+    PSF-weighted mean of their neighbours, clustered outliers by the mean of
+    their non-outlier 3x3 neighbours (or the fitted frame mean).  float32 per
+    pixel as in the reference; ``decorrOutliers`` (random noise) is not
+    restated.  Returns (filtered float32 stack, stats dict)."""
+    info = {**HOTPIX_DEFAULTS, **(info or {})}
+    imgs = np.asarray(images, np.float32)
+    n, h, w = imgs.shape
+    psf = hotpix_psf_kernel(info["psf"])
+    usum = np.sum(imgs, axis=0, dtype=np.float32)
+    st = hotpix_stats(usum, info, cutoffLower, cutoffUpper)
+    mu, sigma = st["mu"], st["sigma"]
+    usig = ((usum - np.float32(mu)) / np.float32(sigma)).astype(np.float32)
+    lk, rel = np.float32(info["logisticK"]), np.float32(info["relax"])
+    cl, cu = st["cutoffLower"], st["cutoffUpper"]
+    # numexpr types (:2610-2611): the exponent and exp in float32 (float32
+    # operands), the 1.0 constants make the rest float64, cast at the end
+    with np.errstate(over="ignore"):
+        e1 = np.exp(lk * (usig - np.float32(cl) * rel)).astype(np.float64)
+        e2 = np.exp(lk * (usig - np.float32(cu) * rel)).astype(np.float64)
+        lm = ((1.0 - 1.0 / (1.0 + e1)) / (1.0 + e2)).astype(np.float32)
+    conv_lm = (np.float32(1.0) - lm).astype(np.float32)
+    nmask = (conv_lm * scipy.ndimage.convolve(conv_lm, psf)).astype(np.float32)    # :2616
+    single = (conv_lm * (nmask <= neighbourThres)).astype(np.float32)              # :2622
+    umean = (usum / np.float32(n)).astype(np.float32)                              # :2624
+    pfm = scipy.ndimage.convolve(umean, psf).astype(np.float32)                    # :2625
+    nb = nmask > neighbourThres                                                    # :2631
+    best_mean = mu / n
+    nfilt = np.zeros_like(pfm)
+    for ny, nx in zip(*np.nonzero(nb)):                                            # :2637-2645
+        hood = nb[ny - 1:ny + 2, nx - 1:nx + 2]
+        with np.errstate(invalid="ignore", divide="ignore"):
+            ratio = np.sum(hood) / hood.size
+        if ratio > 0.66 or ratio <= 0.001 or np.isnan(ratio):
+            nfilt[ny, nx] = best_mean
+        else:
+            nfilt[ny, nx] = conv_lm[ny, nx] * np.mean(umean[ny - 1:ny + 2, nx - 1:nx + 2][~hood])
+    out = (lm * imgs + single * pfm) + nfilt                                       # :2647
+    st.update(guessDeadpix=int(np.sum(usig < np.float32(cl))), guessHotpix=int(np.sum(usig > np.float32(cu))),
+              neighborPix=int(np.count_nonzero(nb)), frameMean=mu / n, frameStd=sigma / math.sqrt(n),
+              logisticMask=lm, unalignedSum=usum)
+    return out.astype(np.float32), st
+
+
+# ---------------------------------------------------------------------------
 # Orchestration
 # ---------------------------------------------------------------------------
 
@@ -837,6 +988,13 @@ def align_image_stack(images: np.ndarray, p: Params):
         images, scale = bin_stack(images, p.shapeBinned)
         if p.pixelsize:
             p = Params(**{**p.__dict__, "pixelsize": p.pixelsize * float(np.mean(scale))})
+    res_hot = None
+    if p.filterMode and "hot" in p.filterMode.lower():                    # :1595-1596, after binning
+        info = dict(p.hotpixInfo or {})
+        if p.hotMuSigma is not None:
+            info["mu_sigma"] = p.hotMuSigma
+        filt, res_hot = hotpix_filter(images, info)
+        images = filt.astype(np.float64)
     n, h, w = images.shape
     padded, mask = pad_stack(images, p.shapePadded)
     xc = xcorrnm2_tri(padded, mask, p)
@@ -848,7 +1006,7 @@ def align_image_stack(images: np.ndarray, p: Params):
         ed = shifts_solver(xc["shiftsTriMat"], xc["corrTriMat"], xc["peaksigTriMat"], p, padded.shape[1:])
     ed["originTriMat"] = xc["originTriMat"]
     aligned, image_sum, trans = apply_shifts(images, ed["translations"], p)
-    res = dict(xcorr=xc, errorDict=ed, translations=trans, imageSum=image_sum, aligned=aligned)
+    res = dict(xcorr=xc, errorDict=ed, translations=trans, imageSum=image_sum, aligned=aligned, hotpix=res_hot)
     if p.doLazyFRC:
         frc, info = lazy_frc(aligned, trans, p.frc_box, p.frc_overlap)
         res.update(FRC=frc, FRC2D=info["FRC2D"], evenSum=info["evenSum"], oddSum=info["oddSum"],

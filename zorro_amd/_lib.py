@@ -37,6 +37,9 @@ class ZrParams(C.Structure):
         ("do_frc", C.c_int), ("frc_box", C.c_int), ("frc_overlap", C.c_double),
         ("auto_max", C.c_int),
         ("shape_binned", C.c_int * 2),
+        ("do_hot", C.c_int), ("hot_logistic_k", C.c_double), ("hot_relax", C.c_double),
+        ("hot_max_sigma", C.c_double), ("hot_psf_k2", C.c_int), ("hot_cutoff_lower", C.c_double),
+        ("hot_cutoff_upper", C.c_double), ("hot_fixed_cutoffs", C.c_int), ("hot_neighbour_thres", C.c_double), ("hot_mu_sigma", C.c_double * 2),
     ]
 
 
@@ -72,6 +75,7 @@ SIGNATURES = {
     "zr_get_trimats": (C.c_int, [_P, _F, _F, _F, _F]),
     "zr_get_windows": (C.c_int, [_P, _F]),
     "zr_get_peak_sigmas": (C.c_int, [_P, _F]),
+    "zr_get_hotpix_info": (C.c_int, [_P, _D]),
     "zr_get_frames": (C.c_int, [_P, _F]),
     "zr_pair_count": (C.c_int, [_P, _I]),
     "zr_equation_count": (C.c_int, [_P, _I]),

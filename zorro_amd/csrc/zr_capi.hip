@@ -11,6 +11,7 @@
 #include <vector>
 
 #include "../../include/zorro_hip.h"
+#include "zr_hot.h"
 #include "zr_launch.h"
 #include "zr_stages.h"
 
@@ -175,6 +176,11 @@ struct zr_movie {
   void* raw_owned = nullptr;
   size_t raw_owned_bytes = 0;
   float* gain_owned = nullptr;
+  // outlier-pixel filter (zr_hot.hip), one device block carved into its buffers
+  void* hot_block = nullptr;
+  HotBuffers hb{};
+  HotArgs ha{};
+  bool hot_ran = false;
   // zr_align as one HIP graph (zr_movie_use_graph)
   bool use_graph = false;
   hipGraph_t graph[3] = {nullptr, nullptr, nullptr};          // spectra | pair transform | the rest
@@ -207,7 +213,7 @@ static void free_movie(zr_movie* mv) {
                   mv->pairs, mv->work, mv->T, mv->S, mv->Sside, mv->Z, mv->part, mv->stats, mv->den, mv->stats_c,
                   mv->windows, mv->shifts, mv->corr, mv->peaksig, mv->origin, mv->bsigma, mv->twMrow, mv->twMcol,
                   mv->twMout, mv->chW_c, mv->chW_B, mv->chH_c, mv->chH_B, mv->chHb_c, mv->chHb_B, mv->chWb_c,
-                  mv->chWb_B, mv->binA, mv->binB, mv->raw_owned, mv->gain_owned};
+                  mv->chWb_B, mv->binA, mv->binB, mv->raw_owned, mv->gain_owned, mv->hot_block};
   for (void* q : ptrs)
     if (q) hipFree(q);
   zr_stage_free(mv->st);
@@ -463,6 +469,48 @@ extern "C" int zr_movie_create(int device, const zr_params* p, int n_frames, int
     if (e == hipSuccess) e = hipStreamSynchronize(mv->own_stream);
     if (e != hipSuccess) { free_movie(mv); return fail(ZR_ERR_HIP, "k2 twiddles: %s", hipGetErrorString(e)); }
   }
+  if (p->do_hot) {
+    // hotpixFilter's inputs (zorro/zorro.py:2510-2519, :2606-2609): PSF
+    // kernel [[p2, p1, p2], [p1, 0, p1], [p2, p1, p2]] / sum in float32
+    const float p1 = p->hot_psf_k2 ? 0.173235968f : 1.f, p2 = p1 * p1;
+    const float ksum = (((p2 + p1) + (p2 + p1)) + ((0.f + p1) + (p2 + p1))) + p2;
+    HotArgs& ha = mv->ha;
+    ha.N = mv->N;
+    ha.logistic_k = (float)p->hot_logistic_k;
+    ha.relax = (float)p->hot_relax;
+    ha.max_sigma = p->hot_max_sigma;
+    ha.psf_k1 = (double)(p1 / ksum);
+    ha.psf_k2 = (double)(p2 / ksum);
+    ha.thres = p->hot_neighbour_thres;
+    ha.cut_lo_default = (double)(float)p->hot_cutoff_lower;
+    ha.cut_hi_default = (double)(float)p->hot_cutoff_upper;
+    ha.fixed_cutoffs = p->hot_fixed_cutoffs ? 1 : 0;
+    ha.mu_fix = p->hot_mu_sigma[0];
+    ha.sigma_fix = p->hot_mu_sigma[1];
+    ha.maxbins = 65536;
+    if (!(ha.max_sigma > 0.0) || !(p->hot_logistic_k > 0.0)) {
+      free_movie(mv);
+      return fail(ZR_ERR_ARG, "hotpixInfo: maxSigma and logisticK must be positive");
+    }
+    const size_t hw = (size_t)mv->H * mv->W, nblk = (hw + 255) / 256;
+    const size_t bytes = zr_hot_bytes(mv->H, mv->W, ha.maxbins);
+    e = hipMalloc(&mv->hot_block, bytes);
+    if (e != hipSuccess) { free_movie(mv); return fail(ZR_ERR_NOMEM, "hipMalloc of %zu bytes failed", bytes); }
+    g_alloc_bytes += bytes;
+    char* q = (char*)mv->hot_block;
+    auto take = [&](size_t n) { char* r = q; q += (n + 255) & ~(size_t)255; return (void*)r; };
+    HotBuffers& hb = mv->hb;
+    hb.usum = (float*)take(hw * 4);
+    hb.lmask = (float*)take(hw * 4);
+    hb.nmask = (float*)take(hw * 4);
+    hb.sp = (float*)take(hw * 4);
+    hb.nf = (float*)take(hw * 4);
+    hb.part = (double*)take(nblk * 8);
+    hb.hist = (unsigned*)take((size_t)ha.maxbins * 4);
+    hb.fitwork = (float*)take((size_t)5 * ha.maxbins * 4);
+    hb.state = (HotState*)take(sizeof(HotState));
+    if ((size_t)(q - (char*)mv->hot_block) > bytes) { free_movie(mv); return fail(ZR_ERR_ARG, "hot buffer layout"); }
+  }
   if (zr_k2_cols_r2_ok(mv->Lper, mv->m) && mv->Hc == 64 * mv->Lper) {
     if ((rc = dev_alloc(&mv->k2ltw, (size_t)(mv->Lper + 1) * 64))) { free_movie(mv); return rc; }
     e = zr_launch_k2_lane_twiddles(mv->Lper, mv->Hc, mv->k2ltw, mv->own_stream);
@@ -635,6 +683,13 @@ static int xcorr_part(zr_movie* mv, int part) {
   const bool refine = mv->p.tri_mode == ZR_TRI_REFINE;
   if (part == 1) return xcorr_pairs(mv, nc2, refine);
   if (part == 2) return xcorr_peaks(mv);
+  if (mv->p.do_hot) {
+    // hotpixFilter before the spectra, after binning (zorro/zorro.py:1591-1596)
+    hipEvent_t th = mv->timer.begin(s);
+    ZR_HIP(zr_hot_filter(const_cast<float*>(mv->frames), mv->N, mv->H, mv->W, mv->ha, mv->hb, s));
+    mv->timer.end(ZR_STAGE_INPUT, th, s);
+    mv->hot_ran = true;
+  }
   hipEvent_t t0 = mv->timer.begin(s);
   const int full = mv->spectral ? 1 : 0;
   const size_t slot = (size_t)mv->Wk * mv->Hs;
@@ -872,6 +927,24 @@ extern "C" int zr_align(zr_movie* mv) {
     ZR_HIP(hipGraphLaunch(mv->graph_exec[seg], mv->stream));
     mv->timer.end(seg_stage[seg], t, mv->stream);
   }
+  return ZR_OK;
+}
+
+extern "C" int zr_get_hotpix_info(zr_movie* mv, double* info10) {
+  if (!mv || !info10) return fail(ZR_ERR_ARG, "NULL argument");
+  if (!mv->p.do_hot || !mv->hot_ran) return fail(ZR_ERR_STATE, "the outlier-pixel filter has not run");
+  ZR_HIP(hipSetDevice(mv->device));
+  HotState h{};
+  ZR_HIP(hipMemcpyAsync(&h, mv->hb.state, sizeof(h), hipMemcpyDeviceToHost, mv->stream));
+  ZR_HIP(hipStreamSynchronize(mv->stream));
+  if (h.status != 0)
+    return fail(ZR_ERR_UNSUPPORTED, "hotpixFilter: the histogram of the unaligned sum (mean %g) needs %g unit bins, "
+                "outside 2..%d", h.sum_mean, 2.0 * mv->p.hot_max_sigma * std::sqrt(std::max(h.sum_mean, 0.0)) + 1.0,
+                mv->ha.maxbins);
+  const double n = (double)mv->N;
+  const double v[10] = {h.mu, h.sigma, h.cut_lo, h.cut_hi, (double)h.n_dead, (double)h.n_hot, (double)h.n_nbr,
+                        h.mu / n, h.sigma / std::sqrt(n), h.objective};
+  std::memcpy(info10, v, sizeof(v));
   return ZR_OK;
 }
 

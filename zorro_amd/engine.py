@@ -27,8 +27,13 @@ def make_params(triMode="diag", diagWidth=5, diagStart=0, maxShift=100, Bmode="o
                 logisticNu=0.15, originMode="centroid", suppressOrigin=True, fouCrop=(3072, 3072),
                 shapePadded=(4096, 4096), filterMode=None, doseFiltParam=(None, 0.24499, -1.6649, 2.8141, 32, 0),
                 pixelsize=None, voltage=300.0, doLazyFRC=True, frc_box=512, frc_overlap=0.5,
-                autoMax=10, shapeBinned=None) -> L.ZrParams:
-    """ZrParams from ImageRegistrator attribute values (defaults of zorro/zorro.py:82-243)."""
+                autoMax=10, shapeBinned=None, hotpixInfo=None, neighbourThres=0.01,
+                hotMuSigma=None, hotCutoffs=None) -> L.ZrParams:
+    """ZrParams from ImageRegistrator attribute values (defaults of zorro/zorro.py:82-243).
+    ``hotpixInfo`` (defaults of zorro/zorro.py:139-141) and ``neighbourThres``
+    configure the 'hot' filter mode; ``hotMuSigma`` fixes the normal-CDF
+    parameters that the filter otherwise fits; ``hotCutoffs`` = (lower, upper)
+    skips the cut-off fits (``hotpixFilter(cutoffLower, cutoffUpper)``)."""
     p = L.ZrParams()
     if triMode not in TRI:
         raise NotImplementedError(f"triMode {triMode!r} is not built")
@@ -68,7 +73,25 @@ def make_params(triMode="diag", diagWidth=5, diagStart=0, maxShift=100, Bmode="o
     p.frc_overlap = float(frc_overlap)
     if shapeBinned is not None and np.any(shapeBinned):
         p.shape_binned[0], p.shape_binned[1] = (int(shapeBinned[0]), int(shapeBinned[1]))
+    hp = {**HOTPIX_DEFAULTS, **(hotpixInfo or {})}
+    p.do_hot = 1 if filterMode and "hot" in filterMode.lower() else 0
+    if p.do_hot and hp.get("decorrOutliers"):
+        raise NotImplementedError("hotpixInfo['decorrOutliers'] adds random noise (zorro/zorro.py:2651-2666): not built")
+    p.hot_logistic_k, p.hot_relax, p.hot_max_sigma = (float(hp[k]) for k in ("logisticK", "relax", "maxSigma"))
+    p.hot_psf_k2 = 1 if hp["psf"] == "K2" else 0
+    p.hot_cutoff_lower, p.hot_cutoff_upper = float(hp["cutoffLower"]), float(hp["cutoffUpper"])
+    p.hot_neighbour_thres = float(neighbourThres)
+    if hotCutoffs is not None:
+        p.hot_cutoff_lower, p.hot_cutoff_upper = (float(v) for v in hotCutoffs)
+        p.hot_fixed_cutoffs = 1
+    if hotMuSigma is not None:
+        p.hot_mu_sigma[0], p.hot_mu_sigma[1] = (float(v) for v in hotMuSigma)
     return p
+
+
+# hotpixInfo defaults (zorro/zorro.py:139-141)
+HOTPIX_DEFAULTS = {"logisticK": 6.0, "relax": 0.925, "maxSigma": 8.0, "psf": "K2", "cutoffLower": -4.0,
+                   "cutoffUpper": 3.25, "decorrOutliers": False}
 
 
 class MoviePlan:
@@ -217,6 +240,16 @@ class MoviePlan:
         f = np.zeros((self.n, self.hb, self.wb), np.float32)
         L.check(self.lib.zr_get_frames(self._mv, L.fptr(f)))
         return f
+
+    def hotpix_info(self):
+        """Results of the 'hot' filter: the hotpixInfo keys the reference fills
+        (zorro/zorro.py:2578-2602, :2637) plus the fitted (mu, sigma) and the
+        fit objective."""
+        v = np.zeros(10, np.float64)
+        L.check(self.lib.zr_get_hotpix_info(self._mv, v.ctypes.data_as(C.POINTER(C.c_double))))
+        return dict(mu=float(v[0]), sigma=float(v[1]), cutoffLower=float(v[2]), cutoffUpper=float(v[3]),
+                    guessDeadpix=int(v[4]), guessHotpix=int(v[5]), neighborPix=int(v[6]), frameMean=float(v[7]),
+                    frameStd=float(v[8]), objective=float(v[9]))
 
     def peak_sigmas(self):
         """Bmode='opti': Gaussian width chosen per pair, [N, N]."""

@@ -46,6 +46,10 @@ class ImageRegistrator:
         self.imageSum = None
         self.filtSum = None
         self.filterMode = None
+        # for 'hot' in filterMode (zorro/zorro.py:139-141)
+        self.hotpixInfo = {"logisticK": 6.0, "relax": 0.925, "maxSigma": 8.0, "psf": "K2", "guessHotpix": 0,
+                           "guessDeadpix": 0, "decorrOutliers": False, "cutoffLower": -4.0, "cutoffUpper": 3.25,
+                           "neighborPix": 0}
         self.doseFiltParam = [None, 0.24499, -1.6649, 2.8141, 32, 0]
         self.masks = None
         self.translations = None
@@ -92,8 +96,6 @@ class ImageRegistrator:
             raise NotImplementedError("user masks are not built (the single automatic mask is)")
         if self.doEvenOddFRC:
             raise NotImplementedError("evenOddFouRingCorr is broken in the reference (zorro/zorro.py:2099-2224)")
-        if self.filterMode and "hot" in self.filterMode.lower():
-            raise NotImplementedError("hotpixFilter is not built")
         return make_params(triMode=self.triMode, diagWidth=self.diagWidth, diagStart=self.diagStart,
                            autoMax=self.autoMax,
                            maxShift=self.maxShift, Bmode=self.Bmode, Brad=self.Brad, subPixReg=self.subPixReg,
@@ -102,7 +104,7 @@ class ImageRegistrator:
                            originMode=self.originMode, suppressOrigin=self.suppressOrigin, fouCrop=self.fouCrop,
                            shapePadded=self.shapePadded, filterMode=self.filterMode,
                            doseFiltParam=self.doseFiltParam, pixelsize=self.pixelsize, voltage=self.voltage,
-                           doLazyFRC=self.doLazyFRC, shapeBinned=self.shapeBinned)
+                           doLazyFRC=self.doLazyFRC, shapeBinned=self.shapeBinned, hotpixInfo=self.hotpixInfo)
 
     def _plan_for(self, n, h, w):
         p = self._params()
@@ -164,6 +166,14 @@ class ImageRegistrator:
         self.bench["xcorr0"] = time.time()
         plan.xcorr()
         plan.synchronize()
+        if self.filterMode and "hot" in self.filterMode.lower():
+            # hotpixFilter ran first and replaced the stack (zorro/zorro.py:1595-1596, :2647);
+            # a device-resident stack was filtered in place
+            hi = plan.hotpix_info()
+            self.hotpixInfo.update({k: hi[k] for k in ("cutoffLower", "cutoffUpper", "guessDeadpix", "guessHotpix",
+                                                       "neighborPix", "frameMean", "frameStd")})
+            if isinstance(self.images, np.ndarray):
+                self.images = plan.frames()
         self.bench["xcorr3"] = self.bench["solve0"] = time.time()
         plan.solve()
         plan.synchronize()
