@@ -82,6 +82,11 @@ typedef struct zr_params {
                                                        (hotpixFilter(cutoffLower, cutoffUpper)) */
   double hot_neighbour_thres;                       /* neighbourThres */
   double hot_mu_sigma[2];                           /* both > 0: fixed normal-CDF (mu, sigma) instead of the fit */
+  /* preShift (zorro/zorro.py:194, :1148-1163): the maxShift disc of pair (I, J)
+   * follows round(shift of (I, J-1)); unmasked square maps, Bmode conv/none,
+   * triMode tri/diag/auto/first; windows of maxShift + 8 + Gaussian radius
+   * (rounded up to 8, at most 64) around each pair's disc */
+  int pre_shift;
 } zr_params;
 
 /* Element types of raw frames (MRC modes: uint8 counts, uint16, float32). */
@@ -167,7 +172,9 @@ int zr_synchronize(zr_movie* mv);
 int zr_movie_use_graph(zr_movie* mv, int enable);
 
 /* Result read-back (host buffers, synchronising).
- * shifts f32 [N][N][2], corr/peaksig/origin f32 [N][N] (errorDict trimats). */
+ * shifts f32 [N][N][2], corr/peaksig/origin f32 [N][N] (errorDict trimats).
+ * Fails under preShift when a pair's disc held no positive correlation (the
+ * reference's argmax then leaves the disc for the first zero of the map). */
 int zr_get_trimats(zr_movie* mv, float* shifts, float* corr, float* peaksig, float* origin);
 /* The frames the registration works on, f32 [N][H][W] (after the gain product
  * and binning of zr_movie_set_frames_raw: self.images after binStack,

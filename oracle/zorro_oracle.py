@@ -70,6 +70,9 @@ PINS = {
     "dose_filter": "N=1 closed form per frequency; huge critical dose -> filtSum == |image| (scaled)",
     "bin_stack": "band-limited image -> 4 x its even samples (2x crop); constant c -> 4c; brute-force DFT",
     "orient_gain/apply_gain": "numpy rot90/flip identities on an index ramp; element-wise product",
+    "locate_peak (preShift roll)": "two-peak whole-map KAT: unrolled vs rolled disc; brute-force disc statistics",
+    "hotpix_filter/hotpix_stats": "PSF kernel sum/ratios; fit of a binned normal = (mu - 1/2, sigma); planted "
+                                  "singletons/clusters vs explicit PSF and neighbour means; cut-offs on a large stack",
 }
 
 

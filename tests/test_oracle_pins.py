@@ -496,3 +496,21 @@ def test_gain_orientation_and_product():
     raw = np.array([[[0, 1, 2, 3]] * 3], np.uint8)
     gr = np.full((3, 4), 1.5, np.float32)
     assert np.array_equal(O.apply_gain(raw, gr, 2.0), raw.astype(np.float32) * 1.5 / 2)
+
+
+def test_preshift_rolled_disc_known_answer():
+    # whole 64x64 map with a band-limited peak at lag (+14, -9) and a higher one
+    # at lag (-2, +1): the unrolled disc (radius 8) finds the near peak, a disc
+    # rolled by (+13, -10) finds the far one at its exact lag
+    n, m = 64, 8
+    yy, xx = np.meshgrid(np.arange(n) - n // 2, np.arange(n) - n // 2, indexing="ij")
+    C = 0.6 * np.exp(-((yy - 14) ** 2 + (xx + 9) ** 2) / 4.0) + 0.9 * np.exp(-((yy + 2) ** 2 + (xx - 1) ** 2) / 4.0)
+    p = O.Params(maxShift=m, Bmode=None, Brad=0, subPixReg=1)
+    s0, _, _, c0 = O.locate_peak(C, p, (n, n), roll=(0, 0))
+    assert tuple(s0) == (-2.0, 1.0)
+    s1, _, ps1, c1 = O.locate_peak(C, p, (n, n), roll=(13, -10))
+    assert tuple(s1) == (14.0, -9.0)
+    # the statistics run over the rolled disc: brute force
+    disc = ((yy - 13) ** 2 + (xx + 10) ** 2) < m * m
+    vals = C[disc]
+    assert np.isclose(ps1, (C[c1] - vals.mean()) / vals.std())

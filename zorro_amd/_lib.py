@@ -40,6 +40,7 @@ class ZrParams(C.Structure):
         ("do_hot", C.c_int), ("hot_logistic_k", C.c_double), ("hot_relax", C.c_double),
         ("hot_max_sigma", C.c_double), ("hot_psf_k2", C.c_int), ("hot_cutoff_lower", C.c_double),
         ("hot_cutoff_upper", C.c_double), ("hot_fixed_cutoffs", C.c_int), ("hot_neighbour_thres", C.c_double), ("hot_mu_sigma", C.c_double * 2),
+        ("pre_shift", C.c_int),
     ]
 
 

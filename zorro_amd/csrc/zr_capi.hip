@@ -176,6 +176,17 @@ struct zr_movie {
   void* raw_owned = nullptr;
   size_t raw_owned_bytes = 0;
   float* gain_owned = nullptr;
+  // preShift: pairs in diagonal order (J - I ascending), diagonal offsets/counts,
+  // pair index of (I, J), window centres, shifts in cropped pixels, lag-0
+  // column sums [P][Wh], and the flag of a disc without positive values
+  bool preshift = false;
+  int disc = 0;
+  std::vector<std::pair<int, int>> diags;
+  int4* dpairs = nullptr;
+  int* pidx = nullptr;
+  int2* ctr = nullptr;
+  float *pshift = nullptr, *s0 = nullptr;
+  int* pflag = nullptr;
   // outlier-pixel filter (zr_hot.hip), one device block carved into its buffers
   void* hot_block = nullptr;
   HotBuffers hb{};
@@ -213,7 +224,8 @@ static void free_movie(zr_movie* mv) {
                   mv->pairs, mv->work, mv->T, mv->S, mv->Sside, mv->Z, mv->part, mv->stats, mv->den, mv->stats_c,
                   mv->windows, mv->shifts, mv->corr, mv->peaksig, mv->origin, mv->bsigma, mv->twMrow, mv->twMcol,
                   mv->twMout, mv->chW_c, mv->chW_B, mv->chH_c, mv->chH_B, mv->chHb_c, mv->chHb_B, mv->chWb_c,
-                  mv->chWb_B, mv->binA, mv->binB, mv->raw_owned, mv->gain_owned, mv->hot_block};
+                  mv->chWb_B, mv->binA, mv->binB, mv->raw_owned, mv->gain_owned, mv->hot_block, mv->dpairs,
+                  mv->pidx, mv->ctr, mv->pshift, mv->s0, mv->pflag};
   for (void* q : ptrs)
     if (q) hipFree(q);
   zr_stage_free(mv->st);
@@ -322,6 +334,7 @@ static PeakParamsHost peak_params(const zr_movie* mv) {
   hp.ry = hp.bopti ? 28 : mv->ry;
   hp.rx = hp.bopti ? 28 : mv->rx;
   hp.subpix = mv->p.sub_pix_reg;
+  hp.disc = mv->disc;
   hp.scale_y = mv->crop ? (float)((double)mv->Hp / (double)mv->Hc) : 1.f;
   hp.scale_x = mv->crop ? (float)((double)mv->Wp / (double)mv->Wc) : 1.f;
   hp.n = mv->N;
@@ -383,6 +396,24 @@ extern "C" int zr_movie_create(int device, const zr_params* p, int n_frames, int
   mv->Wh = mv->Wc / 2 + 1;
   mv->m = p->max_shift;
   if (mv->m <= 0 || 2 * mv->m > 128 || 2 * mv->m > mv->Hc || 2 * mv->m > mv->Wc) { free_movie(mv); return fail(ZR_ERR_UNSUPPORTED, "maxShift %d: built range is 1..64 (and 2*maxShift <= cropped size)", mv->m); }
+  mv->disc = mv->m;
+  mv->preshift = p->pre_shift != 0;
+  if (mv->preshift) {
+    // the disc of each pair moves with the previous pair's shift: windows of
+    // maxShift + 8 (sub-pixel patch) + the Gaussian radius around the disc keep
+    // every filtered value the disc and the patch use free of window edges
+    if (!mv->unmasked) { free_movie(mv); return fail(ZR_ERR_UNSUPPORTED, "preShift is built for unpadded frames (no mask)"); }
+    if (p->tri_mode == ZR_TRI_REFINE) { free_movie(mv); return fail(ZR_ERR_UNSUPPORTED, "preShift with triMode 'refine' is not working in the reference (zorro/zorro.py:1150)"); }
+    if (p->b_mode == ZR_B_OPTI) { free_movie(mv); return fail(ZR_ERR_UNSUPPORTED, "preShift with Bmode 'opti' (whole-map objective) is not built"); }
+    if (mv->Hc != mv->Wc) { free_movie(mv); return fail(ZR_ERR_UNSUPPORTED, "preShift needs square correlation maps (the reference's mask does not broadcast otherwise, zorro/zorro.py:842-844)"); }
+    int rg = 0;
+    if (p->b_mode == ZR_B_CONV && p->b_rad > 0)
+      for (int ax = 0; ax < 2; ++ax)
+        rg = std::max(rg, (int)(4.0 * ((ax == 0 ? mv->Hp : mv->Wp) / (std::sqrt(2.0) * M_PI * p->b_rad)) + 0.5));
+    const int mw = ((p->max_shift + 8 + rg + 7) / 8) * 8;
+    if (mw > 64 || 2 * mw > mv->Hc) { free_movie(mv); return fail(ZR_ERR_UNSUPPORTED, "preShift needs windows of maxShift + 8 + Gaussian radius = %d <= 64 lags (maxShift %d, radius %d)", p->max_shift + 8 + rg, p->max_shift, rg); }
+    mv->m = mw;
+  }
   if (p->sub_pix_reg > 1 && (p->sub_pix_reg & (p->sub_pix_reg - 1)) != 0) { free_movie(mv); return fail(ZR_ERR_UNSUPPORTED, "subPixReg must be a power of two"); }
   if (p->sub_pix_reg > 32) { free_movie(mv); return fail(ZR_ERR_UNSUPPORTED, "subPixReg > 32 is not built"); }
   int rc;
@@ -437,6 +468,24 @@ extern "C" int zr_movie_create(int device, const zr_params* p, int n_frames, int
     }
   }
   if ((rc = build_pairs(mv))) { free_movie(mv); return rc; }
+  if (mv->preshift) {
+    std::vector<int4> dp = mv->pairs_host;
+    std::stable_sort(dp.begin(), dp.end(), [](const int4& a, const int4& b) { return a.y - a.x < b.y - b.x; });
+    for (size_t i = 0; i < dp.size();) {
+      size_t j = i;
+      while (j < dp.size() && dp[j].y - dp[j].x == dp[i].y - dp[i].x) ++j;
+      mv->diags.push_back({(int)i, (int)(j - i)});
+      i = j;
+    }
+    std::vector<int> pi((size_t)mv->N * mv->N, -1);
+    for (const int4& q : mv->pairs_host) pi[(size_t)q.x * mv->N + q.y] = q.z;
+    if ((rc = dev_upload(&mv->dpairs, dp)) || (rc = dev_upload(&mv->pidx, pi)) ||
+        (rc = dev_alloc(&mv->ctr, (size_t)mv->P)) || (rc = dev_alloc(&mv->pshift, (size_t)2 * mv->P)) ||
+        (rc = dev_alloc(&mv->s0, (size_t)mv->P * mv->Wh)) || (rc = dev_alloc(&mv->pflag, 1))) {
+      free_movie(mv);
+      return rc;
+    }
+  }
   // workspaces
   const size_t N = mv->N;
   mv->batch = (int)std::min<size_t>(N, 8);
@@ -744,17 +793,57 @@ static int xcorr_part(zr_movie* mv, int part) {
   return ZR_OK;
 }
 
+// preShift: the pairs run diagonal by diagonal (J - I = 1, 2, ...): each pair's
+// window is centred on the rounded shift of (I, J-1) found by the previous
+// diagonal, then the pair transform and the peak search run for that diagonal.
+static int xcorr_preshift(zr_movie* mv, double nc2) {
+  hipStream_t s = mv->stream;
+  ZR_HIP(zr_launch_fill_zero(mv->shifts, sizeof(float) * 2 * mv->N * mv->N, s));
+  ZR_HIP(zr_launch_fill_zero(mv->corr, sizeof(float) * mv->N * mv->N, s));
+  ZR_HIP(zr_launch_fill_zero(mv->peaksig, sizeof(float) * mv->N * mv->N, s));
+  ZR_HIP(zr_launch_fill_zero(mv->origin, sizeof(float) * mv->N * mv->N, s));
+  ZR_HIP(zr_launch_fill_zero(mv->pflag, sizeof(int), s));
+  const PeakParamsHost hp = peak_params(mv);
+  const PeakPreShift ps{mv->ctr, mv->pshift, mv->pflag};
+  for (const auto& dg : mv->diags) {
+    const int4* dp = mv->dpairs + dg.first;
+    const int cnt = dg.second;
+    ZR_HIP(zr_launch_preshift_centres(dp, cnt, mv->pidx, mv->pshift, mv->N, mv->ctr, s));
+    if (mv->k2ltw)
+      ZR_HIP(zr_launch_k2_cols_r2(mv->Lper, mv->S, mv->Sside, dp, cnt, mv->Wh, mv->Hc, mv->Wc, mv->m, 8, 1, mv->k2ltw,
+                                  mv->Z, mv->Wk, mv->Hs, mv->hoff, mv->side_in_col, mv->ctr, mv->twHc, mv->twWc,
+                                  mv->s0, s));
+    else
+      ZR_HIP(zr_launch_k2_cols(mv->Lper, mv->S, mv->Sside, dp, cnt, mv->Wh, mv->Hc, mv->Wc, mv->m, 8, 1, mv->twHc,
+                               mv->Z, mv->Wk, mv->Hs, mv->hoff, mv->side_in_col, mv->ctr, mv->twWc, mv->s0, s));
+    ZR_HIP(zr_launch_k2_rows(mv->Z, dp, cnt, mv->Wh, mv->Wc, mv->m, mv->twWc, mv->k2btab, mv->den, (float)nc2,
+                             mv->windows, dp, s));
+    ZR_HIP(zr_launch_peaks(mv->windows, dp, cnt, &hp, mv->gy, mv->gx, mv->ktab, mv->shifts, mv->corr, mv->peaksig,
+                           mv->origin, mv->bsigma, &ps, s));
+  }
+  ZR_HIP(zr_launch_origin_sums(mv->pairs, mv->P, mv->s0, mv->Wh, mv->den, nc2, mv->N, mv->origin, s));
+  return ZR_OK;
+}
+
 static int xcorr_pairs(zr_movie* mv, double nc2, bool refine) {
   hipStream_t s = mv->stream;
   hipEvent_t t1 = mv->timer.begin(s);
+  if (mv->preshift) {
+    int rc = xcorr_preshift(mv, nc2);
+    if (rc) return rc;
+    mv->timer.end(ZR_STAGE_PAIRS, t1, s);
+    return ZR_OK;
+  }
   if (mv->k2ltw)
     ZR_HIP(zr_launch_k2_cols_r2(mv->Lper, mv->S, mv->Sside, mv->work, mv->Ptot, mv->Wh, mv->Hc, mv->Wc, mv->m, 8,
-                                mv->unmasked ? 1 : 0, mv->k2ltw, mv->Z, mv->Wk, mv->Hs, mv->hoff, mv->side_in_col, s));
+                                mv->unmasked ? 1 : 0, mv->k2ltw, mv->Z, mv->Wk, mv->Hs, mv->hoff, mv->side_in_col,
+                                nullptr, nullptr, nullptr, nullptr, s));
   else
     ZR_HIP(zr_launch_k2_cols(mv->Lper, mv->S, mv->Sside, mv->work, mv->Ptot, mv->Wh, mv->Hc, mv->Wc, mv->m, 8,
-                             mv->unmasked ? 1 : 0, mv->twHc, mv->Z, mv->Wk, mv->Hs, mv->hoff, mv->side_in_col, s));
+                             mv->unmasked ? 1 : 0, mv->twHc, mv->Z, mv->Wk, mv->Hs, mv->hoff, mv->side_in_col, nullptr,
+                             nullptr, nullptr, s));
   ZR_HIP(zr_launch_k2_rows(mv->Z, mv->pairs, mv->Ptot, mv->Wh, mv->Wc, mv->m, mv->twWc, mv->k2btab, mv->den,
-                           (float)nc2, mv->windows, s));
+                           (float)nc2, mv->windows, nullptr, s));
   if (refine)
     ZR_HIP(zr_launch_refine_combine(mv->windows, mv->N, mv->m, mv->unmasked ? 0 : 1, mv->win_out0, mv->stats_c,
                                     (double)mv->H * (double)mv->W, s));
@@ -766,14 +855,16 @@ static int xcorr_pairs(zr_movie* mv, double nc2, bool refine) {
 static int xcorr_peaks(zr_movie* mv) {
   hipStream_t s = mv->stream;
   hipEvent_t t2 = mv->timer.begin(s);
-  // cleared by a kernel (not hipMemsetAsync) so that the sequence can be captured in a graph
-  ZR_HIP(zr_launch_fill_zero(mv->shifts, sizeof(float) * 2 * mv->N * mv->N, s));
-  ZR_HIP(zr_launch_fill_zero(mv->corr, sizeof(float) * mv->N * mv->N, s));
-  ZR_HIP(zr_launch_fill_zero(mv->peaksig, sizeof(float) * mv->N * mv->N, s));
-  ZR_HIP(zr_launch_fill_zero(mv->origin, sizeof(float) * mv->N * mv->N, s));
-  const PeakParamsHost hp = peak_params(mv);
-  ZR_HIP(zr_launch_peaks(mv->windows, mv->pairs, mv->P, &hp, mv->gy, mv->gx, mv->ktab, mv->shifts, mv->corr,
-                         mv->peaksig, mv->origin, mv->bsigma, s));
+  if (!mv->preshift) {     // (preShift ran its peak searches diagonal by diagonal)
+    // cleared by a kernel (not hipMemsetAsync) so that the sequence can be captured in a graph
+    ZR_HIP(zr_launch_fill_zero(mv->shifts, sizeof(float) * 2 * mv->N * mv->N, s));
+    ZR_HIP(zr_launch_fill_zero(mv->corr, sizeof(float) * mv->N * mv->N, s));
+    ZR_HIP(zr_launch_fill_zero(mv->peaksig, sizeof(float) * mv->N * mv->N, s));
+    ZR_HIP(zr_launch_fill_zero(mv->origin, sizeof(float) * mv->N * mv->N, s));
+    const PeakParamsHost hp = peak_params(mv);
+    ZR_HIP(zr_launch_peaks(mv->windows, mv->pairs, mv->P, &hp, mv->gy, mv->gx, mv->ktab, mv->shifts, mv->corr,
+                           mv->peaksig, mv->origin, mv->bsigma, nullptr, s));
+  }
   if (mv->p.tri_mode == ZR_TRI_AUTO)
     ZR_HIP(zr_launch_auto_prune(mv->shifts, mv->corr, mv->peaksig, mv->origin, mv->N, mv->p.auto_max,
                                 mv->p.peaksig_thres, s));
@@ -972,6 +1063,13 @@ extern "C" int zr_get_trimats(zr_movie* mv, float* shifts, float* corr, float* p
   if (!mv->have_xcorr) return fail(ZR_ERR_STATE, "no correlation results");
   ZR_HIP(hipSetDevice(mv->device));
   ZR_HIP(hipStreamSynchronize(mv->stream));
+  if (mv->preshift) {
+    int flag = 0;
+    ZR_HIP(hipMemcpy(&flag, mv->pflag, sizeof(int), hipMemcpyDeviceToHost));
+    if (flag)
+      return fail(ZR_ERR_UNSUPPORTED, "preShift: a pair's disc holds no positive correlation; the reference's argmax "
+                  "then takes the first zero of the whole map (zorro/zorro.py:1159-1161), which no window holds");
+  }
   size_t nn = (size_t)mv->N * mv->N;
   if (shifts) ZR_HIP(hipMemcpy(shifts, mv->shifts, nn * 2 * sizeof(float), hipMemcpyDeviceToHost));
   if (corr) ZR_HIP(hipMemcpy(corr, mv->corr, nn * sizeof(float), hipMemcpyDeviceToHost));

@@ -6,6 +6,16 @@ struct PeakParamsHost {
   int m, suppress, bconv, bopti, ry, rx, subpix;
   float scale_y, scale_x;
   int n;
+  int disc;   // maxShift disc radius (0: m)
+};
+
+// preShift inputs/outputs of the peak stage: per-pair window centres, the
+// shifts in cropped pixels for the next pair, and a flag set when a disc holds
+// no positive value
+struct PeakPreShift {
+  const int2* ctr;
+  float* pshift;
+  int* flag;
 };
 
 // LDS available to one workgroup on gfx950
@@ -41,27 +51,38 @@ hipError_t zr_launch_mnxc_combine(float* windows, const int4* pairs, int P, int 
 hipError_t zr_launch_k1_cols(int Hp, const float2* T, int Wh, int Hc, int frame0, int nb, const float2* twH,
                              float2* S, float2* Sside, int full, hipStream_t st);
 hipError_t zr_launch_k1_stats(const double* part, int nrb, int n, double* stats, hipStream_t st);
+// ctr (nullable): per-pair window centre (cy, cx), lags [c - m, c + m) (preShift);
+// s0 (nullable): per (pair, column) c(kx) Re sum_ky of the product (lag-0 sums)
 hipError_t zr_launch_k2_cols(int Lper, const float2* S, const float2* Sside, const int4* work, int npairs, int Wh,
                              int Hc, int Wc, int m, int cols_per_wave, int dc_zero, const float2* twHc, float2* Z,
-                             int Wk, int Hs, int hoff, int side_in_col, hipStream_t st);
+                             int Wk, int Hs, int hoff, int side_in_col, const int2* ctr, const float2* twWc,
+                             float* s0, hipStream_t st);
 // column pass for m <= L (L = Hc/64 in {32, 48, 64}): sums over lanes of
 // pre-twiddled values with the [L+1][64] table of zr_launch_k2_lane_twiddles
 bool zr_k2_cols_r2_ok(int Lper, int m);
 hipError_t zr_launch_k2_lane_twiddles(int Lper, int Hc, float2* ltw, hipStream_t st);
 hipError_t zr_launch_k2_cols_r2(int Lper, const float2* S, const float2* Sside, const int4* work, int npairs, int Wh,
                                 int Hc, int Wc, int m, int cols_per_wave, int dc_zero, const float2* ltw, float2* Z,
-                                int Wk, int Hs, int hoff, int side_in_col, hipStream_t st);
+                                int Wk, int Hs, int hoff, int side_in_col, const int2* ctr, const float2* twHc,
+                                const float2* twWc, float* s0, hipStream_t st);
 // btab (non-NULL, 2m a multiple of 16): the matrix-core pass with the table of
 // zr_launch_k2_twiddles; otherwise the vector-ALU pass
+// sub (nullable): run the npairs pairs sub[0..npairs) (window index .z) instead of 0..npairs
 hipError_t zr_launch_k2_rows(const float2* Z, const int4* pairs, int npairs, int Wh, int Wc, int m,
                              const float2* twWc, const float2* btab, const double* den, float nc2, float* windows,
-                             hipStream_t st);
+                             const int4* sub, hipStream_t st);
 hipError_t zr_launch_k2_twiddles(int Wh, int Wc, int m, float2* btab, hipStream_t st);
 hipError_t zr_launch_k2_den(const int4* pairs, int npairs, const double* stats, double npix, double nc2, double* den,
                             hipStream_t st);
 hipError_t zr_launch_peaks(const float* windows, const int4* pairs, int npairs, const PeakParamsHost* hp,
                            const float* gy, const float* gx, const float2* ktab, float* shifts, float* corr,
-                           float* peaksig, float* origin, float* bsigma, hipStream_t st);
+                           float* peaksig, float* origin, float* bsigma, const PeakPreShift* ps, hipStream_t st);
+// preShift: window centres of the pairs dpairs[0..cnt) from the previous pairs' shifts
+hipError_t zr_launch_preshift_centres(const int4* dpairs, int cnt, const int* pidx, const float* pshift, int N,
+                                      int2* ctr, hipStream_t st);
+// preShift: originTriMat from the lag-0 column sums
+hipError_t zr_launch_origin_sums(const int4* pairs, int npairs, const float* s0, int Wh, const double* den, double nc2,
+                                 int N, float* origin, hipStream_t st);
 // triMode 'auto': zero the trimat entries after each row's stopping pair
 hipError_t zr_launch_auto_prune(float* shifts, float* corr, float* peaksig, float* origin, int N, int auto_max,
                                 double thres, hipStream_t st);

@@ -368,12 +368,14 @@ __global__ __launch_bounds__(256) void k_refine_combine(float* __restrict__ wind
 // Loads are issued in groups so that at most K2_LOADG products are in flight,
 // which keeps the kernel at two or more waves per SIMD.
 constexpr int K2_LOADG = 8;
-template <int L, int WPB>
+template <int L, int WPB, bool PS>
 __global__ __launch_bounds__(64 * WPB, 2) void k2_cols(const float2* __restrict__ S, const float2* __restrict__ Sside,
                                                         const int4* __restrict__ work, int nwork_pairs, int Wh,
                                                         int Hc, int Wc, int m, int cols_per_wave, int dc_zero,
                                                         const float2* __restrict__ twHc, float2* __restrict__ Z,
-                                                        int Wk, int Hs, int hoff, int side_in_col) {
+                                                        int Wk, int Hs, int hoff, int side_in_col,
+                                                        const int2* __restrict__ ctr,
+                                                        const float2* __restrict__ twWc, float* __restrict__ s0) {
   constexpr int LP = 65;  // padded LDS row (conflict-free 64-bit reads per half wave)
   extern __shared__ float2 k2_lds[];
   const int lane = threadIdx.x & 63;
@@ -386,9 +388,11 @@ __global__ __launch_bounds__(64 * WPB, 2) void k2_cols(const float2* __restrict_
   const int kgroup = blk / nwork_pairs;
   const int4 wk = work[blk % nwork_pairs];          // (I, J, pair index, unused)
   const int I = wk.x, J = wk.y, p = wk.z;
+  // window centre (preShift): lags t - m + cy, and the sums modulated by e^{2 pi i kx cx/Wc}
+  const int2 cc = PS ? ctr[p] : make_int2(0, 0);
   for (int e = threadIdx.x; e < twoM * 8; e += 64 * WPB) {
     const int t = e >> 3, a0 = e & 7;
-    const int dy = t - m;
+    const int dy = t - m + cc.x;
     const long dym = ((dy % Hc) + Hc) % Hc;
     const float2 u = twHc[(a0 * dym) % Hc], v = twHc[(8 * a0 * dym) % Hc];
     T0[e] = make_float2(u.x, -u.y);
@@ -401,7 +405,7 @@ __global__ __launch_bounds__(64 * WPB, 2) void k2_cols(const float2* __restrict_
   const int span = 64 / sf;                          // lanes per part
   const int o2 = lane % span, part = lane / span;
   const int t1 = lane, t2 = 64 + o2;
-  const int r1 = (((t1 - m) % L) + L) % L, r2 = (((t2 - m) % L) + L) % L;
+  const int r1 = (((t1 - m + cc.x) % L) + L) % L, r2 = (((t2 - m + cc.x) % L) + L) % L;
   const int a1lo = part * (8 / sf), a1hi = a1lo + 8 / sf;
   const int kbase = kgroup * WPB * cols_per_wave;
   float2* my = k2_lds + (size_t)w * LH * LP;
@@ -442,6 +446,13 @@ __global__ __launch_bounds__(64 * WPB, 2) void k2_cols(const float2* __restrict_
       v[L / 2] = l0 ? nv : v[L / 2];
     }
     Dft<L, 1>::run(v);
+    float2 xmod = make_float2(1.f, 0.f);
+    if constexpr (PS) {
+      const float sr = wave_sum_f(v[0].x);    // lag-0 column sum (c(kx) is inside the products)
+      if (lane == 0) s0[(size_t)p * Wh + kx] = sr;
+      const float2 t = twWc[(int)((((long)kx * cc.y) % Wc + Wc) % Wc)];
+      xmod = make_float2(t.x, -t.y);
+    }
     float2* zdst = Z + ((size_t)p * Wh + kx) * twoM;
     // two halves of r through LDS; each output reads the half holding its r
 #pragma unroll
@@ -468,7 +479,7 @@ __global__ __launch_bounds__(64 * WPB, 2) void k2_cols(const float2* __restrict_
           acc.x += tw.x * pa.x - tw.y * pa.y;
           acc.y += tw.x * pa.y + tw.y * pa.x;
         }
-        zdst[t1] = acc;
+        zdst[t1] = PS ? c_mul(acc, xmod) : acc;
       }
       if (c2 > 0) {      // round 2: outputs 64..2m-1, `sf` partial sums each
         float2 acc = make_float2(0.f, 0.f);
@@ -494,7 +505,7 @@ __global__ __launch_bounds__(64 * WPB, 2) void k2_cols(const float2* __restrict_
           acc.x += __shfl_xor(acc.x, o, 64);
           acc.y += __shfl_xor(acc.y, o, 64);
         }
-        if (part == 0 && mine) zdst[t2] = acc;
+        if (part == 0 && mine) zdst[t2] = PS ? c_mul(acc, xmod) : acc;
       }
       __builtin_amdgcn_wave_barrier();
       __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
@@ -514,16 +525,25 @@ __global__ __launch_bounds__(64 * WPB, 2) void k2_cols(const float2* __restrict_
 // block.  The column weight c(kx) is applied to the sums (a power of two: the
 // same result as scaling every product).  Loads: the column's rows in two
 // batches (K2R2_B1 rows of both frames, then the rest), each issued at once.
+// A window centre (cy, cx) per pair (ctr != NULL, preShift) moves the lags to
+// [c - m, c + m): residue r then has lags dy_a = r + L j1(r) (the largest
+// <= cy + m - 1) and dy_a - L, so row r of the block's table becomes
+// e^{+2 pi i a dy_a/Hc} (from twHc), and the sums are modulated by
+// e^{+2 pi i kx cx/Wc}.  s0 != NULL also stores c(kx) Re sum_ky x per column
+// (the lag-0 correlation's column sums, for the origin value).
 constexpr int K2R2_RC = 8, K2R2_B1 = 32, K2R2_WPB = 4, K2R2_LP = 65, K2R2_OCC = 2;
 constexpr size_t k2r2_lds_bytes(int L) {
   return ((size_t)K2R2_WPB * 2 * K2R2_RC * K2R2_LP + (size_t)(L + 1) * 64) * sizeof(float2);
 }
-template <int L, int WPB>
+template <int L, int WPB, bool PS>
 __global__ __launch_bounds__(64 * WPB, K2R2_OCC) void k2_cols_r2(const float2* __restrict__ S, const float2* __restrict__ Sside,
                                                            const int4* __restrict__ work, int nwork_pairs, int Wh,
                                                            int Hc, int Wc, int m, int cols_per_wave, int dc_zero,
                                                            const float2* __restrict__ ltw, float2* __restrict__ Z,
-                                                           int Wk, int Hs, int hoff, int side_in_col) {
+                                                           int Wk, int Hs, int hoff, int side_in_col,
+                                                           const int2* __restrict__ ctr,
+                                                           const float2* __restrict__ twHc,
+                                                           const float2* __restrict__ twWc, float* __restrict__ s0) {
   static_assert(L % K2R2_RC == 0, "residues must split into whole chunks");
   constexpr int LP = K2R2_LP;             // padded LDS row: rows 2 dwords apart in the banks
   constexpr int NCH = L / K2R2_RC;
@@ -538,7 +558,23 @@ __global__ __launch_bounds__(64 * WPB, K2R2_OCC) void k2_cols_r2(const float2* _
   const int4 wk = work[blk % nwork_pairs];
   const int I = wk.x, J = wk.y, p = wk.z;
   const int twoM = 2 * m;
-  for (int e = threadIdx.x; e < (L + 1) * 64; e += 64 * WPB) tab[e] = ltw[e];
+  const int2 cc = PS ? ctr[p] : make_int2(0, 0);
+  auto j1of = [&](int r) -> int {           // largest j with r + L j <= cy + m - 1
+    const int t = cc.x + m - 1 - r;
+    return t >= 0 ? t / L : -((-t + L - 1) / L);
+  };
+  if constexpr (!PS) {
+    for (int e = threadIdx.x; e < (L + 1) * 64; e += 64 * WPB) tab[e] = ltw[e];
+  } else {
+    for (int e = threadIdx.x; e < (L + 1) * 64; e += 64 * WPB) {
+      const int r = e / 64, a = e % 64;
+      if (r == L) { tab[e] = ltw[e]; continue; }
+      const long dy = r + (long)L * j1of(r);
+      const long ph = (((long)a * dy) % Hc + Hc) % Hc;
+      const float2 t = twHc[ph];
+      tab[e] = make_float2(t.x, -t.y);
+    }
+  }
   __syncthreads();
   const float2 wlane = tab[L * 64 + lane];    // e^{-2 pi i lane / 64}
   // reader role: row slot (residue r0 + slot/2, kind slot%2), quarter of the lanes
@@ -591,13 +627,20 @@ __global__ __launch_bounds__(64 * WPB, K2R2_OCC) void k2_cols_r2(const float2* _
       v[L / 2] = l0 ? nv : v[L / 2];
     }
     Dft<L, 1>::run(v);
+    float2 xmod = make_float2(cw, 0.f);
+    if constexpr (PS) {
+      const float sr = wave_sum_f(v[0].x);    // lag-0 column sum: sum over lanes of Y_a[0]
+      if (lane == 0) s0[(size_t)p * Wh + kx] = sr * cw;
+      const float2 t = twWc[(int)((((long)kx * cc.y) % Wc + Wc) % Wc)];
+      xmod = make_float2(t.x * cw, -t.y * cw);
+    }
     float2* zdst = Z + ((size_t)p * Wh + kx) * twoM;
 #pragma unroll
     for (int ch = 0; ch < NCH; ++ch) {
 #pragma unroll
       for (int i = 0; i < K2R2_RC; ++i) {
         const int r = ch * K2R2_RC + i;
-        // rows of outputs outside [-m, m) are written too and never read
+        // rows of outputs outside [c - m, c + m) are written too and never read
         const float2 u = c_mul(v[r], tab[r * 64 + lane]);
         my[(2 * i) * LP + lane] = u;                     // dy = r
         my[(2 * i + 1) * LP + lane] = c_mul(u, wlane);   // dy = r - L
@@ -606,7 +649,9 @@ __global__ __launch_bounds__(64 * WPB, K2R2_OCC) void k2_cols_r2(const float2* _
       __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
       const int r = ch * K2R2_RC + (slot >> 1);
       const int kind = slot & 1;
-      const bool valid = kind == 0 ? (r <= m - 1) : (r >= L - m);
+      // lag of this slot: kind 0 the larger of the residue's two, kind 1 that minus L
+      const int dy = (PS ? r + L * j1of(r) : r) - kind * L;
+      const bool valid = dy >= cc.x - m && dy <= cc.x + m - 1;
       float2 acc = make_float2(0.f, 0.f);
       {
         const float2* row = my + slot * LP + 16 * part;
@@ -617,10 +662,7 @@ __global__ __launch_bounds__(64 * WPB, K2R2_OCC) void k2_cols_r2(const float2* _
       acc.y += __shfl_xor(acc.y, 16, 64);
       acc.x += __shfl_xor(acc.x, 32, 64);
       acc.y += __shfl_xor(acc.y, 32, 64);
-      if (valid && part == 0) {
-        const int dy = kind == 0 ? r : r - L;
-        zdst[dy + m] = c_scale(acc, cw);
-      }
+      if (valid && part == 0) zdst[dy - cc.x + m] = PS ? c_mul(acc, xmod) : c_scale(acc, cw);
       __builtin_amdgcn_wave_barrier();
       __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     }
@@ -644,11 +686,11 @@ constexpr int K2R_DX = 4, K2R_DY = 5, K2R_KC = 32;
 __global__ __launch_bounds__(1024) void k2_rows(const float2* __restrict__ Z, const int4* __restrict__ pairs, int Wh,
                                                 int Wc, int m, const float2* __restrict__ twWc,
                                                 const double* __restrict__ den, float nc2,
-                                                float* __restrict__ windows) {
+                                                float* __restrict__ windows, const int4* __restrict__ sub) {
   extern __shared__ float2 zs[];  // [K2R_KC][2m]
   const int twoM = 2 * m;
   const int ndxg = (twoM + K2R_DX - 1) / K2R_DX;
-  const int p = blockIdx.x;
+  const int p = sub ? sub[blockIdx.x].z : blockIdx.x;     // sub: a subset of the pairs (preShift diagonals)
   const int tid = threadIdx.x;
   const int gx = tid % ndxg, gy = tid / ndxg;
   const int dx0 = gx * K2R_DX, dy0 = gy * K2R_DY;
@@ -722,14 +764,14 @@ template <int NT>
 __global__ __launch_bounds__(64 * NT) void k2_rows_mfma(const float2* __restrict__ Z, int Wh,
                                                         const float2* __restrict__ Btab,
                                                         const double* __restrict__ den, float nc2,
-                                                        float* __restrict__ windows) {
+                                                        float* __restrict__ windows, const int4* __restrict__ sub) {
   constexpr int TWOM = 16 * NT, NTH = 64 * NT;
   constexpr int PER = K2M_KX * TWOM / NTH;        // = 8
   constexpr int ROW = 2 * TWOM + 4;               // floats per kx row in LDS (padded)
   __shared__ float As[K2M_KX * ROW];               // [kx][dy][c]
   __shared__ float Bs[K2M_KX * ROW];               // [kx][dx][c]
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-  const int p = blockIdx.x;
+  const int p = sub ? sub[blockIdx.x].z : blockIdx.x;     // sub: a subset of the pairs (preShift diagonals)
   const float2* zp = Z + (size_t)p * Wh * TWOM;
   const int i16 = lane & 15, kq = lane >> 4;
   zr_v4f acc[NT];
@@ -815,6 +857,8 @@ struct PeakParams {
   float scale_y, scale_x;   // Hp/Hc, Wp/Wc (1 when no crop)
   int n;             // frames (trimat side)
   int tmp_floats;    // LDS floats ahead of the filtered window
+  int disc;          // radius of the maxShift disc (= m unless preShift)
+  int preshift;      // 1: windows centred on per-pair lags ctr[p] (preShift)
 };
 
 ZR_DEV int reflect_idx(int i, int n) {
@@ -993,8 +1037,15 @@ __global__ __launch_bounds__(256) void k_peaks(const float* __restrict__ windows
                                               const float* __restrict__ gx, const float2* __restrict__ ktab,
                                               float* __restrict__ shifts, float* __restrict__ corr,
                                               float* __restrict__ peaksig, float* __restrict__ origin,
-                                              float* __restrict__ bsigma) {
+                                              float* __restrict__ bsigma, const int2* __restrict__ ctr,
+                                              float* __restrict__ pshift, int* __restrict__ pflag) {
   // LDS: [tmp: max(2m*2m, reflect-padded window, 34*16*subpix)] [Cf: 2m*2m]
+  // preShift: the window of pair p covers lags [c - m, c + m), c = ctr[p] (the
+  // rolled disc's centre, zorro/zorro.py:1148-1163); lag 0 sits at (m - cy,
+  // m - cx) when inside, and its suppression stencil is applied only when the
+  // stencil is inside too (elsewhere it cannot reach the disc through the
+  // Gaussian: the window margin is maxShift + 8 + radius).  The origin value
+  // comes from the lag-0 column sums (k_origin_sums).
   extern __shared__ float smem[];
   const int m = pp.m, n2 = 2 * pp.m, nn = n2 * n2;
   float* tmp = smem;          // padded filter input, then sub-pixel scratch
@@ -1006,15 +1057,18 @@ __global__ __launch_bounds__(256) void k_peaks(const float* __restrict__ windows
   const int tid = threadIdx.x;
   const int4 pr = pairs[blockIdx.x];
   const float* win = windows + (size_t)pr.z * nn;
-  const int cidx = m * n2 + m;   // zero lag
+  const int2 cc = pp.preshift ? ctr[pr.z] : make_int2(0, 0);
+  const int li = m - cc.x, lj = m - cc.y;       // window index of lag 0
+  const bool lag0 = li >= 1 && li <= n2 - 2 && lj >= 1 && lj <= n2 - 2;
+  const int cidx = lag0 ? li * n2 + lj : -1;   // zero lag (-1: outside, no substitution)
   if (tid == 0) {
-    float v = win[cidx];
-    origin[pr.x * pp.n + pr.y] = v;
-    if (pp.suppress) {
-      // lag (ly,lx) -> index (m+ly)*2m + (m+lx); stencil with duplicated terms
-      v = 0.125f * (win[(m + 1) * n2 + m] + win[m * n2 + m + 1] + win[(m - 1) * n2 + m] + win[(m - 1) * n2 + m] +
-                    win[(m + 1) * n2 + m + 1] + win[(m - 1) * n2 + m + 1] + win[(m - 1) * n2 + m + 1] +
-                    win[(m - 1) * n2 + m - 1]);
+    float v = lag0 ? win[cidx] : 0.f;
+    if (!pp.preshift) origin[pr.x * pp.n + pr.y] = v;
+    if (pp.suppress && lag0) {
+      // lag (ly,lx) -> index (li+ly)*2m + (lj+lx); stencil with duplicated terms
+      v = 0.125f * (win[(li + 1) * n2 + lj] + win[li * n2 + lj + 1] + win[(li - 1) * n2 + lj] +
+                    win[(li - 1) * n2 + lj] + win[(li + 1) * n2 + lj + 1] + win[(li - 1) * n2 + lj + 1] +
+                    win[(li - 1) * n2 + lj + 1] + win[(li - 1) * n2 + lj - 1]);
     }
     s_sup = v;
   }
@@ -1035,7 +1089,7 @@ __global__ __launch_bounds__(256) void k_peaks(const float* __restrict__ windows
   double s1 = 0.0, s2 = 0.0, cnt = 0.0;
   float bv = -INFINITY;
   int bi = 0x7fffffff;
-  const long m2 = (long)m * m;
+  const long m2 = (long)pp.disc * pp.disc;
   for (int e = tid; e < nn; e += 256) {
     int i = e / n2, j = e % n2;
     long yy = i - m, xx = j - m;
@@ -1076,7 +1130,10 @@ __global__ __launch_bounds__(256) void k_peaks(const float* __restrict__ windows
   const int cy = bi / n2, cx = bi % n2;
   const float psig = (float)(((double)bv - mean) / sqrt(var));
   const int I = pr.x, J = pr.y, n = pp.n;
-  float sh_y = (float)(cy - m), sh_x = (float)(cx - m), cval = bi == cidx ? sup : win[bi];
+  // preShift: with no positive value in the disc the reference's argmax leaves
+  // the disc (first zero of the whole map), which no window holds: flagged
+  if (pp.preshift && tid == 0 && !(bv > 0.f)) atomicOr(pflag, 1);
+  float sh_y = (float)(cy - m + cc.x), sh_x = (float)(cx - m + cc.y), cval = bi == cidx ? sup : win[bi];
   // sub-pixel: python-slice semantics of C[c-8:c+8]
   bool valid = false;
   int sy = cy - 8, sx = cx - 8;
@@ -1146,7 +1203,7 @@ __global__ __launch_bounds__(256) void k_peaks(const float* __restrict__ windows
       for (int k = 1; k < 4; ++k)
         if (rv[k] > v || (rv[k] == v && ri[k] < ix)) { v = rv[k]; ix = ri[k]; }
       int smy = ix / G, smx = ix % G;
-      float rpy = (float)(cy - m), rpx = (float)(cx - m);
+      float rpy = (float)(cy - m + cc.x), rpx = (float)(cx - m + cc.y);
       float sgy = (rpy > 0.f) - (rpy < 0.f), sgx = (rpx > 0.f) - (rpx < 0.f);
       sh_y = rpy + ((float)smy - (float)(G / 2) + sgy) / (float)P;
       sh_x = rpx + ((float)smx - (float)(G / 2) + sgx) / (float)P;
@@ -1158,6 +1215,41 @@ __global__ __launch_bounds__(256) void k_peaks(const float* __restrict__ windows
     shifts[((size_t)I * n + J) * 2 + 1] = sh_x * pp.scale_x;
     corr[I * n + J] = cval;
     peaksig[I * n + J] = psig;
+    if (pshift) {                             // cropped pixels, for the next pair's disc
+      pshift[2 * pr.z] = sh_y;
+      pshift[2 * pr.z + 1] = sh_x;
+    }
+  }
+}
+
+// preShift (zorro/zorro.py:1148-1153): the disc of pair (I, J) is centred on
+// round(shiftsTriMat[I, J-1]) in cropped pixels (banker's rounding, np.round);
+// a pair whose previous one is the diagonal or was not correlated uses 0.
+__global__ __launch_bounds__(64) void k_preshift_centres(const int4* __restrict__ dpairs, int cnt,
+                                                         const int* __restrict__ pidx, const float* __restrict__ pshift,
+                                                         int N, int2* __restrict__ ctr) {
+  for (int i = blockIdx.x * 64 + threadIdx.x; i < cnt; i += gridDim.x * 64) {
+    const int4 pr = dpairs[i];
+    const int q = pr.y - 1 > pr.x ? pidx[pr.x * N + pr.y - 1] : -1;
+    ctr[pr.z] = q >= 0 ? make_int2((int)rintf(pshift[2 * q]), (int)rintf(pshift[2 * q + 1])) : make_int2(0, 0);
+  }
+}
+
+// originTriMat under preShift: C at lag 0 = nc2 sum_kx s0[p][kx] / max(1, D_p)
+// (the column sums of the pair transform), one block per pair, fixed order.
+__global__ __launch_bounds__(256) void k_origin_sums(const int4* __restrict__ pairs, const float* __restrict__ s0,
+                                                     int Wh, const double* __restrict__ den, double nc2, int N,
+                                                     float* __restrict__ origin) {
+  __shared__ double red[4];
+  const int4 pr = pairs[blockIdx.x];
+  double a = 0.0;
+  for (int k = threadIdx.x; k < Wh; k += 256) a += (double)s0[(size_t)pr.z * Wh + k];
+  a = wave_sum_d(a);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = a;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const double t = (red[0] + red[1]) + (red[2] + red[3]);
+    origin[pr.x * N + pr.y] = (float)(nc2 * t / fmax(1.0, den[pr.z]));
   }
 }
 
@@ -1265,14 +1357,19 @@ hipError_t zr_launch_k1_stats(const double* part, int nrb, int n, double* stats,
 static constexpr int K2C_WPB = 2;
 hipError_t zr_launch_k2_cols(int Lper, const float2* S, const float2* Sside, const int4* work, int npairs, int Wh,
                              int Hc, int Wc, int m, int cols_per_wave, int dc_zero, const float2* twHc, float2* Z,
-                             int Wk, int Hs, int hoff, int side_in_col, hipStream_t st) {
+                             int Wk, int Hs, int hoff, int side_in_col, const int2* ctr, const float2* twWc,
+                             float* s0, hipStream_t st) {
   int cols_per_block = K2C_WPB * cols_per_wave;
   int ngroups = (Wh + cols_per_block - 1) / cols_per_block;
   int grid = ngroups * npairs;
   switch (Lper) {
-#define ZR_CASE(X) case X: { size_t sh = ((size_t)K2C_WPB * (X / 2) * 65 + (size_t)2 * 2 * m * 8) * sizeof(float2); static std::once_flag lds_once; \
-      zr_allow_full_lds((const void*)k2_cols<X, K2C_WPB>, lds_once); \
-      hipLaunchKernelGGL((k2_cols<X, K2C_WPB>), dim3(grid), dim3(64 * K2C_WPB), sh, st, S, Sside, work, npairs, Wh, Hc, Wc, m, cols_per_wave, dc_zero, twHc, Z, Wk, Hs, hoff, side_in_col); return hipGetLastError(); }
+#define ZR_CASE(X) case X: { size_t sh = ((size_t)K2C_WPB * (X / 2) * 65 + (size_t)2 * 2 * m * 8) * sizeof(float2); \
+      static std::once_flag lds_once, lds_once_ps; \
+      if (ctr) { zr_allow_full_lds((const void*)k2_cols<X, K2C_WPB, true>, lds_once_ps); \
+        hipLaunchKernelGGL((k2_cols<X, K2C_WPB, true>), dim3(grid), dim3(64 * K2C_WPB), sh, st, S, Sside, work, npairs, Wh, Hc, Wc, m, cols_per_wave, dc_zero, twHc, Z, Wk, Hs, hoff, side_in_col, ctr, twWc, s0); } \
+      else { zr_allow_full_lds((const void*)k2_cols<X, K2C_WPB, false>, lds_once); \
+        hipLaunchKernelGGL((k2_cols<X, K2C_WPB, false>), dim3(grid), dim3(64 * K2C_WPB), sh, st, S, Sside, work, npairs, Wh, Hc, Wc, m, cols_per_wave, dc_zero, twHc, Z, Wk, Hs, hoff, side_in_col, ctr, twWc, s0); } \
+      return hipGetLastError(); }
     ZR_CASE(4) ZR_CASE(8) ZR_CASE(16) ZR_CASE(32) ZR_CASE(48) ZR_CASE(64)
 #undef ZR_CASE
     default: return hipErrorInvalidValue;
@@ -1289,16 +1386,20 @@ hipError_t zr_launch_k2_lane_twiddles(int Lper, int Hc, float2* ltw, hipStream_t
 
 hipError_t zr_launch_k2_cols_r2(int Lper, const float2* S, const float2* Sside, const int4* work, int npairs, int Wh,
                                 int Hc, int Wc, int m, int cols_per_wave, int dc_zero, const float2* ltw, float2* Z,
-                                int Wk, int Hs, int hoff, int side_in_col, hipStream_t st) {
+                                int Wk, int Hs, int hoff, int side_in_col, const int2* ctr, const float2* twHc,
+                                const float2* twWc, float* s0, hipStream_t st) {
   if (!zr_k2_cols_r2_ok(Lper, m) || Hc != 64 * Lper) return hipErrorInvalidValue;
   const int cols_per_block = K2R2_WPB * cols_per_wave;
   const int ngroups = (Wh + cols_per_block - 1) / cols_per_block;
   const int grid = ngroups * npairs;
   const size_t sh = k2r2_lds_bytes(Lper);
   switch (Lper) {
-#define ZR_CASE(X) case X: { static std::once_flag lds_once; \
-      zr_allow_full_lds((const void*)k2_cols_r2<X, K2R2_WPB>, lds_once); \
-      hipLaunchKernelGGL((k2_cols_r2<X, K2R2_WPB>), dim3(grid), dim3(64 * K2R2_WPB), sh, st, S, Sside, work, npairs, Wh, Hc, Wc, m, cols_per_wave, dc_zero, ltw, Z, Wk, Hs, hoff, side_in_col); return hipGetLastError(); }
+#define ZR_CASE(X) case X: { static std::once_flag lds_once, lds_once_ps; \
+      if (ctr) { zr_allow_full_lds((const void*)k2_cols_r2<X, K2R2_WPB, true>, lds_once_ps); \
+        hipLaunchKernelGGL((k2_cols_r2<X, K2R2_WPB, true>), dim3(grid), dim3(64 * K2R2_WPB), sh, st, S, Sside, work, npairs, Wh, Hc, Wc, m, cols_per_wave, dc_zero, ltw, Z, Wk, Hs, hoff, side_in_col, ctr, twHc, twWc, s0); } \
+      else { zr_allow_full_lds((const void*)k2_cols_r2<X, K2R2_WPB, false>, lds_once); \
+        hipLaunchKernelGGL((k2_cols_r2<X, K2R2_WPB, false>), dim3(grid), dim3(64 * K2R2_WPB), sh, st, S, Sside, work, npairs, Wh, Hc, Wc, m, cols_per_wave, dc_zero, ltw, Z, Wk, Hs, hoff, side_in_col, ctr, twHc, twWc, s0); } \
+      return hipGetLastError(); }
     ZR_CASE(32) ZR_CASE(48) ZR_CASE(64)
 #undef ZR_CASE
     default: return hipErrorInvalidValue;
@@ -1307,11 +1408,11 @@ hipError_t zr_launch_k2_cols_r2(int Lper, const float2* S, const float2* Sside, 
 
 hipError_t zr_launch_k2_rows(const float2* Z, const int4* pairs, int npairs, int Wh, int Wc, int m,
                              const float2* twWc, const float2* btab, const double* den, float nc2, float* windows,
-                             hipStream_t st) {
+                             const int4* sub, hipStream_t st) {
   int twoM = 2 * m;
   if (btab && twoM % 16 == 0 && twoM <= 128) {
     switch (twoM / 16) {
-#define ZR_MF(T) case T: hipLaunchKernelGGL(k2_rows_mfma<T>, dim3(npairs), dim3(64 * T), 0, st, Z, Wh, btab, den, nc2, windows); return hipGetLastError();
+#define ZR_MF(T) case T: hipLaunchKernelGGL(k2_rows_mfma<T>, dim3(npairs), dim3(64 * T), 0, st, Z, Wh, btab, den, nc2, windows, sub); return hipGetLastError();
       ZR_MF(1) ZR_MF(2) ZR_MF(3) ZR_MF(4) ZR_MF(5) ZR_MF(6) ZR_MF(7) ZR_MF(8)
 #undef ZR_MF
       default: break;
@@ -1322,7 +1423,7 @@ hipError_t zr_launch_k2_rows(const float2* Z, const int4* pairs, int npairs, int
   if (nt > 1024) return hipErrorInvalidValue;
   nt = ((nt + 63) / 64) * 64;
   size_t sh = (size_t)K2R_KC * twoM * sizeof(float2);
-  hipLaunchKernelGGL(k2_rows, dim3(npairs), dim3(nt), sh, st, Z, pairs, Wh, Wc, m, twWc, den, nc2, windows);
+  hipLaunchKernelGGL(k2_rows, dim3(npairs), dim3(nt), sh, st, Z, pairs, Wh, Wc, m, twWc, den, nc2, windows, sub);
   return hipGetLastError();
 }
 
@@ -1335,17 +1436,32 @@ hipError_t zr_launch_k2_den(const int4* pairs, int npairs, const double* stats, 
 
 hipError_t zr_launch_peaks(const float* windows, const int4* pairs, int npairs, const PeakParamsHost* hp,
                            const float* gy, const float* gx, const float2* ktab, float* shifts, float* corr,
-                           float* peaksig, float* origin, float* bsigma, hipStream_t st) {
+                           float* peaksig, float* origin, float* bsigma, const PeakPreShift* ps, hipStream_t st) {
   PeakParams pp;
   pp.m = hp->m; pp.suppress = hp->suppress; pp.bconv = hp->bconv; pp.bopti = hp->bopti; pp.ry = hp->ry;
   pp.rx = hp->rx;
   pp.subpix = hp->subpix; pp.scale_y = hp->scale_y; pp.scale_x = hp->scale_x; pp.n = hp->n;
+  pp.disc = hp->disc > 0 ? hp->disc : hp->m;
+  pp.preshift = ps ? 1 : 0;
   const size_t sh = zr_peaks_lds_bytes(hp, &pp.tmp_floats);
   if (sh > (size_t)ZR_LDS_BYTES) return hipErrorInvalidConfiguration;
   static std::once_flag lds_once;
   zr_allow_full_lds((const void*)k_peaks, lds_once);
   hipLaunchKernelGGL(k_peaks, dim3(npairs), dim3(256), sh, st, windows, pairs, pp, gy, gx, ktab, shifts, corr,
-                     peaksig, origin, bsigma);
+                     peaksig, origin, bsigma, ps ? ps->ctr : nullptr, ps ? ps->pshift : nullptr,
+                     ps ? ps->flag : nullptr);
+  return hipGetLastError();
+}
+
+hipError_t zr_launch_preshift_centres(const int4* dpairs, int cnt, const int* pidx, const float* pshift, int N,
+                                      int2* ctr, hipStream_t st) {
+  hipLaunchKernelGGL(k_preshift_centres, dim3((cnt + 63) / 64), dim3(64), 0, st, dpairs, cnt, pidx, pshift, N, ctr);
+  return hipGetLastError();
+}
+
+hipError_t zr_launch_origin_sums(const int4* pairs, int npairs, const float* s0, int Wh, const double* den, double nc2,
+                                 int N, float* origin, hipStream_t st) {
+  hipLaunchKernelGGL(k_origin_sums, dim3(npairs), dim3(256), 0, st, pairs, s0, Wh, den, nc2, N, origin);
   return hipGetLastError();
 }
 

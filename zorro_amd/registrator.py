@@ -90,8 +90,6 @@ class ImageRegistrator:
     def _params(self):
         if self.xcorrMode.lower() != "zorro":
             raise NotImplementedError(f"xcorrMode {self.xcorrMode!r}: only 'zorro' runs on the GPU")
-        if self.preShift:
-            raise NotImplementedError("preShift is not built")
         if self.masks is not None:
             raise NotImplementedError("user masks are not built (the single automatic mask is)")
         if self.doEvenOddFRC:
@@ -104,7 +102,8 @@ class ImageRegistrator:
                            originMode=self.originMode, suppressOrigin=self.suppressOrigin, fouCrop=self.fouCrop,
                            shapePadded=self.shapePadded, filterMode=self.filterMode,
                            doseFiltParam=self.doseFiltParam, pixelsize=self.pixelsize, voltage=self.voltage,
-                           doLazyFRC=self.doLazyFRC, shapeBinned=self.shapeBinned, hotpixInfo=self.hotpixInfo)
+                           doLazyFRC=self.doLazyFRC, shapeBinned=self.shapeBinned, hotpixInfo=self.hotpixInfo,
+                           preShift=self.preShift)
 
     def _plan_for(self, n, h, w):
         p = self._params()
