@@ -152,6 +152,12 @@ int zr_apply_shifts(zr_movie* mv, float* aligned_out);
  * Replaces tiledFRC + rotmean (zorro/zorro.py:2246-2310, zorro/zorro_util.py:147-224). */
 int zr_frc(zr_movie* mv);
 
+/* Crop limits and tile counts of the next zr_frc from caller translations
+ * f64 [n][2] instead of the movie's own: tiledFRC(Image1, Image2, trans=...)
+ * (zorro/zorro.py:2246-2274), as evenOddFouRingCorr passes the even half's
+ * translations (:2213-2214).  Call after zr_solve. */
+int zr_set_frc_translations(zr_movie* mv, const double* trans, int n);
+
 /* Unblur-style dose-filtered sum (normalize=False).
  * Replaces doseFilter (zorro/zorro.py:2349-2494). */
 int zr_dose_filter(zr_movie* mv);

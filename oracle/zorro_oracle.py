@@ -51,6 +51,7 @@ __all__ = [
     "hann_apodization", "frc_crop_limits", "tiled_frc", "lazy_frc",
     "dose_filter", "orient_gain", "apply_gain", "bin_stack", "align_image_stack", "PINS",
     "HOTPIX_DEFAULTS", "hotpix_psf_kernel", "hotpix_norm_objective", "hotpix_stats", "hotpix_filter",
+    "even_odd_frc",
 ]
 
 # Which oracle function is pinned by what (mirrored in tests/test_oracle_pins.py).
@@ -982,7 +983,36 @@ def hotpix_filter(images: np.ndarray, info: dict | None = None, neighbourThres: 
 # Orchestration
 # ---------------------------------------------------------------------------
 
-def align_image_stack(images: np.ndarray, p: Params):
+def even_odd_frc(images: np.ndarray, p: Params, box: int = 512, overlap: float = 0.5, edges: int = 64):
+    """``evenOddFouRingCorr`` made runnable (``zorro/zorro.py:2099-2224``; the
+    reference recurses through its saved configuration, Q12): even and odd
+    frames registered on their own under ``util.edge_mask`` (rows/columns
+    ``edges .. size - edges`` inclusive, ``zorro/zorro_util.py:123-134``), their
+    sums aligned with triMode 'first', then ``tiledFRC`` of the two aligned sums
+    with ``trans = hstack([transEven, transOdd])`` (its first two columns: the
+    even half's translations).  Returns (FRC, transEven, transOdd)."""
+    imgs = np.asarray(images, np.float64)
+    n, h, w = imgs.shape
+    ph = Params(**{**p.__dict__, "filterMode": None, "doLazyFRC": False})
+    em = np.zeros((1,) + tuple(int(v) for v in p.shapePadded), bool)
+    em[0, edges:h - edges + 1, edges:w - edges + 1] = True
+
+    def half(frames):
+        r = align_image_stack(frames, ph, mask=em)
+        return r["translations"], r["imageSum"]
+
+    te, se = half(imgs[0::2])
+    to, so = half(imgs[1::2])
+    pe = Params(**{**ph.__dict__, "triMode": "first"})
+    r = align_image_stack(np.stack([se, so]), pe)
+    aligned = r["aligned"]
+    # (hstack needs halves of equal length; only its first two columns are read)
+    trans = np.hstack([te, to]) if len(te) == len(to) else te
+    frc, _, _ = tiled_frc(aligned[0], aligned[1], trans, box, overlap)
+    return frc, te, to
+
+
+def align_image_stack(images: np.ndarray, p: Params, mask: np.ndarray | None = None):
     """``alignImageStack`` for ``xcorrMode='zorro'``.  ``zorro/zorro.py:1569-1716``.
     With ``shapeBinned`` set the stack is Fourier-binned first and the pixel
     size scaled by the mean bin factor (``:1591-1592``, ``:1835-1836``)."""
@@ -999,8 +1029,8 @@ def align_image_stack(images: np.ndarray, p: Params):
         filt, res_hot = hotpix_filter(images, info)
         images = filt.astype(np.float64)
     n, h, w = images.shape
-    padded, mask = pad_stack(images, p.shapePadded)
-    xc = xcorrnm2_tri(padded, mask, p)
+    padded, fmask = pad_stack(images, p.shapePadded)
+    xc = xcorrnm2_tri(padded, fmask if mask is None else mask, p)
     if p.triMode in ("first", "refine"):                                 # :1626-1647, no solver
         st = xc["shiftsTriMat"]
         t = -(st[0, :] if p.triMode == "first" else np.stack([st[k, k] for k in range(n)])).astype(np.float64)

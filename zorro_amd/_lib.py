@@ -77,6 +77,7 @@ SIGNATURES = {
     "zr_get_windows": (C.c_int, [_P, _F]),
     "zr_get_peak_sigmas": (C.c_int, [_P, _F]),
     "zr_get_hotpix_info": (C.c_int, [_P, _D]),
+    "zr_set_frc_translations": (C.c_int, [_P, _D, C.c_int]),
     "zr_get_frames": (C.c_int, [_P, _F]),
     "zr_pair_count": (C.c_int, [_P, _I]),
     "zr_equation_count": (C.c_int, [_P, _I]),

@@ -938,6 +938,13 @@ extern "C" int zr_frc(zr_movie* mv) {
   return rc ? fail(rc, "%s", zr_stage_error()) : ZR_OK;
 }
 
+extern "C" int zr_set_frc_translations(zr_movie* mv, const double* trans, int n) {
+  if (!mv || !trans || n < 1) return fail(ZR_ERR_ARG, "NULL argument");
+  ZR_HIP(hipSetDevice(mv->device));
+  int rc = zr_stage_set_frc_crop(mv->st, trans, n, mv->stream);
+  return rc ? fail(rc, "%s", zr_stage_error()) : ZR_OK;
+}
+
 extern "C" int zr_dose_filter(zr_movie* mv) {
   if (!mv) return fail(ZR_ERR_ARG, "NULL movie");
   ZR_HIP(hipSetDevice(mv->device));

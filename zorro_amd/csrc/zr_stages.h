@@ -24,6 +24,8 @@ int zr_stage_copy_sums_device(zr_stage_state* s, float* image_sum, float* filt_s
 int zr_stage_solve(zr_stage_state* s, const float* shifts, const float* corr, const float* peaksig, hipStream_t st);
 int zr_stage_shifts(zr_stage_state* s, const float* frames, float* aligned_out, hipStream_t st);
 int zr_stage_frc(zr_stage_state* s, hipStream_t st);
+// crop limits / tile counts of the next zr_stage_frc from caller translations [n][2]
+int zr_stage_set_frc_crop(zr_stage_state* s, const double* trans, int n, hipStream_t st);
 // S: kx-major full-resolution half spectra [N][Wk][Hp] (used when the state was
 // created with spectral != 0; may be NULL otherwise).
 int zr_stage_dose(zr_stage_state* s, const float* frames, const float2* S, int Wk, hipStream_t st);

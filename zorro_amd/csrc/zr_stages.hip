@@ -10,6 +10,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstddef>
 #include <cstdarg>
 #include <cstdio>
 #include <string>
@@ -1538,6 +1539,9 @@ __global__ __launch_bounds__(256) void k_dose_border(const float* __restrict__ f
   }
 }
 
+// threads per column of k_dose_cols_spec
+constexpr int dose_spec_nt(int hp) { return hp >= 1024 ? hp / 4 : 256; }
+
 // Per kx column: accumulate the filtered aligned spectra of all frames, then the
 // inverse transform along y.  S is the full-resolution kx-major half spectrum.
 template <int HP, int NT>
@@ -1577,26 +1581,27 @@ __global__ __launch_bounds__(NT) void k_dose_cols_spec(const float2* __restrict_
   }
   const float2 ex = twW[(int)(((long)kx * (a.W - 1)) % a.W)];  // e^{-2 pi i kx (W-1)/W}
   const float ord = (float)a.order;
-  // frame J's column (the HBM stream) is loaded one frame ahead; the border
-  // terms py, c0, cw (one column per frame, shared by every block) come from L2
-  float2 F[PER];
-  auto load = [&](int J, float2* f_) {
+  // frame J's column (the HBM stream) and two of its border terms (py, c0: one
+  // column per frame, shared by every block, from L2) are loaded one frame
+  // ahead, so that their latencies overlap the previous frame's filtering; cw
+  // is loaded in the frame's own iteration (the register file holds no more)
+  float2 F[PER], py[PER], c0[PER];
+  auto load = [&](int J, float2* f_, float2* py_, float2* c0_) {
     const float2* col = S + ((size_t)J * Wk + kx) * HP;
-#pragma unroll
-    for (int i = 0; i < PER; ++i) f_[i] = col[tid + i * NT];
-  };
-  if (a.N > 0) load(0, F);
-  for (int J = 0; J < a.N; ++J) {
-    float2 Fn[PER];
-    if (J + 1 < a.N) load(J + 1, Fn);
-    float2 py[PER], c0[PER], cw[PER];
 #pragma unroll
     for (int i = 0; i < PER; ++i) {
       const int ky = tid + i * NT;
-      py[i] = Py[(size_t)J * a.H + ky];
-      c0[i] = C0[(size_t)J * a.H + ky];
-      cw[i] = CW[(size_t)J * a.H + ky];
+      f_[i] = col[ky];
+      py_[i] = Py[(size_t)J * a.H + ky];
+      c0_[i] = C0[(size_t)J * a.H + ky];
     }
+  };
+  if (a.N > 0) load(0, F, py, c0);
+  for (int J = 0; J < a.N; ++J) {
+    float2 Fn[PER], pyn[PER], c0n[PER], cw[PER];
+    if (J + 1 < a.N) load(J + 1, Fn, pyn, c0n);
+#pragma unroll
+    for (int i = 0; i < PER; ++i) cw[i] = CW[(size_t)J * a.H + tid + i * NT];
     const float2 px = Px[(size_t)J * Whalf + kx];
     const float2 r0 = R0[(size_t)J * Whalf + kx], rh = RH[(size_t)J * Whalf + kx];
     const float mhd = (float)(-0.5 * a.accum[J + 1]);
@@ -1612,7 +1617,7 @@ __global__ __launch_bounds__(NT) void k_dose_cols_spec(const float2* __restrict_
     }
     if (J + 1 < a.N) {
 #pragma unroll
-      for (int i = 0; i < PER; ++i) F[i] = Fn[i];
+      for (int i = 0; i < PER; ++i) { F[i] = Fn[i]; py[i] = pyn[i]; c0[i] = c0n[i]; }
     }
   }
 #pragma unroll
@@ -1940,6 +1945,36 @@ int zr_stage_shifts(zr_stage_state* s, const float* frames, float* aligned_out, 
   return ZR_OK;
 }
 
+// tiledFRC(..., trans=...) with caller translations (zorro/zorro.py:2264-2274):
+// the crop limits and tile counts that k_solution_post derives from the
+// movie's own translations, recomputed on the host from `trans` [n][2].
+int zr_stage_set_frc_crop(zr_stage_state* s, const double* trans, int n, hipStream_t st) {
+  if (!s || !trans || n < 1) return serr(ZR_ERR_ARG, "FRC translations");
+  double mny = INFINITY, mxy = -INFINITY, mnx = INFINITY, mxx = -INFINITY;
+  bool any = false;
+  for (int k = 0; k < n; ++k) {
+    const double ty = trans[2 * k], tx = trans[2 * k + 1];
+    mny = std::min(mny, ty); mxy = std::max(mxy, ty); mnx = std::min(mnx, tx); mxx = std::max(mxx, tx);
+    if (ty != 0.0 || tx != 0.0) any = true;
+  }
+  int v[6];   // tiles_y, tiles_x, crop[4] (contiguous in SolveInfo)
+  int* c = v + 2;
+  if (!any) { c[0] = c[1] = c[2] = c[3] = 5; }
+  else {
+    c[2] = (int)(-std::min(std::floor(mny), 5.0));
+    c[3] = (int)(-std::min(std::floor(mnx), 5.0));
+    c[0] = (int)std::max(std::ceil(mxy), 5.0);
+    c[1] = (int)std::max(std::ceil(mxx), 5.0);
+  }
+  const double ov = s->p.frc_overlap > 0.8 ? 0.8 : s->p.frc_overlap;
+  v[1] = std::max(0, (int)std::floor((double)(s->W - c[1] - c[3] - s->box) / s->box / (1.0 - ov)));
+  v[0] = std::max(0, (int)std::floor((double)(s->H - c[0] - c[2] - s->box) / s->box / (1.0 - ov)));
+  static_assert(offsetof(SolveInfo, crop) == offsetof(SolveInfo, tiles_y) + 2 * sizeof(int), "SolveInfo layout");
+  ZS_HIP(hipMemcpyAsync((char*)s->info + offsetof(SolveInfo, tiles_y), v, sizeof(v), hipMemcpyHostToDevice, st));
+  ZS_HIP(hipStreamSynchronize(st));
+  return ZR_OK;
+}
+
 int zr_stage_frc(zr_stage_state* s, hipStream_t st) {
   if (!s || !s->have_sums) return serr(ZR_ERR_STATE, "FRC before the half sums");
   if (!s->p.do_frc) return serr(ZR_ERR_STATE, "FRC disabled in the parameters");
@@ -2003,7 +2038,7 @@ static int dose_spectral(zr_stage_state* s, const float* frames, const float2* S
   }
   ZS_HIP(hipGetLastError());
   switch (s->Hp) {
-#define ZR_DS(X) case X: hipLaunchKernelGGL((k_dose_cols_spec<X, (X >= 1024 ? X / 4 : 256)>), dim3(s->Wp / 2 + 1), dim3(X >= 1024 ? X / 4 : 256), 0, st, S, Wk, a, s->R0, s->RH, s->C0, s->CW, s->Py, s->Px, s->twHp, s->twWp, s->G); break;
+#define ZR_DS(X) case X: hipLaunchKernelGGL((k_dose_cols_spec<X, dose_spec_nt(X)>), dim3(s->Wp / 2 + 1), dim3(dose_spec_nt(X)), 0, st, S, Wk, a, s->R0, s->RH, s->C0, s->CW, s->Py, s->Px, s->twHp, s->twWp, s->G); break;
     ZR_DS(256) ZR_DS(512) ZR_DS(1024) ZR_DS(2048) ZR_DS(3072) ZR_DS(3840) ZR_DS(4096)
 #undef ZR_DS
     default: return serr(ZR_ERR_UNSUPPORTED, "dose height %d", s->Hp);

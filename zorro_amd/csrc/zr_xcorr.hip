@@ -189,6 +189,43 @@ __global__ __launch_bounds__(2 * NTC) void k1_cols(const float2* __restrict__ T,
   if (gt == 0) Sside[(size_t)f * Wh + k] = col[h2];
 }
 
+// Column pass, one column per block: column k read from the column-pair layout
+// of T (every other float2; the sibling block of column k^1 reads the rest of
+// the same lines, from L2), then as k1_cols.
+template <int HP, int NT>
+__global__ __launch_bounds__(NT) void k1_cols1(const float2* __restrict__ T, int Wh, int Hc, int frame0,
+                                               const float2* __restrict__ twH, float2* __restrict__ S,
+                                               float2* __restrict__ Sside, int full) {
+  constexpr int PER = (HP + NT - 1) / NT;
+  __shared__ float2 buf[HP];
+  const int tid = threadIdx.x;
+  const int KP = (Wh + 1) / 2;
+  const int blk = xcd_remap(blockIdx.x, gridDim.x);
+  const int fb = blk / Wh, k = blk % Wh;
+  const float2* src = T + (((size_t)fb * KP + k / 2) * HP) * 2 + (k & 1);
+  float2 v[PER];
+#pragma unroll
+  for (int i = 0; i < PER; ++i)
+    if (HP % NT == 0 || tid + i * NT < HP) v[i] = src[2 * (tid + i * NT)];
+#pragma unroll
+  for (int i = 0; i < PER; ++i)
+    if (HP % NT == 0 || tid + i * NT < HP) buf[tid + i * NT] = v[i];
+  __syncthreads();
+  wg_fft<HP, -1, NT>(buf, twH, tid);
+  const int f = frame0 + fb;
+  if (full) {
+    float2* dst = S + ((size_t)f * Wh + k) * HP;
+#pragma unroll
+    for (int i = 0; i < PER; ++i)
+      if (HP % NT == 0 || tid + i * NT < HP) dst[tid + i * NT] = buf[tid + i * NT];
+    return;
+  }
+  float2* dst = S + ((size_t)f * Wh + k) * Hc;
+  const int h2 = Hc / 2;
+  for (int n = tid; n < Hc; n += NT) dst[n] = buf[n < h2 ? n : n + HP - Hc];
+  if (tid == 0) Sside[(size_t)f * Wh + k] = buf[h2];
+}
+
 __global__ __launch_bounds__(256) void k1_frame_stats(const double* __restrict__ part, int nrb, int n,
                                                       double* __restrict__ stats) {
   // stats[f] = (sum, sumsq); one block per frame, fixed reduction order
@@ -1324,8 +1361,17 @@ hipError_t zr_launch_k1_rows(int Wp, const float* frames, int H, int W, int Hp, 
   }
 }
 
+constexpr bool K1_COLS_PAIRED = true;   // column pairs per block (else one column per block)
 hipError_t zr_launch_k1_cols(int Hp, const float2* T, int Wh, int Hc, int frame0, int nb, const float2* twH,
                              float2* S, float2* Sside, int full, hipStream_t st) {
+  if (!K1_COLS_PAIRED) {
+    switch (Hp) {
+#define ZR_CASE(X) case X: hipLaunchKernelGGL((k1_cols1<X, (X >= 2048 ? 512 : 256)>), dim3(nb * Wh), dim3(X >= 2048 ? 512 : 256), 0, st, T, Wh, Hc, frame0, twH, S, Sside, full); return hipGetLastError();
+      ZR_CASE(256) ZR_CASE(512) ZR_CASE(1024) ZR_CASE(2048) ZR_CASE(3072) ZR_CASE(3840) ZR_CASE(4096) ZR_CASE(5760)
+#undef ZR_CASE
+      default: return hipErrorInvalidValue;
+    }
+  }
   const int grid = nb * ((Wh + 1) / 2);
   switch (Hp) {
 #define ZR_CASE(X) case X: { constexpr int NTC = X >= 2048 ? 512 : 256; static std::once_flag lds_once; \

@@ -197,6 +197,12 @@ class MoviePlan:
     def frc(self):
         L.check(self.lib.zr_frc(self._mv))
 
+    def set_frc_translations(self, trans):
+        """Crop limits of the next ``frc()`` from ``trans`` [n, 2] (tiledFRC's
+        ``trans`` argument, zorro/zorro.py:2246-2274)."""
+        t = np.ascontiguousarray(np.asarray(trans, np.float64)[:, :2])
+        L.check(self.lib.zr_set_frc_translations(self._mv, t.ctypes.data_as(C.POINTER(C.c_double)), t.shape[0]))
+
     def dose_filter(self):
         L.check(self.lib.zr_dose_filter(self._mv))
 

@@ -92,8 +92,6 @@ class ImageRegistrator:
             raise NotImplementedError(f"xcorrMode {self.xcorrMode!r}: only 'zorro' runs on the GPU")
         if self.masks is not None:
             raise NotImplementedError("user masks are not built (the single automatic mask is)")
-        if self.doEvenOddFRC:
-            raise NotImplementedError("evenOddFouRingCorr is broken in the reference (zorro/zorro.py:2099-2224)")
         return make_params(triMode=self.triMode, diagWidth=self.diagWidth, diagStart=self.diagStart,
                            autoMax=self.autoMax,
                            maxShift=self.maxShift, Bmode=self.Bmode, Brad=self.Brad, subPixReg=self.subPixReg,
@@ -218,7 +216,7 @@ class ImageRegistrator:
             plan.dose_filter()
         else:
             plan.apply_shifts()
-        if self.doLazyFRC:
+        if self.doLazyFRC and not self.doEvenOddFRC:      # even/odd FRC has priority (:1681-1684)
             self.bench["frc0"] = time.time()
             plan.frc()
         plan.synchronize()
@@ -227,9 +225,85 @@ class ImageRegistrator:
         self.translations = sol["translations"].copy()
         self.imageSum = sums["imageSum"]
         self.filtSum = sums.get("filtSum")
-        if self.doLazyFRC:
+        if self.doEvenOddFRC:
+            self.evenOddFouRingCorr()
+        elif self.doLazyFRC:
             self.FRC = plan.frc_curve()
         self.bench["total1"] = time.time()
+        return self
+
+    # ------------------------------------------------------------------
+    _HALF_KEYS = ("triMode", "diagWidth", "diagStart", "autoMax", "maxShift", "Bmode", "Brad", "subPixReg",
+                  "shiftMethod", "weightMode", "peaksigThres", "logisticK", "logisticNu", "originMode",
+                  "suppressOrigin", "fouCrop", "shapePadded", "pixelsize", "voltage", "preShift", "device")
+
+    def _half_params(self, **over):
+        kw = {k: getattr(self, k) for k in self._HALF_KEYS if k != "device"}
+        kw.update(filterMode=None, doLazyFRC=True)
+        kw.update(over)
+        return make_params(**kw)
+
+    def _register_half(self, frames, edges=64):
+        """One half of evenOddFouRingCorr (zorro/zorro.py:2125-2152): the frames
+        registered under ``util.edge_mask`` (``zorro/zorro_util.py:123-134``:
+        rows and columns ``edges .. size - edges`` inclusive) and summed after
+        the shifts.  The edge mask is a rectangle, so the masked correlation
+        equals that of the frames cropped to it (both frames of every pair move
+        by the same offset); the translations then shift the whole frames."""
+        n, h, w = frames.shape
+        crop = np.ascontiguousarray(frames[:, edges:h - edges + 1, edges:w - edges + 1])
+        pa = MoviePlan(self._half_params(), n, crop.shape[1], crop.shape[2], device=self.device)
+        try:
+            pa.set_frames(crop)
+            pa.xcorr()
+            tm = pa.trimats()
+        finally:
+            pa.close()
+        pb = MoviePlan(self._half_params(), n, h, w, device=self.device)
+        try:
+            pb.set_frames(frames)
+            pb.set_trimats(tm["shiftsTriMat"], tm["corrTriMat"], tm["peaksigTriMat"])
+            pb.solve()
+            pb.apply_shifts()
+            pb.synchronize()
+            return pb.solution()["translations"].copy(), pb.sums()["imageSum"]
+        finally:
+            pb.close()
+
+    def evenOddFouRingCorr(self, xcorr="tri", box=(512, 512), overlap=0.5):
+        """Independent even/odd FRC with the reference's steps made runnable
+        (``zorro/zorro.py:2099-2224``; the reference recurses through its saved
+        configuration and needs ``files['sum']``): the even and the odd frames
+        are registered on their own (edge mask), their sums are aligned to each
+        other (triMode 'first') and the FRC of the two aligned sums is tiled with
+        the crop limits of the even half's translations (``trans=np.hstack([
+        transEven, transOdd])``, of which tiledFRC reads the first two columns).
+        Sets transEven, transOdd and FRC; returns self."""
+        if xcorr not in ("tri", None):
+            raise NotImplementedError(f"evenOddFouRingCorr xcorr={xcorr!r}: only the zorro registration is built")
+        if self.filterMode and "hot" in self.filterMode.lower():
+            raise NotImplementedError("evenOddFouRingCorr with the 'hot' filter (each half would filter its own "
+                                      "frames before the edge-masked registration) is not built")
+        if box[0] != box[1]:
+            raise NotImplementedError("square FRC boxes only")
+        imgs = np.asarray(self.images if isinstance(self.images, np.ndarray) else self.images.cpu().numpy(),
+                          np.float32)
+        self.transEven, even_sum = self._register_half(np.ascontiguousarray(imgs[0::2]))
+        self.transOdd, odd_sum = self._register_half(np.ascontiguousarray(imgs[1::2]))
+        h, w = even_sum.shape
+        pair = np.ascontiguousarray(np.stack([even_sum, odd_sum]).astype(np.float32))
+        eo = MoviePlan(self._half_params(triMode="first", frc_box=int(box[0]), frc_overlap=float(overlap)), 2, h, w,
+                       device=self.device)
+        try:
+            eo.set_frames(pair)
+            eo.xcorr()
+            eo.solve()
+            eo.apply_shifts()
+            eo.set_frc_translations(self.transEven)
+            eo.frc()
+            self.FRC = eo.frc_curve()
+        finally:
+            eo.close()
         return self
 
     def close(self):
