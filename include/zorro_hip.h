@@ -89,8 +89,10 @@ typedef struct zr_params {
   int pre_shift;
 } zr_params;
 
-/* Element types of raw frames (MRC modes: uint8 counts, uint16, float32). */
-enum { ZR_DTYPE_F32 = 0, ZR_DTYPE_U8 = 1, ZR_DTYPE_U16 = 2 };
+/* Element types of raw frames (MRC modes 2 float32, 0 uint8 counts, 6 uint16,
+ * 101 4-bit counts packed two per byte with the first pixel in the low nibble
+ * (frame width even), 1 int16). */
+enum { ZR_DTYPE_F32 = 0, ZR_DTYPE_U8 = 1, ZR_DTYPE_U16 = 2, ZR_DTYPE_U4 = 3, ZR_DTYPE_I16 = 4 };
 
 typedef struct zr_movie zr_movie;
 
@@ -119,7 +121,8 @@ int zr_movie_set_frames(zr_movie* mv, const float* frames, int on_device);
  * to shape_binned when set.  Replaces loadData's gain product
  * (zorro/zorro.py:3416-3419, :3455-3482) and binStack('fourier') (:1826-1904).
  * Host inputs (on_device / gain_on_device = 0) are copied; the call returns
- * when they may be reused. */
+ * when they may be reused.  ZR_DTYPE_U4 frames occupy N*h*w/2 bytes and need
+ * an even w (the MRC reader of the reference, mrcz, unpacks them on the host). */
 int zr_movie_set_frames_raw(zr_movie* mv, const void* frames, int dtype, int on_device, const float* gain,
                             int gain_on_device, double gain_scalar);
 

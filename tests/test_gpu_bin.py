@@ -92,3 +92,38 @@ def test_binned_gain_movie_end_to_end():
     relf = np.linalg.norm(sums["filtSum"] - ref["filtSum"]) / np.linalg.norm(ref["filtSum"])
     print(f"[gpu bin] end to end: |dt|max {dt:.4f} px, imageSum rel {rel:.2e}, filtSum rel {relf:.2e}", flush=True)
     assert dt <= 0.02 and rel <= 1e-4 and relf <= 1e-4
+
+
+@pytest.mark.parametrize("binned", [False, True], ids=["unbinned", "binned"])
+def test_packed4_and_int16_inputs(binned):
+    """MRC mode 101 (4-bit counts, two per byte, first pixel in the low nibble)
+    and mode 1 (int16) frames read on the GPU: the gain product is bit-identical
+    to the oracle's float32 product of the unpacked samples, and the binned
+    frames equal those of the same samples given as uint8 / float32."""
+    from zorro_amd import mrc
+    n, h, w = 3, 300, 260
+    sb = (150, 130) if binned else None
+    print(f"[gpu bin] packed4/int16 binned={binned}", flush=True)
+    rng = np.random.default_rng(97)
+    c4 = rng.integers(0, 16, (n, h, w)).astype(np.uint8)
+    s16 = rng.integers(-3000, 3000, (n, h, w)).astype(np.int16)
+    gain = _gain(h, w, 98)
+    kw = dict(triMode="tri", maxShift=20, fouCrop=(0, 0), shapePadded=(256, 256) if binned else (512, 512))
+    if binned:
+        kw["shapeBinned"] = sb
+    outs = {}
+    for name, frames, extra in (("u4", mrc.pack4(c4), {"packed4": True}), ("u8", c4, {}),
+                                ("i16", s16, {}), ("f32", s16.astype(np.float32), {})):
+        plan = MoviePlan(make_params(**kw), n, h, w)
+        plan.set_frames_raw(frames, gain=gain, gain_scalar=1.5, **extra)
+        outs[name] = plan.frames()
+        plan.close()
+    assert np.array_equal(outs["u4"], outs["u8"])
+    assert np.array_equal(outs["i16"], outs["f32"])
+    if not binned:
+        assert np.array_equal(outs["u4"], O.apply_gain(c4, gain, 1.5))
+        assert np.array_equal(outs["i16"], O.apply_gain(s16, gain, 1.5))
+    plan = MoviePlan(make_params(**kw), n, h, w)
+    with pytest.raises(ValueError):
+        plan.set_frames_raw(mrc.pack4(c4)[:, :, :-1], packed4=True)
+    plan.close()

@@ -17,7 +17,7 @@ ZR_TRI_TRI, ZR_TRI_DIAG, ZR_TRI_AUTO, ZR_TRI_FIRST, ZR_TRI_REFINE = 0, 1, 2, 3, 
 ZR_B_NONE, ZR_B_CONV, ZR_B_OPTI = 0, 1, 2
 ZR_SHIFT_INTEGER, ZR_SHIFT_LANCZOS = 0, 1
 ZR_W_AUTOLOGISTIC, ZR_W_LOGISTIC, ZR_W_CORR, ZR_W_UNWEIGHTED, ZR_W_NORM = 0, 1, 2, 3, 4
-ZR_DTYPE_F32, ZR_DTYPE_U8, ZR_DTYPE_U16 = 0, 1, 2
+ZR_DTYPE_F32, ZR_DTYPE_U8, ZR_DTYPE_U16, ZR_DTYPE_U4, ZR_DTYPE_I16 = 0, 1, 2, 3, 4
 
 
 class ZrParams(C.Structure):

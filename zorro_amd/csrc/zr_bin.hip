@@ -21,11 +21,19 @@
 #include "zr_common.h"
 #include "zr_launch.h"
 
-enum { ZR_RAW_F32 = 0, ZR_RAW_U8 = 1, ZR_RAW_U16 = 2 };
+// the values of ZR_DTYPE_* in the public header
+enum { ZR_RAW_F32 = 0, ZR_RAW_U8 = 1, ZR_RAW_U16 = 2, ZR_RAW_U4 = 3, ZR_RAW_I16 = 4 };
 
+// Sample i (row-major over frames) of a raw stack.  U4: two samples per byte,
+// the even one in the low nibble (frame width even, so rows need no padding).
 ZR_DEV float raw_value(const void* raw, int dtype, size_t i) {
   if (dtype == ZR_RAW_U8) return (float)reinterpret_cast<const uint8_t*>(raw)[i];
   if (dtype == ZR_RAW_U16) return (float)reinterpret_cast<const uint16_t*>(raw)[i];
+  if (dtype == ZR_RAW_U4) {
+    const unsigned b = reinterpret_cast<const uint8_t*>(raw)[i >> 1];
+    return (float)((i & 1) ? (b >> 4) : (b & 15u));
+  }
+  if (dtype == ZR_RAW_I16) return (float)reinterpret_cast<const int16_t*>(raw)[i];
   return reinterpret_cast<const float*>(raw)[i];
 }
 

@@ -661,10 +661,13 @@ static int run_input_stage(zr_movie* mv, const void* raw, int dtype, const float
 extern "C" int zr_movie_set_frames_raw(zr_movie* mv, const void* frames, int dtype, int on_device, const float* gain,
                                        int gain_on_device, double gain_scalar) {
   if (!mv || !frames) return fail(ZR_ERR_ARG, "NULL argument");
-  if (dtype != ZR_DTYPE_F32 && dtype != ZR_DTYPE_U8 && dtype != ZR_DTYPE_U16) return fail(ZR_ERR_ARG, "dtype %d", dtype);
+  if (dtype < ZR_DTYPE_F32 || dtype > ZR_DTYPE_I16) return fail(ZR_ERR_ARG, "dtype %d", dtype);
+  if (dtype == ZR_DTYPE_U4 && (mv->Wr & 1))
+    return fail(ZR_ERR_ARG, "4-bit packed frames need an even width (w = %d)", mv->Wr);
   ZR_HIP(hipSetDevice(mv->device));
-  const size_t esz = dtype == ZR_DTYPE_F32 ? 4 : (dtype == ZR_DTYPE_U8 ? 1 : 2);
-  const size_t raw_bytes = (size_t)mv->N * mv->Hr * mv->Wr * esz;
+  const size_t npix = (size_t)mv->N * mv->Hr * mv->Wr;
+  const size_t raw_bytes = dtype == ZR_DTYPE_F32 ? npix * 4
+                         : dtype == ZR_DTYPE_U8 ? npix : dtype == ZR_DTYPE_U4 ? npix / 2 : npix * 2;
   const void* rawd = frames;
   if (!on_device) {
     if (mv->raw_owned_bytes < raw_bytes) {

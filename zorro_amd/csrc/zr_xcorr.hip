@@ -1361,7 +1361,7 @@ hipError_t zr_launch_k1_rows(int Wp, const float* frames, int H, int W, int Hp, 
   }
 }
 
-constexpr bool K1_COLS_PAIRED = true;   // column pairs per block (else one column per block)
+constexpr bool K1_COLS_PAIRED = false;  // column pairs per block (else one column per block, the faster)
 hipError_t zr_launch_k1_cols(int Hp, const float2* T, int Wh, int Hc, int frame0, int nb, const float2* twH,
                              float2* S, float2* Sside, int full, hipStream_t st) {
   if (!K1_COLS_PAIRED) {

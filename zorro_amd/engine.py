@@ -144,11 +144,14 @@ class MoviePlan:
             self._frames_ref = a
             L.check(self.lib.zr_movie_set_frames(self._mv, C.c_void_p(a.ctypes.data), 0))
 
-    def set_frames_raw(self, frames, gain=None, gain_scalar=None):
-        """Raw frames (uint8, uint16 or float32; numpy or a CUDA tensor) and an
-        optional gain reference f32 [h, w] already oriented as loadData does
-        (``zorro/zorro.py:3486-3503``); gain product and binning run on the GPU."""
-        dt = {"uint8": L.ZR_DTYPE_U8, "uint16": L.ZR_DTYPE_U16, "float32": L.ZR_DTYPE_F32}
+    def set_frames_raw(self, frames, gain=None, gain_scalar=None, packed4=False):
+        """Raw frames (uint8, uint16, int16 or float32; numpy or a CUDA tensor) and
+        an optional gain reference f32 [h, w] already oriented as loadData does
+        (``zorro/zorro.py:3486-3503``); gain product and binning run on the GPU.
+        ``packed4``: frames are uint8 [n, h, w/2] holding two 4-bit counts per
+        byte, the first pixel in the low nibble (MRC mode 101)."""
+        dt = {"uint8": L.ZR_DTYPE_U8, "uint16": L.ZR_DTYPE_U16, "float32": L.ZR_DTYPE_F32,
+              "int16": L.ZR_DTYPE_I16}
         def ptr(a, kind):
             if a is None:
                 return C.c_void_p(0), 0, None
@@ -160,9 +163,15 @@ class MoviePlan:
             return C.c_void_p(a.ctypes.data), 0, a
         name = str(frames.dtype).replace("torch.", "")
         if name not in dt:
-            raise ValueError(f"raw frames must be uint8, uint16 or float32, not {frames.dtype}")
-        if tuple(frames.shape) != (self.n, self.h, self.w):
-            raise ValueError(f"frames shape {tuple(frames.shape)} != plan {(self.n, self.h, self.w)}")
+            raise ValueError(f"raw frames must be uint8, uint16, int16 or float32, not {frames.dtype}")
+        want = (self.n, self.h, self.w)
+        code = dt[name]
+        if packed4:
+            if name != "uint8" or self.w % 2:
+                raise ValueError("4-bit packed frames are uint8 [n, h, w/2] with an even w")
+            want, code = (self.n, self.h, self.w // 2), L.ZR_DTYPE_U4
+        if tuple(frames.shape) != want:
+            raise ValueError(f"frames shape {tuple(frames.shape)} != {want}")
         if gain is not None:
             if not hasattr(gain, "data_ptr"):
                 gain = np.asarray(gain, np.float32)
@@ -171,7 +180,7 @@ class MoviePlan:
         fp, fdev, fref = ptr(frames, "frames")
         gp, gdev, gref = ptr(gain, "gain")
         self._frames_ref = (fref, gref)
-        L.check(self.lib.zr_movie_set_frames_raw(self._mv, fp, dt[name], fdev, gp, gdev,
+        L.check(self.lib.zr_movie_set_frames_raw(self._mv, fp, code, fdev, gp, gdev,
                                                  float(gain_scalar) if gain_scalar else 0.0))
 
     # ---- stages

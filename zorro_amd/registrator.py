@@ -19,10 +19,13 @@ Differences from the reference, all deliberate:
 """
 from __future__ import annotations
 
+import os
+import tempfile
 import time
 
 import numpy as np
 
+from . import mrc, zorfile
 from .engine import MoviePlan, make_params
 
 
@@ -81,6 +84,31 @@ class ImageRegistrator:
         self.diagWidth = 5
         self.autoMax = 10
         self.corrThres = None
+        self.velocityThres = None
+        # configuration, IO and results kept for .zor files (zorro/zorro.py:85-243)
+        self.METAstatus = "new"
+        self.umask = 2
+        self.fftw_effort = "FFTW_MEASURE"
+        self.cachePath = tempfile.gettempdir()
+        self.detectorPixelSize = None
+        self.trackCorrStats = False
+        self.corrStats = None
+        self.CTFProgram = None
+        self.CTFInfo = {"DefocusU": None, "DefocusV": None, "DefocusAngle": None, "CtfFigureOfMerit": None,
+                        "FinalResolution": None, "AmplitudeContrast": 0.07, "AdditionalPhaseShift": None}
+        self.files = {"config": None, "stack": None, "mask": None, "sum": None, "align": None, "figurePath": None,
+                      "xc": None, "moveRawPath": None, "original": None, "gainRef": None, "stdout": None,
+                      "automatch": None, "rejected": None, "compressor": None, "clevel": 1}
+        self.savePNG = True
+        self.saveMovie = True
+        self.doCompression = False
+        self.compress_ext = ".bz2"
+        self.plotDict = {}
+        self.transEven = None
+        self.transOdd = None
+        self.velocities = None
+        self.rotations = None
+        self.scales = None
         # device selection and the last plan (kept for re-use on same-shape movies)
         self.device = 0
         self._plan = None
@@ -137,6 +165,110 @@ class ImageRegistrator:
         if gain is not None:
             self.gain = gain
         self.images = stack
+        return self
+
+    def loadData(self, stackNameIn=None, target="stack", leading_zeros=0, useMemmap=False):
+        """MRC/MRCS import of ``loadData`` (``zorro/zorro.py:3268-3526``) for
+        ``target`` 'stack' ('align', 'images'), 'sum' ('imageSum'), 'filt'
+        ('filtSum') or 'gainRef'.  Integer stacks stay integer here (the GPU
+        input stage converts them, times the gain reference, divided by
+        ``gain``, as ``:3418-3419``, ``:3455-3467``, ``:3477-3480`` do); a
+        non-zero header pixel size replaces ``pixelsize`` (nm).  A gain
+        reference named in ``files['gainRef']`` is loaded and flipped per
+        ``gainInfo`` when a stack is loaded.  DM3/DM4/TIFF/HDF5 are not read."""
+        self.bench["loaddata0"] = time.time()
+        if stackNameIn is not None:
+            self.files[target] = stackNameIn
+        fname = self.files.get(target)
+        ext = os.path.splitext(fname)[1].lower() if fname else ""
+        if ext not in (".mrc", ".mrcs", ".mrcz", ".mrczs"):
+            raise NotImplementedError(f"loadData: {ext or 'no file'}: only MRC/MRCS stacks are read")
+        data, header = mrc.read_mrc(fname, memmap=useMemmap)
+        data = np.ascontiguousarray(data)
+        if data.dtype in (np.int8, np.float16):
+            data = data.astype(np.float32)
+        if not np.isclose(header["pixelsize"][-1], 0.0):
+            self.pixelsize = float(np.float32(header["pixelsize"][-1]))
+        if target in ("stack", "align", "images"):
+            if data.ndim != 3:
+                self.METAstatus = "error"
+                raise ValueError("loadData: stacks must be 3D data")
+            gref = self.files.get("gainRef")
+            if gref and self.gainRef is None:
+                self.loadData(gref, target="gainRef")
+            self.images = data if data.dtype != np.float64 else data.astype(np.float32)
+        elif target in ("sum", "imageSum"):
+            self.imageSum = data[0] if data.shape[0] == 1 else data
+        elif target in ("filt", "filtSum"):
+            self.filtSum = data[0] if data.shape[0] == 1 else data
+        elif target == "gainRef":
+            self.loadStack(self.images, gainRef=np.asarray(data[0] if data.shape[0] == 1 else data, np.float32))
+        else:
+            raise NotImplementedError(f"loadData target {target!r}")
+        self.bench["loaddata1"] = time.time()
+        return self
+
+    def saveData(self):
+        """MRC output of ``saveData`` (``zorro/zorro.py:3528-3642``): the aligned
+        sum (``files['sum']``, default ``sum/<stack>_zorro.mrc``), the
+        dose-filtered sum beside it (``<sum>_filt.mrc``) and, when ``saveMovie``,
+        the shifted stack as ``align/<stack>_zorro_movie.mrcs``; pixel size in nm.
+        Compression (``compressor``, ``doCompression``) is not built."""
+        if self.files.get("compressor") or self.doCompression:
+            raise NotImplementedError("compressed output (mrcz/blosc, bz2) is not built")
+        front = os.path.splitext(os.path.basename(self.files["stack"] or "stack.mrc"))[0]
+        if self.files.get("sum") is None:
+            self.files["sum"] = os.path.join("sum", f"{front}_zorro.mrc")
+        def put(path, arr):
+            d = os.path.dirname(os.path.realpath(path))
+            if d and not os.path.isdir(d):
+                os.makedirs(d)
+            a = arr.cpu().numpy() if hasattr(arr, "cpu") else np.asarray(arr)
+            mrc.write_mrc(path, a.astype(np.float32) if a.dtype.kind == "f" else a, pixelsize_nm=self.pixelsize or 0.0)
+        put(self.files["sum"], self.imageSum)
+        if self.saveMovie and self.images is not None:
+            if self.files.get("align") is None:
+                self.files["align"] = os.path.join("align", f"{front}_zorro_movie.mrcs")
+            put(self.files["align"], self.alignedImages())
+        if self.filterMode and self.filtSum is not None:
+            if self.files.get("filt") is None:
+                sp, sf = os.path.split(self.files["sum"])
+                self.files["filt"] = os.path.join(sp, f"{os.path.splitext(sf)[0]}_filt.mrc")
+            put(self.files["filt"], self.filtSum)
+        return self
+
+    def alignedImages(self):
+        """The shifted stack: after ``alignImageStack`` the reference's
+        ``self.images`` holds the frames shifted by ``applyShifts``
+        (``zorro/zorro.py:1734-1808``), which this class does not keep (the
+        shifts are fused with the sums).  Recomputed here from the last plan;
+        ``self.images`` when no alignment has run."""
+        if self._plan is None or self.translations is None or self.imageSum is None:
+            return self.images
+        import torch
+        n = int(self.translations.shape[0])
+        out = torch.empty((n,) + tuple(np.shape(self.imageSum)), dtype=torch.float32, device=f"cuda:{self.device}")
+        self._plan.apply_shifts(out)
+        self._plan.synchronize()
+        return out.cpu().numpy()
+
+    def saveConfig(self, configNameIn=None):
+        """Write the ``.zor`` file (``zorro/zorro.py:3898-4087``); default name
+        ``files['stack'] + '.zor'``."""
+        if configNameIn:
+            self.files["config"] = configNameIn
+        elif self.files.get("config") is None:
+            self.files["config"] = str(self.files["stack"]) + ".zor"
+        return zorfile.save_config(self, self.files["config"])
+
+    def loadConfig(self, configNameIn=None, loadData=False):
+        """Read a ``.zor`` file (``zorro/zorro.py:3682-3896``); ``loadData``
+        then loads ``files['stack']``."""
+        if configNameIn:
+            self.files["config"] = configNameIn
+        zorfile.load_config(self, self.files["config"])
+        if loadData and self.files.get("stack"):
+            self.loadData()
         return self
 
     def alignImageStack(self):
