@@ -74,6 +74,8 @@ def test_host_messages_success_and_failure(tmp_path):
     q.get_nowait()
     _wait_free(host)
     assert q.get_nowait() == [skulk.HOST_ERROR, sb] and sb.state == skulk.PROCESSING
+    assert isinstance(host.lastError, RuntimeError)
+    assert "alignment failed" in open(tmp_path / "bad.zout").read()
     assert host.poll() == skulk.HOST_FREE and q.empty()
     with pytest.raises(NotImplementedError):
         skulk.GpuSkulkHost("h", "qsub", q)
@@ -84,11 +86,11 @@ def test_gpu_job_matches_direct_run(tmp_path, monkeypatch):
     from zorro_amd import mrc
     from zorro_amd.synth import make_movie
     monkeypatch.chdir(tmp_path)
-    n, h, w = 8, 512, 512
+    n, h, w = 8, 1024, 1024
     print(f"[gpu skulk] job n={n} {h}x{w}", flush=True)
     frames, _ = make_movie(n, h, w, seed=71, dose=1.0)
     mrc.write_mrc("m1.mrcs", frames.astype(np.uint16), pixelsize_nm=0.1)
-    settings = dict(triMode="tri", maxShift=20, Bmode="conv", Brad=150, fouCrop=[512, 512], shapePadded=[512, 512],
+    settings = dict(triMode="tri", maxShift=20, Bmode="conv", Brad=150, fouCrop=[1024, 1024], shapePadded=[1024, 1024],
                     filterMode="dose", saveMovie=False)
     z = ImageRegistrator()
     for k, v in settings.items():
@@ -101,7 +103,9 @@ def test_gpu_job_matches_direct_run(tmp_path, monkeypatch):
     host.workerFunction(st)
     _wait_free(host, limit=300.0)
     msgs = [q.get_nowait() for _ in range(q.qsize())]
-    assert [m[0] for m in msgs] == [skulk.HOST_BUSY, skulk.FINISHED, skulk.FINISHED]
+    if os.path.exists("m1.zout"):
+        print(open("m1.zout").read(), flush=True)
+    assert [m[0] for m in msgs] == [skulk.HOST_BUSY, skulk.FINISHED, skulk.FINISHED], repr(host.lastError)
     done = ImageRegistrator().loadConfig("m1.zor")
     assert done.METAstatus == "fini" and done.files["sum"] == os.path.join("sum", "m1_zorro.mrc")
     direct = ImageRegistrator()
@@ -114,4 +118,5 @@ def test_gpu_job_matches_direct_run(tmp_path, monkeypatch):
     assert np.array_equal(s[0], direct.imageSum) and hdr["pixelsize"][-1] == pytest.approx(0.1, rel=1e-6)
     f, _ = mrc.read_mrc(done.files["filt"])
     assert np.array_equal(f[0], direct.filtSum)
+    assert np.array_equal(done.FRC, direct.FRC) and done.FRC.size > 0
     direct.close()

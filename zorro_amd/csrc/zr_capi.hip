@@ -489,7 +489,7 @@ extern "C" int zr_movie_create(int device, const zr_params* p, int n_frames, int
   // workspaces
   const size_t N = mv->N;
   mv->batch = (int)std::min<size_t>(N, 8);
-  mv->rb = (mv->Wp >= 3072) ? 4 : 8;   // rows per k1_rows block
+  mv->rb = zr_k1_rows_per_block(mv->Wp);
   const int nrb = mv->Hp / mv->rb;
   mv->spectral = p->do_dose && mv->unmasked && mv->H == mv->W && mv->Hp <= 4096;
   mv->Wk = mv->spectral ? mv->Wp / 2 + 1 : mv->Wh;

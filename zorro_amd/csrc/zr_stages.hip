@@ -1539,94 +1539,96 @@ __global__ __launch_bounds__(256) void k_dose_border(const float* __restrict__ f
   }
 }
 
-// threads per column of k_dose_cols_spec
-constexpr int dose_spec_nt(int hp) { return hp >= 1024 ? hp / 4 : 256; }
-
-// Per kx column: accumulate the filtered aligned spectra of all frames, then the
-// inverse transform along y.  S is the full-resolution kx-major half spectrum.
-template <int HP, int NT>
-__global__ __launch_bounds__(NT) void k_dose_cols_spec(const float2* __restrict__ S, int Wk, DoseArgs a,
+// Spectral dose path, step 1: the filtered sum of the aligned frame spectra at
+// every (kx, ky) of the half spectrum, kx-major into G (the column inverse
+// transforms of step 2 run in place there).  Every (kx, ky) is independent, so
+// the work is tiled finely over the chip instead of one column per block (a
+// 2049-column half spectrum over 256 CUs would leave one column for a round of
+// its own): a block covers DA_NT rows (one per thread) of DA_U consecutive
+// columns.  A row's border terms (py, c0, cw of frame J) are loaded once for
+// the block's DA_U columns; the column terms (px, r0, rh) are block-uniform.
+constexpr int DA_NT = 256, DA_U = 4;
+template <int HP>
+__global__ __launch_bounds__(DA_NT) void k_dose_accum(const float2* __restrict__ S, int Wk, DoseArgs a,
                                                       const float2* __restrict__ R0, const float2* __restrict__ RH,
                                                       const float2* __restrict__ C0, const float2* __restrict__ CW,
                                                       const float2* __restrict__ Py, const float2* __restrict__ Px,
                                                       const float2* __restrict__ twH, const float2* __restrict__ twW,
                                                       float2* __restrict__ G) {
-  constexpr int PER = HP / NT;
-  static_assert(HP % NT == 0, "column must split evenly over the block");
-  __shared__ float2 buf[HP];
-  const int tid = threadIdx.x;
-  const int kx = xcd_remap(blockIdx.x, gridDim.x);
+  static_assert(HP % DA_NT == 0, "rows must split evenly over the blocks");
+  constexpr int NRB = HP / DA_NT;
   const int Whalf = a.Wp / 2 + 1;
-  float invc[PER], q[PER];
-  float2 acc[PER], ey[PER];
+  const int lin = xcd_remap(blockIdx.x, gridDim.x);
+  const int ky = (lin % NRB) * DA_NT + threadIdx.x;
+  const int kx0 = (lin / NRB) * DA_U;
   auto crit = [&](double yy, double xx) {
     const double qq = sqrt(xx * xx * a.ipx * a.ipx + yy * yy * a.ipy * a.ipy);
     return a.vs * (a.A * pow(qq, a.B) + a.C);
   };
+  float invc[DA_U], q[DA_U];
+  float2 acc[DA_U], ex[DA_U];
+  int kxs[DA_U];
+  const double fy = ky < HP / 2 ? ky : ky - HP;
 #pragma unroll
-  for (int i = 0; i < PER; ++i) {
-    const int ky = tid + i * NT;
-    const double fy = ky < HP / 2 ? ky : ky - HP;
+  for (int u = 0; u < DA_U; ++u) {
+    const int kx = min(kx0 + u, Whalf - 1);     // columns past the end repeat the last one, unwritten
+    kxs[u] = kx;
     const double fx = kx;
     const double qq = sqrt(fx * fx * a.ipx * a.ipx + fy * fy * a.ipy * a.ipy);
     double c = crit(fy, fx);
-    if (ky == HP / 2 && 2 * kx == a.Wp) {   // Nyquist corner takes its left neighbour squared
+    if (ky == HP / 2 && 2 * kx == a.Wp) {        // Nyquist corner takes its left neighbour squared
       const double cl = crit(-HP / 2, a.Wp / 2 - 1);
       c = cl * cl;
     }
-    invc[i] = (float)(1.0 / c);
-    q[i] = (float)qq;
-    acc[i] = make_float2(0.f, 0.f);
-    ey[i] = twH[(int)(((long)ky * (a.H - 1)) % a.H)];          // e^{-2 pi i ky (H-1)/H}
+    invc[u] = (float)(1.0 / c);
+    q[u] = (float)qq;
+    acc[u] = make_float2(0.f, 0.f);
+    ex[u] = twW[(int)(((long)kx * (a.W - 1)) % a.W)];   // e^{-2 pi i kx (W-1)/W}
   }
-  const float2 ex = twW[(int)(((long)kx * (a.W - 1)) % a.W)];  // e^{-2 pi i kx (W-1)/W}
+  const float2 ey = twH[(int)(((long)ky * (a.H - 1)) % a.H)];   // e^{-2 pi i ky (H-1)/H}
   const float ord = (float)a.order;
-  // frame J's column (the HBM stream) and two of its border terms (py, c0: one
-  // column per frame, shared by every block, from L2) are loaded one frame
-  // ahead, so that their latencies overlap the previous frame's filtering; cw
-  // is loaded in the frame's own iteration (the register file holds no more)
-  float2 F[PER], py[PER], c0[PER];
-  auto load = [&](int J, float2* f_, float2* py_, float2* c0_) {
-    const float2* col = S + ((size_t)J * Wk + kx) * HP;
-#pragma unroll
-    for (int i = 0; i < PER; ++i) {
-      const int ky = tid + i * NT;
-      f_[i] = col[ky];
-      py_[i] = Py[(size_t)J * a.H + ky];
-      c0_[i] = C0[(size_t)J * a.H + ky];
-    }
-  };
-  if (a.N > 0) load(0, F, py, c0);
   for (int J = 0; J < a.N; ++J) {
-    float2 Fn[PER], pyn[PER], c0n[PER], cw[PER];
-    if (J + 1 < a.N) load(J + 1, Fn, pyn, c0n);
-#pragma unroll
-    for (int i = 0; i < PER; ++i) cw[i] = CW[(size_t)J * a.H + tid + i * NT];
-    const float2 px = Px[(size_t)J * Whalf + kx];
-    const float2 r0 = R0[(size_t)J * Whalf + kx], rh = RH[(size_t)J * Whalf + kx];
+    const float2 py = Py[(size_t)J * a.H + ky], c0 = C0[(size_t)J * a.H + ky], cw = CW[(size_t)J * a.H + ky];
     const float mhd = (float)(-0.5 * a.accum[J + 1]);
     const float itq = (float)(1.0 / a.thresq[J]);
+    float2 F[DA_U];
 #pragma unroll
-    for (int i = 0; i < PER; ++i) {
-      float2 sp = c_mul(c_mul(F[i], py[i]), px);
-      sp = c_add(sp, c_add(r0, c_mul(rh, ey[i])));
-      sp = c_add(sp, c_add(c0[i], c_mul(cw[i], ex)));
-      const float gval = expf(-cutoff_pow(q[i] * itq, ord, a.order_sq)) * expf(mhd * invc[i]);
-      acc[i].x += sp.x * gval;
-      acc[i].y += sp.y * gval;
-    }
-    if (J + 1 < a.N) {
+    for (int u = 0; u < DA_U; ++u) F[u] = S[((size_t)J * Wk + kxs[u]) * HP + ky];
 #pragma unroll
-      for (int i = 0; i < PER; ++i) { F[i] = Fn[i]; py[i] = pyn[i]; c0[i] = c0n[i]; }
+    for (int u = 0; u < DA_U; ++u) {
+      const float2 px = Px[(size_t)J * Whalf + kxs[u]];
+      const float2 r0 = R0[(size_t)J * Whalf + kxs[u]], rh = RH[(size_t)J * Whalf + kxs[u]];
+      float2 sp = c_mul(c_mul(F[u], py), px);
+      sp = c_add(sp, c_add(r0, c_mul(rh, ey)));
+      sp = c_add(sp, c_add(c0, c_mul(cw, ex[u])));
+      const float gval = expf(-cutoff_pow(q[u] * itq, ord, a.order_sq)) * expf(mhd * invc[u]);
+      acc[u].x += sp.x * gval;
+      acc[u].y += sp.y * gval;
     }
   }
 #pragma unroll
-  for (int i = 0; i < PER; ++i) buf[tid + i * NT] = acc[i];
+  for (int u = 0; u < DA_U; ++u)
+    if (kx0 + u < Whalf) G[(size_t)(kx0 + u) * HP + ky] = acc[u];
+}
+
+// Spectral dose path, step 2: inverse transform along y of every column of G,
+// in place (frames square and unpadded: H = HP).
+template <int HP>
+__global__ __launch_bounds__(256) void k_dose_cols_ifft(const float2* __restrict__ twH, float2* __restrict__ G) {
+  constexpr int NT = 256, PER = HP / NT;
+  static_assert(HP % NT == 0, "column must split evenly over the block");
+  __shared__ float2 buf[HP];
+  const int tid = threadIdx.x;
+  float2* col = G + (size_t)xcd_remap(blockIdx.x, gridDim.x) * HP;
+  float2 v[PER];
+#pragma unroll
+  for (int i = 0; i < PER; ++i) v[i] = col[tid + i * NT];
+#pragma unroll
+  for (int i = 0; i < PER; ++i) buf[tid + i * NT] = v[i];
   __syncthreads();
   wg_fft<HP, 1, NT>(buf, twH, tid);
 #pragma unroll
-  for (int i = 0; i < PER; ++i)
-    if (tid + i * NT < a.H) G[(size_t)kx * a.H + tid + i * NT] = buf[tid + i * NT];
+  for (int i = 0; i < PER; ++i) col[tid + i * NT] = buf[tid + i * NT];
 }
 
 // ==================================================================== host
@@ -2037,11 +2039,19 @@ static int dose_spectral(zr_stage_state* s, const float* frames, const float2* S
     default: return serr(ZR_ERR_UNSUPPORTED, "dose size %d", H);
   }
   ZS_HIP(hipGetLastError());
-  switch (s->Hp) {
-#define ZR_DS(X) case X: hipLaunchKernelGGL((k_dose_cols_spec<X, dose_spec_nt(X)>), dim3(s->Wp / 2 + 1), dim3(dose_spec_nt(X)), 0, st, S, Wk, a, s->R0, s->RH, s->C0, s->CW, s->Py, s->Px, s->twHp, s->twWp, s->G); break;
-    ZR_DS(256) ZR_DS(512) ZR_DS(1024) ZR_DS(2048) ZR_DS(3072) ZR_DS(3840) ZR_DS(4096)
+  if (s->Hp != H || s->Wp != W) return serr(ZR_ERR_UNSUPPORTED, "spectral dose path needs unpadded frames");
+  {
+    const int whalf = s->Wp / 2 + 1;
+    const int grid_acc = (s->Hp / DA_NT) * ((whalf + DA_U - 1) / DA_U);
+    switch (s->Hp) {
+#define ZR_DS(X) case X: \
+      hipLaunchKernelGGL(k_dose_accum<X>, dim3(grid_acc), dim3(DA_NT), 0, st, S, Wk, a, s->R0, s->RH, s->C0, s->CW, s->Py, s->Px, s->twHp, s->twWp, s->G); \
+      ZS_HIP(hipGetLastError()); \
+      hipLaunchKernelGGL(k_dose_cols_ifft<X>, dim3(whalf), dim3(256), 0, st, s->twHp, s->G); break;
+      ZR_DS(256) ZR_DS(512) ZR_DS(1024) ZR_DS(2048) ZR_DS(3072) ZR_DS(3840) ZR_DS(4096)
 #undef ZR_DS
-    default: return serr(ZR_ERR_UNSUPPORTED, "dose height %d", s->Hp);
+      default: return serr(ZR_ERR_UNSUPPORTED, "dose height %d", s->Hp);
+    }
   }
   ZS_HIP(hipGetLastError());
   switch (s->Wp) {

@@ -348,7 +348,13 @@ class ImageRegistrator:
             plan.dose_filter()
         else:
             plan.apply_shifts()
-        if self.doLazyFRC and not self.doEvenOddFRC:      # even/odd FRC has priority (:1681-1684)
+        # tiledFRC's tile count along each axis, floor((size - box) / box / (1 - overlap))
+        # (zorro/zorro.py:2266-2274): a frame that holds no 512-pixel tile gets no FRC
+        # (the reference would average over zero tiles)
+        p = plan.params
+        step = 1.0 - min(p.frc_overlap, 0.8)
+        lazy_frc = self.doLazyFRC and min(plan.hb, plan.wb) >= p.frc_box * (1.0 + step)
+        if lazy_frc and not self.doEvenOddFRC:      # even/odd FRC has priority (:1681-1684)
             self.bench["frc0"] = time.time()
             plan.frc()
         plan.synchronize()
@@ -359,8 +365,10 @@ class ImageRegistrator:
         self.filtSum = sums.get("filtSum")
         if self.doEvenOddFRC:
             self.evenOddFouRingCorr()
-        elif self.doLazyFRC:
+        elif lazy_frc:
             self.FRC = plan.frc_curve()
+        elif self.doLazyFRC:
+            self.FRC = None
         self.bench["total1"] = time.time()
         return self
 

@@ -23,6 +23,7 @@ from __future__ import annotations
 import os
 import threading
 import time
+import traceback
 from concurrent.futures import ThreadPoolExecutor
 
 # state codes of automator/zorroSkulkManager.py:37-52; the manager compares them
@@ -95,6 +96,7 @@ class GpuSkulkHost:
         self._max_inflight = max_inflight
         self._runner = runner
         self._future = None
+        self.lastError = None          # exception of the last failed job
         if workerFunction in ("local", "gpu"):
             self.workerFunction = self.submitLocalJob
         elif workerFunction == "dummy":
@@ -119,8 +121,21 @@ class GpuSkulkHost:
         z.files["stdout"] = os.path.splitext(z.files["config"])[0] + ".zout"
         z.METAstatus = STATE_NAMES[PROCESSING]
         z.saveConfig()
-        cfg = z.files["config"]
-        self._future = _pool(self.device, self._max_inflight).submit(self._runner, cfg)
+        cfg, log = z.files["config"], z.files["stdout"]
+        runner = self._runner
+
+        def job():
+            # the subprocess host appends the job's output to files['stdout']
+            # (zorro -c cfg >> stdout 2>&1); a failing job leaves its traceback there
+            try:
+                return runner(cfg)
+            except BaseException:
+                with open(log, "a") as fh:
+                    fh.write(traceback.format_exc())
+                raise
+
+        self.lastError = None
+        self._future = _pool(self.device, self._max_inflight).submit(job)
         self.messageQueue.put([HOST_BUSY, self.zorroState])
 
     def submitDummyJob(self, stateObj):
@@ -141,6 +156,7 @@ class GpuSkulkHost:
         st = self.zorroState
         failed = self._future.cancelled() or self._future.exception() is not None
         if failed:
+            self.lastError = None if self._future.cancelled() else self._future.exception()
             self.messageQueue.put([HOST_ERROR, st])
         else:
             st.incrementState()
