@@ -105,6 +105,15 @@ int zr_device_count(int* n);
 /* Plan one movie shape: tables, pair list and device workspaces.
  * Stands in for ImageRegistrator.__init_xcorrnm2 (zorro/zorro.py:540-637). */
 int zr_movie_create(int device, const zr_params* p, int n_frames, int h, int w, zr_movie** out);
+
+/* zr_movie_create with the pairs given by a boolean triIndices matrix
+ * (uint8 [n_frames][n_frames], row-major; pair (I, J) correlated when
+ * tri_indices[I][J] != 0) instead of the triMode's pattern.  Replaces
+ * xcorrnm2_tri(triIndices=...) (zorro/zorro.py:773, :552-557, :959-975); the
+ * trimats and the solver then see exactly these pairs.  Needs tri_mode
+ * ZR_TRI_TRI and no pre_shift; entries on or below the diagonal are refused. */
+int zr_movie_create_pairs(int device, const zr_params* p, int n_frames, int h, int w, const uint8_t* tri_indices,
+                          zr_movie** out);
 void zr_movie_destroy(zr_movie* mv);
 
 /* Run the movie's work on a caller-provided HIP stream (NULL: the movie's own). */

@@ -335,15 +335,22 @@ def locate_peak(C: np.ndarray, p: Params, shapePadded, roll=None):
     return shift, float(corr), float(peaksig), cmax
 
 
-def xcorrnm2_tri(images: np.ndarray, mask: np.ndarray, p: Params, return_windows: bool = False):
+def xcorrnm2_tri(images: np.ndarray, mask: np.ndarray, p: Params, return_windows: bool = False, triIndices=None):
     """All-pairs registration.  ``zorro/zorro.py:773-1011``.
 
     ``images`` are already padded to ``p.shapePadded`` and ``mask`` is [1,Hp,Wp].
+    ``triIndices``: boolean [n, n] pairs to correlate in place of the triMode's
+    pattern (``:552-557``); the rows are then walked as for 'tri' (``:959-975``).
     """
     n = images.shape[0]
     shapeP = images.shape[1:]
     shapeC = shape_cropped(p.fouCrop, shapeP)
-    tri = tri_indices(n, p.triMode, p.diagWidth, p.diagStart)
+    if triIndices is not None:
+        tri = np.asarray(triIndices).astype(bool)
+        if tri.shape != (n, n):
+            raise IndexError("triIndices is wrong size, should be of length: " + str(n))
+    else:
+        tri = tri_indices(n, p.triMode, p.diagWidth, p.diagStart)
     if p.triMode not in ("tri", "diag", "auto", "first", "refine"):
         raise NotImplementedError("triMode %r is outside the supported path" % p.triMode)
     unmasked = bool(mask.all())
@@ -1012,7 +1019,7 @@ def even_odd_frc(images: np.ndarray, p: Params, box: int = 512, overlap: float =
     return frc, te, to
 
 
-def align_image_stack(images: np.ndarray, p: Params, mask: np.ndarray | None = None):
+def align_image_stack(images: np.ndarray, p: Params, mask: np.ndarray | None = None, triIndices=None):
     """``alignImageStack`` for ``xcorrMode='zorro'``.  ``zorro/zorro.py:1569-1716``.
     With ``shapeBinned`` set the stack is Fourier-binned first and the pixel
     size scaled by the mean bin factor (``:1591-1592``, ``:1835-1836``)."""
@@ -1030,7 +1037,7 @@ def align_image_stack(images: np.ndarray, p: Params, mask: np.ndarray | None = N
         images = filt.astype(np.float64)
     n, h, w = images.shape
     padded, fmask = pad_stack(images, p.shapePadded)
-    xc = xcorrnm2_tri(padded, fmask if mask is None else mask, p)
+    xc = xcorrnm2_tri(padded, fmask if mask is None else mask, p, triIndices=triIndices)
     if p.triMode in ("first", "refine"):                                 # :1626-1647, no solver
         st = xc["shiftsTriMat"]
         t = -(st[0, :] if p.triMode == "first" else np.stack([st[k, k] for k in range(n)])).astype(np.float64)

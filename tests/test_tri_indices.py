@@ -1,0 +1,110 @@
+"""User-supplied pair masks: xcorrnm2_tri(triIndices=...) (zorro/zorro.py:773,
+:552-557, :959-975).
+
+CPU: the C ABI refuses masks outside the built path before touching a device
+(entries on or below the diagonal, a triMode other than 'tri', preShift); the
+oracle's mask path is pinned by the triMode patterns it must reproduce (a mask
+equal to the 'diag' band gives triMode 'diag''s trimats exactly).
+GPU: an arbitrary connected mask against the oracle (pair shifts identical on
+the 1/16 grid up to one step at near-ties, translations within 0.02 px), and
+the facade's xcorrnm2_tri + shiftsSolver equal to alignImageStack with the
+same mask (bit-identical translations)."""
+import numpy as np
+import pytest
+
+from oracle import zorro_oracle as O
+from zorro_amd._lib import ZrError
+from zorro_amd.engine import MoviePlan, make_params
+
+
+def _kw(mode, size, **extra):
+    kw = dict(triMode=mode, maxShift=16, Bmode="conv", Brad=150, subPixReg=16, shiftMethod="lanczos",
+              weightMode="logistic", peaksigThres=3.115, logisticK=5.0, logisticNu=0.15, originMode="centroid",
+              suppressOrigin=True, fouCrop=(0, 0), shapePadded=(size, size), filterMode=None, doLazyFRC=False)
+    kw.update(extra)
+    return kw
+
+
+def test_mask_refusals_before_any_device_call():
+    n = 6
+    lower = np.zeros((n, n), bool)
+    lower[3, 1] = True
+    with pytest.raises(ZrError, match="above the diagonal"):
+        MoviePlan(make_params(**_kw("tri", 64)), n, 64, 64, triIndices=lower)
+    up = np.triu(np.ones((n, n), bool), 1)
+    with pytest.raises(ZrError, match="triMode 'tri'"):
+        MoviePlan(make_params(**_kw("first", 64)), n, 64, 64, triIndices=up)
+    with pytest.raises(ZrError, match="preShift"):
+        MoviePlan(make_params(**_kw("tri", 64, preShift=True)), n, 64, 64, triIndices=up)
+    with pytest.raises(IndexError):
+        MoviePlan(make_params(**_kw("tri", 64)), n, 64, 64, triIndices=up[:5, :5])
+
+
+def test_oracle_mask_reproduces_trimode_patterns():
+    from zorro_amd.synth import make_movie
+    n, size = 6, 64
+    frames, _ = make_movie(n, size, size, seed=81, dose=4.0)
+    padded, mask = O.pad_stack(frames.astype(np.float64), (size, size))
+    kw = _kw("diag", size, diagWidth=2, maxShift=8)
+    ref = O.xcorrnm2_tri(padded, mask, O.Params(**kw))
+    band = O.tri_indices(n, "diag", 2, 0)
+    d = np.subtract.outer(np.arange(n), np.arange(n))
+    assert np.array_equal(band, (-d >= 1) & (-d <= 2))
+    got = O.xcorrnm2_tri(padded, mask, O.Params(**dict(kw, triMode="tri")), triIndices=band)
+    for k in ("shiftsTriMat", "corrTriMat", "peaksigTriMat"):
+        assert np.array_equal(got[k], ref[k]), k
+    assert np.all(got["corrTriMat"][~band] == 0)
+
+
+def _mask(n, seed):
+    rng = np.random.default_rng(seed)
+    m = np.triu(rng.random((n, n)) < 0.4, 1)
+    m[np.arange(n - 1), np.arange(1, n)] = True        # first superdiagonal: a connected equation graph
+    return m
+
+
+@pytest.mark.gpu
+def test_gpu_mask_matches_oracle():
+    from zorro_amd.synth import make_movie
+    n, size = 10, 256
+    print(f"[gpu triIndices] n={n} {size}^2", flush=True)
+    frames, _ = make_movie(n, size, size, seed=82, dose=4.0)
+    tri = _mask(n, 83)
+    kw = _kw("tri", size)
+    plan = MoviePlan(make_params(**kw), n, size, size, triIndices=tri)
+    assert plan.npairs == int(tri.sum())
+    plan.set_frames(frames)
+    plan.xcorr()
+    plan.solve()
+    tm, sol, m = plan.trimats(), plan.solution(), plan.equation_count()
+    plan.close()
+    assert np.array_equal(tm["corrTriMat"] != 0, tri) and m == int(tri.sum())
+    ref = O.align_image_stack(frames, O.Params(**kw), triIndices=tri)
+    so, sg = ref["xcorr"]["shiftsTriMat"], tm["shiftsTriMat"]
+    steps = np.abs(sg[tri] - so[tri]) * 16
+    assert np.all(np.abs(steps - np.round(steps)) < 1e-3) and np.all(np.round(steps) <= 1)
+    dt = np.max(np.abs(sol["translations"] - ref["translations"]))
+    print(f"[gpu triIndices] pairs {int(tri.sum())}, |dt|max {dt:.4f} px", flush=True)
+    assert dt <= 0.02
+
+
+@pytest.mark.gpu
+def test_gpu_facade_xcorr_and_solver():
+    from zorro_amd.registrator import ImageRegistrator
+    from zorro_amd.synth import make_movie
+    n, size = 8, 256
+    print(f"[gpu triIndices] facade n={n} {size}^2", flush=True)
+    frames, _ = make_movie(n, size, size, seed=84, dose=4.0)
+    tri = _mask(n, 85)
+    kw = _kw("tri", size)
+    reg = ImageRegistrator()
+    for k, v in kw.items():
+        setattr(reg, k, v)
+    reg.images = frames
+    tm = reg.xcorrnm2_tri(triIndices=tri)
+    ed = reg.shiftsSolver(tm["shiftsTriMat"], tm["corrTriMat"], tm["peaksigTriMat"])
+    reg.triIndices = tri
+    reg.alignImageStack()
+    assert np.array_equal(ed["translations"], reg.errorDictList[-1]["translations"])
+    assert ed["M"] == int(tri.sum()) and np.array_equal(ed["corrTriMat"], reg.errorDictList[-1]["corrTriMat"])
+    reg.close()

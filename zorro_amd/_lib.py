@@ -60,6 +60,8 @@ SIGNATURES = {
     "zr_last_error": (C.c_char_p, []),
     "zr_device_count": (C.c_int, [_I]),
     "zr_movie_create": (C.c_int, [C.c_int, C.POINTER(ZrParams), C.c_int, C.c_int, C.c_int, C.POINTER(_P)]),
+    "zr_movie_create_pairs": (C.c_int, [C.c_int, C.POINTER(ZrParams), C.c_int, C.c_int, C.c_int, C.c_void_p,
+                                        C.POINTER(_P)]),
     "zr_movie_destroy": (None, [_P]),
     "zr_movie_set_stream": (C.c_int, [_P, _P]),
     "zr_movie_set_frames": (C.c_int, [_P, _P, C.c_int]),

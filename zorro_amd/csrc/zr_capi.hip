@@ -130,6 +130,7 @@ struct StageTimer {
 }  // namespace
 
 struct zr_movie {
+  std::vector<uint8_t> tri_mask;   // user triIndices [N][N] (empty: the triMode's pairs)
   int device = 0;
   hipStream_t stream = nullptr;
   hipStream_t own_stream = nullptr;
@@ -280,11 +281,13 @@ static int build_pairs(zr_movie* mv) {
   // triIndices of __init_xcorrnm2 (zorro/zorro.py:559-581); 'auto' correlates
   // the band J - I <= autoMax, beyond which every row has stopped (:992-996)
   if (p.tri_mode == ZR_TRI_AUTO && p.auto_max < 1) return fail(ZR_ERR_ARG, "autoMax must be >= 1");
+  const bool masked_pairs = !mv->tri_mask.empty();
   for (int I = 0; I < N; ++I)
     for (int J = I + 1; J < N; ++J) {
       int d = J - I;
       bool take = false;
-      if (p.tri_mode == ZR_TRI_TRI) take = true;
+      if (masked_pairs) take = mv->tri_mask[(size_t)I * N + J] != 0;
+      else if (p.tri_mode == ZR_TRI_TRI) take = true;
       else if (p.tri_mode == ZR_TRI_DIAG) {
         int wdt = p.diag_width < 0 ? N : p.diag_width;
         take = (d <= wdt + p.diag_start) && (d > p.diag_start);
@@ -341,11 +344,25 @@ static PeakParamsHost peak_params(const zr_movie* mv) {
   return hp;
 }
 
-extern "C" int zr_movie_create(int device, const zr_params* p, int n_frames, int h, int w, zr_movie** out) {
+static int create_movie(int device, const zr_params* p, int n_frames, int h, int w, const uint8_t* tri_indices,
+                        zr_movie** out) {
   if (!p || !out) return fail(ZR_ERR_ARG, "NULL argument");
   *out = nullptr;
   if (n_frames < 2 || h < 16 || w < 16) return fail(ZR_ERR_ARG, "need >= 2 frames of >= 16x16 (got %d x %d x %d)", n_frames, h, w);
+  if (tri_indices) {
+    // xcorrnm2_tri(triIndices=...) (zorro/zorro.py:552-557): pairs I < J of the
+    // mask, correlated and solved as triMode 'tri' does
+    if (p->tri_mode != ZR_TRI_TRI)
+      return fail(ZR_ERR_UNSUPPORTED, "a triIndices mask runs with triMode 'tri' (got %d)", p->tri_mode);
+    if (p->pre_shift)
+      return fail(ZR_ERR_UNSUPPORTED, "preShift with a triIndices mask (the reference warns that it breaks, zorro/zorro.py:549-550)");
+    for (int I = 0; I < n_frames; ++I)
+      for (int J = 0; J <= I; ++J)
+        if (tri_indices[(size_t)I * n_frames + J])
+          return fail(ZR_ERR_UNSUPPORTED, "triIndices[%d][%d]: only pairs above the diagonal (I < J) are built", I, J);
+  }
   zr_movie* mv = new zr_movie();
+  if (tri_indices) mv->tri_mask.assign(tri_indices, tri_indices + (size_t)n_frames * n_frames);
   g_alloc_bytes = 0;
   mv->device = device;
   mv->p = *p;
@@ -606,6 +623,16 @@ extern "C" int zr_movie_create(int device, const zr_params* p, int n_frames, int
   mv->bytes = g_alloc_bytes + zr_stage_device_bytes(mv->st);
   *out = mv;
   return ZR_OK;
+}
+
+extern "C" int zr_movie_create(int device, const zr_params* p, int n_frames, int h, int w, zr_movie** out) {
+  return create_movie(device, p, n_frames, h, w, nullptr, out);
+}
+
+extern "C" int zr_movie_create_pairs(int device, const zr_params* p, int n_frames, int h, int w,
+                                     const uint8_t* tri_indices, zr_movie** out) {
+  if (!tri_indices) return fail(ZR_ERR_ARG, "NULL triIndices");
+  return create_movie(device, p, n_frames, h, w, tri_indices, out);
 }
 
 extern "C" int zr_movie_device_bytes(zr_movie* mv, size_t* bytes) {

@@ -100,12 +100,23 @@ class MoviePlan:
     with ``shapeBinned`` the frames are binned on input and every result has the
     binned size ``hb`` x ``wb``)."""
 
-    def __init__(self, params: L.ZrParams, n: int, h: int, w: int, device: int = 0):
+    def __init__(self, params: L.ZrParams, n: int, h: int, w: int, device: int = 0, triIndices=None):
+        """``triIndices``: optional boolean [n, n] matrix of the pairs (I < J) to
+        correlate, as ``xcorrnm2_tri(triIndices=...)`` takes (triMode 'tri')."""
         self.lib = L.load()
         self.params = params
         self.n, self.h, self.w = int(n), int(h), int(w)
         self._mv = C.c_void_p()
-        L.check(self.lib.zr_movie_create(int(device), C.byref(params), self.n, self.h, self.w, C.byref(self._mv)))
+        if triIndices is None:
+            L.check(self.lib.zr_movie_create(int(device), C.byref(params), self.n, self.h, self.w,
+                                             C.byref(self._mv)))
+        else:
+            t = np.asarray(triIndices)
+            if t.shape != (self.n, self.n):
+                raise IndexError(f"triIndices is wrong size, should be of length: {self.n}")
+            t = np.ascontiguousarray(t != 0, dtype=np.uint8)
+            L.check(self.lib.zr_movie_create_pairs(int(device), C.byref(params), self.n, self.h, self.w,
+                                                   C.c_void_p(t.ctypes.data), C.byref(self._mv)))
         npairs = C.c_int()
         L.check(self.lib.zr_pair_count(self._mv, C.byref(npairs)))
         self.npairs = npairs.value

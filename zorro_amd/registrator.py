@@ -85,6 +85,10 @@ class ImageRegistrator:
         self.autoMax = 10
         self.corrThres = None
         self.velocityThres = None
+        # pairs to correlate as a boolean [N, N] matrix (triMode 'tri'); None: the
+        # triMode's pattern.  The reference takes it as xcorrnm2_tri(triIndices=...)
+        # (zorro/zorro.py:773, :552-557)
+        self.triIndices = None
         # configuration, IO and results kept for .zor files (zorro/zorro.py:85-243)
         self.METAstatus = "new"
         self.umask = 2
@@ -131,13 +135,19 @@ class ImageRegistrator:
                            doLazyFRC=self.doLazyFRC, shapeBinned=self.shapeBinned, hotpixInfo=self.hotpixInfo,
                            preShift=self.preShift)
 
-    def _plan_for(self, n, h, w):
+    def _plan_for(self, n, h, w, triIndices=None):
         p = self._params()
-        key = (n, h, w, bytes(memoryview(p)))
+        tri = None if triIndices is None else np.ascontiguousarray(np.asarray(triIndices) != 0)
+        if tri is not None:
+            # the mask replaces the pattern of 'tri' and 'diag' (zorro/zorro.py:552-557)
+            if self.triMode not in ("tri", "diag"):
+                raise NotImplementedError(f"triIndices with triMode {self.triMode!r}: only 'tri' and 'diag' take a mask")
+            p.tri_mode = 0      # ZR_TRI_TRI
+        key = (n, h, w, bytes(memoryview(p)), None if tri is None else tri.tobytes())
         if self._plan is None or self._plan_key != key:
             if self._plan is not None:
                 self._plan.close()
-            self._plan = MoviePlan(p, n, h, w, device=self.device)
+            self._plan = MoviePlan(p, n, h, w, device=self.device, triIndices=tri)
             self._plan_key = key
         return self._plan
 
@@ -271,13 +281,49 @@ class ImageRegistrator:
             self.loadData()
         return self
 
-    def alignImageStack(self):
-        """Register, sum, FRC and dose-filter ``self.images`` (``zorro/zorro.py:1569-1716``)."""
+    def xcorrnm2_tri(self, triIndices=None):
+        """All-pairs correlation of ``self.images`` (``zorro/zorro.py:773-1011``)
+        for ``triIndices`` (boolean [N, N], pairs I < J; None: ``self.triIndices``
+        or the triMode's pattern).  Returns the trimats (shiftsTriMat,
+        corrTriMat, peaksigTriMat, originTriMat), which the reference keeps in
+        private members for ``shiftsSolver``."""
+        plan = self._load_plan(self.triIndices if triIndices is None else triIndices)
+        plan.xcorr()
+        plan.synchronize()
+        return plan.trimats()
+
+    def shiftsSolver(self, shiftsTriMat_in, corrTriMat_in, peaksigTriMat_in, acceptedEqns=None, mode="basin",
+                     Niter=100):
+        """Weights and the weighted-L1 solution for given trimats
+        (``zorro/zorro.py:1260-1567``); the equations are the non-zero entries of
+        ``corrTriMat_in`` above the diagonal (``:1282``).  Returns the errorDict.
+        ``mode``/``Niter`` select the reference's random basin hopping, which the
+        exact solver replaces; ``acceptedEqns`` other than all-true is refused."""
+        if acceptedEqns is not None and not np.all(acceptedEqns):
+            raise NotImplementedError("acceptedEqns with rejected equations")
+        corr = np.asarray(corrTriMat_in, np.float32)
+        n = corr.shape[0]
+        h, w = (int(v) for v in np.shape(self.images)[1:]) if self.images is not None else (64, 64)
+        tri = np.triu(corr != 0, 1)
+        p = self._params()
+        p.tri_mode, p.pre_shift = 0, 0      # ZR_TRI_TRI over the given pairs
+        plan = MoviePlan(p, n, h, w, device=self.device, triIndices=tri)
+        try:
+            plan.set_trimats(np.asarray(shiftsTriMat_in, np.float32), corr, np.asarray(peaksigTriMat_in, np.float32))
+            plan.solve()
+            plan.synchronize()
+            tm = dict(shiftsTriMat=np.asarray(shiftsTriMat_in, np.float32), corrTriMat=corr,
+                      peaksigTriMat=np.asarray(peaksigTriMat_in, np.float32))
+            return self._error_dict(plan, tm, plan.solution())
+        finally:
+            plan.close()
+
+    def _load_plan(self, triIndices=None):
+        """The plan for ``self.images`` with the frames set (gain, binning)."""
         if self.images is None or np.ndim(self.images) != 3:
             raise ValueError("images must be a [N, H, W] stack")
-        self.bench["total0"] = time.time()
         n, h, w = (int(v) for v in self.images.shape)
-        plan = self._plan_for(n, h, w)
+        plan = self._plan_for(n, h, w, triIndices)
         raw_input = (self.gainRef is not None or np.any(self.shapeBinned) or
                      str(getattr(self.images, "dtype", "float32")).replace("torch.", "") != "float32"
                      or (self.gain and not np.isclose(self.gain, 1.0)))
@@ -292,6 +338,13 @@ class ImageRegistrator:
                 self.gainRef = None
         else:
             plan.set_frames(self.images)
+        return plan
+
+    def alignImageStack(self):
+        """Register, sum, FRC and dose-filter ``self.images`` (``zorro/zorro.py:1569-1716``)."""
+        self.bench["total0"] = time.time()
+        plan = self._load_plan(self.triIndices)
+        n = plan.n
         self.bench["xcorr0"] = time.time()
         plan.xcorr()
         plan.synchronize()
@@ -319,6 +372,12 @@ class ImageRegistrator:
                 ed["translations"] = -np.stack([tm["shiftsTriMat"][k, k] for k in range(n)]).astype(np.float64)
             self.errorDictList.append(ed)
             return self._finish(plan, sol)
+        self.errorDictList.append(self._error_dict(plan, tm, sol))
+        return self._finish(plan, sol)
+
+    def _error_dict(self, plan, tm, sol):
+        """errorDict of shiftsSolver (zorro/zorro.py:1550-1565) from a solved plan."""
+        n = plan.n
         info = plan.solver_info()
         pairs = [(I, J) for I in range(n - 1) for J in range(I + 1, n) if tm["corrTriMat"][I, J] != 0]
         ed = dict(tm)
@@ -337,8 +396,7 @@ class ImageRegistrator:
             self.peaksigThres, self.logisticK, self.logisticNu = thr, K, nu
             ed.update(peaksigThres=thr, logisticK=K, logisticNu=nu)
         ed["translations"] = -np.vstack((np.zeros([1, 2]), np.cumsum(sol["relativeEst"], axis=0)))
-        self.errorDictList.append(ed)
-        return self._finish(plan, sol)
+        return ed
 
     def _finish(self, plan, sol):
         """applyShifts, FRC and the dose filter (zorro/zorro.py:1672-1712)."""
