@@ -5,8 +5,11 @@ CPU: the C ABI refuses masks outside the built path before touching a device
 (entries on or below the diagonal, a triMode other than 'tri', preShift); the
 oracle's mask path is pinned by the triMode patterns it must reproduce (a mask
 equal to the 'diag' band gives triMode 'diag''s trimats exactly).
-GPU: an arbitrary connected mask against the oracle (pair shifts identical on
-the 1/16 grid up to one step at near-ties, translations within 0.02 px), and
+GPU: a mask of contiguous rows (the masks whose equations the solver can
+hold; with a row with a gap the pairs are correlated and the solve refused)
+against the oracle (pair shifts identical on the 1/16 grid up to one step at
+near-ties; the solver on those trimats within 1e-9 px of the oracle's exact
+vertex; translations within 0.02 px when every pair shift agrees), and
 the facade's xcorrnm2_tri + shiftsSolver equal to alignImageStack with the
 same mask (bit-identical translations)."""
 import numpy as np
@@ -57,10 +60,23 @@ def test_oracle_mask_reproduces_trimode_patterns():
 
 
 def _mask(n, seed):
+    """Row I takes J = I+1 .. I+len_I (len_I >= 1): the shape of masks whose
+    rot90 equations (zorro/zorro.py:1296-1311) are single frame-to-frame spans;
+    at most three pairs per row, so that every pair is well above the noise."""
     rng = np.random.default_rng(seed)
-    m = np.triu(rng.random((n, n)) < 0.4, 1)
-    m[np.arange(n - 1), np.arange(1, n)] = True        # first superdiagonal: a connected equation graph
+    m = np.zeros((n, n), bool)
+    for I in range(n - 1):
+        m[I, I + 1:I + 1 + int(rng.integers(1, min(3, n - 1 - I) + 1))] = True
     return m
+
+
+def test_mask_gives_parallel_equations():
+    """The reference's rot90 construction repeats a frame span for uneven rows:
+    the GPU test's mask has two identical equation rows, so the solver's
+    handling of parallel edges is exercised there."""
+    A = O.build_equations(_mask(10, 83)[:-1, 1:])
+    rows = [tuple(r) for r in A.astype(int)]
+    assert len(rows) == 14 and len(set(rows)) == 13
 
 
 @pytest.mark.gpu
@@ -83,9 +99,51 @@ def test_gpu_mask_matches_oracle():
     so, sg = ref["xcorr"]["shiftsTriMat"], tm["shiftsTriMat"]
     steps = np.abs(sg[tri] - so[tri]) * 16
     assert np.all(np.abs(steps - np.round(steps)) < 1e-3) and np.all(np.round(steps) <= 1)
+    # the solver on the GPU's own trimats: the oracle's exact vertex of the same equations
+    ed = O.shifts_solver(tm["shiftsTriMat"], tm["corrTriMat"], tm["peaksigTriMat"], O.Params(**kw), (size, size))
+    t_or = ed["translations"] - ed["translations"].mean(axis=0)
+    ds = np.max(np.abs(sol["translations"] - t_or))
+    # weighted-L1 objective of both solutions on the oracle's equations
+    A, w = ed["Acoeff"], np.asarray(ed["weights"])
+    S = tm["shiftsTriMat"][:-1, 1:, :]
+    msk = tm["corrTriMat"][:-1, 1:].astype(bool)
+    objs = []
+    for ax in (0, 1):
+        b = np.ravel(S[msk, ax]).astype(np.float64)
+        objs.append((float(np.sum(w * np.abs(A @ sol["relativeEst"][:, ax] - b))),
+                     float(np.sum(w * np.abs(A @ ed["relativeEst"][:, ax] - b)))))
+    print(f"[gpu triIndices] objectives (gpu, oracle) y {objs[0]}, x {objs[1]}", flush=True)
+    for og, oo in objs:
+        assert og <= oo * (1 + 1e-9) + 1e-12, "the GPU solution is not optimal"
+    # end to end when every pair shift is the oracle's (on a sparse graph one 1/16
+    # step can move the weighted-L1 optimum to another vertex)
+    same = bool(np.all(np.round(steps) == 0))
     dt = np.max(np.abs(sol["translations"] - ref["translations"]))
-    print(f"[gpu triIndices] pairs {int(tri.sum())}, |dt|max {dt:.4f} px", flush=True)
-    assert dt <= 0.02
+    print(f"[gpu triIndices] pairs {int(tri.sum())}, solver |dt|max {ds:.2e} px, end to end |dt|max {dt:.4f} px "
+          f"(pair shifts identical: {same})", flush=True)
+    # equal objectives: a different vertex of the same optimal face is allowed
+    tie = all(abs(og - oo) <= 1e-9 * max(oo, 1e-12) for og, oo in objs)
+    assert ds <= 1e-9 or tie
+    if same and ds <= 1e-9:
+        assert dt <= 0.02
+    # a row with a gap gives an equation over non-adjacent frame steps, which the
+    # exact solver's graph cannot hold: the pairs are correlated, the solve refused
+    gap = tri.copy()
+    gap[0, :] = False
+    gap[0, [1, 3]] = True
+    plan = MoviePlan(make_params(**kw), n, size, size, triIndices=gap)
+    try:
+        plan.set_frames(frames)
+        plan.xcorr()
+        tg = plan.trimats()
+        assert np.array_equal(tg["corrTriMat"] != 0, gap)
+        assert np.array_equal(tg["shiftsTriMat"][gap & tri], tm["shiftsTriMat"][gap & tri])
+        with pytest.raises(ZrError, match="cannot represent"):
+            plan.solve()
+        with pytest.raises(ZrError, match="cannot represent"):
+            plan.align()
+    finally:
+        plan.close()
 
 
 @pytest.mark.gpu

@@ -1024,6 +1024,8 @@ extern "C" int zr_movie_use_graph(zr_movie* mv, int enable) {
 
 extern "C" int zr_align(zr_movie* mv) {
   if (!mv) return fail(ZR_ERR_ARG, "NULL movie");
+  if (!zr_stage_equations_ok(mv->st) && mv->p.tri_mode != ZR_TRI_FIRST && mv->p.tri_mode != ZR_TRI_REFINE)
+    return fail(ZR_ERR_UNSUPPORTED, "pair pattern gives equations the solver cannot represent");
   if (!mv->use_graph) return align_sequence(mv);
   if (!mv->frames) return fail(ZR_ERR_STATE, "frames not set");
   ZR_HIP(hipSetDevice(mv->device));

@@ -22,6 +22,8 @@ size_t zr_stage_device_bytes(const zr_stage_state* s);
 int zr_stage_copy_sums_device(zr_stage_state* s, float* image_sum, float* filt_sum, float* even_sum, float* odd_sum,
                               hipStream_t st);
 int zr_stage_solve(zr_stage_state* s, const float* shifts, const float* corr, const float* peaksig, hipStream_t st);
+// 0 when the pairs' equations cannot be solved (zr_stage_solve then fails)
+int zr_stage_equations_ok(const zr_stage_state* s);
 int zr_stage_shifts(zr_stage_state* s, const float* frames, float* aligned_out, hipStream_t st);
 int zr_stage_frc(zr_stage_state* s, hipStream_t st);
 // crop limits / tile counts of the next zr_stage_frc from caller translations [n][2]

@@ -14,6 +14,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <string>
+#include <map>
 #include <vector>
 
 #include "zr_common.h"
@@ -224,6 +225,7 @@ struct SolveArgs {
   const float* peaksig;   // [N][N]
   const int2* edges;      // [M] graph edge (i, j): the equation p_j - p_i = b
   const int2* meas;       // [M] trimat entry (I, J) holding b and the peak significance
+  const int2* dups;       // [M] (first edge with the same (i, j), next such edge or -1)
   int N, M;
   int weight_mode;
   double logi_fixed[3];
@@ -360,14 +362,27 @@ __global__ __launch_bounds__(L1_NT) void k_l1_solve(SolveArgs a) {
   for (int it = 0; it < IRLS_ITERS; ++it) {
     for (int q = tid; q < N * LDC; q += L1_NT) Lm[q] = 0.0;
     __syncthreads();
+    // per-edge weights staged in cd_s; the first edge of each (i, j) sums its
+    // parallel edges in edge order (the equations can repeat a frame span)
 #pragma unroll
     for (int k = 0; k < L1_EPT; ++k) {
       if (eij[k] < 0) continue;
       const int i = eij[k] & 255, j = eij[k] >> 8;
       const double re = pvs[j] - pvs[i] - eb[k];
-      const double d = fmax(ew[k], 1e-12) / fmax(fabs(re), eps);
+      cd_s[tid + k * L1_NT] = fmax(ew[k], 1e-12) / fmax(fabs(re), eps);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < L1_EPT; ++k) {
+      const int e = tid + k * L1_NT;
+      if (eij[k] < 0) continue;
+      const int2 dp = a.dups[e];
+      if (dp.x != e) continue;
+      double d = cd_s[e], db = cd_s[e] * b_s[e];
+      for (int q = dp.y; q >= 0; q = a.dups[q].y) { d += cd_s[q]; db += cd_s[q] * b_s[q]; }
+      const int i = eij[k] & 255, j = eij[k] >> 8;
       Lm[j * LDC + i] = -d;
-      Lm[i * LDC + j] = -d * eb[k];
+      Lm[i * LDC + j] = -db;
     }
     __syncthreads();
     double gv = 0.0;     // wave 0, lane k: right-hand side of frame k
@@ -575,23 +590,37 @@ __global__ __launch_bounds__(L1_NT) void k_l1_solve(SolveArgs a) {
   bool optimal = false;
   for (; connected && piv <= MAX_PIVOTS; ++piv) {
     const bool bland = streak > L1_BLAND * N;
-    // flow bounds of the non-tree edges: C[i][j] = +w s (into j), C[j][i] = -w s
+    // flow bounds of the non-tree edges: C[i][j] = +w s (into j), C[j][i] = -w s,
+    // summed over the parallel edges of (i, j) in edge order by the first of them
+    // (tree edges carry no bound flow); per-edge values staged in cd_s
     if (tid == 0) s_nc = 0;
 #pragma unroll
     for (int k = 0; k < L1_EPT; ++k) {
       if (eij[k] < 0) continue;
-      const int i = eij[k] & 255, j = eij[k] >> 8;
+      const int e = tid + k * L1_NT;
       if ((tmask >> k) & 1u) {
         er[k] = 0.0;
-        Lm[i * LDC + j] = 0.0;
-        Lm[j * LDC + i] = 0.0;
+        cd_s[e] = 0.0;
       } else {
+        const int i = eij[k] & 255, j = eij[k] >> 8;
         const double r = pvs[j] - pvs[i] - eb[k];
         er[k] = r;
         const double s = r > 0.0 ? 1.0 : (r < 0.0 ? -1.0 : (((smask >> k) & 1u) ? 1.0 : -1.0));
-        Lm[i * LDC + j] = ew[k] * s;
-        Lm[j * LDC + i] = -ew[k] * s;
+        cd_s[e] = ew[k] * s;
       }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < L1_EPT; ++k) {
+      const int e = tid + k * L1_NT;
+      if (eij[k] < 0) continue;
+      const int2 dp = a.dups[e];
+      if (dp.x != e) continue;
+      double f = cd_s[e];
+      for (int q = dp.y; q >= 0; q = a.dups[q].y) f += cd_s[q];
+      const int i = eij[k] & 255, j = eij[k] >> 8;
+      Lm[i * LDC + j] = f;
+      Lm[j * LDC + i] = -f;
     }
     __syncthreads();
     // slopes of moving each subtree by +-delta: w(tree edge) +- (bound flow into the subtree)
@@ -1673,6 +1702,26 @@ hipError_t upload(T** p, const std::vector<T>& v) {
 // holds its measurement.  For tri and diag patterns the edge of each equation
 // is its own pair; other patterns can give an equation whose run differs from
 // the pair whose shift it is matched with, exactly as in the reference.
+// For every equation edge: the first edge with the same (i, j) and the next one
+// (-1 at the end of the group), so that the solver sums parallel edges in order.
+std::vector<int2> duplicate_groups(const std::vector<int2>& graph) {
+  std::vector<int2> d(graph.size(), make_int2(-1, -1));
+  std::map<long long, int> last;
+  for (size_t e = 0; e < graph.size(); ++e) {
+    const long long key = (long long)graph[e].x * 65536 + graph[e].y;
+    auto it = last.find(key);
+    if (it == last.end()) {
+      d[e].x = (int)e;
+      last.emplace(key, (int)e);
+    } else {
+      d[e].x = d[it->second].x;
+      d[it->second].y = (int)e;
+      it->second = (int)e;
+    }
+  }
+  return d;
+}
+
 int build_equations(int N, const std::vector<int4>& pairs, std::vector<int2>& graph, std::vector<int2>& meas) {
   const int n = N - 1;
   std::vector<unsigned char> tri((size_t)n * n, 0);
@@ -1716,8 +1765,9 @@ int build_equations(int N, const std::vector<int4>& pairs, std::vector<int2>& gr
 struct zr_stage_state {
   zr_params p{};
   int N = 0, H = 0, W = 0, Hp = 0, Wp = 0, M = 0, Mcap = 0;
+  bool eq_bad = false;         // the pairs' rot90 equations are not single frame spans
   size_t bytes = 0;            // device memory held
-  int2 *edges = nullptr, *meas = nullptr;
+  int2 *edges = nullptr, *meas = nullptr, *dups = nullptr;
   float *hs = nullptr, *cdf = nullptr;
   double* obj = nullptr;
   SolveInfo* info = nullptr;
@@ -1771,7 +1821,7 @@ int zr_stage_copy_sums_device(zr_stage_state* s, float* image_sum, float* filt_s
 
 void zr_stage_free(zr_stage_state* s) {
   if (!s) return;
-  void* ptrs[] = {s->edges, s->meas, s->hs, s->cdf, s->obj, s->info, s->weights, s->resid, s->pos, s->trans_raw, s->trans,
+  void* ptrs[] = {s->edges, s->meas, s->dups, s->hs, s->cdf, s->obj, s->info, s->weights, s->resid, s->pos, s->trans_raw, s->trans,
                   s->rel, s->err_stats, s->err_frames, s->fs, s->img_sum, s->even, s->odd, s->filt, s->rowsum,
                   s->twB, s->twB2, s->Tfrc, s->frc2d, s->frc, s->frc_part, s->hann, s->twHp, s->twWp, s->twWp2, s->Tdose, s->G,
                   s->accum, s->thresq, s->drow, s->R0, s->RH, s->C0, s->CW, s->Py, s->Px};
@@ -1796,11 +1846,14 @@ int zr_stage_create(zr_stage_state** out, const zr_params& p, int N, int H, int 
     ms = ed;
     s->M = 0;
   } else if (build_equations(N, pairs, ed, ms) != 0) {
-    zr_stage_free(s);
-    return serr(ZR_ERR_UNSUPPORTED, "pair pattern gives equations the solver cannot represent");
+    // the pairs can still be correlated; zr_stage_solve refuses
+    ed.assign(std::max(1, s->Mcap), make_int2(0, 1));   // placeholders: the buffers keep Mcap entries
+    ms = ed;
+    s->eq_bad = true;
   }
 #define ZC(x) do { hipError_t e_ = (x); if (e_ != hipSuccess) { zr_stage_free(s); return serr(ZR_ERR_NOMEM, "%s: %s", #x, hipGetErrorString(e_)); } } while (0)
   ZC(supload(s, &s->edges, ed));
+  ZC(supload(s, &s->dups, duplicate_groups(ed)));
   ZC(supload(s, &s->meas, ms));
   const int maxbins = std::max(1, s->Mcap / 7 + 1);
   ZC(smalloc(s, (void**)&s->hs, sizeof(float) * maxbins));
@@ -1876,8 +1929,13 @@ int zr_stage_set_pairs(zr_stage_state* s, const std::vector<int4>& pairs) {
     return serr(ZR_ERR_UNSUPPORTED, "pair pattern gives equations the solver cannot represent");
   if ((int)ed.size() > s->Mcap || ed.empty()) return serr(ZR_ERR_ARG, "equation count %zu outside 1..%d", ed.size(), s->Mcap);
   ZS_HIP(hipMemcpy(s->edges, ed.data(), ed.size() * sizeof(int2), hipMemcpyHostToDevice));
+  {
+    const std::vector<int2> dg = duplicate_groups(ed);
+    ZS_HIP(hipMemcpy(s->dups, dg.data(), dg.size() * sizeof(int2), hipMemcpyHostToDevice));
+  }
   ZS_HIP(hipMemcpy(s->meas, ms.data(), ms.size() * sizeof(int2), hipMemcpyHostToDevice));
   s->M = (int)ed.size();
+  s->eq_bad = false;
   return ZR_OK;
 }
 
@@ -1891,8 +1949,13 @@ static void fill_post_args(zr_stage_state* s, PostArgs& q) {
   q.err_frames = s->err_frames; q.fs = s->fs; q.info = s->info;
 }
 
+int zr_stage_equations_ok(const zr_stage_state* s) { return s && !s->eq_bad; }
+
 int zr_stage_solve(zr_stage_state* s, const float* shifts, const float* corr, const float* peaksig, hipStream_t st) {
   if (!s) return serr(ZR_ERR_STATE, "no stage state");
+  if (s->eq_bad)
+    return serr(ZR_ERR_UNSUPPORTED, "pair pattern gives equations the solver cannot represent (a row of the pair "
+                "mask with a gap makes an equation over non-adjacent frame steps)");
   const int N = s->N, M = s->M;
   ZS_HIP(zr_launch_fill_zero(s->info, sizeof(SolveInfo), st));   // a kernel, so the solve can be graph-captured
   if (s->p.tri_mode == ZR_TRI_FIRST || s->p.tri_mode == ZR_TRI_REFINE) {
@@ -1918,7 +1981,7 @@ int zr_stage_solve(zr_stage_state* s, const float* shifts, const float* corr, co
     ZS_HIP(hipGetLastError());
   }
   SolveArgs a;
-  a.shifts = shifts; a.corr = corr; a.peaksig = peaksig; a.edges = s->edges; a.meas = s->meas;
+  a.shifts = shifts; a.corr = corr; a.peaksig = peaksig; a.edges = s->edges; a.meas = s->meas; a.dups = s->dups;
   a.N = N; a.M = M; a.weight_mode = wm;
   a.logi_fixed[0] = s->p.peaksig_thres; a.logi_fixed[1] = s->p.logistic_k; a.logi_fixed[2] = s->p.logistic_nu;
   a.bound = 0;
