@@ -1111,46 +1111,64 @@ __global__ __launch_bounds__(256) void k_frc_hann(int box, float* __restrict__ h
   hann[e] = rr <= 1.0 ? (float)(0.5 * (1.0 + cos(M_PI * sqrt(rr)))) : 0.f;
 }
 
+// Rows per k_frc_rows block: one group of FRC_TPR(BOX) threads per tile row
+// (16 complex values per thread), 256 threads per block.
+constexpr int frc_tpr(int box) { return box / 32; }
+constexpr int frc_rows_per_block(int box) { return 256 / frc_tpr(box); }
+
 template <int BOX>
 __global__ __launch_bounds__(256) void k_frc_rows(const float* __restrict__ even, const float* __restrict__ odd,
                                                  const SolveInfo* __restrict__ info, int W, int tmax_x, double step,
                                                  const float2* __restrict__ twL, const float2* __restrict__ tw2L,
                                                  const float* __restrict__ hann, float2* __restrict__ T) {
-  constexpr int L = BOX / 2, RB = 4, NT = 256;
+  constexpr int L = BOX / 2, NT = 256, TPR = frc_tpr(BOX), RB = frc_rows_per_block(BOX), PER = L / TPR;
+  static_assert(BOX % RB == 0 && L % TPR == 0, "tile rows must split evenly");
   __shared__ float2 buf[RB][L];
   const int tid = threadIdx.x;
-  const int nrb = BOX / RB;
+  const int g = tid / TPR, gt = tid - g * TPR;   // row group, thread in group
+  constexpr int nrb = BOX / RB;
   int blk = blockIdx.x;
   const int rb = blk % nrb;
   blk /= nrb;
   const int par = blk % 2;
   const int tile = blk / 2;
   const int ty = tile / tmax_x, tx = tile % tmax_x;
-  if (ty >= info->tiles_y || tx >= info->tiles_x) return;
+  if (ty >= info->tiles_y || tx >= info->tiles_x) return;   // block-uniform
   const int oy = (int)(ty * BOX * step + info->crop[0]);
   const int ox = (int)(tx * BOX * step + info->crop[1]);
   const float* img = par ? odd : even;
-  for (int r = 0; r < RB; ++r) {
-    const int yl = rb * RB + r;
-    const float* row = img + (size_t)(oy + yl) * W + ox;
-    const float* hw = hann + (size_t)yl * BOX;
-    for (int n = tid; n < L; n += NT) {
-      const int xl = 2 * n;
-      buf[r][n] = make_float2(hw[xl] * row[xl], hw[xl + 1] * row[xl + 1]);
+  const int yl = rb * RB + g;
+  const float2* row = reinterpret_cast<const float2*>(img + (size_t)(oy + yl) * W + ox);
+  const float2* hw = reinterpret_cast<const float2*>(hann + (size_t)yl * BOX);
+  const bool even_x = ((((size_t)(oy + yl) * W + ox) & 1) == 0);   // float2 loads need an even offset
+#pragma unroll
+  for (int i = 0; i < PER; ++i) {
+    const int n = gt + i * TPR;
+    const float2 h = hw[n];
+    float2 v;
+    if (even_x) {
+      v = row[n];
+    } else {
+      const float* r1 = img + (size_t)(oy + yl) * W + ox;
+      v = make_float2(r1[2 * n], r1[2 * n + 1]);
     }
+    buf[g][n] = make_float2(h.x * v.x, h.y * v.y);
   }
   __syncthreads();
-  for (int r = 0; r < RB; ++r) wg_fft<L, -1, NT>(buf[r], twL, tid);
+  wg_fft<L, -1, TPR>(buf[g], twL, gt);
+  // real-FFT post-processing; T[tile][par][k][y] takes RB consecutive rows per k
   const int Wh = L + 1;
   for (int k = tid; k < Wh; k += NT) {
-    float2 w = tw2L[k];
+    const float2 w = tw2L[k];
+    float2* dst = T + (((size_t)tile * 2 + par) * Wh + k) * BOX + rb * RB;
+#pragma unroll
     for (int r = 0; r < RB; ++r) {
-      float2 zk = buf[r][k % L];
-      float2 zn = c_conj(buf[r][(L - k) % L]);
-      float2 e = c_scale(c_add(zk, zn), 0.5f);
-      float2 o = c_scale(c_sub(zk, zn), 0.5f);
-      float2 Ok = make_float2(o.y, -o.x);
-      T[(((size_t)tile * 2 + par) * Wh + k) * BOX + rb * RB + r] = c_add(e, c_mul(w, Ok));
+      const float2 zk = buf[r][k % L];
+      const float2 zn = c_conj(buf[r][(L - k) % L]);
+      const float2 e = c_scale(c_add(zk, zn), 0.5f);
+      const float2 o = c_scale(c_sub(zk, zn), 0.5f);
+      const float2 Ok = make_float2(o.y, -o.x);
+      dst[r] = c_add(e, c_mul(w, Ok));
     }
   }
 }
@@ -1159,36 +1177,58 @@ __global__ __launch_bounds__(256) void k_frc_rows(const float* __restrict__ even
 // group and accumulate Re(E conj(O)) / sqrt(|O|^2 |E|^2) in FP64 into that
 // group's partial plane (k_frc_reduce adds the groups in a fixed order).
 constexpr int FRC_GROUPS = 16;
+// Tiles transformed together per k_frc_cols block (even and odd column of each,
+// one group of BOX/16 threads per transform); the ratios are added in the
+// tiles' order, as one tile at a time would add them.
+constexpr int FRC_TB = 4;
+constexpr int frc_cols_nt(int box) { return 2 * FRC_TB * (box / 16); }
 template <int BOX>
-__global__ __launch_bounds__(256) void k_frc_cols(const float2* __restrict__ T, const SolveInfo* __restrict__ info,
-                                                 int tmax_x, int ntiles_max, const float2* __restrict__ twB,
-                                                 double* __restrict__ part) {
-  constexpr int NT = 256, PER = BOX / NT;
-  __shared__ float2 be[BOX], bo[BOX];
+__global__ __launch_bounds__(frc_cols_nt(BOX)) void k_frc_cols(const float2* __restrict__ T,
+                                                              const SolveInfo* __restrict__ info, int tmax_x,
+                                                              int ntiles_max, const float2* __restrict__ twB,
+                                                              double* __restrict__ part) {
+  constexpr int TPC = BOX / 16, NT = frc_cols_nt(BOX), PER = BOX / NT;
+  static_assert(BOX % NT == 0, "column must split evenly over the block");
+  __shared__ float2 buf[2 * FRC_TB][BOX];     // [2 j + parity]
+  __shared__ int live[FRC_TB];
   const int tid = threadIdx.x;
+  const int q = tid / TPC, gt = tid - q * TPC;  // transform group: tile slot q / 2, parity q % 2
   const int Wh = BOX / 2 + 1;
   const int k = blockIdx.x % Wh, g = blockIdx.x / Wh;
   double acc[PER];
+#pragma unroll
   for (int i = 0; i < PER; ++i) acc[i] = 0.0;
-  for (int tile = g; tile < ntiles_max; tile += FRC_GROUPS) {
-    const int ty = tile / tmax_x, tx = tile % tmax_x;
-    if (ty >= info->tiles_y || tx >= info->tiles_x) continue;
-    const float2* ce = T + (((size_t)tile * 2 + 0) * Wh + k) * BOX;
-    const float2* co = T + (((size_t)tile * 2 + 1) * Wh + k) * BOX;
-    for (int n = tid; n < BOX; n += NT) { be[n] = ce[n]; bo[n] = co[n]; }
+  for (int base = g; base < ntiles_max; base += FRC_GROUPS * FRC_TB) {
+    {
+      const int tile = base + (q >> 1) * FRC_GROUPS;
+      const int ty = tile / tmax_x, tx = tile % tmax_x;
+      const bool ok = tile < ntiles_max && ty < info->tiles_y && tx < info->tiles_x;
+      if (gt == 0 && (q & 1) == 0) live[q >> 1] = ok ? 1 : 0;
+      const float2* col = T + (((size_t)tile * 2 + (q & 1)) * Wh + k) * BOX;
+#pragma unroll
+      for (int i = 0; i < BOX / TPC; ++i) {
+        float2 v = make_float2(0.f, 0.f);
+        if (ok) v = col[gt + i * TPC];
+        buf[q][gt + i * TPC] = v;
+      }
+    }
     __syncthreads();
-    wg_fft<BOX, -1, NT>(be, twB, tid);
-    wg_fft<BOX, -1, NT>(bo, twB, tid);
+    wg_fft<BOX, -1, TPC>(buf[q], twB, gt);
+#pragma unroll
     for (int i = 0; i < PER; ++i) {
-      int n = tid + i * NT;
-      float2 E = be[n], O = bo[n];
-      float num = E.x * O.x + E.y * O.y;
-      float den = sqrtf((O.x * O.x + O.y * O.y) * (E.x * E.x + E.y * E.y));
-      acc[i] += (double)(num / den);
+      const int n = tid + i * NT;
+      for (int j = 0; j < FRC_TB; ++j) {
+        if (!live[j]) continue;
+        const float2 E = buf[2 * j][n], O = buf[2 * j + 1][n];
+        const float num = E.x * O.x + E.y * O.y;
+        const float den = sqrtf((O.x * O.x + O.y * O.y) * (E.x * E.x + E.y * E.y));
+        acc[i] += (double)(num / den);
+      }
     }
     __syncthreads();
   }
   double* dst = part + (size_t)g * BOX * Wh;
+#pragma unroll
   for (int i = 0; i < PER; ++i) dst[(size_t)(tid + i * NT) * Wh + k] = acc[i];
 }
 
@@ -2050,10 +2090,10 @@ int zr_stage_frc(zr_stage_state* s, hipStream_t st) {
   switch (B) {
 #define ZR_FRC(BX)                                                                                            \
   case BX:                                                                                                    \
-    hipLaunchKernelGGL(k_frc_rows<BX>, dim3(nt * 2 * (BX / 4)), dim3(256), 0, st, s->even, s->odd, s->info, s->W, \
+    hipLaunchKernelGGL(k_frc_rows<BX>, dim3(nt * 2 * (BX / frc_rows_per_block(BX))), dim3(256), 0, st, s->even, s->odd, s->info, s->W, \
                        s->tmax_x, step, s->twB2, s->twB, s->hann, s->Tfrc);                                   \
     ZS_HIP(hipGetLastError());                                                                                \
-    hipLaunchKernelGGL(k_frc_cols<BX>, dim3((BX / 2 + 1) * FRC_GROUPS), dim3(256), 0, st, s->Tfrc, s->info,     \
+    hipLaunchKernelGGL(k_frc_cols<BX>, dim3((BX / 2 + 1) * FRC_GROUPS), dim3(frc_cols_nt(BX)), 0, st, s->Tfrc, s->info, \
                        s->tmax_x, nt, s->twB, s->frc_part);                                                  \
     ZS_HIP(hipGetLastError());                                                                                \
     hipLaunchKernelGGL(k_frc_reduce, dim3(256), dim3(256), 0, st, s->frc_part, (size_t)BX * (BX / 2 + 1),      \
