@@ -1206,25 +1206,32 @@ __global__ __launch_bounds__(256) void k_peaks(const float* __restrict__ windows
       Tm[e] = acc;
     }
     __syncthreads();
-    // Y[s][t] = sum_p K(s - P p) T[p][t];  argmax |Y|^2, first in C order
+    // Y[s][t] = sum_p K(s - P p) T[p][t];  argmax |Y|^2, first in C order.
+    // With s = P a + b (a < 16, b < P) the table index is P ((a - q) mod 16) + b:
+    // the 16 entries of residue b sit in registers while its 16 rows a are formed
     float best = -1.f;
     int bidx = 0x7fffffff;
     for (int t = tid; t < G; t += 256) {
       float2 tc[16];
-      for (int q = 0; q < 16; ++q) tc[q] = Tm[q * G + t];
-      for (int s = 0; s < G; ++s) {
-        float2 acc = make_float2(0.f, 0.f);
 #pragma unroll
-        for (int q = 0; q < 16; ++q) {
-          int ki = s - P * q;
-          ki += (ki < 0) ? G : 0;
-          float2 kv = Ks[ki];
-          acc.x += kv.x * tc[q].x - kv.y * tc[q].y;
-          acc.y += kv.x * tc[q].y + kv.y * tc[q].x;
+      for (int q = 0; q < 16; ++q) tc[q] = Tm[q * G + t];
+      for (int b = 0; b < P; ++b) {
+        float2 kb[16];
+#pragma unroll
+        for (int j = 0; j < 16; ++j) kb[j] = Ks[P * j + b];
+#pragma unroll
+        for (int a = 0; a < 16; ++a) {
+          float2 acc = make_float2(0.f, 0.f);
+#pragma unroll
+          for (int q = 0; q < 16; ++q) {
+            const float2 kv = kb[(a - q) & 15];
+            acc.x += kv.x * tc[q].x - kv.y * tc[q].y;
+            acc.y += kv.x * tc[q].y + kv.y * tc[q].x;
+          }
+          const float a2 = acc.x * acc.x + acc.y * acc.y;
+          const int id = (P * a + b) * G + t;
+          if (a2 > best || (a2 == best && id < bidx)) { best = a2; bidx = id; }
         }
-        float a2 = acc.x * acc.x + acc.y * acc.y;
-        int id = s * G + t;
-        if (a2 > best || (a2 == best && id < bidx)) { best = a2; bidx = id; }
       }
     }
     for (int o = 32; o > 0; o >>= 1) {
