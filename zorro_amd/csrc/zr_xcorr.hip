@@ -1191,19 +1191,28 @@ __global__ __launch_bounds__(256) void k_peaks(const float* __restrict__ windows
     __syncthreads();
     for (int e = tid; e < G; e += 256) Ks[e] = ktab[e];
     __syncthreads();
-    // T[p][t] = sum_q c[p][q] K(t - P q)   (K is G-periodic)
-    for (int e = tid; e < 16 * G; e += 256) {
-      int pr_ = e / G, t = e % G;
-      float2 acc = make_float2(0.f, 0.f);
-      for (int q = 0; q < 16; ++q) {
-        float cv = Cf[(sy + pr_) * n2 + sx + q];
-        int ki = t - P * q;
-        ki += (ki < 0) ? G : 0;
-        float2 kv = Ks[ki];
-        acc.x += cv * kv.x;
-        acc.y += cv * kv.y;
+    // T[p][t] = sum_q c[p][q] K(t - P q)   (K is G-periodic); with t = P a + b
+    // the table index is P ((a - q) mod 16) + b: one (row p, residue b) per
+    // thread forms the 16 outputs a from registers
+    for (int e = tid; e < 16 * P; e += 256) {
+      const int pr_ = e / P, b = e % P;
+      float cr[16];
+      float2 kb[16];
+#pragma unroll
+      for (int q = 0; q < 16; ++q) cr[q] = Cf[(sy + pr_) * n2 + sx + q];
+#pragma unroll
+      for (int j = 0; j < 16; ++j) kb[j] = Ks[P * j + b];
+#pragma unroll
+      for (int a = 0; a < 16; ++a) {
+        float2 acc = make_float2(0.f, 0.f);
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+          const float2 kv = kb[(a - q) & 15];
+          acc.x += cr[q] * kv.x;
+          acc.y += cr[q] * kv.y;
+        }
+        Tm[pr_ * G + P * a + b] = acc;
       }
-      Tm[e] = acc;
     }
     __syncthreads();
     // Y[s][t] = sum_p K(s - P p) T[p][t];  argmax |Y|^2, first in C order.
