@@ -846,11 +846,12 @@ __global__ __launch_bounds__(64) void k_direct_positions(const float* __restrict
   if (threadIdx.x == 0) { info->status_x = 1; info->status_y = 1; }
 }
 
-__global__ __launch_bounds__(64) void k_solution_post(PostArgs a) {
-  const int lane = threadIdx.x;
+constexpr int POST_NT = 256;
+__global__ __launch_bounds__(POST_NT) void k_solution_post(PostArgs a) {
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
   const int N = a.N, M = a.M;
   __shared__ double cen[2];
-  if (lane == 0) {
+  if (tid == 0) {
     double s0 = 0.0, s1 = 0.0;
     for (int k = 0; k < N; ++k) {
       double ty = -a.pos[k], tx = -a.pos[N + k];
@@ -866,7 +867,7 @@ __global__ __launch_bounds__(64) void k_solution_post(PostArgs a) {
     cen[0] = a.centroid ? s0 / N : 0.0;
     cen[1] = a.centroid ? s1 / N : 0.0;
   }
-  {
+  if (wid == 0) {
     // error statistics (weights clipped to >= 1e-6)
     double sw = 0, swe = 0, swe2 = 0;
     for (int e = lane; e < M; e += 64) {
@@ -883,7 +884,7 @@ __global__ __launch_bounds__(64) void k_solution_post(PostArgs a) {
     }
   }
   __syncthreads();
-  for (int k = lane; k < N; k += 64) {
+  for (int k = tid; k < N; k += POST_NT) {
     double ty = a.trans_raw[2 * k] - cen[0], tx = a.trans_raw[2 * k + 1] - cen[1];
     a.trans[2 * k] = ty;
     a.trans[2 * k + 1] = tx;
@@ -918,26 +919,35 @@ __global__ __launch_bounds__(64) void k_solution_post(PostArgs a) {
       for (int i = 0; i < 3; ++i) { f.u[i] = 0.f; f.v[i] = 0.f; }
     }
     a.fs[k] = f;
-    // per-frame error estimates with the reference's (J, J-1) indexing
-    // padded[i][j] holds the equation (i, j) for frame pair (i, j), i < j
+  }
+  // per-frame error estimates with the reference's (J, J-1) indexing:
+  // padded[i][j] holds the equation (i, j) for frame pair (i, j), i < j.  One
+  // wave per frame, the equations split over its lanes (fixed reduction order)
+  for (int k = wid; k < N; k += POST_NT / 64) {
     double nx = 0, ny = 0, dn = 0;
-    for (int e = 0; e < M; ++e) {
-      int2 ed = a.meas[e];
-      double w = fmax(a.weights[e], 1e-6);
-      bool row = (ed.x == k);                           // padded[k][:]
-      bool col = (ed.y == ((k - 1 + N) % N));           // padded[:][k-1] (k=0 -> last column)
+    const int kc = (k - 1 + N) % N;
+    for (int e = lane; e < M; e += 64) {
+      const int2 ed = a.meas[e];
+      const bool row = (ed.x == k);                     // padded[k][:]
+      const bool col = (ed.y == kc);                    // padded[:][k-1] (k=0 -> last column)
       if (row || col) {
-        double c = (row ? 1.0 : 0.0) + (col ? 1.0 : 0.0);
+        const double w = fmax(a.weights[e], 1e-6);
+        const double c = (row ? 1.0 : 0.0) + (col ? 1.0 : 0.0);
         nx += c * w * fabs(a.resid[2 * e + 1]);
         ny += c * w * fabs(a.resid[2 * e]);
         dn += c * w;
       }
     }
-    a.err_frames[2 * k] = dn > 0.0 ? ny / dn : 0.0;
-    a.err_frames[2 * k + 1] = dn > 0.0 ? nx / dn : 0.0;
+    nx = wave_sum_d(nx);
+    ny = wave_sum_d(ny);
+    dn = wave_sum_d(dn);
+    if (lane == 0) {
+      a.err_frames[2 * k] = dn > 0.0 ? ny / dn : 0.0;
+      a.err_frames[2 * k + 1] = dn > 0.0 ? nx / dn : 0.0;
+    }
   }
   __syncthreads();
-  if (lane == 0) {
+  if (tid == 0) {
     // FRC crop limits (zorro/zorro.py:2264-2274) and tile counts
     double mny = INFINITY, mxy = -INFINITY, mnx = INFINITY, mxx = -INFINITY;
     bool any = false;
@@ -2004,7 +2014,7 @@ int zr_stage_solve(zr_stage_state* s, const float* shifts, const float* corr, co
     ZS_HIP(hipGetLastError());
     PostArgs q;
     fill_post_args(s, q);
-    hipLaunchKernelGGL(k_solution_post, dim3(1), dim3(64), 0, st, q);
+    hipLaunchKernelGGL(k_solution_post, dim3(1), dim3(POST_NT), 0, st, q);
     ZS_HIP(hipGetLastError());
     s->have_solution = true;
     return ZR_OK;
@@ -2034,7 +2044,7 @@ int zr_stage_solve(zr_stage_state* s, const float* shifts, const float* corr, co
   ZS_HIP(hipGetLastError());
   PostArgs q;
   fill_post_args(s, q);
-  hipLaunchKernelGGL(k_solution_post, dim3(1), dim3(64), 0, st, q);
+  hipLaunchKernelGGL(k_solution_post, dim3(1), dim3(POST_NT), 0, st, q);
   ZS_HIP(hipGetLastError());
   s->have_solution = true;
   return ZR_OK;
