@@ -166,3 +166,41 @@ def test_gpu_facade_xcorr_and_solver():
     assert np.array_equal(ed["translations"], reg.errorDictList[-1]["translations"])
     assert ed["M"] == int(tri.sum()) and np.array_equal(ed["corrTriMat"], reg.errorDictList[-1]["corrTriMat"])
     reg.close()
+
+
+@pytest.mark.gpu
+def test_gpu_long_row_mask_solves_to_the_optimum():
+    """Regression: rows reaching up to 9 frames apart repeat frame spans in the
+    rot90 equations; before parallel edges were summed per span, the simplex
+    stopped without proving optimality on this mask.  The GPU objective must
+    not exceed the oracle's exact optimum on the same trimats."""
+    from zorro_amd.synth import make_movie
+    n, size = 10, 256
+    print(f"[gpu triIndices] long rows n={n} {size}^2", flush=True)
+    frames, _ = make_movie(n, size, size, seed=82, dose=4.0)
+    rng = np.random.default_rng(83)
+    tri = np.zeros((n, n), bool)
+    for I in range(n - 1):
+        tri[I, I + 1:I + 1 + int(rng.integers(1, n - I))] = True
+    A = O.build_equations(tri[:-1, 1:])
+    assert len({tuple(r) for r in A.astype(int)}) < A.shape[0]       # parallel equations present
+    kw = _kw("tri", size)
+    plan = MoviePlan(make_params(**kw), n, size, size, triIndices=tri)
+    try:
+        plan.set_frames(frames)
+        plan.xcorr()
+        plan.solve()
+        tm, sol, info = plan.trimats(), plan.solution(), plan.solver_info()
+    finally:
+        plan.close()
+    assert info["status_y"] == 1 and info["status_x"] == 1
+    ed = O.shifts_solver(tm["shiftsTriMat"], tm["corrTriMat"], tm["peaksigTriMat"], O.Params(**kw), (size, size))
+    w = np.asarray(ed["weights"])
+    S = tm["shiftsTriMat"][:-1, 1:, :]
+    msk = tm["corrTriMat"][:-1, 1:].astype(bool)
+    for ax in (0, 1):
+        b = np.ravel(S[msk, ax]).astype(np.float64)
+        og = float(np.sum(w * np.abs(A @ sol["relativeEst"][:, ax] - b)))
+        oo = float(np.sum(w * np.abs(A @ ed["relativeEst"][:, ax] - b)))
+        print(f"[gpu triIndices] long rows axis {ax}: objective gpu {og:.6e} oracle {oo:.6e}", flush=True)
+        assert og <= oo * (1 + 1e-9) + 1e-12
