@@ -139,8 +139,7 @@ def load_config(reg, path: str) -> None:
         except (ValueError, json.JSONDecodeError):
             continue                # unreadable values are skipped, as the reference does
         setattr(reg, key, v)
-    reg.errorDictList = []
-    k = 0
+    k = 0                       # appended to the existing list, as loadConfig does (zorro/zorro.py:3778-3795)
     while cp.has_section(f"errorDict{k}") and cp.has_option(f"errorDict{k}", "keyList"):
         name = f"errorDict{k}"
         ed = {}
