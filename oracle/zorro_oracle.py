@@ -74,6 +74,8 @@ PINS = {
     "locate_peak (preShift roll)": "two-peak whole-map KAT: unrolled vs rolled disc; brute-force disc statistics",
     "hotpix_filter/hotpix_stats": "PSF kernel sum/ratios; fit of a binned normal = (mu - 1/2, sigma); planted "
                                   "singletons/clusters vs explicit PSF and neighbour means; cut-offs on a large stack",
+    "xcorrnm2_tri (triIndices)": "a mask equal to the 'diag' band reproduces triMode 'diag' exactly "
+                                 "(tests/test_tri_indices.py)",
 }
 
 
