@@ -1398,7 +1398,6 @@ struct DoseArgs {
 __global__ __launch_bounds__(256) void k_dose_params(DoseArgs a) {
   __shared__ double opti[2048];
   __shared__ double red[4];
-  __shared__ int redi[4];
   __shared__ double s_dpf;
   const int tid = threadIdx.x;
   double s = 0.0;
@@ -1423,19 +1422,15 @@ __global__ __launch_bounds__(256) void k_dose_params(DoseArgs a) {
   __syncthreads();
   if (tid == 0) opti[0] = opti[1] * opti[1];
   __syncthreads();
-  for (int j = 0; j < a.N; ++j) {
+  // one wave per frame: the last radial index closer to the frame's end dose
+  const int lane = tid & 63, wid = tid >> 6;
+  for (int j = wid; j < a.N; j += 4) {
     const double d1 = a.accum[j + 1], d0 = a.accum[j];
     int last = -1;
-    for (int k = tid; k < nq; k += 256)
+    for (int k = lane; k < nq; k += 64)
       if (fabs(d1 - opti[k]) < fabs(d0 - opti[k])) last = k;
     for (int o = 32; o > 0; o >>= 1) last = max(last, __shfl_xor(last, o, 64));
-    if ((tid & 63) == 0) redi[tid >> 6] = last;
-    __syncthreads();
-    if (tid == 0) {
-      int l = max(max(redi[0], redi[1]), max(redi[2], redi[3]));
-      a.thresq[j] = l >= 0 ? l * qs : NAN;
-    }
-    __syncthreads();
+    if (lane == 0) a.thresq[j] = last >= 0 ? last * qs : NAN;
   }
 }
 
