@@ -87,7 +87,20 @@ typedef struct zr_params {
    * triMode tri/diag/auto/first; windows of maxShift + 8 + Gaussian radius
    * (rounded up to 8, at most 64) around each pair's disc */
   int pre_shift;
+  /* filterMode steps applied after the registration, in list order
+   * (zorro/zorro.py:1689-1711): ZR_FILTER_* values in filters[0 .. n_filters).
+   * n_filters = 0 with do_dose = 1 means the single step ZR_FILTER_DOSE; do_dose
+   * is set by zr_movie_create when a DOSE or DOSENORM step is present. */
+  int n_filters;
+  int filters[4];
 } zr_params;
+
+/* filterMode steps (zorro/zorro.py:1692-1711).  DOSE: doseFilter(normalize=False),
+ * DOSENORM: doseFilter(normalize=True) (each rewrites filtSum from the aligned
+ * frames); GAUSSLP: filtSum = scipy.ndimage.gaussian_filter(filtSum, 3.0), on a
+ * copy of imageSum when no filtSum exists yet.  'background' (a Powell fit of a
+ * 2D Gaussian, zorro/zorro_util.py:497-549) is not built. */
+enum { ZR_FILTER_DOSE = 1, ZR_FILTER_DOSENORM = 2, ZR_FILTER_GAUSSLP = 3 };
 
 /* Element types of raw frames (MRC modes 2 float32, 0 uint8 counts, 6 uint16,
  * 101 4-bit counts packed two per byte with the first pixel in the low nibble
@@ -174,6 +187,19 @@ int zr_set_frc_translations(zr_movie* mv, const double* trans, int n);
  * Replaces doseFilter (zorro/zorro.py:2349-2494). */
 int zr_dose_filter(zr_movie* mv);
 
+/* One filterMode step (ZR_FILTER_*) on filtSum, as the loop of alignImageStack
+ * applies it (zorro/zorro.py:1692-1711): DOSE / DOSENORM run doseFilter with
+ * normalize False / True (zorro/zorro.py:2349-2494; the dose stage also forms
+ * imageSum and the half sums), GAUSSLP the 3-pixel Gaussian low-pass
+ * (scipy.ndimage.gaussian_filter, reflect edges, radius 12; :1707-1711) after
+ * zr_apply_shifts or a dose step. */
+int zr_filter(zr_movie* mv, int step);
+
+/* applyShifts (when no dose step forms the sums) followed by every filterMode
+ * step of the parameters in order: the part of alignImageStack after the
+ * solver (zorro/zorro.py:1672-1711) without the FRC. */
+int zr_post_filters(zr_movie* mv);
+
 /* Every stage enabled by the parameters, in alignImageStack order
  * (zorro/zorro.py:1614-1712). */
 int zr_align(zr_movie* mv);
@@ -222,6 +248,13 @@ int zr_equation_count(zr_movie* mv, int* m);
  * optimality. */
 int zr_get_solution(zr_movie* mv, double* translations, double* relative_est, double* weights, double* logistic,
                     double* error, double* error_stats, double* error_xy_frames);
+/* cdfLogisticCurve's histogram of the positive peak significances
+ * (zorro/zorro.py:1991-1998, errorDict keys pdfPeaks, cdfPeaks, hSigma):
+ * counts as float32, their normalised cumulative sum and the left bin edges
+ * (float32, numpy's binning), int(n/7) bins for n positive values.  Any of the
+ * three arrays may be NULL; each holds at least `cap` floats; *nbins receives
+ * the bin count (0 when weightMode is not 'autologistic').  Call after zr_solve. */
+int zr_get_peaksig_hist(zr_movie* mv, float* pdf, float* cdf, float* hsigma, int cap, int* nbins);
 /* Solver diagnostics: info6 = {positive peak significances, histogram bins,
  * status y, status x (1 optimal vertex, 2 pivot limit, -1 disconnected),
  * simplex pivots y, pivots x}. */

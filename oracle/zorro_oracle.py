@@ -49,7 +49,7 @@ __all__ = [
     "logistic_fit", "cdf_logistic_curve", "l1_solve", "exact_vertex",
     "shifts_solver", "lanczos_kernel", "apply_shifts", "rotmean",
     "hann_apodization", "frc_crop_limits", "tiled_frc", "lazy_frc",
-    "dose_filter", "orient_gain", "apply_gain", "bin_stack", "align_image_stack", "PINS",
+    "dose_filter", "gauss_lowpass", "filter_steps", "orient_gain", "apply_gain", "bin_stack", "align_image_stack", "PINS",
     "HOTPIX_DEFAULTS", "hotpix_psf_kernel", "hotpix_norm_objective", "hotpix_stats", "hotpix_filter",
     "even_odd_frc",
 ]
@@ -76,6 +76,12 @@ PINS = {
                                   "singletons/clusters vs explicit PSF and neighbour means; cut-offs on a large stack",
     "xcorrnm2_tri (triIndices)": "a mask equal to the 'diag' band reproduces triMode 'diag' exactly "
                                  "(tests/test_tri_indices.py)",
+    "shifts_solver (error analysis)": "consistent system -> every error 0; one planted equation: errorXY, mean/std "
+                                      "and per-frame errorX counted over the pair list (row J, column J-1)",
+    "hann_apodization": "closed-form values at r = 0, R/4, R/2, R and outside the circle; radial, decreasing",
+    "cdf_logistic_curve": "hand-counted 3-bin histogram, CDF and left edges of a fixed sample",
+    "dose_filter (normalize)": "constant frames: DC divided by N+1; cosine gain g of the plain filter -> g/(g+1)",
+    "gauss_lowpass": "explicit symmetric-padded separable sum of the normalised radius-12 taps; constant -> constant",
 }
 
 
@@ -757,9 +763,10 @@ def lazy_frc(aligned, trans, box=512, overlap=0.5):
     return r[:int(box / 2)], dict(evenSum=even, oddSum=odd, FRC2D=frc2, n_tiles=nt)
 
 
-def dose_filter(aligned, imageSum, p: Params, shapePadded):
-    """Unblur-style dose filter of the aligned frames, ``normalize=False``.
-    ``zorro/zorro.py:2349-2494``.  Returns (filtSum, info)."""
+def dose_filter(aligned, imageSum, p: Params, shapePadded, normalize: bool = False):
+    """Unblur-style dose filter of the aligned frames.  ``zorro/zorro.py:2349-2494``;
+    ``normalize=True`` divides the filtered sum by (sum of the per-frame filters
+    + 1) before the inverse (``:2472-2486``).  Returns (filtSum, info)."""
     A, B, Cc, order = (float(p.doseFiltParam[i]) for i in (1, 2, 3, 4))
     if p.doseFiltParam[0] is not None:
         raise NotImplementedError("doseFiltParam[0] != None raises NameError in the reference (Q11)")
@@ -787,6 +794,7 @@ def dose_filter(aligned, imageSum, p: Params, shapePadded):
     opti[0] = opti[1] ** 2
     h, w = imageSum.shape
     acc = np.zeros([N, Mw], dtype=np.complex128)
+    mag = np.zeros([N, Mw])
     thresQs = []
     for J in range(m):
         d1, d0 = acc_dose[J + 1], acc_dose[J]
@@ -799,8 +807,29 @@ def dose_filter(aligned, imageSum, p: Params, shapePadded):
         thres = np.exp(-(q / tq) ** order)
         padded = np.pad(aligned[J].astype(np.float64), ((0, N - h), (0, Mw - w)), mode="symmetric")
         acc += np.fft.fft2(padded) * thres * filt
+        mag += thres * filt                                                # :2479-2480
+    if normalize:
+        acc = acc * (1.0 / (mag + 1.0))                                    # :2482-2486
     filt_sum = np.abs(raw_ifft2(acc)[:h, :w])
     return filt_sum, dict(dosePerFrame=dpf, thresQ=np.array(thresQs), accumDose=acc_dose)
+
+
+def gauss_lowpass(img):
+    """filterMode 'gausslp': ``scipy.ndimage.gaussian_filter(filtSum, 3.0)``
+    (reflect edges, truncate 4).  ``zorro/zorro.py:1707-1711``."""
+    return scipy.ndimage.gaussian_filter(np.asarray(img, np.float64), 3.0)
+
+
+def filter_steps(filterMode):
+    """The filterMode tokens alignImageStack applies after the registration, in
+    order (``zorro/zorro.py:1692-1711``): lower-cased, spaces removed, split on
+    commas; tokens other than dose/dosenorm/background/gausslp do nothing there."""
+    if not filterMode:
+        return []
+    toks = filterMode.lower().replace(" ", "").split(",")
+    if "background" in toks:
+        raise NotImplementedError("filterMode 'background' (backgroundEstimate's Powell fit) is outside the oracle")
+    return [t for t in toks if t in ("dose", "dosenorm", "gausslp")]
 
 
 # ---------------------------------------------------------------------------
@@ -1053,7 +1082,13 @@ def align_image_stack(images: np.ndarray, p: Params, mask: np.ndarray | None = N
         frc, info = lazy_frc(aligned, trans, p.frc_box, p.frc_overlap)
         res.update(FRC=frc, FRC2D=info["FRC2D"], evenSum=info["evenSum"], oddSum=info["oddSum"],
                    frc_tiles=info["n_tiles"])
-    if p.filterMode and "dose" in p.filterMode.lower().replace(" ", "").split(","):
-        fs, info = dose_filter(aligned, image_sum, p, p.shapePadded)
-        res.update(filtSum=fs, dose=info)
+    filt = None
+    for step in filter_steps(p.filterMode):                              # :1692-1711
+        if step in ("dose", "dosenorm"):
+            filt, info = dose_filter(aligned, image_sum, p, p.shapePadded, normalize=(step == "dosenorm"))
+            res.update(dose=info)
+        else:                                                            # gausslp
+            filt = gauss_lowpass(image_sum if filt is None else filt)
+    if filt is not None:
+        res.update(filtSum=filt)
     return res

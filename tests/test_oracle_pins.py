@@ -514,3 +514,151 @@ def test_preshift_rolled_disc_known_answer():
     disc = ((yy - 13) ** 2 + (xx + 10) ** 2) < m * m
     vals = C[disc]
     assert np.isclose(ps1, (C[c1] - vals.mean()) / vals.std())
+
+
+# ------------------------------------------------------- solver error analysis
+def _planted_solver_case(N, I0, J0, r, weights=None):
+    """Consistent integer positions, all tri pairs measured exactly except pair
+    (I0, J0) whose x shift is off by r; 'unweighted' or given weights."""
+    pos = np.cumsum(np.concatenate([[0.0], np.arange(1, N) % 3 + 1.0]))     # x positions, exact in float32
+    st = np.zeros((N, N, 2), np.float32)
+    for I in range(N):
+        for J in range(I + 1, N):
+            st[I, J, 1] = pos[J] - pos[I]
+            st[I, J, 0] = -(pos[J] - pos[I]) / 2
+    st[I0, J0, 1] += r
+    corr = np.triu(np.ones((N, N), np.float32), 1)
+    ps = np.triu(np.full((N, N), 10.0, np.float32), 1) if weights is None else weights
+    return pos, st, corr, ps
+
+
+def test_solver_error_analysis_consistent_system_is_error_free():
+    # zorro/zorro.py:1483-1547 on a consistent system: every error is zero
+    N = 6
+    pos, st, corr, ps = _planted_solver_case(N, 1, 3, 0.0)
+    ed = O.shifts_solver(st, corr, ps, O.Params(weightMode="unweighted", maxShift=40), (256, 256))
+    assert np.allclose(ed["relativeEst"][:, 1], np.diff(pos))
+    assert np.all(ed["errorXY"] == 0) and ed["mean_errorNorm"] == 0 and ed["std_errorNorm"] == 0
+    assert np.all(ed["errorX"] == 0) and np.all(ed["errorY"] == 0) and np.all(ed["errorUnraveled"] == 0)
+
+
+@pytest.mark.parametrize("I0,J0", [(1, 3), (0, 5), (2, 4)])
+def test_solver_error_analysis_planted_equation_known_answer(I0, J0):
+    """One bad equation (I0, J0), residual r, uniform weights w = 1: the L1
+    optimum keeps the true positions, so errorXY has the single row -r (A x - b),
+    errorNorm = |r| there, mean = r/M, std = sqrt(r^2/M - mean^2), and the
+    per-frame errorX[J] is the weighted mean of |errorX| over the pairs (J, c)
+    and the pairs (c, J-1) (the reference's column J-1, which wraps to the last
+    column for J = 0), counted here by walking the pair list."""
+    N, r = 6, 0.75
+    pos, st, corr, ps = _planted_solver_case(N, I0, J0, r)
+    ed = O.shifts_solver(st, corr, ps, O.Params(weightMode="unweighted", maxShift=40), (256, 256))
+    assert np.allclose(ed["relativeEst"][:, 1], np.diff(pos))
+    pairs = [(I, J) for I in range(N - 1) for J in range(I + 1, N)]          # row-major trimat order
+    M = len(pairs)
+    e = pairs.index((I0, J0))
+    exy = np.zeros((M, 2))
+    exy[e, 1] = -r
+    assert np.allclose(ed["errorXY"], exy, atol=1e-12)
+    mean = r / M
+    assert np.isclose(ed["mean_errorNorm"], mean)
+    assert np.isclose(ed["std_errorNorm"], math.sqrt(r * r / M - mean * mean))
+    # errorUnraveled[I, J-1] holds pair (I, J)
+    eun = np.zeros((N - 1, N - 1))
+    eun[I0, J0 - 1] = r
+    assert np.allclose(ed["errorUnraveled"], eun)
+    for J in range(N):
+        row = [(I, K) for (I, K) in pairs if I == J]                         # padded row J
+        col_frame = (J - 1) % N                                              # padded column J-1 (wraps)
+        col = [(I, K) for (I, K) in pairs if K == col_frame]
+        den = len(row) + len(col)
+        num = r * (((I0, J0) in row) + ((I0, J0) in col))
+        assert np.isclose(ed["errorX"][J], num / den), (J, ed["errorX"][J], num / den)
+        assert ed["errorY"][J] == 0
+
+
+# ------------------------------------------------------------- Hann window
+def test_hann_apodization_closed_form_values():
+    # zorro/zorro_util.py:62-104 'hann': 0.5 (1 + cos(pi r / R)) inside r <= R, 0 outside,
+    # with the mesh arange(-box/2, box/2) (the centre at index box/2)
+    box = 16
+    w = O.hann_apodization([box, box])
+    c = box // 2
+    assert w.shape == (box, box)
+    assert w[c, c] == 1.0                                             # r = 0
+    assert np.isclose(w[c, c + 2], (2 + math.sqrt(2)) / 4)            # r = R/4: cos(pi/4)
+    assert np.isclose(w[c + 4, c], 0.5)                               # r = R/2
+    assert np.isclose(w[c - 4, c - 4], 0.5 * (1 + math.cos(math.pi * math.sqrt(32) / 8)))
+    assert w[0, c] == 0.0                                             # r = R (on the circle)
+    assert w[0, 0] == 0.0 and w[1, 1] == 0.0                          # outside the circle
+    assert np.allclose(w, w.T)                                        # radial
+    assert np.all(w[c, c:] >= 0) and np.all(np.diff(w[c, c:]) < 0)   # decreasing from the centre
+
+
+# ----------------------------------------------------- logistic histogram / CDF
+def test_cdf_logistic_curve_hand_counted_histogram():
+    # zorro/zorro.py:1988-1998: positive peak significances only, int(n/7) bins,
+    # float32 counts, normalised cumulative sum, left bin edges
+    ps = np.zeros((8, 8), np.float32)
+    vals = [1.0] * 10 + [2.0] * 5 + [4.0] * 6                 # 21 positive values -> 3 bins over [1, 4]
+    iu = np.triu_indices(8, 1)
+    ps[iu[0][:21], iu[1][:21]] = vals
+    ps[iu[0][21], iu[1][21]] = -3.0                           # negatives and zeros are left out
+    fit = O.cdf_logistic_curve(ps)
+    assert fit["pdfPeaks"].dtype == np.float32
+    assert list(fit["pdfPeaks"]) == [10.0, 5.0, 6.0]          # [1,2) [2,3) [3,4]
+    assert np.allclose(fit["cdfPeaks"], [10 / 21, 15 / 21, 1.0])
+    assert list(fit["hSigma"]) == [1.0, 2.0, 3.0]
+    assert 1.0 <= fit["peaksigThres"] <= 4.0
+
+
+# ------------------------------------------------------- dosenorm / gausslp
+def test_dose_filter_normalize_constant_image_divides_dc_by_n_plus_1():
+    # at q = 0 every frame's filter is 1 (infinite critical dose, no cut-off), so the
+    # normalisation 1/(sum + 1) is 1/(N + 1) there (zorro/zorro.py:2472-2486)
+    n, h, w = 4, 32, 32
+    imgs = np.full((n, h, w), 2.5)
+    p = O.Params(pixelsize=0.1, filterMode="dosenorm")
+    fs0, _ = O.dose_filter(imgs, imgs.sum(0), p, (h, w))
+    fs1, _ = O.dose_filter(imgs, imgs.sum(0), p, (h, w), normalize=True)
+    assert np.allclose(fs0, n * 2.5 * h * w)
+    assert np.allclose(fs1, n * 2.5 * h * w / (n + 1))
+
+
+def test_dose_filter_normalize_cosine_gain_is_g_over_g_plus_1():
+    # one frame c + a cos(k x): the plain filter passes the cosine with gain g
+    # (read off the plain result), the normalised one with g / (g + 1) and the
+    # DC term with 1/2: a relation between the two modes, not a restated formula
+    h = w = 64
+    yy, xx = np.mgrid[0:h, 0:w]
+    c, a = 10.0, 1.0
+    img = c + a * np.cos(2 * np.pi * 5 * xx / w)
+    p = O.Params(pixelsize=0.1, filterMode="dose")
+    fs0, _ = O.dose_filter(img[None], img, p, (h, w))
+    fs1, _ = O.dose_filter(img[None], img, p, (h, w), normalize=True)
+    g = (fs0.max() - fs0.min()) / (2 * a * h * w)
+    assert 0.01 < g < 1.0
+    assert np.isclose(fs1.mean(), c * h * w / 2)
+    assert np.isclose((fs1.max() - fs1.min()) / (2 * a * h * w), g / (g + 1))
+
+
+def test_gauss_lowpass_equals_explicit_reflect_separable_sum():
+    # scipy.ndimage.gaussian_filter(img, 3.0): truncate 4 -> radius 12, 'reflect'
+    # = numpy 'symmetric' padding, normalised taps exp(-x^2/18)
+    img = RNG.standard_normal((40, 33))
+    x = np.arange(-12, 13)
+    k = np.exp(-x * x / 18.0)
+    k /= k.sum()
+    pad = np.pad(img, 12, mode="symmetric")
+    tmp = sum(k[i] * pad[i:i + 40, :] for i in range(25))
+    ref = sum(k[i] * tmp[:, i:i + 33] for i in range(25))
+    assert np.allclose(O.gauss_lowpass(img), ref, atol=1e-12)
+    assert np.allclose(O.gauss_lowpass(np.full((30, 30), 7.0)), 7.0)
+
+
+def test_filter_steps_follow_token_order():
+    assert O.filter_steps("Dose, gaussLP") == ["dose", "gausslp"]
+    assert O.filter_steps("hot,dosenorm") == ["dosenorm"]
+    assert O.filter_steps(None) == []
+    with pytest.raises(NotImplementedError):
+        O.filter_steps("dose,background")

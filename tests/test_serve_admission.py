@@ -122,3 +122,52 @@ def test_estimate_covers_the_workspaces():
     spectra = 40 * 2049 * 4096 * 8                    # full-height half spectra kept for the dose filter
     assert est > spectra + 780 * 1537 * 80 * 8
     assert est < 16e9
+
+
+class SlowAlignPlan(FakePlan):
+    def align(self):
+        time.sleep(0.2)
+
+
+def test_cancelled_request_is_dropped_and_released_once(monkeypatch):
+    monkeypatch.setattr(S, "MoviePlan", SlowAlignPlan)
+    FakePlan.live = FakePlan.created = FakePlan.running = FakePlan.peak_running = 0
+    srv = S.MovieServer(hbm_budget=1 << 40, max_inflight=1, return_sums="none")
+    a = srv.submit(movie(4, 64, 64, 1), shapePadded=(64, 64), **KW)
+    b = srv.submit(movie(4, 64, 64, 2), shapePadded=(64, 64), **KW)
+    assert b.cancel()                           # still queued behind a: cancellable
+    c = srv.submit(movie(4, 64, 64, 3), shapePadded=(64, 64), **KW)
+    assert np.all(a.result(timeout=30).translations == 1)
+    assert np.all(c.result(timeout=30).translations == 3)
+    assert b.cancelled()
+    assert not a.cancel()                        # a finished: too late to cancel
+    with srv._lock:
+        assert srv._inflight == 0 and srv.stats["admitted"] == 2
+    srv.close()
+    assert FakePlan.live == 0
+
+
+class SlowCreatePlan(FakePlan):
+    def __init__(self, params, n, h, w, device=0):
+        if h == 128:
+            time.sleep(0.6)                      # a multi-GB plan being allocated
+        super().__init__(params, n, h, w, device)
+
+
+def test_slow_plan_creation_does_not_hold_up_other_movies(monkeypatch):
+    monkeypatch.setattr(S, "MoviePlan", SlowCreatePlan)
+    FakePlan.live = FakePlan.created = FakePlan.running = FakePlan.peak_running = 0
+    srv = S.MovieServer(hbm_budget=1 << 40, max_inflight=2, return_sums="none")
+    assert srv.warm((4, 64, 64), count=1, shapePadded=(64, 64), **KW) == 1
+    t0 = time.perf_counter()
+    big = srv.submit(movie(4, 128, 128, 5), shapePadded=(128, 128), **KW)    # admitted first, plan created slowly
+    time.sleep(0.05)
+    small = srv.submit(movie(4, 64, 64, 6), shapePadded=(64, 64), **KW)     # the warmed plan
+    r = small.result(timeout=30)
+    t_small = time.perf_counter() - t0
+    assert np.all(r.translations == 6)
+    assert t_small < 0.45, t_small               # done while the other plan is still being created
+    assert not big.done()
+    assert np.all(big.result(timeout=30).translations == 5)
+    srv.close()
+    assert FakePlan.live == 0

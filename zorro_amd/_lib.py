@@ -18,6 +18,7 @@ ZR_B_NONE, ZR_B_CONV, ZR_B_OPTI = 0, 1, 2
 ZR_SHIFT_INTEGER, ZR_SHIFT_LANCZOS = 0, 1
 ZR_W_AUTOLOGISTIC, ZR_W_LOGISTIC, ZR_W_CORR, ZR_W_UNWEIGHTED, ZR_W_NORM = 0, 1, 2, 3, 4
 ZR_DTYPE_F32, ZR_DTYPE_U8, ZR_DTYPE_U16, ZR_DTYPE_U4, ZR_DTYPE_I16 = 0, 1, 2, 3, 4
+ZR_FILTER_DOSE, ZR_FILTER_DOSENORM, ZR_FILTER_GAUSSLP = 1, 2, 3
 
 
 class ZrParams(C.Structure):
@@ -41,6 +42,7 @@ class ZrParams(C.Structure):
         ("hot_max_sigma", C.c_double), ("hot_psf_k2", C.c_int), ("hot_cutoff_lower", C.c_double),
         ("hot_cutoff_upper", C.c_double), ("hot_fixed_cutoffs", C.c_int), ("hot_neighbour_thres", C.c_double), ("hot_mu_sigma", C.c_double * 2),
         ("pre_shift", C.c_int),
+        ("n_filters", C.c_int), ("filters", C.c_int * 4),
     ]
 
 
@@ -72,6 +74,9 @@ SIGNATURES = {
     "zr_apply_shifts": (C.c_int, [_P, _P]),
     "zr_frc": (C.c_int, [_P]),
     "zr_dose_filter": (C.c_int, [_P]),
+    "zr_filter": (C.c_int, [_P, C.c_int]),
+    "zr_post_filters": (C.c_int, [_P]),
+    "zr_get_peaksig_hist": (C.c_int, [_P, _F, _F, _F, C.c_int, _I]),
     "zr_align": (C.c_int, [_P]),
     "zr_synchronize": (C.c_int, [_P]),
     "zr_movie_use_graph": (C.c_int, [_P, C.c_int]),

@@ -361,6 +361,20 @@ static int create_movie(int device, const zr_params* p, int n_frames, int h, int
         if (tri_indices[(size_t)I * n_frames + J])
           return fail(ZR_ERR_UNSUPPORTED, "triIndices[%d][%d]: only pairs above the diagonal (I < J) are built", I, J);
   }
+  // filterMode steps (zorro/zorro.py:1689-1711): a legacy do_dose alone is the
+  // single 'dose' step; do_dose then marks the presence of any dose step
+  zr_params pn = *p;
+  if (pn.n_filters == 0 && pn.do_dose) { pn.n_filters = 1; pn.filters[0] = ZR_FILTER_DOSE; }
+  if (pn.n_filters < 0 || pn.n_filters > 4) return fail(ZR_ERR_ARG, "n_filters %d outside 0..4", pn.n_filters);
+  pn.do_dose = 0;
+  for (int i = 0; i < pn.n_filters; ++i) {
+    const int f = pn.filters[i];
+    if (f != ZR_FILTER_DOSE && f != ZR_FILTER_DOSENORM && f != ZR_FILTER_GAUSSLP)
+      return fail(ZR_ERR_UNSUPPORTED, "filter step %d: %d is not a built filterMode step", i, f);
+    if (f != ZR_FILTER_GAUSSLP) pn.do_dose = 1;
+  }
+  for (int i = pn.n_filters; i < 4; ++i) pn.filters[i] = 0;
+  p = &pn;
   zr_movie* mv = new zr_movie();
   if (tri_indices) mv->tri_mask.assign(tri_indices, tri_indices + (size_t)n_frames * n_frames);
   g_alloc_bytes = 0;
@@ -975,26 +989,45 @@ extern "C" int zr_set_frc_translations(zr_movie* mv, const double* trans, int n)
   return rc ? fail(rc, "%s", zr_stage_error()) : ZR_OK;
 }
 
-extern "C" int zr_dose_filter(zr_movie* mv) {
+extern "C" int zr_filter(zr_movie* mv, int step) {
   if (!mv) return fail(ZR_ERR_ARG, "NULL movie");
   ZR_HIP(hipSetDevice(mv->device));
-  hipEvent_t t0 = mv->timer.begin(mv->stream);
-  if (mv->spectral && !mv->have_xcorr) return fail(ZR_ERR_STATE, "dose filter needs the spectra of zr_xcorr");
-  int rc = zr_stage_dose(mv->st, mv->frames, mv->S, mv->Wk, mv->stream);
-  mv->timer.end(ZR_STAGE_DOSE, t0, mv->stream);
+  int rc = ZR_OK;
+  if (step == ZR_FILTER_DOSE || step == ZR_FILTER_DOSENORM) {
+    if (mv->spectral && !mv->have_xcorr) return fail(ZR_ERR_STATE, "dose filter needs the spectra of zr_xcorr");
+    hipEvent_t t0 = mv->timer.begin(mv->stream);
+    rc = zr_stage_dose(mv->st, mv->frames, mv->S, mv->Wk, step == ZR_FILTER_DOSENORM ? 1 : 0, mv->stream);
+    mv->timer.end(ZR_STAGE_DOSE, t0, mv->stream);
+  } else if (step == ZR_FILTER_GAUSSLP) {
+    rc = zr_stage_gausslp(mv->st, mv->stream);
+  } else {
+    return fail(ZR_ERR_ARG, "filter step %d is not a ZR_FILTER_* value", step);
+  }
   return rc ? fail(rc, "%s", zr_stage_error()) : ZR_OK;
 }
 
-// the stages after the pair transform (peaks, solve, shifts/dose, FRC)
+extern "C" int zr_dose_filter(zr_movie* mv) { return zr_filter(mv, ZR_FILTER_DOSE); }
+
+extern "C" int zr_post_filters(zr_movie* mv) {
+  if (!mv) return fail(ZR_ERR_ARG, "NULL movie");
+  // a dose step forms the aligned sums and half sums itself; without one the
+  // frames are shifted and summed first (applyShifts, zorro/zorro.py:1672)
+  int rc;
+  bool summed = false;
+  for (int i = 0; i < mv->p.n_filters && !summed; ++i)
+    if (mv->p.filters[i] != ZR_FILTER_GAUSSLP) summed = true;
+  if (!summed && (rc = zr_apply_shifts(mv, nullptr))) return rc;
+  for (int i = 0; i < mv->p.n_filters; ++i)
+    if ((rc = zr_filter(mv, mv->p.filters[i]))) return rc;
+  return ZR_OK;
+}
+
+// the stages after the pair transform (peaks, solve, shifts/filters, FRC)
 static int align_tail(zr_movie* mv) {
   int rc;
   if ((rc = xcorr_part(mv, 2))) return rc;
   if ((rc = zr_solve(mv))) return rc;
-  if (mv->p.do_dose) {
-    if ((rc = zr_dose_filter(mv))) return rc;
-  } else if ((rc = zr_apply_shifts(mv, nullptr))) {
-    return rc;
-  }
+  if ((rc = zr_post_filters(mv))) return rc;
   if (mv->p.do_frc && (rc = zr_frc(mv))) return rc;
   return ZR_OK;
 }
@@ -1003,12 +1036,7 @@ static int align_sequence(zr_movie* mv) {
   int rc;
   if ((rc = zr_xcorr(mv))) return rc;
   if ((rc = zr_solve(mv))) return rc;
-  if (mv->p.do_dose) {
-    // the dose stage computes the aligned frames, sums and half sums itself
-    if ((rc = zr_dose_filter(mv))) return rc;
-  } else if ((rc = zr_apply_shifts(mv, nullptr))) {
-    return rc;
-  }
+  if ((rc = zr_post_filters(mv))) return rc;
   if (mv->p.do_frc && (rc = zr_frc(mv))) return rc;
   return ZR_OK;
 }
@@ -1153,6 +1181,14 @@ extern "C" int zr_get_solution(zr_movie* mv, double* translations, double* relat
   ZR_HIP(hipStreamSynchronize(mv->stream));
   int rc = zr_stage_get_solution(mv->st, translations, relative_est, weights, logistic, error, error_stats,
                                  error_xy_frames);
+  return rc ? fail(rc, "%s", zr_stage_error()) : ZR_OK;
+}
+
+extern "C" int zr_get_peaksig_hist(zr_movie* mv, float* pdf, float* cdf, float* hsigma, int cap, int* nbins) {
+  if (!mv) return fail(ZR_ERR_ARG, "NULL movie");
+  ZR_HIP(hipSetDevice(mv->device));
+  ZR_HIP(hipStreamSynchronize(mv->stream));
+  int rc = zr_stage_get_hist(mv->st, pdf, cdf, hsigma, cap, nbins);
   return rc ? fail(rc, "%s", zr_stage_error()) : ZR_OK;
 }
 

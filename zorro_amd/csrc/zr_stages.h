@@ -30,7 +30,12 @@ int zr_stage_frc(zr_stage_state* s, hipStream_t st);
 int zr_stage_set_frc_crop(zr_stage_state* s, const double* trans, int n, hipStream_t st);
 // S: kx-major full-resolution half spectra [N][Wk][Hp] (used when the state was
 // created with spectral != 0; may be NULL otherwise).
-int zr_stage_dose(zr_stage_state* s, const float* frames, const float2* S, int Wk, hipStream_t st);
+// normalize != 0: doseFilter(normalize=True) (filter sum + 1 in the denominator)
+int zr_stage_dose(zr_stage_state* s, const float* frames, const float2* S, int Wk, int normalize, hipStream_t st);
+// 'gausslp' on filtSum (on imageSum when no filter step has run since the solve)
+int zr_stage_gausslp(zr_stage_state* s, hipStream_t st);
+// cdfLogisticCurve's histogram (pdf counts, cdf, left edges); nbins = 0 without autologistic weights
+int zr_stage_get_hist(zr_stage_state* s, float* pdf, float* cdf, float* hsigma, int cap, int* nbins);
 int zr_stage_get_solution(zr_stage_state* s, double* translations, double* relative_est, double* weights,
                           double* logistic, double* error, double* error_stats, double* error_xy_frames);
 int zr_stage_get_sums(zr_stage_state* s, float* image_sum, float* filt_sum, float* even_sum, float* odd_sum);

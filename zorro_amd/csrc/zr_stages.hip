@@ -54,7 +54,8 @@ ZR_DEV float f32_div(float a, float b) { return __fdiv_rn(a, b); }
 // bin count on float32 data: float32 edges from linspace, index by scaling,
 // then the +-1 edge corrections), CDF in float32.
 __global__ __launch_bounds__(256) void k_ps_hist(const float* __restrict__ peaksig, int nn, float* __restrict__ hs,
-                                                 float* __restrict__ cdf, SolveInfo* info, int maxbins) {
+                                                 float* __restrict__ cdf, float* __restrict__ pdf, SolveInfo* info,
+                                                 int maxbins) {
   __shared__ float smin[4], smax[4];
   __shared__ int scnt[4];
   __shared__ unsigned counts[2048];
@@ -100,7 +101,7 @@ __global__ __launch_bounds__(256) void k_ps_hist(const float* __restrict__ peaks
   __syncthreads();
   if (tid == 0) {
     float run = 0.f;
-    for (int i = 0; i < nb; ++i) { run = f32_add(run, (float)counts[i]); cdf[i] = run; }
+    for (int i = 0; i < nb; ++i) { pdf[i] = (float)counts[i]; run = f32_add(run, pdf[i]); cdf[i] = run; }
     for (int i = 0; i < nb; ++i) { cdf[i] = f32_div(cdf[i], run); hs[i] = edge(i); }
     info->nps = c;
     info->nbins = nb;
@@ -1390,15 +1391,14 @@ struct DoseArgs {
   double* thresq;   // [N]
   SolveInfo* info;
   int order_sq;     // order = 2^order_sq, or -1
+  int normalize;    // doseFilter(normalize=True): scale by 1 / (sum over frames of the filter + 1)
 };
 
 // dose per frame from mean(imageSum)/N, accumulated dose, per-frame cutoff
 // radius: the last radial index whose optimal exposure is closer to the dose at
 // the end of the frame than at its start.  One block of 256 threads.
 __global__ __launch_bounds__(256) void k_dose_params(DoseArgs a) {
-  __shared__ double opti[2048];
   __shared__ double red[4];
-  __shared__ double s_dpf;
   const int tid = threadIdx.x;
   double s = 0.0;
   for (int i = tid; i < a.nrows; i += 256) s += a.rowsum[i];
@@ -1413,22 +1413,25 @@ __global__ __launch_bounds__(256) void k_dose_params(DoseArgs a) {
     a.accum[0] = missing;
     double run = 0.0;
     for (int j = 1; j <= a.N; ++j) { run += dpf; a.accum[j] = run + missing; }
-    s_dpf = dpf;
   }
+  __syncthreads();
+  // optimal exposure at radial index k, evaluated where it is used (any Hp/2
+  // length); index 0 takes the square of index 1 (zorro/zorro.py:2422)
   const int nq = a.Hp / 2;
   const double qs = sqrt(a.ipx * a.ipy);
-  for (int k = tid; k < nq; k += 256)
-    opti[k] = 2.51284 * a.vs * (a.A * pow(k * qs, a.B) + a.C);
-  __syncthreads();
-  if (tid == 0) opti[0] = opti[1] * opti[1];
-  __syncthreads();
+  auto opti = [&](int k) -> double {
+    const double o1 = 2.51284 * a.vs * (a.A * pow(qs, a.B) + a.C);
+    return k == 0 ? o1 * o1 : 2.51284 * a.vs * (a.A * pow(k * qs, a.B) + a.C);
+  };
   // one wave per frame: the last radial index closer to the frame's end dose
   const int lane = tid & 63, wid = tid >> 6;
   for (int j = wid; j < a.N; j += 4) {
     const double d1 = a.accum[j + 1], d0 = a.accum[j];
     int last = -1;
-    for (int k = lane; k < nq; k += 64)
-      if (fabs(d1 - opti[k]) < fabs(d0 - opti[k])) last = k;
+    for (int k = lane; k < nq; k += 64) {
+      const double o = opti(k);
+      if (fabs(d1 - o) < fabs(d0 - o)) last = k;
+    }
     for (int o = 32; o > 0; o >>= 1) last = max(last, __shfl_xor(last, o, 64));
     if (lane == 0) a.thresq[j] = last >= 0 ? last * qs : NAN;
   }
@@ -1448,10 +1451,11 @@ __global__ __launch_bounds__(dose_cols_nt(HP)) void k_dose_cols(const float2* __
   const int tid = threadIdx.x;
   const int kx = xcd_remap(blockIdx.x, gridDim.x);
   const int Wh = a.Wp / 2 + 1;
-  float invc[PER], q[PER];
+  float invc[PER], q[PER], gsum[PER];
   float2 acc[PER];
   for (int i = 0; i < PER; ++i) {
     const int ky = tid + i * NT;
+    gsum[i] = 0.f;
     const double fy = ky < HP / 2 ? ky : ky - HP;
     const double fx = kx;     // kx = Wp/2 is the Nyquist column (|f| identical)
     auto crit = [&](double yy, double xx) {
@@ -1481,11 +1485,14 @@ __global__ __launch_bounds__(dose_cols_nt(HP)) void k_dose_cols(const float2* __
       float thr = expf(-cutoff_pow(q[i] / tq, ord, a.order_sq));
       float2 v = buf[tid + i * NT];
       float g = thr * filt;
+      gsum[i] += g;
       acc[i].x += v.x * g;
       acc[i].y += v.y * g;
     }
     __syncthreads();
   }
+  if (a.normalize)   // filterMag = 1 / (sum of the filters + 1), float32 (zorro/zorro.py:2480-2486)
+    for (int i = 0; i < PER; ++i) acc[i] = c_scale(acc[i], 1.f / (gsum[i] + 1.f));
   for (int i = 0; i < PER; ++i) buf[tid + i * NT] = acc[i];
   __syncthreads();
   wg_fft<HP, 1, NT>(buf, twH, tid);
@@ -1639,7 +1646,7 @@ __global__ __launch_bounds__(DA_NT) void k_dose_accum(const float2* __restrict__
     const double qq = sqrt(xx * xx * a.ipx * a.ipx + yy * yy * a.ipy * a.ipy);
     return a.vs * (a.A * pow(qq, a.B) + a.C);
   };
-  float invc[DA_U], q[DA_U];
+  float invc[DA_U], q[DA_U], gsum[DA_U];
   float2 acc[DA_U], ex[DA_U];
   int kxs[DA_U];
   const double fy = ky < HP / 2 ? ky : ky - HP;
@@ -1657,6 +1664,7 @@ __global__ __launch_bounds__(DA_NT) void k_dose_accum(const float2* __restrict__
     invc[u] = (float)(1.0 / c);
     q[u] = (float)qq;
     acc[u] = make_float2(0.f, 0.f);
+    gsum[u] = 0.f;
     ex[u] = twW[(int)(((long)kx * (a.W - 1)) % a.W)];   // e^{-2 pi i kx (W-1)/W}
   }
   const float2 ey = twH[(int)(((long)ky * (a.H - 1)) % a.H)];   // e^{-2 pi i ky (H-1)/H}
@@ -1676,13 +1684,17 @@ __global__ __launch_bounds__(DA_NT) void k_dose_accum(const float2* __restrict__
       sp = c_add(sp, c_add(r0, c_mul(rh, ey)));
       sp = c_add(sp, c_add(c0, c_mul(cw, ex[u])));
       const float gval = expf(-cutoff_pow(q[u] * itq, ord, a.order_sq)) * expf(mhd * invc[u]);
+      gsum[u] += gval;
       acc[u].x += sp.x * gval;
       acc[u].y += sp.y * gval;
     }
   }
 #pragma unroll
-  for (int u = 0; u < DA_U; ++u)
-    if (kx0 + u < Whalf) G[(size_t)(kx0 + u) * HP + ky] = acc[u];
+  for (int u = 0; u < DA_U; ++u) {
+    // normalize: filterMag = 1 / (sum of the filters + 1), float32 (zorro/zorro.py:2480-2486)
+    const float2 v = a.normalize ? c_scale(acc[u], 1.f / (gsum[u] + 1.f)) : acc[u];
+    if (kx0 + u < Whalf) G[(size_t)(kx0 + u) * HP + ky] = v;
+  }
 }
 
 // Spectral dose path, step 2: inverse transform along y of every column of G,
@@ -1703,6 +1715,55 @@ __global__ __launch_bounds__(256) void k_dose_cols_ifft(const float2* __restrict
   wg_fft<HP, 1, NT>(buf, twH, tid);
 #pragma unroll
   for (int i = 0; i < PER; ++i) col[tid + i * NT] = buf[tid + i * NT];
+}
+
+// ---------------------------------------------------------------- gausslp
+// filterMode 'gausslp' (zorro/zorro.py:1707-1711):
+// scipy.ndimage.gaussian_filter(img, 3.0) of a float32 image, i.e. for axis 0
+// and then axis 1 a correlation with w[j] = exp(-j^2 / 18) / sum_j (|j| <= 12:
+// truncate 4), 'reflect' edges (d c b a | a b c d | d c b a), sums in FP64 with
+// the symmetric tap pairs added first as scipy's correlate1d does, and each
+// pass rounded to float32 (scipy writes every axis into the float32 output).
+constexpr int GLP_R = 12;
+struct GaussTaps { double w[GLP_R + 1]; };   // w[0] centre, w[j] = w[-j]
+
+ZR_DEV int refl_index(int i, int n) {
+  const int p = 2 * n;
+  i %= p;
+  if (i < 0) i += p;
+  return i < n ? i : p - 1 - i;
+}
+
+// axis 0 (along y): one thread per pixel, a wave reads 64 consecutive x of each row
+__global__ __launch_bounds__(256) void k_gauss_axis0(const float* __restrict__ src, int H, int W, GaussTaps t,
+                                                     float* __restrict__ dst) {
+  const size_t e = (size_t)xcd_remap(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
+  if (e >= (size_t)H * W) return;
+  const int y = (int)(e / W), x = (int)(e % W);
+  double acc = t.w[0] * (double)src[(size_t)y * W + x];
+  for (int j = 1; j <= GLP_R; ++j)
+    acc += ((double)src[(size_t)refl_index(y + j, H) * W + x] + (double)src[(size_t)refl_index(y - j, H) * W + x]) *
+           t.w[j];
+  dst[e] = (float)acc;
+}
+
+// axis 1 (along x): one block per row segment of 256 pixels, staged in LDS with
+// its 12-pixel halo on each side
+__global__ __launch_bounds__(256) void k_gauss_axis1(const float* __restrict__ src, int H, int W, GaussTaps t,
+                                                     float* __restrict__ dst) {
+  __shared__ float row[256 + 2 * GLP_R];
+  const int segs = (W + 255) / 256;
+  const int b = xcd_remap(blockIdx.x, gridDim.x);
+  const int y = b / segs, x0 = (b % segs) * 256;
+  const float* r = src + (size_t)y * W;
+  for (int i = threadIdx.x; i < 256 + 2 * GLP_R; i += 256) row[i] = r[refl_index(x0 - GLP_R + i, W)];
+  __syncthreads();
+  const int x = x0 + threadIdx.x;
+  if (x >= W) return;
+  const float* c = row + GLP_R + threadIdx.x;
+  double acc = t.w[0] * (double)c[0];
+  for (int j = 1; j <= GLP_R; ++j) acc += ((double)c[j] + (double)c[-j]) * t.w[j];
+  dst[(size_t)y * W + x] = (float)acc;
 }
 
 // ==================================================================== host
@@ -1813,7 +1874,7 @@ struct zr_stage_state {
   bool eq_bad = false;         // the pairs' rot90 equations are not single frame spans
   size_t bytes = 0;            // device memory held
   int2 *edges = nullptr, *meas = nullptr, *dups = nullptr;
-  float *hs = nullptr, *cdf = nullptr;
+  float *hs = nullptr, *cdf = nullptr, *pdf = nullptr;
   double* obj = nullptr;
   SolveInfo* info = nullptr;
   double *weights = nullptr, *resid = nullptr, *pos = nullptr;
@@ -1830,6 +1891,7 @@ struct zr_stage_state {
   // dose
   float2 *twHp = nullptr, *twWp = nullptr, *twWp2 = nullptr, *Tdose = nullptr, *G = nullptr;
   double *accum = nullptr, *thresq = nullptr, *drow = nullptr;
+  float* gtmp = nullptr;       // 'gausslp': the axis-0 pass of the Gaussian
   bool spectral = false;       // dose filter from the forward spectra (unpadded frames)
   float2 *R0 = nullptr, *RH = nullptr, *C0 = nullptr, *CW = nullptr, *Py = nullptr, *Px = nullptr;
 };
@@ -1866,7 +1928,7 @@ int zr_stage_copy_sums_device(zr_stage_state* s, float* image_sum, float* filt_s
 
 void zr_stage_free(zr_stage_state* s) {
   if (!s) return;
-  void* ptrs[] = {s->edges, s->meas, s->dups, s->hs, s->cdf, s->obj, s->info, s->weights, s->resid, s->pos, s->trans_raw, s->trans,
+  void* ptrs[] = {s->edges, s->meas, s->dups, s->hs, s->cdf, s->pdf, s->gtmp, s->obj, s->info, s->weights, s->resid, s->pos, s->trans_raw, s->trans,
                   s->rel, s->err_stats, s->err_frames, s->fs, s->img_sum, s->even, s->odd, s->filt, s->rowsum,
                   s->twB, s->twB2, s->Tfrc, s->frc2d, s->frc, s->frc_part, s->hann, s->twHp, s->twWp, s->twWp2, s->Tdose, s->G,
                   s->accum, s->thresq, s->drow, s->R0, s->RH, s->C0, s->CW, s->Py, s->Px};
@@ -1903,6 +1965,7 @@ int zr_stage_create(zr_stage_state** out, const zr_params& p, int N, int H, int 
   const int maxbins = std::max(1, s->Mcap / 7 + 1);
   ZC(smalloc(s, (void**)&s->hs, sizeof(float) * maxbins));
   ZC(smalloc(s, (void**)&s->cdf, sizeof(float) * maxbins));
+  ZC(smalloc(s, (void**)&s->pdf, sizeof(float) * maxbins));
   ZC(smalloc(s, (void**)&s->obj, sizeof(double) * LOGI_G0 * LOGI_G1 * LOGI_G2));
   ZC(smalloc(s, (void**)&s->info, sizeof(SolveInfo)));
   ZC(smalloc(s, (void**)&s->weights, sizeof(double) * std::max(1, s->Mcap)));
@@ -1962,6 +2025,8 @@ int zr_stage_create(zr_stage_state** out, const zr_params& p, int N, int H, int 
     ZC(smalloc(s, (void**)&s->thresq, sizeof(double) * N));
     ZC(smalloc(s, (void**)&s->drow, sizeof(double) * (Hp / 4 + 1)));
   }
+  for (int i = 0; i < p.n_filters; ++i)
+    if (p.filters[i] == ZR_FILTER_GAUSSLP && !s->gtmp) ZC(smalloc(s, (void**)&s->gtmp, sizeof(float) * H * W));
 #undef ZC
   *out = s;
   return ZR_OK;
@@ -2002,6 +2067,7 @@ int zr_stage_solve(zr_stage_state* s, const float* shifts, const float* corr, co
     return serr(ZR_ERR_UNSUPPORTED, "pair pattern gives equations the solver cannot represent (a row of the pair "
                 "mask with a gap makes an equation over non-adjacent frame steps)");
   const int N = s->N, M = s->M;
+  s->have_dose = false;    // a new registration: its filter steps start without a filtSum
   ZS_HIP(zr_launch_fill_zero(s->info, sizeof(SolveInfo), st));   // a kernel, so the solve can be graph-captured
   if (s->p.tri_mode == ZR_TRI_FIRST || s->p.tri_mode == ZR_TRI_REFINE) {
     hipLaunchKernelGGL(k_direct_positions, dim3(1), dim3(64), 0, st, shifts, N,
@@ -2017,7 +2083,7 @@ int zr_stage_solve(zr_stage_state* s, const float* shifts, const float* corr, co
   const int wm = s->p.weight_mode;
   if (wm == ZR_W_AUTOLOGISTIC) {
     const int maxbins = std::max(1, s->Mcap / 7 + 1);
-    hipLaunchKernelGGL(k_ps_hist, dim3(1), dim3(256), 0, st, peaksig, N * N, s->hs, s->cdf, s->info, maxbins);
+    hipLaunchKernelGGL(k_ps_hist, dim3(1), dim3(256), 0, st, peaksig, N * N, s->hs, s->cdf, s->pdf, s->info, maxbins);
     ZS_HIP(hipGetLastError());
     const int total = LOGI_G0 * LOGI_G1 * LOGI_G2;
     hipLaunchKernelGGL(k_logi_grid, dim3((total + 255) / 256), dim3(256), 0, st, s->hs, s->cdf, s->info, s->obj);
@@ -2115,7 +2181,7 @@ int zr_stage_frc(zr_stage_state* s, hipStream_t st) {
   return ZR_OK;
 }
 
-static void fill_dose_args(zr_stage_state* s, DoseArgs& a, const double* rowsum, int nrows) {
+static void fill_dose_args(zr_stage_state* s, DoseArgs& a, const double* rowsum, int nrows, int normalize) {
   a.N = s->N; a.H = s->H; a.W = s->W; a.Hp = s->Hp; a.Wp = s->Wp;
   a.A = s->p.dose_a; a.B = s->p.dose_b; a.C = s->p.dose_c; a.order = s->p.dose_order; a.missing = s->p.dose_missing;
   a.vs = std::sqrt(s->p.voltage_kv / 300.0);
@@ -2123,11 +2189,13 @@ static void fill_dose_args(zr_stage_state* s, DoseArgs& a, const double* rowsum,
   a.ipy = 1.0 / (s->Hp * (s->p.pixelsize_nm * 10.0));
   a.rowsum = rowsum; a.nrows = nrows; a.accum = s->accum; a.thresq = s->thresq; a.info = s->info;
   a.order_sq = -1;
+  a.normalize = normalize;
   for (int k = 0; k < 8; ++k)
     if (a.order == (double)(1 << k)) a.order_sq = k;
 }
 
-static int dose_spectral(zr_stage_state* s, const float* frames, const float2* S, int Wk, hipStream_t st) {
+static int dose_spectral(zr_stage_state* s, const float* frames, const float2* S, int Wk, int normalize,
+                         hipStream_t st) {
   const int H = s->H, W = s->W;
   // aligned sums (imageSum, even, odd) and their total for the dose per frame
   const int gx = (W + 63) / 64, gy = (H + 4 * SS_R - 1) / (4 * SS_R);
@@ -2136,7 +2204,7 @@ static int dose_spectral(zr_stage_state* s, const float* frames, const float2* S
   ZS_HIP(hipGetLastError());
   s->have_sums = true;
   DoseArgs a;
-  fill_dose_args(s, a, s->rowsum, gx * gy);
+  fill_dose_args(s, a, s->rowsum, gx * gy, normalize);
   hipLaunchKernelGGL(k_dose_params, dim3(1), dim3(256), 0, st, a);
   ZS_HIP(hipGetLastError());
   if (H != W) return serr(ZR_ERR_UNSUPPORTED, "spectral dose path needs square frames");
@@ -2173,10 +2241,10 @@ static int dose_spectral(zr_stage_state* s, const float* frames, const float2* S
   return ZR_OK;
 }
 
-int zr_stage_dose(zr_stage_state* s, const float* frames, const float2* S, int Wk, hipStream_t st) {
+int zr_stage_dose(zr_stage_state* s, const float* frames, const float2* S, int Wk, int normalize, hipStream_t st) {
   if (!s || !s->have_solution) return serr(ZR_ERR_STATE, "dose filter before solve");
   if (!s->p.do_dose) return serr(ZR_ERR_STATE, "dose filter disabled in the parameters");
-  if (s->spectral) return dose_spectral(s, frames, S, Wk, st);
+  if (s->spectral) return dose_spectral(s, frames, S, Wk, normalize, st);
   const int Hp = s->Hp, Wp = s->Wp;
   const int nblk = Hp / 4;
   size_t sh = (size_t)4 * (Wp / 2) * sizeof(float2);
@@ -2195,7 +2263,7 @@ int zr_stage_dose(zr_stage_state* s, const float* frames, const float2* S, int W
   ZS_HIP(hipGetLastError());
   s->have_sums = true;
   DoseArgs a;
-  fill_dose_args(s, a, s->drow, nblk);
+  fill_dose_args(s, a, s->drow, nblk, normalize);
   hipLaunchKernelGGL(k_dose_params, dim3(1), dim3(256), 0, st, a);
   ZS_HIP(hipGetLastError());
   switch (Hp) {
@@ -2213,6 +2281,44 @@ int zr_stage_dose(zr_stage_state* s, const float* frames, const float2* S, int W
   }
   ZS_HIP(hipGetLastError());
   s->have_dose = true;
+  return ZR_OK;
+}
+
+int zr_stage_gausslp(zr_stage_state* s, hipStream_t st) {
+  if (!s || !s->have_sums) return serr(ZR_ERR_STATE, "gausslp before the sums");
+  if (!s->gtmp) return serr(ZR_ERR_STATE, "gausslp is not among the parameters' filter steps");
+  GaussTaps t;
+  {
+    double sum = 0.0, w[2 * GLP_R + 1];
+    for (int j = -GLP_R; j <= GLP_R; ++j) sum += (w[j + GLP_R] = std::exp(-0.5 / 9.0 * (double)(j * j)));
+    for (int j = 0; j <= GLP_R; ++j) t.w[j] = w[j + GLP_R] / sum;
+  }
+  // no filtSum yet: the filter starts from a copy of imageSum (zorro/zorro.py:1708-1710)
+  const float* src = s->have_dose ? s->filt : s->img_sum;
+  const size_t n = (size_t)s->H * s->W;
+  hipLaunchKernelGGL(k_gauss_axis0, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, src, s->H, s->W, t, s->gtmp);
+  ZS_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_gauss_axis1, dim3((unsigned)(s->H * ((s->W + 255) / 256))), dim3(256), 0, st, s->gtmp, s->H,
+                     s->W, t, s->filt);
+  ZS_HIP(hipGetLastError());
+  s->have_dose = true;
+  return ZR_OK;
+}
+
+int zr_stage_get_hist(zr_stage_state* s, float* pdf, float* cdf, float* hsigma, int cap, int* nbins) {
+  if (!s || !s->have_solution) return serr(ZR_ERR_STATE, "no solution");
+  SolveInfo inf;
+  ZS_HIP(hipMemcpy(&inf, s->info, sizeof(inf), hipMemcpyDeviceToHost));
+  const bool hist = s->p.weight_mode == ZR_W_AUTOLOGISTIC &&
+                    !(s->p.tri_mode == ZR_TRI_FIRST || s->p.tri_mode == ZR_TRI_REFINE);
+  const int nb = hist ? inf.nbins : 0;
+  if (nbins) *nbins = nb;
+  if (nb > cap && (pdf || cdf || hsigma)) return serr(ZR_ERR_ARG, "histogram has %d bins, buffers hold %d", nb, cap);
+  if (nb > 0) {
+    if (pdf) ZS_HIP(hipMemcpy(pdf, s->pdf, sizeof(float) * nb, hipMemcpyDeviceToHost));
+    if (cdf) ZS_HIP(hipMemcpy(cdf, s->cdf, sizeof(float) * nb, hipMemcpyDeviceToHost));
+    if (hsigma) ZS_HIP(hipMemcpy(hsigma, s->hs, sizeof(float) * nb, hipMemcpyDeviceToHost));
+  }
   return ZR_OK;
 }
 

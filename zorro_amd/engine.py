@@ -22,6 +22,39 @@ WMODE = {"autologistic": L.ZR_W_AUTOLOGISTIC, "logistic": L.ZR_W_LOGISTIC, "corr
          "unweighted": L.ZR_W_UNWEIGHTED, "norm": L.ZR_W_NORM, "normalized": L.ZR_W_NORM}
 
 
+# filterMode steps after the registration (zorro/zorro.py:1692-1711)
+FILTER_STEPS = {"dose": L.ZR_FILTER_DOSE, "dosenorm": L.ZR_FILTER_DOSENORM, "gausslp": L.ZR_FILTER_GAUSSLP}
+
+
+def parse_filter_mode(filterMode):
+    """(steps, hot) of a filterMode string as alignImageStack reads it: 'hot'
+    anywhere in the string runs hotpixFilter before the registration
+    (zorro/zorro.py:1595-1596); the comma-separated tokens, lower-cased with
+    spaces removed, are then applied in order to filtSum (:1692-1711).
+    'background' is not built and raises NotImplementedError; a token that is
+    none of dose, dosenorm, background, gausslp or a 'hot' token raises
+    ValueError (the reference skips it silently, which would hide a misspelt
+    filter)."""
+    if not filterMode:
+        return [], False
+    text = str(filterMode).lower()
+    steps = []
+    for tok in text.replace(" ", "").split(","):
+        if tok in FILTER_STEPS:
+            steps.append(tok)
+        elif tok == "background":
+            raise NotImplementedError("filterMode 'background' (a Powell fit of a 2D Gaussian to the binned sum, "
+                                      "zorro/zorro_util.py:497-549) is not built")
+        elif tok == "" or "hot" in tok:
+            continue
+        else:
+            raise ValueError(f"filterMode token {tok!r} is not one of 'dose', 'dosenorm', 'background', "
+                             "'gausslp', 'hot'")
+    if len(steps) > 4:
+        raise NotImplementedError(f"at most 4 filterMode steps are built (got {len(steps)})")
+    return steps, "hot" in text
+
+
 def make_params(triMode="diag", diagWidth=5, diagStart=0, maxShift=100, Bmode="opti", Brad=512.0, subPixReg=16,
                 shiftMethod="lanczos", weightMode="autologistic", peaksigThres=6.0, logisticK=5.0,
                 logisticNu=0.15, originMode="centroid", suppressOrigin=True, fouCrop=(3072, 3072),
@@ -61,8 +94,11 @@ def make_params(triMode="diag", diagWidth=5, diagStart=0, maxShift=100, Bmode="o
     fc = (0, 0) if fouCrop is None or not np.any(fouCrop) else tuple(int(v) for v in fouCrop)
     p.fou_crop[0], p.fou_crop[1] = fc
     p.shape_padded[0], p.shape_padded[1] = (int(shapePadded[0]), int(shapePadded[1]))
-    modes = [] if not filterMode else filterMode.lower().replace(" ", "").split(",")
-    p.do_dose = 1 if "dose" in modes else 0
+    steps, hot = parse_filter_mode(filterMode)
+    p.n_filters = len(steps)
+    for i, st in enumerate(steps):
+        p.filters[i] = FILTER_STEPS[st]
+    p.do_dose = 1 if any(st in ("dose", "dosenorm") for st in steps) else 0
     if p.do_dose and doseFiltParam[0] is not None:
         raise NotImplementedError("doseFiltParam[0] must be None (the reference raises NameError otherwise, Q11)")
     p.dose_a, p.dose_b, p.dose_c, p.dose_order, p.dose_missing = (float(v) for v in doseFiltParam[1:6])
@@ -74,7 +110,7 @@ def make_params(triMode="diag", diagWidth=5, diagStart=0, maxShift=100, Bmode="o
     if shapeBinned is not None and np.any(shapeBinned):
         p.shape_binned[0], p.shape_binned[1] = (int(shapeBinned[0]), int(shapeBinned[1]))
     hp = {**HOTPIX_DEFAULTS, **(hotpixInfo or {})}
-    p.do_hot = 1 if filterMode and "hot" in filterMode.lower() else 0
+    p.do_hot = 1 if hot else 0
     if p.do_hot and hp.get("decorrOutliers"):
         raise NotImplementedError("hotpixInfo['decorrOutliers'] adds random noise (zorro/zorro.py:2651-2666): not built")
     p.hot_logistic_k, p.hot_relax, p.hot_max_sigma = (float(hp[k]) for k in ("logisticK", "relax", "maxSigma"))
@@ -226,6 +262,15 @@ class MoviePlan:
     def dose_filter(self):
         L.check(self.lib.zr_dose_filter(self._mv))
 
+    def filter(self, step: str):
+        """One filterMode step ('dose', 'dosenorm', 'gausslp') on filtSum."""
+        L.check(self.lib.zr_filter(self._mv, FILTER_STEPS[step]))
+
+    def post_filters(self):
+        """applyShifts (unless a dose step forms the sums) and the parameters'
+        filterMode steps in order (zorro/zorro.py:1672-1711)."""
+        L.check(self.lib.zr_post_filters(self._mv))
+
     def align(self):
         L.check(self.lib.zr_align(self._mv))
 
@@ -311,6 +356,18 @@ class MoviePlan:
         return dict(translations=t, relativeEst=rel, weights=w, logistic=lg, errorXY=err,
                     mean_errorNorm=es[0], std_errorNorm=es[1], errorX=exy[:, 1], errorY=exy[:, 0])
 
+    def peaksig_hist(self):
+        """cdfLogisticCurve's histogram (errorDict pdfPeaks, cdfPeaks, hSigma,
+        zorro/zorro.py:1991-1998) or None without autologistic weights."""
+        nb = C.c_int()
+        L.check(self.lib.zr_get_peaksig_hist(self._mv, None, None, None, 0, C.byref(nb)))
+        if nb.value < 1:
+            return None
+        pdf, cdf, hs = (np.zeros(nb.value, np.float32) for _ in range(3))
+        L.check(self.lib.zr_get_peaksig_hist(self._mv, L.fptr(pdf), L.fptr(cdf), L.fptr(hs), nb.value,
+                                             C.byref(nb)))
+        return dict(pdfPeaks=pdf, cdfPeaks=cdf, hSigma=hs)
+
     def solver_info(self):
         d = (C.c_int * 6)()
         L.check(self.lib.zr_get_solver_info(self._mv, d))
@@ -321,11 +378,12 @@ class MoviePlan:
         h, w = self.hb, self.wb
         out = [np.zeros((h, w), np.float32) for _ in range(4)]
         ptrs = [L.fptr(a) for a in out]
-        if not self.params.do_dose:
+        filt = self.params.n_filters > 0
+        if not filt:
             ptrs[1] = None
         L.check(self.lib.zr_get_sums(self._mv, *ptrs))
         res = dict(imageSum=out[0], evenSum=out[2], oddSum=out[3])
-        if self.params.do_dose:
+        if filt:
             res["filtSum"] = out[1]
         return res
 

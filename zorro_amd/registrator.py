@@ -395,17 +395,16 @@ class ImageRegistrator:
             thr, K, nu = sol["logistic"]
             self.peaksigThres, self.logisticK, self.logisticNu = thr, K, nu
             ed.update(peaksigThres=thr, logisticK=K, logisticNu=nu)
+            hist = plan.peaksig_hist()
+            if hist is not None:            # cdfLogisticCurve's diagnostics (zorro/zorro.py:2016-2018)
+                ed.update(hist)
         ed["translations"] = -np.vstack((np.zeros([1, 2]), np.cumsum(sol["relativeEst"], axis=0)))
         return ed
 
     def _finish(self, plan, sol):
         """applyShifts, FRC and the dose filter (zorro/zorro.py:1672-1712)."""
-        self.bench["shifts0"] = time.time()
-        if self.filterMode and "dose" in self.filterMode.lower():
-            self.bench["dose0"] = time.time()
-            plan.dose_filter()
-        else:
-            plan.apply_shifts()
+        self.bench["shifts0"] = self.bench["dose0"] = time.time()
+        plan.post_filters()
         # tiledFRC's tile count along each axis, floor((size - box) / box / (1 - overlap))
         # (zorro/zorro.py:2266-2274): a frame that holds no 512-pixel tile gets no FRC
         # (the reference would average over zero tiles)

@@ -64,6 +64,8 @@ def estimate_plan_bytes(n, h, w, params) -> int:
     b += ptot * (wc // 2 + 1) * 2 * m * 8              # pair pass 1
     b += ptot * 4 * m * m * 4                          # lag windows
     b += 4 * h * w * 4 + n * h * w * 4                 # sums + an owned copy of the frames
+    if any(params.filters[i] == L.ZR_FILTER_GAUSSLP for i in range(params.n_filters)):
+        b += h * w * 4                                 # the Gaussian's first pass
     if params.do_dose:
         b += wk * h * 8
         if not spectral:
@@ -96,9 +98,9 @@ class MovieResult:
 class _Request:
     __slots__ = ("frames", "n", "h", "w", "params", "key", "future", "arrival", "admitted")
 
-    def __init__(self, frames, params, arrival):
+    def __init__(self, frames, params, arrival, shape=None):
         self.frames = frames
-        self.n, self.h, self.w = (int(v) for v in frames.shape)
+        self.n, self.h, self.w = (int(v) for v in (shape or frames.shape))
         self.params = params
         self.key = plan_key(self.n, self.h, self.w, params)
         self.future = Future()
@@ -132,6 +134,7 @@ class MovieServer:
         self._key_bytes = {}                    # key -> measured bytes of one plan
         self._held = 0                          # bytes in all plans
         self._inflight = 0
+        self._creating = 0                      # plans being created outside the lock
         self._closed = False
         self._pool = ThreadPoolExecutor(max_workers=self.max_inflight, thread_name_prefix="zr-movie")
         self._sched = threading.Thread(target=self._schedule, name="zr-admit", daemon=True)
@@ -188,13 +191,15 @@ class MovieServer:
         return self._held + need <= self.budget
 
     def _take_plan(self, req):
-        """An idle plan for the request, or a new one if memory allows; None if not now."""
+        """Called under the lock.  ('idle', plan) for an idle plan of the
+        request's key, ('create', bytes) when a new plan may be created (its
+        bytes are then reserved), or None if neither is possible now."""
         plans = self._idle.get(req.key)
         if plans:
             p = plans.pop()
             if not plans:
                 del self._idle[req.key]
-            return p
+            return "idle", p
         known = req.key in self._key_bytes
         need = self._bytes_for(req)
         if not self._evict_for(need):
@@ -202,22 +207,53 @@ class MovieServer:
             # else held, create it and check the exact figure instead
             if known or self._held > 0:
                 return None
+        self._held += need
+        return "create", need
+
+    def _create_plan(self, req, reserved):
+        """Create a plan outside the lock (multi-GB allocations and table
+        uploads must not hold up completions), then settle its exact size."""
         p = MoviePlan(req.params, req.n, req.h, req.w, device=self.device)
         nb = C.c_size_t()
         L.check(p.lib.zr_movie_device_bytes(p._mv, C.byref(nb)))
         actual = int(nb.value) + req.n * req.h * req.w * 4
-        self._key_bytes[req.key] = actual
-        if self._held + actual > self.budget and not self._evict_for(actual):
-            p.close()
-            if self._held == 0:
+        with self._lock:
+            self._key_bytes[req.key] = actual
+            self._held += actual - reserved
+            if self._held > self.budget:
+                self._evict_for(0)
+            if self._held > self.budget and self._held == actual:
+                self._held -= actual
+                p.close()
                 raise MemoryError(f"movie {req.n}x{req.h}x{req.w} needs {actual / 1e9:.2f} GB, more than the "
                                   f"HBM budget {self.budget / 1e9:.2f} GB")
-            return None
-        self._plan_bytes[id(p)] = actual
-        self._held += actual
-        self.stats["plans_created"] += 1
-        self.stats["peak_held"] = max(self.stats["peak_held"], self._held)
+            self._plan_bytes[id(p)] = actual
+            self.stats["plans_created"] += 1
+            self.stats["peak_held"] = max(self.stats["peak_held"], self._held)
         return p
+
+    def warm(self, frames_like, count: int = 1, **settings):
+        """Create ``count`` idle plans for movies shaped like ``frames_like``
+        ([N, H, W], or the shape itself) with these settings, ahead of the
+        requests (serving warm-up: no request then waits for a plan)."""
+        shape = tuple(int(v) for v in getattr(frames_like, "shape", frames_like))
+        params = make_params(**settings)
+        req = _Request(np.empty((0,), np.float32), params, 0.0, shape=shape)
+        made = []
+        for _ in range(int(count)):
+            with self._lock:
+                if not self._evict_for(self._bytes_for(req)) and self._held > 0:
+                    break
+                need = self._bytes_for(req)
+                self._held += need
+            made.append(self._create_plan(req, need))
+        with self._lock:
+            for p in made:
+                self._idle.setdefault(req.key, []).append(p)
+            if made:
+                self._idle.move_to_end(req.key)
+            self._lock.notify_all()
+        return len(made)
 
     def _schedule(self):
         while True:
@@ -228,45 +264,70 @@ class MovieServer:
                     return
                 if self._inflight >= self.max_inflight:
                     continue
-                chosen, plan = None, None
+                # requests cancelled while queued are dropped (Future.cancel succeeds
+                # until the request is admitted)
+                for req in [r for r in self._pending if r.future.cancelled()]:
+                    self._pending.remove(req)
+                chosen, got = None, None
                 for req in list(self._pending):
-                    try:
-                        plan = self._take_plan(req)
-                    except Exception as e:          # plan creation failed: report it on the request
-                        self._pending.remove(req)
-                        req.future.set_exception(e)
-                        plan = None
-                        continue
-                    if plan is not None:
+                    got = self._take_plan(req)
+                    if got is not None:
                         chosen = req
                         break
                 if chosen is None:
-                    if self._inflight == 0 and self._pending:
+                    if self._inflight == 0 and self._pending and self._creating == 0:
                         # nothing running and nothing fits even after eviction
                         req = self._pending.popleft()
-                        req.future.set_exception(MemoryError(
-                            f"movie {req.n}x{req.h}x{req.w} needs ~{self._bytes_for(req) / 1e9:.1f} GB, "
-                            f"more than the HBM budget {self.budget / 1e9:.1f} GB"))
+                        if req.future.set_running_or_notify_cancel():
+                            req.future.set_exception(MemoryError(
+                                f"movie {req.n}x{req.h}x{req.w} needs ~{self._bytes_for(req) / 1e9:.1f} GB, "
+                                f"more than the HBM budget {self.budget / 1e9:.1f} GB"))
                         continue
                     self._lock.wait()
                     continue
                 self._pending.remove(chosen)
+                if not chosen.future.set_running_or_notify_cancel():
+                    # cancelled between the sweep and now: give the plan or the reservation back
+                    if got[0] == "idle":
+                        self._idle.setdefault(chosen.key, []).append(got[1])
+                    else:
+                        self._held -= got[1]
+                    continue
                 chosen.admitted = time.perf_counter()
                 self._inflight += 1
+                if got[0] == "create":
+                    self._creating += 1
                 self.stats["admitted"] += 1
-            self._pool.submit(self._run, chosen, plan)
+            self._pool.submit(self._run, chosen, got)
 
     # ------------------------------------------------------------ execution
-    def _run(self, req, plan):
+    def _run(self, req, got):
+        """One admitted movie.  The plan is released exactly once, and the
+        future (already marked running, so it can no longer be cancelled) gets
+        its result or exception after the release."""
         torch = self.torch
+        res = err = None
+        plan = None
         try:
+            if got[0] == "create":
+                try:
+                    plan = self._create_plan(req, got[1])
+                except BaseException:
+                    with self._lock:
+                        self._held -= got[1] if req.key not in self._key_bytes else 0
+                    raise
+                finally:
+                    with self._lock:
+                        self._creating -= 1
+            else:
+                plan = got[1]
             plan.set_frames(req.frames)
             plan.align()
             h, w = req.h, req.w
             img = filt = None
             if self.return_sums != "none":
                 img = torch.empty((h, w), dtype=torch.float32, device=self.dev)
-                filt = torch.empty((h, w), dtype=torch.float32, device=self.dev) if req.params.do_dose else None
+                filt = torch.empty((h, w), dtype=torch.float32, device=self.dev) if req.params.n_filters else None
                 L.check(plan.lib.zr_copy_sums_device(plan._mv, C.c_void_p(img.data_ptr()),
                                                      C.c_void_p(filt.data_ptr() if filt is not None else 0),
                                                      None, None))
@@ -279,21 +340,25 @@ class MovieServer:
                 filt = filt.cpu().numpy() if filt is not None else None
             res = MovieResult(sol["translations"], img, filt, frc, req.arrival, req.admitted, done, req.key)
             plan._frames_ref = None
-            self._release(req, plan)
-            req.future.set_result(res)
         except BaseException as e:
-            self._release(req, plan, broken=True)
-            req.future.set_exception(e)
+            err = e
+        finally:
+            self._release(req, plan, broken=err is not None)
+        if err is None:
+            req.future.set_result(res)
+        else:
+            req.future.set_exception(err)
 
     def _release(self, req, plan, broken=False):
         with self._lock:
             self._inflight -= 1
-            if broken:
-                self._held -= self._plan_bytes.pop(id(plan), 0)
-                plan.close()
-            else:
-                self._idle.setdefault(req.key, []).append(plan)
-                self._idle.move_to_end(req.key)
+            if plan is not None:
+                if broken:
+                    self._held -= self._plan_bytes.pop(id(plan), 0)
+                    plan.close()
+                else:
+                    self._idle.setdefault(req.key, []).append(plan)
+                    self._idle.move_to_end(req.key)
             self._lock.notify_all()
 
 
