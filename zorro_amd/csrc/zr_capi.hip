@@ -1010,13 +1010,12 @@ extern "C" int zr_dose_filter(zr_movie* mv) { return zr_filter(mv, ZR_FILTER_DOS
 
 extern "C" int zr_post_filters(zr_movie* mv) {
   if (!mv) return fail(ZR_ERR_ARG, "NULL movie");
-  // a dose step forms the aligned sums and half sums itself; without one the
-  // frames are shifted and summed first (applyShifts, zorro/zorro.py:1672)
+  // a dose step forms the aligned sums and half sums itself; unless the first
+  // step is one, the frames are shifted and summed first (applyShifts,
+  // zorro/zorro.py:1672), which a Gaussian step needs for its imageSum copy
   int rc;
-  bool summed = false;
-  for (int i = 0; i < mv->p.n_filters && !summed; ++i)
-    if (mv->p.filters[i] != ZR_FILTER_GAUSSLP) summed = true;
-  if (!summed && (rc = zr_apply_shifts(mv, nullptr))) return rc;
+  const bool dose_first = mv->p.n_filters > 0 && mv->p.filters[0] != ZR_FILTER_GAUSSLP;
+  if (!dose_first && (rc = zr_apply_shifts(mv, nullptr))) return rc;
   for (int i = 0; i < mv->p.n_filters; ++i)
     if ((rc = zr_filter(mv, mv->p.filters[i]))) return rc;
   return ZR_OK;

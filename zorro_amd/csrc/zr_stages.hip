@@ -2067,7 +2067,8 @@ int zr_stage_solve(zr_stage_state* s, const float* shifts, const float* corr, co
     return serr(ZR_ERR_UNSUPPORTED, "pair pattern gives equations the solver cannot represent (a row of the pair "
                 "mask with a gap makes an equation over non-adjacent frame steps)");
   const int N = s->N, M = s->M;
-  s->have_dose = false;    // a new registration: its filter steps start without a filtSum
+  // a new registration: its sums and filter steps start afresh
+  s->have_dose = s->have_sums = s->have_frc = false;
   ZS_HIP(zr_launch_fill_zero(s->info, sizeof(SolveInfo), st));   // a kernel, so the solve can be graph-captured
   if (s->p.tri_mode == ZR_TRI_FIRST || s->p.tri_mode == ZR_TRI_REFINE) {
     hipLaunchKernelGGL(k_direct_positions, dim3(1), dim3(64), 0, st, shifts, N,
