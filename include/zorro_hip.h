@@ -295,6 +295,21 @@ int zr_movie_device_bytes(zr_movie* mv, size_t* bytes);
  * The results of applyShifts/doseFilter (zorro/zorro.py:1804, :2487-2494). */
 int zr_copy_sums_device(zr_movie* mv, float* image_sum, float* filt_sum, float* even_sum, float* odd_sum);
 
+/* tiledFRC of two images (zorro/zorro.py:2246-2310): Hann-apodised box x box
+ * tiles at 1 - overlap steps (overlap clamped to 0.8) inside the crop limits
+ * that translations trans f64 [n][2] (host, NULL: none) give (minCrop 5,
+ * :2264-2274); per tile the FP64 sum of Re(E conj(O)) / sqrt(|O|^2 |E|^2) over
+ * the float32 tile spectra; normalised by its [0,0] element, fftshifted and
+ * rotationally averaged (rotmean, zorro/zorro_util.py:147-224).
+ * img1, img2: device f32 [h][w]; frc: device f64 receiving the whole rotmean
+ * curve, ceil(sqrt(2) box/2) + 1 values (lazyFouRingCorr keeps the first
+ * box/2); *nrings (nullable) receives that count.  box 256, 512 or 1024.
+ * Runs on `stream` (NULL: the default stream) and returns when the result is
+ * written; allocates its workspace per call.  Fails with ZR_ERR_ARG when no
+ * tile fits after the crop. */
+int zr_tiled_frc(const float* img1, const float* img2, int h, int w, const double* trans, int n, int box,
+                 double overlap, double* frc, int* nrings, void* stream);
+
 /* Device pointers of results for zero-copy consumers (valid until destroy). */
 int zr_device_results(zr_movie* mv, const float** image_sum, const float** filt_sum, const double** translations);
 

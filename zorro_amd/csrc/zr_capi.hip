@@ -202,6 +202,9 @@ struct zr_movie {
 
 extern "C" const char* zr_last_error(void) { return g_err.c_str(); }
 
+// error reporting for the entry points defined in other source files
+int zr_capi_fail(int code, const char* msg) { return fail(code, "%s", msg); }
+
 extern "C" int zr_device_count(int* n) {
   if (!n) return fail(ZR_ERR_ARG, "n is NULL");
   ZR_HIP(hipGetDeviceCount(n));

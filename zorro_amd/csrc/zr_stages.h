@@ -43,3 +43,15 @@ int zr_stage_get_frc(zr_stage_state* s, double* frc, int* n);
 int zr_stage_solver_info(zr_stage_state* s, int* out6);
 int zr_stage_device_results(zr_stage_state* s, const float** image_sum, const float** filt_sum,
                             const double** translations);
+
+// Standalone tiledFRC of two device images [h][w] (row stride w): crop limits
+// and tile counts from translations [n][2] (NULL: none) as tiledFRC derives
+// them; workspace layout of one block of zr_frc_work_bytes(box, chunk) bytes,
+// tiles transformed `chunk` at a time (the result does not depend on chunk);
+// frc: device f64 [zr_frc_rings_count(box)] (rotmean's length).
+size_t zr_frc_work_bytes(int box, int chunk);
+int zr_frc_rings_count(int box);
+int zr_frc_tiles_from_trans(const double* trans, int n, int h, int w, int box, double overlap, int crop[4],
+                            int* tiles_y, int* tiles_x);
+int zr_frc_tiles_run(const float* img1, const float* img2, int h, int w, const int crop[4], int tiles_y, int tiles_x,
+                     int box, double overlap, void* work, int chunk, double* frc, hipStream_t st);

@@ -1131,7 +1131,7 @@ template <int BOX>
 __global__ __launch_bounds__(256) void k_frc_rows(const float* __restrict__ even, const float* __restrict__ odd,
                                                  const SolveInfo* __restrict__ info, int W, int tmax_x, double step,
                                                  const float2* __restrict__ twL, const float2* __restrict__ tw2L,
-                                                 const float* __restrict__ hann, float2* __restrict__ T) {
+                                                 const float* __restrict__ hann, int tile0, float2* __restrict__ T) {
   constexpr int L = BOX / 2, NT = 256, TPR = frc_tpr(BOX), RB = frc_rows_per_block(BOX), PER = L / TPR;
   static_assert(BOX % RB == 0 && L % TPR == 0, "tile rows must split evenly");
   __shared__ float2 buf[RB][L];
@@ -1142,7 +1142,8 @@ __global__ __launch_bounds__(256) void k_frc_rows(const float* __restrict__ even
   const int rb = blk % nrb;
   blk /= nrb;
   const int par = blk % 2;
-  const int tile = blk / 2;
+  const int lt = blk / 2;                  // tile within the launch's range (T holds only those)
+  const int tile = tile0 + lt;
   const int ty = tile / tmax_x, tx = tile % tmax_x;
   if (ty >= info->tiles_y || tx >= info->tiles_x) return;   // block-uniform
   const int oy = (int)(ty * BOX * step + info->crop[0]);
@@ -1171,7 +1172,7 @@ __global__ __launch_bounds__(256) void k_frc_rows(const float* __restrict__ even
   const int Wh = L + 1;
   for (int k = tid; k < Wh; k += NT) {
     const float2 w = tw2L[k];
-    float2* dst = T + (((size_t)tile * 2 + par) * Wh + k) * BOX + rb * RB;
+    float2* dst = T + (((size_t)lt * 2 + par) * Wh + k) * BOX + rb * RB;
 #pragma unroll
     for (int r = 0; r < RB; ++r) {
       const float2 zk = buf[r][k % L];
@@ -1187,6 +1188,10 @@ __global__ __launch_bounds__(256) void k_frc_rows(const float* __restrict__ even
 // Column pass: per kx, FFT the even and odd columns of the tiles of one tile
 // group and accumulate Re(E conj(O)) / sqrt(|O|^2 |E|^2) in FP64 into that
 // group's partial plane (k_frc_reduce adds the groups in a fixed order).
+// Group g owns the tiles t = g (mod FRC_GROUPS) and adds them in ascending
+// order; a launch covers the tiles [tile0, tile1) held in T, and with
+// `accumulate` it continues the group's sum from `part`, so that any split of
+// the tiles into consecutive ranges gives the same bits as one launch.
 constexpr int FRC_GROUPS = 16;
 // Tiles transformed together per k_frc_cols block (even and odd column of each,
 // one group of BOX/16 threads per transform); the ratios are added in the
@@ -1196,7 +1201,8 @@ constexpr int frc_cols_nt(int box) { return 2 * FRC_TB * (box / 16); }
 template <int BOX>
 __global__ __launch_bounds__(frc_cols_nt(BOX)) void k_frc_cols(const float2* __restrict__ T,
                                                               const SolveInfo* __restrict__ info, int tmax_x,
-                                                              int ntiles_max, const float2* __restrict__ twB,
+                                                              int tile0, int tile1, int accumulate,
+                                                              const float2* __restrict__ twB,
                                                               double* __restrict__ part) {
   constexpr int TPC = BOX / 16, NT = frc_cols_nt(BOX), PER = BOX / NT;
   static_assert(BOX % NT == 0, "column must split evenly over the block");
@@ -1206,16 +1212,19 @@ __global__ __launch_bounds__(frc_cols_nt(BOX)) void k_frc_cols(const float2* __r
   const int q = tid / TPC, gt = tid - q * TPC;  // transform group: tile slot q / 2, parity q % 2
   const int Wh = BOX / 2 + 1;
   const int k = blockIdx.x % Wh, g = blockIdx.x / Wh;
+  double* dst = part + (size_t)g * BOX * Wh;
   double acc[PER];
 #pragma unroll
-  for (int i = 0; i < PER; ++i) acc[i] = 0.0;
-  for (int base = g; base < ntiles_max; base += FRC_GROUPS * FRC_TB) {
+  for (int i = 0; i < PER; ++i) acc[i] = accumulate ? dst[(size_t)(tid + i * NT) * Wh + k] : 0.0;
+  // the group's first tile in the range
+  const int first = tile0 + (((g - tile0) % FRC_GROUPS) + FRC_GROUPS) % FRC_GROUPS;
+  for (int base = first; base < tile1; base += FRC_GROUPS * FRC_TB) {
     {
       const int tile = base + (q >> 1) * FRC_GROUPS;
       const int ty = tile / tmax_x, tx = tile % tmax_x;
-      const bool ok = tile < ntiles_max && ty < info->tiles_y && tx < info->tiles_x;
+      const bool ok = tile < tile1 && ty < info->tiles_y && tx < info->tiles_x;
       if (gt == 0 && (q & 1) == 0) live[q >> 1] = ok ? 1 : 0;
-      const float2* col = T + (((size_t)tile * 2 + (q & 1)) * Wh + k) * BOX;
+      const float2* col = T + (((size_t)(tile - tile0) * 2 + (q & 1)) * Wh + k) * BOX;
 #pragma unroll
       for (int i = 0; i < BOX / TPC; ++i) {
         float2 v = make_float2(0.f, 0.f);
@@ -1238,7 +1247,6 @@ __global__ __launch_bounds__(frc_cols_nt(BOX)) void k_frc_cols(const float2* __r
     }
     __syncthreads();
   }
-  double* dst = part + (size_t)g * BOX * Wh;
 #pragma unroll
   for (int i = 0; i < PER; ++i) dst[(size_t)(tid + i * NT) * Wh + k] = acc[i];
 }
@@ -1717,6 +1725,28 @@ __global__ __launch_bounds__(256) void k_dose_cols_ifft(const float2* __restrict
   for (int i = 0; i < PER; ++i) col[tid + i * NT] = buf[tid + i * NT];
 }
 
+// Standalone tiledFRC: forward twiddle tables of the tile transforms
+// (e^{-2 pi i k/n} from the angle in FP64) and the tile grid, written on the
+// device so that the call needs no host-to-device copy.
+__global__ __launch_bounds__(256) void k_frc_setup(int box, float2* __restrict__ twB, float2* __restrict__ twB2,
+                                                   SolveInfo* __restrict__ info, int tiles_y, int tiles_x, int c0,
+                                                   int c1, int c2, int c3) {
+  const int e = blockIdx.x * 256 + threadIdx.x;
+  if (e < box) {
+    const double a = -2.0 * M_PI * (double)e / (double)box;
+    twB[e] = make_float2((float)cos(a), (float)sin(a));
+  }
+  if (e < box / 2) {
+    const double a = -2.0 * M_PI * (double)e / (double)(box / 2);
+    twB2[e] = make_float2((float)cos(a), (float)sin(a));
+  }
+  if (e == 0) {
+    info->tiles_y = tiles_y;
+    info->tiles_x = tiles_x;
+    info->crop[0] = c0; info->crop[1] = c1; info->crop[2] = c2; info->crop[3] = c3;
+  }
+}
+
 // ---------------------------------------------------------------- gausslp
 // filterMode 'gausslp' (zorro/zorro.py:1707-1711):
 // scipy.ndimage.gaussian_filter(img, 3.0) of a float32 image, i.e. for axis 0
@@ -2163,10 +2193,10 @@ int zr_stage_frc(zr_stage_state* s, hipStream_t st) {
 #define ZR_FRC(BX)                                                                                            \
   case BX:                                                                                                    \
     hipLaunchKernelGGL(k_frc_rows<BX>, dim3(nt * 2 * (BX / frc_rows_per_block(BX))), dim3(256), 0, st, s->even, s->odd, s->info, s->W, \
-                       s->tmax_x, step, s->twB2, s->twB, s->hann, s->Tfrc);                                   \
+                       s->tmax_x, step, s->twB2, s->twB, s->hann, 0, s->Tfrc);                                \
     ZS_HIP(hipGetLastError());                                                                                \
     hipLaunchKernelGGL(k_frc_cols<BX>, dim3((BX / 2 + 1) * FRC_GROUPS), dim3(frc_cols_nt(BX)), 0, st, s->Tfrc, s->info, \
-                       s->tmax_x, nt, s->twB, s->frc_part);                                                  \
+                       s->tmax_x, 0, nt, 0, s->twB, s->frc_part);                                            \
     ZS_HIP(hipGetLastError());                                                                                \
     hipLaunchKernelGGL(k_frc_reduce, dim3(256), dim3(256), 0, st, s->frc_part, (size_t)BX * (BX / 2 + 1),      \
                        s->frc2d);                                                                             \
@@ -2320,6 +2350,89 @@ int zr_stage_get_hist(zr_stage_state* s, float* pdf, float* cdf, float* hsigma, 
     if (cdf) ZS_HIP(hipMemcpy(cdf, s->cdf, sizeof(float) * nb, hipMemcpyDeviceToHost));
     if (hsigma) ZS_HIP(hipMemcpy(hsigma, s->hs, sizeof(float) * nb, hipMemcpyDeviceToHost));
   }
+  return ZR_OK;
+}
+
+// ---- standalone tiledFRC of two images (zr_tiled_frc, zr_tiled_frc_noalloc)
+static size_t frc_al(size_t b) { return (b + 255) & ~(size_t)255; }
+
+size_t zr_frc_work_bytes(int box, int chunk) {
+  const size_t wh = box / 2 + 1;
+  return frc_al(sizeof(SolveInfo)) + frc_al(sizeof(float2) * box) + frc_al(sizeof(float2) * (box / 2)) +
+         frc_al(sizeof(float) * box * box) + frc_al(sizeof(float2) * (size_t)chunk * 2 * wh * box) +
+         frc_al(sizeof(double) * FRC_GROUPS * box * wh) + frc_al(sizeof(double) * box * wh);
+}
+
+int zr_frc_rings_count(int box) {
+  // rotmean's length: ceil(sqrt((box/2)^2 + (box/2)^2)) + 1 (zorro/zorro_util.py:174)
+  const double h = box / 2;
+  return (int)std::ceil(std::sqrt(h * h + h * h)) + 1;
+}
+
+int zr_frc_tiles_from_trans(const double* trans, int n, int h, int w, int box, double overlap, int crop[4],
+                            int* tiles_y, int* tiles_x) {
+  // tiledFRC's crop limits (minCrop 5) and tile counts (zorro/zorro.py:2255-2281)
+  bool any = false;
+  double mny = INFINITY, mxy = -INFINITY, mnx = INFINITY, mxx = -INFINITY;
+  for (int k = 0; trans && k < n; ++k) {
+    const double ty = trans[2 * k], tx = trans[2 * k + 1];
+    mny = std::min(mny, ty); mxy = std::max(mxy, ty); mnx = std::min(mnx, tx); mxx = std::max(mxx, tx);
+    if (ty != 0.0 || tx != 0.0) any = true;
+  }
+  if (!any) { crop[0] = crop[1] = crop[2] = crop[3] = 5; }
+  else {
+    crop[2] = (int)(-std::min(std::floor(mny), 5.0));
+    crop[3] = (int)(-std::min(std::floor(mnx), 5.0));
+    crop[0] = (int)std::max(std::ceil(mxy), 5.0);
+    crop[1] = (int)std::max(std::ceil(mxx), 5.0);
+  }
+  const double ov = overlap > 0.8 ? 0.8 : overlap;
+  *tiles_x = (int)std::floor((double)(w - crop[1] - crop[3] - box) / box / (1.0 - ov));
+  *tiles_y = (int)std::floor((double)(h - crop[0] - crop[2] - box) / box / (1.0 - ov));
+  return (*tiles_x > 0 && *tiles_y > 0) ? ZR_OK : ZR_ERR_ARG;
+}
+
+int zr_frc_tiles_run(const float* img1, const float* img2, int h, int w, const int crop[4], int tiles_y, int tiles_x,
+                     int box, double overlap, void* work, int chunk, double* frc, hipStream_t st) {
+  (void)h;
+  const size_t wh = box / 2 + 1;
+  char* p = static_cast<char*>(work);
+  auto take = [&](size_t b) { char* q = p; p += frc_al(b); return q; };
+  SolveInfo* info = reinterpret_cast<SolveInfo*>(take(sizeof(SolveInfo)));
+  float2* twB = reinterpret_cast<float2*>(take(sizeof(float2) * box));
+  float2* twB2 = reinterpret_cast<float2*>(take(sizeof(float2) * (box / 2)));
+  float* hann = reinterpret_cast<float*>(take(sizeof(float) * box * box));
+  float2* T = reinterpret_cast<float2*>(take(sizeof(float2) * (size_t)chunk * 2 * wh * box));
+  double* part = reinterpret_cast<double*>(take(sizeof(double) * FRC_GROUPS * box * wh));
+  double* frc2d = reinterpret_cast<double*>(take(sizeof(double) * box * wh));
+  const double step = 1.0 - (overlap > 0.8 ? 0.8 : overlap);
+  const int nt = tiles_y * tiles_x;
+  hipLaunchKernelGGL(k_frc_setup, dim3((box + 255) / 256), dim3(256), 0, st, box, twB, twB2, info, tiles_y, tiles_x,
+                     crop[0], crop[1], crop[2], crop[3]);
+  ZS_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_frc_hann, dim3((box * box + 255) / 256), dim3(256), 0, st, box, hann);
+  ZS_HIP(hipGetLastError());
+  for (int t0 = 0; t0 < nt; t0 += chunk) {
+    const int t1 = std::min(nt, t0 + chunk);
+    switch (box) {
+#define ZR_FRCT(BX)                                                                                             \
+  case BX:                                                                                                      \
+    hipLaunchKernelGGL(k_frc_rows<BX>, dim3((t1 - t0) * 2 * (BX / frc_rows_per_block(BX))), dim3(256), 0, st, img1,  \
+                       img2, info, w, tiles_x, step, twB2, twB, hann, t0, T);                                  \
+    ZS_HIP(hipGetLastError());                                                                                  \
+    hipLaunchKernelGGL(k_frc_cols<BX>, dim3((BX / 2 + 1) * FRC_GROUPS), dim3(frc_cols_nt(BX)), 0, st, T, info,   \
+                       tiles_x, t0, t1, t0 > 0 ? 1 : 0, twB, part);                                            \
+    ZS_HIP(hipGetLastError());                                                                                  \
+    break;
+      ZR_FRCT(256) ZR_FRCT(512) ZR_FRCT(1024)
+#undef ZR_FRCT
+      default: return serr(ZR_ERR_UNSUPPORTED, "FRC box %d: built sizes 256, 512, 1024", box);
+    }
+  }
+  hipLaunchKernelGGL(k_frc_reduce, dim3(256), dim3(256), 0, st, part, (size_t)box * wh, frc2d);
+  ZS_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_frc_rings, dim3(zr_frc_rings_count(box)), dim3(256), 0, st, frc2d, box, frc);
+  ZS_HIP(hipGetLastError());
   return ZR_OK;
 }
 

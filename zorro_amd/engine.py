@@ -392,3 +392,32 @@ class MoviePlan:
         buf = np.zeros(4096)
         L.check(self.lib.zr_get_frc(self._mv, L.dptr(buf), C.byref(n)))
         return buf[:n.value].copy()
+
+
+def tiled_frc(img1, img2, trans=None, box=512, overlap=0.5, device=0):
+    """tiledFRC of two images on the GPU (``zorro/zorro.py:2246-2310``):
+    returns the whole rotational-mean FRC curve (float64, ceil(sqrt(2) box/2) + 1
+    values).  ``img1``/``img2``: float32 [h, w] NumPy arrays or CUDA tensors;
+    ``trans``: [n, 2] translations for the crop limits (None: none)."""
+    import torch
+    lib = L.load()
+    dev = torch.device("cuda", device)
+
+    def on_dev(a):
+        t = a if isinstance(a, torch.Tensor) else torch.from_numpy(np.ascontiguousarray(a, np.float32))
+        return t.to(dev, torch.float32).contiguous()
+
+    a, b = on_dev(img1), on_dev(img2)
+    if a.shape != b.shape or a.dim() != 2:
+        raise ValueError(f"tiledFRC needs two images of one shape (got {tuple(a.shape)}, {tuple(b.shape)})")
+    h, w = (int(v) for v in a.shape)
+    t = None if trans is None else np.ascontiguousarray(np.asarray(trans, np.float64)[:, :2])
+    nr = int(np.ceil(np.sqrt(2.0) * (box // 2))) + 2
+    out = torch.zeros(nr, dtype=torch.float64, device=dev)
+    n = C.c_int()
+    stream = torch.cuda.current_stream(dev).cuda_stream
+    L.check(lib.zr_tiled_frc(C.c_void_p(a.data_ptr()), C.c_void_p(b.data_ptr()), h, w,
+                             None if t is None else t.ctypes.data_as(C.POINTER(C.c_double)),
+                             0 if t is None else t.shape[0], int(box), float(overlap),
+                             C.c_void_p(out.data_ptr()), C.byref(n), C.c_void_p(stream)))
+    return out[:n.value].cpu().numpy()

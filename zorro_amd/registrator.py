@@ -26,7 +26,7 @@ import time
 import numpy as np
 
 from . import mrc, zorfile
-from .engine import MoviePlan, make_params
+from .engine import MoviePlan, make_params, tiled_frc
 
 
 class ImageRegistrator:
@@ -502,6 +502,19 @@ class ImageRegistrator:
         finally:
             eo.close()
         return self
+
+    def tiledFRC(self, Image1, Image2, trans=None, box=(512, 512), overlap=0.5):
+        """Tiled, Hann-apodised FRC of two images (``zorro/zorro.py:2246-2310``)
+        on the GPU; sets and returns ``self.FRC`` (the whole rotational mean;
+        lazyFouRingCorr keeps the first box/2 values).  ``trans`` defaults to
+        ``self.translations``, as in the reference.  The 2D FRC stays on the
+        device (``FRC2D`` is not set)."""
+        if box[0] != box[1]:
+            raise NotImplementedError("square FRC boxes only")
+        if trans is None:
+            trans = self.translations
+        self.FRC = tiled_frc(Image1, Image2, trans, int(box[0]), overlap, device=self.device)
+        return self.FRC
 
     def close(self):
         if self._plan is not None:
