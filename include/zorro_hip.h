@@ -310,6 +310,17 @@ int zr_copy_sums_device(zr_movie* mv, float* image_sum, float* filt_sum, float* 
 int zr_tiled_frc(const float* img1, const float* img2, int h, int w, const double* trans, int n, int box,
                  double overlap, double* frc, int* nrings, void* stream);
 
+/* zr_tiled_frc without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * arguments and result as zr_tiled_frc (the same bits), box 256 or 512 only
+ * (ZR_ERR_UNSUPPORTED beyond: its workspace is a 28 MiB array that each device
+ * the library runs on holds until the process ends).  Asynchronous on
+ * `stream`: the result is ready when the stream has passed the call.  Calls
+ * that overlap on one device share that workspace and race; serialise them. */
+int zr_tiled_frc_noalloc(const float* img1, const float* img2, int h, int w, const double* trans, int n, int box,
+                         double overlap, double* frc, int* nrings, void* stream);
+
 /* Device pointers of results for zero-copy consumers (valid until destroy). */
 int zr_device_results(zr_movie* mv, const float** image_sum, const float** filt_sum, const double** translations);
 

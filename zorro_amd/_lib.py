@@ -78,6 +78,7 @@ SIGNATURES = {
     "zr_post_filters": (C.c_int, [_P]),
     "zr_get_peaksig_hist": (C.c_int, [_P, _F, _F, _F, C.c_int, _I]),
     "zr_tiled_frc": (C.c_int, [_P, _P, C.c_int, C.c_int, _D, C.c_int, C.c_int, C.c_double, _P, _I, _P]),
+    "zr_tiled_frc_noalloc": (C.c_int, [_P, _P, C.c_int, C.c_int, _D, C.c_int, C.c_int, C.c_double, _P, _I, _P]),
     "zr_align": (C.c_int, [_P]),
     "zr_synchronize": (C.c_int, [_P]),
     "zr_movie_use_graph": (C.c_int, [_P, C.c_int]),

@@ -6,6 +6,7 @@
 
 #include <cstdarg>
 #include <cstdio>
+#include <mutex>
 #include <string>
 
 #include "../../include/zorro_hip.h"
@@ -53,6 +54,64 @@ extern "C" int zr_tiled_frc(const float* img1, const float* img2, int h, int w, 
   hipFree(work);
   if (rc) return frc_fail(rc, zr_stage_error());
   if (e != hipSuccess) return frc_fail(ZR_ERR_HIP, "tiledFRC: %s", hipGetErrorString(e));
+  if (nrings) *nrings = zr_frc_rings_count(box);
+  return ZR_OK;
+}
+
+// ---------------------------------------------------------------- no allocation
+// Workspace of zr_tiled_frc_noalloc: one block per device that the library runs
+// on, held until the process ends.  28 MiB holds the box-512 workspace with 4
+// tiles per chunk (26.1 MiB: tables, Hann window, 4 tile row-transforms, the
+// 16 group planes and the 2D FRC); smaller boxes take more tiles per chunk.
+constexpr size_t ZR_FRC_SCRATCH_BYTES = (size_t)28 << 20;
+constexpr int ZR_FRC_NOALLOC_MAX_BOX = 512;
+static __device__ __align__(256) unsigned char scratch[ZR_FRC_SCRATCH_BYTES];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch[kMaxDevices] = {};
+
+// The device address of `scratch` on the current device (looked up once per
+// device; each device holds its own copy of the array at its own address).
+int scratch_for_current_device(void** out) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return frc_fail(ZR_ERR_HIP, "hipGetDevice: %s", hipGetErrorString(e));
+  if (dev < 0 || dev >= kMaxDevices) return frc_fail(ZR_ERR_UNSUPPORTED, "device %d beyond %d", dev, kMaxDevices);
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch[dev]) {
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return frc_fail(ZR_ERR_HIP, "hipGetSymbolAddress(scratch) on device %d: %s", dev, hipGetErrorString(e));
+    g_scratch[dev] = p;
+  }
+  *out = g_scratch[dev];
+  return ZR_OK;
+}
+}  // namespace
+
+extern "C" int zr_tiled_frc_noalloc(const float* img1, const float* img2, int h, int w, const double* trans, int n,
+                                    int box, double overlap, double* frc, int* nrings, void* stream) {
+  int crop[4], ty = 0, tx = 0;
+  int rc = frc_check(img1, img2, h, w, trans, n, box, overlap, frc, crop, &ty, &tx);
+  if (rc) return rc;
+  if (box > ZR_FRC_NOALLOC_MAX_BOX)
+    return frc_fail(ZR_ERR_UNSUPPORTED, "tiledFRC without allocation: box %d exceeds %d (its %zu-byte workspace)",
+                    box, ZR_FRC_NOALLOC_MAX_BOX, ZR_FRC_SCRATCH_BYTES);
+  // as many tiles per chunk as the workspace holds (the sums do not depend on it)
+  const int nt = ty * tx;
+  int chunk = 1;
+  while (chunk < nt && zr_frc_work_bytes(box, chunk + 1) <= ZR_FRC_SCRATCH_BYTES) ++chunk;
+  if (zr_frc_work_bytes(box, chunk) > ZR_FRC_SCRATCH_BYTES)
+    return frc_fail(ZR_ERR_UNSUPPORTED, "tiledFRC without allocation: box %d does not fit %zu bytes", box,
+                    ZR_FRC_SCRATCH_BYTES);
+  void* work = nullptr;
+  if ((rc = scratch_for_current_device(&work))) return rc;
+  rc = zr_frc_tiles_run(img1, img2, h, w, crop, ty, tx, box, overlap, work, chunk, frc,
+                        static_cast<hipStream_t>(stream));
+  if (rc) return frc_fail(rc, zr_stage_error());
   if (nrings) *nrings = zr_frc_rings_count(box);
   return ZR_OK;
 }
