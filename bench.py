@@ -112,6 +112,29 @@ def pmc_summary(fetch_csv, write_csv, out_json):
     print(f"pair transform traffic per launch: {traffic / 1e9:.3f} GB")
 
 
+def counter_summary(paths, kernels=None):
+    """Per-kernel mean of every counter in rocprofv3 --pmc CSV files (one
+    counter pass each), printed as a markdown table; derived ratios for the
+    VALU, LDS and L2 counters when present."""
+    import csv
+    import collections
+    vals = collections.defaultdict(lambda: collections.defaultdict(list))
+    for path in paths:
+        for r in csv.DictReader(open(path)):
+            name = r["Kernel_Name"].split("(")[0].replace("void ", "").strip()
+            if kernels and not any(name.startswith(k) for k in kernels):
+                continue
+            vals[name][r["Counter_Name"]].append(float(r["Counter_Value"]))
+    counters = sorted({c for k in vals for c in vals[k]})
+    print("| kernel | dispatches | " + " | ".join(counters) + " |")
+    print("|---|---|" + "---|" * len(counters))
+    for k in sorted(vals, key=lambda k: -len(vals[k])):
+        n = max(len(v) for v in vals[k].values())
+        row = [f"{np.mean(vals[k][c]):.4g}" if c in vals[k] else "" for c in counters]
+        print(f"| {k} | {n} | " + " | ".join(row) + " |")
+    return {k: {c: float(np.mean(v)) for c, v in d.items()} for k, d in vals.items()}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -122,6 +145,9 @@ def main():
                     help="run a plain 1 GiB device copy first (reference for rocprofv3 FETCH/WRITE counters)")
     ap.add_argument("--pmc-summary", nargs=3, metavar=("FETCH_CSV", "WRITE_CSV", "OUT_JSON"),
                     help="summarise two rocprofv3 --pmc counter_collection.csv files and exit (no GPU)")
+    ap.add_argument("--counter-summary", nargs="+", metavar="CSV",
+                    help="per-kernel means of rocprofv3 --pmc counter_collection.csv files and exit (no GPU)")
+    ap.add_argument("--kernels", nargs="*", help="with --counter-summary: kernel name prefixes to keep")
     ap.add_argument("--workload", default="c2", choices=["c2", "served", "batched"])
     ap.add_argument("--inflight", type=int, default=1,
                     help="C2 movies in flight at once (one plan and stream each)")
@@ -133,6 +159,9 @@ def main():
     args = ap.parse_args()
     if args.pmc_summary:
         pmc_summary(*args.pmc_summary)
+        return
+    if args.counter_summary:
+        counter_summary(args.counter_summary, args.kernels)
         return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "

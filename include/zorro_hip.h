@@ -93,6 +93,12 @@ typedef struct zr_params {
    * is set by zr_movie_create when a DOSE or DOSENORM step is present. */
   int n_filters;
   int filters[4];
+  /* launch tuning (no reference counterpart beyond FFTW's planning effort and
+   * wisdom, zorro/zorro_util.py:966-1066); 0 picks the built-in default:
+   * frames per forward-spectra launch pair (default 8) and lag-window columns
+   * per wave of the pair transform (default 8).  Results do not depend on them. */
+  int k1_batch;
+  int k2_cols_per_wave;
 } zr_params;
 
 /* filterMode steps (zorro/zorro.py:1692-1711).  DOSE: doseFilter(normalize=False),

@@ -242,7 +242,7 @@ static hipError_t launch_bin_rows_t(const void* raw, int dtype, int frame0, int 
                                     float2* Tr, hipStream_t st) {
   static std::once_flag lds_once;
   zr_allow_full_lds((const void*)k_bin_rows<M>, lds_once);
-  hipLaunchKernelGGL(k_bin_rows<M>, dim3(nb * H), dim3(1024), M * sizeof(float2), st, raw, dtype, frame0, H, W, gain,
+  hipLaunchKernelGGL(k_bin_rows<M>, dim3(nb * H), dim3(1024), zr_fft_lds(M) * sizeof(float2), st, raw, dtype, frame0, H, W, gain,
                      gscale, use_scale, cw, twM, Wb, Tr);
   return hipGetLastError();
 }
@@ -252,7 +252,7 @@ static hipError_t launch_bin_cols_t(const float2* Tc, int nb, int H, int Wb, int
                                     const BinChirp& chb, const float2* twM, float scale, float2* Gc, hipStream_t st) {
   static std::once_flag lds_once;
   zr_allow_full_lds((const void*)k_bin_cols<M>, lds_once);
-  hipLaunchKernelGGL(k_bin_cols<M>, dim3(nb * Wb), dim3(1024), M * sizeof(float2), st, Tc, H, Wb, Hb, ch, chb, twM,
+  hipLaunchKernelGGL(k_bin_cols<M>, dim3(nb * Wb), dim3(1024), zr_fft_lds(M) * sizeof(float2), st, Tc, H, Wb, Hb, ch, chb, twM,
                      scale, Gc);
   return hipGetLastError();
 }
@@ -262,7 +262,7 @@ static hipError_t launch_bin_out_t(const float2* Gr, int nb, int Hb, int Wb, con
                                    int frame0, float* frames, hipStream_t st) {
   static std::once_flag lds_once;
   zr_allow_full_lds((const void*)k_bin_out<M>, lds_once);
-  hipLaunchKernelGGL(k_bin_out<M>, dim3(nb * Hb), dim3(1024), M * sizeof(float2), st, Gr, Hb, Wb, cwb, twM, frame0,
+  hipLaunchKernelGGL(k_bin_out<M>, dim3(nb * Hb), dim3(1024), zr_fft_lds(M) * sizeof(float2), st, Gr, Hb, Wb, cwb, twM, frame0,
                      frames);
   return hipGetLastError();
 }

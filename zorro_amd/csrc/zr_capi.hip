@@ -142,6 +142,7 @@ struct zr_movie {
   bool spectral = false;
   int Wk = 0, Hs = 0, hoff = 0, side_in_col = -1;
   int batch = 1, rb = 4;
+  int k2_cpw = 8;                  // lag-window columns per wave of the pair transform
   // frames
   const float* frames = nullptr;
   float* frames_owned = nullptr;
@@ -522,7 +523,9 @@ static int create_movie(int device, const zr_params* p, int n_frames, int h, int
   }
   // workspaces
   const size_t N = mv->N;
-  mv->batch = (int)std::min<size_t>(N, 8);
+  mv->batch = (int)std::min<size_t>(N, p->k1_batch > 0 ? p->k1_batch : 8);
+  if (p->k2_cols_per_wave < 0 || p->k2_cols_per_wave > 64) { free_movie(mv); return fail(ZR_ERR_ARG, "k2_cols_per_wave %d outside 0..64", p->k2_cols_per_wave); }
+  mv->k2_cpw = p->k2_cols_per_wave > 0 ? p->k2_cols_per_wave : 8;
   mv->rb = zr_k1_rows_per_block(mv->Wp);
   const int nrb = mv->Hp / mv->rb;
   mv->spectral = p->do_dose && mv->unmasked && mv->H == mv->W && mv->Hp <= 4096;
@@ -882,11 +885,11 @@ static int xcorr_pairs(zr_movie* mv, double nc2, bool refine) {
     return ZR_OK;
   }
   if (mv->k2ltw)
-    ZR_HIP(zr_launch_k2_cols_r2(mv->Lper, mv->S, mv->Sside, mv->work, mv->Ptot, mv->Wh, mv->Hc, mv->Wc, mv->m, 8,
-                                mv->unmasked ? 1 : 0, mv->k2ltw, mv->Z, mv->Wk, mv->Hs, mv->hoff, mv->side_in_col,
-                                nullptr, nullptr, nullptr, nullptr, s));
+    ZR_HIP(zr_launch_k2_cols_r2(mv->Lper, mv->S, mv->Sside, mv->work, mv->Ptot, mv->Wh, mv->Hc, mv->Wc, mv->m,
+                                mv->k2_cpw, mv->unmasked ? 1 : 0, mv->k2ltw, mv->Z, mv->Wk, mv->Hs, mv->hoff,
+                                mv->side_in_col, nullptr, nullptr, nullptr, nullptr, s));
   else
-    ZR_HIP(zr_launch_k2_cols(mv->Lper, mv->S, mv->Sside, mv->work, mv->Ptot, mv->Wh, mv->Hc, mv->Wc, mv->m, 8,
+    ZR_HIP(zr_launch_k2_cols(mv->Lper, mv->S, mv->Sside, mv->work, mv->Ptot, mv->Wh, mv->Hc, mv->Wc, mv->m, mv->k2_cpw,
                              mv->unmasked ? 1 : 0, mv->twHc, mv->Z, mv->Wk, mv->Hs, mv->hoff, mv->side_in_col, nullptr,
                              nullptr, nullptr, s));
   ZR_HIP(zr_launch_k2_rows(mv->Z, mv->pairs, mv->Ptot, mv->Wh, mv->Wc, mv->m, mv->twWc, mv->k2btab, mv->den,

@@ -203,10 +203,20 @@ constexpr bool zr_smooth(int n) {
   return n == 1;
 }
 
+// LDS layout of the passes between the first and the last: one padding
+// element after every 16, so that the radix-R outputs a thread writes side by
+// side (R consecutive float2) land in different banks for neighbouring lanes.
+// The first pass reads the caller's natural order and the last pass writes it
+// back, so callers fill and read buf[0..L) naturally; the buffer must hold
+// zr_fft_lds(L) elements.
+ZR_DEV int zr_pad16(int i) { return i + (i >> 4); }
+constexpr int zr_fft_lds(int L) { return L + L / 16 + 1; }
+
 // One Stockham pass of radix R with stride Ls over an LDS-resident sequence of
 // length L, in place (all reads of the pass complete before any write).
 // tw points at exp(-2*pi*i*t/L), t in [0,L) (forward); DIR=+1 uses conj.
-template <int L, int R, int Ls, int DIR, int NT>
+// INP / OUTP: the pass reads / writes the padded layout.
+template <int L, int R, int Ls, int DIR, int NT, bool INP, bool OUTP>
 ZR_DEV void stockham_pass(float2* buf, const float2* __restrict__ tw, int tid) {
   constexpr int NB = L / R;                       // butterflies
   constexpr int PER = (NB + NT - 1) / NT;         // per thread
@@ -217,7 +227,10 @@ ZR_DEV void stockham_pass(float2* buf, const float2* __restrict__ tw, int tid) {
     if (NB % NT == 0 || j < NB) {
       int k = j % Ls;
 #pragma unroll
-      for (int r = 0; r < R; ++r) v[b][r] = buf[j + r * NB];
+      for (int r = 0; r < R; ++r) {
+        const int i = j + r * NB;
+        v[b][r] = buf[INP ? zr_pad16(i) : i];
+      }
       if (Ls > 1) {
 #pragma unroll
         for (int r = 1; r < R; ++r) {
@@ -237,7 +250,10 @@ ZR_DEV void stockham_pass(float2* buf, const float2* __restrict__ tw, int tid) {
       int k = j % Ls;
       int base = (j / Ls) * Ls * R + k;
 #pragma unroll
-      for (int r = 0; r < R; ++r) buf[base + r * Ls] = v[b][r];
+      for (int r = 0; r < R; ++r) {
+        const int i = base + r * Ls;
+        buf[OUTP ? zr_pad16(i) : i] = v[b][r];
+      }
     }
   }
   __syncthreads();
@@ -247,7 +263,8 @@ template <int L, int Ls, int DIR, int NT> struct WgFft {
   static ZR_DEV void run(float2* buf, const float2* __restrict__ tw, int tid) {
     constexpr int R = zr_next_radix(L / Ls);
     static_assert(R > 1, "FFT length must be 2^a 3^b 5^c");
-    stockham_pass<L, R, Ls, DIR, NT>(buf, tw, tid);
+    constexpr bool first = (Ls == 1), last = (Ls * R == L);
+    stockham_pass<L, R, Ls, DIR, NT, !first, !last>(buf, tw, tid);
     WgFft<L, Ls * R, DIR, NT>::run(buf, tw, tid);
   }
 };
@@ -256,6 +273,7 @@ template <int L, int DIR, int NT> struct WgFft<L, L, DIR, NT> {
 };
 
 // Whole-workgroup complex FFT of length L held in LDS (natural order in/out).
+// buf holds zr_fft_lds(L) elements (the passes in between use the padding).
 // The caller must __syncthreads() after filling buf; returns synchronised.
 template <int L, int DIR, int NT>
 ZR_DEV void wg_fft(float2* buf, const float2* __restrict__ tw, int tid) {

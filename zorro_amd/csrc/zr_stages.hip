@@ -1134,7 +1134,7 @@ __global__ __launch_bounds__(256) void k_frc_rows(const float* __restrict__ even
                                                  const float* __restrict__ hann, int tile0, float2* __restrict__ T) {
   constexpr int L = BOX / 2, NT = 256, TPR = frc_tpr(BOX), RB = frc_rows_per_block(BOX), PER = L / TPR;
   static_assert(BOX % RB == 0 && L % TPR == 0, "tile rows must split evenly");
-  __shared__ float2 buf[RB][L];
+  __shared__ float2 buf[RB][zr_fft_lds(L)];
   const int tid = threadIdx.x;
   const int g = tid / TPR, gt = tid - g * TPR;   // row group, thread in group
   constexpr int nrb = BOX / RB;
@@ -1206,7 +1206,7 @@ __global__ __launch_bounds__(frc_cols_nt(BOX)) void k_frc_cols(const float2* __r
                                                               double* __restrict__ part) {
   constexpr int TPC = BOX / 16, NT = frc_cols_nt(BOX), PER = BOX / NT;
   static_assert(BOX % NT == 0, "column must split evenly over the block");
-  __shared__ float2 buf[2 * FRC_TB][BOX];     // [2 j + parity]
+  __shared__ float2 buf[2 * FRC_TB][zr_fft_lds(BOX)];     // [2 j + parity]
   __shared__ int live[FRC_TB];
   const int tid = threadIdx.x;
   const int q = tid / TPC, gt = tid - q * TPC;  // transform group: tile slot q / 2, parity q % 2
@@ -1321,7 +1321,7 @@ __global__ __launch_bounds__(1024) void k_dose_rows(const float* __restrict__ fr
                                                    double* __restrict__ rowsum) {
   constexpr int L = WP / 2, RB = 4, NTG = 256, PX = (WP + NTG - 1) / NTG;
   extern __shared__ float2 kd_lds[];
-  float2 (*buf)[L] = reinterpret_cast<float2 (*)[L]>(kd_lds);
+  float2 (*buf)[zr_fft_lds(L)] = reinterpret_cast<float2 (*)[zr_fft_lds(L)]>(kd_lds);
   __shared__ double red[16];
   const int grp = threadIdx.x >> 8, gtid = threadIdx.x & 255;
   const int blk = xcd_remap(blockIdx.x, gridDim.x);
@@ -1455,7 +1455,7 @@ __global__ __launch_bounds__(dose_cols_nt(HP)) void k_dose_cols(const float2* __
                                                                float2* __restrict__ G) {
   constexpr int NT = dose_cols_nt(HP), PER = HP / NT;
   static_assert(HP % NT == 0, "column length must split evenly over the block");
-  __shared__ float2 buf[HP];
+  __shared__ float2 buf[zr_fft_lds(HP)];
   const int tid = threadIdx.x;
   const int kx = xcd_remap(blockIdx.x, gridDim.x);
   const int Wh = a.Wp / 2 + 1;
@@ -1513,7 +1513,7 @@ __global__ __launch_bounds__(256) void k_dose_out(const float2* __restrict__ G, 
                                                  const float2* __restrict__ twL, const float2* __restrict__ tw2L,
                                                  float* __restrict__ filt_sum) {
   constexpr int L = WP / 2, NT = 256;
-  __shared__ float2 buf[L];
+  __shared__ float2 buf[zr_fft_lds(L)];
   const int y = xcd_remap(blockIdx.x, gridDim.x);
   const int tid = threadIdx.x;
   for (int k = tid; k < L; k += NT) {
@@ -1574,7 +1574,7 @@ __global__ __launch_bounds__(256) void k_dose_border(const float* __restrict__ f
                                                     float2* __restrict__ C0, float2* __restrict__ CW,
                                                     float2* __restrict__ Py, float2* __restrict__ Px) {
   constexpr int NT = 256;
-  __shared__ float2 buf[LEN];
+  __shared__ float2 buf[zr_fft_lds(LEN)];
   const int J = blockIdx.x / 4, line = blockIdx.x % 4;
   const FrameShift s = reduce_shift(fs[J], H, W);
   const float* f = frames + (size_t)J * H * W;
@@ -1711,7 +1711,7 @@ template <int HP>
 __global__ __launch_bounds__(256) void k_dose_cols_ifft(const float2* __restrict__ twH, float2* __restrict__ G) {
   constexpr int NT = 256, PER = HP / NT;
   static_assert(HP % NT == 0, "column must split evenly over the block");
-  __shared__ float2 buf[HP];
+  __shared__ float2 buf[zr_fft_lds(HP)];
   const int tid = threadIdx.x;
   float2* col = G + (size_t)xcd_remap(blockIdx.x, gridDim.x) * HP;
   float2 v[PER];
@@ -2278,7 +2278,7 @@ int zr_stage_dose(zr_stage_state* s, const float* frames, const float2* S, int W
   if (s->spectral) return dose_spectral(s, frames, S, Wk, normalize, st);
   const int Hp = s->Hp, Wp = s->Wp;
   const int nblk = Hp / 4;
-  size_t sh = (size_t)4 * (Wp / 2) * sizeof(float2);
+  size_t sh = (size_t)4 * zr_fft_lds(Wp / 2) * sizeof(float2);
   switch (Wp) {
 #define ZR_DR(X)                                                                                              \
   case X: {                                                                                                   \

@@ -44,7 +44,7 @@ __global__ __launch_bounds__(RB * TPR) void k1_rows(const float* __restrict__ fr
   constexpr int L = WP / 2, NT = RB * TPR, PER = L / TPR;
   static_assert(L % TPR == 0, "row length must split evenly over the row group");
   extern __shared__ float2 k1_lds[];
-  float2 (*buf)[L] = reinterpret_cast<float2 (*)[L]>(k1_lds);
+  float2 (*buf)[zr_fft_lds(L)] = reinterpret_cast<float2 (*)[zr_fft_lds(L)]>(k1_lds);
   __shared__ double red[2][NT / 64];
   const int tid = threadIdx.x;
   const int g = tid / TPR, gt = tid - g * TPR;     // row group, thread in group
@@ -153,7 +153,7 @@ __global__ __launch_bounds__(2 * NTC) void k1_cols(const float2* __restrict__ T,
   constexpr int PER = (HP + NT - 1) / NT;        // rows per thread in the load
   constexpr int PERC = (HP + NTC - 1) / NTC;     // rows per thread in the store
   extern __shared__ float2 k1c_lds[];
-  float2 (*buf)[HP] = reinterpret_cast<float2 (*)[HP]>(k1c_lds);
+  float2 (*buf)[zr_fft_lds(HP)] = reinterpret_cast<float2 (*)[zr_fft_lds(HP)]>(k1c_lds);
   const int tid = threadIdx.x;
   const int KP = (Wh + 1) / 2;
   const int blk = xcd_remap(blockIdx.x, gridDim.x);
@@ -197,7 +197,7 @@ __global__ __launch_bounds__(NT) void k1_cols1(const float2* __restrict__ T, int
                                                const float2* __restrict__ twH, float2* __restrict__ S,
                                                float2* __restrict__ Sside, int full) {
   constexpr int PER = (HP + NT - 1) / NT;
-  __shared__ float2 buf[HP];
+  __shared__ float2 buf[zr_fft_lds(HP)];
   const int tid = threadIdx.x;
   const int KP = (Wh + 1) / 2;
   const int blk = xcd_remap(blockIdx.x, gridDim.x);
@@ -1359,7 +1359,7 @@ static hipError_t launch_k1_rows_t(const float* frames, int H, int W, int Hp, in
   constexpr int TPR = k1_threads_per_row(WP);
   if (Hp % RB != 0) return hipErrorInvalidValue;
   int grid = nb * (Hp / RB);
-  size_t sh = (size_t)RB * (WP / 2) * sizeof(float2);
+  size_t sh = (size_t)RB * zr_fft_lds(WP / 2) * sizeof(float2);
   static std::once_flag lds_once;
   zr_allow_full_lds((const void*)k1_rows<WP, RB, TPR>, lds_once);
   hipLaunchKernelGGL((k1_rows<WP, RB, TPR>), dim3(grid), dim3(RB * TPR), sh, st, frames, H, W, Hp, Wh, frame0, twL,
@@ -1393,7 +1393,7 @@ hipError_t zr_launch_k1_cols(int Hp, const float2* T, int Wh, int Hc, int frame0
   switch (Hp) {
 #define ZR_CASE(X) case X: { constexpr int NTC = X >= 2048 ? 512 : 256; static std::once_flag lds_once; \
       zr_allow_full_lds((const void*)k1_cols<X, NTC>, lds_once); \
-      hipLaunchKernelGGL((k1_cols<X, NTC>), dim3(grid), dim3(2 * NTC), 2 * X * sizeof(float2), st, T, Wh, Hc, frame0, twH, S, Sside, full); return hipGetLastError(); }
+      hipLaunchKernelGGL((k1_cols<X, NTC>), dim3(grid), dim3(2 * NTC), 2 * zr_fft_lds(X) * sizeof(float2), st, T, Wh, Hc, frame0, twH, S, Sside, full); return hipGetLastError(); }
     ZR_CASE(256) ZR_CASE(512) ZR_CASE(1024) ZR_CASE(2048) ZR_CASE(3072) ZR_CASE(3840) ZR_CASE(4096) ZR_CASE(5760)
 #undef ZR_CASE
     default: return hipErrorInvalidValue;
