@@ -327,6 +327,36 @@ int zr_tiled_frc(const float* img1, const float* img2, int h, int w, const doubl
 int zr_tiled_frc_noalloc(const float* img1, const float* img2, int h, int w, const double* trans, int n, int box,
                          double overlap, double* frc, int* nrings, void* stream);
 
+/* Batches: several movies of one detector shape and one zr_params registered
+ * together.  The reference registers one movie per process and scales by
+ * running processes side by side (automator/zorroSkulkManager.py:1089-1113,
+ * zorro/scripts/zorro_benchmark.py:174-248); a batch instead runs the forward
+ * spectra of every frame, the pair transforms of every movie's pairs (one
+ * ragged pair list) and the peak searches as single launches, then each
+ * movie's solver, shift/filter and FRC stages on a stream of its own, so
+ * that the one-workgroup solvers of the movies overlap.  Every movie's results
+ * equal those of a zr_movie of its own.  triMode 'tri' or 'diag', float32
+ * frames, no preShift, 'hot' or binning; 2..64 frames per movie. */
+typedef struct zr_batch zr_batch;
+int zr_batch_create(int device, const zr_params* p, int n_movies, const int* n_frames, int h, int w,
+                    zr_batch** out);
+void zr_batch_destroy(zr_batch* b);
+/* Movie i of the batch as a zr_movie for the zr_get_* calls, zr_movie_info and
+ * zr_copy_sums_device (owned by the batch: zr_movie_destroy ignores it). */
+int zr_batch_movie(zr_batch* b, int i, zr_movie** view);
+/* Device address of movie i's frames f32 [n_i][h][w] inside the batch (a
+ * producer may write them there directly). */
+int zr_batch_frames(zr_batch* b, int i, float** frames);
+/* Copy movie i's frames in (host: returns when they may be reused). */
+int zr_batch_set_frames(zr_batch* b, int i, const float* frames, int on_device);
+/* enable != 0: the next zr_batch_align captures the whole batch sequence, forked
+ * over the movies' streams and joined back, as one HIP graph and replays it. */
+int zr_batch_use_graph(zr_batch* b, int enable);
+/* alignImageStack of every movie of the batch (zorro/zorro.py:1569-1716). */
+int zr_batch_align(zr_batch* b);
+int zr_batch_synchronize(zr_batch* b);
+int zr_batch_device_bytes(zr_batch* b, size_t* bytes);
+
 /* Device pointers of results for zero-copy consumers (valid until destroy). */
 int zr_device_results(zr_movie* mv, const float** image_sum, const float** filt_sum, const double** translations);
 

@@ -80,6 +80,15 @@ SIGNATURES = {
     "zr_get_peaksig_hist": (C.c_int, [_P, _F, _F, _F, C.c_int, _I]),
     "zr_tiled_frc": (C.c_int, [_P, _P, C.c_int, C.c_int, _D, C.c_int, C.c_int, C.c_double, _P, _I, _P]),
     "zr_tiled_frc_noalloc": (C.c_int, [_P, _P, C.c_int, C.c_int, _D, C.c_int, C.c_int, C.c_double, _P, _I, _P]),
+    "zr_batch_create": (C.c_int, [C.c_int, C.POINTER(ZrParams), C.c_int, _I, C.c_int, C.c_int, C.POINTER(_P)]),
+    "zr_batch_destroy": (None, [_P]),
+    "zr_batch_movie": (C.c_int, [_P, C.c_int, C.POINTER(_P)]),
+    "zr_batch_frames": (C.c_int, [_P, C.c_int, C.POINTER(_P)]),
+    "zr_batch_set_frames": (C.c_int, [_P, C.c_int, _P, C.c_int]),
+    "zr_batch_use_graph": (C.c_int, [_P, C.c_int]),
+    "zr_batch_align": (C.c_int, [_P]),
+    "zr_batch_synchronize": (C.c_int, [_P]),
+    "zr_batch_device_bytes": (C.c_int, [_P, C.POINTER(C.c_size_t)]),
     "zr_align": (C.c_int, [_P]),
     "zr_synchronize": (C.c_int, [_P]),
     "zr_movie_use_graph": (C.c_int, [_P, C.c_int]),

@@ -199,7 +199,19 @@ struct zr_movie {
   hipGraph_t graph[3] = {nullptr, nullptr, nullptr};          // spectra | pair transform | the rest
   hipGraphExec_t graph_exec[3] = {nullptr, nullptr, nullptr};
   StageTimer timer;
+  // batches (zr_batch_*): the batch's correlation plan has no solver state;
+  // a movie of a batch is a view that borrows the batch's frames and spectra
+  bool xcorr_only = false;
+  bool view = false;
+  hipStream_t batch_stream = nullptr;   // a view's results are complete once this stream has passed them
 };
+
+// wait for a movie's work (and, for a movie of a batch, the batch's graph)
+static hipError_t sync_movie(zr_movie* mv) {
+  hipError_t e = hipStreamSynchronize(mv->stream);
+  if (e == hipSuccess && mv->batch_stream) e = hipStreamSynchronize(mv->batch_stream);
+  return e;
+}
 
 extern "C" const char* zr_last_error(void) { return g_err.c_str(); }
 
@@ -225,6 +237,10 @@ static void free_movie(zr_movie* mv) {
   if (!mv) return;
   hipSetDevice(mv->device);
   drop_graph(mv);
+  if (mv->view) {             // borrowed from the batch
+    mv->S = nullptr;
+    mv->frames_owned = nullptr;
+  }
   void* ptrs[] = {mv->frames_owned, mv->k2btab, mv->k2ltw, mv->twWp2, mv->twWp, mv->twHp, mv->twHc, mv->twWc, mv->ktab, mv->gy, mv->gx,
                   mv->pairs, mv->work, mv->T, mv->S, mv->Sside, mv->Z, mv->part, mv->stats, mv->den, mv->stats_c,
                   mv->windows, mv->shifts, mv->corr, mv->peaksig, mv->origin, mv->bsigma, mv->twMrow, mv->twMcol,
@@ -238,7 +254,10 @@ static void free_movie(zr_movie* mv) {
   delete mv;
 }
 
-extern "C" void zr_movie_destroy(zr_movie* mv) { free_movie(mv); }
+extern "C" void zr_movie_destroy(zr_movie* mv) {
+  if (mv && mv->view) return;   // a movie of a batch goes with zr_batch_destroy
+  free_movie(mv);
+}
 
 // triMode 'refine' (zorro/zorro.py:934-956): frame I (I < N-1) against the
 // template (sum_k F_k - F_I) / N.  By linearity its raw window is
@@ -349,7 +368,7 @@ static PeakParamsHost peak_params(const zr_movie* mv) {
 }
 
 static int create_movie(int device, const zr_params* p, int n_frames, int h, int w, const uint8_t* tri_indices,
-                        zr_movie** out) {
+                        zr_movie** out, bool xcorr_only = false) {
   if (!p || !out) return fail(ZR_ERR_ARG, "NULL argument");
   *out = nullptr;
   if (n_frames < 2 || h < 16 || w < 16) return fail(ZR_ERR_ARG, "need >= 2 frames of >= 16x16 (got %d x %d x %d)", n_frames, h, w);
@@ -636,11 +655,13 @@ static int create_movie(int device, const zr_params* p, int n_frames, int h, int
       return rc;
     }
   }
-  if ((rc = zr_stage_create(&mv->st, mv->p, mv->N, mv->H, mv->W, mv->Hp, mv->Wp, mv->pairs_host, mv->spectral ? 1 : 0))) {
+  mv->xcorr_only = xcorr_only;
+  if (!xcorr_only &&
+      (rc = zr_stage_create(&mv->st, mv->p, mv->N, mv->H, mv->W, mv->Hp, mv->Wp, mv->pairs_host, mv->spectral ? 1 : 0))) {
     free_movie(mv);
     return fail(rc, "%s", zr_stage_error());
   }
-  mv->bytes = g_alloc_bytes + zr_stage_device_bytes(mv->st);
+  mv->bytes = g_alloc_bytes + (mv->st ? zr_stage_device_bytes(mv->st) : 0);
   *out = mv;
   return ZR_OK;
 }
@@ -669,6 +690,7 @@ extern "C" int zr_copy_sums_device(zr_movie* mv, float* image_sum, float* filt_s
 }
 
 extern "C" int zr_movie_set_stream(zr_movie* mv, void* s) {
+  if (mv && mv->view) return fail(ZR_ERR_STATE, "a movie of a batch takes its frames and stream from the batch");
   if (!mv) return fail(ZR_ERR_ARG, "NULL movie");
   drop_graph(mv);
   mv->stream = s ? (hipStream_t)s : mv->own_stream;
@@ -707,6 +729,7 @@ static int run_input_stage(zr_movie* mv, const void* raw, int dtype, const float
 
 extern "C" int zr_movie_set_frames_raw(zr_movie* mv, const void* frames, int dtype, int on_device, const float* gain,
                                        int gain_on_device, double gain_scalar) {
+  if (mv && mv->view) return fail(ZR_ERR_STATE, "a movie of a batch takes its frames from the batch");
   if (!mv || !frames) return fail(ZR_ERR_ARG, "NULL argument");
   if (dtype < ZR_DTYPE_F32 || dtype > ZR_DTYPE_I16) return fail(ZR_ERR_ARG, "dtype %d", dtype);
   if (dtype == ZR_DTYPE_U4 && (mv->Wr & 1))
@@ -745,11 +768,12 @@ extern "C" int zr_movie_set_frames_raw(zr_movie* mv, const void* frames, int dty
   }
   int rc = run_input_stage(mv, rawd, dtype, gd, gain_scalar);
   if (rc) return rc;
-  if (!on_device || (gain && !gain_on_device)) ZR_HIP(hipStreamSynchronize(mv->stream));   // host buffers may go away
+  if (!on_device || (gain && !gain_on_device)) ZR_HIP(sync_movie(mv));   // host buffers may go away
   return ZR_OK;
 }
 
 extern "C" int zr_movie_set_frames(zr_movie* mv, const float* frames, int on_device) {
+  if (mv && mv->view) return fail(ZR_ERR_STATE, "a movie of a batch takes its frames and stream from the batch");
   if (!mv || !frames) return fail(ZR_ERR_ARG, "NULL argument");
   if (mv->binned) return zr_movie_set_frames_raw(mv, frames, ZR_DTYPE_F32, on_device, nullptr, 0, 1.0);
   ZR_HIP(hipSetDevice(mv->device));
@@ -925,6 +949,7 @@ static int xcorr_peaks(zr_movie* mv) {
 
 extern "C" int zr_xcorr(zr_movie* mv) {
   if (!mv) return fail(ZR_ERR_ARG, "NULL movie");
+  if (mv->view) return fail(ZR_ERR_STATE, "a movie of a batch is correlated by zr_batch_align");
   if (!mv->frames) return fail(ZR_ERR_STATE, "frames not set");
   ZR_HIP(hipSetDevice(mv->device));
   int rc;
@@ -942,7 +967,7 @@ extern "C" int zr_solve(zr_movie* mv) {
     // triIndices = corrTriMat[:-1, 1:] != 0 (zorro/zorro.py:1282)
     std::vector<float> corr((size_t)mv->N * mv->N);
     ZR_HIP(hipMemcpyAsync(corr.data(), mv->corr, corr.size() * sizeof(float), hipMemcpyDeviceToHost, mv->stream));
-    ZR_HIP(hipStreamSynchronize(mv->stream));
+    ZR_HIP(sync_movie(mv));
     std::vector<int4> pr;
     for (int I = 0; I < mv->N - 1; ++I)
       for (int J = I + 1; J < mv->N; ++J)
@@ -964,7 +989,7 @@ extern "C" int zr_set_trimats(zr_movie* mv, const float* shifts, const float* co
   ZR_HIP(hipMemcpyAsync(mv->corr, corr, nn * sizeof(float), hipMemcpyHostToDevice, mv->stream));
   ZR_HIP(hipMemcpyAsync(mv->peaksig, peaksig, nn * sizeof(float), hipMemcpyHostToDevice, mv->stream));
   ZR_HIP(hipMemsetAsync(mv->origin, 0, nn * sizeof(float), mv->stream));
-  ZR_HIP(hipStreamSynchronize(mv->stream));   // the host arrays may go away on return
+  ZR_HIP(sync_movie(mv));   // the host arrays may go away on return
   mv->have_xcorr = true;
   return ZR_OK;
 }
@@ -1048,6 +1073,7 @@ static int align_sequence(zr_movie* mv) {
 
 extern "C" int zr_movie_use_graph(zr_movie* mv, int enable) {
   if (!mv) return fail(ZR_ERR_ARG, "NULL movie");
+  if (mv->view) return fail(ZR_ERR_STATE, "a movie of a batch is correlated by zr_batch_align");
   if (enable && mv->p.tri_mode == ZR_TRI_AUTO)
     return fail(ZR_ERR_UNSUPPORTED, "triMode 'auto' reads the pair list back to the host between stages: no graph");
   if (!enable) drop_graph(mv);
@@ -1057,6 +1083,7 @@ extern "C" int zr_movie_use_graph(zr_movie* mv, int enable) {
 
 extern "C" int zr_align(zr_movie* mv) {
   if (!mv) return fail(ZR_ERR_ARG, "NULL movie");
+  if (mv->view) return fail(ZR_ERR_STATE, "a movie of a batch is correlated by zr_batch_align");
   if (!zr_stage_equations_ok(mv->st) && mv->p.tri_mode != ZR_TRI_FIRST && mv->p.tri_mode != ZR_TRI_REFINE)
     return fail(ZR_ERR_UNSUPPORTED, "pair pattern gives equations the solver cannot represent");
   if (!mv->use_graph) return align_sequence(mv);
@@ -1099,7 +1126,7 @@ extern "C" int zr_get_hotpix_info(zr_movie* mv, double* info10) {
   ZR_HIP(hipSetDevice(mv->device));
   HotState h{};
   ZR_HIP(hipMemcpyAsync(&h, mv->hb.state, sizeof(h), hipMemcpyDeviceToHost, mv->stream));
-  ZR_HIP(hipStreamSynchronize(mv->stream));
+  ZR_HIP(sync_movie(mv));
   if (h.status != 0)
     return fail(ZR_ERR_UNSUPPORTED, "hotpixFilter: the histogram of the unaligned sum (mean %g) needs %g unit bins, "
                 "outside 2..%d", h.sum_mean, 2.0 * mv->p.hot_max_sigma * std::sqrt(std::max(h.sum_mean, 0.0)) + 1.0,
@@ -1114,7 +1141,7 @@ extern "C" int zr_get_hotpix_info(zr_movie* mv, double* info10) {
 extern "C" int zr_synchronize(zr_movie* mv) {
   if (!mv) return fail(ZR_ERR_ARG, "NULL movie");
   ZR_HIP(hipSetDevice(mv->device));
-  ZR_HIP(hipStreamSynchronize(mv->stream));
+  ZR_HIP(sync_movie(mv));
   return ZR_OK;
 }
 
@@ -1134,7 +1161,7 @@ extern "C" int zr_get_trimats(zr_movie* mv, float* shifts, float* corr, float* p
   if (!mv) return fail(ZR_ERR_ARG, "NULL movie");
   if (!mv->have_xcorr) return fail(ZR_ERR_STATE, "no correlation results");
   ZR_HIP(hipSetDevice(mv->device));
-  ZR_HIP(hipStreamSynchronize(mv->stream));
+  ZR_HIP(sync_movie(mv));
   if (mv->preshift) {
     int flag = 0;
     ZR_HIP(hipMemcpy(&flag, mv->pflag, sizeof(int), hipMemcpyDeviceToHost));
@@ -1154,7 +1181,7 @@ extern "C" int zr_get_frames(zr_movie* mv, float* frames) {
   if (!mv || !frames) return fail(ZR_ERR_ARG, "NULL argument");
   if (!mv->frames) return fail(ZR_ERR_STATE, "frames not set");
   ZR_HIP(hipSetDevice(mv->device));
-  ZR_HIP(hipStreamSynchronize(mv->stream));
+  ZR_HIP(sync_movie(mv));
   ZR_HIP(hipMemcpy(frames, mv->frames, (size_t)mv->N * mv->H * mv->W * sizeof(float), hipMemcpyDeviceToHost));
   return ZR_OK;
 }
@@ -1163,17 +1190,19 @@ extern "C" int zr_get_peak_sigmas(zr_movie* mv, float* sigma) {
   if (!mv || !sigma) return fail(ZR_ERR_ARG, "NULL argument");
   if (!mv->have_xcorr) return fail(ZR_ERR_STATE, "no correlation results");
   if (mv->p.b_mode != ZR_B_OPTI) return fail(ZR_ERR_STATE, "per-pair sigmas exist only for Bmode='opti'");
+  if (!mv->bsigma) return fail(ZR_ERR_UNSUPPORTED, "per-pair sigmas of a movie in a batch are not kept");
   ZR_HIP(hipSetDevice(mv->device));
-  ZR_HIP(hipStreamSynchronize(mv->stream));
+  ZR_HIP(sync_movie(mv));
   ZR_HIP(hipMemcpy(sigma, mv->bsigma, (size_t)mv->N * mv->N * sizeof(float), hipMemcpyDeviceToHost));
   return ZR_OK;
 }
 
 extern "C" int zr_get_windows(zr_movie* mv, float* windows) {
   if (!mv || !windows) return fail(ZR_ERR_ARG, "NULL argument");
+  if (mv->view) return fail(ZR_ERR_UNSUPPORTED, "the windows of a batch are not kept per movie");
   if (!mv->have_xcorr) return fail(ZR_ERR_STATE, "no correlation results");
   ZR_HIP(hipSetDevice(mv->device));
-  ZR_HIP(hipStreamSynchronize(mv->stream));
+  ZR_HIP(sync_movie(mv));
   const size_t nn = 4 * (size_t)mv->m * mv->m;
   ZR_HIP(hipMemcpy(windows, mv->windows + mv->win_out0 * nn, (size_t)mv->P * nn * sizeof(float), hipMemcpyDeviceToHost));
   return ZR_OK;
@@ -1183,7 +1212,7 @@ extern "C" int zr_get_solution(zr_movie* mv, double* translations, double* relat
                                double* logistic, double* error, double* error_stats, double* error_xy_frames) {
   if (!mv) return fail(ZR_ERR_ARG, "NULL movie");
   ZR_HIP(hipSetDevice(mv->device));
-  ZR_HIP(hipStreamSynchronize(mv->stream));
+  ZR_HIP(sync_movie(mv));
   int rc = zr_stage_get_solution(mv->st, translations, relative_est, weights, logistic, error, error_stats,
                                  error_xy_frames);
   return rc ? fail(rc, "%s", zr_stage_error()) : ZR_OK;
@@ -1192,7 +1221,7 @@ extern "C" int zr_get_solution(zr_movie* mv, double* translations, double* relat
 extern "C" int zr_get_peaksig_hist(zr_movie* mv, float* pdf, float* cdf, float* hsigma, int cap, int* nbins) {
   if (!mv) return fail(ZR_ERR_ARG, "NULL movie");
   ZR_HIP(hipSetDevice(mv->device));
-  ZR_HIP(hipStreamSynchronize(mv->stream));
+  ZR_HIP(sync_movie(mv));
   int rc = zr_stage_get_hist(mv->st, pdf, cdf, hsigma, cap, nbins);
   return rc ? fail(rc, "%s", zr_stage_error()) : ZR_OK;
 }
@@ -1200,7 +1229,7 @@ extern "C" int zr_get_peaksig_hist(zr_movie* mv, float* pdf, float* cdf, float* 
 extern "C" int zr_get_solver_info(zr_movie* mv, int* info6) {
   if (!mv || !info6) return fail(ZR_ERR_ARG, "NULL argument");
   ZR_HIP(hipSetDevice(mv->device));
-  ZR_HIP(hipStreamSynchronize(mv->stream));
+  ZR_HIP(sync_movie(mv));
   int rc = zr_stage_solver_info(mv->st, info6);
   return rc ? fail(rc, "%s", zr_stage_error()) : ZR_OK;
 }
@@ -1208,7 +1237,7 @@ extern "C" int zr_get_solver_info(zr_movie* mv, int* info6) {
 extern "C" int zr_get_sums(zr_movie* mv, float* image_sum, float* filt_sum, float* even_sum, float* odd_sum) {
   if (!mv) return fail(ZR_ERR_ARG, "NULL movie");
   ZR_HIP(hipSetDevice(mv->device));
-  ZR_HIP(hipStreamSynchronize(mv->stream));
+  ZR_HIP(sync_movie(mv));
   int rc = zr_stage_get_sums(mv->st, image_sum, filt_sum, even_sum, odd_sum);
   return rc ? fail(rc, "%s", zr_stage_error()) : ZR_OK;
 }
@@ -1216,7 +1245,7 @@ extern "C" int zr_get_sums(zr_movie* mv, float* image_sum, float* filt_sum, floa
 extern "C" int zr_get_frc(zr_movie* mv, double* frc, int* n) {
   if (!mv) return fail(ZR_ERR_ARG, "NULL movie");
   ZR_HIP(hipSetDevice(mv->device));
-  ZR_HIP(hipStreamSynchronize(mv->stream));
+  ZR_HIP(sync_movie(mv));
   int rc = zr_stage_get_frc(mv->st, frc, n);
   return rc ? fail(rc, "%s", zr_stage_error()) : ZR_OK;
 }
@@ -1250,5 +1279,234 @@ extern "C" int zr_movie_info(zr_movie* mv, int* dims) {
   if (!mv || !dims) return fail(ZR_ERR_ARG, "NULL argument");
   const int v[12] = {mv->N, mv->H, mv->W, mv->Hp, mv->Wp, mv->Hc, mv->Wc, mv->Wh, mv->m, mv->P, mv->unmasked ? 1 : 0, mv->batch};
   for (int i = 0; i < 12; ++i) dims[i] = v[i];
+  return ZR_OK;
+}
+
+// ==================================================================== batches
+// Several movies of one detector shape and one set of settings registered
+// together (the reference registers one movie per process and scales by
+// running processes side by side, automator/zorroSkulkManager.py:1089-1113):
+// the forward spectra of every frame of every movie, the pair transforms of
+// all their pairs (one ragged pair list) and the peak searches run as single
+// launches over the batch; each movie's solver, shift/filter and FRC stages
+// then run on a stream of their own, so that the one-workgroup solvers of the
+// movies overlap.  With graphs on, the whole sequence (forked over the movie
+// streams and joined back) is one captured graph.
+struct zr_batch {
+  int device = 0;
+  zr_movie* big = nullptr;               // correlation plan over all frames (block-diagonal pairs)
+  std::vector<zr_movie*> views;          // one per movie
+  std::vector<int> off, nfr;             // first frame and frame count per movie
+  std::vector<hipEvent_t> done;          // per movie: its stages finished
+  hipEvent_t xc = nullptr;               // the batch's correlation results are ready
+  bool use_graph = false;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t graph_exec = nullptr;
+};
+
+static void free_batch(zr_batch* b) {
+  if (!b) return;
+  hipSetDevice(b->device);
+  if (b->graph_exec) hipGraphExecDestroy(b->graph_exec);
+  if (b->graph) hipGraphDestroy(b->graph);
+  for (zr_movie* v : b->views) free_movie(v);
+  for (hipEvent_t e : b->done)
+    if (e) hipEventDestroy(e);
+  if (b->xc) hipEventDestroy(b->xc);
+  free_movie(b->big);
+  delete b;
+}
+
+extern "C" void zr_batch_destroy(zr_batch* b) { free_batch(b); }
+
+// a movie of the batch: frames and spectra are slices of the batch's, the
+// trimats, solver/shift/filter/FRC state and the stream are its own
+static int create_view(zr_batch* b, int i, const zr_params& p, zr_movie** out) {
+  const zr_movie* big = b->big;
+  const int n = b->nfr[i], off = b->off[i];
+  zr_movie* v = new zr_movie();
+  v->view = true;
+  v->device = big->device;
+  v->p = big->p;
+  v->p.tri_mode = p.tri_mode;
+  v->N = n;
+  v->Hr = big->Hr; v->Wr = big->Wr; v->H = big->H; v->W = big->W; v->Hp = big->Hp; v->Wp = big->Wp;
+  v->Hc = big->Hc; v->Wc = big->Wc; v->Wh = big->Wh; v->m = big->m; v->Lper = big->Lper; v->disc = big->disc;
+  v->crop = big->crop; v->unmasked = big->unmasked; v->spectral = big->spectral;
+  v->Wk = big->Wk; v->Hs = big->Hs; v->hoff = big->hoff; v->side_in_col = big->side_in_col;
+  v->batch = big->batch; v->rb = big->rb; v->k2_cpw = big->k2_cpw;
+  v->frames_owned = big->frames_owned + (size_t)off * big->H * big->W;
+  v->frames = v->frames_owned;
+  v->S = big->S + (size_t)off * big->Wk * big->Hs;     // frame f's spectrum is slot f (unmasked and masked)
+  for (const int4& q : big->pairs_host)
+    if (q.x >= off && q.y < off + n) v->pairs_host.push_back(make_int4(q.x - off, q.y - off, (int)v->pairs_host.size(), 0));
+  v->P = v->Ptot = (int)v->pairs_host.size();
+  v->nspec = big->unmasked ? n : 2 * n + 1;
+  g_alloc_bytes = 0;
+  const size_t nn = (size_t)n * n;
+  int rc;
+  hipError_t e = hipStreamCreateWithFlags(&v->own_stream, hipStreamNonBlocking);
+  if (e != hipSuccess) { free_movie(v); return fail(ZR_ERR_HIP, "hipStreamCreate: %s", hipGetErrorString(e)); }
+  v->stream = v->own_stream;
+  v->batch_stream = big->stream;
+  if ((rc = dev_alloc(&v->shifts, nn * 2)) || (rc = dev_alloc(&v->corr, nn)) || (rc = dev_alloc(&v->peaksig, nn)) ||
+      (rc = dev_alloc(&v->origin, nn))) {
+    free_movie(v);
+    return rc;
+  }
+  if ((rc = zr_stage_create(&v->st, v->p, n, v->H, v->W, v->Hp, v->Wp, v->pairs_host, v->spectral ? 1 : 0))) {
+    free_movie(v);
+    return fail(rc, "%s", zr_stage_error());
+  }
+  v->bytes = g_alloc_bytes + zr_stage_device_bytes(v->st);
+  *out = v;
+  return ZR_OK;
+}
+
+extern "C" int zr_batch_create(int device, const zr_params* p, int n_movies, const int* n_frames, int h, int w,
+                               zr_batch** out) {
+  if (!p || !n_frames || !out || n_movies < 1) return fail(ZR_ERR_ARG, "NULL argument or no movies");
+  *out = nullptr;
+  if (p->tri_mode != ZR_TRI_TRI && p->tri_mode != ZR_TRI_DIAG)
+    return fail(ZR_ERR_UNSUPPORTED, "batches run triMode 'tri' or 'diag' (got %d)", p->tri_mode);
+  if (p->pre_shift || p->do_hot || p->shape_binned[0] > 0 || p->shape_binned[1] > 0)
+    return fail(ZR_ERR_UNSUPPORTED, "batches take float32 frames without preShift, 'hot' or binning");
+  zr_batch* b = new zr_batch();
+  b->device = device;
+  int total = 0;
+  for (int i = 0; i < n_movies; ++i) {
+    if (n_frames[i] < 2 || n_frames[i] > 64) { free_batch(b); return fail(ZR_ERR_ARG, "movie %d: %d frames outside 2..64", i, n_frames[i]); }
+    b->off.push_back(total);
+    b->nfr.push_back(n_frames[i]);
+    total += n_frames[i];
+  }
+  // the pairs of every movie, in frame numbers of the whole batch
+  std::vector<uint8_t> mask((size_t)total * total, 0);
+  for (int i = 0; i < n_movies; ++i)
+    for (int I = 0; I < b->nfr[i]; ++I)
+      for (int J = I + 1; J < b->nfr[i]; ++J) {
+        const int d = J - I;
+        const int wdt = p->diag_width < 0 ? b->nfr[i] : p->diag_width;
+        const bool take = p->tri_mode == ZR_TRI_TRI || (d <= wdt + p->diag_start && d > p->diag_start);
+        if (take) mask[(size_t)(b->off[i] + I) * total + b->off[i] + J] = 1;
+      }
+  zr_params pb = *p;
+  pb.tri_mode = ZR_TRI_TRI;
+  int rc = create_movie(device, &pb, total, h, w, mask.data(), &b->big, true);
+  if (rc) { delete b; return rc; }
+  zr_movie* big = b->big;
+  g_alloc_bytes = 0;
+  if ((rc = dev_alloc(&big->frames_owned, (size_t)total * big->H * big->W))) { free_batch(b); return rc; }
+  big->frames = big->frames_owned;
+  big->bytes += g_alloc_bytes;
+  for (int i = 0; i < n_movies; ++i) {
+    zr_movie* v = nullptr;
+    if ((rc = create_view(b, i, *p, &v))) { free_batch(b); return rc; }
+    b->views.push_back(v);
+    hipEvent_t e = nullptr;
+    hipError_t he = hipEventCreateWithFlags(&e, hipEventDisableTiming);
+    if (he != hipSuccess) { free_batch(b); return fail(ZR_ERR_HIP, "hipEventCreate: %s", hipGetErrorString(he)); }
+    b->done.push_back(e);
+  }
+  hipError_t he = hipEventCreateWithFlags(&b->xc, hipEventDisableTiming);
+  if (he != hipSuccess) { free_batch(b); return fail(ZR_ERR_HIP, "hipEventCreate: %s", hipGetErrorString(he)); }
+  *out = b;
+  return ZR_OK;
+}
+
+extern "C" int zr_batch_movie(zr_batch* b, int i, zr_movie** view) {
+  if (!b || !view || i < 0 || i >= (int)b->views.size()) return fail(ZR_ERR_ARG, "batch movie index");
+  *view = b->views[i];
+  return ZR_OK;
+}
+
+extern "C" int zr_batch_frames(zr_batch* b, int i, float** frames) {
+  if (!b || !frames || i < 0 || i >= (int)b->views.size()) return fail(ZR_ERR_ARG, "batch movie index");
+  *frames = b->views[i]->frames_owned;
+  return ZR_OK;
+}
+
+extern "C" int zr_batch_set_frames(zr_batch* b, int i, const float* frames, int on_device) {
+  if (!b || !frames || i < 0 || i >= (int)b->views.size()) return fail(ZR_ERR_ARG, "batch movie index");
+  zr_movie* v = b->views[i];
+  ZR_HIP(hipSetDevice(b->device));
+  const size_t bytes = (size_t)v->N * v->H * v->W * sizeof(float);
+  ZR_HIP(hipMemcpyAsync(v->frames_owned, frames, bytes, on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice,
+                        b->big->stream));
+  if (!on_device) ZR_HIP(hipStreamSynchronize(b->big->stream));   // the host array may go away on return
+  return ZR_OK;
+}
+
+extern "C" int zr_batch_device_bytes(zr_batch* b, size_t* bytes) {
+  if (!b || !bytes) return fail(ZR_ERR_ARG, "NULL argument");
+  size_t t = b->big->bytes;
+  for (zr_movie* v : b->views) t += v->bytes;
+  *bytes = t;
+  return ZR_OK;
+}
+
+static int batch_sequence(zr_batch* b) {
+  zr_movie* big = b->big;
+  int rc;
+  for (int part = 0; part < 3; ++part)
+    if ((rc = xcorr_part(big, part))) return rc;
+  ZR_HIP(hipEventRecord(b->xc, big->stream));
+  for (size_t i = 0; i < b->views.size(); ++i) {
+    zr_movie* v = b->views[i];
+    ZR_HIP(hipStreamWaitEvent(v->stream, b->xc, 0));
+    ZR_HIP(zr_launch_gather_trimats(big->shifts, big->corr, big->peaksig, big->origin, big->N, b->off[i], v->N,
+                                    v->shifts, v->corr, v->peaksig, v->origin, v->stream));
+    v->have_xcorr = true;
+    if ((rc = zr_solve(v))) return rc;
+    if ((rc = zr_post_filters(v))) return rc;
+    if (v->p.do_frc && (rc = zr_frc(v))) return rc;
+    ZR_HIP(hipEventRecord(b->done[i], v->stream));
+  }
+  for (hipEvent_t e : b->done) ZR_HIP(hipStreamWaitEvent(big->stream, e, 0));
+  return ZR_OK;
+}
+
+extern "C" int zr_batch_use_graph(zr_batch* b, int enable) {
+  if (!b) return fail(ZR_ERR_ARG, "NULL batch");
+  if (!enable && b->graph_exec) {
+    hipGraphExecDestroy(b->graph_exec);
+    hipGraphDestroy(b->graph);
+    b->graph_exec = nullptr;
+    b->graph = nullptr;
+  }
+  b->use_graph = enable != 0;
+  return ZR_OK;
+}
+
+extern "C" int zr_batch_align(zr_batch* b) {
+  if (!b) return fail(ZR_ERR_ARG, "NULL batch");
+  ZR_HIP(hipSetDevice(b->device));
+  if (!b->use_graph) return batch_sequence(b);
+  hipStream_t st = b->big->stream;
+  if (!b->graph_exec) {
+    hipError_t e = hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed);
+    if (e != hipSuccess) return fail(ZR_ERR_HIP, "hipStreamBeginCapture: %s", hipGetErrorString(e));
+    int rc = batch_sequence(b);
+    hipGraph_t g = nullptr;
+    e = hipStreamEndCapture(st, &g);
+    if (rc == ZR_OK && e != hipSuccess) rc = fail(ZR_ERR_HIP, "hipStreamEndCapture: %s", hipGetErrorString(e));
+    if (rc != ZR_OK) { if (g) hipGraphDestroy(g); return rc; }
+    b->graph = g;
+    e = hipGraphInstantiate(&b->graph_exec, g, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+      hipGraphDestroy(g);
+      b->graph = nullptr;
+      return fail(ZR_ERR_HIP, "hipGraphInstantiate: %s", hipGetErrorString(e));
+    }
+  }
+  ZR_HIP(hipGraphLaunch(b->graph_exec, st));
+  return ZR_OK;
+}
+
+extern "C" int zr_batch_synchronize(zr_batch* b) {
+  if (!b) return fail(ZR_ERR_ARG, "NULL batch");
+  ZR_HIP(hipSetDevice(b->device));
+  ZR_HIP(hipStreamSynchronize(b->big->stream));
+  for (zr_movie* v : b->views) ZR_HIP(hipStreamSynchronize(v->stream));
   return ZR_OK;
 }

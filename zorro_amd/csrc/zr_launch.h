@@ -118,3 +118,8 @@ hipError_t zr_launch_gain_convert(const void* raw, int dtype, size_t npix_frame,
                                   float gscale, int use_scale, float* frames, hipStream_t st);
 // zero fill by a kernel (graph-capturable); bytes multiple of 4
 hipError_t zr_launch_fill_zero(void* p, size_t bytes, hipStream_t st);
+
+// batches: copy the block [off, off + n)^2 of a batch's trimats into a movie's [n][n] ones
+hipError_t zr_launch_gather_trimats(const float* shifts, const float* corr, const float* peaksig, const float* origin,
+                                    int Nt, int off, int n, float* d_shifts, float* d_corr, float* d_peaksig,
+                                    float* d_origin, hipStream_t st);

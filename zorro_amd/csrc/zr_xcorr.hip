@@ -1569,3 +1569,31 @@ hipError_t zr_launch_k2_twiddles(int Wh, int Wc, int m, float2* btab, hipStream_
   hipLaunchKernelGGL(k2_rows_twiddles, dim3((n + 255) / 256), dim3(256), 0, st, Wh, Wc, m, btab);
   return hipGetLastError();
 }
+
+// ---------------------------------------------------------------- batches
+// A movie of a batch reads its block [off, off + n)^2 of the batch's
+// triangular matrices (shifts [Nt][Nt][2], corr, peaksig, origin [Nt][Nt])
+// into its own [n][n] ones.
+__global__ __launch_bounds__(256) void k_gather_trimats(const float* __restrict__ shifts, const float* __restrict__ corr,
+                                                        const float* __restrict__ peaksig,
+                                                        const float* __restrict__ origin, int Nt, int off, int n,
+                                                        float* __restrict__ d_shifts, float* __restrict__ d_corr,
+                                                        float* __restrict__ d_peaksig, float* __restrict__ d_origin) {
+  const int e = blockIdx.x * 256 + threadIdx.x;
+  if (e >= n * n) return;
+  const int I = e / n, J = e % n;
+  const size_t src = (size_t)(off + I) * Nt + (off + J);
+  d_shifts[2 * e] = shifts[2 * src];
+  d_shifts[2 * e + 1] = shifts[2 * src + 1];
+  d_corr[e] = corr[src];
+  d_peaksig[e] = peaksig[src];
+  d_origin[e] = origin[src];
+}
+
+hipError_t zr_launch_gather_trimats(const float* shifts, const float* corr, const float* peaksig, const float* origin,
+                                    int Nt, int off, int n, float* d_shifts, float* d_corr, float* d_peaksig,
+                                    float* d_origin, hipStream_t st) {
+  hipLaunchKernelGGL(k_gather_trimats, dim3((n * n + 255) / 256), dim3(256), 0, st, shifts, corr, peaksig, origin, Nt,
+                     off, n, d_shifts, d_corr, d_peaksig, d_origin);
+  return hipGetLastError();
+}

@@ -140,14 +140,18 @@ class MoviePlan:
     with ``shapeBinned`` the frames are binned on input and every result has the
     binned size ``hb`` x ``wb``)."""
 
-    def __init__(self, params: L.ZrParams, n: int, h: int, w: int, device: int = 0, triIndices=None):
+    def __init__(self, params: L.ZrParams, n: int, h: int, w: int, device: int = 0, triIndices=None, _handle=None):
         """``triIndices``: optional boolean [n, n] matrix of the pairs (I < J) to
-        correlate, as ``xcorrnm2_tri(triIndices=...)`` takes (triMode 'tri')."""
+        correlate, as ``xcorrnm2_tri(triIndices=...)`` takes (triMode 'tri').
+        ``_handle``: a movie of a batch (owned by the batch, see batch.py)."""
         self.lib = L.load()
         self.params = params
         self.n, self.h, self.w = int(n), int(h), int(w)
         self._mv = C.c_void_p()
-        if triIndices is None:
+        self._owned = _handle is None
+        if _handle is not None:
+            self._mv = _handle
+        elif triIndices is None:
             L.check(self.lib.zr_movie_create(int(device), C.byref(params), self.n, self.h, self.w,
                                              C.byref(self._mv)))
         else:
@@ -167,7 +171,8 @@ class MoviePlan:
 
     def close(self):
         if self._mv:
-            self.lib.zr_movie_destroy(self._mv)
+            if self._owned:
+                self.lib.zr_movie_destroy(self._mv)
             self._mv = C.c_void_p()
 
     def __del__(self):
