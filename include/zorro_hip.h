@@ -347,7 +347,8 @@ int zr_batch_movie(zr_batch* b, int i, zr_movie** view);
 /* Device address of movie i's frames f32 [n_i][h][w] inside the batch (a
  * producer may write them there directly). */
 int zr_batch_frames(zr_batch* b, int i, float** frames);
-/* Copy movie i's frames in (host: returns when they may be reused). */
+/* Copy movie i's frames in (host or device memory; returns when the copy is
+ * done).  A device source written on another stream must be complete first. */
 int zr_batch_set_frames(zr_batch* b, int i, const float* frames, int on_device);
 /* enable != 0: the next zr_batch_align captures the whole batch sequence, forked
  * over the movies' streams and joined back, as one HIP graph and replays it. */

@@ -80,6 +80,34 @@ def test_batch_movies_equal_their_own_plans(name):
         batch.close()
 
 
+def test_batch_takes_device_frames_written_on_another_stream():
+    """Frames produced by torch kernels that may still be running on torch's
+    stream when they are handed to the batch (the copy must wait for them)."""
+    import torch
+    c = CASES["512sq_unmasked_20_33_47"]
+    ns = (20, 33)
+    movies = [make_movie(n, c["h"], c["w"], seed=80 + k, dose=1.5)[0] for k, n in enumerate(ns)]
+    kw = _kw(c)
+    batch = BatchPlan(make_params(**kw), ns, c["h"], c["w"])
+    ref = BatchPlan(make_params(**kw), ns, c["h"], c["w"])
+    try:
+        for k, f in enumerate(movies):
+            ref.set_frames(k, f)
+            t = torch.from_numpy(f).to("cuda:0")
+            for _ in range(20):          # queue work on torch's stream ahead of the hand-over
+                t = t * 1.0
+            batch.set_frames(k, t)
+            del t
+        for b in (batch, ref):
+            b.align()
+            b.synchronize()
+        for k in range(len(ns)):
+            _same(_results(batch.movies[k]), _results(ref.movies[k]), f"device frames movie {k}")
+    finally:
+        batch.close()
+        ref.close()
+
+
 def test_batch_refuses_unbuilt_settings():
     from zorro_amd import _lib as L
     with pytest.raises(L.ZrError):

@@ -37,7 +37,6 @@ class BatchPlan:
             v = C.c_void_p()
             L.check(self.lib.zr_batch_movie(self._b, i, C.byref(v)))
             self.movies.append(MoviePlan(params, n, self.h, self.w, device=device, _handle=v))
-        self._refs = [None] * len(self.n_frames)
 
     def __len__(self):
         return len(self.n_frames)
@@ -67,8 +66,9 @@ class BatchPlan:
         if hasattr(frames, "data_ptr") and getattr(frames, "is_cuda", False):
             if str(frames.dtype) != "torch.float32" or not frames.is_contiguous() or tuple(frames.shape) != want:
                 raise ValueError(f"movie {i}: frames must be a contiguous float32 CUDA tensor of shape {want}")
+            import torch
+            torch.cuda.current_stream(frames.device).synchronize()    # the producer's kernels have written it
             L.check(self.lib.zr_batch_set_frames(self._b, int(i), C.c_void_p(frames.data_ptr()), 1))
-            self._refs[i] = frames
         else:
             a = np.ascontiguousarray(frames, dtype=np.float32)
             if a.shape != want:

@@ -1433,7 +1433,7 @@ extern "C" int zr_batch_set_frames(zr_batch* b, int i, const float* frames, int 
   const size_t bytes = (size_t)v->N * v->H * v->W * sizeof(float);
   ZR_HIP(hipMemcpyAsync(v->frames_owned, frames, bytes, on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice,
                         b->big->stream));
-  if (!on_device) ZR_HIP(hipStreamSynchronize(b->big->stream));   // the host array may go away on return
+  ZR_HIP(hipStreamSynchronize(b->big->stream));   // the source may be reused or freed on return
   return ZR_OK;
 }
 

@@ -27,6 +27,7 @@ constexpr int LOGI_G0 = 32, LOGI_G1 = 32, LOGI_G2 = 32;
 constexpr int LOGI_ROUNDS = 30, LOGI_MAX_ITERS = 400;
 constexpr double LOGI_K0 = 0.1, LOGI_K1 = 20.0, LOGI_N0 = 0.05, LOGI_N1 = 5.0;
 constexpr int MAXN = 64;          // frames handled by the one-wave solver
+constexpr int DOSE_CHUNK = 8;     // frames per row/column pass of the direct dose path
 constexpr int IRLS_ITERS = 2;
 constexpr int MAX_PIVOTS = 4000;
 
@@ -1309,76 +1310,62 @@ __global__ __launch_bounds__(256) void k_frc_rings(const double* __restrict__ fr
 }
 
 // ------------------------------------------------------------------ KD
-// Row pass: aligned (and symmetric-padded) rows of every frame -> real FFT (Wp)
-// -> T[J][kx][Y]; the image, even and odd sums of the unpadded rows are formed
-// on the way (one block = 4 rows, 1024 threads = 4 groups of 256).
+// Direct dose path, row pass over a chunk of frames: the aligned frame (roll +
+// 3x3 Lanczos kernel), symmetric-padded to Hp x WP, real FFT of each padded
+// row, written transposed in column pairs T[fb][k/2][Y][k%2] so that a thread
+// stores RB rows x 2 columns (16 RB contiguous bytes).  One block = RB padded
+// rows of one frame, one group of TPR threads per row.
+constexpr int dose_rows_rb(int wp) { return wp >= 3072 ? 4 : 8; }
+constexpr int dose_rows_tpr(int wp) {
+  return (wp >= 2048 && (wp / 2) % 128 == 0) ? 128 : ((wp / 2) % 192 == 0 ? 192 : 64);
+}
 template <int WP>
-__global__ __launch_bounds__(1024) void k_dose_rows(const float* __restrict__ frames, const FrameShift* __restrict__ fs,
-                                                   int N, int H, int W, int Hp, const float2* __restrict__ twL,
-                                                   const float2* __restrict__ tw2L, float2* __restrict__ T,
-                                                   float* __restrict__ img_sum, float* __restrict__ even_sum,
-                                                   float* __restrict__ odd_sum, float* __restrict__ aligned,
-                                                   double* __restrict__ rowsum) {
-  constexpr int L = WP / 2, RB = 4, NTG = 256, PX = (WP + NTG - 1) / NTG;
+__global__ __launch_bounds__(dose_rows_rb(WP) * dose_rows_tpr(WP)) void k_dose_rows(
+    const float* __restrict__ frames, const FrameShift* __restrict__ fs, int j0, int H, int W, int Hp,
+    const float2* __restrict__ twL, const float2* __restrict__ tw2L, float2* __restrict__ T) {
+  constexpr int L = WP / 2, RB = dose_rows_rb(WP), TPR = dose_rows_tpr(WP), NT = RB * TPR, PER = L / TPR;
+  static_assert(L % TPR == 0, "row length must split evenly over the row group");
   extern __shared__ float2 kd_lds[];
   float2 (*buf)[zr_fft_lds(L)] = reinterpret_cast<float2 (*)[zr_fft_lds(L)]>(kd_lds);
-  __shared__ double red[16];
-  const int grp = threadIdx.x >> 8, gtid = threadIdx.x & 255;
+  const int tid = threadIdx.x;
+  const int g = tid / TPR, gt = tid - g * TPR;
+  const int nrb = Hp / RB;
   const int blk = xcd_remap(blockIdx.x, gridDim.x);
-  const int Y = blk * RB + grp;                 // padded row
-  const int ysrc = sym_idx(Y, H);               // aligned row it copies
-  const bool own = (Y < H);
-  const int Wh = L + 1;
-  float st[PX], se[PX], so[PX];
-  for (int i = 0; i < PX; ++i) { st[i] = 0.f; se[i] = 0.f; so[i] = 0.f; }
-  for (int J = 0; J < N; ++J) {
-    const FrameShift s = reduce_shift(fs[J], H, W);
-    const RowSrc rs = row_sources(frames + (size_t)J * H * W, s, ysrc, H, W);
-    float* br = reinterpret_cast<float*>(buf[grp]);
-    for (int i = 0; i < PX; ++i) {
-      int X = gtid + i * NTG;
-      if (X < WP) {
-        const float v = aligned_px_row(rs, s, X < W ? X : sym_idx(X, W), W);
-        br[X] = v;
-        if (own && X < W) {
-          st[i] += v;
-          if (J & 1) so[i] += v; else se[i] += v;
-          if (aligned) aligned[((size_t)J * H + Y) * W + X] = v;
-        }
-      }
-    }
-    __syncthreads();
-    wg_fft<L, -1, NTG>(buf[grp], twL, gtid);
-    for (int k = gtid; k < Wh; k += NTG) {
-      float2 w = tw2L[k];
-      float2 zk = buf[grp][k % L];
-      float2 zn = c_conj(buf[grp][(L - k) % L]);
-      float2 e = c_scale(c_add(zk, zn), 0.5f);
-      float2 o = c_scale(c_sub(zk, zn), 0.5f);
-      float2 Ok = make_float2(o.y, -o.x);
-      T[((size_t)J * Wh + k) * Hp + Y] = c_add(e, c_mul(w, Ok));
-    }
-    __syncthreads();
+  const int fb = blk / nrb, y0 = (blk % nrb) * RB;
+  const int J = j0 + fb;
+  const int Y = y0 + g;                        // padded row
+  const FrameShift sh = reduce_shift(fs[J], H, W);
+  const RowSrc rs = row_sources(frames + (size_t)J * H * W, sh, sym_idx(Y, H), H, W);
+#pragma unroll
+  for (int i = 0; i < PER; ++i) {
+    const int n = gt + i * TPR, x0 = 2 * n, x1 = 2 * n + 1;
+    buf[g][n] = make_float2(aligned_px_row(rs, sh, x0 < W ? x0 : sym_idx(x0, W), W),
+                            aligned_px_row(rs, sh, x1 < W ? x1 : sym_idx(x1, W), W));
   }
-  double part = 0.0;
-  if (own) {
-    for (int i = 0; i < PX; ++i) {
-      int X = gtid + i * NTG;
-      if (X < W) {
-        img_sum[(size_t)Y * W + X] = st[i];
-        even_sum[(size_t)Y * W + X] = se[i];
-        odd_sum[(size_t)Y * W + X] = so[i];
-        part += st[i];
-      }
-    }
-  }
-  part = wave_sum_d(part);
-  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = part;
   __syncthreads();
-  if (threadIdx.x == 0) {
-    double t = 0.0;
-    for (int i = 0; i < 16; ++i) t += red[i];
-    rowsum[blk] = t;
+  wg_fft<L, -1, TPR>(buf[g], twL, gt);
+  const int Wh = L + 1, KP = (Wh + 1) / 2;
+  for (int kp = tid; kp < KP; kp += NT) {
+    float2 out[2][RB];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int k = 2 * kp + h;
+      const bool live = k < Wh;
+      const int kk = live ? k : 0;
+      const float2 w = tw2L[kk];
+#pragma unroll
+      for (int r = 0; r < RB; ++r) {
+        const float2 zk = buf[r][kk % L];
+        const float2 zn = c_conj(buf[r][(L - kk) % L]);
+        const float2 e = c_scale(c_add(zk, zn), 0.5f);
+        const float2 o = c_scale(c_sub(zk, zn), 0.5f);
+        const float2 Ok = make_float2(o.y, -o.x);
+        out[h][r] = live ? c_add(e, c_mul(w, Ok)) : make_float2(0.f, 0.f);
+      }
+    }
+    float4* dst = reinterpret_cast<float4*>(T + (((size_t)fb * KP + kp) * Hp + y0) * 2);
+#pragma unroll
+    for (int r = 0; r < RB; ++r) dst[r] = make_float4(out[0][r].x, out[0][r].y, out[1][r].x, out[1][r].y);
   }
 }
 
@@ -1446,24 +1433,27 @@ __global__ __launch_bounds__(256) void k_dose_params(DoseArgs a) {
 }
 
 // Column pass: per kx, accumulate FFT_y(T_J[kx]) * exp(-D_J/(2 Nc(q))) * exp(-(q/qc_J)^order)
-// over all frames, then the inverse transform along y.
+// over the chunk's frames [j0, j1) into the running sum G[kx][0..HP) (frame
+// order, so a split into chunks gives the bits of one pass over all frames);
+// the last chunk applies the dosenorm scale and the inverse transform along y.
+// T holds the chunk in the row pass's column-pair layout.
 // threads of one column block: HP must split evenly over them
 constexpr int dose_cols_nt(int hp) { return hp % 256 == 0 ? 256 : 384; }
 template <int HP>
-__global__ __launch_bounds__(dose_cols_nt(HP)) void k_dose_cols(const float2* __restrict__ T, DoseArgs a,
-                                                               const float2* __restrict__ twH,
+__global__ __launch_bounds__(dose_cols_nt(HP)) void k_dose_cols(const float2* __restrict__ T, DoseArgs a, int j0,
+                                                               int j1, const float2* __restrict__ twH,
                                                                float2* __restrict__ G) {
   constexpr int NT = dose_cols_nt(HP), PER = HP / NT;
   static_assert(HP % NT == 0, "column length must split evenly over the block");
   __shared__ float2 buf[zr_fft_lds(HP)];
   const int tid = threadIdx.x;
   const int kx = xcd_remap(blockIdx.x, gridDim.x);
-  const int Wh = a.Wp / 2 + 1;
-  float invc[PER], q[PER], gsum[PER];
+  const int Wh = a.Wp / 2 + 1, KP = (Wh + 1) / 2;
+  float invc[PER], q[PER];
   float2 acc[PER];
+  float2* gcol = G + (size_t)kx * HP;
   for (int i = 0; i < PER; ++i) {
     const int ky = tid + i * NT;
-    gsum[i] = 0.f;
     const double fy = ky < HP / 2 ? ky : ky - HP;
     const double fx = kx;     // kx = Wp/2 is the Nyquist column (|f| identical)
     auto crit = [&](double yy, double xx) {
@@ -1478,38 +1468,52 @@ __global__ __launch_bounds__(dose_cols_nt(HP)) void k_dose_cols(const float2* __
     }
     invc[i] = (float)(1.0 / c);   // q = 0 -> c = inf -> 0
     q[i] = (float)qq;
-    acc[i] = make_float2(0.f, 0.f);
+    acc[i] = j0 > 0 ? gcol[ky] : make_float2(0.f, 0.f);
   }
-  for (int J = 0; J < a.N; ++J) {
-    const float2* src = T + ((size_t)J * Wh + kx) * HP;
-    for (int i = 0; i < PER; ++i) buf[tid + i * NT] = src[tid + i * NT];
+  const float ord = (float)a.order;
+  for (int J = j0; J < j1; ++J) {
+    const float2* src = T + (((size_t)(J - j0) * KP + kx / 2) * HP) * 2 + (kx & 1);
+    for (int i = 0; i < PER; ++i) buf[tid + i * NT] = src[2 * (tid + i * NT)];
     __syncthreads();
     wg_fft<HP, -1, NT>(buf, twH, tid);
     const float mhd = (float)(-0.5 * a.accum[J + 1]);
     const float tq = (float)a.thresq[J];
-    const float ord = (float)a.order;
     for (int i = 0; i < PER; ++i) {
       float filt = expf(mhd * invc[i]);
       float thr = expf(-cutoff_pow(q[i] / tq, ord, a.order_sq));
       float2 v = buf[tid + i * NT];
       float g = thr * filt;
-      gsum[i] += g;
       acc[i].x += v.x * g;
       acc[i].y += v.y * g;
     }
     __syncthreads();
   }
-  if (a.normalize)   // filterMag = 1 / (sum of the filters + 1), float32 (zorro/zorro.py:2480-2486)
-    for (int i = 0; i < PER; ++i) acc[i] = c_scale(acc[i], 1.f / (gsum[i] + 1.f));
+  if (j1 < a.N) {   // more chunks follow: keep the running sum
+    for (int i = 0; i < PER; ++i) gcol[tid + i * NT] = acc[i];
+    return;
+  }
+  if (a.normalize) {
+    // filterMag = 1 / (sum of the filters + 1), float32 (zorro/zorro.py:2480-2486); the
+    // filters depend only on the frequency, so their sum is formed here in frame order
+    for (int i = 0; i < PER; ++i) {
+      float gsum = 0.f;
+      for (int J = 0; J < a.N; ++J) {
+        const float filt = expf((float)(-0.5 * a.accum[J + 1]) * invc[i]);
+        const float thr = expf(-cutoff_pow(q[i] / (float)a.thresq[J], ord, a.order_sq));
+        gsum += thr * filt;
+      }
+      acc[i] = c_scale(acc[i], 1.f / (gsum + 1.f));
+    }
+  }
   for (int i = 0; i < PER; ++i) buf[tid + i * NT] = acc[i];
   __syncthreads();
   wg_fft<HP, 1, NT>(buf, twH, tid);
-  for (int y = tid; y < a.H; y += NT) G[(size_t)kx * a.H + y] = buf[y];
+  for (int y = tid; y < a.H; y += NT) gcol[y] = buf[y];
 }
 
 // Final rows: c2r of the accumulated half spectrum (raw), modulus, crop to W.
 template <int WP>
-__global__ __launch_bounds__(256) void k_dose_out(const float2* __restrict__ G, int H, int W,
+__global__ __launch_bounds__(256) void k_dose_out(const float2* __restrict__ G, int Gs, int H, int W,
                                                  const float2* __restrict__ twL, const float2* __restrict__ tw2L,
                                                  float* __restrict__ filt_sum) {
   constexpr int L = WP / 2, NT = 256;
@@ -1517,8 +1521,8 @@ __global__ __launch_bounds__(256) void k_dose_out(const float2* __restrict__ G, 
   const int y = xcd_remap(blockIdx.x, gridDim.x);
   const int tid = threadIdx.x;
   for (int k = tid; k < L; k += NT) {
-    float2 xk = G[(size_t)k * H + y];
-    float2 xn = c_conj(G[(size_t)(L - k) * H + y]);
+    float2 xk = G[(size_t)k * Gs + y];
+    float2 xn = c_conj(G[(size_t)(L - k) * Gs + y]);
     float2 e = c_scale(c_add(xk, xn), 0.5f);
     float2 d = c_scale(c_sub(xk, xn), 0.5f);
     float2 w = tw2L[k];
@@ -1920,7 +1924,7 @@ struct zr_stage_state {
   float* hann = nullptr;
   // dose
   float2 *twHp = nullptr, *twWp = nullptr, *twWp2 = nullptr, *Tdose = nullptr, *G = nullptr;
-  double *accum = nullptr, *thresq = nullptr, *drow = nullptr;
+  double *accum = nullptr, *thresq = nullptr;
   float* gtmp = nullptr;       // 'gausslp': the axis-0 pass of the Gaussian
   bool spectral = false;       // dose filter from the forward spectra (unpadded frames)
   float2 *R0 = nullptr, *RH = nullptr, *C0 = nullptr, *CW = nullptr, *Py = nullptr, *Px = nullptr;
@@ -1961,7 +1965,7 @@ void zr_stage_free(zr_stage_state* s) {
   void* ptrs[] = {s->edges, s->meas, s->dups, s->hs, s->cdf, s->pdf, s->gtmp, s->obj, s->info, s->weights, s->resid, s->pos, s->trans_raw, s->trans,
                   s->rel, s->err_stats, s->err_frames, s->fs, s->img_sum, s->even, s->odd, s->filt, s->rowsum,
                   s->twB, s->twB2, s->Tfrc, s->frc2d, s->frc, s->frc_part, s->hann, s->twHp, s->twWp, s->twWp2, s->Tdose, s->G,
-                  s->accum, s->thresq, s->drow, s->R0, s->RH, s->C0, s->CW, s->Py, s->Px};
+                  s->accum, s->thresq, s->R0, s->RH, s->C0, s->CW, s->Py, s->Px};
   for (void* q : ptrs)
     if (q) hipFree(q);
   delete s;
@@ -2048,12 +2052,13 @@ int zr_stage_create(zr_stage_state** out, const zr_params& p, int N, int H, int 
       ZC(smalloc(s, (void**)&s->CW, sizeof(float2) * hh));
       ZC(smalloc(s, (void**)&s->Py, sizeof(float2) * hh));
     } else {
-      ZC(smalloc(s, (void**)&s->Tdose, sizeof(float2) * (size_t)N * (Wp / 2 + 1) * Hp));
+      // a chunk of frames in the row pass's column-pair layout (an odd Wp/2+1 padded by one column)
+      const size_t kp = (size_t)(Wp / 2 + 2) / 2;
+      ZC(smalloc(s, (void**)&s->Tdose, sizeof(float2) * (size_t)std::min(N, DOSE_CHUNK) * kp * 2 * Hp));
     }
-    ZC(smalloc(s, (void**)&s->G, sizeof(float2) * (size_t)(Wp / 2 + 1) * H));
+    ZC(smalloc(s, (void**)&s->G, sizeof(float2) * (size_t)(Wp / 2 + 1) * Hp));
     ZC(smalloc(s, (void**)&s->accum, sizeof(double) * (N + 1)));
     ZC(smalloc(s, (void**)&s->thresq, sizeof(double) * N));
-    ZC(smalloc(s, (void**)&s->drow, sizeof(double) * (Hp / 4 + 1)));
   }
   for (int i = 0; i < p.n_filters; ++i)
     if (p.filters[i] == ZR_FILTER_GAUSSLP && !s->gtmp) ZC(smalloc(s, (void**)&s->gtmp, sizeof(float) * H * W));
@@ -2262,7 +2267,7 @@ static int dose_spectral(zr_stage_state* s, const float* frames, const float2* S
   }
   ZS_HIP(hipGetLastError());
   switch (s->Wp) {
-#define ZR_DO(X) case X: hipLaunchKernelGGL(k_dose_out<X>, dim3(s->H), dim3(256), 0, st, s->G, s->H, s->W, s->twWp2, s->twWp, s->filt); break;
+#define ZR_DO(X) case X: hipLaunchKernelGGL(k_dose_out<X>, dim3(s->H), dim3(256), 0, st, s->G, s->Hp, s->H, s->W, s->twWp2, s->twWp, s->filt); break;
     ZR_DO(256) ZR_DO(512) ZR_DO(1024) ZR_DO(2048) ZR_DO(3072) ZR_DO(3840) ZR_DO(4096)
 #undef ZR_DO
     default: return serr(ZR_ERR_UNSUPPORTED, "dose width %d", s->Wp);
@@ -2276,36 +2281,45 @@ int zr_stage_dose(zr_stage_state* s, const float* frames, const float2* S, int W
   if (!s || !s->have_solution) return serr(ZR_ERR_STATE, "dose filter before solve");
   if (!s->p.do_dose) return serr(ZR_ERR_STATE, "dose filter disabled in the parameters");
   if (s->spectral) return dose_spectral(s, frames, S, Wk, normalize, st);
-  const int Hp = s->Hp, Wp = s->Wp;
-  const int nblk = Hp / 4;
-  size_t sh = (size_t)4 * zr_fft_lds(Wp / 2) * sizeof(float2);
-  switch (Wp) {
-#define ZR_DR(X)                                                                                              \
-  case X: {                                                                                                   \
-    static std::once_flag lds_once;                                                                           \
-    zr_allow_full_lds((const void*)k_dose_rows<X>, lds_once);                                                 \
-    hipLaunchKernelGGL(k_dose_rows<X>, dim3(nblk), dim3(1024), sh, st, frames, s->fs, s->N, s->H, s->W, Hp, s->twWp2, \
-                       s->twWp, s->Tdose, s->img_sum, s->even, s->odd, (float*)nullptr, s->drow);            \
-  } break;
-    ZR_DR(256) ZR_DR(512) ZR_DR(1024) ZR_DR(2048) ZR_DR(3072) ZR_DR(3840) ZR_DR(4096) ZR_DR(5760)
-#undef ZR_DR
-    default: return serr(ZR_ERR_UNSUPPORTED, "dose width %d", Wp);
-  }
+  const int H = s->H, W = s->W, Hp = s->Hp, Wp = s->Wp;
+  // aligned sums (imageSum, even, odd) and their total for the dose per frame
+  const int gx = (W + 63) / 64, gy = (H + 4 * SS_R - 1) / (4 * SS_R);
+  hipLaunchKernelGGL(k_shift_sum, dim3(gx * gy), dim3(256), 0, st, frames, s->fs, s->N, H, W, s->img_sum, s->even,
+                     s->odd, (float*)nullptr, s->rowsum, gx);
   ZS_HIP(hipGetLastError());
   s->have_sums = true;
   DoseArgs a;
-  fill_dose_args(s, a, s->drow, nblk, normalize);
+  fill_dose_args(s, a, s->rowsum, gx * gy, normalize);
   hipLaunchKernelGGL(k_dose_params, dim3(1), dim3(256), 0, st, a);
   ZS_HIP(hipGetLastError());
-  switch (Hp) {
-#define ZR_DC(X) case X: hipLaunchKernelGGL(k_dose_cols<X>, dim3(Wp / 2 + 1), dim3(dose_cols_nt(X)), 0, st, s->Tdose, a, s->twHp, s->G); break;
-    ZR_DC(256) ZR_DC(512) ZR_DC(1024) ZR_DC(2048) ZR_DC(3072) ZR_DC(3840) ZR_DC(4096) ZR_DC(5760)
+  // the filtered spectra of the aligned, symmetric-padded frames, DOSE_CHUNK frames at a time
+  for (int j0 = 0; j0 < s->N; j0 += DOSE_CHUNK) {
+    const int j1 = std::min(s->N, j0 + DOSE_CHUNK);
+    switch (Wp) {
+#define ZR_DR(X)                                                                                                  \
+  case X: {                                                                                                       \
+    constexpr int RB = dose_rows_rb(X), NT = RB * dose_rows_tpr(X);                                               \
+    const size_t sh = (size_t)RB * zr_fft_lds(X / 2) * sizeof(float2);                                           \
+    static std::once_flag lds_once;                                                                               \
+    zr_allow_full_lds((const void*)k_dose_rows<X>, lds_once);                                                     \
+    hipLaunchKernelGGL(k_dose_rows<X>, dim3((j1 - j0) * (Hp / RB)), dim3(NT), sh, st, frames, s->fs, j0, H, W, Hp, \
+                       s->twWp2, s->twWp, s->Tdose);                                                              \
+  } break;
+      ZR_DR(256) ZR_DR(512) ZR_DR(1024) ZR_DR(2048) ZR_DR(3072) ZR_DR(3840) ZR_DR(4096) ZR_DR(5760)
+#undef ZR_DR
+      default: return serr(ZR_ERR_UNSUPPORTED, "dose width %d", Wp);
+    }
+    ZS_HIP(hipGetLastError());
+    switch (Hp) {
+#define ZR_DC(X) case X: hipLaunchKernelGGL(k_dose_cols<X>, dim3(Wp / 2 + 1), dim3(dose_cols_nt(X)), 0, st, s->Tdose, a, j0, j1, s->twHp, s->G); break;
+      ZR_DC(256) ZR_DC(512) ZR_DC(1024) ZR_DC(2048) ZR_DC(3072) ZR_DC(3840) ZR_DC(4096) ZR_DC(5760)
 #undef ZR_DC
-    default: return serr(ZR_ERR_UNSUPPORTED, "dose height %d", Hp);
+      default: return serr(ZR_ERR_UNSUPPORTED, "dose height %d", Hp);
+    }
+    ZS_HIP(hipGetLastError());
   }
-  ZS_HIP(hipGetLastError());
   switch (Wp) {
-#define ZR_DO(X) case X: hipLaunchKernelGGL(k_dose_out<X>, dim3(s->H), dim3(256), 0, st, s->G, s->H, s->W, s->twWp2, s->twWp, s->filt); break;
+#define ZR_DO(X) case X: hipLaunchKernelGGL(k_dose_out<X>, dim3(s->H), dim3(256), 0, st, s->G, s->Hp, s->H, s->W, s->twWp2, s->twWp, s->filt); break;
     ZR_DO(256) ZR_DO(512) ZR_DO(1024) ZR_DO(2048) ZR_DO(3072) ZR_DO(3840) ZR_DO(4096) ZR_DO(5760)
 #undef ZR_DO
     default: return serr(ZR_ERR_UNSUPPORTED, "dose width %d", Wp);

@@ -67,9 +67,9 @@ def estimate_plan_bytes(n, h, w, params) -> int:
     if any(params.filters[i] == L.ZR_FILTER_GAUSSLP for i in range(params.n_filters)):
         b += h * w * 4                                 # the Gaussian's first pass
     if params.do_dose:
-        b += wk * h * 8
+        b += wk * hp * 8
         if not spectral:
-            b += n * wk * hp * 8
+            b += min(n, 8) * (wk + 1) * hp * 8         # a chunk of the direct dose path
         else:
             b += 3 * n * (wk + hp) * 8
     if params.do_frc:
