@@ -55,13 +55,26 @@ def k2_flops(info):
 FP32_PEAK_TFLOPS = 157.3
 
 
-def load_traffic():
-    """HBM bytes per K2 launch from the committed rocprofv3 PMC summary, if any."""
-    p = os.path.join(HERE, "profiles", "k2_pairs_traffic.json")
+def k2_compulsory_bytes(info):
+    """Bytes the pair transform cannot avoid per launch: every cropped half
+    spectrum read once, every lag window written once."""
+    hs = info["Hc"] * (info["Wc"] // 2 + 1) * 8
+    return info["N"] * hs + info["pairs"] * (4 * (2 * info["maxShift"]) ** 2 + 16)
+
+
+def load_profile_json(name):
+    """A committed rocprofv3 summary (profiles/<name>), if present."""
+    p = os.path.join(HERE, "profiles", name)
     if os.path.exists(p):
         with open(p) as f:
-            return json.load(f).get("traffic_bytes")
+            return json.load(f)
     return None
+
+
+def load_traffic():
+    """HBM bytes per K2 launch from the committed rocprofv3 PMC summary, if any."""
+    d = load_profile_json("k2_pairs_traffic.json")
+    return d.get("traffic_bytes") if d else None
 
 
 PAIR_KERNELS = ("k2_cols", "k2_cols_r2", "k2_rows", "k2_rows_mfma")
@@ -135,6 +148,26 @@ def counter_summary(paths, kernels=None):
     return {k: {c: float(np.mean(v)) for c, v in d.items()} for k, d in vals.items()}
 
 
+def k2_counter_json(paths, out_json):
+    """Executed FP32 work of the pair transform from rocprofv3 --pmc passes:
+    64 lanes x (ADD + MUL + 2 FMA + TRANS) of the VALU plus 512 x the F32 MFMA
+    MOPS counter, per launch, and the VALU busy percentage of k2_cols_r2."""
+    stats = counter_summary(paths, kernels=["k2_cols", "k2_rows"])
+    flop = 0.0
+    for k, c in stats.items():
+        flop += 64.0 * (c.get("SQ_INSTS_VALU_ADD_F32", 0.0) + c.get("SQ_INSTS_VALU_MUL_F32", 0.0) +
+                        2.0 * c.get("SQ_INSTS_VALU_FMA_F32", 0.0) + c.get("SQ_INSTS_VALU_TRANS_F32", 0.0))
+        flop += 512.0 * c.get("SQ_INSTS_VALU_MFMA_MOPS_F32", 0.0)
+    cols = [v for k, v in stats.items() if k.startswith("k2_cols")]
+    res = dict(executed_flop=flop, valu_busy=(cols[0].get("VALUBusy") if cols else None),
+               lds_bank_conflict=(cols[0].get("LDSBankConflict") if cols else None),
+               kernels={k: {c: round(v, 4) for c, v in d.items()} for k, d in stats.items()})
+    with open(out_json, "w") as f:
+        json.dump(res, f, indent=1)
+    print(f"pair transform: executed {flop / 1e9:.1f} GFLOP per launch, VALU busy {res['valu_busy']}")
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -148,6 +181,8 @@ def main():
     ap.add_argument("--counter-summary", nargs="+", metavar="CSV",
                     help="per-kernel means of rocprofv3 --pmc counter_collection.csv files and exit (no GPU)")
     ap.add_argument("--kernels", nargs="*", help="with --counter-summary: kernel name prefixes to keep")
+    ap.add_argument("--k2-counters", nargs="+", metavar="CSV_OR_OUT",
+                    help="counter CSVs then the output JSON: executed FP32 work of the pair transform (no GPU)")
     ap.add_argument("--workload", default="c2", choices=["c2", "served", "batched"])
     ap.add_argument("--inflight", type=int, default=1,
                     help="C2 movies in flight at once (one plan and stream each)")
@@ -163,6 +198,9 @@ def main():
         return
     if args.counter_summary:
         counter_summary(args.counter_summary, args.kernels)
+        return
+    if args.k2_counters:
+        k2_counter_json(args.k2_counters[:-1], args.k2_counters[-1])
         return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -259,17 +297,25 @@ def bench_c2(args):
     rms = float(np.sqrt(np.mean((sol["translations"] - truth) ** 2)))
 
     k2_ms = st["pairs"][0] / max(st["pairs"][1], 1)
-    alg = k2_algorithmic_bytes(info)
-    achieved = alg / (k2_ms * 1e-3) / 1e9
-    traffic = load_traffic()
+    sec = k2_ms * 1e-3
     flops = k2_flops(info)
-    roofline = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
-                    frac=round(achieved / HBM_PEAK_GBS, 4), traffic=traffic,
-                    kernel="pair transform (k2_cols + k2_rows_mfma)", launch_ms=round(k2_ms, 4),
-                    algorithmic_bytes=alg,
-                    # a reuse implementation above 100% also reports its FLOP fraction
-                    flop_tflops=round(flops / (k2_ms * 1e-3) / 1e12, 2),
-                    flop_frac=round(flops / (k2_ms * 1e-3) / (FP32_PEAK_TFLOPS * 1e12), 4))
+    traffic = load_traffic()
+    comp = k2_compulsory_bytes(info)
+    counters = load_profile_json("k2_pairs_counters.json") or {}
+    # The pair transform is fp32-compute bound (VALU busy and the FP32 instruction
+    # counts in profiles/), so its roofline is the fp32 peak, with the scope
+    # table's algorithmic FLOP count (a 2D inverse FFT per pair) as the work.
+    roofline = dict(bound="mfma", achieved=round(flops / sec / 1e12, 2), peak=FP32_PEAK_TFLOPS, unit="TFLOP/s",
+                    frac=round(flops / sec / (FP32_PEAK_TFLOPS * 1e12), 4), traffic=traffic,
+                    kernel="pair transform (k2_cols_r2 + k2_rows_mfma), fp32 vector ALU",
+                    launch_ms=round(k2_ms, 4), algorithmic_flop=flops,
+                    hbm=dict(compulsory_bytes=comp, compulsory_frac=round(comp / sec / (HBM_PEAK_GBS * 1e9), 4),
+                             traffic_frac=(round(traffic / sec / (HBM_PEAK_GBS * 1e9), 4) if traffic else None),
+                             no_reuse_bytes=k2_algorithmic_bytes(info)),
+                    executed_flop=counters.get("executed_flop"),
+                    executed_frac=(round(counters["executed_flop"] / sec / (FP32_PEAK_TFLOPS * 1e12), 4)
+                                   if counters.get("executed_flop") else None),
+                    valu_busy=counters.get("valu_busy"))
 
     out = dict(metric=METRIC, value=round(movies_min, 2), unit="movies/min", n_gpus=1, steps=args.steps,
                warmup=args.warmup, ms_per_step=round(ms_step, 3), higher_is_better=True, scaling="weak",

@@ -44,8 +44,12 @@ enum {
 enum { ZR_TRI_TRI = 0, ZR_TRI_DIAG = 1, ZR_TRI_AUTO = 2, ZR_TRI_FIRST = 3, ZR_TRI_REFINE = 4 };
 /* Bmode (zorro/zorro.py:117, :1104-1141) */
 enum { ZR_B_NONE = 0, ZR_B_CONV = 1, ZR_B_OPTI = 2 };
-/* shiftMethod (zorro/zorro.py:190, :1734-1808) */
-enum { ZR_SHIFT_INTEGER = 0, ZR_SHIFT_LANCZOS = 1 };
+/* shiftMethod (zorro/zorro.py:190, :1734-1808).  FOURIER: the shift theorem on
+ * the zero-padded frame's spectrum (the reference's :1748-1772 as intended; it
+ * squares the y phase term and cannot run, xcorrnm2_tri failing first at
+ * :895-896): aligned = crop(Re(IFFT(FFT(pad0(frame)) exp(-2 pi i (fy ty + fx tx))))
+ * / (Hp Wp)) with the fftshifted frequency mesh (Nyquist at -1/2). */
+enum { ZR_SHIFT_INTEGER = 0, ZR_SHIFT_LANCZOS = 1, ZR_SHIFT_FOURIER = 2 };
 /* weightMode (zorro/zorro.py:199, :1335-1360) */
 enum { ZR_W_AUTOLOGISTIC = 0, ZR_W_LOGISTIC = 1, ZR_W_CORR = 2, ZR_W_UNWEIGHTED = 3, ZR_W_NORM = 4 };
 

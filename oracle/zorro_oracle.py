@@ -47,7 +47,7 @@ __all__ = [
     "mnxc2_window_unmasked", "window_from_full", "b_filter", "locate_peak",
     "xcorrnm2_tri", "build_equations", "logistic", "logistic_objective",
     "logistic_fit", "cdf_logistic_curve", "l1_solve", "exact_vertex",
-    "shifts_solver", "lanczos_kernel", "apply_shifts", "rotmean",
+    "shifts_solver", "lanczos_kernel", "apply_shifts", "fourier_shift", "rotmean",
     "hann_apodization", "frc_crop_limits", "tiled_frc", "lazy_frc",
     "dose_filter", "gauss_lowpass", "filter_steps", "orient_gain", "apply_gain", "bin_stack", "align_image_stack", "PINS",
     "HOTPIX_DEFAULTS", "hotpix_psf_kernel", "hotpix_norm_objective", "hotpix_stats", "hotpix_filter",
@@ -82,6 +82,8 @@ PINS = {
     "cdf_logistic_curve": "hand-counted 3-bin histogram, CDF and left edges of a fixed sample",
     "dose_filter (normalize)": "constant frames: DC divided by N+1; cosine gain g of the plain filter -> g/(g+1)",
     "gauss_lowpass": "explicit symmetric-padded separable sum of the normalised radius-12 taps; constant -> constant",
+    "fourier_shift": "integer shift == np.roll (of the zero-padded frame); shift then unshift == identity on a "
+                     "band-limited image; fractional shift of a cosine == its closed form; Nyquist row -> cos(pi t)",
 }
 
 
@@ -683,9 +685,11 @@ def apply_shifts(images: np.ndarray, translations: np.ndarray, p: Params):
     if p.originMode == "centroid":
         t = t - np.mean(t, axis=0)
     out = np.empty_like(images, dtype=np.float64)
+    if p.subPixReg > 1 and p.shiftMethod == "fourier":
+        for J in range(images.shape[0]):
+            out[J] = fourier_shift(images[J], t[J], p.shapePadded)
+        return out, out.sum(axis=0), t
     for J in range(images.shape[0]):
-        if p.subPixReg > 1 and p.shiftMethod == "fourier":
-            raise NotImplementedError("shiftMethod='fourier' is broken in the reference (Q10)")
         it = np.round(t[J]).astype(int)
         g = np.roll(np.roll(images[J].astype(np.float64), it[0], axis=0), it[1], axis=1)
         if p.subPixReg > 1 and p.shiftMethod == "lanczos":
@@ -694,6 +698,22 @@ def apply_shifts(images: np.ndarray, translations: np.ndarray, p: Params):
                 g = scipy.ndimage.convolve(g, lanczos_kernel(rem), mode="reflect")
         out[J] = g
     return out, out.sum(axis=0), t
+
+
+def fourier_shift(img: np.ndarray, t, shapePadded):
+    """shiftMethod='fourier' with the shift theorem (``zorro/zorro.py:1748-1772``
+    as intended; the reference squares the y phase term and its xcorrnm2_tri
+    fails first, Q10): the frame zero-padded to shapePadded (mask x frame, as
+    the stored spectra of ``:884-895``), times exp(-2 pi i (fy ty + fx tx)) on
+    the fftshifted frequency mesh (Nyquist at -1/2), the real part of the
+    normalised inverse, cropped back to the frame."""
+    h, w = img.shape
+    hp, wp = (int(shapePadded[0]), int(shapePadded[1])) if shapePadded is not None else (h, w)
+    pad = np.zeros((hp, wp))
+    pad[:h, :w] = img
+    xm, ym = np.meshgrid(np.arange(-wp / 2, wp / 2) / wp, np.arange(-hp / 2, hp / 2) / hp)
+    ramp = np.fft.fftshift(np.exp(-2j * np.pi * (xm * t[1] + ym * t[0])))
+    return np.real(np.fft.ifft2(np.fft.fft2(pad) * ramp))[:h, :w]
 
 
 def rotmean(mage: np.ndarray):

@@ -662,3 +662,41 @@ def test_filter_steps_follow_token_order():
     assert O.filter_steps(None) == []
     with pytest.raises(NotImplementedError):
         O.filter_steps("dose,background")
+
+
+# ------------------------------------------------------------- Fourier shifts
+def test_fourier_shift_integer_is_roll_and_padded_roll():
+    img = RNG.standard_normal((24, 20))
+    got = O.fourier_shift(img, (3.0, -5.0), (24, 20))
+    assert np.allclose(got, np.roll(np.roll(img, 3, axis=0), -5, axis=1), atol=1e-12)
+    # padded: a circular shift of the zero-padded frame, cropped back
+    pad = np.zeros((32, 32))
+    pad[:24, :20] = img
+    got = O.fourier_shift(img, (2.0, 4.0), (32, 32))
+    assert np.allclose(got, np.roll(np.roll(pad, 2, axis=0), 4, axis=1)[:24, :20], atol=1e-12)
+
+
+def test_fourier_shift_round_trip_and_cosine_closed_form():
+    h = w = 32
+    yy, xx = np.mgrid[0:h, 0:w]
+    # band-limited (no Nyquist content): shift then unshift is the identity
+    F = np.zeros((h, w), complex)
+    F[1:6, 1:6] = RNG.standard_normal((5, 5)) + 1j * RNG.standard_normal((5, 5))
+    img = np.real(np.fft.ifft2(F))
+    t = np.array([0.37, -1.62])
+    back = O.fourier_shift(O.fourier_shift(img, t, (h, w)), -t, (h, w))
+    assert np.allclose(back, img, atol=1e-12)
+    # a fractional shift of a cosine: cos(2 pi k (x - t) / w)
+    img = np.cos(2 * np.pi * 3 * xx / w) + 2.0
+    got = O.fourier_shift(img, (0.0, 0.4), (h, w))
+    assert np.allclose(got, np.cos(2 * np.pi * 3 * (xx - 0.4) / w) + 2.0, atol=1e-12)
+
+
+def test_fourier_shift_nyquist_row_takes_cos_pi_t():
+    # the fftshifted mesh puts the Nyquist frequency at -1/2: (-1)^y shifted by ty
+    # keeps the real part cos(pi ty) (-1)^y
+    h = w = 16
+    yy, xx = np.mgrid[0:h, 0:w]
+    img = (-1.0) ** yy
+    got = O.fourier_shift(img, (0.3, 0.0), (h, w))
+    assert np.allclose(got, np.cos(np.pi * 0.3) * img, atol=1e-12)

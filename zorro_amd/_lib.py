@@ -15,7 +15,7 @@ LIB_PATH = os.path.join(HERE, "libzorro_hip.so")
 ZR_OK, ZR_ERR_ARG, ZR_ERR_HIP, ZR_ERR_UNSUPPORTED, ZR_ERR_STATE, ZR_ERR_NOMEM = 0, -1, -2, -3, -4, -5
 ZR_TRI_TRI, ZR_TRI_DIAG, ZR_TRI_AUTO, ZR_TRI_FIRST, ZR_TRI_REFINE = 0, 1, 2, 3, 4
 ZR_B_NONE, ZR_B_CONV, ZR_B_OPTI = 0, 1, 2
-ZR_SHIFT_INTEGER, ZR_SHIFT_LANCZOS = 0, 1
+ZR_SHIFT_INTEGER, ZR_SHIFT_LANCZOS, ZR_SHIFT_FOURIER = 0, 1, 2
 ZR_W_AUTOLOGISTIC, ZR_W_LOGISTIC, ZR_W_CORR, ZR_W_UNWEIGHTED, ZR_W_NORM = 0, 1, 2, 3, 4
 ZR_DTYPE_F32, ZR_DTYPE_U8, ZR_DTYPE_U16, ZR_DTYPE_U4, ZR_DTYPE_I16 = 0, 1, 2, 3, 4
 ZR_FILTER_DOSE, ZR_FILTER_DOSENORM, ZR_FILTER_GAUSSLP = 1, 2, 3

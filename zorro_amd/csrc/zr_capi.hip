@@ -139,7 +139,7 @@ struct zr_movie {
   bool crop = false, unmasked = true;
   // spectra layout: Wk kept columns, Hs stored rows per column, row offset of the
   // cropped rows >= Hc/2, index of the +Hc/2 row inside a stored column (-1: Sside)
-  bool spectral = false;
+  bool spectral = false, full_spec = false;
   int Wk = 0, Hs = 0, hoff = 0, side_in_col = -1;
   int batch = 1, rb = 4;
   int k2_cpw = 8;                  // lag-window columns per wave of the pair transform
@@ -203,6 +203,7 @@ struct zr_movie {
   // a movie of a batch is a view that borrows the batch's frames and spectra
   bool xcorr_only = false;
   bool view = false;
+  bool fourier_aligned = false;          // shiftMethod 'fourier': aligned frames formed since the last solve
   hipStream_t batch_stream = nullptr;   // a view's results are complete once this stream has passed them
 };
 
@@ -547,17 +548,20 @@ static int create_movie(int device, const zr_params* p, int n_frames, int h, int
   mv->k2_cpw = p->k2_cols_per_wave > 0 ? p->k2_cols_per_wave : 8;
   mv->rb = zr_k1_rows_per_block(mv->Wp);
   const int nrb = mv->Hp / mv->rb;
-  mv->spectral = p->do_dose && mv->unmasked && mv->H == mv->W && mv->Hp <= 4096;
-  mv->Wk = mv->spectral ? mv->Wp / 2 + 1 : mv->Wh;
-  mv->Hs = mv->spectral ? mv->Hp : mv->Hc;
-  mv->hoff = mv->spectral ? mv->Hp - mv->Hc : 0;
-  mv->side_in_col = mv->spectral ? mv->Hc / 2 : -1;
+  const bool fourier = p->shift_method == ZR_SHIFT_FOURIER && p->sub_pix_reg > 1;
+  mv->spectral = p->do_dose && mv->unmasked && mv->H == mv->W && mv->Hp <= 4096 && !fourier;
+  // full-height half spectra: kept for the spectral dose path and for Fourier shifts
+  mv->full_spec = mv->spectral || fourier;
+  mv->Wk = mv->full_spec ? mv->Wp / 2 + 1 : mv->Wh;
+  mv->Hs = mv->full_spec ? mv->Hp : mv->Hc;
+  mv->hoff = mv->full_spec ? mv->Hp - mv->Hc : 0;
+  mv->side_in_col = mv->full_spec ? mv->Hc / 2 : -1;
   const size_t nspec = (size_t)mv->nspec;
   const size_t nrb_sums = (size_t)(mv->H + 7) / 8;   // k1_frame_sums row blocks
   // K1 intermediate: column pairs, an odd Wk padded by one zero column
   if ((rc = dev_alloc(&mv->T, (size_t)mv->batch * ((mv->Wk + 1) / 2) * 2 * mv->Hp)) ||
       (rc = dev_alloc(&mv->S, nspec * mv->Wk * mv->Hs)) ||
-      (rc = dev_alloc(&mv->Sside, mv->spectral ? 0 : nspec * mv->Wk)) ||
+      (rc = dev_alloc(&mv->Sside, mv->full_spec ? 0 : nspec * mv->Wk)) ||
       (rc = dev_alloc(&mv->Z, (size_t)mv->Ptot * mv->Wh * 2 * mv->m)) ||
       (rc = dev_alloc(&mv->part, N * std::max<size_t>(nrb, nrb_sums) * 2)) ||
       (rc = dev_alloc(&mv->stats, N * 2)) || (rc = dev_alloc(&mv->stats_c, N * 2)) ||
@@ -814,7 +818,7 @@ static int xcorr_part(zr_movie* mv, int part) {
     mv->hot_ran = true;
   }
   hipEvent_t t0 = mv->timer.begin(s);
-  const int full = mv->spectral ? 1 : 0;
+  const int full = mv->full_spec ? 1 : 0;
   const size_t slot = (size_t)mv->Wk * mv->Hs;
   if (mv->unmasked) {
     // K1: spectra in batches of frames; frame sums on the way
@@ -978,6 +982,7 @@ extern "C" int zr_solve(zr_movie* mv) {
   hipEvent_t t0 = mv->timer.begin(mv->stream);
   int rc = zr_stage_solve(mv->st, mv->shifts, mv->corr, mv->peaksig, mv->stream);
   mv->timer.end(ZR_STAGE_SOLVE, t0, mv->stream);
+  mv->fourier_aligned = false;
   return rc ? fail(rc, "%s", zr_stage_error()) : ZR_OK;
 }
 
@@ -994,10 +999,24 @@ extern "C" int zr_set_trimats(zr_movie* mv, const float* shifts, const float* co
   return ZR_OK;
 }
 
+// shiftMethod 'fourier': the aligned frames from the resident full-height
+// spectra, once per solve, before the stages that read them
+static int ensure_fourier(zr_movie* mv) {
+  if (mv->p.shift_method != ZR_SHIFT_FOURIER || mv->p.sub_pix_reg <= 1 || mv->fourier_aligned) return ZR_OK;
+  if (!mv->have_xcorr || !mv->S) return fail(ZR_ERR_STATE, "Fourier shifts need the spectra of zr_xcorr");
+  // masked frames were centred by their means (the common mean for 'refine') before the transforms
+  const double* mu = mv->unmasked ? nullptr : (mv->p.tri_mode == ZR_TRI_REFINE ? mv->stats_c : mv->stats);
+  int rc = zr_stage_fourier_align(mv->st, mv->S, mv->Wk, mv->Hs, mv->unmasked ? -1 : 2 * mv->N, mu, mv->stream);
+  if (rc) return fail(rc, "%s", zr_stage_error());
+  mv->fourier_aligned = true;
+  return ZR_OK;
+}
+
 extern "C" int zr_apply_shifts(zr_movie* mv, float* aligned_out) {
   if (!mv) return fail(ZR_ERR_ARG, "NULL movie");
   if (!mv->frames) return fail(ZR_ERR_STATE, "frames not set");
   ZR_HIP(hipSetDevice(mv->device));
+  if (int rf = ensure_fourier(mv)) return rf;
   hipEvent_t t0 = mv->timer.begin(mv->stream);
   int rc = zr_stage_shifts(mv->st, mv->frames, aligned_out, mv->stream);
   mv->timer.end(ZR_STAGE_SHIFTS, t0, mv->stream);
@@ -1026,6 +1045,7 @@ extern "C" int zr_filter(zr_movie* mv, int step) {
   int rc = ZR_OK;
   if (step == ZR_FILTER_DOSE || step == ZR_FILTER_DOSENORM) {
     if (mv->spectral && !mv->have_xcorr) return fail(ZR_ERR_STATE, "dose filter needs the spectra of zr_xcorr");
+    if (int rf = ensure_fourier(mv)) return rf;
     hipEvent_t t0 = mv->timer.begin(mv->stream);
     rc = zr_stage_dose(mv->st, mv->frames, mv->S, mv->Wk, step == ZR_FILTER_DOSENORM ? 1 : 0, mv->stream);
     mv->timer.end(ZR_STAGE_DOSE, t0, mv->stream);
@@ -1332,7 +1352,7 @@ static int create_view(zr_batch* b, int i, const zr_params& p, zr_movie** out) {
   v->N = n;
   v->Hr = big->Hr; v->Wr = big->Wr; v->H = big->H; v->W = big->W; v->Hp = big->Hp; v->Wp = big->Wp;
   v->Hc = big->Hc; v->Wc = big->Wc; v->Wh = big->Wh; v->m = big->m; v->Lper = big->Lper; v->disc = big->disc;
-  v->crop = big->crop; v->unmasked = big->unmasked; v->spectral = big->spectral;
+  v->crop = big->crop; v->unmasked = big->unmasked; v->spectral = big->spectral; v->full_spec = big->full_spec;
   v->Wk = big->Wk; v->Hs = big->Hs; v->hoff = big->hoff; v->side_in_col = big->side_in_col;
   v->batch = big->batch; v->rb = big->rb; v->k2_cpw = big->k2_cpw;
   v->frames_owned = big->frames_owned + (size_t)off * big->H * big->W;
@@ -1371,6 +1391,8 @@ extern "C" int zr_batch_create(int device, const zr_params* p, int n_movies, con
     return fail(ZR_ERR_UNSUPPORTED, "batches run triMode 'tri' or 'diag' (got %d)", p->tri_mode);
   if (p->pre_shift || p->do_hot || p->shape_binned[0] > 0 || p->shape_binned[1] > 0)
     return fail(ZR_ERR_UNSUPPORTED, "batches take float32 frames without preShift, 'hot' or binning");
+  if (p->shift_method == ZR_SHIFT_FOURIER)
+    return fail(ZR_ERR_UNSUPPORTED, "batches shift by Lanczos kernels or integers ('fourier' runs per movie)");
   zr_batch* b = new zr_batch();
   b->device = device;
   int total = 0;

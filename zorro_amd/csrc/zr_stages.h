@@ -32,6 +32,11 @@ int zr_stage_set_frc_crop(zr_stage_state* s, const double* trans, int n, hipStre
 // created with spectral != 0; may be NULL otherwise).
 // normalize != 0: doseFilter(normalize=True) (filter sum + 1 in the denominator)
 int zr_stage_dose(zr_stage_state* s, const float* frames, const float2* S, int Wk, int normalize, hipStream_t st);
+// shiftMethod 'fourier': the aligned frames from the full-height half spectra S
+// (slots = frames; mask_slot >= 0: the masked path's mask spectrum, added
+// back with the frame means mu[2J] / (H W) that were taken out)
+int zr_stage_fourier_align(zr_stage_state* s, const float2* S, int Wk, int Hs, int mask_slot, const double* mu,
+                           hipStream_t st);
 // 'gausslp' on filtSum (on imageSum when no filter step has run since the solve)
 int zr_stage_gausslp(zr_stage_state* s, hipStream_t st);
 // cdfLogisticCurve's histogram (pdf counts, cdf, left edges); nbins = 0 without autologistic weights

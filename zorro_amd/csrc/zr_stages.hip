@@ -1751,6 +1751,93 @@ __global__ __launch_bounds__(256) void k_frc_setup(int box, float2* __restrict__
   }
 }
 
+// ---------------------------------------------------------------- Fourier shifts
+// shiftMethod 'fourier' (zorro/zorro.py:1748-1772, with the shift theorem the
+// reference intends): aligned_J = crop(Re(R[F_J exp(-2 pi i (fy ty + fx tx))]) /
+// (Hp Wp)), F_J the spectrum of the zero-padded frame, fy = k/Hp over the
+// fftshifted range (Nyquist at -1/2).  The real part of the inverse equals the
+// inverse of the Hermitian part of the product, F_J (ramp(k) + conj(ramp(-k)))/2
+// (F_J is Hermitian), which a real-output inverse then takes exactly.
+ZR_DEV double signed_freq(int k, int n) { return (double)(k < n / 2 ? k : k - n) / (double)n; }
+
+// columns: one block per (frame of the chunk, kx); the frame's half-spectrum
+// column (plus mean x mask spectrum on the masked path, whose frames were
+// centred before their transforms), times the Hermitian ramp, inverse along y;
+// rows y < H into Gf[frame][kx][y]
+template <int HP>
+__global__ __launch_bounds__(dose_cols_nt(HP)) void k_fshift_cols(const float2* __restrict__ S, int Wk, int Hs,
+                                                                  int mask_slot, const double* __restrict__ mu,
+                                                                  double npix, const double* __restrict__ trans,
+                                                                  int Wp, int H, int j0,
+                                                                  const float2* __restrict__ twH,
+                                                                  float2* __restrict__ Gf) {
+  constexpr int NT = dose_cols_nt(HP), PER = HP / NT;
+  __shared__ float2 buf[zr_fft_lds(HP)];
+  const int tid = threadIdx.x;
+  const int blk = xcd_remap(blockIdx.x, gridDim.x);
+  const int jl = blk / Wk, kx = blk % Wk, J = j0 + jl;
+  const double ty = trans[2 * J], tx = trans[2 * J + 1];
+  const float m = mask_slot >= 0 ? (float)(mu[2 * J] / npix) : 0.f;
+  const float2* col = S + ((size_t)J * Wk + kx) * Hs;
+  const float2* mcol = S + ((size_t)(mask_slot >= 0 ? mask_slot : 0) * Wk + kx) * Hs;
+  const double fx = signed_freq(kx, Wp), fx2 = signed_freq((Wp - kx) % Wp, Wp);
+#pragma unroll
+  for (int i = 0; i < PER; ++i) {
+    const int ky = tid + i * NT;
+    float2 v = col[ky];
+    if (mask_slot >= 0) {
+      const float2 mk = mcol[ky];
+      v = make_float2(v.x + m * mk.x, v.y + m * mk.y);
+    }
+    // phases in cycles, reduced to [-1/2, 1/2] before the trigonometry
+    double p1 = -(signed_freq(ky, HP) * ty + fx * tx);
+    double p2 = signed_freq((HP - ky) % HP, HP) * ty + fx2 * tx;
+    p1 -= rint(p1);
+    p2 -= rint(p2);
+    double s1, c1, s2, c2;
+    sincos(2.0 * M_PI * p1, &s1, &c1);
+    sincos(2.0 * M_PI * p2, &s2, &c2);
+    const float2 r = make_float2((float)(0.5 * (c1 + c2)), (float)(0.5 * (s1 + s2)));
+    buf[ky] = c_mul(v, r);
+  }
+  __syncthreads();
+  wg_fft<HP, 1, NT>(buf, twH, tid);
+  float2* dst = Gf + ((size_t)jl * Wk + kx) * H;
+  for (int y = tid; y < H; y += NT) dst[y] = buf[y];
+}
+
+// rows: one block per (frame of the chunk, y < H): real-output inverse along x
+// from the half spectrum (columns 0..WP/2), scaled by 1 / (Hp WP), cropped to W
+template <int WP>
+__global__ __launch_bounds__(256) void k_fshift_rows(const float2* __restrict__ Gf, int Wk, int H, int W, int j0,
+                                                     float scale, const float2* __restrict__ twL,
+                                                     const float2* __restrict__ tw2L, float* __restrict__ aligned) {
+  constexpr int L = WP / 2, NT = 256;
+  __shared__ float2 buf[zr_fft_lds(L)];
+  const int blk = xcd_remap(blockIdx.x, gridDim.x);
+  const int jl = blk / H, y = blk % H, J = j0 + jl;
+  const int tid = threadIdx.x;
+  const float2* g = Gf + (size_t)jl * Wk * H + y;
+  for (int k = tid; k < L; k += NT) {
+    const float2 xk = g[(size_t)k * H];
+    const float2 xn = c_conj(g[(size_t)(L - k) * H]);
+    const float2 e = c_scale(c_add(xk, xn), 0.5f);
+    const float2 d = c_scale(c_sub(xk, xn), 0.5f);
+    const float2 w = tw2L[k];
+    const float2 o = c_mul(d, make_float2(w.x, -w.y));    // * exp(+2 pi i k / WP)
+    buf[k] = make_float2(e.x - o.y, e.y + o.x);            // E + i O
+  }
+  __syncthreads();
+  wg_fft<L, 1, NT>(buf, twL, tid);
+  float* out = aligned + ((size_t)J * H + y) * W;
+  for (int n = tid; n < L; n += NT) {
+    const float2 z = buf[n];
+    const int x0 = 2 * n, x1 = 2 * n + 1;
+    if (x0 < W) out[x0] = 2.f * z.x * scale;
+    if (x1 < W) out[x1] = 2.f * z.y * scale;
+  }
+}
+
 // ---------------------------------------------------------------- gausslp
 // filterMode 'gausslp' (zorro/zorro.py:1707-1711):
 // scipy.ndimage.gaussian_filter(img, 3.0) of a float32 image, i.e. for axis 0
@@ -1926,6 +2013,12 @@ struct zr_stage_state {
   float2 *twHp = nullptr, *twWp = nullptr, *twWp2 = nullptr, *Tdose = nullptr, *G = nullptr;
   double *accum = nullptr, *thresq = nullptr;
   float* gtmp = nullptr;       // 'gausslp': the axis-0 pass of the Gaussian
+  // shiftMethod 'fourier': the aligned frames, identity shifts for the stages
+  // that read them, and a chunk of column-inverted spectra
+  bool fourier = false;
+  float* aligned = nullptr;
+  FrameShift* fs_id = nullptr;
+  float2* Gf = nullptr;
   bool spectral = false;       // dose filter from the forward spectra (unpadded frames)
   float2 *R0 = nullptr, *RH = nullptr, *C0 = nullptr, *CW = nullptr, *Py = nullptr, *Px = nullptr;
 };
@@ -1962,7 +2055,7 @@ int zr_stage_copy_sums_device(zr_stage_state* s, float* image_sum, float* filt_s
 
 void zr_stage_free(zr_stage_state* s) {
   if (!s) return;
-  void* ptrs[] = {s->edges, s->meas, s->dups, s->hs, s->cdf, s->pdf, s->gtmp, s->obj, s->info, s->weights, s->resid, s->pos, s->trans_raw, s->trans,
+  void* ptrs[] = {s->edges, s->meas, s->dups, s->hs, s->cdf, s->pdf, s->gtmp, s->aligned, s->fs_id, s->Gf, s->obj, s->info, s->weights, s->resid, s->pos, s->trans_raw, s->trans,
                   s->rel, s->err_stats, s->err_frames, s->fs, s->img_sum, s->even, s->odd, s->filt, s->rowsum,
                   s->twB, s->twB2, s->Tfrc, s->frc2d, s->frc, s->frc_part, s->hann, s->twHp, s->twWp, s->twWp2, s->Tdose, s->G,
                   s->accum, s->thresq, s->R0, s->RH, s->C0, s->CW, s->Py, s->Px};
@@ -2062,6 +2155,18 @@ int zr_stage_create(zr_stage_state** out, const zr_params& p, int N, int H, int 
   }
   for (int i = 0; i < p.n_filters; ++i)
     if (p.filters[i] == ZR_FILTER_GAUSSLP && !s->gtmp) ZC(smalloc(s, (void**)&s->gtmp, sizeof(float) * H * W));
+  s->fourier = p.shift_method == ZR_SHIFT_FOURIER && p.sub_pix_reg > 1;
+  if (s->fourier) {
+    ZC(smalloc(s, (void**)&s->aligned, sizeof(float) * (size_t)N * H * W));
+    ZC(smalloc(s, (void**)&s->fs_id, sizeof(FrameShift) * N));
+    ZC(hipMemset(s->fs_id, 0, sizeof(FrameShift) * N));          // no roll, no kernel: the frames as they are
+    ZC(smalloc(s, (void**)&s->Gf, sizeof(float2) * (size_t)std::min(N, DOSE_CHUNK) * (Wp / 2 + 1) * H));
+    if (!s->twHp) {
+      ZC(supload(s, &s->twHp, fwd_table(Hp)));
+      ZC(supload(s, &s->twWp, fwd_table(Wp)));
+      ZC(supload(s, &s->twWp2, fwd_table(Wp / 2)));
+    }
+  }
 #undef ZC
   *out = s;
   return ZR_OK;
@@ -2150,7 +2255,12 @@ int zr_stage_solve(zr_stage_state* s, const float* shifts, const float* corr, co
 int zr_stage_shifts(zr_stage_state* s, const float* frames, float* aligned_out, hipStream_t st) {
   if (!s || !s->have_solution) return serr(ZR_ERR_STATE, "shifts before solve");
   const int gx = (s->W + 63) / 64, gy = (s->H + 4 * SS_R - 1) / (4 * SS_R);
-  hipLaunchKernelGGL(k_shift_sum, dim3(gx * gy), dim3(256), 0, st, frames, s->fs, s->N, s->H, s->W, s->img_sum,
+  const FrameShift* fs = s->fs;
+  if (s->fourier) {            // the frames Fourier-shifted by zr_stage_fourier_align, summed as they are
+    frames = s->aligned;
+    fs = s->fs_id;
+  }
+  hipLaunchKernelGGL(k_shift_sum, dim3(gx * gy), dim3(256), 0, st, frames, fs, s->N, s->H, s->W, s->img_sum,
                      s->even, s->odd, aligned_out, s->rowsum, gx);
   ZS_HIP(hipGetLastError());
   s->have_sums = true;
@@ -2282,9 +2392,14 @@ int zr_stage_dose(zr_stage_state* s, const float* frames, const float2* S, int W
   if (!s->p.do_dose) return serr(ZR_ERR_STATE, "dose filter disabled in the parameters");
   if (s->spectral) return dose_spectral(s, frames, S, Wk, normalize, st);
   const int H = s->H, W = s->W, Hp = s->Hp, Wp = s->Wp;
+  const FrameShift* fs = s->fs;
+  if (s->fourier) {            // the Fourier-shifted frames, taken as they are
+    frames = s->aligned;
+    fs = s->fs_id;
+  }
   // aligned sums (imageSum, even, odd) and their total for the dose per frame
   const int gx = (W + 63) / 64, gy = (H + 4 * SS_R - 1) / (4 * SS_R);
-  hipLaunchKernelGGL(k_shift_sum, dim3(gx * gy), dim3(256), 0, st, frames, s->fs, s->N, H, W, s->img_sum, s->even,
+  hipLaunchKernelGGL(k_shift_sum, dim3(gx * gy), dim3(256), 0, st, frames, fs, s->N, H, W, s->img_sum, s->even,
                      s->odd, (float*)nullptr, s->rowsum, gx);
   ZS_HIP(hipGetLastError());
   s->have_sums = true;
@@ -2302,7 +2417,7 @@ int zr_stage_dose(zr_stage_state* s, const float* frames, const float2* S, int W
     const size_t sh = (size_t)RB * zr_fft_lds(X / 2) * sizeof(float2);                                           \
     static std::once_flag lds_once;                                                                               \
     zr_allow_full_lds((const void*)k_dose_rows<X>, lds_once);                                                     \
-    hipLaunchKernelGGL(k_dose_rows<X>, dim3((j1 - j0) * (Hp / RB)), dim3(NT), sh, st, frames, s->fs, j0, H, W, Hp, \
+    hipLaunchKernelGGL(k_dose_rows<X>, dim3((j1 - j0) * (Hp / RB)), dim3(NT), sh, st, frames, fs, j0, H, W, Hp, \
                        s->twWp2, s->twWp, s->Tdose);                                                              \
   } break;
       ZR_DR(256) ZR_DR(512) ZR_DR(1024) ZR_DR(2048) ZR_DR(3072) ZR_DR(3840) ZR_DR(4096) ZR_DR(5760)
@@ -2326,6 +2441,33 @@ int zr_stage_dose(zr_stage_state* s, const float* frames, const float2* S, int W
   }
   ZS_HIP(hipGetLastError());
   s->have_dose = true;
+  return ZR_OK;
+}
+
+int zr_stage_fourier_align(zr_stage_state* s, const float2* S, int Wk, int Hs, int mask_slot, const double* mu,
+                           hipStream_t st) {
+  if (!s || !s->have_solution) return serr(ZR_ERR_STATE, "Fourier shifts before solve");
+  if (!s->fourier) return serr(ZR_ERR_STATE, "shiftMethod is not 'fourier'");
+  const int N = s->N, H = s->H, W = s->W, Hp = s->Hp, Wp = s->Wp;
+  if (Wk != Wp / 2 + 1 || Hs != Hp) return serr(ZR_ERR_STATE, "Fourier shifts need full-height half spectra");
+  const float scale = (float)(1.0 / ((double)Hp * (double)Wp));
+  for (int j0 = 0; j0 < N; j0 += DOSE_CHUNK) {
+    const int nb = std::min(N, j0 + DOSE_CHUNK) - j0;
+    switch (Hp) {
+#define ZR_FC(X) case X: hipLaunchKernelGGL(k_fshift_cols<X>, dim3(nb * Wk), dim3(dose_cols_nt(X)), 0, st, S, Wk, Hs, mask_slot, mu, (double)H * W, s->trans, Wp, H, j0, s->twHp, s->Gf); break;
+      ZR_FC(256) ZR_FC(512) ZR_FC(1024) ZR_FC(2048) ZR_FC(3072) ZR_FC(3840) ZR_FC(4096) ZR_FC(5760)
+#undef ZR_FC
+      default: return serr(ZR_ERR_UNSUPPORTED, "Fourier shift height %d", Hp);
+    }
+    ZS_HIP(hipGetLastError());
+    switch (Wp) {
+#define ZR_FR(X) case X: hipLaunchKernelGGL(k_fshift_rows<X>, dim3(nb * H), dim3(256), 0, st, s->Gf, Wk, H, W, j0, scale, s->twWp2, s->twWp, s->aligned); break;
+      ZR_FR(256) ZR_FR(512) ZR_FR(1024) ZR_FR(2048) ZR_FR(3072) ZR_FR(3840) ZR_FR(4096) ZR_FR(5760)
+#undef ZR_FR
+      default: return serr(ZR_ERR_UNSUPPORTED, "Fourier shift width %d", Wp);
+    }
+    ZS_HIP(hipGetLastError());
+  }
   return ZR_OK;
 }
 

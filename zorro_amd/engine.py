@@ -86,9 +86,10 @@ def make_params(triMode="diag", diagWidth=5, diagStart=0, maxShift=100, Bmode="o
     p.b_mode = BMODE.get(Bmode, L.ZR_B_NONE)
     p.b_rad = float(Brad or 0.0)
     p.sub_pix_reg = int(subPixReg or 1)
-    if shiftMethod == "fourier":
-        raise NotImplementedError("shiftMethod='fourier' is broken in the reference (zorro/zorro.py:891-898)")
-    p.shift_method = L.ZR_SHIFT_LANCZOS if shiftMethod == "lanczos" else L.ZR_SHIFT_INTEGER
+    # 'fourier' is broken in the reference (zorro/zorro.py:891-898, :1762-1763); here it is
+    # the shift theorem on the zero-padded frames' spectra
+    p.shift_method = {"lanczos": L.ZR_SHIFT_LANCZOS, "fourier": L.ZR_SHIFT_FOURIER}.get(shiftMethod,
+                                                                                       L.ZR_SHIFT_INTEGER)
     p.weight_mode = WMODE.get(weightMode, L.ZR_W_CORR)
     p.peaksig_thres, p.logistic_k, p.logistic_nu = float(peaksigThres), float(logisticK), float(logisticNu)
     p.origin_centroid = 1 if originMode == "centroid" else 0

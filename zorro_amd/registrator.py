@@ -14,8 +14,11 @@ Differences from the reference, all deliberate:
   uses random basinhopping, ``zorro/zorro.py:1407-1425``, ``:2008``);
 * ``Bmode='conv'`` accepts list-valued ``shapePadded`` (the reference divides
   a list by a float, ``zorro/zorro.py:1136``);
-* modes that are broken in the reference raise ``NotImplementedError``
-  (``shiftMethod='fourier'``, ``Bmode='fourier'``, ``doseFiltParam[0]`` set).
+* ``shiftMethod='fourier'`` (broken in the reference, ``zorro/zorro.py:891-898``,
+  ``:1762-1763``) applies the shift theorem to the zero-padded frames' spectra;
+* modes that are broken in the reference and have no intended meaning to
+  restore raise ``NotImplementedError`` (``Bmode='fourier'``,
+  ``doseFiltParam[0]`` set).
 """
 from __future__ import annotations
 
