@@ -183,7 +183,7 @@ def main():
     ap.add_argument("--kernels", nargs="*", help="with --counter-summary: kernel name prefixes to keep")
     ap.add_argument("--k2-counters", nargs="+", metavar="CSV_OR_OUT",
                     help="counter CSVs then the output JSON: executed FP32 work of the pair transform (no GPU)")
-    ap.add_argument("--workload", default="c2", choices=["c2", "served", "batched"])
+    ap.add_argument("--workload", default="c2", choices=["c2", "served", "batched", "hostinput"])
     ap.add_argument("--inflight", type=int, default=1,
                     help="C2 movies in flight at once (one plan and stream each)")
     ap.add_argument("--seconds", type=float, default=6.0, help="served: arrival window per rate")
@@ -215,6 +215,8 @@ def main():
         return bench_served(args)
     if args.workload == "batched":
         return bench_batched(args)
+    if args.workload == "hostinput":
+        return bench_hostinput(args)
     return bench_c2(args)
 
 
@@ -345,6 +347,87 @@ def bench_c2(args):
         out["speedup_vs_cpu"] = round(movies_min / v, 1)
     for p in plans:
         p.close()
+    print(json.dumps(out), flush=True)
+
+
+def bench_hostinput(args):
+    """C2 movies whose frames start in pinned host memory: float32 frames
+    (2.7 GB per movie) and 4-bit counting frames (two counts per byte, 0.34 GB,
+    unpacked on the GPU).  Two plans on two streams; the host-to-device copy of
+    the next movie runs on a copy stream while the other plan computes, so the
+    rate is the slower of the PCIe copy and the path itself."""
+    import torch
+    from zorro_amd.engine import MoviePlan, make_params
+    from zorro_amd.synth import make_movie_torch
+    dev = torch.device("cuda:0")
+    c = WORKLOAD
+    frames, drift = make_movie_torch(c["n"], c["h"], c["w"], seed=c["seed"], dose=c["dose"], jump=c["jump"],
+                                     device=dev)
+    counts = torch.clamp(frames, max=15).to(torch.uint8)
+    packed = (counts[..., 0::2] | (counts[..., 1::2] << 4)).contiguous()
+    host = {"f32": torch.empty(frames.shape, dtype=torch.float32, pin_memory=True),
+            "u4": torch.empty(packed.shape, dtype=torch.uint8, pin_memory=True)}
+    host["f32"].copy_(frames)
+    host["u4"].copy_(packed)
+    del frames, counts, packed
+    torch.cuda.synchronize()
+    out = dict(metric="movies/min with host-resident frames (40x4096^2, copy overlapped)", unit="movies/min",
+               n_gpus=1, data="synthetic", inflight=2, modes={})
+    steps = max(args.steps, 6)
+    for mode in ("f32", "u4"):
+        h_src = host[mode]
+        bufs = [torch.empty(h_src.shape, dtype=h_src.dtype, device=dev) for _ in range(2)]
+        comp = [torch.cuda.Stream(device=dev) for _ in range(2)]
+        copy_s = torch.cuda.Stream(device=dev)
+        plans = [MoviePlan(make_params(**PARAMS), c["n"], c["h"], c["w"]) for _ in range(2)]
+        for k, p in enumerate(plans):
+            p.set_stream(comp[k].cuda_stream)
+            if mode == "f32":
+                p.set_frames(bufs[k])
+            p.use_graph(True)
+        done = [torch.cuda.Event() for _ in range(2)]
+        copied = [torch.cuda.Event() for _ in range(2)]
+
+        def movie(k):
+            copy_s.wait_event(done[k])
+            with torch.cuda.stream(copy_s):
+                bufs[k].copy_(h_src, non_blocking=True)
+            copied[k].record(copy_s)
+            comp[k].wait_event(copied[k])
+            if mode == "u4":
+                plans[k].set_frames_raw(bufs[k], packed4=True)
+            plans[k].align()
+            done[k].record(comp[k])
+
+        for s_ in range(4):                 # warm-up: graph capture on both plans
+            movie(s_ % 2)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for s_ in range(steps):
+            movie(s_ % 2)
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - t0) * 1e3 / steps
+        # the copy alone, for the PCIe rate
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        with torch.cuda.stream(copy_s):
+            for _ in range(3):
+                bufs[0].copy_(h_src, non_blocking=True)
+        torch.cuda.synchronize()
+        copy_ms = (time.perf_counter() - t1) * 1e3 / 3
+        nbytes = h_src.numel() * h_src.element_size()
+        sol = plans[0].solution()
+        truth = -(drift - drift.mean(axis=0))
+        out["modes"][mode] = dict(value=round(60000.0 / ms, 1), ms_per_movie=round(ms, 3),
+                                  host_bytes_per_movie=nbytes, copy_ms=round(copy_ms, 3),
+                                  h2d_gbs=round(nbytes / copy_ms / 1e6, 2),
+                                  rms_vs_true_drift_px=round(float(np.sqrt(np.mean((sol["translations"] - truth) ** 2))), 4))
+        print(f"[bench hostinput] {mode}: {ms:.2f} ms/movie, copy {copy_ms:.2f} ms ({nbytes / copy_ms / 1e6:.1f} GB/s)",
+              file=sys.stderr, flush=True)
+        for p in plans:
+            p.close()
+        del bufs
+        torch.cuda.empty_cache()
     print(json.dumps(out), flush=True)
 
 
