@@ -100,9 +100,14 @@ typedef struct zr_params {
   /* launch tuning (no reference counterpart beyond FFTW's planning effort and
    * wisdom, zorro/zorro_util.py:966-1066); 0 picks the built-in default:
    * frames per forward-spectra launch pair (default 8) and lag-window columns
-   * per wave of the pair transform (default 8).  Results do not depend on them. */
+   * per wave of the pair transform's column-group walk (default 8), and
+   * pair-list entries per wave of its pair walk (one column per workgroup over
+   * 4 * k2_pairs_per_wave pair-list entries, default 16).  The pair walk runs
+   * unless k2_cols_per_wave > 0 and k2_pairs_per_wave == 0.  Results do not
+   * depend on them. */
   int k1_batch;
   int k2_cols_per_wave;
+  int k2_pairs_per_wave;
 } zr_params;
 
 /* filterMode steps (zorro/zorro.py:1692-1711).  DOSE: doseFilter(normalize=False),

@@ -43,7 +43,7 @@ class ZrParams(C.Structure):
         ("hot_cutoff_upper", C.c_double), ("hot_fixed_cutoffs", C.c_int), ("hot_neighbour_thres", C.c_double), ("hot_mu_sigma", C.c_double * 2),
         ("pre_shift", C.c_int),
         ("n_filters", C.c_int), ("filters", C.c_int * 4),
-        ("k1_batch", C.c_int), ("k2_cols_per_wave", C.c_int),
+        ("k1_batch", C.c_int), ("k2_cols_per_wave", C.c_int), ("k2_pairs_per_wave", C.c_int),
     ]
 
 

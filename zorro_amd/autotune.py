@@ -5,13 +5,16 @@ pickle (``zorro/zorro_util.py:966-1066``, ``fftw_wisdom.pkl``).  The GPU path
 has no FFT planner to call, but a few launch shapes of its own kernels matter
 per detector shape: the number of frames per forward-spectra launch pair
 (``k1_batch``: the row pass's intermediate of a small batch stays in the
-256 MB last-level cache) and the lag-window columns each wave of the pair
+256 MB last-level cache), the lag-window columns each wave of the pair
 transform walks (``k2_cols_per_wave``: L2 reuse of a column group against the
-number of workgroups).  ``tune`` times whole movies for candidate values
-(coordinate descent: the batch first, then the columns), and the winners are
-kept per (frames, shape, padding, crop, maxShift, masked, dose, device) in a
-JSON wisdom file, ``tuning.json`` beside this module by default.  Results do
-not depend on the knobs; only the time does.
+number of workgroups) and the pair-list entries each wave walks in the pair
+transform's pair walk (``k2_pairs_per_wave``: one column of every frame per
+XCD at a time, against the per-workgroup twiddle-table load).  ``tune`` times
+whole movies for candidate values (coordinate descent: the batch, then the
+columns, then the pairs), and the winners are kept per (frames, shape,
+padding, crop, maxShift, masked, dose, device) in a JSON wisdom file,
+``tuning.json`` beside this module by default.  Results do not depend on the
+knobs; only the time does.
 
     python -m zorro_amd.autotune --frames 40 --shape 4096 4096 --crop 3072 3072
 """
@@ -28,7 +31,8 @@ from .engine import MoviePlan, make_params
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 WISDOM = os.path.join(HERE, "tuning.json")
-CANDIDATES = {"k1_batch": [1, 2, 3, 4, 8], "k2_cols_per_wave": [2, 4, 6, 8, 12, 16]}
+CANDIDATES = {"k1_batch": [1, 2, 3, 4, 8], "k2_cols_per_wave": [2, 4, 6, 8, 12, 16],
+              "k2_pairs_per_wave": [8, 16, 32, 48]}
 KNOBS = tuple(CANDIDATES)
 
 

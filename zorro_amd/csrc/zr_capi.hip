@@ -142,7 +142,8 @@ struct zr_movie {
   bool spectral = false, full_spec = false;
   int Wk = 0, Hs = 0, hoff = 0, side_in_col = -1;
   int batch = 1, rb = 4;
-  int k2_cpw = 8;                  // lag-window columns per wave of the pair transform
+  int k2_cpw = 8;                  // lag-window columns per wave of the pair transform (column-group mode)
+  int k2_ppw = 16;                 // pair-list entries per wave (pair mode; 0: column-group mode)
   // frames
   const float* frames = nullptr;
   float* frames_owned = nullptr;
@@ -546,6 +547,9 @@ static int create_movie(int device, const zr_params* p, int n_frames, int h, int
   mv->batch = (int)std::min<size_t>(N, p->k1_batch > 0 ? p->k1_batch : 8);
   if (p->k2_cols_per_wave < 0 || p->k2_cols_per_wave > 64) { free_movie(mv); return fail(ZR_ERR_ARG, "k2_cols_per_wave %d outside 0..64", p->k2_cols_per_wave); }
   mv->k2_cpw = p->k2_cols_per_wave > 0 ? p->k2_cols_per_wave : 8;
+  if (p->k2_pairs_per_wave < 0 || p->k2_pairs_per_wave > 64) { free_movie(mv); return fail(ZR_ERR_ARG, "k2_pairs_per_wave %d outside 0..64", p->k2_pairs_per_wave); }
+  // explicit columns per wave select the column-group walk; else pair mode
+  mv->k2_ppw = p->k2_pairs_per_wave > 0 ? p->k2_pairs_per_wave : (p->k2_cols_per_wave > 0 ? 0 : 16);
   mv->rb = zr_k1_rows_per_block(mv->Wp);
   const int nrb = mv->Hp / mv->rb;
   const bool fourier = p->shift_method == ZR_SHIFT_FOURIER && p->sub_pix_reg > 1;
@@ -888,7 +892,7 @@ static int xcorr_preshift(zr_movie* mv, double nc2) {
     const int cnt = dg.second;
     ZR_HIP(zr_launch_preshift_centres(dp, cnt, mv->pidx, mv->pshift, mv->N, mv->ctr, s));
     if (mv->k2ltw)
-      ZR_HIP(zr_launch_k2_cols_r2(mv->Lper, mv->S, mv->Sside, dp, cnt, mv->Wh, mv->Hc, mv->Wc, mv->m, 8, 1, mv->k2ltw,
+      ZR_HIP(zr_launch_k2_cols_r2(mv->Lper, mv->S, mv->Sside, dp, cnt, mv->Wh, mv->Hc, mv->Wc, mv->m, 8, 0, 1, mv->k2ltw,
                                   mv->Z, mv->Wk, mv->Hs, mv->hoff, mv->side_in_col, mv->ctr, mv->twHc, mv->twWc,
                                   mv->s0, s));
     else
@@ -914,7 +918,7 @@ static int xcorr_pairs(zr_movie* mv, double nc2, bool refine) {
   }
   if (mv->k2ltw)
     ZR_HIP(zr_launch_k2_cols_r2(mv->Lper, mv->S, mv->Sside, mv->work, mv->Ptot, mv->Wh, mv->Hc, mv->Wc, mv->m,
-                                mv->k2_cpw, mv->unmasked ? 1 : 0, mv->k2ltw, mv->Z, mv->Wk, mv->Hs, mv->hoff,
+                                mv->k2_cpw, mv->k2_ppw, mv->unmasked ? 1 : 0, mv->k2ltw, mv->Z, mv->Wk, mv->Hs, mv->hoff,
                                 mv->side_in_col, nullptr, nullptr, nullptr, nullptr, s));
   else
     ZR_HIP(zr_launch_k2_cols(mv->Lper, mv->S, mv->Sside, mv->work, mv->Ptot, mv->Wh, mv->Hc, mv->Wc, mv->m, mv->k2_cpw,
@@ -1354,7 +1358,7 @@ static int create_view(zr_batch* b, int i, const zr_params& p, zr_movie** out) {
   v->Hc = big->Hc; v->Wc = big->Wc; v->Wh = big->Wh; v->m = big->m; v->Lper = big->Lper; v->disc = big->disc;
   v->crop = big->crop; v->unmasked = big->unmasked; v->spectral = big->spectral; v->full_spec = big->full_spec;
   v->Wk = big->Wk; v->Hs = big->Hs; v->hoff = big->hoff; v->side_in_col = big->side_in_col;
-  v->batch = big->batch; v->rb = big->rb; v->k2_cpw = big->k2_cpw;
+  v->batch = big->batch; v->rb = big->rb; v->k2_cpw = big->k2_cpw; v->k2_ppw = big->k2_ppw;
   v->frames_owned = big->frames_owned + (size_t)off * big->H * big->W;
   v->frames = v->frames_owned;
   v->S = big->S + (size_t)off * big->Wk * big->Hs;     // frame f's spectrum is slot f (unmasked and masked)

@@ -212,6 +212,15 @@ constexpr bool zr_smooth(int n) {
 ZR_DEV int zr_pad16(int i) { return i + (i >> 4); }
 constexpr int zr_fft_lds(int L) { return L + L / 16 + 1; }
 
+// Workgroup barrier that orders LDS only: the fences are scoped to the local
+// address space, so global loads still in flight (a prefetch of the next
+// input) and stores are not waited for, as __syncthreads() would.
+ZR_DEV void zr_lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 // One Stockham pass of radix R with stride Ls over an LDS-resident sequence of
 // length L, in place (all reads of the pass complete before any write).
 // tw points at exp(-2*pi*i*t/L), t in [0,L) (forward); DIR=+1 uses conj.
@@ -242,7 +251,7 @@ ZR_DEV void stockham_pass(float2* buf, const float2* __restrict__ tw, int tid) {
       Dft<R, DIR>::run(v[b]);
     }
   }
-  __syncthreads();
+  zr_lds_barrier();
 #pragma unroll
   for (int b = 0; b < PER; ++b) {
     int j = tid + b * NT;
@@ -256,7 +265,7 @@ ZR_DEV void stockham_pass(float2* buf, const float2* __restrict__ tw, int tid) {
       }
     }
   }
-  __syncthreads();
+  zr_lds_barrier();
 }
 
 template <int L, int Ls, int DIR, int NT> struct WgFft {
@@ -274,7 +283,8 @@ template <int L, int DIR, int NT> struct WgFft<L, L, DIR, NT> {
 
 // Whole-workgroup complex FFT of length L held in LDS (natural order in/out).
 // buf holds zr_fft_lds(L) elements (the passes in between use the padding).
-// The caller must __syncthreads() after filling buf; returns synchronised.
+// The caller must synchronise the workgroup after filling buf; returns
+// synchronised (for LDS: its barriers do not wait for global memory).
 template <int L, int DIR, int NT>
 ZR_DEV void wg_fft(float2* buf, const float2* __restrict__ tw, int tid) {
   static_assert(zr_smooth(L), "FFT length must be 2^a 3^b 5^c");
@@ -295,6 +305,19 @@ ZR_DEV int xcd_remap(int id, int nblocks) {
 ZR_DEV double wave_sum_d(double v) {
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
   return v;
+}
+// x + x[lane ^ 16] and x + x[lane ^ 32] in every lane, through the gfx950
+// row-swap permutes (VALU ops: no LDS round trip as with a bpermute).  The
+// swap leaves a lane's own value in one result and its partner's in the other,
+// so the sum is own + partner (an IEEE add is commutative: the same bits as
+// x + shfl_xor(x)).
+ZR_DEV float xor16_sum(float x) {
+  const auto r = __builtin_amdgcn_permlane16_swap(__float_as_uint(x), __float_as_uint(x), false, false);
+  return __uint_as_float(r[0]) + __uint_as_float(r[1]);
+}
+ZR_DEV float xor32_sum(float x) {
+  const auto r = __builtin_amdgcn_permlane32_swap(__float_as_uint(x), __float_as_uint(x), false, false);
+  return __uint_as_float(r[0]) + __uint_as_float(r[1]);
 }
 ZR_DEV float wave_sum_f(float v) {
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);

@@ -64,10 +64,13 @@ hipError_t zr_launch_k2_cols(int Lper, const float2* S, const float2* Sside, con
 // pre-twiddled values with the [L+1][64] table of zr_launch_k2_lane_twiddles
 bool zr_k2_cols_r2_ok(int Lper, int m);
 hipError_t zr_launch_k2_lane_twiddles(int Lper, int Hc, float2* ltw, hipStream_t st);
+// pairs_per_wave > 0 (and ctr == NULL): pair mode, one column per block over
+// 4 * pairs_per_wave consecutive pair-list entries; else column groups of
+// 4 * cols_per_wave columns per pair.
 hipError_t zr_launch_k2_cols_r2(int Lper, const float2* S, const float2* Sside, const int4* work, int npairs, int Wh,
-                                int Hc, int Wc, int m, int cols_per_wave, int dc_zero, const float2* ltw, float2* Z,
-                                int Wk, int Hs, int hoff, int side_in_col, const int2* ctr, const float2* twHc,
-                                const float2* twWc, float* s0, hipStream_t st);
+                                int Hc, int Wc, int m, int cols_per_wave, int pairs_per_wave, int dc_zero,
+                                const float2* ltw, float2* Z, int Wk, int Hs, int hoff, int side_in_col,
+                                const int2* ctr, const float2* twHc, const float2* twWc, float* s0, hipStream_t st);
 // btab (non-NULL, 2m a multiple of 16): the matrix-core pass with the table of
 // zr_launch_k2_twiddles; otherwise the vector-ALU pass
 // sub (nullable): run the npairs pairs sub[0..npairs) (window index .z) instead of 0..npairs

@@ -575,7 +575,8 @@ constexpr size_t k2r2_lds_bytes(int L) {
 template <int L, int WPB, bool PS>
 __global__ __launch_bounds__(64 * WPB, K2R2_OCC) void k2_cols_r2(const float2* __restrict__ S, const float2* __restrict__ Sside,
                                                            const int4* __restrict__ work, int nwork_pairs, int Wh,
-                                                           int Hc, int Wc, int m, int cols_per_wave, int dc_zero,
+                                                           int Hc, int Wc, int m, int cols_per_wave, int pairs_per_wave,
+                                                           int dc_zero,
                                                            const float2* __restrict__ ltw, float2* __restrict__ Z,
                                                            int Wk, int Hs, int hoff, int side_in_col,
                                                            const int2* __restrict__ ctr,
@@ -591,11 +592,19 @@ __global__ __launch_bounds__(64 * WPB, K2R2_OCC) void k2_cols_r2(const float2* _
   float2* my = k2_lds + (size_t)w * ROWS * LP;
   float2* tab = k2_lds + (size_t)WPB * ROWS * LP;                    // [L+1][64]
   const int blk = xcd_remap(blockIdx.x, gridDim.x);
-  const int kgroup = blk / nwork_pairs;
-  const int4 wk = work[blk % nwork_pairs];
-  const int I = wk.x, J = wk.y, p = wk.z;
+  // Column-group mode (pairs_per_wave == 0): block = (group of WPB *
+  // cols_per_wave columns, one pair), each wave walks its columns.  Pair mode
+  // (pairs_per_wave > 0, not with PS): block = (one column, WPB * pairs_per_wave
+  // consecutive entries of the pair list), each wave walks pairs w, w + WPB, ...
+  // so the blocks resident on an XCD at one time read one column of every frame
+  // (40 frames x 24 KiB at Hc = 3072) and the spectra are fetched from HBM about
+  // once.
+  const bool pmode = !PS && pairs_per_wave > 0;
+  const int npc = pmode ? (nwork_pairs + WPB * pairs_per_wave - 1) / (WPB * pairs_per_wave) : 1;
+  const int kgroup = pmode ? blk / npc : blk / nwork_pairs;
+  const int4 wk = work[pmode ? 0 : blk % nwork_pairs];
   const int twoM = 2 * m;
-  const int2 cc = PS ? ctr[p] : make_int2(0, 0);
+  const int2 cc = PS ? ctr[wk.z] : make_int2(0, 0);
   auto j1of = [&](int r) -> int {           // largest j with r + L j <= cy + m - 1
     const int t = cc.x + m - 1 - r;
     return t >= 0 ? t / L : -((-t + L - 1) / L);
@@ -618,9 +627,20 @@ __global__ __launch_bounds__(64 * WPB, K2R2_OCC) void k2_cols_r2(const float2* _
   const int slot = lane & (ROWS - 1), part = lane / ROWS;
   static_assert(ROWS * 4 == 64, "four readers per row");
   const int kbase = kgroup * WPB * cols_per_wave;
-  for (int c = 0; c < cols_per_wave; ++c) {
-    const int kx = kbase + c * WPB + w;
-    if (kx >= Wh) break;
+  const int pbase = (blk % npc) * WPB * pairs_per_wave;
+  const int nsteps = pmode ? pairs_per_wave : cols_per_wave;
+  for (int c = 0; c < nsteps; ++c) {
+    int kx, I, J, p;
+    if (pmode) {
+      const int e = pbase + c * WPB + w;
+      if (e >= nwork_pairs) break;
+      const int4 we = work[e];
+      kx = kgroup; I = we.x; J = we.y; p = we.z;
+    } else {
+      kx = kbase + c * WPB + w;
+      if (kx >= Wh) break;
+      I = wk.x; J = wk.y; p = wk.z;
+    }
     const float2* sI = S + ((size_t)I * Wk + kx) * Hs;
     const float2* sJ = S + ((size_t)J * Wk + kx) * Hs;
     const float cw = (kx == 0 || 2 * kx == Wc) ? 1.f : 2.f;
@@ -695,10 +715,8 @@ __global__ __launch_bounds__(64 * WPB, K2R2_OCC) void k2_cols_r2(const float2* _
 #pragma unroll
         for (int i = 0; i < 16; ++i) acc = c_add(acc, row[i]);
       }
-      acc.x += __shfl_xor(acc.x, 16, 64);
-      acc.y += __shfl_xor(acc.y, 16, 64);
-      acc.x += __shfl_xor(acc.x, 32, 64);
-      acc.y += __shfl_xor(acc.y, 32, 64);
+      acc.x = xor32_sum(xor16_sum(acc.x));
+      acc.y = xor32_sum(xor16_sum(acc.y));
       if (valid && part == 0) zdst[dy - cc.x + m] = PS ? c_mul(acc, xmod) : c_scale(acc, cw);
       __builtin_amdgcn_wave_barrier();
       __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
@@ -1379,11 +1397,14 @@ hipError_t zr_launch_k1_rows(int Wp, const float* frames, int H, int W, int Hp, 
 }
 
 constexpr bool K1_COLS_PAIRED = false;  // column pairs per block (else one column per block, the faster)
+// threads per column of k1_cols1 (timed on MI355X for 8 frames: 128 is the
+// fastest at heights up to 2048 and at 4096, 256 at 3072 and 3840, 512 at 5760)
+constexpr int k1_cols_threads(int hp) { return hp == 5760 ? 512 : ((hp == 3072 || hp == 3840) ? 256 : 128); }
 hipError_t zr_launch_k1_cols(int Hp, const float2* T, int Wh, int Hc, int frame0, int nb, const float2* twH,
                              float2* S, float2* Sside, int full, hipStream_t st) {
   if (!K1_COLS_PAIRED) {
     switch (Hp) {
-#define ZR_CASE(X) case X: hipLaunchKernelGGL((k1_cols1<X, (X >= 2048 ? 512 : 256)>), dim3(nb * Wh), dim3(X >= 2048 ? 512 : 256), 0, st, T, Wh, Hc, frame0, twH, S, Sside, full); return hipGetLastError();
+#define ZR_CASE(X) case X: hipLaunchKernelGGL((k1_cols1<X, k1_cols_threads(X)>), dim3(nb * Wh), dim3(k1_cols_threads(X)), 0, st, T, Wh, Hc, frame0, twH, S, Sside, full); return hipGetLastError();
       ZR_CASE(256) ZR_CASE(512) ZR_CASE(1024) ZR_CASE(2048) ZR_CASE(3072) ZR_CASE(3840) ZR_CASE(4096) ZR_CASE(5760)
 #undef ZR_CASE
       default: return hipErrorInvalidValue;
@@ -1448,20 +1469,27 @@ hipError_t zr_launch_k2_lane_twiddles(int Lper, int Hc, float2* ltw, hipStream_t
 }
 
 hipError_t zr_launch_k2_cols_r2(int Lper, const float2* S, const float2* Sside, const int4* work, int npairs, int Wh,
-                                int Hc, int Wc, int m, int cols_per_wave, int dc_zero, const float2* ltw, float2* Z,
-                                int Wk, int Hs, int hoff, int side_in_col, const int2* ctr, const float2* twHc,
-                                const float2* twWc, float* s0, hipStream_t st) {
-  if (!zr_k2_cols_r2_ok(Lper, m) || Hc != 64 * Lper) return hipErrorInvalidValue;
-  const int cols_per_block = K2R2_WPB * cols_per_wave;
-  const int ngroups = (Wh + cols_per_block - 1) / cols_per_block;
-  const int grid = ngroups * npairs;
+                                int Hc, int Wc, int m, int cols_per_wave, int pairs_per_wave, int dc_zero,
+                                const float2* ltw, float2* Z, int Wk, int Hs, int hoff, int side_in_col,
+                                const int2* ctr, const float2* twHc, const float2* twWc, float* s0, hipStream_t st) {
+  if (!zr_k2_cols_r2_ok(Lper, m) || Hc != 64 * Lper || npairs < 1) return hipErrorInvalidValue;
+  if (ctr) pairs_per_wave = 0;                    // the window centres are per pair: column-group mode
+  if (pairs_per_wave <= 0 && cols_per_wave <= 0) return hipErrorInvalidValue;
+  int grid;
+  if (pairs_per_wave > 0) {
+    const int per_block = K2R2_WPB * pairs_per_wave;
+    grid = Wh * ((npairs + per_block - 1) / per_block);
+  } else {
+    const int cols_per_block = K2R2_WPB * cols_per_wave;
+    grid = ((Wh + cols_per_block - 1) / cols_per_block) * npairs;
+  }
   const size_t sh = k2r2_lds_bytes(Lper);
   switch (Lper) {
 #define ZR_CASE(X) case X: { static std::once_flag lds_once, lds_once_ps; \
       if (ctr) { zr_allow_full_lds((const void*)k2_cols_r2<X, K2R2_WPB, true>, lds_once_ps); \
-        hipLaunchKernelGGL((k2_cols_r2<X, K2R2_WPB, true>), dim3(grid), dim3(64 * K2R2_WPB), sh, st, S, Sside, work, npairs, Wh, Hc, Wc, m, cols_per_wave, dc_zero, ltw, Z, Wk, Hs, hoff, side_in_col, ctr, twHc, twWc, s0); } \
+        hipLaunchKernelGGL((k2_cols_r2<X, K2R2_WPB, true>), dim3(grid), dim3(64 * K2R2_WPB), sh, st, S, Sside, work, npairs, Wh, Hc, Wc, m, cols_per_wave, pairs_per_wave, dc_zero, ltw, Z, Wk, Hs, hoff, side_in_col, ctr, twHc, twWc, s0); } \
       else { zr_allow_full_lds((const void*)k2_cols_r2<X, K2R2_WPB, false>, lds_once); \
-        hipLaunchKernelGGL((k2_cols_r2<X, K2R2_WPB, false>), dim3(grid), dim3(64 * K2R2_WPB), sh, st, S, Sside, work, npairs, Wh, Hc, Wc, m, cols_per_wave, dc_zero, ltw, Z, Wk, Hs, hoff, side_in_col, ctr, twHc, twWc, s0); } \
+        hipLaunchKernelGGL((k2_cols_r2<X, K2R2_WPB, false>), dim3(grid), dim3(64 * K2R2_WPB), sh, st, S, Sside, work, npairs, Wh, Hc, Wc, m, cols_per_wave, pairs_per_wave, dc_zero, ltw, Z, Wk, Hs, hoff, side_in_col, ctr, twHc, twWc, s0); } \
       return hipGetLastError(); }
     ZR_CASE(32) ZR_CASE(48) ZR_CASE(64)
 #undef ZR_CASE

@@ -61,13 +61,13 @@ def make_params(triMode="diag", diagWidth=5, diagStart=0, maxShift=100, Bmode="o
                 shapePadded=(4096, 4096), filterMode=None, doseFiltParam=(None, 0.24499, -1.6649, 2.8141, 32, 0),
                 pixelsize=None, voltage=300.0, doLazyFRC=True, frc_box=512, frc_overlap=0.5,
                 autoMax=10, shapeBinned=None, hotpixInfo=None, neighbourThres=0.01,
-                hotMuSigma=None, hotCutoffs=None, preShift=False, k1_batch=0, k2_cols_per_wave=0) -> L.ZrParams:
+                hotMuSigma=None, hotCutoffs=None, preShift=False, k1_batch=0, k2_cols_per_wave=0, k2_pairs_per_wave=0) -> L.ZrParams:
     """ZrParams from ImageRegistrator attribute values (defaults of zorro/zorro.py:82-243).
     ``hotpixInfo`` (defaults of zorro/zorro.py:139-141) and ``neighbourThres``
     configure the 'hot' filter mode; ``hotMuSigma`` fixes the normal-CDF
     parameters that the filter otherwise fits; ``hotCutoffs`` = (lower, upper)
     skips the cut-off fits (``hotpixFilter(cutoffLower, cutoffUpper)``).
-    ``k1_batch`` / ``k2_cols_per_wave``: launch tuning (0: built-in default;
+    ``k1_batch`` / ``k2_cols_per_wave`` / ``k2_pairs_per_wave``: launch tuning (0: built-in default;
     ``zorro_amd.autotune`` picks them per shape); results do not depend on them."""
     p = L.ZrParams()
     if triMode not in TRI:
@@ -123,6 +123,7 @@ def make_params(triMode="diag", diagWidth=5, diagStart=0, maxShift=100, Bmode="o
     p.pre_shift = 1 if preShift else 0
     p.k1_batch = int(k1_batch or 0)
     p.k2_cols_per_wave = int(k2_cols_per_wave or 0)
+    p.k2_pairs_per_wave = int(k2_pairs_per_wave or 0)
     if hotCutoffs is not None:
         p.hot_cutoff_lower, p.hot_cutoff_upper = (float(v) for v in hotCutoffs)
         p.hot_fixed_cutoffs = 1
