@@ -582,10 +582,11 @@ __global__ __launch_bounds__(64 * WPB, K2R2_OCC) void k2_cols_r2(const float2* _
                                                            const int2* __restrict__ ctr,
                                                            const float2* __restrict__ twHc,
                                                            const float2* __restrict__ twWc, float* __restrict__ s0) {
-  static_assert(L % K2R2_RC == 0, "residues must split into whole chunks");
+  constexpr int RC = K2R2_RC;             // residues per LDS chunk
   constexpr int LP = K2R2_LP;             // padded LDS row: rows 2 dwords apart in the banks
-  constexpr int NCH = L / K2R2_RC;
-  constexpr int ROWS = 2 * K2R2_RC;
+  constexpr int ROWS = 2 * RC;
+  constexpr int PLEN = 16;                // terms each of a row's four readers adds
+  static_assert(ROWS * 4 == 64, "four readers per row");
   extern __shared__ float2 k2_lds[];
   const int lane = threadIdx.x & 63;
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);   // wave index (uniform)
@@ -623,9 +624,8 @@ __global__ __launch_bounds__(64 * WPB, K2R2_OCC) void k2_cols_r2(const float2* _
   }
   __syncthreads();
   const float2 wlane = tab[L * 64 + lane];    // e^{-2 pi i lane / 64}
-  // reader role: row slot (residue r0 + slot/2, kind slot%2), quarter of the lanes
+  // reader role: row slot (residue r0 + slot/2, kind slot%2), part of the lanes
   const int slot = lane & (ROWS - 1), part = lane / ROWS;
-  static_assert(ROWS * 4 == 64, "four readers per row");
   const int kbase = kgroup * WPB * cols_per_wave;
   const int pbase = (blk % npc) * WPB * pairs_per_wave;
   const int nsteps = pmode ? pairs_per_wave : cols_per_wave;
@@ -668,6 +668,7 @@ __global__ __launch_bounds__(64 * WPB, K2R2_OCC) void k2_cols_r2(const float2* _
       __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
       for (int b = 0; b < B1; ++b) v[b] = c_cjmul(v[b], t[b]);
+      __builtin_amdgcn_sched_barrier(0);   // the second batch's loads after these products
     }
     if constexpr (L > B1) {
       float2 t[L - B1];
@@ -683,43 +684,75 @@ __global__ __launch_bounds__(64 * WPB, K2R2_OCC) void k2_cols_r2(const float2* _
       v[0] = dc ? make_float2(0.f, 0.f) : v[0];
       v[L / 2] = l0 ? nv : v[L / 2];
     }
-    Dft<L, 1>::run(v);
-    float2 xmod = make_float2(cw, 0.f);
-    if constexpr (PS) {
-      const float sr = wave_sum_f(v[0].x);    // lag-0 column sum: sum over lanes of Y_a[0]
-      if (lane == 0) s0[(size_t)p * Wh + kx] = sr * cw;
-      const float2 t = twWc[(int)((((long)kx * cc.y) % Wc + Wc) % Wc)];
-      xmod = make_float2(t.x * cw, -t.y * cw);
+    // The length-L DFT as L/16-point DFTs then L/16 sixteen-point DFTs
+    // (DftCT<L/16, 16>: input n = 16 n1 + n2, output r = k1 + (L/16) k2), with
+    // each sixteen-point group's residues twiddled and lane-summed as soon as
+    // it is done, so that at most one group's outputs are live beside the
+    // remaining inputs.
+    constexpr int R1 = L / 16;
+    static_assert(16 % RC == 0, "a group's sixteen residues split into whole chunks");
+    float2 t[L];
+#pragma unroll
+    for (int n2 = 0; n2 < 16; ++n2) {
+      float2 u[R1];
+#pragma unroll
+      for (int n1 = 0; n1 < R1; ++n1) u[n1] = v[16 * n1 + n2];
+      Dft<R1, 1>::run(u);
+#pragma unroll
+      for (int k1 = 0; k1 < R1; ++k1) t[n2 * R1 + k1] = u[k1];
     }
+#pragma unroll
+    for (int n2 = 1; n2 < 16; ++n2)
+#pragma unroll
+      for (int k1 = 1; k1 < R1; ++k1) t[n2 * R1 + k1] = c_mul(t[n2 * R1 + k1], DftCT<R1, 16, 1>::tw_ct(n2 * k1));
+    float2 xmod = make_float2(cw, 0.f);
     float2* zdst = Z + ((size_t)p * Wh + kx) * twoM;
 #pragma unroll
-    for (int ch = 0; ch < NCH; ++ch) {
+    for (int k1 = 0; k1 < R1; ++k1) {
+      __builtin_amdgcn_sched_barrier(0);
+      float2 y[16];                            // residues k1 + R1 k2, k2 = 0 .. 15
 #pragma unroll
-      for (int i = 0; i < K2R2_RC; ++i) {
-        const int r = ch * K2R2_RC + i;
-        // rows of outputs outside [c - m, c + m) are written too and never read
-        const float2 u = c_mul(v[r], tab[r * 64 + lane]);
-        my[(2 * i) * LP + lane] = u;                     // dy = r
-        my[(2 * i + 1) * LP + lane] = c_mul(u, wlane);   // dy = r - L
+      for (int n2 = 0; n2 < 16; ++n2) y[n2] = t[n2 * R1 + k1];
+      Dft<16, 1>::run(y);
+      if constexpr (PS) {
+        if (k1 == 0) {
+          const float sr = wave_sum_f(y[0].x);    // lag-0 column sum: sum over lanes of Y_a[0]
+          if (lane == 0) s0[(size_t)p * Wh + kx] = sr * cw;
+          const float2 tt = twWc[(int)((((long)kx * cc.y) % Wc + Wc) % Wc)];
+          xmod = make_float2(tt.x * cw, -tt.y * cw);
+        }
       }
-      __builtin_amdgcn_wave_barrier();
-      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-      const int r = ch * K2R2_RC + (slot >> 1);
-      const int kind = slot & 1;
-      // lag of this slot: kind 0 the larger of the residue's two, kind 1 that minus L
-      const int dy = (PS ? r + L * j1of(r) : r) - kind * L;
-      const bool valid = dy >= cc.x - m && dy <= cc.x + m - 1;
-      float2 acc = make_float2(0.f, 0.f);
-      {
-        const float2* row = my + slot * LP + 16 * part;
 #pragma unroll
-        for (int i = 0; i < 16; ++i) acc = c_add(acc, row[i]);
+      for (int c = 0; c < 16 / RC; ++c) {
+        // one chunk's twiddle reads at a time
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int i = 0; i < RC; ++i) {
+          const int r = k1 + R1 * (c * RC + i);
+          // rows of outputs outside [c - m, c + m) are written too and never read
+          const float2 u = c_mul(y[c * RC + i], tab[r * 64 + lane]);
+          my[(2 * i) * LP + lane] = u;                     // dy = r
+          my[(2 * i + 1) * LP + lane] = c_mul(u, wlane);   // dy = r - L
+        }
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        const int r = k1 + R1 * (c * RC + (slot >> 1));
+        const int kind = slot & 1;
+        // lag of this slot: kind 0 the larger of the residue's two, kind 1 that minus L
+        const int dy = (PS ? r + L * j1of(r) : r) - kind * L;
+        const bool valid = dy >= cc.x - m && dy <= cc.x + m - 1;
+        float2 acc = make_float2(0.f, 0.f);
+        {
+          const float2* row = my + slot * LP + PLEN * part;
+#pragma unroll
+          for (int i = 0; i < PLEN; ++i) acc = c_add(acc, row[i]);
+        }
+        acc.x = xor32_sum(xor16_sum(acc.x));
+        acc.y = xor32_sum(xor16_sum(acc.y));
+        if (valid && part == 0) zdst[dy - cc.x + m] = PS ? c_mul(acc, xmod) : c_scale(acc, cw);
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
       }
-      acc.x = xor32_sum(xor16_sum(acc.x));
-      acc.y = xor32_sum(xor16_sum(acc.y));
-      if (valid && part == 0) zdst[dy - cc.x + m] = PS ? c_mul(acc, xmod) : c_scale(acc, cw);
-      __builtin_amdgcn_wave_barrier();
-      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     }
   }
 }
