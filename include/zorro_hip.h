@@ -108,6 +108,10 @@ typedef struct zr_params {
   int k1_batch;
   int k2_cols_per_wave;
   int k2_pairs_per_wave;
+  /* user_mask = 1: the correlation runs under the frame region times one
+   * binary mask [H][W] given by zr_movie_set_mask (ImageRegistrator.masks of
+   * shape [1,H,W], zorro/zorro.py:584-589, padded by padStack :1942-1955). */
+  int user_mask;
 } zr_params;
 
 /* filterMode steps (zorro/zorro.py:1692-1711).  DOSE: doseFilter(normalize=False),
@@ -151,6 +155,14 @@ int zr_movie_set_stream(zr_movie* mv, void* hip_stream);
  * caller keeps alive (no copy); on_device=0: host memory, copied.
  * Replaces the assignment of ImageRegistrator.images (zorro/zorro.py:116). */
 int zr_movie_set_frames(zr_movie* mv, const float* frames, int on_device);
+
+/* The user mask of a plan created with user_mask = 1: H x W float32 values, 0
+ * or 1 (host masks are checked), for every frame; replaces the reference's
+ * masks [1,H,W] (zorro/zorro.py:584-589, used by mnxc2 :1014-1097).  Returns
+ * after the copy.  A mask per frame (masks.shape[0] > 1) has no counterpart:
+ * the reference's own path raises (xcorrnm2_tri reads an unbound local
+ * baseMaskFFT at :970, mnxc2 an undefined baseMaskFFT_block at :1048). */
+int zr_movie_set_mask(zr_movie* mv, const float* mask, int on_device);
 
 /* Provide raw frames [N][h][w] (h, w as given to zr_movie_create) of type
  * dtype, optionally times a gain reference f32 [h][w] (already rotated/flipped

@@ -44,6 +44,7 @@ class ZrParams(C.Structure):
         ("pre_shift", C.c_int),
         ("n_filters", C.c_int), ("filters", C.c_int * 4),
         ("k1_batch", C.c_int), ("k2_cols_per_wave", C.c_int), ("k2_pairs_per_wave", C.c_int),
+        ("user_mask", C.c_int),
     ]
 
 
@@ -68,6 +69,7 @@ SIGNATURES = {
     "zr_movie_destroy": (None, [_P]),
     "zr_movie_set_stream": (C.c_int, [_P, _P]),
     "zr_movie_set_frames": (C.c_int, [_P, _P, C.c_int]),
+    "zr_movie_set_mask": (C.c_int, [_P, _P, C.c_int]),
     "zr_movie_set_frames_raw": (C.c_int, [_P, _P, C.c_int, C.c_int, _P, C.c_int, C.c_double]),
     "zr_xcorr": (C.c_int, [_P]),
     "zr_solve": (C.c_int, [_P]),

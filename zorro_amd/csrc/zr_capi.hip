@@ -205,6 +205,8 @@ struct zr_movie {
   bool xcorr_only = false;
   bool view = false;
   bool fourier_aligned = false;          // shiftMethod 'fourier': aligned frames formed since the last solve
+  float* umask = nullptr;                // user mask [H][W] (params user_mask), 0 or 1
+  bool umask_set = false;
   hipStream_t batch_stream = nullptr;   // a view's results are complete once this stream has passed them
 };
 
@@ -248,7 +250,7 @@ static void free_movie(zr_movie* mv) {
                   mv->windows, mv->shifts, mv->corr, mv->peaksig, mv->origin, mv->bsigma, mv->twMrow, mv->twMcol,
                   mv->twMout, mv->chW_c, mv->chW_B, mv->chH_c, mv->chH_B, mv->chHb_c, mv->chHb_B, mv->chWb_c,
                   mv->chWb_B, mv->binA, mv->binB, mv->raw_owned, mv->gain_owned, mv->hot_block, mv->dpairs,
-                  mv->pidx, mv->ctr, mv->pshift, mv->s0, mv->pflag};
+                  mv->pidx, mv->ctr, mv->pshift, mv->s0, mv->pflag, mv->umask};
   for (void* q : ptrs)
     if (q) hipFree(q);
   zr_stage_free(mv->st);
@@ -433,6 +435,13 @@ static int create_movie(int device, const zr_params* p, int n_frames, int h, int
   // padStack (zorro/zorro.py:1908-1958): frames smaller than shapePadded are
   // zero-padded and correlated under the mask of the frame region
   mv->unmasked = (mv->Hp == h && mv->Wp == w);
+  if (p->user_mask) {
+    // one user mask [H][W] for every frame (masks.shape[0] == 1): the masked
+    // correlation under the frame region times that mask
+    if (mv->binned) { free_movie(mv); return fail(ZR_ERR_UNSUPPORTED, "a user mask with shapeBinned is not built"); }
+    if (p->shift_method == ZR_SHIFT_FOURIER) { free_movie(mv); return fail(ZR_ERR_UNSUPPORTED, "a user mask with shiftMethod 'fourier' is not built"); }
+    mv->unmasked = false;
+  }
   mv->crop = (p->fou_crop[0] != 0 || p->fou_crop[1] != 0);
   if (mv->crop) {
     mv->Hc = 2 * (p->fou_crop[0] / 2);
@@ -572,7 +581,8 @@ static int create_movie(int device, const zr_params* p, int n_frames, int h, int
       (rc = dev_alloc(&mv->den, (size_t)mv->Ptot)) ||
       (rc = dev_alloc(&mv->windows, (size_t)mv->nwin * 4 * mv->m * mv->m)) || (rc = dev_alloc(&mv->shifts, N * N * 2)) ||
       (rc = dev_alloc(&mv->corr, N * N)) || (rc = dev_alloc(&mv->peaksig, N * N)) || (rc = dev_alloc(&mv->origin, N * N)) ||
-      (rc = dev_alloc(&mv->bsigma, N * N))) {
+      (rc = dev_alloc(&mv->bsigma, N * N)) ||
+      (rc = dev_alloc(&mv->umask, p->user_mask ? (size_t)mv->H * mv->W : 0))) {
     free_movie(mv);
     return rc;
   }
@@ -801,6 +811,27 @@ extern "C" int zr_movie_set_frames(zr_movie* mv, const float* frames, int on_dev
   return ZR_OK;
 }
 
+extern "C" int zr_movie_set_mask(zr_movie* mv, const float* mask, int on_device) {
+  if (!mv || !mask) return fail(ZR_ERR_ARG, "NULL argument");
+  if (!mv->p.user_mask || !mv->umask) return fail(ZR_ERR_STATE, "the plan was created without user_mask");
+  ZR_HIP(hipSetDevice(mv->device));
+  const size_t n = (size_t)mv->H * mv->W;
+  if (!on_device) {
+    size_t ones = 0;
+    for (size_t i = 0; i < n; ++i) {
+      if (mask[i] != 0.f && mask[i] != 1.f) return fail(ZR_ERR_ARG, "mask values must be 0 or 1 (element %zu is %g)", i, (double)mask[i]);
+      ones += mask[i] != 0.f;
+    }
+    if (ones == 0) return fail(ZR_ERR_ARG, "the mask has no ones");
+  }
+  ZR_HIP(hipMemcpyAsync(mv->umask, mask, n * sizeof(float), on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice,
+                        mv->stream));
+  if (!on_device) ZR_HIP(hipStreamSynchronize(mv->stream));   // the caller may reuse its buffer on return
+  mv->umask_set = true;
+  mv->have_xcorr = false;
+  return ZR_OK;
+}
+
 static int xcorr_pairs(zr_movie* mv, double nc2, bool refine);
 static int xcorr_peaks(zr_movie* mv);
 
@@ -808,6 +839,8 @@ static int xcorr_peaks(zr_movie* mv);
 // movie captures the first two as their own graphs so that the pair transform
 // can be timed by events recorded between graph launches.
 static int xcorr_part(zr_movie* mv, int part) {
+  if (mv->p.user_mask && !mv->umask_set)
+    return fail(ZR_ERR_STATE, "the plan takes a user mask: call zr_movie_set_mask before aligning");
   hipStream_t s = mv->stream;
   int rb = 0;
   const double nc2 = 1.0 / std::pow((double)mv->Hc * (double)mv->Wc, 2.0);
@@ -829,7 +862,7 @@ static int xcorr_part(zr_movie* mv, int part) {
     for (int f0 = 0; f0 < mv->N; f0 += mv->batch) {
       int nb = std::min(mv->batch, mv->N - f0);
       ZR_HIP(zr_launch_k1_rows(mv->Wp, mv->frames, mv->H, mv->W, mv->Hp, mv->Wk, f0, nb, mv->twWp2, mv->twWp, mv->T,
-                               mv->part, 0, nullptr, &rb, s));
+                               mv->part, 0, nullptr, nullptr, &rb, s));
       ZR_HIP(zr_launch_k1_cols(mv->Hp, mv->T, mv->Wk, mv->Hc, f0, nb, mv->twHp, mv->S, mv->Sside, full, s));
     }
     ZR_HIP(zr_launch_k1_stats(mv->part, mv->Hp / rb, mv->N, mv->stats, s));
@@ -856,13 +889,13 @@ static int xcorr_part(zr_movie* mv, int part) {
       int nb = std::min(mv->batch, mv->N - f0);
       for (int sq = 0; sq < 2; ++sq) {
         ZR_HIP(zr_launch_k1_rows(mv->Wp, mv->frames, mv->H, mv->W, mv->Hp, mv->Wk, f0, nb, mv->twWp2, mv->twWp,
-                                 mv->T, mv->part, sq ? 2 : 1, mu, &rb, s));
+                                 mv->T, mv->part, sq ? 2 : 1, mu, mv->umask, &rb, s));
         ZR_HIP(zr_launch_k1_cols(mv->Hp, mv->T, mv->Wk, mv->Hc, f0 + sq * mv->N, nb, mv->twHp, mv->S, mv->Sside,
                                  full, s));
       }
     }
     ZR_HIP(zr_launch_k1_rows(mv->Wp, mv->frames, mv->H, mv->W, mv->Hp, mv->Wk, 0, 1, mv->twWp2, mv->twWp, mv->T,
-                             mv->part, 3, mv->stats, &rb, s));
+                             mv->part, 3, mv->stats, mv->umask, &rb, s));
     ZR_HIP(zr_launch_k1_cols(mv->Hp, mv->T, mv->Wk, mv->Hc, 2 * mv->N, 1, mv->twHp, mv->S, mv->Sside, full, s));
     if (refine) {
       for (int q = 0; q < 2; ++q) {
@@ -1397,6 +1430,7 @@ extern "C" int zr_batch_create(int device, const zr_params* p, int n_movies, con
     return fail(ZR_ERR_UNSUPPORTED, "batches take float32 frames without preShift, 'hot' or binning");
   if (p->shift_method == ZR_SHIFT_FOURIER)
     return fail(ZR_ERR_UNSUPPORTED, "batches shift by Lanczos kernels or integers ('fourier' runs per movie)");
+  if (p->user_mask) return fail(ZR_ERR_UNSUPPORTED, "batches correlate under the frame region (user masks run per movie)");
   zr_batch* b = new zr_batch();
   b->device = device;
   int total = 0;

@@ -40,13 +40,14 @@ inline size_t zr_peaks_lds_bytes(const PeakParamsHost* hp, int* tmp_floats) {
 
 // zr_xcorr.hip
 // mode: 0 frame (sums into part), 1 mean-centred masked frame, 2 its square,
-// 3 the mask (stats: per-frame (sum, sumsq) for modes 1 and 2)
+// 3 the mask (stats: per-frame (sum, sumsq) for modes 1 and 2); umask: the
+// user's binary mask [H][W] multiplying the frame region in modes 1-3, or NULL
 // rows per k1_rows block for a padded width (the frame-sum partials come in
 // Hp / rows blocks per frame)
 int zr_k1_rows_per_block(int Wp);
 hipError_t zr_launch_k1_rows(int Wp, const float* frames, int H, int W, int Hp, int Wh, int frame0, int nb,
                              const float2* twL, const float2* tw2L, float2* T, double* part, int mode,
-                             const double* stats, int* rb_out, hipStream_t st);
+                             const double* stats, const float* umask, int* rb_out, hipStream_t st);
 // per-frame FP64 partial sums over H x W; nrb_out = row blocks per frame
 hipError_t zr_launch_k1_sums(const float* frames, int n, int H, int W, double* part, int* nrb_out, hipStream_t st);
 // masked NCC from the pair and mask-term windows (in place on the P pair windows)

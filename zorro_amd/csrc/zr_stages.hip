@@ -1409,22 +1409,24 @@ __global__ __launch_bounds__(256) void k_dose_params(DoseArgs a) {
     double run = 0.0;
     for (int j = 1; j <= a.N; ++j) { run += dpf; a.accum[j] = run + missing; }
   }
-  __syncthreads();
-  // optimal exposure at radial index k, evaluated where it is used (any Hp/2
-  // length); index 0 takes the square of index 1 (zorro/zorro.py:2422)
+  // optimal exposure at radial index k < Hp/2, one pow per index into the
+  // dynamic LDS table (the launch sizes it for Hp/2 doubles); index 0 takes
+  // the square of index 1 (zorro/zorro.py:2422)
+  extern __shared__ double opt_tab[];
   const int nq = a.Hp / 2;
   const double qs = sqrt(a.ipx * a.ipy);
-  auto opti = [&](int k) -> double {
+  for (int k = tid; k < nq; k += 256) {
     const double o1 = 2.51284 * a.vs * (a.A * pow(qs, a.B) + a.C);
-    return k == 0 ? o1 * o1 : 2.51284 * a.vs * (a.A * pow(k * qs, a.B) + a.C);
-  };
+    opt_tab[k] = k == 0 ? o1 * o1 : 2.51284 * a.vs * (a.A * pow(k * qs, a.B) + a.C);
+  }
+  __syncthreads();
   // one wave per frame: the last radial index closer to the frame's end dose
   const int lane = tid & 63, wid = tid >> 6;
   for (int j = wid; j < a.N; j += 4) {
     const double d1 = a.accum[j + 1], d0 = a.accum[j];
     int last = -1;
     for (int k = lane; k < nq; k += 64) {
-      const double o = opti(k);
+      const double o = opt_tab[k];
       if (fabs(d1 - o) < fabs(d0 - o)) last = k;
     }
     for (int o = 32; o > 0; o >>= 1) last = max(last, __shfl_xor(last, o, 64));
@@ -2351,7 +2353,7 @@ static int dose_spectral(zr_stage_state* s, const float* frames, const float2* S
   s->have_sums = true;
   DoseArgs a;
   fill_dose_args(s, a, s->rowsum, gx * gy, normalize);
-  hipLaunchKernelGGL(k_dose_params, dim3(1), dim3(256), 0, st, a);
+  hipLaunchKernelGGL(k_dose_params, dim3(1), dim3(256), (size_t)(a.Hp / 2) * sizeof(double), st, a);
   ZS_HIP(hipGetLastError());
   if (H != W) return serr(ZR_ERR_UNSUPPORTED, "spectral dose path needs square frames");
   switch (H) {
@@ -2405,7 +2407,7 @@ int zr_stage_dose(zr_stage_state* s, const float* frames, const float2* S, int W
   s->have_sums = true;
   DoseArgs a;
   fill_dose_args(s, a, s->rowsum, gx * gy, normalize);
-  hipLaunchKernelGGL(k_dose_params, dim3(1), dim3(256), 0, st, a);
+  hipLaunchKernelGGL(k_dose_params, dim3(1), dim3(256), (size_t)(a.Hp / 2) * sizeof(double), st, a);
   ZS_HIP(hipGetLastError());
   // the filtered spectra of the aligned, symmetric-padded frames, DOSE_CHUNK frames at a time
   for (int j0 = 0; j0 < s->N; j0 += DOSE_CHUNK) {

@@ -30,7 +30,8 @@
 //   K1_CENTRED m (f - mu_f)           mu_f = stats[2f] / (H W)
 //   K1_SQUARED (m (f - mu_f))^2
 //   K1_MASK    m
-// Subtracting the frame mean inside the mask leaves the masked normalised
+// (m: ones on the H x W frame region, times the user's binary mask umask
+// [H][W] when one is given).  Subtracting the frame mean inside the mask leaves the masked normalised
 // cross-correlation unchanged (m^2 = m makes every mask term shift with the
 // mean in a way that cancels exactly in numerator and denominator) and keeps
 // the float32 spectra free of the large DC term.
@@ -40,7 +41,8 @@ __global__ __launch_bounds__(RB * TPR) void k1_rows(const float* __restrict__ fr
                                                     int frame0, const float2* __restrict__ twL,
                                                     const float2* __restrict__ tw2L, float2* __restrict__ T,
                                                     double* __restrict__ part, int mode,
-                                                    const double* __restrict__ stats) {
+                                                    const double* __restrict__ stats,
+                                                    const float* __restrict__ umask) {
   constexpr int L = WP / 2, NT = RB * TPR, PER = L / TPR;
   static_assert(L % TPR == 0, "row length must split evenly over the row group");
   extern __shared__ float2 k1_lds[];
@@ -56,12 +58,14 @@ __global__ __launch_bounds__(RB * TPR) void k1_rows(const float* __restrict__ fr
   const int y = y0 + g;
   const float* row = frames + ((size_t)f * H + y) * W;
   float2 v[PER];
+  // in-mask test: the frame region, times the user mask where one is given
+  auto inm = [&](int x) -> bool { return y < H && x < W && (!umask || umask[(size_t)y * W + x] != 0.f); };
   if (mode == K1_MASK) {
 #pragma unroll
     for (int i = 0; i < PER; ++i) {
       const int x = 2 * (gt + i * TPR);
-      v[i].x = (y < H && x < W) ? 1.f : 0.f;
-      v[i].y = (y < H && x + 1 < W) ? 1.f : 0.f;
+      v[i].x = inm(x) ? 1.f : 0.f;
+      v[i].y = inm(x + 1) ? 1.f : 0.f;
     }
   } else if (y < H && W == WP) {
     const float2* row2 = reinterpret_cast<const float2*>(row);
@@ -82,8 +86,8 @@ __global__ __launch_bounds__(RB * TPR) void k1_rows(const float* __restrict__ fr
 #pragma unroll
     for (int i = 0; i < PER; ++i) {
       const int x = 2 * (gt + i * TPR);
-      const float a = (y < H && x < W) ? v[i].x - mu : 0.f;
-      const float b = (y < H && x + 1 < W) ? v[i].y - mu : 0.f;
+      const float a = inm(x) ? v[i].x - mu : 0.f;
+      const float b = inm(x + 1) ? v[i].y - mu : 0.f;
       v[i].x = sq ? a * a : a;
       v[i].y = sq ? b * b : b;
     }
@@ -1405,7 +1409,7 @@ constexpr int k1_threads_per_row(int wp) {
 template <int WP>
 static hipError_t launch_k1_rows_t(const float* frames, int H, int W, int Hp, int Wh, int frame0, int nb,
                                    const float2* twL, const float2* tw2L, float2* T, double* part, int mode,
-                                   const double* stats, hipStream_t st) {
+                                   const double* stats, const float* umask, hipStream_t st) {
   constexpr int RB = k1_rows_per_block(WP);
   constexpr int TPR = k1_threads_per_row(WP);
   if (Hp % RB != 0) return hipErrorInvalidValue;
@@ -1414,15 +1418,15 @@ static hipError_t launch_k1_rows_t(const float* frames, int H, int W, int Hp, in
   static std::once_flag lds_once;
   zr_allow_full_lds((const void*)k1_rows<WP, RB, TPR>, lds_once);
   hipLaunchKernelGGL((k1_rows<WP, RB, TPR>), dim3(grid), dim3(RB * TPR), sh, st, frames, H, W, Hp, Wh, frame0, twL,
-                     tw2L, T, part, mode, stats);
+                     tw2L, T, part, mode, stats, mode == 0 ? nullptr : umask);
   return hipGetLastError();
 }
 
 hipError_t zr_launch_k1_rows(int Wp, const float* frames, int H, int W, int Hp, int Wh, int frame0, int nb,
                              const float2* twL, const float2* tw2L, float2* T, double* part, int mode,
-                             const double* stats, int* rb_out, hipStream_t st) {
+                             const double* stats, const float* umask, int* rb_out, hipStream_t st) {
   switch (Wp) {
-#define ZR_CASE(X) case X: *rb_out = k1_rows_per_block(X); return launch_k1_rows_t<X>(frames, H, W, Hp, Wh, frame0, nb, twL, tw2L, T, part, mode, stats, st);
+#define ZR_CASE(X) case X: *rb_out = k1_rows_per_block(X); return launch_k1_rows_t<X>(frames, H, W, Hp, Wh, frame0, nb, twL, tw2L, T, part, mode, stats, umask, st);
     ZR_CASE(256) ZR_CASE(512) ZR_CASE(1024) ZR_CASE(2048) ZR_CASE(3072) ZR_CASE(3840) ZR_CASE(4096) ZR_CASE(5760)
 #undef ZR_CASE
     default: return hipErrorInvalidValue;

@@ -61,12 +61,15 @@ def make_params(triMode="diag", diagWidth=5, diagStart=0, maxShift=100, Bmode="o
                 shapePadded=(4096, 4096), filterMode=None, doseFiltParam=(None, 0.24499, -1.6649, 2.8141, 32, 0),
                 pixelsize=None, voltage=300.0, doLazyFRC=True, frc_box=512, frc_overlap=0.5,
                 autoMax=10, shapeBinned=None, hotpixInfo=None, neighbourThres=0.01,
-                hotMuSigma=None, hotCutoffs=None, preShift=False, k1_batch=0, k2_cols_per_wave=0, k2_pairs_per_wave=0) -> L.ZrParams:
+                hotMuSigma=None, hotCutoffs=None, preShift=False, k1_batch=0, k2_cols_per_wave=0, k2_pairs_per_wave=0,
+                user_mask=False) -> L.ZrParams:
     """ZrParams from ImageRegistrator attribute values (defaults of zorro/zorro.py:82-243).
     ``hotpixInfo`` (defaults of zorro/zorro.py:139-141) and ``neighbourThres``
     configure the 'hot' filter mode; ``hotMuSigma`` fixes the normal-CDF
     parameters that the filter otherwise fits; ``hotCutoffs`` = (lower, upper)
     skips the cut-off fits (``hotpixFilter(cutoffLower, cutoffUpper)``).
+    ``user_mask``: the plan correlates under a binary mask given by
+    ``MoviePlan.set_mask`` (the reference's ``masks`` [1,H,W]).
     ``k1_batch`` / ``k2_cols_per_wave`` / ``k2_pairs_per_wave``: launch tuning (0: built-in default;
     ``zorro_amd.autotune`` picks them per shape); results do not depend on them."""
     p = L.ZrParams()
@@ -124,6 +127,7 @@ def make_params(triMode="diag", diagWidth=5, diagStart=0, maxShift=100, Bmode="o
     p.k1_batch = int(k1_batch or 0)
     p.k2_cols_per_wave = int(k2_cols_per_wave or 0)
     p.k2_pairs_per_wave = int(k2_pairs_per_wave or 0)
+    p.user_mask = 1 if user_mask else 0
     if hotCutoffs is not None:
         p.hot_cutoff_lower, p.hot_cutoff_upper = (float(v) for v in hotCutoffs)
         p.hot_fixed_cutoffs = 1
@@ -201,6 +205,21 @@ class MoviePlan:
                 raise ValueError(f"frames shape {a.shape} != plan {(self.n, self.h, self.w)}")
             self._frames_ref = a
             L.check(self.lib.zr_movie_set_frames(self._mv, C.c_void_p(a.ctypes.data), 0))
+
+    def set_mask(self, mask):
+        """The user mask of a plan made with ``make_params(user_mask=True)``:
+        [h, w] (or [1, h, w]) of 0 and 1, numpy (copied) or a float32 CUDA tensor."""
+        if hasattr(mask, "data_ptr") and getattr(mask, "is_cuda", False):
+            m = mask.reshape(self.h, self.w)
+            if str(m.dtype) != "torch.float32" or not m.is_contiguous():
+                raise ValueError("mask must be a contiguous float32 CUDA tensor")
+            L.check(self.lib.zr_movie_set_mask(self._mv, C.c_void_p(m.data_ptr()), 1))
+            return
+        a = np.asarray(mask)
+        if a.size != self.h * self.w:
+            raise ValueError(f"mask shape {a.shape} != frame {(self.h, self.w)}")
+        a = np.ascontiguousarray(a.reshape(self.h, self.w), dtype=np.float32)
+        L.check(self.lib.zr_movie_set_mask(self._mv, C.c_void_p(a.ctypes.data), 0))
 
     def set_frames_raw(self, frames, gain=None, gain_scalar=None, packed4=False):
         """Raw frames (uint8, uint16, int16 or float32; numpy or a CUDA tensor) and

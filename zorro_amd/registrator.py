@@ -125,8 +125,6 @@ class ImageRegistrator:
     def _params(self):
         if self.xcorrMode.lower() != "zorro":
             raise NotImplementedError(f"xcorrMode {self.xcorrMode!r}: only 'zorro' runs on the GPU")
-        if self.masks is not None:
-            raise NotImplementedError("user masks are not built (the single automatic mask is)")
         return make_params(triMode=self.triMode, diagWidth=self.diagWidth, diagStart=self.diagStart,
                            autoMax=self.autoMax,
                            maxShift=self.maxShift, Bmode=self.Bmode, Brad=self.Brad, subPixReg=self.subPixReg,
@@ -136,7 +134,28 @@ class ImageRegistrator:
                            shapePadded=self.shapePadded, filterMode=self.filterMode,
                            doseFiltParam=self.doseFiltParam, pixelsize=self.pixelsize, voltage=self.voltage,
                            doLazyFRC=self.doLazyFRC, shapeBinned=self.shapeBinned, hotpixInfo=self.hotpixInfo,
-                           preShift=self.preShift)
+                           preShift=self.preShift, user_mask=self._user_mask() is not None)
+
+    def _user_mask(self):
+        """The user's single mask as float32 [H, W] of 0 and 1, or None for the
+        frame-region mask (``masks`` None or all ones).  ``masks`` of shape
+        [H, W] or [1, H, W] (``zorro/zorro.py:584-589``); padStack zero-pads it
+        with the frames (``:1942-1955``)."""
+        if self.masks is None:
+            return None
+        m = np.asarray(self.masks)
+        if m.ndim == 3 and m.shape[0] > 1:
+            raise NotImplementedError(
+                "a mask per frame (masks.shape[0] > 1) does not run in the reference: xcorrnm2_tri reads the "
+                "unbound local baseMaskFFT (zorro/zorro.py:970) and mnxc2 the undefined baseMaskFFT_block (:1048)")
+        if m.ndim == 3:
+            m = m[0]
+        if self.images is not None and m.shape != tuple(np.shape(self.images)[1:]):
+            raise ValueError(f"masks shape {m.shape} != frame shape {tuple(np.shape(self.images)[1:])}")
+        m = np.asarray(m, dtype=np.float32)
+        if not np.all((m == 0) | (m == 1)):
+            raise ValueError("masks must hold 0 and 1 only")
+        return None if np.all(m == 1) else m
 
     def _plan_for(self, n, h, w, triIndices=None):
         p = self._params()
@@ -341,6 +360,9 @@ class ImageRegistrator:
                 self.gainRef = None
         else:
             plan.set_frames(self.images)
+        um = self._user_mask()
+        if um is not None:
+            plan.set_mask(um)
         return plan
 
     def alignImageStack(self):
@@ -481,6 +503,9 @@ class ImageRegistrator:
         Sets transEven, transOdd and FRC; returns self."""
         if xcorr not in ("tri", None):
             raise NotImplementedError(f"evenOddFouRingCorr xcorr={xcorr!r}: only the zorro registration is built")
+        if self._user_mask() is not None:
+            raise NotImplementedError("evenOddFouRingCorr with a user mask (the halves would correlate under it, "
+                                      "zorro/zorro.py:2144-2146): only the edge mask of masks=None is built")
         if self.filterMode and "hot" in self.filterMode.lower():
             raise NotImplementedError("evenOddFouRingCorr with the 'hot' filter (each half would filter its own "
                                       "frames before the edge-masked registration) is not built")
