@@ -478,8 +478,9 @@ def bench_served(args):
     probs = np.array([p for _, p in classes])
     rng = np.random.default_rng(7)
     out = dict(metric="served movies/min and p50/p99 latency (30% 2048^2x20, 40% 4096^2x40, 30% 5760x4092x60)",
-               unit="movies/min", n_gpus=1, data="synthetic", inflight=args.inflight, rates=[])
+               unit="movies/min", n_gpus=1, data="synthetic", rates=[])
     inflight = max(2, args.inflight)
+    out["inflight"] = inflight
     with MovieServer(max_inflight=inflight, return_sums="device") as srv:
         # warm-up: every in-flight slot of every class gets its plan before the clock
         # starts, then one movie of each class runs through it
@@ -487,8 +488,9 @@ def bench_served(args):
             srv.warm(f, count=inflight, **kw)
         for f, d, kw in pool:
             srv.submit(f, **kw).result()
-        # saturation: a closed backlog in the mix proportions
-        seq = rng.choice(len(pool), size=30, p=probs)
+        # saturation: a closed backlog in the mix proportions (90 movies, so that
+        # the rate is not set by a few large ones)
+        seq = rng.choice(len(pool), size=90, p=probs)
         t0 = time.perf_counter()
         futs = [srv.submit(pool[k][0], **pool[k][2]) for k in seq]
         res = [f.result() for f in futs]
@@ -512,10 +514,20 @@ def bench_served(args):
             wall = max(r.done for _, r in res) - t0
             lat = latency_summary([r for _, r in res])
             err = max(_rms_truth(r, pool[k][1]) for k, r in res)
+            # per class: the mix spans a 20x range of service times (2048^2 x 20 to
+            # 5760x4092 x 60), so the pooled p99/p50 is set by the class mix
+            per = {}
+            for ci, (c, _) in enumerate(classes):
+                rc = [r for k, r in res if k == ci]
+                if rc:
+                    lc = latency_summary(rc)
+                    per[f"{c[0]}x{c[1]}x{c[2]}"] = dict(movies=len(rc), p50_ms=round(lc["p50_ms"], 2),
+                                                        p99_ms=round(lc["p99_ms"], 2),
+                                                        p99_over_p50=round(lc["p99_ms"] / lc["p50_ms"], 3))
             row = dict(rate_frac=frac, offered_movies_min=round(lam * 60, 1), movies=n,
                        achieved_movies_min=round(n / wall * 60, 1), p50_ms=round(lat["p50_ms"], 2),
                        p99_ms=round(lat["p99_ms"], 2), p99_over_p50=round(lat["p99_ms"] / lat["p50_ms"], 3),
-                       max_rms_vs_true_drift_px=round(err, 4))
+                       max_rms_vs_true_drift_px=round(err, 4), per_class=per)
             print(f"[bench served] {row}", file=sys.stderr, flush=True)
             out["rates"].append(row)
         out["server"] = dict(srv.stats)
