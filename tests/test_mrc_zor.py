@@ -163,3 +163,57 @@ def test_zor_round_trip(tmp_path):
     assert ed["solverInfo"] == {"a": 2.0}
     with pytest.raises(IOError):
         ImageRegistrator().loadConfig(str(tmp_path / "missing.zor"))
+
+
+def _tiff_bytes(pages, bits, sample_format):
+    """A baseline little-endian multi-page TIFF written tag by tag (TIFF 6.0:
+    one uncompressed strip per page, BitsPerSample, SampleFormat 1 = unsigned,
+    3 = IEEE float), independent of the library under test."""
+    n, h, w = pages.shape
+    out = bytearray(b"II*\x00") + struct.pack("<I", 0)
+    first_ifd_pos = 4
+    prev_next = first_ifd_pos
+    tags = lambda data_off: [(256, 3, 1, w), (257, 3, 1, h), (258, 3, 1, bits), (259, 3, 1, 1), (262, 3, 1, 1),
+                             (273, 4, 1, data_off), (277, 3, 1, 1), (278, 3, 1, h),
+                             (279, 4, 1, h * w * bits // 8), (339, 3, 1, sample_format)]
+    for k in range(n):
+        data_off = len(out)
+        out += pages[k].astype(pages.dtype.newbyteorder("<")).tobytes()
+        if len(out) % 2:
+            out += b"\x00"
+        ifd = len(out)
+        struct.pack_into("<I", out, prev_next, ifd)
+        t = tags(data_off)
+        out += struct.pack("<H", len(t))
+        for tag, typ, cnt, val in t:
+            out += struct.pack("<HHI", tag, typ, cnt) + (struct.pack("<HH", val, 0) if typ == 3 else struct.pack("<I", val))
+        prev_next = len(out)
+        out += struct.pack("<I", 0)
+    return bytes(out)
+
+
+@pytest.mark.parametrize("dtype,bits,fmt", [(np.uint16, 16, 1), (np.uint8, 8, 1), (np.float32, 32, 3)])
+def test_tiff_stack_read_from_hand_written_file(tmp_path, dtype, bits, fmt):
+    """loadData of a .tif stack (zorro/zorro.py:3330-3344): every page, samples
+    unchanged; 8/16-bit counts stay integers for the GPU input stage."""
+    pytest.importorskip("PIL")
+    rng = np.random.default_rng(bits)
+    top = 256 if dtype == np.uint8 else 60000
+    pages = (rng.integers(0, top, (3, 5, 7)) if dtype != np.float32 else rng.normal(0, 1e3, (3, 5, 7))).astype(dtype)
+    path = tmp_path / "movie.tif"
+    path.write_bytes(_tiff_bytes(pages, bits, fmt))
+    reg = ImageRegistrator()
+    reg.pixelsize = 0.1
+    reg.loadData(str(path))
+    assert reg.images.dtype == (np.float32 if dtype == np.float32 else dtype)
+    assert np.array_equal(reg.images, pages)
+    assert reg.pixelsize == 0.1          # the reference's TIFF path sets no pixel size
+
+
+def test_tiff_write_read_round_trip(tmp_path):
+    pytest.importorskip("PIL")
+    from zorro_amd.tiff import read_tiff, write_tiff
+    a = np.random.default_rng(3).integers(0, 4000, (4, 9, 6)).astype(np.uint16)
+    write_tiff(str(tmp_path / "s.tiff"), a)
+    b = read_tiff(str(tmp_path / "s.tiff"))
+    assert b.dtype == np.uint16 and np.array_equal(a, b)

@@ -207,19 +207,24 @@ class ImageRegistrator:
         ``gain``, as ``:3418-3419``, ``:3455-3467``, ``:3477-3480`` do); a
         non-zero header pixel size replaces ``pixelsize`` (nm).  A gain
         reference named in ``files['gainRef']`` is loaded and flipped per
-        ``gainInfo`` when a stack is loaded.  DM3/DM4/TIFF/HDF5 are not read."""
+        ``gainInfo`` when a stack is loaded.  TIFF stacks are read with Pillow
+        (``:3330-3344``, no pixel size); DM3/DM4/HDF5 are not read."""
         self.bench["loaddata0"] = time.time()
         if stackNameIn is not None:
             self.files[target] = stackNameIn
         fname = self.files.get(target)
         ext = os.path.splitext(fname)[1].lower() if fname else ""
-        if ext not in (".mrc", ".mrcs", ".mrcz", ".mrczs"):
-            raise NotImplementedError(f"loadData: {ext or 'no file'}: only MRC/MRCS stacks are read")
-        data, header = mrc.read_mrc(fname, memmap=useMemmap)
+        if ext in (".tif", ".tiff"):
+            from .tiff import read_tiff
+            data, header = read_tiff(fname), None
+        elif ext in (".mrc", ".mrcs", ".mrcz", ".mrczs"):
+            data, header = mrc.read_mrc(fname, memmap=useMemmap)
+        else:
+            raise NotImplementedError(f"loadData: {ext or 'no file'}: MRC/MRCS and TIFF stacks are read")
         data = np.ascontiguousarray(data)
         if data.dtype in (np.int8, np.float16):
             data = data.astype(np.float32)
-        if not np.isclose(header["pixelsize"][-1], 0.0):
+        if header is not None and not np.isclose(header["pixelsize"][-1], 0.0):
             self.pixelsize = float(np.float32(header["pixelsize"][-1]))
         if target in ("stack", "align", "images"):
             if data.ndim != 3:
