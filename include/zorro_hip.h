@@ -116,10 +116,13 @@ typedef struct zr_params {
 
 /* filterMode steps (zorro/zorro.py:1692-1711).  DOSE: doseFilter(normalize=False),
  * DOSENORM: doseFilter(normalize=True) (each rewrites filtSum from the aligned
- * frames); GAUSSLP: filtSum = scipy.ndimage.gaussian_filter(filtSum, 3.0), on a
- * copy of imageSum when no filtSum exists yet.  'background' (a Powell fit of a
- * 2D Gaussian, zorro/zorro_util.py:497-549) is not built. */
-enum { ZR_FILTER_DOSE = 1, ZR_FILTER_DOSENORM = 2, ZR_FILTER_GAUSSLP = 3 };
+ * frames); GAUSSLP: filtSum = scipy.ndimage.gaussian_filter(filtSum, 3.0);
+ * BACKGROUND: filtSum -= backgroundEstimate(filtSum) (zorro/zorro_util.py:497-549:
+ * a 2D Gaussian fitted under an L1 error by Powell's method to the sigma-4
+ * low-passed sum sampled every 128 pixels; left unchanged when the fit does not
+ * converge).  The last two start from a copy of imageSum when no filtSum exists
+ * yet. */
+enum { ZR_FILTER_DOSE = 1, ZR_FILTER_DOSENORM = 2, ZR_FILTER_GAUSSLP = 3, ZR_FILTER_BACKGROUND = 4 };
 
 /* Element types of raw frames (MRC modes 2 float32, 0 uint8 counts, 6 uint16,
  * 101 4-bit counts packed two per byte with the first pixel in the low nibble
@@ -282,6 +285,13 @@ int zr_get_solution(zr_movie* mv, double* translations, double* relative_est, do
  * three arrays may be NULL; each holds at least `cap` floats; *nbins receives
  * the bin count (0 when weightMode is not 'autologistic').  Call after zr_solve. */
 int zr_get_peaksig_hist(zr_movie* mv, float* pdf, float* cdf, float* hsigma, int cap, int* nbins);
+
+/* The last 'background' step's fit: p[6] = (offset, centre x, centre y,
+ * amplitude, width x, width y) in the fit's grid coordinates, the L1 error and
+ * the status (0 converged, 1 evaluation limit, 2 iteration limit, 3 NaN: the
+ * reference then leaves filtSum unchanged).  No counterpart: the reference
+ * keeps only the subtracted result. */
+int zr_get_background_fit(zr_movie* mv, double* p6, double* fval, int* status);
 /* Solver diagnostics: info6 = {positive peak significances, histogram bins,
  * status y, status x (1 optimal vertex, 2 pivot limit, -1 disconnected),
  * simplex pivots y, pivots x}. */

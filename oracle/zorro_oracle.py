@@ -43,7 +43,7 @@ import scipy.optimize
 
 __all__ = [
     "Params", "tri_indices", "pad_stack", "shape_cropped", "fourier_crop",
-    "raw_ifft2", "frame_spectra", "mask_spectrum", "mnxc2_window",
+    "raw_ifft2", "frame_spectra", "mask_spectrum", "mnxc2_window", "background_estimate",
     "mnxc2_window_unmasked", "window_from_full", "b_filter", "locate_peak",
     "xcorrnm2_tri", "build_equations", "logistic", "logistic_objective",
     "logistic_fit", "cdf_logistic_curve", "l1_solve", "exact_vertex",
@@ -57,6 +57,9 @@ __all__ = [
 # Which oracle function is pinned by what (mirrored in tests/test_oracle_pins.py).
 PINS = {
     "raw_ifft2": "closed forms: delta, constant; Parseval",
+    "background_estimate": "started at a 2D Gaussian surface drawn in the fit's coordinates, the fit keeps it "
+                           "and the full-grid surface reproduces the frame (square); the y-centring quirk on "
+                           "non-square frames; a constant frame fails the fit (NaN) and gives no background",
     "frame_spectra/fourier_crop": "crop of a band-limited image == decimation; DC == sum",
     "mnxc2_window": "brute-force direct masked NCC on tiny frames; roll KAT (peak at d, C=1)",
     "mnxc2_window_unmasked": "equals mnxc2_window with an all-ones mask (Q2 closed form)",
@@ -840,6 +843,50 @@ def gauss_lowpass(img):
     return scipy.ndimage.gaussian_filter(np.asarray(img, np.float64), 3.0)
 
 
+def background_estimate(img, binFact: int = 128, lpSigma: float = 4.0, p_start=None):
+    """``backgroundEstimate(fitType='gauss2')`` (``zorro/zorro_util.py:497-549``):
+    the float32 image low-passed by ``gaussian_filter(sigma 4)``, sampled every
+    ``binFact`` pixels (``interp2_nn`` at integer positions, ``:482-495``),
+    cropped [1:-2, 1:-2], and a 2D Gaussian p0 + p3 exp(-(x-p1)^2/p4^2 -
+    (y-p2)^2/p5^2) fitted under an L1 error with ``minimize(method='Powell')``
+    from (mean, 1, 1, mean, mean*std, mean*std).  The fit's y coordinates are
+    centred by the x sample count (``ycrop.shape[1]``, ``:512``).  Returns
+    (background f64 [H, W] evaluated at ((x - W/2)/binFact, (y - H/2)/binFact),
+    or zeros when the fit fails; the scipy result).  ``p_start`` replaces the
+    reference's starting point (tests only)."""
+    im = np.asarray(img, np.float32)
+    h, w = im.shape
+    xcrop, ycrop = np.meshgrid(np.arange(0, w, binFact), np.arange(0, h, binFact))
+    g = scipy.ndimage.gaussian_filter(im, lpSigma)
+    nn = g[np.clip(np.floor(ycrop).astype(int), 0, h - 1), np.clip(np.floor(xcrop).astype(int), 0, w - 1)]
+    xmesh = (xcrop.astype(np.float32) / binFact - xcrop.shape[1] / 2.0)[1:-2, 1:-2]
+    ymesh = (ycrop.astype(np.float32) / binFact - ycrop.shape[1] / 2.0)[1:-2, 1:-2]
+    nn = nn[1:-2, 1:-2]
+
+    def gauss2(q, x, y):
+        xc, yc = x - q[1], y - q[2]
+        return q[0] + q[3] * np.exp(-xc * xc / q[4] ** 2 - yc * yc / q[5] ** 2)
+
+    def err(q, c):
+        return np.sum(np.abs(c - gauss2(q, xmesh, ymesh)))
+
+    p0 = np.ones(6, np.float64)
+    p0[0] = np.mean(nn)
+    p0[3] = np.mean(nn)
+    p0[4] = np.mean(nn) * np.std(nn)
+    p0[5] = p0[4]
+    if p_start is not None:
+        p0 = np.asarray(p_start, np.float64)
+    fit = scipy.optimize.minimize(err, p0, method="Powell", args=(nn,))
+    fit.samples, fit.xmesh, fit.ymesh = nn, xmesh, ymesh      # for checks of other fits on the same samples
+    xback, yback = np.meshgrid(np.arange(w), np.arange(h))
+    xback = (xback.astype(np.float32) - w / 2.0) / np.float32(binFact)
+    yback = (yback.astype(np.float32) - h / 2.0) / np.float32(binFact)
+    if fit.success:
+        return gauss2(fit.x, xback, yback), fit
+    return np.zeros((h, w)), fit
+
+
 def filter_steps(filterMode):
     """The filterMode tokens alignImageStack applies after the registration, in
     order (``zorro/zorro.py:1692-1711``): lower-cased, spaces removed, split on
@@ -847,9 +894,7 @@ def filter_steps(filterMode):
     if not filterMode:
         return []
     toks = filterMode.lower().replace(" ", "").split(",")
-    if "background" in toks:
-        raise NotImplementedError("filterMode 'background' (backgroundEstimate's Powell fit) is outside the oracle")
-    return [t for t in toks if t in ("dose", "dosenorm", "gausslp")]
+    return [t for t in toks if t in ("dose", "dosenorm", "gausslp", "background")]
 
 
 # ---------------------------------------------------------------------------
@@ -1107,6 +1152,10 @@ def align_image_stack(images: np.ndarray, p: Params, mask: np.ndarray | None = N
         if step in ("dose", "dosenorm"):
             filt, info = dose_filter(aligned, image_sum, p, p.shapePadded, normalize=(step == "dosenorm"))
             res.update(dose=info)
+        elif step == "background":                                       # :1702-1706, in float32
+            base = np.asarray(image_sum if filt is None else filt, np.float32)
+            back, _ = background_estimate(base)
+            filt = (base.astype(np.float64) - back).astype(np.float32)
         else:                                                            # gausslp
             filt = gauss_lowpass(image_sum if filt is None else filt)
     if filt is not None:

@@ -2,8 +2,8 @@
 
 alignImageStack runs hotpixFilter when 'hot' occurs anywhere in filterMode
 (zorro/zorro.py:1595-1596) and then applies the comma-separated tokens in
-order (:1692-1711).  make_params keeps that order in zr_params.filters, refuses
-'background' (not built) and rejects tokens the reference would skip silently.
+order (:1692-1711).  make_params keeps that order in zr_params.filters and
+rejects tokens the reference would skip silently.
 """
 import pytest
 
@@ -38,9 +38,10 @@ def test_hot_anywhere_runs_the_outlier_filter():
     assert parse_filter_mode("dose,") == (["dose"], False)
 
 
-def test_background_is_refused_and_unknown_tokens_raise():
-    with pytest.raises(NotImplementedError):
-        make_params(filterMode="dose,background", pixelsize=0.1)
+def test_background_is_a_step_and_unknown_tokens_raise():
+    p = make_params(filterMode="dose,background", pixelsize=0.1)
+    assert _steps(p) == [L.ZR_FILTER_DOSE, L.ZR_FILTER_BACKGROUND] and p.do_dose == 1
+    assert make_params(filterMode="background").do_dose == 0
     with pytest.raises(ValueError):
         make_params(filterMode="dsoe", pixelsize=0.1)
     with pytest.raises(NotImplementedError):

@@ -660,8 +660,7 @@ def test_filter_steps_follow_token_order():
     assert O.filter_steps("Dose, gaussLP") == ["dose", "gausslp"]
     assert O.filter_steps("hot,dosenorm") == ["dosenorm"]
     assert O.filter_steps(None) == []
-    with pytest.raises(NotImplementedError):
-        O.filter_steps("dose,background")
+    assert O.filter_steps("dose,background, gausslp") == ["dose", "background", "gausslp"]
 
 
 # ------------------------------------------------------------- Fourier shifts
@@ -700,3 +699,48 @@ def test_fourier_shift_nyquist_row_takes_cos_pi_t():
     img = (-1.0) ** yy
     got = O.fourier_shift(img, (0.3, 0.0), (h, w))
     assert np.allclose(got, np.cos(np.pi * 0.3) * img, atol=1e-12)
+
+
+def _bg_model(q, x, y):
+    xc, yc = x - q[1], y - q[2]
+    return q[0] + q[3] * np.exp(-xc * xc / q[4] ** 2 - yc * yc / q[5] ** 2)
+
+
+def test_background_estimate_keeps_a_drawn_surface():
+    """backgroundEstimate (zorro/zorro_util.py:497-549) on a square frame whose
+    content is a 2D Gaussian in the fit's grid units, (x - W/2)/128, started at
+    that surface: the L1 error there is only the sigma-4 low-pass's (< 1e-4 of the
+    amplitude), Powell keeps it, and the full-grid surface reproduces the frame.
+    A wrong sample crop, grid centre or evaluation grid moves the fit away."""
+    h = w = 2048
+    q = (40.0, 1.3, -0.7, 4.0, 4.0, 6.0)
+    y, x = np.mgrid[:h, :w]
+    img = _bg_model(q, (x - w / 2) / 128.0, (y - h / 2) / 128.0).astype(np.float32)
+    back, fit = O.background_estimate(img, p_start=q)
+    assert fit.success
+    assert np.max(np.abs(back - img)) < 2e-3 * q[3], np.max(np.abs(back - img))
+
+
+def test_background_estimate_y_centring_by_x_count():
+    """On a W > H frame the fit centres y by the x sample count (ycrop.shape[1],
+    zorro/zorro_util.py:512) but the surface is evaluated at (y - H/2)/128: a
+    surface drawn in the fit's coordinates comes back shifted in y by
+    (nx - ny)/2 sample steps."""
+    h, w = 1024, 2048
+    nx, ny = w // 128, h // 128
+    q = (40.0, 0.5, -1.0, 4.0, 4.0, 5.0)
+    y, x = np.mgrid[:h, :w]
+    img = _bg_model(q, x / 128.0 - nx / 2.0, y / 128.0 - nx / 2.0).astype(np.float32)
+    back, fit = O.background_estimate(img, p_start=q)
+    assert fit.success
+    shifted = _bg_model(q, x / 128.0 - nx / 2.0, y / 128.0 - ny / 2.0)
+    assert np.max(np.abs(back - shifted)) < 2e-3 * q[3]
+    assert np.max(np.abs(back - img)) > 0.05 * q[3]      # the quirk is visible
+
+
+def test_background_estimate_of_a_constant_fails_the_fit():
+    """A constant frame has std 0, so the start's widths are 0 and the error is
+    NaN: the fit reports failure and the background is zero (filtSum unchanged)."""
+    img = np.full((1024, 1024), 3.25, np.float32)
+    back, fit = O.background_estimate(img)
+    assert not fit.success and np.all(back == 0.0)

@@ -18,7 +18,7 @@ ZR_B_NONE, ZR_B_CONV, ZR_B_OPTI = 0, 1, 2
 ZR_SHIFT_INTEGER, ZR_SHIFT_LANCZOS, ZR_SHIFT_FOURIER = 0, 1, 2
 ZR_W_AUTOLOGISTIC, ZR_W_LOGISTIC, ZR_W_CORR, ZR_W_UNWEIGHTED, ZR_W_NORM = 0, 1, 2, 3, 4
 ZR_DTYPE_F32, ZR_DTYPE_U8, ZR_DTYPE_U16, ZR_DTYPE_U4, ZR_DTYPE_I16 = 0, 1, 2, 3, 4
-ZR_FILTER_DOSE, ZR_FILTER_DOSENORM, ZR_FILTER_GAUSSLP = 1, 2, 3
+ZR_FILTER_DOSE, ZR_FILTER_DOSENORM, ZR_FILTER_GAUSSLP, ZR_FILTER_BACKGROUND = 1, 2, 3, 4
 
 
 class ZrParams(C.Structure):
@@ -80,6 +80,7 @@ SIGNATURES = {
     "zr_filter": (C.c_int, [_P, C.c_int]),
     "zr_post_filters": (C.c_int, [_P]),
     "zr_get_peaksig_hist": (C.c_int, [_P, _F, _F, _F, C.c_int, _I]),
+    "zr_get_background_fit": (C.c_int, [_P, _D, _D, _I]),
     "zr_tiled_frc": (C.c_int, [_P, _P, C.c_int, C.c_int, _D, C.c_int, C.c_int, C.c_double, _P, _I, _P]),
     "zr_tiled_frc_noalloc": (C.c_int, [_P, _P, C.c_int, C.c_int, _D, C.c_int, C.c_int, C.c_double, _P, _I, _P]),
     "zr_batch_create": (C.c_int, [C.c_int, C.POINTER(ZrParams), C.c_int, _I, C.c_int, C.c_int, C.POINTER(_P)]),

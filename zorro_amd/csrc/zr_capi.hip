@@ -396,9 +396,9 @@ static int create_movie(int device, const zr_params* p, int n_frames, int h, int
   pn.do_dose = 0;
   for (int i = 0; i < pn.n_filters; ++i) {
     const int f = pn.filters[i];
-    if (f != ZR_FILTER_DOSE && f != ZR_FILTER_DOSENORM && f != ZR_FILTER_GAUSSLP)
+    if (f != ZR_FILTER_DOSE && f != ZR_FILTER_DOSENORM && f != ZR_FILTER_GAUSSLP && f != ZR_FILTER_BACKGROUND)
       return fail(ZR_ERR_UNSUPPORTED, "filter step %d: %d is not a built filterMode step", i, f);
-    if (f != ZR_FILTER_GAUSSLP) pn.do_dose = 1;
+    if (f == ZR_FILTER_DOSE || f == ZR_FILTER_DOSENORM) pn.do_dose = 1;
   }
   for (int i = pn.n_filters; i < 4; ++i) pn.filters[i] = 0;
   p = &pn;
@@ -1088,6 +1088,12 @@ extern "C" int zr_filter(zr_movie* mv, int step) {
     mv->timer.end(ZR_STAGE_DOSE, t0, mv->stream);
   } else if (step == ZR_FILTER_GAUSSLP) {
     rc = zr_stage_gausslp(mv->st, mv->stream);
+  } else if (step == ZR_FILTER_BACKGROUND) {
+    const int ny = (mv->H + 127) / 128, nx = (mv->W + 127) / 128;
+    if (ny < 4 || nx < 4 || ny > 80 || nx > 80)
+      return fail(ZR_ERR_UNSUPPORTED, "background: %dx%d frames give a %dx%d sample grid (built 4..80 per axis)",
+                  mv->H, mv->W, ny, nx);
+    rc = zr_stage_background(mv->st, mv->stream);
   } else {
     return fail(ZR_ERR_ARG, "filter step %d is not a ZR_FILTER_* value", step);
   }
@@ -1096,13 +1102,22 @@ extern "C" int zr_filter(zr_movie* mv, int step) {
 
 extern "C" int zr_dose_filter(zr_movie* mv) { return zr_filter(mv, ZR_FILTER_DOSE); }
 
+extern "C" int zr_get_background_fit(zr_movie* mv, double* p6, double* fval, int* status) {
+  if (!mv) return fail(ZR_ERR_ARG, "NULL movie");
+  ZR_HIP(hipSetDevice(mv->device));
+  ZR_HIP(sync_movie(mv));
+  const int rc = zr_stage_get_background(mv->st, p6, fval, status);
+  return rc ? fail(rc, "%s", zr_stage_error()) : ZR_OK;
+}
+
 extern "C" int zr_post_filters(zr_movie* mv) {
   if (!mv) return fail(ZR_ERR_ARG, "NULL movie");
   // a dose step forms the aligned sums and half sums itself; unless the first
   // step is one, the frames are shifted and summed first (applyShifts,
   // zorro/zorro.py:1672), which a Gaussian step needs for its imageSum copy
   int rc;
-  const bool dose_first = mv->p.n_filters > 0 && mv->p.filters[0] != ZR_FILTER_GAUSSLP;
+  const bool dose_first = mv->p.n_filters > 0 &&
+                          (mv->p.filters[0] == ZR_FILTER_DOSE || mv->p.filters[0] == ZR_FILTER_DOSENORM);
   if (!dose_first && (rc = zr_apply_shifts(mv, nullptr))) return rc;
   for (int i = 0; i < mv->p.n_filters; ++i)
     if ((rc = zr_filter(mv, mv->p.filters[i]))) return rc;

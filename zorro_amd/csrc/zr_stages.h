@@ -39,6 +39,9 @@ int zr_stage_fourier_align(zr_stage_state* s, const float2* S, int Wk, int Hs, i
                            hipStream_t st);
 // 'gausslp' on filtSum (on imageSum when no filter step has run since the solve)
 int zr_stage_gausslp(zr_stage_state* s, hipStream_t st);
+// filterMode 'background': backgroundEstimate's fitted 2D Gaussian subtracted from filtSum
+int zr_stage_background(zr_stage_state* s, hipStream_t st);
+int zr_stage_get_background(zr_stage_state* s, double* p6, double* fval, int* status);
 // cdfLogisticCurve's histogram (pdf counts, cdf, left edges); nbins = 0 without autologistic weights
 int zr_stage_get_hist(zr_stage_state* s, float* pdf, float* cdf, float* hsigma, int cap, int* nbins);
 int zr_stage_get_solution(zr_stage_state* s, double* translations, double* relative_est, double* weights,

@@ -1889,6 +1889,275 @@ __global__ __launch_bounds__(256) void k_gauss_axis1(const float* __restrict__ s
   dst[(size_t)y * W + x] = (float)acc;
 }
 
+// ---------------------------------------------------- filterMode 'background'
+// backgroundEstimate (zorro/zorro_util.py:497-549, applied at zorro/zorro.py:
+// 1702-1706): the sum low-passed by gaussian_filter(sigma 4, truncate 4, reflect),
+// sampled every 128 pixels (interp2_nn at integer positions), cropped by 1 at
+// the start and 2 at the end of each axis, fitted by a 2D Gaussian
+// g(p; x, y) = p0 + p3 exp(-(x - p1)^2/p4^2 - (y - p2)^2/p5^2) under an L1 error
+// with Powell's method from p = (mean, 1, 1, mean, mean*std, mean*std), and the
+// fitted surface, evaluated at ((x - W/2)/128, (y - H/2)/128), subtracted.  The
+// fit's grid coordinates are x = kx - nx/2 and y = ky - nx/2 (the reference
+// centres y by the x count, ycrop.shape[1]).
+constexpr int BG_BIN = 128, BG_R = 16, BG_NT = 256;
+constexpr int BG_MAXG = 80;          // samples per axis: frames up to 10240 pixels
+struct BgTaps { double w[BG_R + 1]; };
+struct BgFit { double p[6]; double f; int nfev; int status; };   // status 0: converged
+
+// axis-0 pass of the sigma-4 Gaussian at the sample rows y = 128 ky only
+__global__ __launch_bounds__(256) void k_bg_rows(const float* __restrict__ src, int H, int W, int ny, BgTaps t,
+                                                 float* __restrict__ g0) {
+  const size_t e = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (e >= (size_t)ny * W) return;
+  const int ky = (int)(e / W), x = (int)(e % W), y = ky * BG_BIN;
+  double acc = t.w[0] * (double)src[(size_t)y * W + x];
+  for (int j = 1; j <= BG_R; ++j)
+    acc += ((double)src[(size_t)refl_index(y + j, H) * W + x] + (double)src[(size_t)refl_index(y - j, H) * W + x]) *
+           t.w[j];
+  g0[e] = (float)acc;
+}
+
+// axis-1 pass at the sample columns x = 128 kx of those rows: nn[ky][kx]
+__global__ __launch_bounds__(256) void k_bg_sample(const float* __restrict__ g0, int W, int ny, int nx, BgTaps t,
+                                                   float* __restrict__ nn) {
+  const int e = blockIdx.x * 256 + threadIdx.x;
+  if (e >= ny * nx) return;
+  const int ky = e / nx, x = (e % nx) * BG_BIN;
+  const float* r = g0 + (size_t)ky * W;
+  double acc = t.w[0] * (double)r[x];
+  for (int j = 1; j <= BG_R; ++j) acc += ((double)r[refl_index(x + j, W)] + (double)r[refl_index(x - j, W)]) * t.w[j];
+  nn[e] = (float)acc;
+}
+
+// The L1 fit on one block.  Every thread runs the same scalar control flow of
+// Powell's direction-set method (Powell 1964) with Brent's bracketing and
+// line minimisation (Brent 1973), the steps scipy's minimize(method='Powell')
+// takes with its defaults (xtol = ftol = 1e-4, line tolerance 100 xtol, at most
+// 6000 iterations and 6000 evaluations); each objective evaluation is summed
+// over the samples by the whole block in a fixed order.
+struct BgObj {
+  const float* c;      // cropped samples [cy][cx], LDS
+  int cx, n;
+  float x0, y0;        // grid coordinates of the first sample (1 - nx/2 on both axes)
+  double* red;         // LDS [BG_NT / 64]
+  int nfev;
+  ZR_DEV double operator()(const double* p) {
+    const int tid = threadIdx.x;
+    const double p4 = p[4] * p[4], p5 = p[5] * p[5];
+    double acc = 0.0;
+    for (int i = tid; i < n; i += BG_NT) {
+      const double xc = (double)(x0 + (float)(i % cx)) - p[1], yc = (double)(y0 + (float)(i / cx)) - p[2];
+      acc += fabs((double)c[i] - (p[0] + p[3] * exp(-xc * xc / p4 - yc * yc / p5)));
+    }
+    acc = wave_sum_d(acc);
+    __syncthreads();                      // the previous evaluation's reads of red are done
+    if ((tid & 63) == 0) red[tid >> 6] = acc;
+    __syncthreads();
+    double s = 0.0;
+    for (int k = 0; k < BG_NT / 64; ++k) s += red[k];
+    ++nfev;
+    return s;
+  }
+};
+
+struct BgLine {          // f(x0 + a d) along one direction
+  BgObj* f;
+  double x0[6], d[6];
+  ZR_DEV double operator()(double a) {
+    double q[6];
+    for (int i = 0; i < 6; ++i) q[i] = x0[i] + a * d[i];
+    return (*f)(q);
+  }
+};
+
+// Brent's minimisation of line(a) from the bracket grown from (0, 1); returns
+// the minimiser and its value.
+ZR_DEV void bg_brent(BgLine& line, double tol, double* amin, double* fmin) {
+  const double gold = 1.618034, vsmall = 1e-21, grow = 110.0;
+  double xa = 0.0, xb = 1.0;
+  double fa = line(xa), fb = line(xb);
+  if (fa < fb) { double t = xa; xa = xb; xb = t; t = fa; fa = fb; fb = t; }
+  double xc = xb + gold * (xb - xa), fc = line(xc);
+  for (int it = 0; fc < fb && it <= 1000; ++it) {
+    const double t1 = (xb - xa) * (fb - fc), t2 = (xb - xc) * (fb - fa);
+    const double val = t2 - t1;
+    const double denom = fabs(val) < vsmall ? 2.0 * vsmall : 2.0 * val;
+    double w = xb - ((xb - xc) * t2 - (xb - xa) * t1) / denom, fw;
+    const double wlim = xb + grow * (xc - xb);
+    if ((w - xc) * (xb - w) > 0.0) {
+      fw = line(w);
+      if (fw < fc) { xa = xb; xb = w; fa = fb; fb = fw; break; }
+      if (fw > fb) { xc = w; fc = fw; break; }
+      w = xc + gold * (xc - xb);
+      fw = line(w);
+    } else if ((w - wlim) * (wlim - xc) >= 0.0) {
+      w = wlim;
+      fw = line(w);
+    } else if ((w - wlim) * (xc - w) > 0.0) {
+      fw = line(w);
+      if (fw < fc) {
+        xb = xc; xc = w; w = xc + gold * (xc - xb);
+        fb = fc; fc = fw; fw = line(w);
+      }
+    } else {
+      w = xc + gold * (xc - xb);
+      fw = line(w);
+    }
+    xa = xb; xb = xc; xc = w;
+    fa = fb; fb = fc; fc = fw;
+  }
+  const double mintol = 1e-11, cg = 0.3819660;
+  double x = xb, w = xb, v = xb, fx = fb, fw = fb, fv = fb;
+  double a = xa < xc ? xa : xc, b = xa < xc ? xc : xa;
+  double deltax = 0.0, rat = 0.0;
+  for (int it = 0; it < 500; ++it) {
+    const double tol1 = tol * fabs(x) + mintol, tol2 = 2.0 * tol1;
+    const double xmid = 0.5 * (a + b);
+    if (fabs(x - xmid) < (tol2 - 0.5 * (b - a))) break;
+    if (fabs(deltax) <= tol1) {
+      deltax = (x >= xmid) ? a - x : b - x;
+      rat = cg * deltax;
+    } else {
+      double t1 = (x - w) * (fx - fv), t2 = (x - v) * (fx - fw);
+      double pp = (x - v) * t2 - (x - w) * t1;
+      t2 = 2.0 * (t2 - t1);
+      if (t2 > 0.0) pp = -pp;
+      t2 = fabs(t2);
+      const double dx_temp = deltax;
+      deltax = rat;
+      if (pp > t2 * (a - x) && pp < t2 * (b - x) && fabs(pp) < fabs(0.5 * t2 * dx_temp)) {
+        rat = pp / t2;
+        const double u = x + rat;
+        if ((u - a) < tol2 || (b - u) < tol2) rat = (xmid - x >= 0.0) ? tol1 : -tol1;
+      } else {
+        deltax = (x >= xmid) ? a - x : b - x;
+        rat = cg * deltax;
+      }
+    }
+    const double u = fabs(rat) < tol1 ? (rat >= 0.0 ? x + tol1 : x - tol1) : x + rat;
+    const double fu = line(u);
+    if (fu > fx) {
+      if (u < x) a = u; else b = u;
+      if (fu <= fw || w == x) { v = w; w = u; fv = fw; fw = fu; }
+      else if (fu <= fv || v == x || v == w) { v = u; fv = fu; }
+    } else {
+      if (u >= x) a = x; else b = x;
+      v = w; w = x; x = u;
+      fv = fw; fw = fx; fx = fu;
+    }
+  }
+  *amin = x;
+  *fmin = fx;
+}
+
+__global__ __launch_bounds__(BG_NT) void k_bg_fit(const float* __restrict__ nn, int ny, int nx, BgFit* __restrict__ out) {
+  __shared__ float c[BG_MAXG * BG_MAXG];
+  __shared__ double red[BG_NT / 64];
+  const int tid = threadIdx.x;
+  const int cy = ny - 3, cx = nx - 3, n = cy * cx;
+  for (int i = tid; i < n; i += BG_NT) c[i] = nn[(1 + i / cx) * nx + 1 + i % cx];
+  __syncthreads();
+  // start: mean and population std of the cropped samples (float32 results, as numpy gives them)
+  double s1 = 0.0, s2 = 0.0;
+  for (int i = tid; i < n; i += BG_NT) s1 += c[i];
+  s1 = wave_sum_d(s1);
+  if ((tid & 63) == 0) red[tid >> 6] = s1;
+  __syncthreads();
+  double mean = 0.0;
+  for (int k = 0; k < BG_NT / 64; ++k) mean += red[k];
+  mean /= n;
+  __syncthreads();
+  for (int i = tid; i < n; i += BG_NT) s2 += ((double)c[i] - mean) * ((double)c[i] - mean);
+  s2 = wave_sum_d(s2);
+  if ((tid & 63) == 0) red[tid >> 6] = s2;
+  __syncthreads();
+  double var = 0.0;
+  for (int k = 0; k < BG_NT / 64; ++k) var += red[k];
+  const double m32 = (double)(float)mean, sd32 = (double)(float)sqrt(var / n);
+  const float g0 = 1.0f - (float)(nx / 2.0);    // kx - nx/2 and ky - nx/2 at kx = ky = 1 (exact in float32)
+  BgObj f{c, cx, n, g0, g0, red, 0};
+  double x[6] = {m32, 1.0, 1.0, m32, (double)(float)(m32 * sd32), (double)(float)(m32 * sd32)};
+  double dir[6][6];
+  for (int i = 0; i < 6; ++i)
+    for (int j = 0; j < 6; ++j) dir[i][j] = i == j ? 1.0 : 0.0;
+  const double xtol = 1e-4, ftol = 1e-4;
+  const int maxit = 6000, maxfev = 6000;
+  double fval = f(x), x1[6];
+  for (int i = 0; i < 6; ++i) x1[i] = x[i];
+  int status = 0, iter = 0;
+  auto line_min = [&](const double* d, double* xnew, double* dnew) -> double {
+    // returns the new value; xnew = x + a d, dnew = a d (a zero direction is kept as it is)
+    bool any = false;
+    for (int i = 0; i < 6; ++i) any |= d[i] != 0.0;
+    if (!any) {
+      for (int i = 0; i < 6; ++i) { xnew[i] = x[i]; dnew[i] = d[i]; }
+      return fval;
+    }
+    BgLine ln{&f, {x[0], x[1], x[2], x[3], x[4], x[5]}, {d[0], d[1], d[2], d[3], d[4], d[5]}};
+    double a, fa;
+    bg_brent(ln, xtol * 100.0, &a, &fa);
+    for (int i = 0; i < 6; ++i) { dnew[i] = a * d[i]; xnew[i] = x[i] + dnew[i]; }
+    return fa;
+  };
+  while (true) {
+    const double fx = fval;
+    int bigind = 0;
+    double delta = 0.0;
+    for (int i = 0; i < 6; ++i) {
+      const double fx2 = fval;
+      double xn[6], dn[6];
+      fval = line_min(dir[i], xn, dn);
+      for (int k = 0; k < 6; ++k) { x[k] = xn[k]; dir[i][k] = dn[k]; }
+      if (fx2 - fval > delta) { delta = fx2 - fval; bigind = i; }
+    }
+    ++iter;
+    const double bnd = ftol * (fabs(fx) + fabs(fval)) + 1e-20;
+    if (2.0 * (fx - fval) <= bnd) break;
+    if (f.nfev >= maxfev) { status = 1; break; }
+    if (iter >= maxit) { status = 2; break; }
+    if (isnan(fx) && isnan(fval)) { status = 3; break; }
+    double d1[6], x2[6];
+    for (int k = 0; k < 6; ++k) { d1[k] = x[k] - x1[k]; x2[k] = 2.0 * x[k] - x1[k]; x1[k] = x[k]; }
+    const double fx2 = f(x2);
+    if (fx > fx2) {
+      double t = 2.0 * (fx + fx2 - 2.0 * fval);
+      double tmp = fx - fval - delta;
+      t *= tmp * tmp;
+      tmp = fx - fx2;
+      t -= delta * tmp * tmp;
+      if (t < 0.0) {
+        double xn[6], dn[6];
+        fval = line_min(d1, xn, dn);
+        bool any = false;
+        for (int k = 0; k < 6; ++k) { x[k] = xn[k]; any |= dn[k] != 0.0; }
+        if (any) {
+          for (int k = 0; k < 6; ++k) { dir[bigind][k] = dir[5][k]; dir[5][k] = dn[k]; }
+        }
+      }
+    }
+  }
+  if (f.nfev >= maxfev && status == 0) status = 1;
+  if (tid == 0) {
+    for (int i = 0; i < 6; ++i) out->p[i] = x[i];
+    out->f = fval;
+    out->nfev = f.nfev;
+    out->status = (isnan(fval) || status != 0) ? (status ? status : 3) : 0;
+  }
+}
+
+// filtSum -= the fitted surface (in double, rounded back to float32), when the fit converged
+__global__ __launch_bounds__(256) void k_bg_subtract(float* __restrict__ img, int H, int W,
+                                                     const BgFit* __restrict__ fit) {
+  const size_t e = (size_t)xcd_remap(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
+  if (e >= (size_t)H * W || fit->status != 0) return;
+  const int y = (int)(e / W), x = (int)(e % W);
+  const double* p = fit->p;
+  const double xb = ((double)x - W / 2.0) / BG_BIN, yb = ((double)y - H / 2.0) / BG_BIN;
+  const double xc = xb - p[1], yc = yb - p[2];
+  const double back = p[0] + p[3] * exp(-xc * xc / (p[4] * p[4]) - yc * yc / (p[5] * p[5]));
+  img[e] = (float)((double)img[e] - back);
+}
+
 // ==================================================================== host
 namespace {
 thread_local std::string s_err;
@@ -2015,6 +2284,9 @@ struct zr_stage_state {
   float2 *twHp = nullptr, *twWp = nullptr, *twWp2 = nullptr, *Tdose = nullptr, *G = nullptr;
   double *accum = nullptr, *thresq = nullptr;
   float* gtmp = nullptr;       // 'gausslp': the axis-0 pass of the Gaussian
+  float* bg_rows = nullptr;    // 'background': the sigma-4 axis-0 pass at the sample rows
+  float* bg_nn = nullptr;      //   the samples every 128 pixels
+  BgFit* bg_fit = nullptr;     //   the fitted surface
   // shiftMethod 'fourier': the aligned frames, identity shifts for the stages
   // that read them, and a chunk of column-inverted spectra
   bool fourier = false;
@@ -2057,7 +2329,7 @@ int zr_stage_copy_sums_device(zr_stage_state* s, float* image_sum, float* filt_s
 
 void zr_stage_free(zr_stage_state* s) {
   if (!s) return;
-  void* ptrs[] = {s->edges, s->meas, s->dups, s->hs, s->cdf, s->pdf, s->gtmp, s->aligned, s->fs_id, s->Gf, s->obj, s->info, s->weights, s->resid, s->pos, s->trans_raw, s->trans,
+  void* ptrs[] = {s->edges, s->meas, s->dups, s->hs, s->cdf, s->pdf, s->gtmp, s->bg_rows, s->bg_nn, s->bg_fit, s->aligned, s->fs_id, s->Gf, s->obj, s->info, s->weights, s->resid, s->pos, s->trans_raw, s->trans,
                   s->rel, s->err_stats, s->err_frames, s->fs, s->img_sum, s->even, s->odd, s->filt, s->rowsum,
                   s->twB, s->twB2, s->Tfrc, s->frc2d, s->frc, s->frc_part, s->hann, s->twHp, s->twWp, s->twWp2, s->Tdose, s->G,
                   s->accum, s->thresq, s->R0, s->RH, s->C0, s->CW, s->Py, s->Px};
@@ -2155,8 +2427,15 @@ int zr_stage_create(zr_stage_state** out, const zr_params& p, int N, int H, int 
     ZC(smalloc(s, (void**)&s->accum, sizeof(double) * (N + 1)));
     ZC(smalloc(s, (void**)&s->thresq, sizeof(double) * N));
   }
-  for (int i = 0; i < p.n_filters; ++i)
+  for (int i = 0; i < p.n_filters; ++i) {
     if (p.filters[i] == ZR_FILTER_GAUSSLP && !s->gtmp) ZC(smalloc(s, (void**)&s->gtmp, sizeof(float) * H * W));
+    if (p.filters[i] == ZR_FILTER_BACKGROUND && !s->bg_nn) {
+      const int ny = (H + BG_BIN - 1) / BG_BIN, nx = (W + BG_BIN - 1) / BG_BIN;
+      ZC(smalloc(s, (void**)&s->bg_rows, sizeof(float) * (size_t)ny * W));
+      ZC(smalloc(s, (void**)&s->bg_nn, sizeof(float) * (size_t)ny * nx));
+      ZC(smalloc(s, (void**)&s->bg_fit, sizeof(BgFit)));
+    }
+  }
   s->fourier = p.shift_method == ZR_SHIFT_FOURIER && p.sub_pix_reg > 1;
   if (s->fourier) {
     ZC(smalloc(s, (void**)&s->aligned, sizeof(float) * (size_t)N * H * W));
@@ -2491,6 +2770,43 @@ int zr_stage_gausslp(zr_stage_state* s, hipStream_t st) {
                      s->W, t, s->filt);
   ZS_HIP(hipGetLastError());
   s->have_dose = true;
+  return ZR_OK;
+}
+
+int zr_stage_background(zr_stage_state* s, hipStream_t st) {
+  if (!s || !s->have_sums) return serr(ZR_ERR_STATE, "background before the sums");
+  if (!s->bg_nn) return serr(ZR_ERR_STATE, "background is not among the parameters' filter steps");
+  const int H = s->H, W = s->W;
+  const int ny = (H + BG_BIN - 1) / BG_BIN, nx = (W + BG_BIN - 1) / BG_BIN;
+  BgTaps t;
+  {
+    double sum = 0.0, w[2 * BG_R + 1];
+    for (int j = -BG_R; j <= BG_R; ++j) sum += (w[j + BG_R] = std::exp(-0.5 / 16.0 * (double)(j * j)));
+    for (int j = 0; j <= BG_R; ++j) t.w[j] = w[j + BG_R] / sum;
+  }
+  // no filtSum yet: the estimate starts from a copy of imageSum (zorro/zorro.py:1703-1704)
+  if (!s->have_dose) ZS_HIP(hipMemcpyAsync(s->filt, s->img_sum, sizeof(float) * H * W, hipMemcpyDeviceToDevice, st));
+  hipLaunchKernelGGL(k_bg_rows, dim3((unsigned)(((size_t)ny * W + 255) / 256)), dim3(256), 0, st, s->filt, H, W, ny, t,
+                     s->bg_rows);
+  ZS_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_bg_sample, dim3((ny * nx + 255) / 256), dim3(256), 0, st, s->bg_rows, W, ny, nx, t, s->bg_nn);
+  ZS_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_bg_fit, dim3(1), dim3(BG_NT), 0, st, s->bg_nn, ny, nx, s->bg_fit);
+  ZS_HIP(hipGetLastError());
+  const size_t n = (size_t)H * W;
+  hipLaunchKernelGGL(k_bg_subtract, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, s->filt, H, W, s->bg_fit);
+  ZS_HIP(hipGetLastError());
+  s->have_dose = true;
+  return ZR_OK;
+}
+
+int zr_stage_get_background(zr_stage_state* s, double* p6, double* fval, int* status) {
+  if (!s || !s->bg_fit) return serr(ZR_ERR_STATE, "background is not among the parameters' filter steps");
+  BgFit f;
+  ZS_HIP(hipMemcpy(&f, s->bg_fit, sizeof(f), hipMemcpyDeviceToHost));
+  if (p6) for (int i = 0; i < 6; ++i) p6[i] = f.p[i];
+  if (fval) *fval = f.f;
+  if (status) *status = f.status;
   return ZR_OK;
 }
 

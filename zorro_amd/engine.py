@@ -23,18 +23,18 @@ WMODE = {"autologistic": L.ZR_W_AUTOLOGISTIC, "logistic": L.ZR_W_LOGISTIC, "corr
 
 
 # filterMode steps after the registration (zorro/zorro.py:1692-1711)
-FILTER_STEPS = {"dose": L.ZR_FILTER_DOSE, "dosenorm": L.ZR_FILTER_DOSENORM, "gausslp": L.ZR_FILTER_GAUSSLP}
+FILTER_STEPS = {"dose": L.ZR_FILTER_DOSE, "dosenorm": L.ZR_FILTER_DOSENORM, "gausslp": L.ZR_FILTER_GAUSSLP,
+                "background": L.ZR_FILTER_BACKGROUND}
 
 
 def parse_filter_mode(filterMode):
     """(steps, hot) of a filterMode string as alignImageStack reads it: 'hot'
     anywhere in the string runs hotpixFilter before the registration
     (zorro/zorro.py:1595-1596); the comma-separated tokens, lower-cased with
-    spaces removed, are then applied in order to filtSum (:1692-1711).
-    'background' is not built and raises NotImplementedError; a token that is
-    none of dose, dosenorm, background, gausslp or a 'hot' token raises
-    ValueError (the reference skips it silently, which would hide a misspelt
-    filter)."""
+    spaces removed, are then applied in order to filtSum (:1692-1711).  A
+    token that is none of dose, dosenorm, background, gausslp or a 'hot'
+    token raises ValueError (the reference skips it silently, which would hide
+    a misspelt filter)."""
     if not filterMode:
         return [], False
     text = str(filterMode).lower()
@@ -42,9 +42,6 @@ def parse_filter_mode(filterMode):
     for tok in text.replace(" ", "").split(","):
         if tok in FILTER_STEPS:
             steps.append(tok)
-        elif tok == "background":
-            raise NotImplementedError("filterMode 'background' (a Powell fit of a 2D Gaussian to the binned sum, "
-                                      "zorro/zorro_util.py:497-549) is not built")
         elif tok == "" or "hot" in tok:
             continue
         else:
@@ -293,7 +290,7 @@ class MoviePlan:
         L.check(self.lib.zr_dose_filter(self._mv))
 
     def filter(self, step: str):
-        """One filterMode step ('dose', 'dosenorm', 'gausslp') on filtSum."""
+        """One filterMode step ('dose', 'dosenorm', 'gausslp', 'background') on filtSum."""
         L.check(self.lib.zr_filter(self._mv, FILTER_STEPS[step]))
 
     def post_filters(self):
@@ -397,6 +394,16 @@ class MoviePlan:
         L.check(self.lib.zr_get_peaksig_hist(self._mv, L.fptr(pdf), L.fptr(cdf), L.fptr(hs), nb.value,
                                              C.byref(nb)))
         return dict(pdfPeaks=pdf, cdfPeaks=cdf, hSigma=hs)
+
+    def background_fit(self):
+        """The last 'background' step's fitted surface: dict(p=(offset, centre
+        x, centre y, amplitude, width x, width y), error, status) (status 0:
+        converged and subtracted)."""
+        p = np.zeros(6)
+        fv = C.c_double()
+        st = C.c_int()
+        L.check(self.lib.zr_get_background_fit(self._mv, p.ctypes.data_as(L._D), C.byref(fv), C.byref(st)))
+        return dict(p=p, error=fv.value, status=st.value)
 
     def solver_info(self):
         d = (C.c_int * 6)()
