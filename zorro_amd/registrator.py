@@ -325,13 +325,31 @@ class ImageRegistrator:
         (``zorro/zorro.py:1260-1567``); the equations are the non-zero entries of
         ``corrTriMat_in`` above the diagonal (``:1282``).  Returns the errorDict.
         ``mode``/``Niter`` select the reference's random basin hopping, which the
-        exact solver replaces; ``acceptedEqns`` other than all-true is refused."""
-        if acceptedEqns is not None and not np.all(acceptedEqns):
-            raise NotImplementedError("acceptedEqns with rejected equations")
+        exact solver replaces.
+
+        ``acceptedEqns`` (one flag per equation): all accepted runs as ``None``.
+        With rejected equations the reference zeroes their weights in the
+        minimisation (``:1390``, ``:1413``) and then fails in its weighted error
+        analysis, which multiplies the M weights by the Maccepted accepted
+        errors (``:1485-1487``, a numpy broadcast error for 1 != Maccepted < M);
+        the same ValueError is raised here before any work.  The single accepted
+        equation that numpy would broadcast is refused."""
         corr = np.asarray(corrTriMat_in, np.float32)
         n = corr.shape[0]
         h, w = (int(v) for v in np.shape(self.images)[1:]) if self.images is not None else (64, 64)
         tri = np.triu(corr != 0, 1)
+        if acceptedEqns is not None:
+            acc = np.ravel(np.asarray(acceptedEqns, bool))
+            m_eq = int(np.count_nonzero(tri[:-1, 1:]))
+            if acc.size != m_eq:
+                raise ValueError(f"acceptedEqns has {acc.size} entries for {m_eq} equations")
+            m_acc = int(acc.sum())
+            if m_acc == 1 and m_eq > 1:
+                raise NotImplementedError("acceptedEqns with a single accepted equation")
+            if m_acc != m_eq:
+                raise ValueError(f"operands could not be broadcast together with shapes ({m_eq},) ({m_acc},): "
+                                 "the weighted error analysis of shiftsSolver (zorro/zorro.py:1485) takes the "
+                                 "weights of all equations against the errors of the accepted ones")
         p = self._params()
         p.tri_mode, p.pre_shift = 0, 0      # ZR_TRI_TRI over the given pairs
         plan = MoviePlan(p, n, h, w, device=self.device, triIndices=tri)
