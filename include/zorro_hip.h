@@ -392,6 +392,90 @@ int zr_batch_device_bytes(zr_batch* b, size_t* bytes);
 /* Device pointers of results for zero-copy consumers (valid until destroy). */
 int zr_device_results(zr_movie* mv, const float** image_sum, const float** filt_sum, const double** translations);
 
+/* ---------------------------------------------------------------- serving
+ * A movie server: registration requests of mixed frame counts, shapes and
+ * settings on one device, several in flight at once.  Replaces the reference's
+ * scale-out by one registration subprocess per movie under the Skulk manager
+ * (automator/zorroSkulkManager.py:1089-1113; its per-movie work is
+ * alignImageStack, zorro/zorro.py:1569-1716).  Requests queue in arrival order
+ * and are admitted when an in-flight slot is free and the movie's plan fits the
+ * HBM budget: an idle plan of the same settings and shape is reused, otherwise
+ * one is created after evicting idle plans of other keys (least recently used
+ * first); a movie that fits may go ahead of a larger head of the queue.  Plans
+ * are created and movies run in the server's worker threads; each movie runs
+ * on its plan's own stream, so movies overlap on the device.  Results do not
+ * depend on what else runs.  A movie larger than the whole budget fails with
+ * ZR_ERR_NOMEM. */
+typedef struct zr_server zr_server;
+enum { ZR_REQ_QUEUED = 0, ZR_REQ_RUNNING = 1, ZR_REQ_DONE = 2, ZR_REQ_FAILED = 3, ZR_REQ_CANCELLED = 4 };
+
+/* One movie as a plan operation sees it (outputs written by run). */
+typedef struct zr_serve_job {
+  const zr_params* params;
+  const void* frames;              /* f32 [n][h][w], device memory */
+  int n, h, w;
+  float* image_sum;                /* device f32 [h][w] or NULL */
+  float* filt_sum;                 /* device f32 [h][w] or NULL */
+  double* translations;            /* host f64 [n][2] */
+  double* frc;                     /* host f64 [frc_cap] */
+  int frc_cap;
+  int* frc_n;
+} zr_serve_job;
+
+/* The plan operations the scheduler drives: the built-in ones create zr_movie
+ * plans of this library (zr_server_create); callers may supply stand-ins (a
+ * host-only test, another backend).  create returns the plan and the device
+ * bytes it holds while a movie runs on it; estimate gives an upper bound for a
+ * key before its first plan exists.  Operations run in the server's threads;
+ * they return 0 or a ZR_ERR_* code. */
+typedef struct zr_plan_ops {
+  void* ctx;
+  size_t (*estimate)(void* ctx, const zr_params* p, int n, int h, int w);
+  int (*create)(void* ctx, const zr_params* p, int n, int h, int w, void** plan, size_t* bytes);
+  int (*run)(void* ctx, void* plan, const zr_serve_job* job);
+  void (*destroy)(void* ctx, void* plan);
+} zr_plan_ops;
+
+/* hbm_budget 0: 85% of the device's free memory now; max_inflight 1..64. */
+int zr_server_create(int device, size_t hbm_budget, int max_inflight, zr_server** out);
+/* A server over caller-supplied plan operations (hbm_budget must be given). */
+int zr_server_create_ops(const zr_plan_ops* ops, size_t hbm_budget, int max_inflight, zr_server** out);
+/* Finishes every queued and running request, frees every plan and the server. */
+void zr_server_destroy(zr_server* s);
+/* Queue one movie.  frames: device f32 [n][h][w] that the caller keeps alive
+ * and unchanged until the request finishes; image_sum / filt_sum: device f32
+ * [h][w] buffers the results are copied into (NULL: not wanted); arrival: the
+ * arrival time in seconds of CLOCK_MONOTONIC (0: now), from which latency is
+ * measured.  *id receives the request's id. */
+int zr_server_submit(zr_server* s, const zr_params* p, int n, int h, int w, const void* frames, float* image_sum,
+                     float* filt_sum, double arrival, long long* id);
+/* Create up to `count` idle plans for movies of this key ahead of the requests;
+ * *made receives how many fit the budget. */
+int zr_server_warm(zr_server* s, const zr_params* p, int n, int h, int w, int count, int* made);
+/* Cancel a request that is still queued (ZR_ERR_STATE once it is admitted). */
+int zr_server_cancel(zr_server* s, long long id);
+/* *state = ZR_REQ_* of a request. */
+int zr_server_poll(zr_server* s, long long id, int* state);
+/* Wait until the request has finished (done, failed or cancelled) or timeout_s
+ * seconds have passed (< 0: no limit); *state as zr_server_poll. */
+int zr_server_wait(zr_server* s, long long id, double timeout_s, int* state);
+/* Wait for the next request to finish, in completion order; *id = 0 when none
+ * finished within timeout_s. */
+int zr_server_wait_any(zr_server* s, double timeout_s, long long* id, int* state);
+/* Results of a finished request: translations f64 [n][2] (the request's n),
+ * FRC f64 [*nfrc] (at most cap), times3 = arrival, admission and completion
+ * (CLOCK_MONOTONIC seconds), the failure message.  Returns the request's own
+ * code: 0 done, its ZR_ERR_* code when it failed, ZR_ERR_STATE when cancelled. */
+int zr_server_result(zr_server* s, long long id, double* translations, int n, double* frc, int cap, int* nfrc,
+                     double* times3, char* error, int error_cap);
+/* Forget a finished request. */
+int zr_server_release(zr_server* s, long long id);
+/* out8 = admitted, plans created, plans evicted, peak bytes held, bytes held,
+ * in flight, queued, failed. */
+int zr_server_stats(zr_server* s, long long* out8);
+/* The server's upper estimate of a movie plan's device bytes. */
+size_t zr_server_estimate_bytes(const zr_params* p, int n, int h, int w);
+
 #ifdef __cplusplus
 }
 #endif

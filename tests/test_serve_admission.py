@@ -1,6 +1,8 @@
-"""Admission logic of the movie server (host code, no GPU): in-flight limit,
-HBM budget with LRU eviction of idle plans, reuse of plans of the same key,
-and refusal of a movie larger than the budget.  Plans are stand-ins that
+"""Admission logic of the native movie server (zr_server_*, host code, no
+GPU): in-flight limit, HBM budget with LRU eviction of idle plans, reuse of
+plans of the same key, refusal of a movie larger than the budget,
+cancellation, and plan creation outside the scheduler's lock.  Plans are
+Python stand-ins, driven by the native scheduler through zr_plan_ops, that
 record their life cycle; the GPU behaviour is in tests/test_gpu_serve.py."""
 import threading
 import time
@@ -66,8 +68,7 @@ class FakePlan:
 
 
 @pytest.fixture
-def fake(monkeypatch):
-    monkeypatch.setattr(S, "MoviePlan", FakePlan)
+def fake():
     FakePlan.live = FakePlan.created = FakePlan.running = FakePlan.peak_running = 0
     return FakePlan
 
@@ -80,7 +81,7 @@ def movie(n, h, w, tag):
 
 
 def test_results_in_flight_limit_and_plan_reuse(fake):
-    srv = S.MovieServer(hbm_budget=1 << 40, max_inflight=2, return_sums="none")
+    srv = S.MovieServer(hbm_budget=1 << 40, max_inflight=2, return_sums="none", plan_class=FakePlan)
     futs = [srv.submit(movie(4, 64, 64, i), shapePadded=(64, 64), **KW) for i in range(8)]
     res = [f.result(timeout=30) for f in futs]
     srv.close()
@@ -93,7 +94,7 @@ def test_results_in_flight_limit_and_plan_reuse(fake):
 
 def test_budget_evicts_idle_plans_of_other_shapes(fake):
     one = 4 * 64 * 64 * (FakePlan.bytes_per_pixel + 4)          # plan bytes + frames, as the server charges
-    srv = S.MovieServer(hbm_budget=int(one * 1.5), max_inflight=2, return_sums="none")
+    srv = S.MovieServer(hbm_budget=int(one * 1.5), max_inflight=2, return_sums="none", plan_class=FakePlan)
     shapes = [(64, 64), (64, 64), (64, 64)]
     a = [srv.submit(movie(4, h, w, i), shapePadded=(h, w), **KW) for i, (h, w) in enumerate(shapes)]
     b = srv.submit(movie(4, 64, 64, 9), shapePadded=(128, 128), **KW)    # another key of the same size
@@ -106,7 +107,7 @@ def test_budget_evicts_idle_plans_of_other_shapes(fake):
 
 
 def test_movie_larger_than_budget_is_refused(fake):
-    srv = S.MovieServer(hbm_budget=1 << 20, max_inflight=2, return_sums="none")
+    srv = S.MovieServer(hbm_budget=1 << 20, max_inflight=2, return_sums="none", plan_class=FakePlan)
     f = srv.submit(movie(4, 512, 512, 1), shapePadded=(512, 512), **KW)
     with pytest.raises(MemoryError):
         f.result(timeout=30)
@@ -129,10 +130,9 @@ class SlowAlignPlan(FakePlan):
         time.sleep(0.2)
 
 
-def test_cancelled_request_is_dropped_and_released_once(monkeypatch):
-    monkeypatch.setattr(S, "MoviePlan", SlowAlignPlan)
+def test_cancelled_request_is_dropped_and_released_once():
     FakePlan.live = FakePlan.created = FakePlan.running = FakePlan.peak_running = 0
-    srv = S.MovieServer(hbm_budget=1 << 40, max_inflight=1, return_sums="none")
+    srv = S.MovieServer(hbm_budget=1 << 40, max_inflight=1, return_sums="none", plan_class=SlowAlignPlan)
     a = srv.submit(movie(4, 64, 64, 1), shapePadded=(64, 64), **KW)
     b = srv.submit(movie(4, 64, 64, 2), shapePadded=(64, 64), **KW)
     assert b.cancel()                           # still queued behind a: cancellable
@@ -141,8 +141,8 @@ def test_cancelled_request_is_dropped_and_released_once(monkeypatch):
     assert np.all(c.result(timeout=30).translations == 3)
     assert b.cancelled()
     assert not a.cancel()                        # a finished: too late to cancel
-    with srv._lock:
-        assert srv._inflight == 0 and srv.stats["admitted"] == 2
+    st = srv.stats
+    assert st["inflight"] == 0 and st["queued"] == 0 and st["admitted"] == 2
     srv.close()
     assert FakePlan.live == 0
 
@@ -154,10 +154,9 @@ class SlowCreatePlan(FakePlan):
         super().__init__(params, n, h, w, device)
 
 
-def test_slow_plan_creation_does_not_hold_up_other_movies(monkeypatch):
-    monkeypatch.setattr(S, "MoviePlan", SlowCreatePlan)
+def test_slow_plan_creation_does_not_hold_up_other_movies():
     FakePlan.live = FakePlan.created = FakePlan.running = FakePlan.peak_running = 0
-    srv = S.MovieServer(hbm_budget=1 << 40, max_inflight=2, return_sums="none")
+    srv = S.MovieServer(hbm_budget=1 << 40, max_inflight=2, return_sums="none", plan_class=SlowCreatePlan)
     assert srv.warm((4, 64, 64), count=1, shapePadded=(64, 64), **KW) == 1
     t0 = time.perf_counter()
     big = srv.submit(movie(4, 128, 128, 5), shapePadded=(128, 128), **KW)    # admitted first, plan created slowly
@@ -171,3 +170,55 @@ def test_slow_plan_creation_does_not_hold_up_other_movies(monkeypatch):
     assert np.all(big.result(timeout=30).translations == 5)
     srv.close()
     assert FakePlan.live == 0
+
+
+def test_smaller_movie_goes_ahead_of_a_head_that_does_not_fit():
+    FakePlan.live = FakePlan.created = FakePlan.running = FakePlan.peak_running = 0
+    one = 4 * 64 * 64 * (FakePlan.bytes_per_pixel + 4)
+    srv = S.MovieServer(hbm_budget=int(one * 2.5), max_inflight=2, return_sums="none", plan_class=SlowAlignPlan)
+    a = srv.submit(movie(4, 64, 64, 1), shapePadded=(64, 64), **KW)
+    time.sleep(0.05)                               # a is running on the first plan
+    # b: a new key, charged the server's upper estimate until its first plan exists,
+    # which cannot be admitted while anything else is held
+    b = srv.submit(movie(4, 64, 64, 2), shapePadded=(128, 128), **KW)
+    c = srv.submit(movie(4, 64, 64, 3), shapePadded=(64, 64), **KW)    # a's key: a second plan fits
+    rb, rc = b.result(timeout=30), c.result(timeout=30)
+    assert rc.admitted < rb.admitted
+    assert np.all(rb.translations == 2) and np.all(rc.translations == 3)
+    a.result(timeout=30)
+    srv.close()
+    assert FakePlan.live == 0
+
+
+def test_c_abi_poll_wait_result_release():
+    import ctypes as C
+    from zorro_amd import _lib as L
+    lib = L.load()
+    ops = S.PythonPlanOps(SlowAlignPlan)
+    h = C.c_void_p()
+    L.check(lib.zr_server_create_ops(C.byref(ops.ops), C.c_size_t(1 << 40), 1, C.byref(h)))
+    try:
+        frames = movie(3, 32, 32, 7)
+        params = S.make_params(shapePadded=(32, 32), **KW)
+        rid = C.c_longlong()
+        L.check(lib.zr_server_submit(h, C.byref(params), 3, 32, 32, C.c_void_p(frames.ctypes.data), None, None,
+                                     C.c_double(0.0), C.byref(rid)))
+        st = C.c_int()
+        L.check(lib.zr_server_poll(h, rid, C.byref(st)))
+        assert st.value in (L.ZR_REQ_QUEUED, L.ZR_REQ_RUNNING)
+        t = np.zeros((3, 2))
+        assert lib.zr_server_result(h, rid, L.dptr(t), 3, None, 0, None, None, None, 0) == L.ZR_ERR_STATE
+        assert lib.zr_server_poll(h, C.c_longlong(rid.value + 99), C.byref(st)) == L.ZR_ERR_ARG
+        L.check(lib.zr_server_wait(h, rid, C.c_double(-1.0), C.byref(st)))
+        assert st.value == L.ZR_REQ_DONE
+        times = np.zeros(3)
+        L.check(lib.zr_server_result(h, rid, L.dptr(t), 3, None, 0, None, L.dptr(times), None, 0))
+        assert np.all(t == 7) and times[2] >= times[1] >= times[0] > 0
+        assert lib.zr_server_cancel(h, rid) == L.ZR_ERR_STATE          # finished: too late
+        L.check(lib.zr_server_release(h, rid))
+        assert lib.zr_server_poll(h, rid, C.byref(st)) == L.ZR_ERR_ARG
+        v = (C.c_longlong * 8)()
+        L.check(lib.zr_server_stats(h, v))
+        assert v[0] == 1 and v[1] == 1 and v[5] == 0 and v[6] == 0
+    finally:
+        lib.zr_server_destroy(h)

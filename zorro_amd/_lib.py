@@ -48,6 +48,31 @@ class ZrParams(C.Structure):
     ]
 
 
+ZR_REQ_QUEUED, ZR_REQ_RUNNING, ZR_REQ_DONE, ZR_REQ_FAILED, ZR_REQ_CANCELLED = 0, 1, 2, 3, 4
+
+
+class ZrServeJob(C.Structure):
+    """zr_serve_job: one movie as a plan operation of the server sees it."""
+    _fields_ = [
+        ("params", C.POINTER(ZrParams)), ("frames", C.c_void_p), ("n", C.c_int), ("h", C.c_int), ("w", C.c_int),
+        ("image_sum", C.c_void_p), ("filt_sum", C.c_void_p), ("translations", C.POINTER(C.c_double)),
+        ("frc", C.POINTER(C.c_double)), ("frc_cap", C.c_int), ("frc_n", C.POINTER(C.c_int)),
+    ]
+
+
+PLAN_ESTIMATE = C.CFUNCTYPE(C.c_size_t, C.c_void_p, C.POINTER(ZrParams), C.c_int, C.c_int, C.c_int)
+PLAN_CREATE = C.CFUNCTYPE(C.c_int, C.c_void_p, C.POINTER(ZrParams), C.c_int, C.c_int, C.c_int,
+                          C.POINTER(C.c_void_p), C.POINTER(C.c_size_t))
+PLAN_RUN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_void_p, C.POINTER(ZrServeJob))
+PLAN_DESTROY = C.CFUNCTYPE(None, C.c_void_p, C.c_void_p)
+
+
+class ZrPlanOps(C.Structure):
+    """zr_plan_ops: the plan operations a server drives."""
+    _fields_ = [("ctx", C.c_void_p), ("estimate", PLAN_ESTIMATE), ("create", PLAN_CREATE), ("run", PLAN_RUN),
+                ("destroy", PLAN_DESTROY)]
+
+
 class ZrError(RuntimeError):
     def __init__(self, code, msg):
         super().__init__(f"libzorro_hip error {code}: {msg}")
@@ -113,6 +138,20 @@ SIGNATURES = {
     "zr_movie_info": (C.c_int, [_P, _I]),
     "zr_movie_device_bytes": (C.c_int, [_P, C.POINTER(C.c_size_t)]),
     "zr_copy_sums_device": (C.c_int, [_P, _P, _P, _P, _P]),
+    "zr_server_create": (C.c_int, [C.c_int, C.c_size_t, C.c_int, C.POINTER(_P)]),
+    "zr_server_create_ops": (C.c_int, [C.POINTER(ZrPlanOps), C.c_size_t, C.c_int, C.POINTER(_P)]),
+    "zr_server_destroy": (None, [_P]),
+    "zr_server_submit": (C.c_int, [_P, C.POINTER(ZrParams), C.c_int, C.c_int, C.c_int, _P, _P, _P, C.c_double,
+                                   C.POINTER(C.c_longlong)]),
+    "zr_server_warm": (C.c_int, [_P, C.POINTER(ZrParams), C.c_int, C.c_int, C.c_int, C.c_int, _I]),
+    "zr_server_cancel": (C.c_int, [_P, C.c_longlong]),
+    "zr_server_poll": (C.c_int, [_P, C.c_longlong, _I]),
+    "zr_server_wait": (C.c_int, [_P, C.c_longlong, C.c_double, _I]),
+    "zr_server_wait_any": (C.c_int, [_P, C.c_double, C.POINTER(C.c_longlong), _I]),
+    "zr_server_result": (C.c_int, [_P, C.c_longlong, _D, C.c_int, _D, C.c_int, _I, _D, C.c_char_p, C.c_int]),
+    "zr_server_release": (C.c_int, [_P, C.c_longlong]),
+    "zr_server_stats": (C.c_int, [_P, C.POINTER(C.c_longlong)]),
+    "zr_server_estimate_bytes": (C.c_size_t, [C.POINTER(ZrParams), C.c_int, C.c_int, C.c_int]),
 }
 
 STAGES = ["spectra", "pairs", "peaks", "solve", "shifts", "dose", "frc", "input", "tail"]

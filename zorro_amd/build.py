@@ -15,7 +15,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libzorro_hip.so")
-SOURCES = ["zr_xcorr.hip", "zr_stages.hip", "zr_bin.hip", "zr_hot.hip", "zr_capi.hip", "zr_frc_tiles.hip"]
+SOURCES = ["zr_xcorr.hip", "zr_stages.hip", "zr_bin.hip", "zr_hot.hip", "zr_capi.hip", "zr_frc_tiles.hip",
+           "zr_server.hip"]
 ARCH = os.environ.get("ZR_OFFLOAD_ARCH", "gfx950")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 

@@ -1,4 +1,4 @@
-"""Continuous-batching movie server on one MI355X.
+"""Continuous-batching movie server on one MI355X (Python face of zr_server).
 
 The reference registers one movie per process and scales by running several
 processes (the Skulk manager, ``automator/zorroSkulkManager.py:917-1203``).
@@ -7,35 +7,33 @@ movies at once, each on its own HIP stream, so that the latency-bound stages
 of one movie (the one-block shift solver, the logistic fit) overlap with the
 bandwidth-bound pair transforms of another.
 
-* Requests (frames + ``ImageRegistrator`` settings) queue in arrival order.
-* A request is admitted when an in-flight slot is free and its plan fits the
-  HBM budget: an idle plan of the same shape and settings is reused, otherwise
-  a new one is created, evicting idle plans of other shapes (least recently
-  used first) when the budget requires it.  A request that does not fit behind
-  a larger head of the queue may go first ("admitted as slots and memory free
-  up"); nothing is starved, because admission always retries the oldest
-  request first.
-* Each admitted movie runs ``zr_align`` on its plan's stream in a worker
-  thread (the C calls release the GIL), copies its sums into caller-owned
-  device tensors and its translations to the host, and frees the plan for the
-  next request.
+The scheduler is native (``zr_server_*`` in ``include/zorro_hip.h``): requests
+queue in arrival order and are admitted when an in-flight slot is free and
+their plan fits the HBM budget (an idle plan of the same key is reused,
+otherwise one is created after evicting idle plans of other keys, least
+recently used first; a movie that fits may go ahead of a larger head of the
+queue); plans are created and movies run in the server's worker threads.  This
+module turns requests into futures: ``submit`` queues a movie and returns a
+``concurrent.futures.Future`` of a ``MovieResult``; a completion thread waits
+on the server (the ctypes call releases the GIL) and settles the futures.
 
-Results do not depend on what else is in flight: every kernel of a movie
-reads and writes only that movie's buffers, and no reduction depends on
-scheduling (tests/test_gpu_serve.py checks alone == in the mix).
+Results do not depend on what else is in flight: every kernel of a movie reads
+and writes only that movie's buffers, and no reduction depends on scheduling
+(tests/test_gpu_serve.py checks alone == in the mix).  ``plan_class`` swaps
+the plans for host-side stand-ins with MoviePlan's interface, which drive the
+same native scheduler without a GPU (tests/test_serve_admission.py).
 """
 from __future__ import annotations
 
-import collections
 import ctypes as C
 import threading
 import time
-from concurrent.futures import Future, ThreadPoolExecutor
+from concurrent.futures import Future
 
 import numpy as np
 
 from . import _lib as L
-from .engine import MoviePlan, make_params
+from .engine import make_params
 
 
 def plan_key(n, h, w, params):
@@ -43,40 +41,10 @@ def plan_key(n, h, w, params):
 
 
 def estimate_plan_bytes(n, h, w, params) -> int:
-    """Upper estimate of a plan's device memory before it exists (the exact
-    figure, ``zr_movie_device_bytes``, replaces it once a plan of the key has
-    been created)."""
-    hp = params.shape_padded[0] or h
-    wp = params.shape_padded[1] or w
-    crop = params.fou_crop[0] != 0 or params.fou_crop[1] != 0
-    hc = 2 * (params.fou_crop[0] // 2) if crop else hp
-    wc = 2 * (params.fou_crop[1] // 2) if crop else wp
-    m = params.max_shift
-    masked = (hp != h or wp != w)
-    npairs = n * (n - 1) // 2
-    nspec = 2 * n + 1 if masked else n
-    ptot = npairs + (4 * n + 1 if masked else 0)
-    wk = wp // 2 + 1
-    spectral = params.do_dose and not masked and h == w
-    hs = hp if spectral else hc
-    b = min(n, 8) * wk * hp * 8                       # row-pass scratch
-    b += nspec * wk * hs * 8 + nspec * wk * 8          # spectra
-    b += ptot * (wc // 2 + 1) * 2 * m * 8              # pair pass 1
-    b += ptot * 4 * m * m * 4                          # lag windows
-    b += 4 * h * w * 4 + n * h * w * 4                 # sums + an owned copy of the frames
-    if any(params.filters[i] == L.ZR_FILTER_GAUSSLP for i in range(params.n_filters)):
-        b += h * w * 4                                 # the Gaussian's first pass
-    if params.do_dose:
-        b += wk * hp * 8
-        if not spectral:
-            b += min(n, 8) * (wk + 1) * hp * 8         # a chunk of the direct dose path
-        else:
-            b += 3 * n * (wk + hp) * 8
-    if params.do_frc:
-        box = params.frc_box
-        tiles = max(1, (h // (box // 2)) * (w // (box // 2)))
-        b += tiles * 2 * (box // 2 + 1) * box * 8
-    return int(b * 1.05) + (16 << 20)
+    """The server's upper estimate of a plan's device memory before it exists
+    (``zr_server_estimate_bytes``; the exact figure, ``zr_movie_device_bytes``,
+    replaces it once a plan of the key has been created)."""
+    return int(L.load().zr_server_estimate_bytes(C.byref(params), int(n), int(h), int(w)))
 
 
 class MovieResult:
@@ -95,17 +63,83 @@ class MovieResult:
         return self.done - self.arrival
 
 
-class _Request:
-    __slots__ = ("frames", "n", "h", "w", "params", "key", "future", "arrival", "admitted")
+class PythonPlanOps:
+    """zr_plan_ops over a Python plan class with MoviePlan's interface
+    (``cls(params, n, h, w, device=)``, ``set_frames``, ``align``,
+    ``synchronize``, ``solution``, ``frc_curve``, ``close`` and
+    ``lib.zr_movie_device_bytes``).  Frames reach the plan as a numpy view of
+    the submitted host array."""
 
-    def __init__(self, frames, params, arrival, shape=None):
-        self.frames = frames
-        self.n, self.h, self.w = (int(v) for v in (shape or frames.shape))
-        self.params = params
-        self.key = plan_key(self.n, self.h, self.w, params)
-        self.future = Future()
-        self.arrival = arrival
-        self.admitted = None
+    def __init__(self, plan_class, device=0):
+        self.cls = plan_class
+        self.device = device
+        self.plans = {}
+        self.lock = threading.Lock()
+        self.next = 1
+        self.ops = L.ZrPlanOps(None, L.PLAN_ESTIMATE(self._estimate), L.PLAN_CREATE(self._create),
+                               L.PLAN_RUN(self._run), L.PLAN_DESTROY(self._destroy))
+
+    @staticmethod
+    def _estimate(ctx, params, n, h, w):
+        return estimate_plan_bytes(n, h, w, params.contents)
+
+    def _create(self, ctx, params, n, h, w, plan_out, bytes_out):
+        try:
+            p = L.ZrParams()
+            C.memmove(C.byref(p), params, C.sizeof(L.ZrParams))
+            plan = self.cls(p, n, h, w, device=self.device)
+            nb = C.c_size_t()
+            L.check(plan.lib.zr_movie_device_bytes(plan._mv, C.byref(nb)))
+            with self.lock:
+                handle, self.next = self.next, self.next + 1
+                self.plans[handle] = plan
+            plan_out[0] = handle
+            bytes_out[0] = int(nb.value) + n * h * w * 4
+            return L.ZR_OK
+        except MemoryError:
+            return L.ZR_ERR_NOMEM
+        except Exception:
+            return L.ZR_ERR_HIP
+
+    def _run(self, ctx, handle, job):
+        try:
+            j = job.contents
+            plan = self.plans[handle]
+            frames = np.ctypeslib.as_array(C.cast(j.frames, C.POINTER(C.c_float)), shape=(j.n, j.h, j.w))
+            plan.set_frames(frames)
+            plan.align()
+            plan.synchronize()
+            t = np.asarray(plan.solution()["translations"], np.float64).reshape(j.n, 2)
+            np.ctypeslib.as_array(j.translations, shape=(j.n, 2))[:] = t
+            if j.params.contents.do_frc and j.frc_n:
+                f = np.asarray(plan.frc_curve(), np.float64)[: j.frc_cap]
+                np.ctypeslib.as_array(j.frc, shape=(j.frc_cap,))[: f.size] = f
+                j.frc_n[0] = f.size
+            return L.ZR_OK
+        except Exception:
+            return L.ZR_ERR_HIP
+
+    def _destroy(self, ctx, handle):
+        with self.lock:
+            plan = self.plans.pop(handle, None)
+        if plan is not None:
+            plan.close()
+
+
+class _ServerFuture(Future):
+    """A request's future: cancel() succeeds while the request is still queued
+    in the native server."""
+
+    def __init__(self, server, rid):
+        super().__init__()
+        self._server, self._rid = server, rid
+
+    def cancel(self):
+        if self.done():
+            return super().cancel()
+        if self._server._lib.zr_server_cancel(self._server._h, C.c_longlong(self._rid)) != L.ZR_OK:
+            return False
+        return super().cancel()
 
 
 class MovieServer:
@@ -113,124 +147,82 @@ class MovieServer:
 
     ``hbm_budget``: bytes of device memory the server may hold in plans
     (default: 85% of the free memory at start).  ``max_inflight``: movies
-    running at once (each on its own stream)."""
+    running at once (each on its own stream).  ``return_sums``: "device"
+    (torch tensors), "host" (numpy) or "none"."""
+
+    FRC_CAP = 2048
 
     def __init__(self, device: int = 0, hbm_budget: int | None = None, max_inflight: int = 3,
-                 return_sums: str = "device"):
-        import torch
-        self.torch = torch
+                 return_sums: str = "device", plan_class=None):
+        self._lib = L.load()
         self.device = int(device)
-        self.dev = torch.device("cuda", self.device)
-        if hbm_budget is None:
-            free, _ = torch.cuda.mem_get_info(self.dev)
-            hbm_budget = int(free * 0.85)
-        self.budget = int(hbm_budget)
         self.max_inflight = int(max_inflight)
-        self.return_sums = return_sums          # "device" (torch tensors), "host" (numpy) or "none"
-        self._lock = threading.Condition()
-        self._pending = collections.deque()
-        self._idle = collections.OrderedDict()  # key -> [plans], LRU order of keys
-        self._plan_bytes = {}                   # id(plan) -> bytes
-        self._key_bytes = {}                    # key -> measured bytes of one plan
-        self._held = 0                          # bytes in all plans
-        self._inflight = 0
-        self._creating = 0                      # plans being created outside the lock
+        self.return_sums = return_sums
+        self._ops = PythonPlanOps(plan_class, self.device) if plan_class is not None else None
+        if self._ops is None:
+            import torch
+            self.torch = torch
+            self.dev = torch.device("cuda", self.device)
+            if hbm_budget is None:
+                free, _ = torch.cuda.mem_get_info(self.dev)
+                hbm_budget = int(free * 0.85)
+        elif hbm_budget is None:
+            raise ValueError("a server over stand-in plans needs an HBM budget")
+        self.budget = int(hbm_budget)
+        h = C.c_void_p()
+        if self._ops is None:
+            L.check(self._lib.zr_server_create(self.device, C.c_size_t(self.budget), self.max_inflight, C.byref(h)))
+        else:
+            L.check(self._lib.zr_server_create_ops(C.byref(self._ops.ops), C.c_size_t(self.budget),
+                                                   self.max_inflight, C.byref(h)))
+        self._h = h
+        self._lock = threading.Lock()
+        self._outstanding = {}                 # id -> (future, request record)
+        self._closing = False
         self._closed = False
-        self._pool = ThreadPoolExecutor(max_workers=self.max_inflight, thread_name_prefix="zr-movie")
-        self._sched = threading.Thread(target=self._schedule, name="zr-admit", daemon=True)
-        self.stats = dict(admitted=0, plans_created=0, plans_evicted=0, peak_held=0)
-        self._sched.start()
+        self._done_thread = threading.Thread(target=self._complete_loop, name="zr-complete", daemon=True)
+        self._done_thread.start()
 
     # ------------------------------------------------------------ public
     def submit(self, frames, arrival: float | None = None, **settings) -> Future:
         """Queue one movie: frames f32 [N,H,W] (numpy or a CUDA tensor kept
         alive by the caller until the future completes) and ImageRegistrator
-        settings as keywords (``engine.make_params``)."""
+        settings as keywords (``engine.make_params``).  ``arrival`` is a
+        ``time.perf_counter()`` value (the server's clock; default now)."""
         params = make_params(**settings)
-        req = _Request(frames, params, time.perf_counter() if arrival is None else arrival)
+        n, h, w = (int(v) for v in frames.shape)
+        keep = [frames]
+        if self._ops is None:
+            torch = self.torch
+            if not isinstance(frames, torch.Tensor):
+                frames = torch.from_numpy(np.ascontiguousarray(frames, np.float32)).to(self.dev)
+                keep.append(frames)
+            if frames.dtype != torch.float32 or not frames.is_contiguous() or frames.device != self.dev:
+                raise ValueError(f"frames must be a contiguous float32 tensor on {self.dev}")
+            fptr = frames.data_ptr()
+            img = filt = None
+            if self.return_sums != "none":
+                img = torch.empty((h, w), dtype=torch.float32, device=self.dev)
+                if params.n_filters or params.do_dose:
+                    filt = torch.empty((h, w), dtype=torch.float32, device=self.dev)
+        else:
+            a = np.ascontiguousarray(frames, np.float32)
+            keep.append(a)
+            fptr = a.ctypes.data
+            img = filt = None
+        rid = C.c_longlong()
         with self._lock:
-            if self._closed:
+            if self._closing:
                 raise RuntimeError("server is closed")
-            self._pending.append(req)
-            self._lock.notify_all()
-        return req.future
-
-    def close(self):
-        with self._lock:
-            self._closed = True
-            self._lock.notify_all()
-        self._sched.join()
-        self._pool.shutdown(wait=True)
-        with self._lock:
-            for plans in self._idle.values():
-                for p in plans:
-                    p.close()
-            self._idle.clear()
-            self._held = 0
-
-    def __enter__(self):
-        return self
-
-    def __exit__(self, *exc):
-        self.close()
-
-    # ------------------------------------------------------------ admission
-    def _bytes_for(self, req):
-        return self._key_bytes.get(req.key) or estimate_plan_bytes(req.n, req.h, req.w, req.params)
-
-    def _evict_for(self, need):
-        """Free idle plans (least recently used keys first) until `need` bytes fit."""
-        while self._held + need > self.budget and self._idle:
-            key, plans = next(iter(self._idle.items()))
-            p = plans.pop()
-            if not plans:
-                del self._idle[key]
-            self._held -= self._plan_bytes.pop(id(p), 0)
-            p.close()
-            self.stats["plans_evicted"] += 1
-        return self._held + need <= self.budget
-
-    def _take_plan(self, req):
-        """Called under the lock.  ('idle', plan) for an idle plan of the
-        request's key, ('create', bytes) when a new plan may be created (its
-        bytes are then reserved), or None if neither is possible now."""
-        plans = self._idle.get(req.key)
-        if plans:
-            p = plans.pop()
-            if not plans:
-                del self._idle[req.key]
-            return "idle", p
-        known = req.key in self._key_bytes
-        need = self._bytes_for(req)
-        if not self._evict_for(need):
-            # the first plan of a key is charged an upper estimate; with nothing
-            # else held, create it and check the exact figure instead
-            if known or self._held > 0:
-                return None
-        self._held += need
-        return "create", need
-
-    def _create_plan(self, req, reserved):
-        """Create a plan outside the lock (multi-GB allocations and table
-        uploads must not hold up completions), then settle its exact size."""
-        p = MoviePlan(req.params, req.n, req.h, req.w, device=self.device)
-        nb = C.c_size_t()
-        L.check(p.lib.zr_movie_device_bytes(p._mv, C.byref(nb)))
-        actual = int(nb.value) + req.n * req.h * req.w * 4
-        with self._lock:
-            self._key_bytes[req.key] = actual
-            self._held += actual - reserved
-            if self._held > self.budget:
-                self._evict_for(0)
-            if self._held > self.budget and self._held == actual:
-                self._held -= actual
-                p.close()
-                raise MemoryError(f"movie {req.n}x{req.h}x{req.w} needs {actual / 1e9:.2f} GB, more than the "
-                                  f"HBM budget {self.budget / 1e9:.2f} GB")
-            self._plan_bytes[id(p)] = actual
-            self.stats["plans_created"] += 1
-            self.stats["peak_held"] = max(self.stats["peak_held"], self._held)
-        return p
+            L.check(self._lib.zr_server_submit(
+                self._h, C.byref(params), n, h, w, C.c_void_p(fptr),
+                C.c_void_p(img.data_ptr() if img is not None else 0),
+                C.c_void_p(filt.data_ptr() if filt is not None else 0),
+                C.c_double(time.perf_counter() if arrival is None else float(arrival)), C.byref(rid)))
+            fut = _ServerFuture(self, rid.value)
+            self._outstanding[rid.value] = (fut, dict(keep=keep, img=img, filt=filt, n=n, params=params,
+                                                      key=plan_key(n, h, w, params)))
+        return fut
 
     def warm(self, frames_like, count: int = 1, **settings):
         """Create ``count`` idle plans for movies shaped like ``frames_like``
@@ -238,128 +230,79 @@ class MovieServer:
         requests (serving warm-up: no request then waits for a plan)."""
         shape = tuple(int(v) for v in getattr(frames_like, "shape", frames_like))
         params = make_params(**settings)
-        req = _Request(np.empty((0,), np.float32), params, 0.0, shape=shape)
-        made = []
-        for _ in range(int(count)):
-            with self._lock:
-                if not self._evict_for(self._bytes_for(req)) and self._held > 0:
-                    break
-                need = self._bytes_for(req)
-                self._held += need
-            made.append(self._create_plan(req, need))
-        with self._lock:
-            for p in made:
-                self._idle.setdefault(req.key, []).append(p)
-            if made:
-                self._idle.move_to_end(req.key)
-            self._lock.notify_all()
-        return len(made)
+        made = C.c_int()
+        L.check(self._lib.zr_server_warm(self._h, C.byref(params), *shape, int(count), C.byref(made)))
+        return made.value
 
-    def _schedule(self):
+    @property
+    def stats(self):
+        v = (C.c_longlong * 8)()
+        L.check(self._lib.zr_server_stats(self._h, v))
+        keys = ("admitted", "plans_created", "plans_evicted", "peak_held", "held", "inflight", "queued", "failed")
+        return dict(zip(keys, (int(x) for x in v)))
+
+    def close(self):
+        """Finish every queued and running movie, then free the plans."""
+        with self._lock:
+            if self._closed:
+                return
+            self._closing = True
+            futs = [f for f, _ in self._outstanding.values()]
+        for f in futs:
+            try:
+                f.exception()
+            except BaseException:
+                pass
+        self._done_thread.join()
+        self._lib.zr_server_destroy(self._h)
+        self._closed = True
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # ------------------------------------------------------------ completion
+    def _complete_loop(self):
+        rid, state = C.c_longlong(), C.c_int()
         while True:
-            with self._lock:
-                while not self._closed and (not self._pending or self._inflight >= self.max_inflight):
-                    self._lock.wait()
-                if self._closed and not self._pending:
-                    return
-                if self._inflight >= self.max_inflight:
-                    continue
-                # requests cancelled while queued are dropped (Future.cancel succeeds
-                # until the request is admitted)
-                for req in [r for r in self._pending if r.future.cancelled()]:
-                    self._pending.remove(req)
-                chosen, got = None, None
-                for req in list(self._pending):
-                    got = self._take_plan(req)
-                    if got is not None:
-                        chosen = req
-                        break
-                if chosen is None:
-                    if self._inflight == 0 and self._pending and self._creating == 0:
-                        # nothing running and nothing fits even after eviction
-                        req = self._pending.popleft()
-                        if req.future.set_running_or_notify_cancel():
-                            req.future.set_exception(MemoryError(
-                                f"movie {req.n}x{req.h}x{req.w} needs ~{self._bytes_for(req) / 1e9:.1f} GB, "
-                                f"more than the HBM budget {self.budget / 1e9:.1f} GB"))
-                        continue
-                    self._lock.wait()
-                    continue
-                self._pending.remove(chosen)
-                if not chosen.future.set_running_or_notify_cancel():
-                    # cancelled between the sweep and now: give the plan or the reservation back
-                    if got[0] == "idle":
-                        self._idle.setdefault(chosen.key, []).append(got[1])
-                    else:
-                        self._held -= got[1]
-                    continue
-                chosen.admitted = time.perf_counter()
-                self._inflight += 1
-                if got[0] == "create":
-                    self._creating += 1
-                self.stats["admitted"] += 1
-            self._pool.submit(self._run, chosen, got)
+            L.check(self._lib.zr_server_wait_any(self._h, C.c_double(0.05), C.byref(rid), C.byref(state)))
+            if rid.value == 0:
+                with self._lock:
+                    if self._closing and not self._outstanding:
+                        return
+                continue
+            self._settle(rid.value, state.value)
 
-    # ------------------------------------------------------------ execution
-    def _run(self, req, got):
-        """One admitted movie.  The plan is released exactly once, and the
-        future (already marked running, so it can no longer be cancelled) gets
-        its result or exception after the release."""
-        torch = self.torch
-        res = err = None
-        plan = None
-        try:
-            if got[0] == "create":
-                try:
-                    plan = self._create_plan(req, got[1])
-                except BaseException:
-                    with self._lock:
-                        self._held -= got[1] if req.key not in self._key_bytes else 0
-                    raise
-                finally:
-                    with self._lock:
-                        self._creating -= 1
-            else:
-                plan = got[1]
-            plan.set_frames(req.frames)
-            plan.align()
-            h, w = req.h, req.w
-            img = filt = None
-            if self.return_sums != "none":
-                img = torch.empty((h, w), dtype=torch.float32, device=self.dev)
-                filt = torch.empty((h, w), dtype=torch.float32, device=self.dev) if req.params.n_filters else None
-                L.check(plan.lib.zr_copy_sums_device(plan._mv, C.c_void_p(img.data_ptr()),
-                                                     C.c_void_p(filt.data_ptr() if filt is not None else 0),
-                                                     None, None))
-            plan.synchronize()
-            sol = plan.solution()
-            frc = plan.frc_curve() if req.params.do_frc else None
-            done = time.perf_counter()
-            if self.return_sums == "host":
-                img = img.cpu().numpy()
-                filt = filt.cpu().numpy() if filt is not None else None
-            res = MovieResult(sol["translations"], img, filt, frc, req.arrival, req.admitted, done, req.key)
-            plan._frames_ref = None
-        except BaseException as e:
-            err = e
-        finally:
-            self._release(req, plan, broken=err is not None)
-        if err is None:
-            req.future.set_result(res)
-        else:
-            req.future.set_exception(err)
-
-    def _release(self, req, plan, broken=False):
+    def _settle(self, rid, state):
         with self._lock:
-            self._inflight -= 1
-            if plan is not None:
-                if broken:
-                    self._held -= self._plan_bytes.pop(id(plan), 0)
-                    plan.close()
-                else:
-                    self._idle.setdefault(req.key, []).append(plan)
-                    self._idle.move_to_end(req.key)
-            self._lock.notify_all()
+            fut, rec = self._outstanding.pop(rid)
+        n = rec["n"]
+        t = np.zeros((n, 2), np.float64)
+        frc = np.zeros(self.FRC_CAP, np.float64)
+        nfrc = C.c_int()
+        times = np.zeros(3, np.float64)
+        msg = C.create_string_buffer(512)
+        rc = self._lib.zr_server_result(self._h, C.c_longlong(rid), L.dptr(t), n, L.dptr(frc), self.FRC_CAP,
+                                        C.byref(nfrc), L.dptr(times), msg, 512)
+        self._lib.zr_server_release(self._h, C.c_longlong(rid))
+        if state == L.ZR_REQ_CANCELLED:
+            super(_ServerFuture, fut).cancel()
+            return
+        if rc != L.ZR_OK:
+            text = msg.value.decode(errors="replace")
+            fut.set_exception(MemoryError(text) if rc == L.ZR_ERR_NOMEM else L.ZrError(rc, text))
+            return
+        img, filt = rec["img"], rec["filt"]
+        if self.return_sums == "host":
+            img = img.cpu().numpy() if img is not None else None
+            filt = filt.cpu().numpy() if filt is not None else None
+        elif self.return_sums == "none":
+            img = filt = None
+        res = MovieResult(t, img, filt, frc[: nfrc.value].copy() if rec["params"].do_frc else None,
+                          times[0], times[1], times[2], rec["key"])
+        fut.set_result(res)
 
 
 def latency_summary(results):
