@@ -221,12 +221,37 @@ ZR_DEV void zr_lds_barrier() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
+// Twiddle sources of the LDS FFT: get(t) = exp(-2*pi*i*t/L) for t in [0, L).
+// GlobalTw reads a full table in global memory (one load per twiddle, from L2);
+// LdsTw forms it from two short tables staged in LDS, hi[t / 64] * lo[t % 64]
+// (two LDS reads and a complex product: no global-memory latency inside the
+// passes; within about one float32 ulp of the full table).
+struct GlobalTw {
+  const float2* __restrict__ tw;
+  ZR_DEV float2 get(int t) const { return tw[t]; }
+};
+struct LdsTw {
+  const float2* hi;   // exp(-2 pi i 64 h / L), h < L / 64
+  const float2* lo;   // exp(-2 pi i l / L), l < 64
+  ZR_DEV float2 get(int t) const { return c_mul(hi[t >> 6], lo[t & 63]); }
+};
+// LDS floats2 an LdsTw table for length L takes.
+constexpr int zr_ldstw_size(int L) { return (L + 63) / 64 + 64; }
+// Fill an LdsTw table (hi first, then lo) from the full global table tw of
+// length L; a block's threads 0 .. nthreads-1 share the work.  The caller
+// synchronises the workgroup before the first use.
+ZR_DEV LdsTw zr_ldstw_fill(float2* lds, const float2* __restrict__ tw, int L, int tid, int nthreads) {
+  const int nh = (L + 63) / 64;
+  for (int e = tid; e < nh + 64; e += nthreads) lds[e] = e < nh ? tw[64 * e] : (e - nh < L ? tw[e - nh] : make_float2(1.f, 0.f));
+  return LdsTw{lds, lds + nh};
+}
+
 // One Stockham pass of radix R with stride Ls over an LDS-resident sequence of
 // length L, in place (all reads of the pass complete before any write).
-// tw points at exp(-2*pi*i*t/L), t in [0,L) (forward); DIR=+1 uses conj.
+// tws.get(t) = exp(-2*pi*i*t/L), t in [0,L) (forward); DIR=+1 uses conj.
 // INP / OUTP: the pass reads / writes the padded layout.
-template <int L, int R, int Ls, int DIR, int NT, bool INP, bool OUTP>
-ZR_DEV void stockham_pass(float2* buf, const float2* __restrict__ tw, int tid) {
+template <int L, int R, int Ls, int DIR, int NT, bool INP, bool OUTP, class TW>
+ZR_DEV void stockham_pass(float2* buf, const TW& tws, int tid) {
   constexpr int NB = L / R;                       // butterflies
   constexpr int PER = (NB + NT - 1) / NT;         // per thread
   float2 v[PER][R];
@@ -243,7 +268,7 @@ ZR_DEV void stockham_pass(float2* buf, const float2* __restrict__ tw, int tid) {
       if (Ls > 1) {
 #pragma unroll
         for (int r = 1; r < R; ++r) {
-          float2 w = tw[(k * r) * (L / (Ls * R))];
+          float2 w = tws.get((k * r) * (L / (Ls * R)));
           if (DIR > 0) w.y = -w.y;
           v[b][r] = c_mul(v[b][r], w);
         }
@@ -269,26 +294,35 @@ ZR_DEV void stockham_pass(float2* buf, const float2* __restrict__ tw, int tid) {
 }
 
 template <int L, int Ls, int DIR, int NT> struct WgFft {
-  static ZR_DEV void run(float2* buf, const float2* __restrict__ tw, int tid) {
+  template <class TW>
+  static ZR_DEV void run(float2* buf, const TW& tws, int tid) {
     constexpr int R = zr_next_radix(L / Ls);
     static_assert(R > 1, "FFT length must be 2^a 3^b 5^c");
     constexpr bool first = (Ls == 1), last = (Ls * R == L);
-    stockham_pass<L, R, Ls, DIR, NT, !first, !last>(buf, tw, tid);
-    WgFft<L, Ls * R, DIR, NT>::run(buf, tw, tid);
+    stockham_pass<L, R, Ls, DIR, NT, !first, !last>(buf, tws, tid);
+    WgFft<L, Ls * R, DIR, NT>::run(buf, tws, tid);
   }
 };
 template <int L, int DIR, int NT> struct WgFft<L, L, DIR, NT> {
-  static ZR_DEV void run(float2*, const float2* __restrict__, int) {}
+  template <class TW>
+  static ZR_DEV void run(float2*, const TW&, int) {}
 };
 
 // Whole-workgroup complex FFT of length L held in LDS (natural order in/out).
 // buf holds zr_fft_lds(L) elements (the passes in between use the padding).
 // The caller must synchronise the workgroup after filling buf; returns
 // synchronised (for LDS: its barriers do not wait for global memory).
+// Twiddles from the full global table tw of length L.
 template <int L, int DIR, int NT>
 ZR_DEV void wg_fft(float2* buf, const float2* __restrict__ tw, int tid) {
   static_assert(zr_smooth(L), "FFT length must be 2^a 3^b 5^c");
-  WgFft<L, 1, DIR, NT>::run(buf, tw, tid);
+  WgFft<L, 1, DIR, NT>::run(buf, GlobalTw{tw}, tid);
+}
+// The same transform with twiddles from an LDS table (zr_ldstw_fill).
+template <int L, int DIR, int NT>
+ZR_DEV void wg_fft(float2* buf, const LdsTw& tws, int tid) {
+  static_assert(zr_smooth(L), "FFT length must be 2^a 3^b 5^c");
+  WgFft<L, 1, DIR, NT>::run(buf, tws, tid);
 }
 
 // ---------------------------------------------------------- XCD remap

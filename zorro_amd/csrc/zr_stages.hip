@@ -1033,22 +1033,33 @@ ZR_DEV FrameShift reduce_shift(FrameShift s, int H, int W) {
 // Shift + sums: a thread owns one column x and SS_R consecutive rows; lanes of a
 // wave own consecutive columns, so every load of the 3 x (SS_R + 2) source
 // neighbourhood is one contiguous 256-byte row segment.  Frames are summed in
-// order into imageSum and the even/odd half sums.  The per-frame shifts are
-// reduced once per block into LDS; a frame with an integer shift takes the
-// identity 3x3 kernel (k = delta), which gives exactly the rolled pixel.
-constexpr int SS_R = 8;
+// order into imageSum and the even/odd half sums.  The 3x3 Lanczos kernel is
+// the outer product u[a] v[b] of its row and column factors, so each source row
+// is filtered once along x (h = sum_b v[b] g[b]) and each output row takes the
+// three filtered rows it needs (sum_a u[a] h).  The source rows of a wave are
+// wave-uniform (rolled, reflected row indices in scalar registers) and reach
+// the loads as the scalar offsets of a per-frame buffer resource, so the vector
+// work per frame is the column indices, 30 loads and the filter.  A frame with
+// an integer shift takes u = v = (0, 1, 0), which gives exactly the rolled
+// pixel.  The per-frame shifts are reduced once per block into LDS.
+// Tile of a block: SS_WX waves side by side (64 SS_WX columns) times 4 / SS_WX
+// waves stacked, each wave SS_R rows.
+constexpr int SS_R = 16, SS_WX = 4;
+template <int R, int WX>
 __global__ __launch_bounds__(256) void k_shift_sum(const float* __restrict__ frames, const FrameShift* __restrict__ fs,
                                                   int N, int H, int W, float* __restrict__ img_sum,
                                                   float* __restrict__ even_sum, float* __restrict__ odd_sum,
                                                   float* __restrict__ aligned, double* __restrict__ rowsum,
                                                   int gx) {
+  constexpr int WY = 4 / WX;
+  static_assert(WX * WY == 4, "four waves per block");
   __shared__ FrameShift sfs[MAXN];
   __shared__ double red[4];
   for (int j = threadIdx.x; j < N; j += 256) {
     FrameShift f = reduce_shift(fs[j], H, W);
     if (!f.conv) {
 #pragma unroll
-      for (int i = 0; i < 9; ++i) f.k[i] = (i == 4) ? 1.f : 0.f;
+      for (int i = 0; i < 3; ++i) f.u[i] = f.v[i] = (i == 1) ? 1.f : 0.f;
       f.conv = 1;
     }
     sfs[j] = f;
@@ -1058,49 +1069,55 @@ __global__ __launch_bounds__(256) void k_shift_sum(const float* __restrict__ fra
   // halo rows and columns) run on the same XCD and meet in its L2
   const int lin = xcd_remap(blockIdx.x, gridDim.x);
   const int bx = lin % gx, by = lin / gx;
-  const int x = bx * 64 + (threadIdx.x & 63);
-  const int y0 = (by * 4 + (threadIdx.x >> 6)) * SS_R;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int x = (bx * WX + wave % WX) * 64 + (threadIdx.x & 63);
+  const int xs = x < W ? x : W - 1;           // lanes past the right edge read a valid column, store nothing
+  const int y0 = (by * WY + wave / WX) * R;   // wave-uniform
+  const unsigned frame_bytes = (unsigned)((size_t)H * W * sizeof(float));
   double part = 0.0;
-  if (x < W && y0 < H) {
-    float st[SS_R], se[SS_R], so[SS_R];
+  if (y0 < H) {
+    float st[R], se[R], so[R];
 #pragma unroll
-    for (int i = 0; i < SS_R; ++i) { st[i] = 0.f; se[i] = 0.f; so[i] = 0.f; }
+    for (int i = 0; i < R; ++i) { st[i] = 0.f; se[i] = 0.f; so[i] = 0.f; }
     for (int J = 0; J < N; ++J) {
       const FrameShift& s = sfs[J];
-      const float* f = frames + (size_t)J * H * W;
-      // source columns of x+1, x, x-1 (reflected at the frame edge, then rolled)
-      const int c0 = wrap1(refl1(x + 1, W) - s.ix, W);
-      const int c1 = wrap1(x - s.ix, W);
-      const int c2 = wrap1(refl1(x - 1, W) - s.ix, W);
-      float g[SS_R + 2][3];
+      const int iy = __builtin_amdgcn_readfirstlane(s.iy), ix = __builtin_amdgcn_readfirstlane(s.ix);
+      const float u0 = s.u[0], u1 = s.u[1], u2 = s.u[2], v0 = s.v[0], v1 = s.v[1], v2 = s.v[2];
+      const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+          (void*)(frames + (size_t)J * H * W), (short)0, (int)frame_bytes, 0x00020000);
+      // byte offsets of the source columns of x+1, x, x-1 (reflected at the frame edge, then rolled)
+      const int c0 = wrap1(refl1(xs + 1, W) - ix, W) * 4;
+      const int c1 = wrap1(xs - ix, W) * 4;
+      const int c2 = wrap1(refl1(xs - 1, W) - ix, W) * 4;
+      float h[R + 2];
 #pragma unroll
-      for (int r = 0; r < SS_R + 2; ++r) {      // source row of output row y0 - 1 + r
+      for (int r = 0; r < R + 2; ++r) {      // source row of output row y0 - 1 + r
         const int yy = min(y0 - 1 + r, H);
-        const float* row = f + (size_t)wrap1(refl1(yy, H) - s.iy, H) * W;
-        g[r][0] = row[c0];
-        g[r][1] = row[c1];
-        g[r][2] = row[c2];
+        const int ro = __builtin_amdgcn_readfirstlane(wrap1(refl1(yy, H) - iy, H) * W * 4);
+        const float g0 = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rs, c0, ro, 0));
+        const float g1 = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rs, c1, ro, 0));
+        const float g2 = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rs, c2, ro, 0));
+        h[r] = v0 * g0 + v1 * g1 + v2 * g2;
       }
 #pragma unroll
-      for (int i = 0; i < SS_R; ++i) {
-        // out[y] = sum_a sum_b k[a][b] g[y+1-a][x+1-b]; row y+1-a sits at g[i+2-a]
-        float v = 0.f;
-#pragma unroll
-        for (int a = 0; a < 3; ++a)
-          v += s.k[a * 3 + 0] * g[i + 2 - a][0] + s.k[a * 3 + 1] * g[i + 2 - a][1] + s.k[a * 3 + 2] * g[i + 2 - a][2];
+      for (int i = 0; i < R; ++i) {
+        // out[y] = sum_a u[a] h[y+1-a]; row y+1-a sits at h[i+2-a]
+        const float v = u0 * h[i + 2] + u1 * h[i + 1] + u2 * h[i];
         st[i] += v;
         if (J & 1) so[i] += v; else se[i] += v;
-        if (aligned && y0 + i < H) aligned[((size_t)J * H + y0 + i) * W + x] = v;
+        if (aligned && x < W && y0 + i < H) aligned[((size_t)J * H + y0 + i) * W + x] = v;
       }
     }
+    if (x < W) {
 #pragma unroll
-    for (int i = 0; i < SS_R; ++i) {
-      if (y0 + i < H) {
-        const size_t o = (size_t)(y0 + i) * W + x;
-        img_sum[o] = st[i];
-        even_sum[o] = se[i];
-        odd_sum[o] = so[i];
-        part += st[i];
+      for (int i = 0; i < R; ++i) {
+        if (y0 + i < H) {
+          const size_t o = (size_t)(y0 + i) * W + x;
+          img_sum[o] = st[i];
+          even_sum[o] = se[i];
+          odd_sum[o] = so[i];
+          part += st[i];
+        }
       }
     }
   }
@@ -1387,6 +1404,10 @@ struct DoseArgs {
   SolveInfo* info;
   int order_sq;     // order = 2^order_sq, or -1
   int normalize;    // doseFilter(normalize=True): scale by 1 / (sum over frames of the filter + 1)
+  // (q/qc) above which the cut-off filter exp(-(q/qc)^order) is exactly zero in
+  // float32 ((q/qc)^order >= 200; expf underflows below -104): those
+  // frequencies of a frame add nothing to the filtered sum
+  float cut_skip;
 };
 
 // dose per frame from mean(imageSum)/N, accumulated dose, per-frame cutoff
@@ -1684,20 +1705,32 @@ __global__ __launch_bounds__(DA_NT) void k_dose_accum(const float2* __restrict__
   const float2 ey = twH[(int)(((long)ky * (a.H - 1)) % a.H)];   // e^{-2 pi i ky (H-1)/H}
   const float ord = (float)a.order;
   for (int J = 0; J < a.N; ++J) {
-    const float2 py = Py[(size_t)J * a.H + ky], c0 = C0[(size_t)J * a.H + ky], cw = CW[(size_t)J * a.H + ky];
-    const float mhd = (float)(-0.5 * a.accum[J + 1]);
     const float itq = (float)(1.0 / a.thresq[J]);
-    float2 F[DA_U];
-#pragma unroll
-    for (int u = 0; u < DA_U; ++u) F[u] = S[((size_t)J * Wk + kxs[u]) * HP + ky];
+    // frequencies past frame J's cut-off (its filter is exactly 0 there) add
+    // +0 to the sums: their loads and products are skipped, with the same bits
+    float x[DA_U];
+    bool live[DA_U], any = false;
 #pragma unroll
     for (int u = 0; u < DA_U; ++u) {
+      x[u] = q[u] * itq;
+      live[u] = !(x[u] > a.cut_skip);
+      any = any || live[u];
+    }
+    if (!any) continue;
+    const float2 py = Py[(size_t)J * a.H + ky], c0 = C0[(size_t)J * a.H + ky], cw = CW[(size_t)J * a.H + ky];
+    const float mhd = (float)(-0.5 * a.accum[J + 1]);
+    float2 F[DA_U];
+#pragma unroll
+    for (int u = 0; u < DA_U; ++u) F[u] = live[u] ? S[((size_t)J * Wk + kxs[u]) * HP + ky] : make_float2(0.f, 0.f);
+#pragma unroll
+    for (int u = 0; u < DA_U; ++u) {
+      if (!live[u]) continue;
       const float2 px = Px[(size_t)J * Whalf + kxs[u]];
       const float2 r0 = R0[(size_t)J * Whalf + kxs[u]], rh = RH[(size_t)J * Whalf + kxs[u]];
       float2 sp = c_mul(c_mul(F[u], py), px);
       sp = c_add(sp, c_add(r0, c_mul(rh, ey)));
       sp = c_add(sp, c_add(c0, c_mul(cw, ex[u])));
-      const float gval = expf(-cutoff_pow(q[u] * itq, ord, a.order_sq)) * expf(mhd * invc[u]);
+      const float gval = expf(-cutoff_pow(x[u], ord, a.order_sq)) * expf(mhd * invc[u]);
       gsum[u] += gval;
       acc[u].x += sp.x * gval;
       acc[u].y += sp.y * gval;
@@ -2382,7 +2415,9 @@ int zr_stage_create(zr_stage_state** out, const zr_params& p, int N, int H, int 
   ZC(smalloc(s, (void**)&s->even, sizeof(float) * H * W));
   ZC(smalloc(s, (void**)&s->odd, sizeof(float) * H * W));
   ZC(smalloc(s, (void**)&s->filt, sizeof(float) * H * W));
+  // partial sums of the shift+sum blocks (at most one per 64 columns x 4 rows)
   const int nrow_blocks = ((W + 63) / 64) * ((H + 3) / 4);
+  static_assert(SS_R * (4 / SS_WX) >= 4 && SS_WX >= 1, "shift+sum tiles cover at least 64 x 4 pixels");
   ZC(smalloc(s, (void**)&s->rowsum, sizeof(double) * std::max(nrow_blocks, Hp / 4 + 1)));
   if (p.do_frc) {
     s->box = p.frc_box;
@@ -2533,16 +2568,23 @@ int zr_stage_solve(zr_stage_state* s, const float* shifts, const float* corr, co
   return ZR_OK;
 }
 
+// Shift + sums over the frames; returns the number of blocks (the partial sums in rowsum).
+static int launch_shift_sum(const float* frames, const FrameShift* fs, int N, int H, int W, float* img_sum,
+                            float* even, float* odd, float* aligned, double* rowsum, hipStream_t st) {
+  const int gx = (W + 64 * SS_WX - 1) / (64 * SS_WX), gy = (H + (4 / SS_WX) * SS_R - 1) / ((4 / SS_WX) * SS_R);
+  hipLaunchKernelGGL((k_shift_sum<SS_R, SS_WX>), dim3(gx * gy), dim3(256), 0, st, frames, fs, N, H, W, img_sum, even,
+                     odd, aligned, rowsum, gx);
+  return gx * gy;
+}
+
 int zr_stage_shifts(zr_stage_state* s, const float* frames, float* aligned_out, hipStream_t st) {
   if (!s || !s->have_solution) return serr(ZR_ERR_STATE, "shifts before solve");
-  const int gx = (s->W + 63) / 64, gy = (s->H + 4 * SS_R - 1) / (4 * SS_R);
   const FrameShift* fs = s->fs;
   if (s->fourier) {            // the frames Fourier-shifted by zr_stage_fourier_align, summed as they are
     frames = s->aligned;
     fs = s->fs_id;
   }
-  hipLaunchKernelGGL(k_shift_sum, dim3(gx * gy), dim3(256), 0, st, frames, fs, s->N, s->H, s->W, s->img_sum,
-                     s->even, s->odd, aligned_out, s->rowsum, gx);
+  launch_shift_sum(frames, fs, s->N, s->H, s->W, s->img_sum, s->even, s->odd, aligned_out, s->rowsum, st);
   ZS_HIP(hipGetLastError());
   s->have_sums = true;
   return ZR_OK;
@@ -2617,6 +2659,7 @@ static void fill_dose_args(zr_stage_state* s, DoseArgs& a, const double* rowsum,
   a.rowsum = rowsum; a.nrows = nrows; a.accum = s->accum; a.thresq = s->thresq; a.info = s->info;
   a.order_sq = -1;
   a.normalize = normalize;
+  a.cut_skip = a.order > 0.0 ? (float)std::exp(std::log(200.0) / a.order) : INFINITY;
   for (int k = 0; k < 8; ++k)
     if (a.order == (double)(1 << k)) a.order_sq = k;
 }
@@ -2625,13 +2668,11 @@ static int dose_spectral(zr_stage_state* s, const float* frames, const float2* S
                          hipStream_t st) {
   const int H = s->H, W = s->W;
   // aligned sums (imageSum, even, odd) and their total for the dose per frame
-  const int gx = (W + 63) / 64, gy = (H + 4 * SS_R - 1) / (4 * SS_R);
-  hipLaunchKernelGGL(k_shift_sum, dim3(gx * gy), dim3(256), 0, st, frames, s->fs, s->N, H, W, s->img_sum, s->even,
-                     s->odd, (float*)nullptr, s->rowsum, gx);
+  const int nblk = launch_shift_sum(frames, s->fs, s->N, H, W, s->img_sum, s->even, s->odd, nullptr, s->rowsum, st);
   ZS_HIP(hipGetLastError());
   s->have_sums = true;
   DoseArgs a;
-  fill_dose_args(s, a, s->rowsum, gx * gy, normalize);
+  fill_dose_args(s, a, s->rowsum, nblk, normalize);
   hipLaunchKernelGGL(k_dose_params, dim3(1), dim3(256), (size_t)(a.Hp / 2) * sizeof(double), st, a);
   ZS_HIP(hipGetLastError());
   if (H != W) return serr(ZR_ERR_UNSUPPORTED, "spectral dose path needs square frames");
@@ -2679,13 +2720,11 @@ int zr_stage_dose(zr_stage_state* s, const float* frames, const float2* S, int W
     fs = s->fs_id;
   }
   // aligned sums (imageSum, even, odd) and their total for the dose per frame
-  const int gx = (W + 63) / 64, gy = (H + 4 * SS_R - 1) / (4 * SS_R);
-  hipLaunchKernelGGL(k_shift_sum, dim3(gx * gy), dim3(256), 0, st, frames, fs, s->N, H, W, s->img_sum, s->even,
-                     s->odd, (float*)nullptr, s->rowsum, gx);
+  const int nblk = launch_shift_sum(frames, fs, s->N, H, W, s->img_sum, s->even, s->odd, nullptr, s->rowsum, st);
   ZS_HIP(hipGetLastError());
   s->have_sums = true;
   DoseArgs a;
-  fill_dose_args(s, a, s->rowsum, gx * gy, normalize);
+  fill_dose_args(s, a, s->rowsum, nblk, normalize);
   hipLaunchKernelGGL(k_dose_params, dim3(1), dim3(256), (size_t)(a.Hp / 2) * sizeof(double), st, a);
   ZS_HIP(hipGetLastError());
   // the filtered spectra of the aligned, symmetric-padded frames, DOSE_CHUNK frames at a time

@@ -57,6 +57,10 @@ __global__ __launch_bounds__(RB * TPR) void k1_rows(const float* __restrict__ fr
   const int f = frame0 + fb;
   const int y = y0 + g;
   const float* row = frames + ((size_t)f * H + y) * W;
+  // twiddles of the row transform (length L) and of the real-FFT split
+  // (length WP) in LDS, after the row buffers
+  const LdsTw tws = zr_ldstw_fill(k1_lds + RB * zr_fft_lds(L), twL, L, tid, NT);
+  const LdsTw tws2 = zr_ldstw_fill(k1_lds + RB * zr_fft_lds(L) + zr_ldstw_size(L), tw2L, WP, tid, NT);
   float2 v[PER];
   // in-mask test: the frame region, times the user mask where one is given
   auto inm = [&](int x) -> bool { return y < H && x < W && (!umask || umask[(size_t)y * W + x] != 0.f); };
@@ -101,7 +105,7 @@ __global__ __launch_bounds__(RB * TPR) void k1_rows(const float* __restrict__ fr
     buf[g][gt + i * TPR] = v[i];
   }
   __syncthreads();
-  wg_fft<L, -1, TPR>(buf[g], twL, gt);
+  wg_fft<L, -1, TPR>(buf[g], tws, gt);
   // post-process and write transposed in column pairs: T[fb][k/2][y][k%2], so a
   // thread writes RB rows x 2 columns = 16 RB contiguous bytes (the odd pad
   // column of an odd Wh is written as zero)
@@ -113,7 +117,7 @@ __global__ __launch_bounds__(RB * TPR) void k1_rows(const float* __restrict__ fr
       const int k = 2 * kp + h;
       const bool live = k < Wh;
       const int kk = live ? k : 0;
-      const float2 w = tw2L[kk];   // exp(-2*pi*i*k/WP)
+      const float2 w = tws2.get(kk);   // exp(-2*pi*i*k/WP)
 #pragma unroll
       for (int r = 0; r < RB; ++r) {
         const float2 zk = buf[r][kk % L];
@@ -202,6 +206,7 @@ __global__ __launch_bounds__(NT) void k1_cols1(const float2* __restrict__ T, int
                                                float2* __restrict__ Sside, int full) {
   constexpr int PER = (HP + NT - 1) / NT;
   __shared__ float2 buf[zr_fft_lds(HP)];
+  __shared__ float2 twl[zr_ldstw_size(HP)];
   const int tid = threadIdx.x;
   const int KP = (Wh + 1) / 2;
   const int blk = xcd_remap(blockIdx.x, gridDim.x);
@@ -211,11 +216,12 @@ __global__ __launch_bounds__(NT) void k1_cols1(const float2* __restrict__ T, int
 #pragma unroll
   for (int i = 0; i < PER; ++i)
     if (HP % NT == 0 || tid + i * NT < HP) v[i] = src[2 * (tid + i * NT)];
+  const LdsTw tws = zr_ldstw_fill(twl, twH, HP, tid, NT);
 #pragma unroll
   for (int i = 0; i < PER; ++i)
     if (HP % NT == 0 || tid + i * NT < HP) buf[tid + i * NT] = v[i];
   __syncthreads();
-  wg_fft<HP, -1, NT>(buf, twH, tid);
+  wg_fft<HP, -1, NT>(buf, tws, tid);
   const int f = frame0 + fb;
   if (full) {
     float2* dst = S + ((size_t)f * Wh + k) * HP;
@@ -1414,7 +1420,7 @@ static hipError_t launch_k1_rows_t(const float* frames, int H, int W, int Hp, in
   constexpr int TPR = k1_threads_per_row(WP);
   if (Hp % RB != 0) return hipErrorInvalidValue;
   int grid = nb * (Hp / RB);
-  size_t sh = (size_t)RB * zr_fft_lds(WP / 2) * sizeof(float2);
+  size_t sh = ((size_t)RB * zr_fft_lds(WP / 2) + zr_ldstw_size(WP / 2) + zr_ldstw_size(WP)) * sizeof(float2);
   static std::once_flag lds_once;
   zr_allow_full_lds((const void*)k1_rows<WP, RB, TPR>, lds_once);
   hipLaunchKernelGGL((k1_rows<WP, RB, TPR>), dim3(grid), dim3(RB * TPR), sh, st, frames, H, W, Hp, Wh, frame0, twL,
@@ -1434,9 +1440,12 @@ hipError_t zr_launch_k1_rows(int Wp, const float* frames, int H, int W, int Hp, 
 }
 
 constexpr bool K1_COLS_PAIRED = false;  // column pairs per block (else one column per block, the faster)
-// threads per column of k1_cols1 (timed on MI355X for 8 frames: 128 is the
-// fastest at heights up to 2048 and at 4096, 256 at 3072 and 3840, 512 at 5760)
-constexpr int k1_cols_threads(int hp) { return hp == 5760 ? 512 : ((hp == 3072 || hp == 3840) ? 256 : 128); }
+// threads per column of k1_cols1 (timed on MI355X: 128 at heights up to 2048,
+// 256 at 3072 to 4096, 512 at 5760).  At 4096, 128 threads are as fast alone,
+// but 256 (about 70 VGPRs instead of 135) let the column blocks of one movie
+// share CUs with another movie's pair transform (two waves of about 200
+// VGPRs per SIMD), which is what two movies in flight need.
+constexpr int k1_cols_threads(int hp) { return hp == 5760 ? 512 : (hp >= 3072 ? 256 : 128); }
 hipError_t zr_launch_k1_cols(int Hp, const float2* T, int Wh, int Hc, int frame0, int nb, const float2* twH,
                              float2* S, float2* Sside, int full, hipStream_t st) {
   if (!K1_COLS_PAIRED) {
