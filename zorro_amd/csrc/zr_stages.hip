@@ -1184,8 +1184,10 @@ __global__ __launch_bounds__(256) void k_frc_rows(const float* __restrict__ even
     }
     buf[g][n] = make_float2(h.x * v.x, h.y * v.y);
   }
+  __shared__ float2 twl_[zr_ldstw_size(L)];
+  const LdsTw tws_ = zr_ldstw_fill(twl_, twL, L, threadIdx.x, blockDim.x);
   __syncthreads();
-  wg_fft<L, -1, TPR>(buf[g], twL, gt);
+  wg_fft<L, -1, TPR>(buf[g], tws_, gt);
   // real-FFT post-processing; T[tile][par][k][y] takes RB consecutive rows per k
   const int Wh = L + 1;
   for (int k = tid; k < Wh; k += NT) {
@@ -1234,6 +1236,8 @@ __global__ __launch_bounds__(frc_cols_nt(BOX)) void k_frc_cols(const float2* __r
   double acc[PER];
 #pragma unroll
   for (int i = 0; i < PER; ++i) acc[i] = accumulate ? dst[(size_t)(tid + i * NT) * Wh + k] : 0.0;
+  __shared__ float2 twl_[zr_ldstw_size(BOX)];
+  const LdsTw tws_ = zr_ldstw_fill(twl_, twB, BOX, threadIdx.x, blockDim.x);
   // the group's first tile in the range
   const int first = tile0 + (((g - tile0) % FRC_GROUPS) + FRC_GROUPS) % FRC_GROUPS;
   for (int base = first; base < tile1; base += FRC_GROUPS * FRC_TB) {
@@ -1251,7 +1255,7 @@ __global__ __launch_bounds__(frc_cols_nt(BOX)) void k_frc_cols(const float2* __r
       }
     }
     __syncthreads();
-    wg_fft<BOX, -1, TPC>(buf[q], twB, gt);
+    wg_fft<BOX, -1, TPC>(buf[q], tws_, gt);
 #pragma unroll
     for (int i = 0; i < PER; ++i) {
       const int n = tid + i * NT;
@@ -1359,8 +1363,10 @@ __global__ __launch_bounds__(dose_rows_rb(WP) * dose_rows_tpr(WP)) void k_dose_r
     buf[g][n] = make_float2(aligned_px_row(rs, sh, x0 < W ? x0 : sym_idx(x0, W), W),
                             aligned_px_row(rs, sh, x1 < W ? x1 : sym_idx(x1, W), W));
   }
+  __shared__ float2 twl_[zr_ldstw_size(L)];
+  const LdsTw tws_ = zr_ldstw_fill(twl_, twL, L, threadIdx.x, blockDim.x);
   __syncthreads();
-  wg_fft<L, -1, TPR>(buf[g], twL, gt);
+  wg_fft<L, -1, TPR>(buf[g], tws_, gt);
   const int Wh = L + 1, KP = (Wh + 1) / 2;
   for (int kp = tid; kp < KP; kp += NT) {
     float2 out[2][RB];
@@ -1494,11 +1500,13 @@ __global__ __launch_bounds__(dose_cols_nt(HP)) void k_dose_cols(const float2* __
     acc[i] = j0 > 0 ? gcol[ky] : make_float2(0.f, 0.f);
   }
   const float ord = (float)a.order;
+  __shared__ float2 twl_[zr_ldstw_size(HP)];
+  const LdsTw tws_ = zr_ldstw_fill(twl_, twH, HP, threadIdx.x, blockDim.x);
   for (int J = j0; J < j1; ++J) {
     const float2* src = T + (((size_t)(J - j0) * KP + kx / 2) * HP) * 2 + (kx & 1);
     for (int i = 0; i < PER; ++i) buf[tid + i * NT] = src[2 * (tid + i * NT)];
     __syncthreads();
-    wg_fft<HP, -1, NT>(buf, twH, tid);
+    wg_fft<HP, -1, NT>(buf, tws_, tid);
     const float mhd = (float)(-0.5 * a.accum[J + 1]);
     const float tq = (float)a.thresq[J];
     for (int i = 0; i < PER; ++i) {
@@ -1530,7 +1538,7 @@ __global__ __launch_bounds__(dose_cols_nt(HP)) void k_dose_cols(const float2* __
   }
   for (int i = 0; i < PER; ++i) buf[tid + i * NT] = acc[i];
   __syncthreads();
-  wg_fft<HP, 1, NT>(buf, twH, tid);
+  wg_fft<HP, 1, NT>(buf, tws_, tid);
   for (int y = tid; y < a.H; y += NT) gcol[y] = buf[y];
 }
 
@@ -1552,8 +1560,10 @@ __global__ __launch_bounds__(256) void k_dose_out(const float2* __restrict__ G, 
     float2 o = c_mul(d, make_float2(w.x, -w.y));    // * exp(+2 pi i k / Wp)
     buf[k] = make_float2(e.x - o.y, e.y + o.x);      // E + i O
   }
+  __shared__ float2 twl_[zr_ldstw_size(L)];
+  const LdsTw tws_ = zr_ldstw_fill(twl_, twL, L, threadIdx.x, blockDim.x);
   __syncthreads();
-  wg_fft<L, 1, NT>(buf, twL, tid);
+  wg_fft<L, 1, NT>(buf, tws_, tid);
   for (int n = tid; n < L; n += NT) {
     float2 z = buf[n];
     int x0 = 2 * n, x1 = 2 * n + 1;
@@ -1758,8 +1768,10 @@ __global__ __launch_bounds__(256) void k_dose_cols_ifft(const float2* __restrict
   for (int i = 0; i < PER; ++i) v[i] = col[tid + i * NT];
 #pragma unroll
   for (int i = 0; i < PER; ++i) buf[tid + i * NT] = v[i];
+  __shared__ float2 twl_[zr_ldstw_size(HP)];
+  const LdsTw tws_ = zr_ldstw_fill(twl_, twH, HP, threadIdx.x, blockDim.x);
   __syncthreads();
-  wg_fft<HP, 1, NT>(buf, twH, tid);
+  wg_fft<HP, 1, NT>(buf, tws_, tid);
 #pragma unroll
   for (int i = 0; i < PER; ++i) col[tid + i * NT] = buf[tid + i * NT];
 }
@@ -1835,8 +1847,10 @@ __global__ __launch_bounds__(dose_cols_nt(HP)) void k_fshift_cols(const float2* 
     const float2 r = make_float2((float)(0.5 * (c1 + c2)), (float)(0.5 * (s1 + s2)));
     buf[ky] = c_mul(v, r);
   }
+  __shared__ float2 twl_[zr_ldstw_size(HP)];
+  const LdsTw tws_ = zr_ldstw_fill(twl_, twH, HP, threadIdx.x, blockDim.x);
   __syncthreads();
-  wg_fft<HP, 1, NT>(buf, twH, tid);
+  wg_fft<HP, 1, NT>(buf, tws_, tid);
   float2* dst = Gf + ((size_t)jl * Wk + kx) * H;
   for (int y = tid; y < H; y += NT) dst[y] = buf[y];
 }
@@ -1862,8 +1876,10 @@ __global__ __launch_bounds__(256) void k_fshift_rows(const float2* __restrict__ 
     const float2 o = c_mul(d, make_float2(w.x, -w.y));    // * exp(+2 pi i k / WP)
     buf[k] = make_float2(e.x - o.y, e.y + o.x);            // E + i O
   }
+  __shared__ float2 twl_[zr_ldstw_size(L)];
+  const LdsTw tws_ = zr_ldstw_fill(twl_, twL, L, threadIdx.x, blockDim.x);
   __syncthreads();
-  wg_fft<L, 1, NT>(buf, twL, tid);
+  wg_fft<L, 1, NT>(buf, tws_, tid);
   float* out = aligned + ((size_t)J * H + y) * W;
   for (int n = tid; n < L; n += NT) {
     const float2 z = buf[n];
