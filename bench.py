@@ -15,6 +15,9 @@ Other workloads (each prints its own JSON line; the line above is the default):
 
     python bench.py --workload served    # Poisson arrivals of a 3-class movie mix, p50/p99 vs rate
     python bench.py --workload batched   # 48 uneven movies (20-60 frames, 4 detector shapes) at once
+    python bench.py --workload hostinput # C2 from pinned host memory (float32 and 4-bit counts)
+    python bench.py --workload configs   # the 20x1024^2 Fourier-shift movie and the 7676x7420 uint8
+                                         # super-resolution counting movie (gain, binning, 0.5 A/px)
 """
 from __future__ import annotations
 
@@ -187,7 +190,7 @@ def main():
     ap.add_argument("--kernels", nargs="*", help="with --counter-summary: kernel name prefixes to keep")
     ap.add_argument("--k2-counters", nargs="+", metavar="CSV_OR_OUT",
                     help="counter CSVs then the output JSON: executed FP32 work of the pair transform (no GPU)")
-    ap.add_argument("--workload", default="c2", choices=["c2", "served", "batched", "hostinput"])
+    ap.add_argument("--workload", default="c2", choices=["c2", "served", "batched", "hostinput", "configs"])
     ap.add_argument("--inflight", type=int, default=2,
                     help="C2 movies in flight at once for the headline (one plan and stream each); "
                          "single-movie figures are measured first and reported beside it")
@@ -222,6 +225,8 @@ def main():
         return bench_batched(args)
     if args.workload == "hostinput":
         return bench_hostinput(args)
+    if args.workload == "configs":
+        return bench_configs(args)
     return bench_c2(args)
 
 
@@ -434,6 +439,101 @@ def bench_hostinput(args):
         for p in plans:
             p.close()
         del bufs
+        torch.cuda.empty_cache()
+    print(json.dumps(out), flush=True)
+
+
+# ---------------------------------------------------------------- C1 and C4
+CONFIGS = {
+    # 20 frames of 1024^2 float32 (1.5 e/px/frame), no crop, Fourier applyShifts
+    "c1_20x1024sq_fourier": dict(
+        n=20, h=1024, w=1024, seed=21, dose=1.5, raw=False, scale=1.0,
+        params=dict(PARAMS, fouCrop=(0, 0), shapePadded=(1024, 1024), shiftMethod="fourier")),
+    # 50 frames of 7676x7420 uint8 counts (0.25 e/superres px/frame) times a float32
+    # gain reference N(1, 0.05), Fourier-binned to 3838x3710, padded to 3840^2,
+    # cropped to 3072 for the correlation, dose filter at 0.5 A/px
+    "c4_50x7676x7420_u8_gain_bin3838x3710": dict(
+        n=50, h=7676, w=7420, seed=76, dose=0.25, raw=True, scale=0.5,
+        params=dict(PARAMS, fouCrop=(3072, 3072), shapePadded=(3840, 3840), shapeBinned=(3838, 3710),
+                    pixelsize=0.025)),
+}
+
+
+def bench_configs(args):
+    """The other two single-movie configurations, each with its input resident in
+    HBM: a step is one movie through the path (for the counting movie: the gain
+    product and the Fourier binning, then registration, dose filter and sums).
+    One movie at a time first (latency, stage times), then two in flight."""
+    import torch
+    from zorro_amd.engine import MoviePlan, make_params
+    from zorro_amd.synth import make_movie_torch
+    dev = torch.device("cuda:0")
+    out = dict(metric="movies/min per configuration (single GPU, inputs resident in HBM)", unit="movies/min",
+               n_gpus=1, data="synthetic", configs={})
+    for name, c in CONFIGS.items():
+        print(f"[bench configs] {name}: generating", file=sys.stderr, flush=True)
+        frames, drift = make_movie_torch(c["n"], c["h"], c["w"], seed=c["seed"], dose=c["dose"], device=dev)
+        gain = None
+        if c["raw"]:
+            frames = torch.clamp(frames, max=255).to(torch.uint8)
+            g = torch.Generator(device=dev)
+            g.manual_seed(c["seed"] + 1)
+            gain = (1.0 + 0.05 * torch.randn((c["h"], c["w"]), generator=g, device=dev)).contiguous()
+        torch.cuda.synchronize()
+        plans = [MoviePlan(make_params(**c["params"]), c["n"], c["h"], c["w"]) for _ in range(2)]
+
+        def movie(k):
+            if c["raw"]:
+                plans[k].set_frames_raw(frames, gain=gain)
+            plans[k].align()
+
+        for k, p in enumerate(plans):
+            if not c["raw"]:
+                p.set_frames(frames)
+            p.use_graph(True)
+            p.set_timing(k == 0)
+            for _ in range(max(2, args.warmup)):
+                movie(k)
+            p.synchronize()
+        plans[0].stage_times(reset=True)
+        res = {}
+        for kin in (1, 2):
+            lat, start = [], [0.0, 0.0]
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for s_ in range(args.steps):
+                k = s_ % kin
+                if s_ >= kin:
+                    plans[k].synchronize()
+                    lat.append((time.perf_counter() - start[k]) * 1e3)
+                start[k] = time.perf_counter()
+                movie(k)
+            for s_ in range(max(0, args.steps - kin), args.steps):
+                k = s_ % kin
+                plans[k].synchronize()
+                lat.append((time.perf_counter() - start[k]) * 1e3)
+            ms = (time.perf_counter() - t0) * 1e3 / args.steps
+            res[kin] = dict(ms_per_movie=round(ms, 3), value=round(60000.0 / ms, 1),
+                            frames_per_s=round(c["n"] * 1000.0 / ms, 1),
+                            latency_ms=dict(p50=round(float(np.percentile(lat, 50)), 3),
+                                            p99=round(float(np.percentile(lat, 99)), 3)))
+            if kin == 1:
+                st = plans[0].stage_times()
+                plans[0].set_timing(False)
+        sol = plans[0].solution()
+        truth = -(drift - drift.mean(axis=0)) * c["scale"]
+        rms = float(np.sqrt(np.mean((sol["translations"] - truth) ** 2)))
+        info = plans[0].info()
+        out["configs"][name] = dict(value=res[2]["value"], unit="movies/min", inflight=2, two_in_flight=res[2],
+                                    single_movie=res[1], frames=c["n"], input_shape=[c["h"], c["w"]],
+                                    registered_shape=[info["H"], info["W"]], padded=[info["Hp"], info["Wp"]],
+                                    crop=[info["Hc"], info["Wc"]], pairs=info["pairs"],
+                                    stages_ms={k: round(v[0] / max(v[1], 1), 4) for k, v in st.items() if v[1]},
+                                    rms_vs_true_drift_px=round(rms, 4))
+        print(f"[bench configs] {name}: {out['configs'][name]}", file=sys.stderr, flush=True)
+        for p in plans:
+            p.close()
+        del frames, gain, plans
         torch.cuda.empty_cache()
     print(json.dumps(out), flush=True)
 

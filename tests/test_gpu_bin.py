@@ -5,7 +5,8 @@ whose DFT lengths are not 2^a 3^b 5^c.
 
 Tolerances (stated per result):
 * binned frames: relative L2 error <= 2e-6 per frame against the FP64 oracle
-  (float32 chirp-z transforms);
+  (float32 transforms: the polyphase form for exact halving, chirp-z
+  transforms otherwise);
 * gain product without binning: bit-identical (both float32 gainRef * data / gain);
 * end to end on a binned, gain-corrected, padded movie: translations within
   0.02 px of the oracle; imageSum relative L2 <= 1e-4.

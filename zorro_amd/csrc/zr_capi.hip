@@ -176,6 +176,12 @@ struct zr_movie {
   float2 *chW_c = nullptr, *chW_B = nullptr, *chH_c = nullptr, *chH_B = nullptr;
   float2 *chHb_c = nullptr, *chHb_B = nullptr, *chWb_c = nullptr, *chWb_B = nullptr;
   float2 *binA = nullptr, *binB = nullptr;   // [nbb][Hr][W] / [nbb][W][Hr] (then Gc/Gr [nbb][W][H], [nbb][H][W])
+  // binning by exactly two (Hr = 2 H, Wr = 2 W, H and W even): the polyphase
+  // form (zr_launch_bin2) with its convolution size, kernel spectra and sums
+  bool bin2 = false;
+  int M2 = 0;
+  float2 *bin2Gx = nullptr, *bin2Gy = nullptr, *twM2 = nullptr;
+  double *bin2_cpart = nullptr, *bin2_q = nullptr;
   void* raw_owned = nullptr;
   size_t raw_owned_bytes = 0;
   float* gain_owned = nullptr;
@@ -250,7 +256,8 @@ static void free_movie(zr_movie* mv) {
                   mv->windows, mv->shifts, mv->corr, mv->peaksig, mv->origin, mv->bsigma, mv->twMrow, mv->twMcol,
                   mv->twMout, mv->chW_c, mv->chW_B, mv->chH_c, mv->chH_B, mv->chHb_c, mv->chHb_B, mv->chWb_c,
                   mv->chWb_B, mv->binA, mv->binB, mv->raw_owned, mv->gain_owned, mv->hot_block, mv->dpairs,
-                  mv->pidx, mv->ctr, mv->pshift, mv->s0, mv->pflag, mv->umask};
+                  mv->pidx, mv->ctr, mv->pshift, mv->s0, mv->pflag, mv->umask, mv->bin2Gx, mv->bin2Gy, mv->twM2,
+                  mv->bin2_cpart, mv->bin2_q};
   for (void* q : ptrs)
     if (q) hipFree(q);
   zr_stage_free(mv->st);
@@ -646,7 +653,24 @@ static int create_movie(int device, const zr_params* p, int n_frames, int h, int
     e = hipMemset(mv->den, 0, sizeof(double) * mv->Ptot);
     if (e != hipSuccess) { free_movie(mv); return fail(ZR_ERR_HIP, "hipMemset: %s", hipGetErrorString(e)); }
   }
-  if (mv->binned) {
+  if (mv->binned && mv->Hr == 2 * mv->H && mv->Wr == 2 * mv->W && mv->H % 2 == 0 && mv->W % 2 == 0) {
+    // binning by two: half-sample delay kernels along x and y, one convolution size
+    mv->bin2 = true;
+    mv->M2 = zr_bin_conv_size(std::max(mv->H, mv->W));
+    mv->nbb = std::min(mv->N, 4);
+    std::vector<float2> Gx, Gy;
+    zr_bin2_kernel_spectrum(mv->W, mv->M2, Gx);
+    zr_bin2_kernel_spectrum(mv->H, mv->M2, Gy);
+    const size_t per = (size_t)mv->Hr * mv->W;   // floats of Z and its transpose per frame
+    if ((rc = dev_upload(&mv->bin2Gx, Gx)) || (rc = dev_upload(&mv->bin2Gy, Gy)) ||
+        (rc = dev_upload(&mv->twM2, fwd_table(mv->M2))) ||
+        (rc = dev_alloc(&mv->bin2_cpart, (size_t)mv->nbb * (mv->Hr / 2))) || (rc = dev_alloc(&mv->bin2_q, (size_t)mv->nbb)) ||
+        (rc = dev_alloc(&mv->binA, (per * mv->nbb + 1) / 2)) || (rc = dev_alloc(&mv->binB, (per * mv->nbb + 1) / 2)) ||
+        (rc = dev_alloc(&mv->frames_owned, (size_t)mv->N * mv->H * mv->W))) {
+      free_movie(mv);
+      return rc;
+    }
+  } else if (mv->binned) {
     // Bluestein tables: rows at the raw width, columns at the raw height with the
     // inverse at the binned height (same convolution size), out rows at the binned width
     mv->Mrow = zr_bin_conv_size(mv->Wr);
@@ -727,6 +751,17 @@ static int run_input_stage(zr_movie* mv, const void* raw, int dtype, const float
     ZR_HIP(zr_launch_gain_convert(raw, dtype, npix, npix * mv->N, gain, gs, use_scale, mv->frames_owned, s));
   } else {
     const int Hr = mv->Hr, Wr = mv->Wr, H = mv->H, W = mv->W;
+    if (mv->bin2) {
+      for (int f0 = 0; f0 < mv->N; f0 += mv->nbb) {
+        const int nb = std::min(mv->nbb, mv->N - f0);
+        ZR_HIP(zr_launch_bin2(mv->M2, raw, dtype, f0, nb, Hr, Wr, gain, gs, use_scale, mv->bin2Gx, mv->bin2Gy, mv->twM2,
+                              (float*)mv->binA, (float*)mv->binB, mv->bin2_cpart, mv->bin2_q, mv->frames_owned, s));
+      }
+      mv->timer.end(ZR_STAGE_INPUT, t0, s);
+      mv->frames = mv->frames_owned;
+      mv->have_xcorr = false;
+      return ZR_OK;
+    }
     const BinChirp cw{mv->chW_c, mv->chW_B, Wr}, ch{mv->chH_c, mv->chH_B, Hr}, chb{mv->chHb_c, mv->chHb_B, H},
         cwb{mv->chWb_c, mv->chWb_B, W};
     const float scale = (float)(1.0 / ((double)H * (double)W));

@@ -118,6 +118,15 @@ hipError_t zr_launch_bin_cols(int M, const float2* Tc, int nb, int H, int Wb, in
 hipError_t zr_launch_bin_out(int M, const float2* Gr, int nb, int Hb, int Wb, const BinChirp& cwb, const float2* twM,
                              int frame0, float* frames, hipStream_t st);
 hipError_t zr_launch_bin_transpose(const float2* in, int nb, int R, int C, float2* out, hipStream_t st);
+// Binning by exactly two with even binned sizes (Hr = 2 Hb, Wr = 2 Wb): the
+// polyphase form (even samples + half-sample-delayed odd samples per axis);
+// M: power-of-two convolution size >= 2 max(Hb, Wb) - 1; Gx/Gy from
+// zr_bin2_kernel_spectrum; A, B: float scratch of nb * Hr * Wb each; cpart:
+// nb * Hr / 2 doubles; q: nb doubles.  Writes frames [frame0 .. frame0 + nb).
+void zr_bin2_kernel_spectrum(int nh, int M, std::vector<float2>& G);
+hipError_t zr_launch_bin2(int M, const void* raw, int dtype, int frame0, int nb, int Hr, int Wr, const float* gain,
+                          float gscale, int use_scale, const float2* Gx, const float2* Gy, const float2* twM, float* A,
+                          float* B, double* cpart, double* q, float* frames, hipStream_t st);
 hipError_t zr_launch_gain_convert(const void* raw, int dtype, size_t npix_frame, size_t total, const float* gain,
                                   float gscale, int use_scale, float* frames, hipStream_t st);
 // zero fill by a kernel (graph-capturable); bytes multiple of 4
