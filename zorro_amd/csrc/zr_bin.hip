@@ -37,6 +37,28 @@ ZR_DEV float raw_value(const void* raw, int dtype, size_t i) {
   return reinterpret_cast<const float*>(raw)[i];
 }
 
+// Samples i and i+1 (i even) of a raw stack with one load: two bytes, two
+// 16-bit words, two floats, or the two nibbles of one 4-bit byte.
+ZR_DEV float2 raw_pair(const void* raw, int dtype, size_t i) {
+  if (dtype == ZR_RAW_U8) {
+    const unsigned v = reinterpret_cast<const uint16_t*>(raw)[i >> 1];
+    return make_float2((float)(v & 255u), (float)(v >> 8));
+  }
+  if (dtype == ZR_RAW_U16) {
+    const unsigned v = reinterpret_cast<const uint32_t*>(raw)[i >> 1];
+    return make_float2((float)(v & 65535u), (float)(v >> 16));
+  }
+  if (dtype == ZR_RAW_U4) {
+    const unsigned b = reinterpret_cast<const uint8_t*>(raw)[i >> 1];
+    return make_float2((float)(b & 15u), (float)(b >> 4));
+  }
+  if (dtype == ZR_RAW_I16) {
+    const int v = reinterpret_cast<const int*>(raw)[i >> 1];
+    return make_float2((float)(short)(v & 65535), (float)(short)(v >> 16));
+  }
+  return reinterpret_cast<const float2*>(raw)[i >> 1];
+}
+
 // gain product in float32 as loadData does it: gainRef * data, then / gain
 ZR_DEV float gained(float v, const float* gain, size_t pix, float gscale, int use_scale) {
   if (gain) v = gain[pix] * v;
@@ -229,18 +251,36 @@ __global__ __launch_bounds__(bin_threads(M)) void k_bin2_rows(const void* __rest
   const size_t b0 = ((size_t)(frame0 + fb) * Hr + y0) * Wr, b1 = b0 + Wr;
   const size_t g0 = (size_t)y0 * Wr, g1 = g0 + Wr;
   const LdsTw tws = bin_twiddles<M, NT>(bin_lds, twM, tid);
+  // each pixel pair (2j, 2j+1) of both rows in one load each: the odd samples
+  // go to the transform, the even ones stay in registers for the output
+  constexpr int PJ = (M / 2 + NT - 1) / NT;     // Wb <= M / 2
+  float2 ev[PJ];
   double cs = 0.0;
-#pragma unroll 4
-  for (int j = tid; j < M; j += NT) {
-    float2 v = make_float2(0.f, 0.f);
+#pragma unroll
+  for (int i = 0; i < PJ; ++i) {
+    const int j = tid + i * NT;
+    float2 v = make_float2(0.f, 0.f), e = make_float2(0.f, 0.f);
     if (j < Wb) {
-      const int x = 2 * j + 1;
-      v.x = gained(raw_value(raw, dtype, b0 + x), gain, g0 + x, gscale, use_scale);
-      v.y = gained(raw_value(raw, dtype, b1 + x), gain, g1 + x, gscale, use_scale);
+      const int x = 2 * j;
+      const float2 r0 = raw_pair(raw, dtype, b0 + x), r1 = raw_pair(raw, dtype, b1 + x);
+      float2 q0 = make_float2(1.f, 1.f), q1 = make_float2(1.f, 1.f);
+      if (gain) {
+        q0 = *reinterpret_cast<const float2*>(gain + g0 + x);
+        q1 = *reinterpret_cast<const float2*>(gain + g1 + x);
+      }
+      // gainRef * data, then / gain (float32, as loadData)
+      float a0 = gain ? q0.x * r0.x : r0.x, a1 = gain ? q0.y * r0.y : r0.y;
+      float c0 = gain ? q1.x * r1.x : r1.x, c1 = gain ? q1.y * r1.y : r1.y;
+      if (use_scale) { a0 /= gscale; a1 /= gscale; c0 /= gscale; c1 /= gscale; }
+      e = make_float2(a0, c0);
+      v = make_float2(a1, c1);
       cs += (j & 1) ? -(double)v.y : (double)v.y;
     }
+    ev[i] = e;
     buf[j] = v;
   }
+#pragma unroll 4
+  for (int j = PJ * NT + tid; j < M; j += NT) buf[j] = make_float2(0.f, 0.f);
   cs = wave_sum_d(cs);
   if ((tid & 63) == 0) red[tid >> 6] = cs;
   __syncthreads();
@@ -256,12 +296,14 @@ __global__ __launch_bounds__(bin_threads(M)) void k_bin2_rows(const void* __rest
   wg_fft<M, 1, NT>(buf, tws, tid);
   float* z0 = Z + ((size_t)fb * Hr + y0) * Wb;
   float* z1 = z0 + Wb;
-#pragma unroll 4
-  for (int mx = tid; mx < Wb; mx += NT) {
-    const int x = 2 * mx;
-    const float2 w = buf[mx];
-    z0[mx] = gained(raw_value(raw, dtype, b0 + x), gain, g0 + x, gscale, use_scale) + w.x;
-    z1[mx] = gained(raw_value(raw, dtype, b1 + x), gain, g1 + x, gscale, use_scale) + w.y;
+#pragma unroll
+  for (int i = 0; i < PJ; ++i) {
+    const int mx = tid + i * NT;
+    if (mx < Wb) {
+      const float2 w = buf[mx];
+      z0[mx] = ev[i].x + w.x;
+      z1[mx] = ev[i].y + w.y;
+    }
   }
 }
 

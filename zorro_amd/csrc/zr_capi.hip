@@ -819,6 +819,9 @@ extern "C" int zr_movie_set_frames_raw(zr_movie* mv, const void* frames, int dty
     ZR_HIP(hipMalloc((void**)&mv->frames_owned, fb));
     mv->bytes += fb;
   }
+  // binning by two reads pixel pairs (and gain pairs) with one load each
+  if (mv->bin2 && (((uintptr_t)rawd & 7) || ((uintptr_t)gd & 7)))
+    return fail(ZR_ERR_ARG, "raw frames and gain reference must be 8-byte aligned device memory");
   int rc = run_input_stage(mv, rawd, dtype, gd, gain_scalar);
   if (rc) return rc;
   if (!on_device || (gain && !gain_on_device)) ZR_HIP(sync_movie(mv));   // host buffers may go away
