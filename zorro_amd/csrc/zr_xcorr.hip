@@ -720,6 +720,12 @@ __global__ __launch_bounds__(64 * WPB, K2R2_OCC) void k2_cols_r2(const float2* _
 #pragma unroll
     for (int k1 = 0; k1 < R1; ++k1) {
       __builtin_amdgcn_sched_barrier(0);
+      // the group's sixteen lane twiddles, read before its DFT so that the
+      // LDS latency hides behind the DFT
+      float2 tw16[16];
+#pragma unroll
+      for (int n = 0; n < 16; ++n) tw16[n] = tab[(k1 + R1 * n) * 64 + lane];
+      __builtin_amdgcn_sched_barrier(0);
       float2 y[16];                            // residues k1 + R1 k2, k2 = 0 .. 15
 #pragma unroll
       for (int n2 = 0; n2 < 16; ++n2) y[n2] = t[n2 * R1 + k1];
@@ -740,7 +746,7 @@ __global__ __launch_bounds__(64 * WPB, K2R2_OCC) void k2_cols_r2(const float2* _
         for (int i = 0; i < RC; ++i) {
           const int r = k1 + R1 * (c * RC + i);
           // rows of outputs outside [c - m, c + m) are written too and never read
-          const float2 u = c_mul(y[c * RC + i], tab[r * 64 + lane]);
+          const float2 u = c_mul(y[c * RC + i], tw16[c * RC + i]);
           my[(2 * i) * LP + lane] = u;                     // dy = r
           my[(2 * i + 1) * LP + lane] = c_mul(u, wlane);   // dy = r - L
         }
