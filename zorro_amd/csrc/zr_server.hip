@@ -310,9 +310,13 @@ struct zr_server {
         break;
       }
       if (!dead.empty()) {
+        // plans evicted for this pass are destroyed outside the lock; the
+        // state may change meanwhile (a movie finishing frees its plan), so a
+        // pass that admitted nothing is repeated rather than judged stale
         lk.unlock();
         destroy_plans(dead);
         lk.lock();
+        if (!found) continue;
       }
       if (!found) {
         if (inflight == 0 && creating == 0 && !pending.empty()) {

@@ -1330,6 +1330,15 @@ __global__ __launch_bounds__(256) void k_frc_rings(const double* __restrict__ fr
   if (threadIdx.x == 0) frc[k] = (sn[0] + sn[1] + sn[2] + sn[3]) / (sd[0] + sd[1] + sd[2] + sd[3]);
 }
 
+// Column kx of frame J is past the frame's cut-off everywhere when its
+// smallest frequency (ky = 0, computed as the column pass computes q) is:
+// the dose filter is exactly 0 on the whole column, which then adds nothing.
+ZR_DEV bool dose_column_dead(int kx, double ipx, double thresq, float cut_skip) {
+  const double fx = kx;
+  const float q0 = (float)sqrt(fx * fx * ipx * ipx);
+  return q0 / (float)thresq > cut_skip;
+}
+
 // ------------------------------------------------------------------ KD
 // Direct dose path, row pass over a chunk of frames: the aligned frame (roll +
 // 3x3 Lanczos kernel), symmetric-padded to Hp x WP, real FFT of each padded
@@ -1343,7 +1352,8 @@ constexpr int dose_rows_tpr(int wp) {
 template <int WP>
 __global__ __launch_bounds__(dose_rows_rb(WP) * dose_rows_tpr(WP)) void k_dose_rows(
     const float* __restrict__ frames, const FrameShift* __restrict__ fs, int j0, int H, int W, int Hp,
-    const float2* __restrict__ twL, const float2* __restrict__ tw2L, float2* __restrict__ T) {
+    const float2* __restrict__ twL, const float2* __restrict__ tw2L, const double* __restrict__ thresq,
+    double ipx, float cut_skip, float2* __restrict__ T) {
   constexpr int L = WP / 2, RB = dose_rows_rb(WP), TPR = dose_rows_tpr(WP), NT = RB * TPR, PER = L / TPR;
   static_assert(L % TPR == 0, "row length must split evenly over the row group");
   extern __shared__ float2 kd_lds[];
@@ -1357,18 +1367,25 @@ __global__ __launch_bounds__(dose_rows_rb(WP) * dose_rows_tpr(WP)) void k_dose_r
   const int Y = y0 + g;                        // padded row
   const FrameShift sh = reduce_shift(fs[J], H, W);
   const RowSrc rs = row_sources(frames + (size_t)J * H * W, sh, sym_idx(Y, H), H, W);
+  // the padded row as WP floats in natural order (complex element n holds
+  // pixels 2n, 2n+1), lanes on consecutive pixels so that every source-row
+  // load of a wave is one contiguous segment
+  float* rowf = reinterpret_cast<float*>(buf[g]);
 #pragma unroll
-  for (int i = 0; i < PER; ++i) {
-    const int n = gt + i * TPR, x0 = 2 * n, x1 = 2 * n + 1;
-    buf[g][n] = make_float2(aligned_px_row(rs, sh, x0 < W ? x0 : sym_idx(x0, W), W),
-                            aligned_px_row(rs, sh, x1 < W ? x1 : sym_idx(x1, W), W));
+  for (int i = 0; i < 2 * PER; ++i) {
+    const int X = gt + i * TPR;
+    rowf[X] = aligned_px_row(rs, sh, X < W ? X : (X < 2 * W ? 2 * W - 1 - X : sym_idx(X, W)), W);
   }
   __shared__ float2 twl_[zr_ldstw_size(L)];
   const LdsTw tws_ = zr_ldstw_fill(twl_, twL, L, threadIdx.x, blockDim.x);
   __syncthreads();
   wg_fft<L, -1, TPR>(buf[g], tws_, gt);
   const int Wh = L + 1, KP = (Wh + 1) / 2;
+  const double tq = thresq[J];
   for (int kp = tid; kp < KP; kp += NT) {
+    // a pair of columns past the frame's cut-off is never read by the column
+    // pass (the second column's frequencies are higher than the first's)
+    if (dose_column_dead(2 * kp, ipx, tq, cut_skip)) continue;
     float2 out[2][RB];
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
@@ -1415,6 +1432,7 @@ struct DoseArgs {
   // frequencies of a frame add nothing to the filtered sum
   float cut_skip;
 };
+
 
 // dose per frame from mean(imageSum)/N, accumulated dose, per-frame cutoff
 // radius: the last radial index whose optimal exposure is closer to the dose at
@@ -1503,6 +1521,9 @@ __global__ __launch_bounds__(dose_cols_nt(HP)) void k_dose_cols(const float2* __
   __shared__ float2 twl_[zr_ldstw_size(HP)];
   const LdsTw tws_ = zr_ldstw_fill(twl_, twH, HP, threadIdx.x, blockDim.x);
   for (int J = j0; J < j1; ++J) {
+    // frame J's filter is exactly 0 on this whole column: it adds +0 (the row
+    // pass wrote nothing there), so its load and transform are skipped
+    if (dose_column_dead(kx, a.ipx, a.thresq[J], a.cut_skip)) continue;
     const float2* src = T + (((size_t)(J - j0) * KP + kx / 2) * HP) * 2 + (kx & 1);
     for (int i = 0; i < PER; ++i) buf[tid + i * NT] = src[2 * (tid + i * NT)];
     __syncthreads();
@@ -2754,7 +2775,7 @@ int zr_stage_dose(zr_stage_state* s, const float* frames, const float2* S, int W
     static std::once_flag lds_once;                                                                               \
     zr_allow_full_lds((const void*)k_dose_rows<X>, lds_once);                                                     \
     hipLaunchKernelGGL(k_dose_rows<X>, dim3((j1 - j0) * (Hp / RB)), dim3(NT), sh, st, frames, fs, j0, H, W, Hp, \
-                       s->twWp2, s->twWp, s->Tdose);                                                              \
+                       s->twWp2, s->twWp, a.thresq, a.ipx, a.cut_skip, s->Tdose);                                  \
   } break;
       ZR_DR(256) ZR_DR(512) ZR_DR(1024) ZR_DR(2048) ZR_DR(3072) ZR_DR(3840) ZR_DR(4096) ZR_DR(5760)
 #undef ZR_DR
