@@ -1344,8 +1344,9 @@ ZR_DEV bool dose_column_dead(int kx, double ipx, double thresq, float cut_skip) 
 // 3x3 Lanczos kernel), symmetric-padded to Hp x WP, real FFT of each padded
 // row, written transposed in column pairs T[fb][k/2][Y][k%2] so that a thread
 // stores RB rows x 2 columns (16 RB contiguous bytes).  One block = RB padded
-// rows of one frame, one group of TPR threads per row.
-constexpr int dose_rows_rb(int wp) { return wp >= 3072 ? 4 : 8; }
+// rows of one frame, one group of TPR threads per row (two rows at 5760, where
+// four would leave one block per CU).
+constexpr int dose_rows_rb(int wp) { return wp == 5760 ? 2 : (wp >= 3072 ? 4 : 8); }
 constexpr int dose_rows_tpr(int wp) {
   return (wp >= 2048 && (wp / 2) % 128 == 0) ? 128 : ((wp / 2) % 192 == 0 ? 192 : 64);
 }

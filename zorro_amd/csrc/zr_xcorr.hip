@@ -1410,8 +1410,9 @@ __global__ __launch_bounds__(256) void k_fill_zero(uint32_t* __restrict__ p, siz
 #include <algorithm>
 
 // rows per k1_rows block for a padded width (also the granularity of the
-// frame-sum partials)
-constexpr int k1_rows_per_block(int wp) { return wp >= 3072 ? 4 : 8; }
+// frame-sum partials); at 5760 two rows (49 KB of LDS, three blocks per CU)
+// beat four (98 KB, one block per CU)
+constexpr int k1_rows_per_block(int wp) { return wp == 5760 ? 2 : (wp >= 3072 ? 4 : 8); }
 int zr_k1_rows_per_block(int Wp) { return k1_rows_per_block(Wp); }
 // threads per row: the half-length L = WP/2 must split evenly over them
 constexpr int k1_threads_per_row(int wp) {
