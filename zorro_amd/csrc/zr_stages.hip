@@ -991,9 +991,12 @@ ZR_DEV int sym_idx(int i, int n) {
 
 // Aligned frame = roll by (iy, ix) then 3x3 convolution with reflect edges:
 //   out[y,x] = sum_ab K[a][b] g[refl(y+1-a), refl(x+1-b)],  g[i,j] = f[(i-iy) mod H, (j-ix) mod W]
-// RowSrc holds the three source rows of one output row (computed once per row).
+// RowSrc holds the frame as a buffer resource and the byte offsets of the three
+// source rows of one output row (computed once per row; wave-uniform when the
+// row is, so that they live in scalar registers).
 struct RowSrc {
-  const float* r[3];
+  __amdgpu_buffer_rsrc_t rsrc;
+  int ro[3];
 };
 // wrap v into [0, n) for v in (-n, 2n)
 ZR_DEV int wrap1(int v, int n) {
@@ -1002,25 +1005,27 @@ ZR_DEV int wrap1(int v, int n) {
 }
 ZR_DEV RowSrc row_sources(const float* __restrict__ f, const FrameShift& s, int y, int H, int W) {
   RowSrc rs;
+  rs.rsrc = __builtin_amdgcn_make_buffer_rsrc((void*)f, (short)0, (int)((size_t)H * W * sizeof(float)), 0x00020000);
   const int iy = s.iy;
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
     const int yy = s.conv ? wrap1(refl1(y + 1 - a, H) - iy, H) : wrap1(y - iy, H);
-    rs.r[a] = f + (size_t)yy * W;
+    rs.ro[a] = __builtin_amdgcn_readfirstlane(yy * W * (int)sizeof(float));
   }
   return rs;
 }
+ZR_DEV float rs_load(const RowSrc& rs, int a, int col) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rs.rsrc, col * 4, rs.ro[a], 0));
+}
 ZR_DEV float aligned_px_row(const RowSrc& rs, const FrameShift& s, int x, int W) {
-  if (!s.conv) return rs.r[0][wrap1(x - s.ix, W)];
+  if (!s.conv) return rs_load(rs, 0, wrap1(x - s.ix, W));
   const int xm = wrap1(refl1(x + 1, W) - s.ix, W);
   const int x0 = wrap1(x - s.ix, W);
   const int xp = wrap1(refl1(x - 1, W) - s.ix, W);
   float acc = 0.f;
 #pragma unroll
-  for (int a = 0; a < 3; ++a) {
-    const float* row = rs.r[a];
-    acc += s.k[a * 3 + 0] * row[xm] + s.k[a * 3 + 1] * row[x0] + s.k[a * 3 + 2] * row[xp];
-  }
+  for (int a = 0; a < 3; ++a)
+    acc += s.k[a * 3 + 0] * rs_load(rs, a, xm) + s.k[a * 3 + 1] * rs_load(rs, a, x0) + s.k[a * 3 + 2] * rs_load(rs, a, xp);
   return acc;
 }
 // integer shifts reduced once per frame so |shift| < size (the wrap helpers need it)
