@@ -149,7 +149,7 @@ struct zr_movie {
   float* frames_owned = nullptr;
   // tables
   float2 *twWp2 = nullptr, *twWp = nullptr, *twHp = nullptr, *twHc = nullptr, *twWc = nullptr, *ktab = nullptr;
-  float2* k2btab = nullptr;        // [Wh][2m] twiddles of the matrix-core row pass (NULL: vector pass)
+  float2* k2btab = nullptr;        // [zr_k2_btab_rows(Wh)][2m] twiddles of the matrix-core row pass (NULL: vector pass)
   float2* k2ltw = nullptr;         // [L+1][64] lane twiddles of the m <= L column pass (NULL: general pass)
   float *gy = nullptr, *gx = nullptr;
   int ry = 0, rx = 0;
@@ -594,7 +594,7 @@ static int create_movie(int device, const zr_params* p, int n_frames, int h, int
     return rc;
   }
   if ((2 * mv->m) % 16 == 0) {
-    if ((rc = dev_alloc(&mv->k2btab, (size_t)mv->Wh * 2 * mv->m))) { free_movie(mv); return rc; }
+    if ((rc = dev_alloc(&mv->k2btab, (size_t)zr_k2_btab_rows(mv->Wh) * 2 * mv->m))) { free_movie(mv); return rc; }
     e = zr_launch_k2_twiddles(mv->Wh, mv->Wc, mv->m, mv->k2btab, mv->own_stream);
     if (e == hipSuccess) e = hipStreamSynchronize(mv->own_stream);
     if (e != hipSuccess) { free_movie(mv); return fail(ZR_ERR_HIP, "k2 twiddles: %s", hipGetErrorString(e)); }

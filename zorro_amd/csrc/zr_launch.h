@@ -79,6 +79,9 @@ hipError_t zr_launch_k2_rows(const float2* Z, const int4* pairs, int npairs, int
                              const float2* twWc, const float2* btab, const double* den, float nc2, float* windows,
                              const int4* sub, hipStream_t st);
 hipError_t zr_launch_k2_twiddles(int Wh, int Wc, int m, float2* btab, hipStream_t st);
+// rows of the matrix-core twiddle table: Wh padded to whole pairs of the pass's
+// kx chunks (the padding rows are zero)
+int zr_k2_btab_rows(int Wh);
 hipError_t zr_launch_k2_den(const int4* pairs, int npairs, const double* stats, double npix, double nc2, double* den,
                             hipStream_t st);
 hipError_t zr_launch_peaks(const float* windows, const int4* pairs, int npairs, const PeakParamsHost* hp,

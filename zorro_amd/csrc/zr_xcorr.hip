@@ -852,84 +852,114 @@ __global__ __launch_bounds__(1024) void k2_rows(const float2* __restrict__ Z, co
 
 // Pass 2 on the matrix cores (2m a multiple of 16).  The row DFT over kx is a
 // real GEMM per pair with K = 2 Wh (real and imaginary parts along kx):
-//   A[(kx, c)][dy] = Z[kx][dy].c,  B[(kx, c)][dx] = (cos, -sin)(2 pi kx (dx - m) / Wc)
-//   W[dy][dx] = sum_K A B = sum_kx Re(Z[kx][dy] e^{+2 pi i kx (dx - m) / Wc}),
-// then C = nc2 W / max(1, den).  One block per pair; wave w owns the 16-row tile
-// strip dy in [16w, 16w + 16) and all 2m/16 column tiles; K in chunks of
-// K2M_KX columns kx staged through LDS (the twiddle table B is shared by all
-// pairs and stays in L2).  v_mfma_f32_16x16x4f32: lane l supplies A[l % 16][l / 16]
-// and B[l / 16][l % 16] of a 16x4 / 4x16 step and holds D[4 (l / 16) + v][l % 16].
+//   W[dy][dx] = sum_kx Re(Z[kx][dy] e^{+2 pi i kx (dx - m) / Wc}) = sum_kx (Zr Bc + Zi Bs)
+// with Btab[kx][dx] = (Bc, Bs) = (cos, -sin)(2 pi kx (dx - m) / Wc), then
+// C = nc2 W / max(1, den).  v_mfma_f32_16x16x4f32: lane l supplies
+// A[l % 16][l / 16] and B[l / 16][l % 16] of a 16x4 / 4x16 step and holds
+// D[4 (l / 16) + v][l % 16].
+//
+// Work item = (pair, 16-row strip dy in [16 s, 16 s + 16)) with all 2m/16
+// column tiles, one wave each; a block is K2M_WPB = 4 items, so that its waves
+// fill the CU's four SIMDs evenly (one block per pair, with 2m/16 = 5 waves,
+// left one SIMD with two of them and every barrier waited for it).  Any items
+// can share a block: the twiddle table is the same for every pair.  K order:
+// for the four columns kx = kx0 + 4u + (0..3) one step takes the real parts and
+// the next the imaginary parts, so both of a lane's A operands are one float2
+// of Z (column kx0 + 4u + l / 16, row 16 s + l % 16).  The strip is the wave's
+// alone, so Z goes from global memory straight into registers one chunk ahead
+// of its use; the twiddle chunk (L2-resident) is staged through a
+// double-buffered LDS tile with one barrier per chunk.  The table's zero rows
+// pad Wh to an even number of chunks: the chunks are taken two at a time with
+// no branch in the loop body (a branch lets the compiler sink the next chunk's
+// Z loads past the current chunk's MFMAs), and the clamped Z reads past the
+// last column meet zero twiddles.
 typedef float zr_v4f __attribute__((ext_vector_type(4)));
-constexpr int K2M_KX = 32;           // kx columns per LDS chunk (K = 64)
-// NT = 2m/16 tile strips (one wave each).  Per chunk every thread holds
-// K2M_KX*2m/(64 NT) = 8 elements of Z and of the twiddles; the next chunk's
-// loads are in flight while the current chunk's MFMAs run.
+constexpr int K2M_WPB = 4;           // work items (waves) per block
+constexpr int K2M_KC = 16;           // kx per chunk (LDS: 2 x K2M_KC x 2m float2)
 template <int NT>
-__global__ __launch_bounds__(64 * NT) void k2_rows_mfma(const float2* __restrict__ Z, int Wh,
-                                                        const float2* __restrict__ Btab,
-                                                        const double* __restrict__ den, float nc2,
-                                                        float* __restrict__ windows, const int4* __restrict__ sub) {
-  constexpr int TWOM = 16 * NT, NTH = 64 * NT;
-  constexpr int PER = K2M_KX * TWOM / NTH;        // = 8
-  constexpr int ROW = 2 * TWOM + 4;               // floats per kx row in LDS (padded)
-  __shared__ float As[K2M_KX * ROW];               // [kx][dy][c]
-  __shared__ float Bs[K2M_KX * ROW];               // [kx][dx][c]
+__global__ __launch_bounds__(64 * K2M_WPB) void k2_rows_mfma(const float2* __restrict__ Z, int Wh, int nitems,
+                                                             const float2* __restrict__ Btab,
+                                                             const double* __restrict__ den, float nc2,
+                                                             float* __restrict__ windows,
+                                                             const int4* __restrict__ sub) {
+  constexpr int TWOM = 16 * NT, NTH = 64 * K2M_WPB;
+  constexpr int KC = K2M_KC, UG = KC / 4;            // four-column groups per chunk
+  constexpr int PERB = KC * TWOM / NTH;             // twiddle float2 per thread per chunk
+  static_assert(PERB * NTH == KC * TWOM, "the twiddle chunk splits evenly over the block");
+  __shared__ float2 Bs[2][KC * TWOM];               // [buffer][kx][dx]
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-  const int p = sub ? sub[blockIdx.x].z : blockIdx.x;     // sub: a subset of the pairs (preShift diagonals)
-  const float2* zp = Z + (size_t)p * Wh * TWOM;
-  const int i16 = lane & 15, kq = lane >> 4;
+  // past the last item a wave only helps stage the twiddles
+  const int item = min((int)blockIdx.x * K2M_WPB + w, nitems - 1);
+  const int pi = item / NT, strip = item - pi * NT;
+  const int p = sub ? sub[pi].z : pi;               // sub: a subset of the pairs (preShift diagonals)
+  const float2* zp = Z + (size_t)p * Wh * TWOM + 16 * strip + (lane & 15);
+  const int kq = lane >> 4;
+  const int nch = (Wh + 2 * KC - 1) / (2 * KC) * 2;
   zr_v4f acc[NT];
 #pragma unroll
   for (int t = 0; t < NT; ++t) acc[t] = zr_v4f{0.f, 0.f, 0.f, 0.f};
-  float2 ra[PER], rb[PER];
-  auto load = [&](int kx0) {
+  auto loadA = [&](float2 (&a)[UG], int c) {
 #pragma unroll
-    for (int i = 0; i < PER; ++i) {
-      const int e = tid + i * NTH;                 // e = kl * TWOM + j
-      const int kx = kx0 + e / TWOM;
-      const bool ok = kx < Wh;
-      ra[i] = ok ? zp[(size_t)kx0 * TWOM + e] : make_float2(0.f, 0.f);
-      rb[i] = ok ? Btab[(size_t)kx0 * TWOM + e] : make_float2(0.f, 0.f);
-    }
+    for (int u = 0; u < UG; ++u) a[u] = zp[(size_t)min(c * KC + 4 * u + kq, Wh - 1) * TWOM];
   };
-  load(0);
-  for (int kx0 = 0; kx0 < Wh; kx0 += K2M_KX) {
-    __syncthreads();
+  float2 rb[PERB];
+  auto loadB = [&](int c) {
 #pragma unroll
-    for (int i = 0; i < PER; ++i) {
-      const int e = tid + i * NTH;
-      const int kl = e / TWOM, j = e - kl * TWOM;
-      *reinterpret_cast<float2*>(&As[kl * ROW + 2 * j]) = ra[i];
-      *reinterpret_cast<float2*>(&Bs[kl * ROW + 2 * j]) = rb[i];
+    for (int i = 0; i < PERB; ++i) rb[i] = Btab[(size_t)c * KC * TWOM + tid + i * NTH];
+  };
+  auto storeB = [&](int buf) {
+#pragma unroll
+    for (int i = 0; i < PERB; ++i) Bs[buf][tid + i * NTH] = rb[i];
+  };
+  // chunk c: its MFMAs (A from a, B from Bs[c & 1]); then chunk c + 1's
+  // twiddles go to the other buffer and chunk c + 2's are requested
+  auto chunk = [&](const float2 (&a)[UG], int c) {
+    const float2* bs = Bs[c & 1] + kq * TWOM + (lane & 15);
+#pragma unroll
+    for (int u = 0; u < UG; ++u) {
+      __builtin_amdgcn_sched_barrier(0);   // one group's twiddle reads live at a time
+      float2 b[NT];
+#pragma unroll
+      for (int t = 0; t < NT; ++t) b[t] = bs[4 * u * TWOM + 16 * t];
+#pragma unroll
+      for (int t = 0; t < NT; ++t) acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[u].x, b[t].x, acc[t], 0, 0, 0);
+#pragma unroll
+      for (int t = 0; t < NT; ++t) acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[u].y, b[t].y, acc[t], 0, 0, 0);
     }
-    __syncthreads();
-    if (kx0 + K2M_KX < Wh) load(kx0 + K2M_KX);
-#pragma unroll
-    for (int s2 = 0; s2 < K2M_KX / 2; ++s2) {
-      // K index k = 2 kl + c: this step covers kl = 2 s2, 2 s2 + 1
-      const int kl = 2 * s2 + (kq >> 1), c = kq & 1;
-      const float a = As[kl * ROW + 2 * (16 * w + i16) + c];
-      float b[NT];
-#pragma unroll
-      for (int t = 0; t < NT; ++t) b[t] = Bs[kl * ROW + 2 * (16 * t + i16) + c];
-#pragma unroll
-      for (int t = 0; t < NT; ++t) acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b[t], acc[t], 0, 0, 0);
-    }
+    __builtin_amdgcn_sched_barrier(0);
+    storeB((c + 1) & 1);                   // after the last chunk: padding rows, never read
+    zr_lds_barrier();                      // LDS only: the Z loads stay in flight
+    loadB(min(c + 2, nch - 1));
+  };
+  float2 a0[UG], a1[UG];
+  loadB(0);
+  loadA(a0, 0);
+  storeB(0);
+  loadB(1);
+  zr_lds_barrier();
+  for (int c = 0; c < nch; c += 2) {
+    loadA(a1, c + 1);
+    chunk(a0, c);
+    loadA(a0, c + 2);
+    chunk(a1, c + 1);
   }
+  if ((int)blockIdx.x * K2M_WPB + w >= nitems) return;
   const float sc = (float)((double)nc2 / fmax(1.0, den[p]));
   float* wdst = windows + (size_t)p * TWOM * TWOM;
 #pragma unroll
   for (int t = 0; t < NT; ++t)
 #pragma unroll
-    for (int v = 0; v < 4; ++v) wdst[(16 * w + 4 * kq + v) * TWOM + 16 * t + i16] = acc[t][v] * sc;
+    for (int v = 0; v < 4; ++v) wdst[(16 * strip + 4 * kq + v) * TWOM + 16 * t + (lane & 15)] = acc[t][v] * sc;
 }
 
 // Twiddles of the matrix-core row pass: Btab[kx][dx] = (cos, -sin)(2 pi kx (dx - m) / Wc),
-// phase index kx (dx - m) mod Wc taken exactly in integers, FP64 sin/cos.
-__global__ __launch_bounds__(256) void k2_rows_twiddles(int Wh, int Wc, int m, float2* __restrict__ Btab) {
+// phase index kx (dx - m) mod Wc taken exactly in integers, FP64 sin/cos; rows
+// Wh .. Whp-1 (the padding to whole pairs of the pass's chunks) are zero.
+__global__ __launch_bounds__(256) void k2_rows_twiddles(int Wh, int Whp, int Wc, int m, float2* __restrict__ Btab) {
   const int twoM = 2 * m;
-  for (int e = blockIdx.x * 256 + threadIdx.x; e < Wh * twoM; e += gridDim.x * 256) {
+  for (int e = blockIdx.x * 256 + threadIdx.x; e < Whp * twoM; e += gridDim.x * 256) {
     const int kx = e / twoM, dx = e - kx * twoM;
+    if (kx >= Wh) { Btab[e] = make_float2(0.f, 0.f); continue; }
     const long d = (((long)(dx - m) % Wc) + Wc) % Wc;
     const long ph = ((long)kx * d) % Wc;
     const double a = 2.0 * 3.14159265358979323846 * (double)ph / (double)Wc;
@@ -1554,9 +1584,10 @@ hipError_t zr_launch_k2_rows(const float2* Z, const int4* pairs, int npairs, int
                              const float2* twWc, const float2* btab, const double* den, float nc2, float* windows,
                              const int4* sub, hipStream_t st) {
   int twoM = 2 * m;
+  if (npairs <= 0) return hipSuccess;
   if (btab && twoM % 16 == 0 && twoM <= 128) {
     switch (twoM / 16) {
-#define ZR_MF(T) case T: hipLaunchKernelGGL(k2_rows_mfma<T>, dim3(npairs), dim3(64 * T), 0, st, Z, Wh, btab, den, nc2, windows, sub); return hipGetLastError();
+#define ZR_MF(T) case T: hipLaunchKernelGGL(k2_rows_mfma<T>, dim3((npairs * T + K2M_WPB - 1) / K2M_WPB), dim3(64 * K2M_WPB), 0, st, Z, Wh, npairs * T, btab, den, nc2, windows, sub); return hipGetLastError();
       ZR_MF(1) ZR_MF(2) ZR_MF(3) ZR_MF(4) ZR_MF(5) ZR_MF(6) ZR_MF(7) ZR_MF(8)
 #undef ZR_MF
       default: break;
@@ -1645,9 +1676,12 @@ hipError_t zr_launch_fill_zero(void* p, size_t bytes, hipStream_t st) {
   return hipGetLastError();
 }
 
+int zr_k2_btab_rows(int Wh) { return (Wh + 2 * K2M_KC - 1) / (2 * K2M_KC) * (2 * K2M_KC); }
+
 hipError_t zr_launch_k2_twiddles(int Wh, int Wc, int m, float2* btab, hipStream_t st) {
-  const int n = Wh * 2 * m;
-  hipLaunchKernelGGL(k2_rows_twiddles, dim3((n + 255) / 256), dim3(256), 0, st, Wh, Wc, m, btab);
+  const int Whp = zr_k2_btab_rows(Wh);
+  const int n = Whp * 2 * m;
+  hipLaunchKernelGGL(k2_rows_twiddles, dim3((n + 255) / 256), dim3(256), 0, st, Wh, Whp, Wc, m, btab);
   return hipGetLastError();
 }
 
