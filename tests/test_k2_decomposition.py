@@ -43,3 +43,37 @@ def test_decomposition_equals_oracle_window(Hp, Hc, Wc):
     den = max(1.0, nc2 * np.sqrt((Hp * Hp * (g * g).sum() - g.sum() ** 2) * (Hp * Hp * (f * f).sum() - f.sum() ** 2)))
     Wd = _window_by_decomposition(F, G, Hp, Hc, Wc, m) * nc2 / den
     assert np.max(np.abs(Wd - Wo)) <= 1e-12 * np.max(np.abs(Wo))
+
+
+def _inv(x, m):
+    return next(i for i in range(1, m) if (x * i) % m == 1)
+
+
+@pytest.mark.parametrize("L", [32, 48, 64])
+def test_lane_dft_split_and_residue_map(L):
+    """The length-L lane DFT of the pruned column pass (Y[r] = sum_b v[b]
+    e^{+2 pi i b r / L}) as R1 = L/16 point DFTs then sixteen-point DFTs.  R1
+    odd: prime-factor split, input n = (16 n1 + R1 n2) mod L, no twiddles,
+    output k2 of group k1 at r = (16 a k1 + R1 b k2) mod L, a = 16^-1 mod R1,
+    b = R1^-1 mod 16.  R1 even: Cooley-Tukey, n = 16 n1 + n2, twiddles
+    e^{+2 pi i n2 k1 / L}, r = k1 + R1 k2."""
+    rng = np.random.default_rng(L)
+    v = rng.standard_normal(L) + 1j * rng.standard_normal(L)
+    R1 = L // 16
+    direct = np.array([np.sum(v * np.exp(2j * np.pi * np.arange(L) * r / L)) for r in range(L)])
+    pfa = R1 % 2 == 1
+    got = np.zeros(L, complex)
+    for k1 in range(R1):
+        u = np.zeros(16, complex)
+        for n2 in range(16):
+            idx = [(16 * n1 + R1 * n2) % L if pfa else 16 * n1 + n2 for n1 in range(R1)]
+            t = np.sum(v[idx] * np.exp(2j * np.pi * np.arange(R1) * k1 / R1))
+            u[n2] = t if pfa else t * np.exp(2j * np.pi * n2 * k1 / L)
+        y = np.array([np.sum(u * np.exp(2j * np.pi * np.arange(16) * k2 / 16)) for k2 in range(16)])
+        for k2 in range(16):
+            r = (16 * _inv(16 % R1, R1) * k1 + R1 * _inv(R1, 16) * k2) % L if pfa else k1 + R1 * k2
+            got[r] = y[k2]
+    np.testing.assert_allclose(got, direct, rtol=0, atol=1e-10 * L)
+    if pfa:   # every residue is produced exactly once
+        rs = {(16 * _inv(16 % R1, R1) * k1 + R1 * _inv(R1, 16) * k2) % L for k1 in range(R1) for k2 in range(16)}
+        assert rs == set(range(L))

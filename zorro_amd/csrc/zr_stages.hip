@@ -276,9 +276,9 @@ ZR_DEV bool lex_less(double d1, int e1, double d2, int e2) { return d1 < d2 || (
 
 // dynamic LDS of k_l1_solve: matrix region (Cholesky / slope sums / sort),
 // then b[M], w[M], breakpoints[M] (double), endpoints[M], tree flags[M],
-// breakpoint edges[M] (int)
+// breakpoint edges[M], next parallel edge[M] (int)
 inline __host__ __device__ size_t l1_dyn_bytes(int M) {
-  return sizeof(double) * ((size_t)MAXN * L1_LDC + 3 * (size_t)M) + 3 * sizeof(int) * (size_t)M;
+  return sizeof(double) * ((size_t)MAXN * L1_LDC + 3 * (size_t)M) + 4 * sizeof(int) * (size_t)M;
 }
 
 __global__ __launch_bounds__(L1_NT) void k_l1_solve(SolveArgs a) {
@@ -294,6 +294,7 @@ __global__ __launch_bounds__(L1_NT) void k_l1_solve(SolveArgs a) {
   int* es_s = reinterpret_cast<int*>(cd_s + M);   // i | j << 8
   int* tfl = es_s + M;
   int* ce_s = tfl + M;                            // breakpoint list: edges
+  int* dn_s = ce_s + M;                           // next edge with the same (i, j), or -1
   __shared__ double pvs[MAXN], idiag[MAXN];
   __shared__ int t_par[MAXN], t_ped[MAXN];
   __shared__ unsigned long long t_sub[MAXN];
@@ -310,6 +311,7 @@ __global__ __launch_bounds__(L1_NT) void k_l1_solve(SolveArgs a) {
   int eij[L1_EPT];
   double eb[L1_EPT], ew[L1_EPT], er[L1_EPT];
   unsigned tmask = 0u, smask = 0u;      // bit k: tree edge / side +1
+  unsigned dfirst = 0u;                 // bit k: first edge of its (i, j) (it sums the parallel ones)
 #pragma unroll
   for (int k = 0; k < L1_EPT; ++k) {
     const int e = tid + k * L1_NT;
@@ -328,6 +330,9 @@ __global__ __launch_bounds__(L1_NT) void k_l1_solve(SolveArgs a) {
       eij[k] = ed.x | (ed.y << 8);
       eb[k] = bb;
       ew[k] = we;
+      const int2 dp = a.dups[e];        // the parallel-edge chain, read once into LDS
+      if (dp.x == e) dfirst |= 1u << k;
+      dn_s[e] = dp.y;
     }
   }
   if (a.weight_mode == 4) {   // 'norm': (ps - min)/(max - min)
@@ -377,11 +382,9 @@ __global__ __launch_bounds__(L1_NT) void k_l1_solve(SolveArgs a) {
 #pragma unroll
     for (int k = 0; k < L1_EPT; ++k) {
       const int e = tid + k * L1_NT;
-      if (eij[k] < 0) continue;
-      const int2 dp = a.dups[e];
-      if (dp.x != e) continue;
+      if (eij[k] < 0 || !((dfirst >> k) & 1u)) continue;
       double d = cd_s[e], db = cd_s[e] * b_s[e];
-      for (int q = dp.y; q >= 0; q = a.dups[q].y) { d += cd_s[q]; db += cd_s[q] * b_s[q]; }
+      for (int q = dn_s[e]; q >= 0; q = dn_s[q]) { d += cd_s[q]; db += cd_s[q] * b_s[q]; }
       const int i = eij[k] & 255, j = eij[k] >> 8;
       Lm[j * LDC + i] = -d;
       Lm[i * LDC + j] = -db;
@@ -615,11 +618,9 @@ __global__ __launch_bounds__(L1_NT) void k_l1_solve(SolveArgs a) {
 #pragma unroll
     for (int k = 0; k < L1_EPT; ++k) {
       const int e = tid + k * L1_NT;
-      if (eij[k] < 0) continue;
-      const int2 dp = a.dups[e];
-      if (dp.x != e) continue;
+      if (eij[k] < 0 || !((dfirst >> k) & 1u)) continue;
       double f = cd_s[e];
-      for (int q = dp.y; q >= 0; q = a.dups[q].y) f += cd_s[q];
+      for (int q = dn_s[e]; q >= 0; q = dn_s[q]) f += cd_s[q];
       const int i = eij[k] & 255, j = eij[k] >> 8;
       Lm[i * LDC + j] = f;
       Lm[j * LDC + i] = -f;

@@ -579,6 +579,26 @@ __global__ __launch_bounds__(64 * WPB, 2) void k2_cols(const float2* __restrict_
 // e^{+2 pi i kx cx/Wc}.  s0 != NULL also stores c(kx) Re sum_ky x per column
 // (the lag-0 correlation's column sums, for the origin value).
 constexpr int K2R2_RC = 8, K2R2_B1 = 32, K2R2_WPB = 4, K2R2_LP = 65, K2R2_OCC = 2;
+// Residue r of output k2 of sixteen-point group k1 in the length-L = 16 R1 lane
+// DFT of k2_cols_r2.  R1 odd (coprime to 16): the prime-factor (Good-Thomas)
+// split, input n = (16 n1 + R1 n2) mod L, output r = (16 a k1 + R1 b k2) mod L
+// with a = 16^-1 mod R1, b = R1^-1 mod 16, which needs no twiddles between the
+// R1-point and sixteen-point DFTs; R1 even: Cooley-Tukey, n = 16 n1 + n2,
+// r = k1 + R1 k2, with the twiddles.
+constexpr int k2r2_inv(int x, int m) {
+  for (int i = 1; i < m; ++i)
+    if ((x * i) % m == 1) return i;
+  return 1;
+}
+template <int L> ZR_DEV constexpr int k2r2_residue(int k1, int k2) {
+  constexpr int R1 = L / 16;
+  if constexpr (R1 % 2 == 1) {
+    constexpr int A = 16 * k2r2_inv(16 % R1, R1), B = R1 * k2r2_inv(R1, 16);
+    return (A * k1 + B * k2) % L;
+  } else {
+    return k1 + R1 * k2;
+  }
+}
 constexpr size_t k2r2_lds_bytes(int L) {
   return ((size_t)K2R2_WPB * 2 * K2R2_RC * K2R2_LP + (size_t)(L + 1) * 64) * sizeof(float2);
 }
@@ -694,27 +714,30 @@ __global__ __launch_bounds__(64 * WPB, K2R2_OCC) void k2_cols_r2(const float2* _
       v[0] = dc ? make_float2(0.f, 0.f) : v[0];
       v[L / 2] = l0 ? nv : v[L / 2];
     }
-    // The length-L DFT as L/16-point DFTs then L/16 sixteen-point DFTs
-    // (DftCT<L/16, 16>: input n = 16 n1 + n2, output r = k1 + (L/16) k2), with
-    // each sixteen-point group's residues twiddled and lane-summed as soon as
-    // it is done, so that at most one group's outputs are live beside the
-    // remaining inputs.
+    // The length-L DFT as L/16-point DFTs then L/16 sixteen-point DFTs (output
+    // k2 of group k1 is residue k2r2_residue<L>(k1, k2)), with each
+    // sixteen-point group's residues twiddled and lane-summed as soon as it is
+    // done, so that at most one group's outputs are live beside the remaining
+    // inputs.
     constexpr int R1 = L / 16;
+    constexpr bool PFA = (R1 % 2) == 1;
     static_assert(16 % RC == 0, "a group's sixteen residues split into whole chunks");
     float2 t[L];
 #pragma unroll
     for (int n2 = 0; n2 < 16; ++n2) {
       float2 u[R1];
 #pragma unroll
-      for (int n1 = 0; n1 < R1; ++n1) u[n1] = v[16 * n1 + n2];
+      for (int n1 = 0; n1 < R1; ++n1) u[n1] = v[PFA ? (16 * n1 + R1 * n2) % L : 16 * n1 + n2];
       Dft<R1, 1>::run(u);
 #pragma unroll
       for (int k1 = 0; k1 < R1; ++k1) t[n2 * R1 + k1] = u[k1];
     }
+    if constexpr (!PFA) {
 #pragma unroll
-    for (int n2 = 1; n2 < 16; ++n2)
+      for (int n2 = 1; n2 < 16; ++n2)
 #pragma unroll
-      for (int k1 = 1; k1 < R1; ++k1) t[n2 * R1 + k1] = c_mul(t[n2 * R1 + k1], DftCT<R1, 16, 1>::tw_ct(n2 * k1));
+        for (int k1 = 1; k1 < R1; ++k1) t[n2 * R1 + k1] = c_mul(t[n2 * R1 + k1], DftCT<R1, 16, 1>::tw_ct(n2 * k1));
+    }
     float2 xmod = make_float2(cw, 0.f);
     float2* zdst = Z + ((size_t)p * Wh + kx) * twoM;
 #pragma unroll
@@ -724,9 +747,9 @@ __global__ __launch_bounds__(64 * WPB, K2R2_OCC) void k2_cols_r2(const float2* _
       // LDS latency hides behind the DFT
       float2 tw16[16];
 #pragma unroll
-      for (int n = 0; n < 16; ++n) tw16[n] = tab[(k1 + R1 * n) * 64 + lane];
+      for (int n = 0; n < 16; ++n) tw16[n] = tab[k2r2_residue<L>(k1, n) * 64 + lane];
       __builtin_amdgcn_sched_barrier(0);
-      float2 y[16];                            // residues k1 + R1 k2, k2 = 0 .. 15
+      float2 y[16];                            // residues k2r2_residue<L>(k1, k2), k2 = 0 .. 15
 #pragma unroll
       for (int n2 = 0; n2 < 16; ++n2) y[n2] = t[n2 * R1 + k1];
       Dft<16, 1>::run(y);
@@ -744,7 +767,6 @@ __global__ __launch_bounds__(64 * WPB, K2R2_OCC) void k2_cols_r2(const float2* _
         __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
         for (int i = 0; i < RC; ++i) {
-          const int r = k1 + R1 * (c * RC + i);
           // rows of outputs outside [c - m, c + m) are written too and never read
           const float2 u = c_mul(y[c * RC + i], tw16[c * RC + i]);
           my[(2 * i) * LP + lane] = u;                     // dy = r
@@ -752,7 +774,7 @@ __global__ __launch_bounds__(64 * WPB, K2R2_OCC) void k2_cols_r2(const float2* _
         }
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-        const int r = k1 + R1 * (c * RC + (slot >> 1));
+        const int r = k2r2_residue<L>(k1, c * RC + (slot >> 1));
         const int kind = slot & 1;
         // lag of this slot: kind 0 the larger of the residue's two, kind 1 that minus L
         const int dy = (PS ? r + L * j1of(r) : r) - kind * L;
