@@ -563,13 +563,14 @@ __global__ __launch_bounds__(64 * WPB, 2) void k2_cols(const float2* __restrict_
 // Pass 1 for m <= L (every residue r = dy mod L has at most two outputs:
 // dy = r and dy = r - L).  With Hc = 64 L and dy = r + L j,
 //   Z[dy] = sum_a e^{+2 pi i a r / Hc} e^{+2 pi i a j / 64} Y_a[r],
-// so lane a multiplies its Y_a[r] by the twiddle e^{+2 pi i a r/Hc} (output
-// dy = r) and that product by e^{-2 pi i a/64} (output dy = r - L): every
-// output is then a plain sum over the 64 lanes, taken through LDS, K2R2_RC
-// residues (two rows each) at a time; each row is summed by four lanes (16
-// terms each) and the four partial sums are added by two lane swaps.  The
-// twiddle table ([L+1][64], row L = e^{-2 pi i a/64}) is copied to LDS once per
-// block.  The column weight c(kx) is applied to the sums (a power of two: the
+// so lane a multiplies its Y_a[r] by the twiddle e^{+2 pi i a r/Hc}; output
+// dy = r is the plain sum of these products over the 64 lanes and output
+// dy = r - L their sum weighted by e^{-2 pi i a/64}.  The sums go through LDS,
+// one sixteen-residue group at a time: each residue's row of 64 products is
+// read by four lanes (16 terms each, lanes a = 16 p + i), each forming both
+// sums (the weight is (-i)^p e^{-2 pi i i/64}, with compile-time constants),
+// and the four partial sums are added by two lane swaps.  The twiddle table
+// ([L+1][64]) is copied to LDS once per block.  The column weight c(kx) is applied to the sums (a power of two: the
 // same result as scaling every product).  Loads: the column's rows in two
 // batches (K2R2_B1 rows of both frames, then the rest), each issued at once.
 // A window centre (cy, cx) per pair (ctr != NULL, preShift) moves the lags to
@@ -578,7 +579,17 @@ __global__ __launch_bounds__(64 * WPB, 2) void k2_cols(const float2* __restrict_
 // e^{+2 pi i a dy_a/Hc} (from twHc), and the sums are modulated by
 // e^{+2 pi i kx cx/Wc}.  s0 != NULL also stores c(kx) Re sum_ky x per column
 // (the lag-0 correlation's column sums, for the origin value).
-constexpr int K2R2_RC = 8, K2R2_B1 = 32, K2R2_WPB = 4, K2R2_LP = 65, K2R2_OCC = 2;
+constexpr int K2R2_ROWS = 16, K2R2_B1 = 32, K2R2_WPB = 4, K2R2_LP = 65, K2R2_OCC = 2;
+// cos and sin of 2 pi i / 64, i < 16 (float32 of the FP64 values): the lag
+// dy - L reader weights e^{-2 pi i i / 64}
+constexpr float K2R2_WC[16] = {1.000000000e+00f, 9.951847196e-01f, 9.807852507e-01f, 9.569403529e-01f,
+                               9.238795042e-01f, 8.819212914e-01f, 8.314695954e-01f, 7.730104327e-01f,
+                               7.071067691e-01f, 6.343932748e-01f, 5.555702448e-01f, 4.713967443e-01f,
+                               3.826834261e-01f, 2.902846634e-01f, 1.950903237e-01f, 9.801714122e-02f};
+constexpr float K2R2_WS[16] = {0.000000000e+00f, 9.801714122e-02f, 1.950903237e-01f, 2.902846634e-01f,
+                               3.826834261e-01f, 4.713967443e-01f, 5.555702448e-01f, 6.343932748e-01f,
+                               7.071067691e-01f, 7.730104327e-01f, 8.314695954e-01f, 8.819212914e-01f,
+                               9.238795042e-01f, 9.569403529e-01f, 9.807852507e-01f, 9.951847196e-01f};
 // Residue r of output k2 of sixteen-point group k1 in the length-L = 16 R1 lane
 // DFT of k2_cols_r2.  R1 odd (coprime to 16): the prime-factor (Good-Thomas)
 // split, input n = (16 n1 + R1 n2) mod L, output r = (16 a k1 + R1 b k2) mod L
@@ -600,7 +611,7 @@ template <int L> ZR_DEV constexpr int k2r2_residue(int k1, int k2) {
   }
 }
 constexpr size_t k2r2_lds_bytes(int L) {
-  return ((size_t)K2R2_WPB * 2 * K2R2_RC * K2R2_LP + (size_t)(L + 1) * 64) * sizeof(float2);
+  return ((size_t)K2R2_WPB * K2R2_ROWS * K2R2_LP + (size_t)(L + 1) * 64) * sizeof(float2);
 }
 template <int L, int WPB, bool PS>
 __global__ __launch_bounds__(64 * WPB, K2R2_OCC) void k2_cols_r2(const float2* __restrict__ S, const float2* __restrict__ Sside,
@@ -612,10 +623,8 @@ __global__ __launch_bounds__(64 * WPB, K2R2_OCC) void k2_cols_r2(const float2* _
                                                            const int2* __restrict__ ctr,
                                                            const float2* __restrict__ twHc,
                                                            const float2* __restrict__ twWc, float* __restrict__ s0) {
-  constexpr int RC = K2R2_RC;             // residues per LDS chunk
   constexpr int LP = K2R2_LP;             // padded LDS row: rows 2 dwords apart in the banks
-  constexpr int ROWS = 2 * RC;
-  constexpr int PLEN = 16;                // terms each of a row's four readers adds
+  constexpr int ROWS = K2R2_ROWS;         // one sixteen-residue group
   static_assert(ROWS * 4 == 64, "four readers per row");
   extern __shared__ float2 k2_lds[];
   const int lane = threadIdx.x & 63;
@@ -653,7 +662,6 @@ __global__ __launch_bounds__(64 * WPB, K2R2_OCC) void k2_cols_r2(const float2* _
     }
   }
   __syncthreads();
-  const float2 wlane = tab[L * 64 + lane];    // e^{-2 pi i lane / 64}
   // reader role: row slot (residue r0 + slot/2, kind slot%2), part of the lanes
   const int slot = lane & (ROWS - 1), part = lane / ROWS;
   const int kbase = kgroup * WPB * cols_per_wave;
@@ -721,7 +729,6 @@ __global__ __launch_bounds__(64 * WPB, K2R2_OCC) void k2_cols_r2(const float2* _
     // inputs.
     constexpr int R1 = L / 16;
     constexpr bool PFA = (R1 % 2) == 1;
-    static_assert(16 % RC == 0, "a group's sixteen residues split into whole chunks");
     float2 t[L];
 #pragma unroll
     for (int n2 = 0; n2 < 16; ++n2) {
@@ -761,36 +768,46 @@ __global__ __launch_bounds__(64 * WPB, K2R2_OCC) void k2_cols_r2(const float2* _
           xmod = make_float2(tt.x * cw, -tt.y * cw);
         }
       }
+      // the group's sixteen residues, u = Y_a[r] times the lane twiddle, as the
+      // rows of the wave's LDS tile (rows of lags outside the window are
+      // written too and never stored)
+      __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-      for (int c = 0; c < 16 / RC; ++c) {
-        // one chunk's twiddle reads at a time
-        __builtin_amdgcn_sched_barrier(0);
+      for (int i = 0; i < 16; ++i) my[i * LP + lane] = c_mul(y[i], tw16[i]);
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      {
+        // reader (row slot, part p = lanes a = 16 p + i): its sixteen terms as
+        // they are (lag dy) and times e^{-2 pi i a / 64} = (-i)^p e^{-2 pi i i / 64}
+        // (lag dy - L)
+        const int r = k2r2_residue<L>(k1, slot);
+        const int dy = PS ? r + L * j1of(r) : r;     // the larger of the residue's two lags
+        const float2* row = my + slot * LP + 16 * part;
+        float2 a0 = make_float2(0.f, 0.f), a1 = make_float2(0.f, 0.f);
 #pragma unroll
-        for (int i = 0; i < RC; ++i) {
-          // rows of outputs outside [c - m, c + m) are written too and never read
-          const float2 u = c_mul(y[c * RC + i], tw16[c * RC + i]);
-          my[(2 * i) * LP + lane] = u;                     // dy = r
-          my[(2 * i + 1) * LP + lane] = c_mul(u, wlane);   // dy = r - L
+        for (int i = 0; i < 16; ++i) {
+          const float2 x = row[i];
+          a0 = c_add(a0, x);
+          a1.x = fmaf(x.x, K2R2_WC[i], fmaf(x.y, K2R2_WS[i], a1.x));
+          a1.y = fmaf(x.y, K2R2_WC[i], fmaf(-x.x, K2R2_WS[i], a1.y));
         }
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-        const int r = k2r2_residue<L>(k1, c * RC + (slot >> 1));
-        const int kind = slot & 1;
-        // lag of this slot: kind 0 the larger of the residue's two, kind 1 that minus L
-        const int dy = (PS ? r + L * j1of(r) : r) - kind * L;
-        const bool valid = dy >= cc.x - m && dy <= cc.x + m - 1;
-        float2 acc = make_float2(0.f, 0.f);
         {
-          const float2* row = my + slot * LP + PLEN * part;
-#pragma unroll
-          for (int i = 0; i < PLEN; ++i) acc = c_add(acc, row[i]);
+          const bool sw = part & 1, nx = part >= 2, ny = part == 1 || part == 2;
+          const float bx = sw ? a1.y : a1.x, by = sw ? a1.x : a1.y;
+          a1 = make_float2(nx ? -bx : bx, ny ? -by : by);
         }
-        acc.x = xor32_sum(xor16_sum(acc.x));
-        acc.y = xor32_sum(xor16_sum(acc.y));
-        if (valid && part == 0) zdst[dy - cc.x + m] = PS ? c_mul(acc, xmod) : c_scale(acc, cw);
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        a0.x = xor32_sum(xor16_sum(a0.x));
+        a0.y = xor32_sum(xor16_sum(a0.y));
+        a1.x = xor32_sum(xor16_sum(a1.x));
+        a1.y = xor32_sum(xor16_sum(a1.y));
+        if (part == 0) {
+          if (dy >= cc.x - m && dy <= cc.x + m - 1) zdst[dy - cc.x + m] = PS ? c_mul(a0, xmod) : c_scale(a0, cw);
+          const int dy1 = dy - L;
+          if (dy1 >= cc.x - m && dy1 <= cc.x + m - 1) zdst[dy1 - cc.x + m] = PS ? c_mul(a1, xmod) : c_scale(a1, cw);
+        }
       }
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     }
   }
 }
