@@ -308,6 +308,29 @@ template <int L, int DIR, int NT> struct WgFft<L, L, DIR, NT> {
   static ZR_DEV void run(float2*, const TW&, int) {}
 };
 
+// First pass of wg_fft taken from registers: when the first radix R =
+// zr_next_radix(L) has one butterfly per thread (L / R == NT), thread tid
+// holds x[r] = element tid + r (L / R), r < R (the pass's own inputs), so the
+// fill of buf and the pass's reads are skipped.  The pass output goes to buf
+// in the layout of the following passes; the caller synchronises the
+// workgroup (zr_lds_barrier), then runs wg_fft_rest.
+template <int L, int NT> constexpr bool wg_fft_regs_first() { return L / zr_next_radix(L) == NT; }
+template <int L, int DIR, int NT>
+ZR_DEV void wg_fft_first_regs(float2 (&x)[zr_next_radix(L)], float2* buf, int tid) {
+  constexpr int R = zr_next_radix(L);
+  static_assert(L / R == NT, "one first-pass butterfly per thread");
+  Dft<R, DIR>::run(x);
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int i = tid * R + r;
+    buf[R == L ? i : zr_pad16(i)] = x[r];
+  }
+}
+template <int L, int DIR, int NT>
+ZR_DEV void wg_fft_rest(float2* buf, const LdsTw& tws, int tid) {
+  WgFft<L, zr_next_radix(L), DIR, NT>::run(buf, tws, tid);
+}
+
 // Whole-workgroup complex FFT of length L held in LDS (natural order in/out).
 // buf holds zr_fft_lds(L) elements (the passes in between use the padding).
 // The caller must synchronise the workgroup after filling buf; returns

@@ -102,10 +102,17 @@ __global__ __launch_bounds__(RB * TPR) void k1_rows(const float* __restrict__ fr
     const double a = v[i].x, b = v[i].y;
     s1 += a + b;
     s2 += a * a + b * b;
-    buf[g][gt + i * TPR] = v[i];
   }
-  __syncthreads();
-  wg_fft<L, -1, TPR>(buf[g], tws, gt);
+  if constexpr (wg_fft_regs_first<L, TPR>()) {   // v[i] = element gt + i TPR: the first pass's inputs
+    wg_fft_first_regs<L, -1, TPR>(v, buf[g], gt);
+    zr_lds_barrier();
+    wg_fft_rest<L, -1, TPR>(buf[g], tws, gt);
+  } else {
+#pragma unroll
+    for (int i = 0; i < PER; ++i) buf[g][gt + i * TPR] = v[i];
+    __syncthreads();
+    wg_fft<L, -1, TPR>(buf[g], tws, gt);
+  }
   // post-process and write transposed in column pairs: T[fb][k/2][y][k%2], so a
   // thread writes RB rows x 2 columns = 16 RB contiguous bytes (the odd pad
   // column of an odd Wh is written as zero)
@@ -217,11 +224,17 @@ __global__ __launch_bounds__(NT) void k1_cols1(const float2* __restrict__ T, int
   for (int i = 0; i < PER; ++i)
     if (HP % NT == 0 || tid + i * NT < HP) v[i] = src[2 * (tid + i * NT)];
   const LdsTw tws = zr_ldstw_fill(twl, twH, HP, tid, NT);
+  if constexpr (wg_fft_regs_first<HP, NT>()) {   // v[i] = element tid + i NT: the first pass's inputs
+    wg_fft_first_regs<HP, -1, NT>(v, buf, tid);
+    zr_lds_barrier();
+    wg_fft_rest<HP, -1, NT>(buf, tws, tid);
+  } else {
 #pragma unroll
-  for (int i = 0; i < PER; ++i)
-    if (HP % NT == 0 || tid + i * NT < HP) buf[tid + i * NT] = v[i];
-  __syncthreads();
-  wg_fft<HP, -1, NT>(buf, tws, tid);
+    for (int i = 0; i < PER; ++i)
+      if (HP % NT == 0 || tid + i * NT < HP) buf[tid + i * NT] = v[i];
+    __syncthreads();
+    wg_fft<HP, -1, NT>(buf, tws, tid);
+  }
   const int f = frame0 + fb;
   if (full) {
     float2* dst = S + ((size_t)f * Wh + k) * HP;
