@@ -1431,6 +1431,7 @@ struct DoseArgs {
   int nrows;
   double* accum;    // [N+1]
   double* thresq;   // [N]
+  float* fjc;       // [N][2]: float(1 / thresq), float(-accum[J+1] / 2) (the spectral pass's per-frame constants)
   SolveInfo* info;
   int order_sq;     // order = 2^order_sq, or -1
   int normalize;    // doseFilter(normalize=True): scale by 1 / (sum over frames of the filter + 1)
@@ -1482,7 +1483,12 @@ __global__ __launch_bounds__(256) void k_dose_params(DoseArgs a) {
       if (fabs(d1 - o) < fabs(d0 - o)) last = k;
     }
     for (int o = 32; o > 0; o >>= 1) last = max(last, __shfl_xor(last, o, 64));
-    if (lane == 0) a.thresq[j] = last >= 0 ? last * qs : NAN;
+    if (lane == 0) {
+      const double tq = last >= 0 ? last * qs : NAN;
+      a.thresq[j] = tq;
+      a.fjc[2 * j] = (float)(1.0 / tq);
+      a.fjc[2 * j + 1] = (float)(-0.5 * a.accum[j + 1]);
+    }
   }
 }
 
@@ -1743,7 +1749,7 @@ __global__ __launch_bounds__(DA_NT) void k_dose_accum(const float2* __restrict__
   const float2 ey = twH[(int)(((long)ky * (a.H - 1)) % a.H)];   // e^{-2 pi i ky (H-1)/H}
   const float ord = (float)a.order;
   for (int J = 0; J < a.N; ++J) {
-    const float itq = (float)(1.0 / a.thresq[J]);
+    const float itq = a.fjc[2 * J];
     // frequencies past frame J's cut-off (its filter is exactly 0 there) add
     // +0 to the sums: their loads and products are skipped, with the same bits
     float x[DA_U];
@@ -1756,7 +1762,7 @@ __global__ __launch_bounds__(DA_NT) void k_dose_accum(const float2* __restrict__
     }
     if (!any) continue;
     const float2 py = Py[(size_t)J * a.H + ky], c0 = C0[(size_t)J * a.H + ky], cw = CW[(size_t)J * a.H + ky];
-    const float mhd = (float)(-0.5 * a.accum[J + 1]);
+    const float mhd = a.fjc[2 * J + 1];
     float2 F[DA_U];
 #pragma unroll
     for (int u = 0; u < DA_U; ++u) F[u] = live[u] ? S[((size_t)J * Wk + kxs[u]) * HP + ky] : make_float2(0.f, 0.f);
@@ -1768,7 +1774,8 @@ __global__ __launch_bounds__(DA_NT) void k_dose_accum(const float2* __restrict__
       float2 sp = c_mul(c_mul(F[u], py), px);
       sp = c_add(sp, c_add(r0, c_mul(rh, ey)));
       sp = c_add(sp, c_add(c0, c_mul(cw, ex[u])));
-      const float gval = expf(-cutoff_pow(x[u], ord, a.order_sq)) * expf(mhd * invc[u]);
+      // exp(-D/(2 Nc(q))) exp(-(q/qc)^order) as one exponential (both arguments <= 0)
+      const float gval = expf(fmaf(mhd, invc[u], -cutoff_pow(x[u], ord, a.order_sq)));
       gsum[u] += gval;
       acc[u].x += sp.x * gval;
       acc[u].y += sp.y * gval;
@@ -2360,6 +2367,7 @@ struct zr_stage_state {
   // dose
   float2 *twHp = nullptr, *twWp = nullptr, *twWp2 = nullptr, *Tdose = nullptr, *G = nullptr;
   double *accum = nullptr, *thresq = nullptr;
+  float* fjc = nullptr;
   float* gtmp = nullptr;       // 'gausslp': the axis-0 pass of the Gaussian
   float* bg_rows = nullptr;    // 'background': the sigma-4 axis-0 pass at the sample rows
   float* bg_nn = nullptr;      //   the samples every 128 pixels
@@ -2409,7 +2417,7 @@ void zr_stage_free(zr_stage_state* s) {
   void* ptrs[] = {s->edges, s->meas, s->dups, s->hs, s->cdf, s->pdf, s->gtmp, s->bg_rows, s->bg_nn, s->bg_fit, s->aligned, s->fs_id, s->Gf, s->obj, s->info, s->weights, s->resid, s->pos, s->trans_raw, s->trans,
                   s->rel, s->err_stats, s->err_frames, s->fs, s->img_sum, s->even, s->odd, s->filt, s->rowsum,
                   s->twB, s->twB2, s->Tfrc, s->frc2d, s->frc, s->frc_part, s->hann, s->twHp, s->twWp, s->twWp2, s->Tdose, s->G,
-                  s->accum, s->thresq, s->R0, s->RH, s->C0, s->CW, s->Py, s->Px};
+                  s->accum, s->thresq, s->fjc, s->R0, s->RH, s->C0, s->CW, s->Py, s->Px};
   for (void* q : ptrs)
     if (q) hipFree(q);
   delete s;
@@ -2505,6 +2513,7 @@ int zr_stage_create(zr_stage_state** out, const zr_params& p, int N, int H, int 
     ZC(smalloc(s, (void**)&s->G, sizeof(float2) * (size_t)(Wp / 2 + 1) * Hp));
     ZC(smalloc(s, (void**)&s->accum, sizeof(double) * (N + 1)));
     ZC(smalloc(s, (void**)&s->thresq, sizeof(double) * N));
+    ZC(smalloc(s, (void**)&s->fjc, sizeof(float) * 2 * N));
   }
   for (int i = 0; i < p.n_filters; ++i) {
     if (p.filters[i] == ZR_FILTER_GAUSSLP && !s->gtmp) ZC(smalloc(s, (void**)&s->gtmp, sizeof(float) * H * W));
@@ -2700,7 +2709,7 @@ static void fill_dose_args(zr_stage_state* s, DoseArgs& a, const double* rowsum,
   a.vs = std::sqrt(s->p.voltage_kv / 300.0);
   a.ipx = 1.0 / (s->Wp * (s->p.pixelsize_nm * 10.0));
   a.ipy = 1.0 / (s->Hp * (s->p.pixelsize_nm * 10.0));
-  a.rowsum = rowsum; a.nrows = nrows; a.accum = s->accum; a.thresq = s->thresq; a.info = s->info;
+  a.rowsum = rowsum; a.nrows = nrows; a.accum = s->accum; a.thresq = s->thresq; a.fjc = s->fjc; a.info = s->info;
   a.order_sq = -1;
   a.normalize = normalize;
   a.cut_skip = a.order > 0.0 ? (float)std::exp(std::log(200.0) / a.order) : INFINITY;
