@@ -111,8 +111,13 @@ __global__ __launch_bounds__(256) void k_ps_hist(const float* __restrict__ peaks
   }
 }
 
+// 1 - (1 + e^{-K (thr - h)})^{-nu} (zorro/zorro.py:1344), the power taken as
+// exp(-nu log1p(.)): the log term depends on (thr, K) only, so a pattern-search
+// step shares it between the candidates that differ in nu
+ZR_DEV double logistic_lse(double h, double thr, double K) { return log1p(exp(-K * (-h + thr))); }
+ZR_DEV double logistic_from_lse(double lse, double nu) { return 1.0 - exp(-nu * lse); }
 ZR_DEV double logistic_d(double h, double thr, double K, double nu) {
-  return 1.0 - 1.0 / pow(1.0 + exp(-K * (-h + thr)), nu);
+  return logistic_from_lse(logistic_lse(h, thr, K), nu);
 }
 
 ZR_DEV double logi_obj(const float* hs, const float* cdf, int nb, double thr, double K, double nu) {
@@ -136,13 +141,17 @@ __global__ __launch_bounds__(256) void k_logi_grid(const float* __restrict__ hs,
 }
 
 // argmin of the grid (first index) then the 3x3x3 pattern search.  One block of
-// LOGI_NT: every (candidate, bin) term is evaluated in parallel (K = e^lk and
-// nu = e^ln once per candidate), each candidate's objective is then summed over
-// bins in index order.
-constexpr int LOGI_NT = 1024;
+// LOGI_NT: the log terms of the nine (thr, K) pairs of a step, then every
+// (candidate, bin) term, are evaluated in parallel (K = e^lk and nu = e^ln once
+// per candidate), each candidate's objective is then summed over bins in index
+// order.  Dynamic LDS: logi_refine_lds_bytes(), for up to LOGI_MAXB bins.
+constexpr int LOGI_NT = 1024, LOGI_MAXB = 290;
+constexpr size_t logi_refine_lds_bytes() { return sizeof(double) * (27 + 9) * LOGI_MAXB; }
 __global__ __launch_bounds__(LOGI_NT) void k_logi_refine(const float* __restrict__ hs, const float* __restrict__ cdf,
                                                         const double* __restrict__ obj, SolveInfo* info) {
-  __shared__ double terms[27 * 290];
+  extern __shared__ double logi_dyn[];
+  double* terms = logi_dyn;                   // [27][nb]
+  double* lse = logi_dyn + 27 * LOGI_MAXB;    // [9][nb]: (thr index o0 + 1) * 3 + (K index o1 + 1)
   __shared__ double cval[27], ccand[27][5];
   __shared__ double rv[LOGI_NT / 64];
   __shared__ int ri[LOGI_NT / 64];
@@ -189,9 +198,14 @@ __global__ __launch_bounds__(LOGI_NT) void k_logi_refine(const float* __restrict
       ccand[tid][4] = exp(ccand[tid][2]);
     }
     __syncthreads();
+    for (int e = tid; e < 9 * nb; e += LOGI_NT) {
+      const int a = e / nb, i = e % nb, c = 3 * a + 1;   // a candidate with this (thr, K)
+      lse[e] = logistic_lse((double)hs[i], ccand[c][0], ccand[c][3]);
+    }
+    __syncthreads();
     for (int e = tid; e < 27 * nb; e += LOGI_NT) {
       int c = e / nb, i = e % nb;
-      terms[e] = fabs((double)cdf[i] - logistic_d((double)hs[i], ccand[c][0], ccand[c][3], ccand[c][4]));
+      terms[e] = fabs((double)cdf[i] - logistic_from_lse(lse[(c / 3) * nb + i], ccand[c][4]));
     }
     __syncthreads();
     if (tid < 27) {
@@ -2449,6 +2463,11 @@ int zr_stage_create(zr_stage_state** out, const zr_params& p, int N, int H, int 
   ZC(supload(s, &s->dups, duplicate_groups(ed)));
   ZC(supload(s, &s->meas, ms));
   const int maxbins = std::max(1, s->Mcap / 7 + 1);
+  if (maxbins > LOGI_MAXB) {    // N <= MAXN gives at most 2016 equations, 289 bins
+    zr_stage_free(s);
+    return serr(ZR_ERR_UNSUPPORTED, "%d equations: the logistic fit handles up to %d histogram bins", s->Mcap,
+                LOGI_MAXB);
+  }
   ZC(smalloc(s, (void**)&s->hs, sizeof(float) * maxbins));
   ZC(smalloc(s, (void**)&s->cdf, sizeof(float) * maxbins));
   ZC(smalloc(s, (void**)&s->pdf, sizeof(float) * maxbins));
@@ -2598,7 +2617,10 @@ int zr_stage_solve(zr_stage_state* s, const float* shifts, const float* corr, co
     const int total = LOGI_G0 * LOGI_G1 * LOGI_G2;
     hipLaunchKernelGGL(k_logi_grid, dim3((total + 255) / 256), dim3(256), 0, st, s->hs, s->cdf, s->info, s->obj);
     ZS_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_logi_refine, dim3(1), dim3(LOGI_NT), 0, st, s->hs, s->cdf, s->obj, s->info);
+    static std::once_flag lds_once;
+    zr_allow_full_lds((const void*)k_logi_refine, lds_once);
+    hipLaunchKernelGGL(k_logi_refine, dim3(1), dim3(LOGI_NT), logi_refine_lds_bytes(), st, s->hs, s->cdf, s->obj,
+                       s->info);
     ZS_HIP(hipGetLastError());
   }
   SolveArgs a;
