@@ -287,6 +287,37 @@ ZR_DEV unsigned long long shfl_u64(unsigned long long v, int src) {
   return ((unsigned long long)hi << 32) | lo;
 }
 ZR_DEV bool lex_less(double d1, int e1, double d2, int e2) { return d1 < d2 || (d1 == d2 && e1 < e2); }
+// Lexicographic (d, e) minimum over the wave (all 64 lanes active) with a
+// payload pl that follows the winner, in every lane: within rows of 16 lanes
+// by DPP exchanges (lanes 1 and 2 apart, then the half-row and row mirrors),
+// then the four row minima in row order.  DPP moves instead of LDS permutes:
+// the solver's reductions are single-wave latency chains.
+template <int CTRL> ZR_DEV void lexmin_dpp(double& d, int& e, int& pl) {
+  const unsigned long long u = __double_as_longlong(d);
+  const unsigned lo = __builtin_amdgcn_update_dpp((unsigned)u, (unsigned)u, CTRL, 0xF, 0xF, false);
+  const unsigned hi = __builtin_amdgcn_update_dpp((unsigned)(u >> 32), (unsigned)(u >> 32), CTRL, 0xF, 0xF, false);
+  const int oe = __builtin_amdgcn_update_dpp(e, e, CTRL, 0xF, 0xF, false);
+  const int op = __builtin_amdgcn_update_dpp(pl, pl, CTRL, 0xF, 0xF, false);
+  const double od = __longlong_as_double(((unsigned long long)hi << 32) | lo);
+  if (lex_less(od, oe, d, e)) { d = od; e = oe; pl = op; }
+}
+ZR_DEV void wave_lexmin(double& d, int& e, int& pl) {
+  lexmin_dpp<0xB1>(d, e, pl);    // quad_perm [1, 0, 3, 2]
+  lexmin_dpp<0x4E>(d, e, pl);    // quad_perm [2, 3, 0, 1]
+  lexmin_dpp<0x141>(d, e, pl);   // row_half_mirror
+  lexmin_dpp<0x140>(d, e, pl);   // row_mirror
+  double bd = lane_bcast_d(d, 0);
+  int be = __builtin_amdgcn_readlane(e, 0), bp = __builtin_amdgcn_readlane(pl, 0);
+#pragma unroll
+  for (int r = 1; r < 4; ++r) {
+    const double od = lane_bcast_d(d, 16 * r);
+    const int oe = __builtin_amdgcn_readlane(e, 16 * r), op = __builtin_amdgcn_readlane(pl, 16 * r);
+    if (lex_less(od, oe, bd, be)) { bd = od; be = oe; bp = op; }
+  }
+  d = bd;
+  e = be;
+  pl = bp;
+}
 
 // dynamic LDS of k_l1_solve: matrix region (Cholesky / slope sums / sort),
 // then b[M], w[M], breakpoints[M] (double), endpoints[M], tree flags[M],
@@ -320,6 +351,7 @@ __global__ __launch_bounds__(L1_NT) void k_l1_solve(SolveArgs a) {
   __shared__ int red_i[2][4];
   __shared__ int s_key, s_v, s_used;
   __shared__ double s_best;
+  __shared__ double avs_s[MAXN];     // net bound flow into each frame (the slope sums read it by broadcast)
 
   // ---- data: each thread keeps its edges in registers
   int eij[L1_EPT];
@@ -646,10 +678,22 @@ __global__ __launch_bounds__(L1_NT) void k_l1_solve(SolveArgs a) {
       if (lane < N)
         for (int x = 0; x < N; ++x) av += Lm[x * LDC + lane];
       const unsigned long long sb = (lane < N) ? sub_s[lane] : 0ull;
-      double A = 0.0;
-      for (int u = 0; u < N; ++u) {
-        const double au = lane_bcast_d(av, u);
-        if ((sb >> u) & 1ull) A += au;
+      // A = sum of av over the lane's subtree: av by LDS broadcast (zero past
+      // N), four partial sums over u mod 4
+      avs_s[lane] = lane < N ? av : 0.0;
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      double A;
+      {
+        double a4[4] = {0.0, 0.0, 0.0, 0.0};
+        const unsigned slo = (unsigned)sb, shi = (unsigned)(sb >> 32);
+#pragma unroll
+        for (int u = 0; u < MAXN; ++u) {
+          const double au = avs_s[u];
+          const bool in = ((u < 32 ? slo >> u : shi >> (u - 32)) & 1u) != 0;
+          a4[u & 3] += in ? au : 0.0;
+        }
+        A = (a4[0] + a4[1]) + (a4[2] + a4[3]);
       }
       double bval = INFINITY;
       int bkey = 0x7fffffff;
@@ -665,12 +709,7 @@ __global__ __launch_bounds__(L1_NT) void k_l1_solve(SolveArgs a) {
         }
       }
       int bvv = lane;
-      for (int o = 32; o > 0; o >>= 1) {
-        const double ov = __shfl_xor(bval, o, 64);
-        const int ok = __shfl_xor(bkey, o, 64);
-        const int ovv = __shfl_xor(bvv, o, 64);
-        if (ov < bval || (ov == bval && ok < bkey)) { bval = ov; bkey = ok; bvv = ovv; }
-      }
+      wave_lexmin(bval, bkey, bvv);
       if (lane == 0) { s_key = bkey; s_v = bvv; }
       if (bkey != 0x7fffffff && lane == bvv) {   // slope of the chosen move (also under Bland's rule)
         const double wt = w_s[ped_s[lane]];
@@ -729,10 +768,9 @@ __global__ __launch_bounds__(L1_NT) void k_l1_solve(SolveArgs a) {
           const int e = ce_s[t];
           if (lex_less(last, last_e, d, e) && lex_less(d, e, md, me)) { md = d; me = e; }
         }
-        for (int o = 32; o > 0; o >>= 1) {
-          const double od = __shfl_xor(md, o, 64);
-          const int oe = __shfl_xor(me, o, 64);
-          if (lex_less(od, oe, md, me)) { md = od; me = oe; }
+        {
+          int unused = 0;
+          wave_lexmin(md, me, unused);
         }
         if (me == 0x7fffffff) break;
         slope += 2.0 * w_s[me];
