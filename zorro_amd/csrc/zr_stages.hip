@@ -272,6 +272,7 @@ struct SolveArgs {
 // end by summing measured shifts along the final tree, so the vertex is exact.
 constexpr int L1_NT = 256;
 constexpr int L1_EPT = 8;                 // edges per thread: M <= 2048
+static_assert(L1_EPT < 16, "a thread's breakpoint count fits the four-bit prefix sum");
 constexpr int L1_LDC = MAXN + 1;          // matrix row stride
 constexpr int L1_BLAND = 3;               // zero-length pivots (x N) before Bland's rule
 
@@ -742,15 +743,18 @@ __global__ __launch_bounds__(L1_NT) void k_l1_solve(SolveArgs a) {
         ekk[nmine] = tid + k * L1_NT;
         ++nmine;
       }
-      int incl = nmine;                       // wave prefix sum of the counts
-      for (int o = 1; o < 64; o <<= 1) {
-        const int t = __shfl_up(incl, o, 64);
-        if (lane >= o) incl += t;
+      // wave prefix sum of the counts (nmine <= L1_EPT = 8: four bits), one
+      // ballot and lane-masked bit count per bit
+      int before = 0, tot = 0;
+#pragma unroll
+      for (int b = 0; b < 4; ++b) {
+        const unsigned long long m = __ballot((nmine >> b) & 1);
+        before += (int)__builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u)) << b;
+        tot += __popcll(m) << b;
       }
-      const int tot = __shfl(incl, 63, 64);
       int base = 0;
       if (lane == 0 && tot > 0) base = atomicAdd(&s_nc, tot);
-      base = __shfl(base, 0, 64) + incl - nmine;
+      base = __builtin_amdgcn_readfirstlane(base) + before;   // every lane is active here: lane 0's value
       for (int q = 0; q < nmine; ++q) { cd_s[base + q] = dk[q]; ce_s[base + q] = ekk[q]; }
     }
     __syncthreads();
