@@ -141,7 +141,7 @@ template <int R1, int R2, int DIR> struct DftCT {
 #pragma unroll
     for (int n2 = 1; n2 < R2; ++n2)
 #pragma unroll
-      for (int k1 = 1; k1 < R1; ++k1) t[n2 * R1 + k1] = c_mul(t[n2 * R1 + k1], tw_ct(n2 * k1));
+      for (int k1 = 1; k1 < R1; ++k1) t[n2 * R1 + k1] = mul_tw(t[n2 * R1 + k1], n2 * k1);
 #pragma unroll
     for (int k1 = 0; k1 < R1; ++k1) {
       float2 u[R2];
@@ -151,6 +151,20 @@ template <int R1, int R2, int DIR> struct DftCT {
 #pragma unroll
       for (int k2 = 0; k2 < R2; ++k2) v[k1 + R1 * k2] = u[k2];
     }
+  }
+  // a * exp(DIR*2*pi*i*j/R): a quarter turn (4 j a multiple of R) as a swap and
+  // sign change (a product with the constants 0 and +-1 is not folded without
+  // fast math), otherwise a complex product with the constant
+  static ZR_DEV float2 mul_tw(float2 a, int j) {
+    constexpr int R = R1 * R2;
+    if ((4 * j) % R == 0) {
+      const int q = ((4 * j) / R) % 4;
+      if (q == 0) return a;
+      if (q == 2) return make_float2(-a.x, -a.y);
+      const float2 b = c_muli<DIR>(a);          // a * (DIR i)
+      return q == 1 ? b : make_float2(-b.x, -b.y);
+    }
+    return c_mul(a, tw_ct(j));
   }
   static ZR_DEV float2 tw_ct(int j) {
     // exp(DIR*2*pi*i*j/R) from a compile-time table (j is a constant after unrolling)

@@ -756,7 +756,7 @@ __global__ __launch_bounds__(64 * WPB, K2R2_OCC) void k2_cols_r2(const float2* _
 #pragma unroll
       for (int n2 = 1; n2 < 16; ++n2)
 #pragma unroll
-        for (int k1 = 1; k1 < R1; ++k1) t[n2 * R1 + k1] = c_mul(t[n2 * R1 + k1], DftCT<R1, 16, 1>::tw_ct(n2 * k1));
+        for (int k1 = 1; k1 < R1; ++k1) t[n2 * R1 + k1] = DftCT<R1, 16, 1>::mul_tw(t[n2 * R1 + k1], n2 * k1);
     }
     float2 xmod = make_float2(cw, 0.f);
     float2* zdst = Z + ((size_t)p * Wh + kx) * twoM;
