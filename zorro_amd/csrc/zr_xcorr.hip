@@ -796,9 +796,9 @@ __global__ __launch_bounds__(64 * WPB, K2R2_OCC) void k2_cols_r2(const float2* _
         const int r = k2r2_residue<L>(k1, slot);
         const int dy = PS ? r + L * j1of(r) : r;     // the larger of the residue's two lags
         const float2* row = my + slot * LP + 16 * part;
-        float2 a0 = make_float2(0.f, 0.f), a1 = make_float2(0.f, 0.f);
+        float2 a0 = row[0], a1 = a0;            // term 0 has weight 1
 #pragma unroll
-        for (int i = 0; i < 16; ++i) {
+        for (int i = 1; i < 16; ++i) {
           const float2 x = row[i];
           a0 = c_add(a0, x);
           a1.x = fmaf(x.x, K2R2_WC[i], fmaf(x.y, K2R2_WS[i], a1.x));
