@@ -211,8 +211,8 @@ def main():
         k2_counter_json(args.k2_counters[:-1], args.k2_counters[-1])
         return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one GPU only: this build has no multi-GPU path "
-              "(movies are independent; see DESIGN.md for how the work would shard).", file=sys.stderr)
+        print("bench.py measures one GPU only: there is no multi-GPU path (movies are independent: "
+              "each GPU would run its own movies, one process per GPU).", file=sys.stderr)
         sys.exit(2)
 
     import torch

@@ -459,7 +459,7 @@ __global__ __launch_bounds__(64 * WPB, 2) void k2_cols(const float2* __restrict_
     T1[e] = make_float2(v.x, -v.y);
   }
   __syncthreads();
-  // round 2 (outputs 64..2m-1) is spread over `sf` lanes per output
+  // the second sweep (outputs 64..2m-1) is spread over `sf` lanes per output
   const int c2 = twoM > 64 ? twoM - 64 : 0;
   const int sf = c2 == 0 ? 1 : (c2 <= 16 ? 4 : (c2 <= 32 ? 2 : 1));
   const int span = 64 / sf;                          // lanes per part
@@ -521,7 +521,7 @@ __global__ __launch_bounds__(64 * WPB, 2) void k2_cols(const float2* __restrict_
       for (int r = 0; r < LH; ++r) my[r * LP + lane] = v[h * LH + r];
       __builtin_amdgcn_wave_barrier();
       __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-      if (t1 < twoM && (r1 / LH) == h) {   // round 1: outputs 0..63, one per lane
+      if (t1 < twoM && (r1 / LH) == h) {   // first sweep: outputs 0..63, one per lane
         const float2* yr = my + (r1 - h * LH) * LP;
         const float2* t0 = T0 + t1 * 8;
         const float2* tt1 = T1 + t1 * 8;
@@ -541,7 +541,7 @@ __global__ __launch_bounds__(64 * WPB, 2) void k2_cols(const float2* __restrict_
         }
         zdst[t1] = PS ? c_mul(acc, xmod) : acc;
       }
-      if (c2 > 0) {      // round 2: outputs 64..2m-1, `sf` partial sums each
+      if (c2 > 0) {      // second sweep: outputs 64..2m-1, `sf` partial sums each
         float2 acc = make_float2(0.f, 0.f);
         const bool mine = (o2 < c2) && ((r2 / LH) == h);
         if (mine) {
