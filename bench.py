@@ -191,6 +191,7 @@ def main():
     ap.add_argument("--k2-counters", nargs="+", metavar="CSV_OR_OUT",
                     help="counter CSVs then the output JSON: executed FP32 work of the pair transform (no GPU)")
     ap.add_argument("--workload", default="c2", choices=["c2", "served", "batched", "hostinput", "configs"])
+    ap.add_argument("--config", default=None, help="configs: run only this entry of CONFIGS")
     ap.add_argument("--inflight", type=int, default=2,
                     help="C2 movies in flight at once for the headline (one plan and stream each); "
                          "single-movie figures are measured first and reported beside it")
@@ -471,6 +472,8 @@ def bench_configs(args):
     out = dict(metric="movies/min per configuration (single GPU, inputs resident in HBM)", unit="movies/min",
                n_gpus=1, data="synthetic", configs={})
     for name, c in CONFIGS.items():
+        if args.config and name != args.config:
+            continue
         print(f"[bench configs] {name}: generating", file=sys.stderr, flush=True)
         frames, drift = make_movie_torch(c["n"], c["h"], c["w"], seed=c["seed"], dose=c["dose"], device=dev)
         gain = None
